@@ -1,0 +1,315 @@
+"""Python host binding of the MI355X rigid-body step (ctypes over include/bb_capi.h).
+
+`RigidBodyManager` mirrors the reference's C++ manager
+(src/rigid_body_manager.hpp:14-116): create(iterations), simulate(),
+read_back_sim_config(), set/clear/get_sim_flags, set/get_sim_type,
+skip_warm_starting_once(); bodies are added with the load_scene rules.
+`Batch` steps many worlds together (no reference counterpart).
+
+The library is the HIP build `libbeatbox_amd.so` next to this file.  There is
+no CPU fallback: loading fails loudly when the library is missing, and
+stepping fails with BB_ERR_NO_DEVICE when no GPU is visible.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libbeatbox_amd.so")
+
+# ------------------------------------------------------------------ structs
+class WorldDesc(C.Structure):
+    _fields_ = [
+        ("capacity_bodies", C.c_uint32),
+        ("solver", C.c_uint32),
+        ("iterations", C.c_uint32),
+        ("flags", C.c_uint32),
+        ("dt", C.c_float),
+        ("gravity", C.c_float),
+        ("morton_bits", C.c_uint32),
+        ("quirks", C.c_uint32),
+    ]
+
+    @classmethod
+    def make(cls, **kw) -> "WorldDesc":
+        d = cls()
+        for k, v in kw.items():
+            setattr(d, k, v)
+        return d
+
+
+class BodyState(C.Structure):
+    _fields_ = [
+        ("id", C.c_uint32),
+        ("flags", C.c_uint32),
+        ("island_index", C.c_uint32),
+        ("row", C.c_uint32),
+        ("position", C.c_float * 3),
+        ("rotation", C.c_float * 4),
+        ("velocity", C.c_float * 3),
+        ("omega", C.c_float * 3),
+    ]
+
+
+BODY_STATE = np.dtype(
+    [("id", "<u4"), ("flags", "<u4"), ("island_index", "<u4"), ("row", "<u4"),
+     ("position", "<f4", 3), ("rotation", "<f4", 4), ("velocity", "<f4", 3),
+     ("omega", "<f4", 3)]
+)
+assert BODY_STATE.itemsize == C.sizeof(BodyState)
+
+
+class StepStats(C.Structure):
+    _fields_ = [(n, C.c_uint32) for n in (
+        "rigid_body_count", "active_rigid_body_count", "broad_phase_collision_count",
+        "collision_count", "manifold_node_count", "contact_count", "island_count",
+        "contact_island_count", "sleeping_count", "graph_color_count",
+        "graph_color_violations", "graph_color_overflow", "avbd_color_count",
+        "avbd_violations", "avbd_stick_count", "dbg_fresh", "dbg_pen", "dbg_id_sum",
+        "dbg_maxv", "overflow")] + [("step_index", C.c_uint64)]
+
+    def as_dict(self) -> dict:
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+LBVH_NODE = np.dtype([("minimum", "<f4", 3), ("maximum", "<f4", 3), ("left", "<i4"),
+                      ("right", "<i4"), ("primitive_idx", "<u4"), ("pad", "<u4")])
+CONTACT_TAP = np.dtype([
+    ("position", "<f4", 3), ("penetration", "<f4"), ("normal_impulse", "<f4"),
+    ("tangent_impulse", "<f4", 2), ("mass_coefficient", "<f4"), ("tangent_mass", "<f4", 2),
+    ("anchor_a", "<f4", 3), ("anchor_b", "<f4", 3), ("stick", "<u4"), ("fp", "<u4", 4),
+    ("warm_start", "<u4")])
+MANIFOLD_TAP = np.dtype([
+    ("obb1_index", "<u4"), ("obb2_index", "<u4"), ("key", "<i4"), ("error", "<i4"),
+    ("contact_count", "<i4"), ("color", "<u4"), ("normal", "<f4", 3),
+    ("tangent", "<f4", (2, 3)), ("anchor_a", "<f4", 3), ("anchor_b", "<f4", 3),
+    ("contacts", CONTACT_TAP, 8)])
+
+TAP_MORTON, TAP_SORTED, TAP_LBVH, TAP_PAIRS, TAP_MANIFOLDS = 0, 1, 2, 3, 4
+TAP_ISLANDS, TAP_EDGE_COLORS, TAP_BODY_COLORS, TAP_ROWS = 5, 6, 7, 8
+ORDER_BY_ID, ORDER_BY_ROW = 0, 1
+
+STATUS = {0: "BB_OK", 1: "BB_ERR_INVALID", 2: "BB_ERR_CAPACITY", 3: "BB_ERR_DEVICE",
+          4: "BB_ERR_NO_DEVICE", 5: "BB_ERR_OVERFLOW", 6: "BB_ERR_STATE", 7: "BB_ERR_BUDGET"}
+
+
+class BBError(RuntimeError):
+    def __init__(self, code: int, what: str):
+        super().__init__(f"{what}: {STATUS.get(code, code)}")
+        self.code = code
+
+
+def _check(code: int, what: str) -> None:
+    if code != 0:
+        raise BBError(code, what)
+
+
+# ----------------------------------------------------------------- loading
+_LIB = None
+
+EXPORTS = {
+    # name: (restype, argtypes)
+    "bb_world_desc_default": (None, [C.POINTER(WorldDesc)]),
+    "bb_world_create": (C.c_int, [C.POINTER(WorldDesc), C.POINTER(C.c_void_p)]),
+    "bb_world_add_bodies": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]),
+    "bb_world_set_flags": (C.c_int, [C.c_void_p, C.c_uint32]),
+    "bb_world_clear_flags": (C.c_int, [C.c_void_p, C.c_uint32]),
+    "bb_world_get_flags": (C.c_uint32, [C.c_void_p]),
+    "bb_world_set_solver": (C.c_int, [C.c_void_p, C.c_uint32]),
+    "bb_world_set_iterations": (C.c_int, [C.c_void_p, C.c_uint32]),
+    "bb_world_skip_warm_start_once": (C.c_int, [C.c_void_p]),
+    "bb_world_step": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p]),
+    "bb_world_sync": (C.c_int, [C.c_void_p]),
+    "bb_world_read_bodies": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_int]),
+    "bb_world_read_stats": (C.c_int, [C.c_void_p, C.POINTER(StepStats)]),
+    "bb_world_debug_tap": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_size_t,
+                                     C.POINTER(C.c_size_t)]),
+    "bb_world_body_count": (C.c_uint32, [C.c_void_p]),
+    "bb_world_destroy": (None, [C.c_void_p]),
+    "bb_batch_create": (C.c_int, [C.c_uint32, C.c_uint32, C.POINTER(C.c_void_p)]),
+    "bb_batch_add_world": (C.c_int, [C.c_void_p, C.POINTER(WorldDesc), C.c_void_p,
+                                     C.c_uint32, C.POINTER(C.c_uint32)]),
+    "bb_batch_step": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_int]),
+    "bb_batch_sync": (C.c_int, [C.c_void_p]),
+    "bb_batch_read_world": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_int]),
+    "bb_batch_world_stats": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(StepStats)]),
+    "bb_batch_debug_tap": (C.c_int, [C.c_void_p, C.c_uint32, C.c_int, C.c_void_p, C.c_size_t,
+                                     C.POINTER(C.c_size_t)]),
+    "bb_batch_world_count": (C.c_uint32, [C.c_void_p]),
+    "bb_batch_body_count": (C.c_uint32, [C.c_void_p]),
+    "bb_batch_set_block_size": (C.c_int, [C.c_void_p, C.c_uint32]),
+    "bb_batch_destroy": (None, [C.c_void_p]),
+    "bb_version": (C.c_char_p, []),
+}
+
+
+def lib() -> C.CDLL:
+    """Loads the HIP library (raises if it has not been built)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with "
+                               "`python __graft_entry__.py build` (there is no CPU fallback)")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in EXPORTS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def default_desc() -> WorldDesc:
+    d = WorldDesc()
+    lib().bb_world_desc_default(C.byref(d))
+    return d
+
+
+def _bodies_ptr(bodies: np.ndarray):
+    from scenes import BODY_DESC  # local module next to this file
+    arr = np.ascontiguousarray(bodies, dtype=BODY_DESC)
+    return arr, arr.ctypes.data_as(C.c_void_p)
+
+
+# ------------------------------------------------------------ world object
+class RigidBodyManager:
+    """Mirror of the reference RigidBodyManager over the C-ABI."""
+
+    def __init__(self, desc: WorldDesc | None = None, **kw):
+        self._h = C.c_void_p()
+        d = desc if desc is not None else default_desc()
+        for k, v in kw.items():
+            setattr(d, k, v)
+        _check(lib().bb_world_create(C.byref(d), C.byref(self._h)), "bb_world_create")
+
+    # SceneManager::load_scene rules
+    def add_bodies(self, bodies: np.ndarray) -> np.ndarray:
+        arr, ptr = _bodies_ptr(bodies)
+        ids = np.zeros(len(arr), np.uint32)
+        _check(lib().bb_world_add_bodies(self._h, ptr, len(arr), ids.ctypes.data_as(C.c_void_p)),
+               "bb_world_add_bodies")
+        return ids
+
+    def simulate(self, nsteps: int = 1) -> None:
+        _check(lib().bb_world_step(self._h, nsteps, None), "bb_world_step")
+
+    def sync(self) -> None:
+        _check(lib().bb_world_sync(self._h), "bb_world_sync")
+
+    def read_bodies(self, order: int = ORDER_BY_ID) -> np.ndarray:
+        n = lib().bb_world_body_count(self._h)
+        out = np.zeros(n, BODY_STATE)
+        _check(lib().bb_world_read_bodies(self._h, out.ctypes.data_as(C.c_void_p), n, order),
+               "bb_world_read_bodies")
+        return out
+
+    def read_back_sim_config(self) -> dict:
+        s = StepStats()
+        _check(lib().bb_world_read_stats(self._h, C.byref(s)), "bb_world_read_stats")
+        return s.as_dict()
+
+    def set_sim_flags(self, flags: int) -> int:
+        _check(lib().bb_world_set_flags(self._h, flags), "bb_world_set_flags")
+        return self.get_sim_flags()
+
+    def clear_sim_flags(self, flags: int) -> int:
+        _check(lib().bb_world_clear_flags(self._h, flags), "bb_world_clear_flags")
+        return self.get_sim_flags()
+
+    def get_sim_flags(self) -> int:
+        return lib().bb_world_get_flags(self._h)
+
+    def set_sim_type(self, solver: int) -> None:
+        _check(lib().bb_world_set_solver(self._h, solver), "bb_world_set_solver")
+
+    def set_iterations(self, it: int) -> None:
+        _check(lib().bb_world_set_iterations(self._h, it), "bb_world_set_iterations")
+
+    def skip_warm_starting_once(self) -> None:
+        _check(lib().bb_world_skip_warm_start_once(self._h), "bb_world_skip_warm_start_once")
+
+    def tap(self, kind: int, dtype) -> np.ndarray:
+        return _tap(lambda dst, cap, wr: lib().bb_world_debug_tap(self._h, kind, dst, cap, wr), dtype)
+
+    def destroy(self) -> None:
+        if self._h:
+            lib().bb_world_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+def _tap(call, dtype) -> np.ndarray:
+    written = C.c_size_t(0)
+    call(None, 0, C.byref(written))
+    nbytes = written.value
+    buf = np.zeros(max(nbytes, 1), np.uint8)
+    code = call(buf.ctypes.data_as(C.c_void_p), nbytes, C.byref(written))
+    _check(code, "debug_tap")
+    return buf[:nbytes].view(dtype).copy()
+
+
+class Batch:
+    """Many independent worlds stepped together (segmented, one workgroup per world)."""
+
+    def __init__(self, capacity_worlds: int = 0, capacity_bodies: int = 0):
+        self._h = C.c_void_p()
+        _check(lib().bb_batch_create(capacity_worlds, capacity_bodies, C.byref(self._h)),
+               "bb_batch_create")
+
+    def add_world(self, desc: WorldDesc, bodies: np.ndarray) -> int:
+        arr, ptr = _bodies_ptr(bodies)
+        w = C.c_uint32(0)
+        _check(lib().bb_batch_add_world(self._h, C.byref(desc), ptr, len(arr), C.byref(w)),
+               "bb_batch_add_world")
+        return w.value
+
+    def step(self, nsteps: int = 1, use_graph: bool = False) -> None:
+        _check(lib().bb_batch_step(self._h, nsteps, None, 1 if use_graph else 0), "bb_batch_step")
+
+    def sync(self) -> None:
+        _check(lib().bb_batch_sync(self._h), "bb_batch_sync")
+
+    def set_block_size(self, threads: int) -> None:
+        _check(lib().bb_batch_set_block_size(self._h, threads), "bb_batch_set_block_size")
+
+    @property
+    def world_count(self) -> int:
+        return lib().bb_batch_world_count(self._h)
+
+    @property
+    def body_count(self) -> int:
+        return lib().bb_batch_body_count(self._h)
+
+    def read_world(self, w: int, n: int, order: int = ORDER_BY_ID) -> np.ndarray:
+        out = np.zeros(n, BODY_STATE)
+        _check(lib().bb_batch_read_world(self._h, w, out.ctypes.data_as(C.c_void_p), n, order),
+               "bb_batch_read_world")
+        return out
+
+    def stats(self, w: int) -> dict:
+        s = StepStats()
+        _check(lib().bb_batch_world_stats(self._h, w, C.byref(s)), "bb_batch_world_stats")
+        return s.as_dict()
+
+    def tap(self, w: int, kind: int, dtype) -> np.ndarray:
+        return _tap(lambda dst, cap, wr: lib().bb_batch_debug_tap(self._h, w, kind, dst, cap, wr),
+                    dtype)
+
+    def destroy(self) -> None:
+        if self._h:
+            lib().bb_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass

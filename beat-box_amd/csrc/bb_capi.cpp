@@ -1,0 +1,971 @@
+// bb_capi.cpp — host side of the C-ABI (include/bb_capi.h).
+//
+// A bb_world is a bb_batch holding one world.  A batch keeps every world's
+// initial descriptors on the host and all simulation state on the device;
+// the device layout (bb_dev.hpp) is rebuilt whenever worlds or bodies are
+// added, carrying the current body state across the rebuild.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "../../include/bb_capi.h"
+#include "bb_dev.hpp"
+
+namespace bb {
+void launch_step(const Dev& d, hipStream_t st, uint32_t block);
+}  // namespace bb
+
+using namespace bb;
+
+namespace {
+
+#define HIPCHK(x)                                                         \
+  do {                                                                    \
+    hipError_t e_ = (x);                                                  \
+    if (e_ != hipSuccess) {                                               \
+      std::fprintf(stderr, "beatbox: %s failed: %s\n", #x, hipGetErrorString(e_)); \
+      return BB_ERR_DEVICE;                                               \
+    }                                                                     \
+  } while (0)
+
+struct HostWorld {
+  bb_world_desc desc;
+  std::vector<bb_body_desc> bodies;  // id order
+  bool skip_ws_once = false;
+  bool drop_prev = false;
+  uint32_t body_off = 0, act_off = 0, pair_off = 0, pair_cap = 0, n = 0, a = 0;
+};
+
+struct Buffers {
+  std::vector<void*> ptrs;
+  template <class T>
+  T* alloc(size_t count) {
+    void* p = nullptr;
+    size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+    if (hipMalloc(&p, bytes) != hipSuccess) p = nullptr;
+    ptrs.push_back(p);
+    return (T*)p;
+  }
+  void release() {
+    for (void* p : ptrs)
+      if (p) (void)hipFree(p);
+    ptrs.clear();
+  }
+};
+
+}  // namespace
+
+struct bb_batch {
+  std::vector<HostWorld> worlds;
+  Buffers buf;
+  bool built = false;
+  bool dirty = true;
+  bool stepped = false;
+  Dev base{};  // pointers common to both parities
+  Rows rows[2];
+  uint32_t* entries[2] = {nullptr, nullptr};
+  uint2* pairs[2] = {nullptr, nullptr};
+  uint32_t* npairs[2] = {nullptr, nullptr};
+  Manifolds man[2];
+  int latest = 0;  // buffer set holding the newest state
+  uint32_t NB = 0, NA = 0, NP = 0;
+  hipStream_t stream = nullptr;
+  uint32_t block = 0;
+  uint64_t step_index = 0;
+  std::vector<WorldInfo> infos;
+  hipGraphExec_t graph = nullptr;
+  int graph_parity = -1;
+};
+
+struct bb_world {
+  bb_batch* b;
+};
+
+namespace {
+
+uint32_t pair_capacity(uint32_t n) { return 8u * n + 64u; }
+
+bool has_device() {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return false;
+  return n > 0;
+}
+
+void inertia_of(const bb_world_desc& wd, const bb_body_desc& b, float inv[3]) {
+  // SceneManager::load_scene (src/scene_manager.hpp:850-887):
+  // inv_mass = mass == 0 ? 0 : 1/mass; cuboid_get_inverse_intertia
+  // (include/math.hpp:106-130) called with inv_mass under the parity quirk.
+  float mass = b.mass;
+  float inv_mass = mass == 0.0f ? 0.0f : 1.0f / mass;
+  float m_in = (wd.quirks & BB_QUIRK_INERTIA_FROM_INV_MASS) ? inv_mass : mass;
+  inv[0] = inv[1] = inv[2] = 0.0f;
+  if (m_in == 0.0f) return;
+  if (b.shape == BB_SHAPE_SPHERE) {
+    float I = (2.0f / 5.0f) * m_in * (b.radius * b.radius);
+    inv[0] = inv[1] = inv[2] = 1.0f / I;
+    return;
+  }
+  float w = b.maximum[0] - b.minimum[0], h = b.maximum[1] - b.minimum[1], d = b.maximum[2] - b.minimum[2];
+  float Ixx = (1.0f / 12.0f) * m_in * (h * h + d * d);
+  float Iyy = (1.0f / 12.0f) * m_in * (w * w + d * d);
+  float Izz = (1.0f / 12.0f) * m_in * (w * w + h * h);
+  inv[0] = 1.0f / Ixx;
+  inv[1] = 1.0f / Iyy;
+  inv[2] = 1.0f / Izz;
+}
+
+WorldInfo info_of(const HostWorld& hw) {
+  WorldInfo wi{};
+  wi.body_off = hw.body_off;
+  wi.n = hw.n;
+  wi.a = hw.a;
+  wi.act_off = hw.act_off;
+  wi.pair_off = hw.pair_off;
+  wi.pair_cap = hw.pair_cap;
+  wi.flags = hw.desc.flags;
+  wi.solver = hw.desc.solver == BB_SOLVER_PGS ? (uint32_t)BB_SOLVER_PGS_SOFT : hw.desc.solver;
+  wi.iterations = hw.desc.iterations;
+  wi.morton_bits = hw.desc.morton_bits == 63 ? 63u : 30u;
+  wi.quirks = hw.desc.quirks;
+  wi.prev_n = hw.n;
+  wi.dt = hw.desc.dt;
+  wi.gravity = hw.desc.gravity;
+  wi.damp = std::exp(-hw.desc.dt * 0.1f);
+  wi.drop_prev = 0;
+  return wi;
+}
+
+// Reads the newest rows of the batch back into per-world host descriptors so a
+// rebuild (new worlds/bodies) continues from the current state.
+bb_status pull_state(bb_batch* b) {
+  if (!b->built || !b->stepped) return BB_OK;
+  const Rows& R = b->rows[b->latest];
+  std::vector<uint32_t> id(b->NB), fl(b->NB);
+  std::vector<float4> pos(b->NB), rot(b->NB), vel(b->NB), omg(b->NB);
+  HIPCHK(hipStreamSynchronize(b->stream));
+  HIPCHK(hipMemcpy(id.data(), R.id, b->NB * 4, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(fl.data(), R.flags, b->NB * 4, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(pos.data(), R.pos, b->NB * 16, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(rot.data(), R.rot, b->NB * 16, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(vel.data(), R.vel, b->NB * 16, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(omg.data(), R.omg, b->NB * 16, hipMemcpyDeviceToHost));
+  for (auto& hw : b->worlds) {
+    for (uint32_t r = 0; r < hw.n; ++r) {
+      uint32_t g = hw.body_off + r;
+      bb_body_desc& bd = hw.bodies[id[g]];
+      for (int k = 0; k < 3; ++k) {
+        bd.position[k] = (&pos[g].x)[k];
+        bd.velocity[k] = (&vel[g].x)[k];
+        bd.omega[k] = (&omg[g].x)[k];
+      }
+      for (int k = 0; k < 4; ++k) bd.rotation[k] = (&rot[g].x)[k];
+    }
+    hw.drop_prev = true;
+  }
+  b->stepped = false;  // the state now lives in the host descriptors
+  return BB_OK;
+}
+
+bb_status build(bb_batch* b) {
+  if (!b->dirty) return BB_OK;
+  if (!has_device()) return BB_ERR_NO_DEVICE;
+  bb_status st = pull_state(b);
+  if (st != BB_OK) return st;
+  if (b->graph) {
+    (void)hipGraphExecDestroy(b->graph);
+    b->graph = nullptr;
+  }
+  if (!b->stream) HIPCHK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+  b->buf.release();
+  // layout
+  uint32_t NB = 0, NA = 0, NP = 0;
+  for (auto& hw : b->worlds) {
+    hw.n = (uint32_t)hw.bodies.size();
+    hw.a = 0;
+    for (auto& bd : hw.bodies)
+      if (bd.flags & BB_BODY_DYNAMIC) ++hw.a;
+    hw.body_off = NB;
+    hw.act_off = NA;
+    hw.pair_off = NP;
+    hw.pair_cap = pair_capacity(hw.n);
+    NB += hw.n;
+    NA += hw.a;
+    NP += hw.pair_cap;
+  }
+  b->NB = NB;
+  b->NA = NA;
+  b->NP = NP;
+  const uint32_t W = (uint32_t)b->worlds.size();
+  Buffers& A = b->buf;
+  Dev& d = b->base;
+  d = Dev{};
+  d.W = W;
+  d.worlds = A.alloc<WorldInfo>(W);
+  for (int s = 0; s < 2; ++s) {
+    Rows& R = b->rows[s];
+    R.id = A.alloc<uint32_t>(NB);
+    R.flags = A.alloc<uint32_t>(NB);
+    R.sleep = A.alloc<uint32_t>(NB);
+    R.pos = A.alloc<float4>(NB);
+    R.rot = A.alloc<float4>(NB);
+    R.vel = A.alloc<float4>(NB);
+    R.omg = A.alloc<float4>(NB);
+    R.pvel = A.alloc<float4>(NB);
+    b->entries[s] = A.alloc<uint32_t>(NB);
+    b->pairs[s] = A.alloc<uint2>(NP);
+    b->npairs[s] = A.alloc<uint32_t>(W);
+    Manifolds& M = b->man[s];
+    M.hdr = A.alloc<int4>(NP);
+    M.nrm = A.alloc<float4>(NP);
+    M.t0 = A.alloc<float4>(NP);
+    M.t1 = A.alloc<float4>(NP);
+    M.anca = A.alloc<float4>(NP);
+    M.ancb = A.alloc<float4>(NP);
+    M.cpos = A.alloc<float4>((size_t)NP * 8);
+    M.clam = A.alloc<float4>((size_t)NP * 8);
+    M.cmas = A.alloc<float4>((size_t)NP * 8);
+    M.cana = A.alloc<float4>((size_t)NP * 8);
+    M.canb = A.alloc<float4>((size_t)NP * 8);
+    M.cfp = A.alloc<uint4>((size_t)NP * 8);
+    M.cws = A.alloc<uint32_t>((size_t)NP * 8);
+  }
+  Props& P = d.props;
+  P.pmin = A.alloc<float4>(NB);
+  P.pmax = A.alloc<float4>(NB);
+  P.mass = A.alloc<float4>(NB);
+  P.ii0 = A.alloc<float4>(NB);
+  P.ii1 = A.alloc<float4>(NB);
+  P.ii2 = A.alloc<float4>(NB);
+  P.active_rank = A.alloc<uint32_t>(NB);
+  P.active_ids = A.alloc<uint32_t>(NA);
+  d.keys_a = A.alloc<uint64_t>(NB);
+  d.keys_b = A.alloc<uint64_t>(NB);
+  d.idx_a = A.alloc<uint32_t>(NB);
+  d.idx_b = A.alloc<uint32_t>(NB);
+  d.nmin = A.alloc<float4>(2 * (size_t)NB);
+  d.nmax = A.alloc<float4>(2 * (size_t)NB);
+  d.parent = A.alloc<uint32_t>(2 * (size_t)NB);
+  d.visit = A.alloc<uint32_t>(2 * (size_t)NB);
+  d.pcount = A.alloc<uint32_t>(NB);
+  d.poff = A.alloc<uint32_t>(NB);
+  d.link = A.alloc<uint32_t>(NA);
+  d.deg = A.alloc<uint32_t>(NB);
+  d.island = A.alloc<uint32_t>(NB);
+  d.islot = A.alloc<uint32_t>(NB);
+  d.ascratch = A.alloc<uint32_t>(NA);
+  d.color = A.alloc<uint32_t>(NP);
+  d.pick = A.alloc<uint32_t>(NP);
+  d.mask = A.alloc<uint32_t>(NB);
+  d.owner = A.alloc<uint32_t>((size_t)NB * kColors);
+  d.bucket = A.alloc<uint32_t>(NP);
+  d.bucket_off = A.alloc<uint32_t>((size_t)W * (kBuckets + 1));
+  d.iw = A.alloc<float4>((size_t)NB * 3);
+  d.s_pos0 = A.alloc<float4>(NB);
+  d.s_rot0 = A.alloc<float4>(NB);
+  d.s_post = A.alloc<float4>(NB);
+  d.s_rott = A.alloc<float4>(NB);
+  d.depth = A.alloc<uint32_t>(NB);
+  d.depth_snap = A.alloc<uint32_t>(NB);
+  d.bcolor = A.alloc<uint32_t>(NB);
+  d.bcolor_snap = A.alloc<uint32_t>(NB);
+  d.adj_off = A.alloc<uint32_t>(NB);
+  d.adj = A.alloc<uint32_t>(2 * (size_t)NP);
+  d.bbucket = A.alloc<uint32_t>(NB);
+  d.bbucket_off = A.alloc<uint32_t>((size_t)W * (kAvbdColors * kAvbdLayers + 1));
+  d.stats = A.alloc<uint32_t>((size_t)W * ST_COUNT);
+  if (std::find(A.ptrs.begin(), A.ptrs.end(), nullptr) != A.ptrs.end()) return BB_ERR_CAPACITY;
+  // host images
+  std::vector<uint32_t> id(NB), fl(NB), zero(NB, 0), ent(NB), rank(NB), aids(std::max(NA, 1u));
+  std::vector<float4> pos(NB), rot(NB), vel(NB), omg(NB), z4(NB, make_float4(0, 0, 0, 0));
+  std::vector<float4> pmin(NB), pmax(NB), mass(NB), i0(NB), i1(NB), i2(NB);
+  b->infos.resize(W);
+  for (uint32_t w = 0; w < W; ++w) {
+    HostWorld& hw = b->worlds[w];
+    WorldInfo wi = info_of(hw);
+    wi.drop_prev = 1;  // fresh layout: no previous manifolds
+    b->infos[w] = wi;
+    uint32_t k = 0;
+    for (uint32_t i = 0; i < hw.n; ++i) {
+      const bb_body_desc& bd = hw.bodies[i];
+      uint32_t g = hw.body_off + i;
+      id[g] = i;
+      fl[g] = bd.flags & (BB_BODY_DYNAMIC | BB_BODY_GRAVITY);
+      ent[g] = i;
+      pos[g] = make_float4(bd.position[0], bd.position[1], bd.position[2], 0.0f);
+      rot[g] = make_float4(bd.rotation[0], bd.rotation[1], bd.rotation[2], bd.rotation[3]);
+      vel[g] = make_float4(bd.velocity[0], bd.velocity[1], bd.velocity[2], 0.0f);
+      omg[g] = make_float4(bd.omega[0], bd.omega[1], bd.omega[2], 0.0f);
+      float r = bd.radius;
+      if (bd.shape == BB_SHAPE_SPHERE) {
+        pmin[g] = make_float4(-r, -r, -r, r);
+        pmax[g] = make_float4(r, r, r, 1.0f);
+      } else {
+        pmin[g] = make_float4(bd.minimum[0], bd.minimum[1], bd.minimum[2], r);
+        pmax[g] = make_float4(bd.maximum[0], bd.maximum[1], bd.maximum[2], 0.0f);
+      }
+      float inv_mass = bd.mass == 0.0f ? 0.0f : 1.0f / bd.mass;
+      mass[g] = make_float4(bd.mass, inv_mass, bd.friction, bd.restitution);
+      float inv[3];
+      inertia_of(hw.desc, bd, inv);
+      i0[g] = make_float4(inv[0], 0.0f, 0.0f, 0.0f);
+      i1[g] = make_float4(0.0f, inv[1], 0.0f, 0.0f);
+      i2[g] = make_float4(0.0f, 0.0f, inv[2], 0.0f);
+      if (bd.flags & BB_BODY_DYNAMIC) {
+        rank[g] = k;
+        aids[hw.act_off + k] = i;
+        ++k;
+      } else {
+        rank[g] = kMAX;
+      }
+    }
+  }
+  auto up = [&](void* dst, const void* src, size_t bytes) {
+    return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
+  };
+  for (int s = 0; s < 2; ++s) {
+    Rows& R = b->rows[s];
+    HIPCHK(up(R.id, id.data(), NB * 4));
+    HIPCHK(up(R.flags, fl.data(), NB * 4));
+    HIPCHK(up(R.sleep, zero.data(), NB * 4));
+    HIPCHK(up(R.pos, pos.data(), NB * 16));
+    HIPCHK(up(R.rot, rot.data(), NB * 16));
+    HIPCHK(up(R.vel, vel.data(), NB * 16));
+    HIPCHK(up(R.omg, omg.data(), NB * 16));
+    HIPCHK(up(R.pvel, z4.data(), NB * 16));
+    HIPCHK(up(b->entries[s], ent.data(), NB * 4));
+    HIPCHK(hipMemset(b->npairs[s], 0, W * 4));
+  }
+  HIPCHK(up(P.pmin, pmin.data(), NB * 16));
+  HIPCHK(up(P.pmax, pmax.data(), NB * 16));
+  HIPCHK(up(P.mass, mass.data(), NB * 16));
+  HIPCHK(up(P.ii0, i0.data(), NB * 16));
+  HIPCHK(up(P.ii1, i1.data(), NB * 16));
+  HIPCHK(up(P.ii2, i2.data(), NB * 16));
+  HIPCHK(up(P.active_rank, rank.data(), NB * 4));
+  HIPCHK(up(P.active_ids, aids.data(), std::max(NA, 1u) * 4));
+  HIPCHK(hipMemset(d.owner, 0, (size_t)NB * kColors * 4));
+  HIPCHK(hipMemset(d.stats, 0, (size_t)W * ST_COUNT * 4));
+  HIPCHK(up(d.worlds, b->infos.data(), W * sizeof(WorldInfo)));
+  b->latest = 1;  // the first step reads set 1 and writes set 0
+  b->built = true;
+  b->dirty = false;
+  b->stepped = false;
+  return BB_OK;
+}
+
+Dev dev_for(const bb_batch* b, int cur) {
+  Dev d = b->base;
+  int prev = cur ^ 1;
+  d.cur = b->rows[cur];
+  d.prev = b->rows[prev];
+  d.entries_cur = b->entries[cur];
+  d.entries_prev = b->entries[prev];
+  d.pairs_cur = b->pairs[cur];
+  d.pairs_prev = b->pairs[prev];
+  d.npairs_cur = b->npairs[cur];
+  d.npairs_prev = b->npairs[prev];
+  d.man_cur = b->man[cur];
+  d.man_prev = b->man[prev];
+  return d;
+}
+
+uint32_t pick_block(const bb_batch* b) {
+  if (b->block) return b->block;
+  uint32_t mx = 0;
+  for (auto& hw : b->worlds) mx = std::max(mx, hw.n);
+  if (mx >= 8192) return 1024;
+  if (mx >= 2048 && b->worlds.size() < 64) return 512;
+  return 256;
+}
+
+bb_status launch_one(bb_batch* b) {
+  const int cur = b->latest ^ 1;
+  Dev d = dev_for(b, cur);
+  launch_step(d, b->stream, pick_block(b));
+  HIPCHK(hipGetLastError());
+  b->latest = cur;
+  b->stepped = true;
+  ++b->step_index;
+  return BB_OK;
+}
+
+// One-shot per-world parameters (skip warm start / drop previous manifolds)
+// apply to exactly one step: the world table is rewritten before and after.
+bool any_one_shot(const bb_batch* b) {
+  for (auto& hw : b->worlds)
+    if (hw.skip_ws_once || hw.drop_prev) return true;
+  for (auto& wi : b->infos)
+    if (wi.drop_prev) return true;
+  return false;
+}
+
+bb_status upload_infos(bb_batch* b, bool one_shot) {
+  std::vector<WorldInfo> v = b->infos;
+  for (size_t w = 0; w < v.size(); ++w) {
+    HostWorld& hw = b->worlds[w];
+    if (one_shot) {
+      if (hw.skip_ws_once) v[w].flags &= ~(uint32_t)BB_SIM_WARM_STARTING;
+      if (hw.drop_prev) v[w].drop_prev = 1;
+    } else {
+      v[w].drop_prev = 0;
+    }
+  }
+  HIPCHK(hipStreamSynchronize(b->stream));
+  HIPCHK(hipMemcpy(b->base.worlds, v.data(), v.size() * sizeof(WorldInfo), hipMemcpyHostToDevice));
+  if (one_shot) {
+    for (auto& hw : b->worlds) hw.skip_ws_once = hw.drop_prev = false;
+  } else {
+    for (auto& wi : b->infos) wi.drop_prev = 0;
+  }
+  return BB_OK;
+}
+
+bb_status refresh_infos(bb_batch* b) {
+  for (size_t w = 0; w < b->worlds.size(); ++w) {
+    WorldInfo wi = info_of(b->worlds[w]);
+    wi.drop_prev = b->infos[w].drop_prev;
+    b->infos[w] = wi;
+  }
+  return BB_OK;
+}
+
+bb_status step_batch(bb_batch* b, uint32_t nsteps, bool use_graph) {
+  if (b->worlds.empty()) return BB_ERR_STATE;
+  bb_status st = build(b);
+  if (st != BB_OK) return st;
+  uint32_t done = 0;
+  while (done < nsteps && any_one_shot(b)) {
+    st = upload_infos(b, true);
+    if (st != BB_OK) return st;
+    st = launch_one(b);
+    if (st != BB_OK) return st;
+    st = upload_infos(b, false);
+    if (st != BB_OK) return st;
+    ++done;
+  }
+  if (!use_graph) {
+    for (; done < nsteps; ++done) {
+      st = launch_one(b);
+      if (st != BB_OK) return st;
+    }
+    return BB_OK;
+  }
+  // hipGraph: two consecutive steps (both parities) captured once, replayed
+  const int first = b->latest ^ 1;
+  if (b->graph && b->graph_parity != first) {
+    (void)hipGraphExecDestroy(b->graph);
+    b->graph = nullptr;
+  }
+  if (!b->graph && nsteps - done >= 2) {
+    hipGraph_t g;
+    HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    launch_step(dev_for(b, first), b->stream, pick_block(b));
+    launch_step(dev_for(b, first ^ 1), b->stream, pick_block(b));
+    HIPCHK(hipStreamEndCapture(b->stream, &g));
+    HIPCHK(hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0));
+    HIPCHK(hipGraphDestroy(g));
+    b->graph_parity = first;
+  }
+  while (nsteps - done >= 2 && b->graph) {
+    HIPCHK(hipGraphLaunch(b->graph, b->stream));
+    b->stepped = true;
+    b->step_index += 2;
+    done += 2;
+  }
+  for (; done < nsteps; ++done) {
+    st = launch_one(b);
+    if (st != BB_OK) return st;
+  }
+  return BB_OK;
+}
+
+bb_status sync_batch(bb_batch* b) {
+  if (!b->stream) return BB_OK;
+  HIPCHK(hipStreamSynchronize(b->stream));
+  return BB_OK;
+}
+
+bb_status read_world(bb_batch* b, uint32_t w, bb_body_state* out, uint32_t n, bb_order order) {
+  if (w >= b->worlds.size() || !out) return BB_ERR_INVALID;
+  HostWorld& hw = b->worlds[w];
+  if (n < hw.bodies.size()) return BB_ERR_INVALID;
+  bb_status st = build(b);
+  if (st != BB_OK) return st;
+  HIPCHK(hipStreamSynchronize(b->stream));
+  const uint32_t N = hw.n, g0 = hw.body_off;
+  const Rows& R = b->rows[b->latest];
+  std::vector<uint32_t> id(N), fl(N), isl(N, kMAX), ent(N);
+  std::vector<float4> pos(N), rot(N), vel(N), omg(N);
+  if (N) {
+    HIPCHK(hipMemcpy(id.data(), R.id + g0, N * 4, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(fl.data(), R.flags + g0, N * 4, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(pos.data(), R.pos + g0, N * 16, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(rot.data(), R.rot + g0, N * 16, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(vel.data(), R.vel + g0, N * 16, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(omg.data(), R.omg + g0, N * 16, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(ent.data(), b->entries[b->latest] + g0, N * 4, hipMemcpyDeviceToHost));
+    if (b->stepped) HIPCHK(hipMemcpy(isl.data(), b->base.islot + g0, N * 4, hipMemcpyDeviceToHost));
+  }
+  for (uint32_t k = 0; k < N; ++k) {
+    uint32_t row = order == BB_ORDER_BY_ID ? ent[k] : k;
+    bb_body_state& o = out[k];
+    o.id = id[row];
+    o.flags = fl[row];
+    o.island_index = isl[row];
+    o.row = row;
+    o.position[0] = pos[row].x; o.position[1] = pos[row].y; o.position[2] = pos[row].z;
+    o.rotation[0] = rot[row].x; o.rotation[1] = rot[row].y; o.rotation[2] = rot[row].z; o.rotation[3] = rot[row].w;
+    o.velocity[0] = vel[row].x; o.velocity[1] = vel[row].y; o.velocity[2] = vel[row].z;
+    o.omega[0] = omg[row].x; o.omega[1] = omg[row].y; o.omega[2] = omg[row].z;
+  }
+  return BB_OK;
+}
+
+bb_status world_stats(bb_batch* b, uint32_t w, bb_step_stats* out) {
+  if (w >= b->worlds.size() || !out) return BB_ERR_INVALID;
+  std::memset(out, 0, sizeof(*out));
+  if (!b->built || !b->stepped) {
+    out->rigid_body_count = (uint32_t)b->worlds[w].bodies.size();
+    return BB_OK;
+  }
+  HIPCHK(hipStreamSynchronize(b->stream));
+  uint32_t s[ST_COUNT];
+  HIPCHK(hipMemcpy(s, b->base.stats + (size_t)w * ST_COUNT, sizeof(s), hipMemcpyDeviceToHost));
+  out->rigid_body_count = s[ST_N];
+  out->active_rigid_body_count = s[ST_A];
+  out->broad_phase_collision_count = s[ST_P];
+  out->collision_count = s[ST_M];
+  out->manifold_node_count = s[ST_NODES];
+  out->contact_count = s[ST_C];
+  out->island_count = s[ST_ISLANDS];
+  out->contact_island_count = s[ST_CISLANDS];
+  out->sleeping_count = s[ST_SLEEPING];
+  out->graph_color_count = s[ST_GC_COUNT];
+  out->graph_color_violations = s[ST_GC_VIOL];
+  out->graph_color_overflow = s[ST_GC_OVF];
+  out->avbd_color_count = s[ST_AV_COUNT];
+  out->avbd_violations = s[ST_AV_VIOL];
+  out->avbd_stick_count = s[ST_AV_STICK];
+  out->dbg_fresh = s[ST_FRESH];
+  out->dbg_pen = s[ST_PEN];
+  out->dbg_id_sum = s[ST_IDSUM];
+  out->dbg_maxv = s[ST_MAXV];
+  out->overflow = s[ST_OVERFLOW];
+  out->step_index = b->step_index ? b->step_index - 1 : 0;
+  return BB_OK;
+}
+
+template <class T>
+bb_status copy_down(std::vector<T>& v, const T* src, size_t n) {
+  v.resize(n);
+  if (n) HIPCHK(hipMemcpy(v.data(), src, n * sizeof(T), hipMemcpyDeviceToHost));
+  return BB_OK;
+}
+
+bb_status tap_world(bb_batch* b, uint32_t w, bb_tap kind, void* dst, size_t cap, size_t* written) {
+  if (w >= b->worlds.size()) return BB_ERR_INVALID;
+  if (!b->built || !b->stepped) {
+    if (written) *written = 0;
+    return BB_OK;
+  }
+  HIPCHK(hipStreamSynchronize(b->stream));
+  const HostWorld& hw = b->worlds[w];
+  const WorldInfo& wi = b->infos[w];
+  const uint32_t N = hw.n, g0 = hw.body_off;
+  const int cur = b->latest;
+  std::vector<uint8_t> out;
+  auto emit = [&](const void* p, size_t bytes) {
+    out.assign((const uint8_t*)p, (const uint8_t*)p + bytes);
+  };
+  uint32_t P = 0;
+  HIPCHK(hipMemcpy(&P, b->npairs[cur] + w, 4, hipMemcpyDeviceToHost));
+  switch (kind) {
+    case BB_TAP_MORTON:
+    case BB_TAP_SORTED: {
+      std::vector<uint64_t> keys;
+      std::vector<uint32_t> idx;
+      bb_status s = copy_down(keys, b->base.keys_a + g0, N);
+      if (s != BB_OK) return s;
+      s = copy_down(idx, b->base.idx_a + g0, N);
+      if (s != BB_OK) return s;
+      if (kind == BB_TAP_SORTED) {
+        std::vector<uint32_t> v(4 * (size_t)N);
+        for (uint32_t i = 0; i < N; ++i) {
+          v[4 * i] = (uint32_t)keys[i];
+          v[4 * i + 1] = (uint32_t)(keys[i] >> 32);
+          v[4 * i + 2] = idx[i];
+          v[4 * i + 3] = 0;
+        }
+        emit(v.data(), v.size() * 4);
+      } else if (wi.morton_bits == 30) {
+        std::vector<uint32_t> v(N);
+        for (uint32_t i = 0; i < N; ++i) v[idx[i]] = (uint32_t)keys[i];
+        emit(v.data(), v.size() * 4);
+      } else {
+        std::vector<uint64_t> v(N);
+        for (uint32_t i = 0; i < N; ++i) v[idx[i]] = keys[i];
+        emit(v.data(), v.size() * 8);
+      }
+      break;
+    }
+    case BB_TAP_LBVH: {
+      std::vector<float4> mn, mx;
+      size_t nn = N ? 2 * (size_t)N - 1 : 0;
+      bb_status s = copy_down(mn, b->base.nmin + 2 * (size_t)g0, nn);
+      if (s != BB_OK) return s;
+      s = copy_down(mx, b->base.nmax + 2 * (size_t)g0, nn);
+      if (s != BB_OK) return s;
+      std::vector<bb_lbvh_node> v(nn);
+      for (size_t i = 0; i < nn; ++i) {
+        v[i].minimum[0] = mn[i].x; v[i].minimum[1] = mn[i].y; v[i].minimum[2] = mn[i].z;
+        v[i].maximum[0] = mx[i].x; v[i].maximum[1] = mx[i].y; v[i].maximum[2] = mx[i].z;
+        int l, r;
+        std::memcpy(&l, &mn[i].w, 4);
+        std::memcpy(&r, &mx[i].w, 4);
+        if (l < 0) {
+          v[i].left = -1;
+          v[i].right = -1;
+          v[i].primitive_idx = (uint32_t)r;
+        } else {
+          v[i].left = l;
+          v[i].right = r;
+          v[i].primitive_idx = kMAX;
+        }
+        v[i].pad = 0;
+      }
+      emit(v.data(), v.size() * sizeof(bb_lbvh_node));
+      break;
+    }
+    case BB_TAP_PAIRS: {
+      std::vector<uint2> v;
+      bb_status s = copy_down(v, b->pairs[cur] + hw.pair_off, P);
+      if (s != BB_OK) return s;
+      emit(v.data(), v.size() * 8);
+      break;
+    }
+    case BB_TAP_MANIFOLDS:
+    case BB_TAP_EDGE_COLORS: {
+      const Manifolds& M = b->man[cur];
+      std::vector<int4> hdr;
+      std::vector<float4> nrm, t0, t1, aa, ab, cpos, clam, cmas, cana, canb;
+      std::vector<uint4> cfp;
+      std::vector<uint32_t> cws, col;
+      const size_t so = hw.pair_off;
+      bb_status s = copy_down(hdr, M.hdr + so, P);
+      if (s == BB_OK) s = copy_down(col, b->base.color + so, P);
+      if (kind == BB_TAP_EDGE_COLORS) {
+        if (s != BB_OK) return s;
+        std::vector<uint32_t> v;
+        for (uint32_t p = 0; p < P; ++p)
+          if (hdr[p].w > 0) v.push_back(wi.solver == BB_SOLVER_AVBD ? kMAX : col[p]);
+        emit(v.data(), v.size() * 4);
+        break;
+      }
+      if (s == BB_OK) s = copy_down(nrm, M.nrm + so, P);
+      if (s == BB_OK) s = copy_down(t0, M.t0 + so, P);
+      if (s == BB_OK) s = copy_down(t1, M.t1 + so, P);
+      if (s == BB_OK) s = copy_down(aa, M.anca + so, P);
+      if (s == BB_OK) s = copy_down(ab, M.ancb + so, P);
+      if (s == BB_OK) s = copy_down(cpos, M.cpos + so * 8, (size_t)P * 8);
+      if (s == BB_OK) s = copy_down(clam, M.clam + so * 8, (size_t)P * 8);
+      if (s == BB_OK) s = copy_down(cmas, M.cmas + so * 8, (size_t)P * 8);
+      if (s == BB_OK) s = copy_down(cana, M.cana + so * 8, (size_t)P * 8);
+      if (s == BB_OK) s = copy_down(canb, M.canb + so * 8, (size_t)P * 8);
+      if (s == BB_OK) s = copy_down(cfp, M.cfp + so * 8, (size_t)P * 8);
+      if (s == BB_OK) s = copy_down(cws, M.cws + so * 8, (size_t)P * 8);
+      if (s != BB_OK) return s;
+      std::vector<bb_manifold_tap> v;
+      for (uint32_t p = 0; p < P; ++p) {
+        if (hdr[p].w <= 0) continue;
+        bb_manifold_tap t;
+        std::memset(&t, 0, sizeof(t));
+        t.obb1_index = (uint32_t)hdr[p].x;
+        t.obb2_index = (uint32_t)hdr[p].y;
+        t.key = hdr[p].z;
+        t.contact_count = hdr[p].w;
+        std::memcpy(&t.error, &nrm[p].w, 4);
+        t.color = wi.solver == BB_SOLVER_AVBD ? kMAX : col[p];
+        t.normal[0] = nrm[p].x; t.normal[1] = nrm[p].y; t.normal[2] = nrm[p].z;
+        t.tangent[0][0] = t0[p].x; t.tangent[0][1] = t0[p].y; t.tangent[0][2] = t0[p].z;
+        t.tangent[1][0] = t1[p].x; t.tangent[1][1] = t1[p].y; t.tangent[1][2] = t1[p].z;
+        t.anchor_a[0] = aa[p].x; t.anchor_a[1] = aa[p].y; t.anchor_a[2] = aa[p].z;
+        t.anchor_b[0] = ab[p].x; t.anchor_b[1] = ab[p].y; t.anchor_b[2] = ab[p].z;
+        for (int c = 0; c < hdr[p].w && c < 8; ++c) {
+          size_t k = (size_t)p * 8 + c;
+          bb_contact_tap& o = t.contacts[c];
+          o.position[0] = cpos[k].x; o.position[1] = cpos[k].y; o.position[2] = cpos[k].z;
+          o.penetration = cpos[k].w;
+          o.normal_impulse = clam[k].x;
+          o.tangent_impulse[0] = clam[k].y;
+          o.tangent_impulse[1] = clam[k].z;
+          o.mass_coefficient = clam[k].w;
+          o.tangent_mass[0] = cmas[k].y;
+          o.tangent_mass[1] = cmas[k].z;
+          o.anchor_a[0] = cana[k].x; o.anchor_a[1] = cana[k].y; o.anchor_a[2] = cana[k].z;
+          o.anchor_b[0] = canb[k].x; o.anchor_b[1] = canb[k].y; o.anchor_b[2] = canb[k].z;
+          std::memcpy(&o.stick, &canb[k].w, 4);
+          o.fp[0] = cfp[k].x; o.fp[1] = cfp[k].y; o.fp[2] = cfp[k].z; o.fp[3] = cfp[k].w;
+          o.warm_start = cws[k];
+        }
+        v.push_back(t);
+      }
+      emit(v.data(), v.size() * sizeof(bb_manifold_tap));
+      break;
+    }
+    case BB_TAP_ISLANDS: {
+      std::vector<uint32_t> v;
+      bb_status s = copy_down(v, b->base.island + g0, N);
+      if (s != BB_OK) return s;
+      emit(v.data(), v.size() * 4);
+      break;
+    }
+    case BB_TAP_BODY_COLORS: {
+      std::vector<uint32_t> v(N, kMAX);
+      if (wi.solver == BB_SOLVER_AVBD) {
+        bb_status s = copy_down(v, b->base.bcolor + g0, N);
+        if (s != BB_OK) return s;
+      }
+      emit(v.data(), v.size() * 4);
+      break;
+    }
+    case BB_TAP_ROWS: {
+      std::vector<bb_body_state> v(N);
+      bb_status s = read_world(b, w, v.data(), N, BB_ORDER_BY_ROW);
+      if (s != BB_OK) return s;
+      emit(v.data(), v.size() * sizeof(bb_body_state));
+      break;
+    }
+    default:
+      return BB_ERR_INVALID;
+  }
+  if (written) *written = out.size();
+  if (!dst) return BB_OK;
+  if (out.size() > cap) return BB_ERR_INVALID;
+  std::memcpy(dst, out.data(), out.size());
+  return BB_OK;
+}
+
+bool valid_desc(const bb_world_desc* d) {
+  if (!d) return false;
+  if (d->solver > BB_SOLVER_AVBD) return false;
+  if (!(d->dt > 0.0f) || !std::isfinite(d->dt) || !std::isfinite(d->gravity)) return false;
+  if (d->morton_bits != 30 && d->morton_bits != 63) return false;
+  return true;
+}
+
+bool valid_body(const bb_body_desc& b) {
+  if (b.shape > BB_SHAPE_SPHERE) return false;
+  if (!(b.mass >= 0.0f) || !std::isfinite(b.mass)) return false;
+  for (int k = 0; k < 3; ++k)
+    if (!std::isfinite(b.position[k]) || !std::isfinite(b.velocity[k]) || !std::isfinite(b.omega[k])) return false;
+  for (int k = 0; k < 4; ++k)
+    if (!std::isfinite(b.rotation[k])) return false;
+  if (b.shape == BB_SHAPE_SPHERE && !(b.radius > 0.0f)) return false;
+  if (b.shape == BB_SHAPE_BOX)
+    for (int k = 0; k < 3; ++k)
+      if (!(b.maximum[k] > b.minimum[k])) return false;
+  return true;
+}
+
+}  // namespace
+
+// =================================================================== C ABI
+extern "C" {
+
+const char* bb_version(void) { return "beatbox-amd 0.1 (gfx950)"; }
+
+void bb_world_desc_default(bb_world_desc* d) {
+  if (!d) return;
+  d->capacity_bodies = 0;
+  d->solver = BB_SOLVER_AVBD;
+  d->iterations = 5;
+  d->flags = BB_SIM_DEFAULT_FLAGS;
+  d->dt = 0.01667f;
+  d->gravity = -9.81f;
+  d->morton_bits = 30;
+  d->quirks = BB_QUIRKS_PARITY;
+}
+
+bb_status bb_batch_create(uint32_t capacity_worlds, uint32_t capacity_bodies, bb_batch** out) {
+  if (!out) return BB_ERR_INVALID;
+  bb_batch* b = new (std::nothrow) bb_batch();
+  if (!b) return BB_ERR_CAPACITY;
+  b->worlds.reserve(capacity_worlds);
+  (void)capacity_bodies;
+  *out = b;
+  return BB_OK;
+}
+
+bb_status bb_batch_add_world(bb_batch* b, const bb_world_desc* desc, const bb_body_desc* bodies, uint32_t n,
+                             uint32_t* out_world) {
+  if (!b || !valid_desc(desc) || (n && !bodies)) return BB_ERR_INVALID;
+  for (uint32_t i = 0; i < n; ++i)
+    if (!valid_body(bodies[i])) return BB_ERR_INVALID;
+  if (desc->capacity_bodies && n > desc->capacity_bodies) return BB_ERR_CAPACITY;
+  if (desc->morton_bits == 30 && n > 1024 && (desc->quirks & BB_QUIRKS_PARITY) == BB_QUIRKS_PARITY) {
+    // parity mode keeps the reference's cap (MAX_RIGID_BODY_COUNT = 1024,
+    // include/defines.hpp:357; build_accel_structs refuses larger scenes)
+    return BB_ERR_CAPACITY;
+  }
+  bb_status st = pull_state(b);
+  if (st != BB_OK) return st;
+  HostWorld hw;
+  hw.desc = *desc;
+  hw.bodies.assign(bodies, bodies + n);
+  b->worlds.push_back(hw);
+  b->dirty = true;
+  if (out_world) *out_world = (uint32_t)b->worlds.size() - 1;
+  return BB_OK;
+}
+
+bb_status bb_batch_step(bb_batch* b, uint32_t nsteps, void* stream, int use_graph) {
+  if (!b) return BB_ERR_INVALID;
+  (void)stream;
+  return step_batch(b, nsteps, use_graph != 0);
+}
+
+bb_status bb_batch_sync(bb_batch* b) { return b ? sync_batch(b) : BB_ERR_INVALID; }
+
+bb_status bb_batch_read_world(bb_batch* b, uint32_t w, bb_body_state* out, uint32_t n, bb_order order) {
+  return b ? read_world(b, w, out, n, order) : BB_ERR_INVALID;
+}
+
+bb_status bb_batch_world_stats(bb_batch* b, uint32_t w, bb_step_stats* out) {
+  return b ? world_stats(b, w, out) : BB_ERR_INVALID;
+}
+
+bb_status bb_batch_debug_tap(bb_batch* b, uint32_t w, bb_tap kind, void* dst, size_t cap, size_t* written) {
+  return b ? tap_world(b, w, kind, dst, cap, written) : BB_ERR_INVALID;
+}
+
+uint32_t bb_batch_world_count(const bb_batch* b) { return b ? (uint32_t)b->worlds.size() : 0; }
+
+uint32_t bb_batch_body_count(const bb_batch* b) {
+  if (!b) return 0;
+  uint32_t n = 0;
+  for (auto& hw : b->worlds) n += (uint32_t)hw.bodies.size();
+  return n;
+}
+
+bb_status bb_batch_set_block_size(bb_batch* b, uint32_t threads) {
+  if (!b) return BB_ERR_INVALID;
+  if (threads != 0 && threads != 256 && threads != 512 && threads != 1024) return BB_ERR_INVALID;
+  b->block = threads;
+  if (b->graph) {
+    (void)hipGraphExecDestroy(b->graph);
+    b->graph = nullptr;
+  }
+  return BB_OK;
+}
+
+void bb_batch_destroy(bb_batch* b) {
+  if (!b) return;
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  if (b->graph) (void)hipGraphExecDestroy(b->graph);
+  b->buf.release();
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+}
+
+// ------------------------------------------------------------ world API
+bb_status bb_world_create(const bb_world_desc* desc, bb_world** out) {
+  if (!out || !valid_desc(desc)) return BB_ERR_INVALID;
+  bb_world* w = new (std::nothrow) bb_world();
+  if (!w) return BB_ERR_CAPACITY;
+  bb_status st = bb_batch_create(1, desc->capacity_bodies, &w->b);
+  if (st != BB_OK) {
+    delete w;
+    return st;
+  }
+  HostWorld hw;
+  hw.desc = *desc;
+  w->b->worlds.push_back(hw);
+  *out = w;
+  return BB_OK;
+}
+
+bb_status bb_world_add_bodies(bb_world* w, const bb_body_desc* bodies, uint32_t n, uint32_t* out_ids) {
+  if (!w || (n && !bodies)) return BB_ERR_INVALID;
+  HostWorld& hw = w->b->worlds[0];
+  for (uint32_t i = 0; i < n; ++i)
+    if (!valid_body(bodies[i])) return BB_ERR_INVALID;
+  size_t total = hw.bodies.size() + n;
+  if (hw.desc.capacity_bodies && total > hw.desc.capacity_bodies) return BB_ERR_CAPACITY;
+  if (hw.desc.morton_bits == 30 && (hw.desc.quirks & BB_QUIRKS_PARITY) == BB_QUIRKS_PARITY && total > 1024)
+    return BB_ERR_CAPACITY;
+  bb_status st = pull_state(w->b);
+  if (st != BB_OK) return st;
+  for (uint32_t i = 0; i < n; ++i) {
+    if (out_ids) out_ids[i] = (uint32_t)hw.bodies.size();
+    hw.bodies.push_back(bodies[i]);
+  }
+  w->b->dirty = true;
+  return BB_OK;
+}
+
+bb_status bb_world_set_flags(bb_world* w, uint32_t flags) {
+  if (!w) return BB_ERR_INVALID;
+  w->b->worlds[0].desc.flags |= flags;
+  return w->b->built ? (refresh_infos(w->b), upload_infos(w->b, false)) : BB_OK;
+}
+
+bb_status bb_world_clear_flags(bb_world* w, uint32_t flags) {
+  if (!w) return BB_ERR_INVALID;
+  w->b->worlds[0].desc.flags &= ~flags;
+  return w->b->built ? (refresh_infos(w->b), upload_infos(w->b, false)) : BB_OK;
+}
+
+uint32_t bb_world_get_flags(const bb_world* w) { return w ? w->b->worlds[0].desc.flags : 0u; }
+
+bb_status bb_world_set_solver(bb_world* w, uint32_t solver) {
+  if (!w || solver > BB_SOLVER_AVBD) return BB_ERR_INVALID;
+  w->b->worlds[0].desc.solver = solver;
+  w->b->worlds[0].drop_prev = true;
+  return w->b->built ? (refresh_infos(w->b), upload_infos(w->b, false)) : BB_OK;
+}
+
+bb_status bb_world_set_iterations(bb_world* w, uint32_t iterations) {
+  if (!w) return BB_ERR_INVALID;
+  w->b->worlds[0].desc.iterations = iterations;
+  return w->b->built ? (refresh_infos(w->b), upload_infos(w->b, false)) : BB_OK;
+}
+
+bb_status bb_world_skip_warm_start_once(bb_world* w) {
+  if (!w) return BB_ERR_INVALID;
+  w->b->worlds[0].skip_ws_once = true;
+  return BB_OK;
+}
+
+bb_status bb_world_step(bb_world* w, uint32_t nsteps, void* stream) {
+  if (!w) return BB_ERR_INVALID;
+  return bb_batch_step(w->b, nsteps, stream, 0);
+}
+
+bb_status bb_world_sync(bb_world* w) { return w ? sync_batch(w->b) : BB_ERR_INVALID; }
+
+bb_status bb_world_read_bodies(bb_world* w, bb_body_state* out, uint32_t n, bb_order order) {
+  return w ? read_world(w->b, 0, out, n, order) : BB_ERR_INVALID;
+}
+
+bb_status bb_world_read_stats(bb_world* w, bb_step_stats* out) {
+  return w ? world_stats(w->b, 0, out) : BB_ERR_INVALID;
+}
+
+bb_status bb_world_debug_tap(bb_world* w, bb_tap kind, void* dst, size_t cap, size_t* written) {
+  return w ? tap_world(w->b, 0, kind, dst, cap, written) : BB_ERR_INVALID;
+}
+
+uint32_t bb_world_body_count(const bb_world* w) { return w ? (uint32_t)w->b->worlds[0].bodies.size() : 0; }
+
+void bb_world_destroy(bb_world* w) {
+  if (!w) return;
+  bb_batch_destroy(w->b);
+  delete w;
+}
+
+}  // extern "C"

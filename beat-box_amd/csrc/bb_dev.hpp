@@ -1,0 +1,133 @@
+// bb_dev.hpp — device data layout of a batch of worlds (host + device).
+//
+// All worlds of a batch live in one set of arrays.  World w owns body rows
+// [body_off, body_off + n), LBVH node slots [2*body_off, 2*body_off + 2n),
+// pair/manifold slots [pair_off, pair_off + pair_cap) and active slots
+// [act_off, act_off + a).  Body state is structure-of-arrays, double
+// buffered: step K reads parity K^1 ("prev", the last step's Morton rows) and
+// gathers it into parity K ("cur") in the new Morton order, so the row copy at
+// the end of the reference step (entry_update_rigid_bodies) is the reorder
+// itself.  Immutable per-body properties are indexed by id and never move.
+#pragma once
+#include <stdint.h>
+
+#include <hip/hip_runtime.h>
+
+namespace bb {
+
+static constexpr uint32_t kMaxContacts = 8;   // MAX_CONTACT_POINT_COUNT
+static constexpr uint32_t kColors = 32;       // BB_MAX_COLORS
+static constexpr uint32_t kBuckets = kColors + 1;  // + overflow bucket
+static constexpr uint32_t kAvbdColors = 32;
+static constexpr uint32_t kAvbdLayers = 12;
+static constexpr uint32_t kMAX = 0xFFFFFFFFu;
+
+// per-world statistics slots (bb_step_stats order)
+enum StatSlot : uint32_t {
+  ST_N = 0, ST_A, ST_P, ST_M, ST_NODES, ST_C, ST_ISLANDS, ST_CISLANDS, ST_SLEEPING,
+  ST_GC_COUNT, ST_GC_VIOL, ST_GC_OVF, ST_AV_COUNT, ST_AV_VIOL, ST_AV_STICK, ST_FRESH,
+  ST_PEN, ST_IDSUM, ST_MAXV, ST_OVERFLOW, ST_USED_COLORS, ST_COUNT
+};
+
+struct WorldInfo {  // 64 bytes
+  uint32_t body_off, n, a, act_off;
+  uint32_t pair_off, pair_cap, flags, solver;
+  uint32_t iterations, morton_bits, quirks, prev_n;
+  float dt, gravity, damp;
+  uint32_t drop_prev;
+};
+
+struct Rows {  // one parity of body rows
+  uint32_t* id;
+  uint32_t* flags;
+  uint32_t* sleep;
+  float4* pos;
+  float4* rot;
+  float4* vel;
+  float4* omg;
+  float4* pvel;
+};
+
+struct Props {  // by global id slot (body_off + id)
+  float4* pmin;  // minimum xyz, w = radius
+  float4* pmax;  // maximum xyz, w = shape (0 box, 1 sphere) as float
+  float4* mass;  // mass, inv_mass, friction, restitution
+  float4* ii0;   // inv_inertia rows
+  float4* ii1;
+  float4* ii2;
+  uint32_t* active_rank;  // active index (ascending id among DYNAMIC), MAX if static
+  uint32_t* active_ids;   // [act_off + k] = id
+};
+
+struct Manifolds {  // pair-indexed slots; contact arrays are [slot*8 + c]
+  int4* hdr;     // a, b, key, count (count 0: no manifold)
+  float4* nrm;   // normal xyz, error (int bits)
+  float4* t0;
+  float4* t1;
+  float4* anca;  // body a position at detection
+  float4* ancb;
+  float4* cpos;  // contact position xyz, penetration
+  float4* clam;  // normal impulse, tangent impulse 0/1, mass coefficient
+  float4* cmas;  // normal mass, tangent mass 0/1, bias factor
+  float4* cana;  // anchor a xyz, impulse coefficient
+  float4* canb;  // anchor b xyz, stick (uint bits)
+  uint4* cfp;    // feature pair in_ref, out_ref, in_inc, out_inc
+  uint32_t* cws; // warm start counter
+};
+
+struct Dev {
+  WorldInfo* worlds;
+  uint32_t W;
+  Rows cur, prev;
+  Props props;
+  uint32_t* entries_cur;   // [body_off + id] = row
+  uint32_t* entries_prev;
+  // sort scratch [body]
+  uint64_t* keys_a;
+  uint64_t* keys_b;
+  uint32_t* idx_a;
+  uint32_t* idx_b;
+  // LBVH [2*body]
+  float4* nmin;  // xyz, w = left child (int bits, -1 for a leaf)
+  float4* nmax;  // xyz, w = right child (int bits) or the leaf row
+  uint32_t* parent;
+  uint32_t* visit;
+  // broad phase [body]
+  uint32_t* pcount;
+  uint32_t* poff;
+  // pairs / manifolds [pair slot]
+  uint2* pairs_cur;
+  uint2* pairs_prev;
+  uint32_t* npairs_cur;  // [W]
+  uint32_t* npairs_prev;
+  Manifolds man_cur, man_prev;
+  // islands / sleep / colouring
+  uint32_t* link;     // [active slot]
+  uint32_t* deg;      // [body] manifolds per row
+  uint32_t* island;   // [body] island root (tap)
+  uint32_t* islot;    // [body] island slot (ascending root order)
+  uint32_t* ascratch; // [active slot] per-island scratch
+  uint32_t* color;    // [pair slot]
+  uint32_t* pick;     // [pair slot]
+  uint32_t* mask;     // [body]
+  uint32_t* owner;    // [body*32], zero between rounds
+  uint32_t* bucket;   // [pair slot] manifold slots grouped by colour
+  uint32_t* bucket_off;  // [W * (kBuckets+1)]
+  float4* iw;         // [body*3] cached world inverse inertia rows
+  // AVBD
+  float4* s_pos0;     // [body] pos_start, w = support depth bits
+  float4* s_rot0;
+  float4* s_post;
+  float4* s_rott;
+  uint32_t* depth;    // [body]
+  uint32_t* depth_snap;
+  uint32_t* bcolor;   // [body]
+  uint32_t* bcolor_snap;
+  uint32_t* adj_off;  // [body] CSR offsets (world-local, exclusive)
+  uint32_t* adj;      // [2*pair slot]
+  uint32_t* bbucket;  // [body] rows grouped by (colour, layer)
+  uint32_t* bbucket_off;  // [W * (32*12+1)]
+  uint32_t* stats;    // [W * ST_COUNT]
+};
+
+}  // namespace bb

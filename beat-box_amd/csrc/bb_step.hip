@@ -1,0 +1,1722 @@
+// bb_step.hip — the rigid-body step on gfx950, one workgroup per world.
+//
+// The reference step (RigidBodyManager::simulate, src/rigid_body_manager.cpp:
+// 1606-1622) issues ~1,350-1,550 barrier-separated dispatches per step.  Here
+// each stage is a kernel whose grid has one workgroup per world; inside a
+// workgroup the stage's passes are separated by s_barrier instead of kernel
+// boundaries, so a batch of thousands of worlds runs every stage in one
+// launch and a single world pays a handful of launches per step.
+//
+// Stages (reference pass order, src/rigid_body_manager.cpp:1284-1427):
+//   k_build   Morton keys, stable LSD radix sort, Karras LBVH, bounds, reorder
+//   k_broad   per-row BVH traversal, pairs in (a, b) order
+//   k_narrow  SAT + clipping (+ sphere closed forms), warm start, union-find
+//   k_post    advect, islands, sleeping, edge colouring, colour buckets
+//   k_pgs     PGS_SOFT colour sweeps + position integration
+//   k_avbd_*  AVBD body colouring, prepare, warm start, primal/dual sweeps
+//   k_finish  end-of-step counters
+// Everything is deterministic: results do not depend on thread timing, and a
+// world's results do not depend on the other worlds of the batch.
+#include <hip/hip_runtime.h>
+
+#include "../../include/bb_capi.h"
+#include "bb_dev.hpp"
+#include "bb_math.hpp"
+
+namespace bb {
+
+#define WAVE 64
+
+// ============================================================ block helpers
+template <int BLOCK>
+struct ScanLds {
+  uint32_t w[BLOCK / WAVE + 1];
+};
+
+// Exclusive block scan of one value per thread; returns the prefix and the
+// block total.  All threads of the block must call it.
+template <int BLOCK>
+BB_DI uint32_t block_scan(uint32_t v, ScanLds<BLOCK>& s, uint32_t& total) {
+  const uint32_t lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x / WAVE;
+  uint32_t x = v;
+#pragma unroll
+  for (int o = 1; o < WAVE; o <<= 1) {
+    uint32_t y = __shfl_up(x, o, WAVE);
+    if (lane >= (uint32_t)o) x += y;
+  }
+  if (lane == WAVE - 1) s.w[wave] = x;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t sum = 0;
+    for (int k = 0; k < BLOCK / WAVE; ++k) {
+      uint32_t t = s.w[k];
+      s.w[k] = sum;
+      sum += t;
+    }
+    s.w[BLOCK / WAVE] = sum;
+  }
+  __syncthreads();
+  uint32_t r = s.w[wave] + x - v;
+  total = s.w[BLOCK / WAVE];
+  __syncthreads();
+  return r;
+}
+
+BB_DI void stat_add(const Dev& d, uint32_t w, uint32_t slot, uint32_t v) {
+  if (v) atomicAdd(&d.stats[w * ST_COUNT + slot], v);
+}
+BB_DI void stat_max(const Dev& d, uint32_t w, uint32_t slot, uint32_t v) {
+  atomicMax(&d.stats[w * ST_COUNT + slot], v);
+}
+
+// ============================================================ body helpers
+struct Shape {
+  v3 mn, mx;
+  float radius;
+  uint32_t kind;
+};
+BB_DI Shape shape_of(const Dev& d, uint32_t gid) {
+  float4 a = d.props.pmin[gid], b = d.props.pmax[gid];
+  return Shape{ld3(a), ld3(b), a.w, b.w != 0.0f ? 1u : 0u};
+}
+
+// RigidBody::get_AABB_from_OBB (src/shaders/extensions.slang:195-210);
+// spheres (extension) use centre +- radius.
+BB_DI void body_aabb(const Shape& s, v3 pos, qt q, v3& mn, v3& mx) {
+  if (s.kind == BB_SHAPE_SPHERE) {
+    v3 r = V(s.radius, s.radius, s.radius);
+    mn = pos - r;
+    mx = pos + r;
+    return;
+  }
+  mn = V(3.402823466e+38f, 3.402823466e+38f, 3.402823466e+38f);
+  mx = V(-3.402823466e+38f, -3.402823466e+38f, -3.402823466e+38f);
+#pragma unroll
+  for (uint32_t i = 0; i < 8; ++i) {
+    v3 c;
+    c.x = (i & 1) == 0 ? s.mn.x : s.mx.x;
+    c.y = (i & 2) == 0 ? s.mn.y : s.mx.y;
+    c.z = (i & 4) == 0 ? s.mn.z : s.mx.z;
+    c = qrot(q, c) + pos;
+    mn = vmin(mn, c);
+    mx = vmax(mx, c);
+  }
+}
+
+// World inverse inertia (src/shaders/extensions.slang:184-188): with the
+// parity quirk Rt*(I*transpose(Rt)) where Rt = get_rotation_matrix() =
+// transpose(R); otherwise R*(I*R^T).
+BB_DI mat3 world_inv_inertia(const Dev& d, uint32_t gid, qt q, bool quirk) {
+  mat3 I = mrows(ld3(d.props.ii0[gid]), ld3(d.props.ii1[gid]), ld3(d.props.ii2[gid]));
+  mat3 R = qmat(q);
+  if (quirk) {
+    mat3 Rt = mT(R);
+    return mmul(Rt, mmul(I, mT(Rt)));
+  }
+  return mmul(R, mmul(I, mT(R)));
+}
+
+// ================================================================== Morton
+// MortonCodeGenerator (src/shaders/simulation/LBVH.slang:4-32)
+BB_DI uint32_t expand10(uint32_t v) {
+  v = (v * 0x00010001u) & 0xFF0000FFu;
+  v = (v * 0x00000101u) & 0x0F00F00Fu;
+  v = (v * 0x00000011u) & 0xC30C30C3u;
+  v = (v * 0x00000005u) & 0x49249249u;
+  return v;
+}
+BB_DI uint32_t morton30(v3 p) {
+  float x = fmn(fmx(p.x * 1024.0f, 0.0f), 1023.0f);
+  float y = fmn(fmx(p.y * 1024.0f, 0.0f), 1023.0f);
+  float z = fmn(fmx(p.z * 1024.0f, 0.0f), 1023.0f);
+  return expand10((uint32_t)x) * 4 + expand10((uint32_t)y) * 2 + expand10((uint32_t)z);
+}
+// 63-bit extension: 21 bits per axis, x in the highest position.
+BB_DI uint64_t expand21(uint64_t x) {
+  x &= 0x1fffffull;
+  x = (x | x << 32) & 0x1f00000000ffffull;
+  x = (x | x << 16) & 0x1f0000ff0000ffull;
+  x = (x | x << 8) & 0x100f00f00f00f00full;
+  x = (x | x << 4) & 0x10c30c30c30c30c3ull;
+  x = (x | x << 2) & 0x1249249249249249ull;
+  return x;
+}
+BB_DI uint64_t morton63(v3 p) {
+  float x = fmn(fmx(p.x * 2097152.0f, 0.0f), 2097151.0f);
+  float y = fmn(fmx(p.y * 2097152.0f, 0.0f), 2097151.0f);
+  float z = fmn(fmx(p.z * 2097152.0f, 0.0f), 2097151.0f);
+  return (expand21((uint64_t)x) << 2) | (expand21((uint64_t)y) << 1) | expand21((uint64_t)z);
+}
+
+// =========================================================== block sort
+// Stable LSD radix sort of (key, index) with 8-bit digits.  Per pass: an LDS
+// digit histogram, a block scan of the 256 bins, then a stable scatter in
+// chunks of BLOCK elements: wave ballots over the 8 digit bits give each lane
+// its peers (same digit) in the wave, per-wave digit counts in LDS give the
+// offsets of lower waves.  Passes whose digit is constant are skipped (a
+// stable pass over one digit value is the identity).
+template <int BLOCK>
+struct SortLds {
+  uint32_t base[256];
+  uint32_t wcnt[(BLOCK / WAVE) * 256];
+  ScanLds<BLOCK> scan;
+};
+
+template <int BLOCK>
+BB_DI bool block_radix_sort(uint64_t* ka, uint32_t* ia, uint64_t* kb, uint32_t* ib, uint32_t n, int passes,
+                            SortLds<BLOCK>& L) {
+  const uint32_t tid = threadIdx.x, lane = tid & (WAVE - 1), wave = tid / WAVE;
+  for (uint32_t i = tid; i < (BLOCK / WAVE) * 256; i += BLOCK) L.wcnt[i] = 0;
+  bool in_a = true;
+  for (int pass = 0; pass < passes; ++pass) {
+    const uint32_t shift = 8u * pass;
+    uint64_t* kin = in_a ? ka : kb;
+    uint32_t* iin = in_a ? ia : ib;
+    uint64_t* kout = in_a ? kb : ka;
+    uint32_t* iout = in_a ? ib : ia;
+    for (uint32_t i = tid; i < 256; i += BLOCK) L.base[i] = 0;
+    __syncthreads();
+    for (uint32_t i = tid; i < n; i += BLOCK) atomicAdd(&L.base[(uint32_t)(kin[i] >> shift) & 255u], 1u);
+    __syncthreads();
+    const bool skip = n == 0 || L.base[(uint32_t)(kin[0] >> shift) & 255u] == n;
+    __syncthreads();
+    if (skip) continue;
+    {
+      uint32_t total;
+      uint32_t v = tid < 256 ? L.base[tid] : 0u;
+      uint32_t ex = block_scan<BLOCK>(v, L.scan, total);
+      if (tid < 256) L.base[tid] = ex;
+      __syncthreads();
+    }
+    for (uint32_t c0 = 0; c0 < n; c0 += BLOCK) {
+      const uint32_t i = c0 + tid;
+      const bool valid = i < n;
+      uint64_t key = valid ? kin[i] : 0ull;
+      uint32_t idx = valid ? iin[i] : 0u;
+      uint32_t dg = (uint32_t)(key >> shift) & 255u;
+      uint64_t peers = __ballot(valid);
+#pragma unroll
+      for (int b = 0; b < 8; ++b) {
+        uint64_t m = __ballot((dg >> b) & 1u);
+        peers &= ((dg >> b) & 1u) ? m : ~m;
+      }
+      const uint32_t rank = __popcll(peers & ((1ull << lane) - 1ull));
+      const uint32_t cnt = __popcll(peers);
+      const bool leader = valid && lane == (uint32_t)(__ffsll((unsigned long long)peers) - 1);
+      if (leader) L.wcnt[wave * 256 + dg] = cnt;
+      __syncthreads();
+      if (valid) {
+        uint32_t off = L.base[dg] + rank;
+        for (uint32_t w2 = 0; w2 < wave; ++w2) off += L.wcnt[w2 * 256 + dg];
+        kout[off] = key;
+        iout[off] = idx;
+      }
+      __syncthreads();
+      if (leader) {
+        atomicAdd(&L.base[dg], cnt);
+        L.wcnt[wave * 256 + dg] = 0;
+      }
+      __syncthreads();
+    }
+    in_a = !in_a;
+  }
+  return in_a;  // true: sorted data in (ka, ia)
+}
+
+// =================================================================== LBVH
+struct KeyView {
+  const uint64_t* k;
+  int n;
+  int bits;
+};
+// LBVH::delta (src/shaders/simulation/LBVH.slang:159-170)
+BB_DI int lb_delta(const KeyView& K, int i, uint64_t ci, int j) {
+  if (j < 0 || j > K.n - 1) return -1;
+  uint64_t cj = K.k[j];
+  if (K.bits == 30) {
+    if (ci == cj) return 32 + __clz((uint32_t)(i ^ j));
+    return __clz((uint32_t)(ci ^ cj));
+  }
+  if (ci == cj) return 64 + __clz((uint32_t)(i ^ j));
+  return __clzll(ci ^ cj);
+}
+// LBVH::determine_range / find_split (LBVH.slang:172-221)
+BB_DI void lb_range(const KeyView& K, int index, int& first, int& last) {
+  uint64_t code = K.k[index];
+  int dl = lb_delta(K, index, code, index - 1);
+  int dr = lb_delta(K, index, code, index + 1);
+  int dd = (dr >= dl) ? 1 : -1;
+  int dmin = dl < dr ? dl : dr;
+  int lmax = 2;
+  while (lb_delta(K, index, code, index + lmax * dd) > dmin) lmax <<= 1;
+  int l = 0;
+  for (int t = lmax >> 1; t > 0; t >>= 1)
+    if (lb_delta(K, index, code, index + (l + t) * dd) > dmin) l += t;
+  int j = index + l * dd;
+  first = index < j ? index : j;
+  last = index < j ? j : index;
+}
+BB_DI int lb_split(const KeyView& K, int first, int last) {
+  uint64_t fc = K.k[first];
+  int common = lb_delta(K, first, fc, last);
+  int split = first, step = last - first;
+  do {
+    step = (step + 1) >> 1;
+    int ns = split + step;
+    if (ns < last) {
+      if (lb_delta(K, first, fc, ns) > common) split = ns;
+    }
+  } while (step > 1);
+  return split;
+}
+
+// ================================================================ k_build
+template <int BLOCK>
+struct BuildLds {
+  SortLds<BLOCK> sort;
+  uint32_t lo[3], hi[3];
+};
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_build(Dev d) {
+  __shared__ BuildLds<BLOCK> L;
+  const uint32_t w = blockIdx.x, tid = threadIdx.x;
+  const WorldInfo W = d.worlds[w];
+  const uint32_t N = W.n, off = W.body_off;
+  if (N == 0) return;
+  // per-step resets
+  if (tid < ST_COUNT) d.stats[w * ST_COUNT + tid] = 0;
+  for (uint32_t k = tid; k < W.a; k += BLOCK) d.link[W.act_off + k] = k;
+  for (uint32_t i = tid; i < N; i += BLOCK) d.deg[off + i] = 0;
+  // ---- Morton keys from the previous rows (RB_sim.slang:75-111)
+  uint64_t* ka = d.keys_a + off;
+  uint64_t* kb = d.keys_b + off;
+  uint32_t* ia = d.idx_a + off;
+  uint32_t* ib = d.idx_b + off;
+  if (W.morton_bits == 30) {
+    for (uint32_t i = tid; i < N; i += BLOCK) {
+      v3 p = ld3(d.prev.pos[off + i]);
+      v3 q = p - V(-64.0f, -64.0f, -64.0f);
+      v3 ext = V(128.0f, 128.0f, 128.0f);
+      ka[i] = morton30(V(q.x / ext.x, q.y / ext.y, q.z / ext.z));
+      ia[i] = i;
+    }
+  } else {
+    if (tid < 3) {
+      L.lo[tid] = 0xFFFFFFFFu;
+      L.hi[tid] = 0u;
+    }
+    __syncthreads();
+    for (uint32_t i = tid; i < N; i += BLOCK) {
+      float4 p = d.prev.pos[off + i];
+      atomicMin(&L.lo[0], omap(p.x)); atomicMin(&L.lo[1], omap(p.y)); atomicMin(&L.lo[2], omap(p.z));
+      atomicMax(&L.hi[0], omap(p.x)); atomicMax(&L.hi[1], omap(p.y)); atomicMax(&L.hi[2], omap(p.z));
+    }
+    __syncthreads();
+    v3 lo = V(ounmap(L.lo[0]), ounmap(L.lo[1]), ounmap(L.lo[2]));
+    v3 hi = V(ounmap(L.hi[0]), ounmap(L.hi[1]), ounmap(L.hi[2]));
+    v3 ext = hi - lo;
+    for (uint32_t i = tid; i < N; i += BLOCK) {
+      v3 dd = ld3(d.prev.pos[off + i]) - lo;
+      v3 p;
+      p.x = ext.x > 0.0f ? dd.x / ext.x : 0.0f;
+      p.y = ext.y > 0.0f ? dd.y / ext.y : 0.0f;
+      p.z = ext.z > 0.0f ? dd.z / ext.z : 0.0f;
+      ka[i] = morton63(p);
+      ia[i] = i;
+    }
+  }
+  __syncthreads();
+  // tap copy of the unsorted keys lives in keys_b until the sort overwrites it;
+  // the host reads taps from a dedicated copy made by k_finish instead.
+  // ---- stable LSD radix sort (LBVH.slang:34-152)
+  const int passes = W.morton_bits == 30 ? 4 : 8;
+  bool in_a = block_radix_sort<BLOCK>(ka, ia, kb, ib, N, passes, L.sort);
+  __syncthreads();
+  if (!in_a) {
+    for (uint32_t i = tid; i < N; i += BLOCK) {
+      ka[i] = kb[i];
+      ia[i] = ib[i];
+    }
+    __syncthreads();
+  }
+  // ---- LBVH hierarchy (RB_sim.slang:206-278)
+  const uint32_t nb = 2 * off;
+  KeyView K{ka, (int)N, W.morton_bits == 30 ? 30 : 63};
+  const uint32_t leaf_off = N - 1;
+  for (uint32_t i = tid; i < N; i += BLOCK) {
+    uint32_t src = off + ia[i];
+    uint32_t id = d.prev.id[src];
+    Shape s = shape_of(d, off + id);
+    v3 mn, mx;
+    body_aabb(s, ld3(d.prev.pos[src]), ldq(d.prev.rot[src]), mn, mx);
+    d.nmin[nb + leaf_off + i] = st3(mn, __int_as_float(-1));
+    d.nmax[nb + leaf_off + i] = st3(mx, __int_as_float((int)i));
+  }
+  for (uint32_t i = tid; i + 1 < N; i += BLOCK) {
+    int first, last;
+    lb_range(K, (int)i, first, last);
+    int split = lb_split(K, first, last);
+    int ca = (split == first) ? (int)leaf_off + split : split;
+    int cb = (split + 1 == last) ? (int)leaf_off + split + 1 : split + 1;
+    d.nmin[nb + i] = make_float4(0.0f, 0.0f, 0.0f, __int_as_float(ca));
+    d.nmax[nb + i] = make_float4(0.0f, 0.0f, 0.0f, __int_as_float(cb));
+    d.parent[nb + ca] = i;
+    d.parent[nb + cb] = i;
+    d.visit[nb + i] = 0;
+  }
+  __syncthreads();
+  // ---- bounds, bottom-up: the second child to arrive merges (exact min/max
+  // through the order-preserving map, same values as the reference's integer
+  // atomics, RB_sim.slang:282-349)
+  if (N > 1) {
+    for (uint32_t i = tid; i < N; i += BLOCK) {
+      uint32_t p = d.parent[nb + leaf_off + i];
+      while (p + 1 < N) {  // p is an internal node index in [0, N-2]
+        __threadfence();
+        uint32_t old = atomicAdd(&d.visit[nb + p], 1u);
+        if (old == 0) break;
+        __threadfence();
+        float4 a0 = d.nmin[nb + p], a1 = d.nmax[nb + p];
+        int l = __float_as_int(a0.w), r = __float_as_int(a1.w);
+        if (l < 0 || r < 0 || l >= (int)(2 * N - 1) || r >= (int)(2 * N - 1)) break;
+        float4 lm = d.nmin[nb + l], lx = d.nmax[nb + l], rm = d.nmin[nb + r], rx = d.nmax[nb + r];
+        d.nmin[nb + p] = make_float4(omin(lm.x, rm.x), omin(lm.y, rm.y), omin(lm.z, rm.z), a0.w);
+        d.nmax[nb + p] = make_float4(omax(lx.x, rx.x), omax(lx.y, rx.y), omax(lx.z, rx.z), a1.w);
+        if (p == 0) break;
+        p = d.parent[nb + p];
+      }
+    }
+  }
+  // ---- reorder: rows[i] = prev rows[sorted index] (RB_sim.slang:351-386)
+  for (uint32_t i = tid; i < N; i += BLOCK) {
+    uint32_t src = off + ia[i], dst = off + i;
+    uint32_t id = d.prev.id[src];
+    d.cur.id[dst] = id;
+    d.cur.flags[dst] = d.prev.flags[src] & ~(uint32_t)BB_BODY_COLLIDING;
+    d.cur.sleep[dst] = d.prev.sleep[src];
+    d.cur.pos[dst] = d.prev.pos[src];
+    d.cur.rot[dst] = d.prev.rot[src];
+    d.cur.vel[dst] = d.prev.vel[src];
+    d.cur.omg[dst] = d.prev.omg[src];
+    d.cur.pvel[dst] = d.prev.pvel[src];
+    d.entries_cur[off + id] = i;
+  }
+}
+
+// ================================================================ k_broad
+BB_DI bool overlap(v3 amn, v3 amx, float4 bmn, float4 bmx) {  // Aabb::overlap, math.hpp:213-217
+  return amn.x <= bmx.x && amx.x >= bmn.x && amn.y <= bmx.y && amx.y >= bmn.y && amn.z <= bmx.z &&
+         amx.z >= bmn.z;
+}
+
+// LBVH::broad_phase (LBVH.slang:223-286): stack traversal from the root,
+// one pair (i, j) per overlapping leaf with row j > i.
+template <bool WRITE>
+BB_DI uint32_t traverse(const Dev& d, uint32_t nb, uint32_t N, uint32_t i, v3 qmn, v3 qmx, uint2* out,
+                        uint32_t cap_left) {
+  if (N < 2) return 0;
+  // depth of a Karras tree is at most key bits + index bits (63 + 32 < 128)
+  const int nn = (int)(2 * N - 1);
+  uint32_t stack[128];
+  uint32_t sp = 0;
+  stack[sp++] = 0;
+  uint32_t node = 0, cnt = 0;
+  do {
+    float4 pm = d.nmin[nb + node], px = d.nmax[nb + node];
+    int l = __float_as_int(pm.w), r = __float_as_int(px.w);
+    if (l < 0 || r < 0 || l >= nn || r >= nn) break;
+    float4 lm = d.nmin[nb + l], lx = d.nmax[nb + l], rm = d.nmin[nb + r], rx = d.nmax[nb + r];
+    bool ol = overlap(qmn, qmx, lm, lx), orr = overlap(qmn, qmx, rm, rx);
+    bool ll = __float_as_int(lm.w) < 0, rl = __float_as_int(rm.w) < 0;
+    uint32_t lp = (uint32_t)__float_as_int(lx.w), rp = (uint32_t)__float_as_int(rx.w);
+    if (ol && ll && i < lp) {
+      if (WRITE && cnt < cap_left) out[cnt] = make_uint2(i, lp);
+      ++cnt;
+    }
+    if (orr && rl && i < rp) {
+      if (WRITE && cnt < cap_left) out[cnt] = make_uint2(i, rp);
+      ++cnt;
+    }
+    bool tl = ol && !ll, tr = orr && !rl;
+    if (!tl && !tr) {
+      sp--;
+      if (sp == 0) break;
+      node = stack[sp];
+    } else {
+      if (tl && tr) {
+        if (sp < 128) stack[sp++] = (uint32_t)r;
+      }
+      node = tl ? (uint32_t)l : (uint32_t)r;
+    }
+  } while (sp > 0);
+  return cnt;
+}
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_broad(Dev d) {
+  __shared__ ScanLds<BLOCK> S;
+  __shared__ uint32_t carry;
+  const uint32_t w = blockIdx.x, tid = threadIdx.x;
+  const WorldInfo W = d.worlds[w];
+  const uint32_t N = W.n, off = W.body_off, nb = 2 * off;
+  if (tid == 0) carry = 0;
+  __syncthreads();
+  for (uint32_t c0 = 0; c0 < N; c0 += BLOCK) {
+    uint32_t i = c0 + tid;
+    uint32_t cnt = 0;
+    if (i < N) {
+      float4 mn = d.nmin[nb + N - 1 + i], mx = d.nmax[nb + N - 1 + i];
+      cnt = traverse<false>(d, nb, N, i, ld3(mn), ld3(mx), nullptr, 0);
+      d.pcount[off + i] = cnt;
+    }
+    uint32_t total;
+    uint32_t ex = block_scan<BLOCK>(cnt, S, total);
+    if (i < N) d.poff[off + i] = carry + ex;
+    __syncthreads();
+    if (tid == 0) carry += total;
+    __syncthreads();
+  }
+  const uint32_t P = carry;
+  const uint32_t cap = W.pair_cap;
+  if (tid == 0) {
+    d.npairs_cur[w] = P < cap ? P : cap;
+    d.stats[w * ST_COUNT + ST_P] = P;
+    if (P > cap) d.stats[w * ST_COUNT + ST_OVERFLOW] = 1;
+  }
+  uint2* pairs = d.pairs_cur + W.pair_off;
+  for (uint32_t i = tid; i < N; i += BLOCK) {
+    uint32_t o = d.poff[off + i];
+    if (o >= cap) continue;
+    uint32_t room = cap - o;
+    float4 mn = d.nmin[nb + N - 1 + i], mx = d.nmax[nb + N - 1 + i];
+    uint32_t cnt = traverse<true>(d, nb, N, i, ld3(mn), ld3(mx), pairs + o, room);
+    if (cnt > room) cnt = room;
+    // canonical order: ascending partner row (insertion sort; lists are short)
+    for (uint32_t a = 1; a < cnt; ++a) {
+      uint2 x = pairs[o + a];
+      uint32_t b = a;
+      while (b > 0 && pairs[o + b - 1].y > x.y) {
+        pairs[o + b] = pairs[o + b - 1];
+        --b;
+      }
+      pairs[o + b] = x;
+    }
+  }
+}
+
+// ==================================================== narrow phase (SAT)
+// OBBvsOBBCollision (src/shaders/simulation/collision_detection.slang:5-877),
+// restated for one pair per thread.
+struct Tx {
+  mat3 R;
+  v3 p;
+};
+BB_DI v3 tx_rot(const Tx& t, v3 v) { return mulvm(v, t.R); }
+BB_DI v3 tx_rotT(const Tx& t, v3 v) { return mulvm(v, mT(t.R)); }
+BB_DI v3 tx_tf(const Tx& t, v3 v) { return mulvm(v, t.R) + t.p; }
+BB_DI v3 tx_tfT(const Tx& t, v3 v) { return mulvm(v - t.p, mT(t.R)); }
+
+struct ClipV {
+  v3 v;
+  uint32_t f[4];  // in_ref, out_ref, in_inc, out_inc
+};
+
+enum : uint32_t {
+  PXPY = 0, PXNY = 1, PXPZ = 2, PXNZ = 3, NXPY = 4, NXNY = 5, NXPZ = 6, NXNZ = 7,
+  PYPZ = 8, PYNZ = 9, NYPZ = 10, NYNZ = 11
+};
+
+BB_DI void set_inc(ClipV* o, int i, v3 v, uint32_t in, uint32_t out) {
+  o[i].v = v;
+  o[i].f[0] = 0;
+  o[i].f[1] = 0;
+  o[i].f[2] = in;
+  o[i].f[3] = out;
+}
+
+// compute_incident_face (collision_detection.slang:124-264)
+BB_DI void incident_face(const Tx& itx, v3 e, v3 n, ClipV* o) {
+  n = -tx_rotT(itx, n);
+  v3 an = vabs(n);
+  if (an.x > an.y && an.x > an.z) {
+    if (n.x > 0.0f) {
+      set_inc(o, 0, V(e.x, e.y, -e.z), PYNZ, PXNY);
+      set_inc(o, 1, V(e.x, e.y, e.z), PXNY, PYPZ);
+      set_inc(o, 2, V(e.x, -e.y, e.z), PYPZ, NXNZ);
+      set_inc(o, 3, V(e.x, -e.y, -e.z), NXNZ, PYNZ);
+    } else {
+      set_inc(o, 0, V(-e.x, -e.y, e.z), NXNY, NYNZ);
+      set_inc(o, 1, V(-e.x, e.y, e.z), NYNZ, PXNZ);
+      set_inc(o, 2, V(-e.x, e.y, -e.z), PXNZ, NYPZ);
+      set_inc(o, 3, V(-e.x, -e.y, -e.z), NYPZ, NXNY);
+    }
+  } else if (an.y > an.x && an.y > an.z) {
+    if (n.y > 0.0f) {
+      set_inc(o, 0, V(-e.x, e.y, e.z), PXNZ, PXPY);
+      set_inc(o, 1, V(e.x, e.y, e.z), PXPY, PXNY);
+      set_inc(o, 2, V(e.x, e.y, -e.z), PXNY, PXPZ);
+      set_inc(o, 3, V(-e.x, e.y, -e.z), PXPZ, PXNZ);
+    } else {
+      set_inc(o, 0, V(e.x, -e.y, e.z), NXNZ, NXPY);
+      set_inc(o, 1, V(-e.x, -e.y, e.z), NXPY, NXNY);
+      set_inc(o, 2, V(-e.x, -e.y, -e.z), NXNY, NXPZ);
+      set_inc(o, 3, V(e.x, -e.y, -e.z), NXPZ, NXNZ);
+    }
+  } else {
+    if (n.z > 0.0f) {
+      set_inc(o, 0, V(-e.x, e.y, e.z), PXPY, NYNZ);
+      set_inc(o, 1, V(-e.x, -e.y, e.z), NYNZ, NXPY);
+      set_inc(o, 2, V(e.x, -e.y, e.z), NXPY, PYPZ);
+      set_inc(o, 3, V(e.x, e.y, e.z), PYPZ, PXPY);
+    } else {
+      set_inc(o, 0, V(e.x, -e.y, -e.z), PYNZ, NXPZ);
+      set_inc(o, 1, V(-e.x, -e.y, -e.z), NXPZ, NYPZ);
+      set_inc(o, 2, V(-e.x, e.y, -e.z), NYPZ, PXPZ);
+      set_inc(o, 3, V(e.x, e.y, -e.z), PXPZ, PYNZ);
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i) o[i].v = tx_tf(itx, o[i].v);
+}
+
+// compute_reference_edges_and_basis (collision_detection.slang:266-336)
+BB_DI void reference_edges(v3 er, const Tx& rtx, v3 n, int axis, uint32_t* ce, mat3& basis, v3& e) {
+  n = tx_rotT(rtx, n);
+  if (axis > 2) axis -= 3;
+  v3 r0 = mrow(rtx.R, 0), r1 = mrow(rtx.R, 1), r2 = mrow(rtx.R, 2);
+  if (axis == 0) {
+    if (n.x > 0.0f) {
+      ce[0] = PXNY; ce[1] = PYPZ; ce[2] = NXNZ; ce[3] = PYNZ;
+      e = V(er.y, er.z, er.x); basis = mrows(r1, r2, r0);
+    } else {
+      ce[0] = NYNZ; ce[1] = PXNZ; ce[2] = NYPZ; ce[3] = NXNY;
+      e = V(er.z, er.y, er.x); basis = mrows(r2, r1, -r0);
+    }
+  } else if (axis == 1) {
+    if (n.y > 0.0f) {
+      ce[0] = PXPY; ce[1] = PXNY; ce[2] = PXPZ; ce[3] = PXNZ;
+      e = V(er.z, er.x, er.y); basis = mrows(r2, r0, r1);
+    } else {
+      ce[0] = NXPY; ce[1] = NXNY; ce[2] = NXPZ; ce[3] = NXNZ;
+      e = V(er.z, er.x, er.y); basis = mrows(r2, -r0, -r1);
+    }
+  } else {
+    if (n.z > 0.0f) {
+      ce[0] = NYNZ; ce[1] = NXPY; ce[2] = PYPZ; ce[3] = PXPY;
+      e = V(er.y, er.x, er.z); basis = mrows(-r1, r0, r2);
+    } else {
+      ce[0] = NXPZ; ce[1] = NYPZ; ce[2] = PXPZ; ce[3] = PYNZ;
+      e = V(er.y, er.x, er.z); basis = mrows(-r1, -r0, -r2);
+    }
+  }
+}
+
+BB_DI bool clip_on(float v) { return (v < 0.005f) && (v > -0.005f); }
+
+// ortographic_clip (collision_detection.slang:352-388); output capped at 8.
+BB_DI int ortho_clip(float sign, float e, int axis, uint32_t ce, const ClipV* in, int nin, ClipV* out) {
+  int nout = 0;
+  ClipV a = in[nin - 1];
+  for (int i = 0; i < nin; ++i) {
+    ClipV b = in[i];
+    float da = sign * cmp(a.v, axis) - e;
+    float db = sign * cmp(b.v, axis) - e;
+    if ((da < 0.0f && db < 0.0f) || clip_on(da) || clip_on(db)) {
+      if (nout < 8) out[nout++] = b;
+    } else if (da < 0.0f && db >= 0.0f) {
+      ClipV cv = b;
+      cv.v = a.v + (b.v - a.v) * (da / (da - db));
+      cv.f[1] = ce;
+      if (nout < 8) out[nout++] = cv;
+    } else if (da >= 0.0f && db < 0.0f) {
+      ClipV cv = a;
+      cv.v = a.v + (b.v - a.v) * (da / (da - db));
+      cv.f[0] = ce;
+      if (nout < 8) out[nout++] = cv;
+      if (nout < 8) out[nout++] = b;
+    }
+    a = b;
+  }
+  return nout;
+}
+
+BB_DI float c_sign(float x) { return x < 0.0f ? -1.0f : 1.0f; }
+
+// support_edge (collision_detection.slang:459-500)
+BB_DI void support_edge(const Tx& tx, v3 e, v3 n, v3& ca, v3& cb) {
+  n = tx_rotT(tx, n);
+  v3 an = vabs(n);
+  int axis;
+  if (an.x <= an.y && an.x <= an.z) axis = 0;
+  else if (an.y <= an.x && an.y <= an.z) axis = 1;
+  else axis = 2;
+  v3 v0 = V(c_sign(n.x) * e.x, c_sign(n.y) * e.y, c_sign(n.z) * e.z);
+  v3 v1 = v0;
+  if (axis == 0) { v0.x = e.x; v1.x = -e.x; }
+  else if (axis == 1) { v0.y = e.y; v1.y = -e.y; }
+  else { v0.z = e.z; v1.z = -e.z; }
+  ca = tx_tf(tx, v0);
+  cb = tx_tf(tx, v1);
+}
+
+// edges_contact (collision_detection.slang:502-553)
+BB_DI void edges_contact(v3 PA, v3 QA, v3 PB, v3 QB, v3& CA, v3& CB) {
+  v3 DA = QA - PA, DB = QB - PB, r = PA - PB;
+  float a = dot(DA, DA), e = dot(DB, DB), f = dot(DB, r), c = dot(DA, r), b = dot(DA, DB);
+  float denom = a * e - b * b;
+  if (denom == 0.0f) {
+    float t = dot(r, DA) / a;
+    t = fcl(t, 0.0f, 1.0f);
+    CA = PA + DA * t;
+    v3 s = CA - PB;
+    float u = dot(s, DB) / e;
+    u = fcl(u, 0.0f, 1.0f);
+    CB = PB + DB * u;
+    return;
+  }
+  float TA = (b * f - c * e) / denom;
+  TA = fcl(TA, 0.0f, 1.0f);
+  float TB = (b * TA + f) / e;
+  if (TB < 0.0f) {
+    TB = 0.0f;
+    TA = fcl(-c / a, 0.0f, 1.0f);
+  } else if (TB > 1.0f) {
+    TB = 1.0f;
+    TA = fcl((b - c) / a, 0.0f, 1.0f);
+  }
+  CA = PA + DA * TA;
+  CB = PB + DB * TB;
+}
+
+struct Contact {
+  v3 pos;
+  float pen;
+  uint32_t fp[4];
+};
+struct MDet {  // freshly detected manifold
+  int count, key;
+  v3 normal;
+  Contact c[8];
+};
+
+struct BodyView {
+  v3 pos;
+  qt rot;
+  v3 mn, mx;
+  float radius;
+  uint32_t kind;
+};
+
+// OBBvsOBBCollision::collision_detection (collision_detection.slang:568-877)
+BB_DI bool obb_obb(const BodyView& A, const BodyView& B, MDet& m) {
+  m.count = 0;
+  m.key = -1;
+  Tx txA{mT(qmat(A.rot)), A.pos}, txB{mT(qmat(B.rot)), B.pos};
+  v3 eA = (A.mx - A.mn) * 0.5f, eB = (B.mx - B.mn) * 0.5f;
+  mat3 C = mmul(mT(txA.R), txB.R);
+  mat3 aC = mabs(C);
+  bool parallel = false;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+      if (aC.m[i][j] > 0.9998f) parallel = true;
+  v3 t = qrot_inv(A.rot, B.pos - A.pos);
+  float a_max = -3.402823466e+38f, b_max = -3.402823466e+38f, e_max = -3.402823466e+38f;
+  int a_axis = -1, b_axis = -1, e_axis = -1;
+  v3 n_a = V(0, 0, 0), n_b = V(0, 0, 0), n_e = V(0, 0, 0);
+  for (int i = 0; i < 3; ++i) {
+    float s = fab(cmp(t, i)) - (cmp(eA, i) + dot(mcol(aC, i), eB));
+    if (s > 0.0f) return false;
+    if (s > a_max) { a_max = s; a_axis = i; n_a = mrow(txA.R, i); }
+  }
+  for (int i = 0; i < 3; ++i) {
+    float proj = dot(t, mrow(C, i));
+    float s = fab(proj) - (cmp(eB, i) + dot(mrow(aC, i), eA));
+    if (s > 0.0f) return false;
+    if (s > b_max) { b_max = s; b_axis = 3 + i; n_b = mrow(txB.R, i); }
+  }
+  if (!parallel) {
+    for (int i = 0; i < 3; ++i) {
+      for (int j = 0; j < 3; ++j) {
+        int i1 = (i + 1) % 3, i2 = (i + 2) % 3, j1 = (j + 1) % 3, j2 = (j + 2) % 3;
+        float rA = cmp(eA, i1) * aC.m[j][i2] + cmp(eA, i2) * aC.m[j][i1];
+        float rB = cmp(eB, j1) * aC.m[j2][i] + cmp(eB, j2) * aC.m[j1][i];
+        float s = fab(cmp(t, i2) * C.m[j][i1] - cmp(t, i1) * C.m[j][i2]) - (rA + rB);
+        v3 ax = V(0, 0, 0);
+        setcmp(ax, i1, -C.m[j][i2]);
+        setcmp(ax, i2, C.m[j][i1]);
+        if (s > 0.0f) return false;
+        float l = 1.0f / vlen(ax);
+        s *= l;
+        if (s > e_max) { e_max = s; e_axis = 6 + i * 3 + j; n_e = ax * l; }
+      }
+    }
+  }
+  int axis;
+  float s_max;
+  v3 normal;
+  float face_max = fmx(a_max, b_max);
+  if (0.95f * e_max > face_max + 0.01f) { axis = e_axis; s_max = e_max; normal = n_e; }
+  else if (0.95f * b_max > a_max + 0.01f) { axis = b_axis; s_max = b_max; normal = n_b; }
+  else { axis = a_axis; s_max = a_max; normal = n_a; }
+  if (dot(normal, B.pos - A.pos) < 0.0f) normal = -normal;
+  if (axis == -1) return false;
+  if (axis <= 5) {
+    Tx rtx, itx;
+    v3 er, ei;
+    bool flip = false;
+    if (axis <= 2) { rtx = txA; itx = txB; er = eA; ei = eB; }
+    else { rtx = txB; itx = txA; er = eB; ei = eA; flip = true; normal = -normal; }
+    ClipV inc[8], tmp[8];
+    incident_face(itx, ei, normal, inc);
+    uint32_t ce[4];
+    mat3 basis;
+    v3 e;
+    reference_edges(er, rtx, normal, axis, ce, basis, e);
+    Tx tx{basis, rtx.p};
+    // clip (collision_detection.slang:390-452)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) inc[i].v = tx_tfT(tx, inc[i].v);
+    int nout = ortho_clip(1.0f, e.x, 0, ce[0], inc, 4, tmp);
+    if (nout == 0) return false;
+    int nin = ortho_clip(1.0f, e.y, 1, ce[1], tmp, nout, inc);
+    if (nin == 0) return false;
+    nout = ortho_clip(-1.0f, e.x, 0, ce[2], inc, nin, tmp);
+    if (nout == 0) return false;
+    nin = ortho_clip(-1.0f, e.y, 1, ce[3], tmp, nout, inc);
+    if (nin == 0) return false;
+    int k = 0;
+    for (int i = 0; i < nin; ++i) {
+      float dz = inc[i].v.z - e.z;
+      if (dz <= 0.0f) {
+        Contact& c = m.c[k++];
+        c.pos = tx_tf(tx, inc[i].v);
+        c.pen = dz;
+        if (flip) {
+          c.fp[0] = inc[i].f[2]; c.fp[1] = inc[i].f[3]; c.fp[2] = inc[i].f[0]; c.fp[3] = inc[i].f[1];
+        } else {
+          c.fp[0] = inc[i].f[0]; c.fp[1] = inc[i].f[1]; c.fp[2] = inc[i].f[2]; c.fp[3] = inc[i].f[3];
+        }
+      }
+    }
+    if (k > 0) {
+      m.count = k;
+      m.normal = flip ? -normal : normal;
+    }
+  } else {
+    normal = tx_rot(txA, normal);
+    if (dot(normal, B.pos - A.pos) < 0.0f) normal = -normal;
+    v3 PA, QA, PB, QB, CA, CB;
+    support_edge(txA, eA, normal, PA, QA);
+    support_edge(txB, eB, -normal, PB, QB);
+    edges_contact(PA, QA, PB, QB, CA, CB);
+    m.count = 1;
+    m.normal = normal;
+    m.c[0].pos = (CA + CB) * 0.5f;
+    m.c[0].pen = s_max;
+    m.c[0].fp[0] = m.c[0].fp[1] = m.c[0].fp[2] = m.c[0].fp[3] = (uint32_t)axis;
+  }
+  m.key = axis;
+  return m.count > 0;
+}
+
+// Sphere contacts (extension; the reference has no spheres): sphere-sphere
+// along the centre line, box-sphere through the clamped local closest point
+// or, for a centre inside the box, the face of least penetration.
+static constexpr int KEY_SPHERE_SPHERE = 20, KEY_BOX_SPHERE = 21;
+BB_DI bool sphere_sphere(const BodyView& A, const BodyView& B, MDet& m) {
+  m.count = 0;
+  m.key = KEY_SPHERE_SPHERE;
+  v3 dd = B.pos - A.pos;
+  float dist = vlen(dd);
+  float pen = dist - (A.radius + B.radius);
+  if (pen > 0.0f) return false;
+  v3 n = dist > 0.0f ? vdiv(dd, dist) : V(0.0f, 1.0f, 0.0f);
+  v3 pa = A.pos + n * A.radius, pb = B.pos - n * B.radius;
+  m.count = 1;
+  m.normal = n;
+  m.c[0].pos = (pa + pb) * 0.5f;
+  m.c[0].pen = pen;
+  m.c[0].fp[0] = m.c[0].fp[1] = m.c[0].fp[2] = m.c[0].fp[3] = (uint32_t)KEY_SPHERE_SPHERE;
+  return true;
+}
+BB_DI bool box_sphere(const BodyView& A, const BodyView& B, MDet& m) {
+  m.count = 0;
+  m.key = KEY_BOX_SPHERE;
+  const bool a_box = A.kind == BB_SHAPE_BOX;
+  const BodyView& X = a_box ? A : B;
+  const BodyView& S = a_box ? B : A;
+  v3 e = (X.mx - X.mn) * 0.5f;
+  v3 local = qrot_inv(X.rot, S.pos - X.pos);
+  v3 q = V(fcl(local.x, -e.x, e.x), fcl(local.y, -e.y, e.y), fcl(local.z, -e.z, e.z));
+  v3 dd = local - q;
+  float dist2 = dot(dd, dd);
+  v3 nl, surf;
+  float pen;
+  if (dist2 > 0.0f) {
+    float dist = __builtin_sqrtf(dist2);
+    pen = dist - S.radius;
+    if (pen > 0.0f) return false;
+    nl = vdiv(dd, dist);
+    surf = q;
+  } else {
+    float dx = e.x - fab(local.x), dy = e.y - fab(local.y), dz = e.z - fab(local.z);
+    int ax = 0;
+    float dmin = dx;
+    if (dy < dmin) { dmin = dy; ax = 1; }
+    if (dz < dmin) { dmin = dz; ax = 2; }
+    nl = V(0, 0, 0);
+    setcmp(nl, ax, cmp(local, ax) < 0.0f ? -1.0f : 1.0f);
+    surf = local;
+    setcmp(surf, ax, cmp(nl, ax) * cmp(e, ax));
+    pen = -dmin - S.radius;
+  }
+  v3 n = qrot(X.rot, nl);
+  v3 surf_w = qrot(X.rot, surf) + X.pos;
+  v3 sph_w = S.pos - n * S.radius;
+  m.count = 1;
+  m.normal = a_box ? n : -n;
+  m.c[0].pos = (surf_w + sph_w) * 0.5f;
+  m.c[0].pen = pen;
+  m.c[0].fp[0] = m.c[0].fp[1] = m.c[0].fp[2] = m.c[0].fp[3] = (uint32_t)KEY_BOX_SPHERE;
+  return true;
+}
+
+BB_DI BodyView body_view(const Dev& d, uint32_t off, uint32_t row) {
+  uint32_t g = off + row;
+  uint32_t id = d.cur.id[g];
+  float4 a = d.props.pmin[off + id], b = d.props.pmax[off + id];
+  return BodyView{ld3(d.cur.pos[g]), ldq(d.cur.rot[g]), ld3(a), ld3(b), a.w, b.w != 0.0f ? 1u : 0u};
+}
+
+// Full contact record as it travels through warm start.
+struct CFull {
+  v3 pos;
+  float pen;
+  float ln, lt0, lt1, kcoef;
+  float tm0, tm1;
+  v3 anca, ancb;
+  uint32_t stick;
+  uint32_t fp[4];
+  uint32_t ws;
+};
+
+// IslandBuilder::link_bodies (src/shaders/simulation/islands.slang:153-213):
+// lock-free union-find, links always point to the lower index, so every
+// component's root is its minimum active index whatever the timing.
+BB_DI uint32_t uf_root(uint32_t* link, uint32_t x) {
+  for (;;) {
+    uint32_t y = atomicAdd(&link[x], 0u);
+    if (y == x) return x;
+    x = y;
+  }
+}
+BB_DI void uf_link(uint32_t* link, uint32_t f, uint32_t s) {
+  uint32_t fl = f, sl = s;
+  for (;;) {
+    fl = uf_root(link, fl);
+    sl = uf_root(link, sl);
+    if (fl != sl) {
+      if (fl < sl) {
+        if (atomicCAS(&link[sl], sl, fl) != sl) continue;
+      } else {
+        if (atomicCAS(&link[fl], fl, sl) != fl) continue;
+      }
+    }
+    uint32_t lo = fl < sl ? fl : sl;
+    atomicMin(&link[f], lo);
+    atomicMin(&link[s], lo);
+    break;
+  }
+}
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_narrow(Dev d) {
+  const uint32_t w = blockIdx.x, tid = threadIdx.x;
+  const WorldInfo W = d.worlds[w];
+  const uint32_t off = W.body_off, P = d.npairs_cur[w];
+  const uint2* pairs = d.pairs_cur + W.pair_off;
+  const bool warm = (W.flags & BB_SIM_WARM_STARTING) != 0 && W.drop_prev == 0;
+  const uint32_t Pprev = d.npairs_prev[w];
+  const uint2* ppairs = d.pairs_prev + W.pair_off;
+  uint32_t fresh = 0, pen_mm = 0, ccount = 0, mcount = 0;
+  for (uint32_t p = tid; p < P; p += BLOCK) {
+    const uint32_t slot = W.pair_off + p;
+    uint2 pr = pairs[p];
+    const uint32_t ga = off + pr.x, gb = off + pr.y;
+    const uint32_t fa = d.cur.flags[ga], fb = d.cur.flags[gb];
+    const bool da = (fa & BB_BODY_DYNAMIC) != 0, db = (fb & BB_BODY_DYNAMIC) != 0;
+    d.man_cur.hdr[slot] = make_int4((int)pr.x, (int)pr.y, -1, 0);
+    if (!da && !db) continue;
+    BodyView A = body_view(d, off, pr.x), B = body_view(d, off, pr.y);
+    MDet m;
+    bool have;
+    if (A.kind == BB_SHAPE_BOX && B.kind == BB_SHAPE_BOX) {
+      have = obb_obb(A, B, m);
+      if (have)
+        for (int ci = 0; ci < m.count; ++ci) m.c[ci].pen = fmx(m.c[ci].pen, -0.25f);
+    } else if (A.kind == BB_SHAPE_SPHERE && B.kind == BB_SHAPE_SPHERE) {
+      have = sphere_sphere(A, B, m);
+    } else {
+      have = box_sphere(A, B, m);
+    }
+    if (!have) continue;
+    // Manifold::compute_basis (src/shaders/extensions.slang:262-272)
+    v3 n = m.normal, t0, t1;
+    if (fab(n.x) > fab(n.y)) t0 = V(n.z, 0.0f, -n.x);
+    else t0 = V(0.0f, -n.z, n.y);
+    t0 = vnorm(t0);
+    t1 = cross(t0, n);
+    CFull c[8];
+    for (int i = 0; i < m.count; ++i) {
+      c[i].pos = m.c[i].pos;
+      c[i].pen = m.c[i].pen;
+      c[i].ln = c[i].lt0 = c[i].lt1 = c[i].kcoef = c[i].tm0 = c[i].tm1 = 0.0f;
+      c[i].anca = c[i].ancb = V(0, 0, 0);
+      c[i].stick = 0;
+      c[i].fp[0] = m.c[i].fp[0]; c[i].fp[1] = m.c[i].fp[1]; c[i].fp[2] = m.c[i].fp[2]; c[i].fp[3] = m.c[i].fp[3];
+      c[i].ws = 0;
+    }
+    int32_t error = 0x7FFFFFFF;
+    // warm start: the previous step's manifold of this pair
+    // (find_old_collision + Manifold::update_from_old, collision_detection.slang:
+    // 914-963, src/shaders/extensions.slang:305-473)
+    const uint32_t ida = d.cur.id[ga], idb = d.cur.id[gb];
+    const uint32_t pa = d.entries_prev[off + ida], pb = d.entries_prev[off + idb];
+    if (warm && pa < W.prev_n && pb < W.prev_n) {
+      const uint32_t lo = pa < pb ? pa : pb, hi = pa < pb ? pb : pa;
+      uint32_t L0 = 0, L1 = Pprev;
+      while (L0 < L1) {
+        uint32_t mid = (L0 + L1) >> 1;
+        uint2 q = ppairs[mid];
+        if (q.x < lo || (q.x == lo && q.y < hi)) L0 = mid + 1;
+        else L1 = mid;
+      }
+      int4 oh = make_int4(0, 0, 0, 0);
+      uint32_t os = 0;
+      if (L0 < Pprev && ppairs[L0].x == lo && ppairs[L0].y == hi) {
+        os = W.pair_off + L0;
+        oh = d.man_prev.hdr[os];
+      }
+      if (oh.w > 0) {
+        const bool swapped = pa != lo;
+        const float sgn = swapped ? -1.0f : 1.0f;
+        const bool cross_key = m.key != oh.z || swapped;
+        const v3 on = ld3(d.man_prev.nrm[os]);
+        const int ocount = oh.w;
+        bool go = true;
+        if (cross_key) go = dot(n, on * sgn) > 0.7f;
+        if (go) {
+          uint32_t used_old = 0, matched_new = 0;
+          for (int i = 0; !cross_key && i < m.count; ++i) {
+            for (int j = 0; j < ocount; ++j) {
+              if (used_old & (1u << j)) continue;
+              const uint32_t oc = os * 8 + j;
+              uint4 ofp = d.man_prev.cfp[oc];
+              v3 opos = ld3(d.man_prev.cpos[oc]);
+              v3 dp = opos - c[i].pos;
+              bool feat = ofp.z == c[i].fp[2] && ofp.w == c[i].fp[3];
+              bool close = dot(dp, dp) < 0.1f * 0.1f;
+              if (feat && close) {
+                float4 ol = d.man_prev.clam[oc], om = d.man_prev.cmas[oc];
+                float4 oa = d.man_prev.cana[oc], ob = d.man_prev.canb[oc];
+                c[i].ln = ol.x; c[i].lt0 = ol.y; c[i].lt1 = ol.z; c[i].kcoef = ol.w;
+                c[i].tm0 = om.y; c[i].tm1 = om.z;
+                uint32_t ost = __float_as_uint(ob.w);
+                c[i].stick = ost;
+                if (ost != 0) { c[i].anca = ld3(oa); c[i].ancb = ld3(ob); }
+                v3 d2 = c[i].pos - opos;
+                if (dot(d2, d2) < 0.01f) c[i].pos = opos;
+                uint32_t ows = d.man_prev.cws[oc];
+                c[i].ws = ows + 1u < 255u ? ows + 1u : 255u;
+                used_old |= 1u << j;
+                matched_new |= 1u << i;
+                break;
+              }
+            }
+          }
+          for (int round = 0; round < m.count; ++round) {
+            float best = 0.1225f;
+            int bi = -1, bj = -1;
+            for (int i = 0; i < m.count; ++i) {
+              if (matched_new & (1u << i)) continue;
+              for (int j = 0; j < ocount; ++j) {
+                if (used_old & (1u << j)) continue;
+                v3 dp = c[i].pos - ld3(d.man_prev.cpos[os * 8 + j]);
+                float dd2 = dot(dp, dp);
+                if (dd2 < best) { best = dd2; bi = i; bj = j; }
+              }
+            }
+            if (bi < 0) break;
+            const uint32_t oc = os * 8 + bj;
+            float4 ol = d.man_prev.clam[oc], om = d.man_prev.cmas[oc];
+            float4 oa = d.man_prev.cana[oc], ob = d.man_prev.canb[oc];
+            c[bi].ln = ol.x; c[bi].lt0 = ol.y * sgn; c[bi].lt1 = ol.z; c[bi].kcoef = ol.w;
+            c[bi].tm0 = om.y; c[bi].tm1 = om.z;
+            uint32_t ost = __float_as_uint(ob.w);
+            c[bi].stick = ost;
+            if (ost != 0) {
+              c[bi].anca = swapped ? ld3(ob) : ld3(oa);
+              c[bi].ancb = swapped ? ld3(oa) : ld3(ob);
+            }
+            uint4 ofp = d.man_prev.cfp[oc];
+            c[bi].fp[0] = ofp.x; c[bi].fp[1] = ofp.y; c[bi].fp[2] = ofp.z; c[bi].fp[3] = ofp.w;
+            uint32_t ows = d.man_prev.cws[oc];
+            c[bi].ws = ows + 1u < 255u ? ows + 1u : 255u;
+            matched_new |= 1u << bi;
+            used_old |= 1u << bj;
+          }
+          float deep_new = 0.0f, deep_old = 0.0f;
+          for (int i = 0; i < m.count; ++i) deep_new = fmn(deep_new, c[i].pen);
+          for (int j = 0; j < ocount; ++j) deep_old = fmn(deep_old, d.man_prev.cpos[os * 8 + j].w);
+          error = (int32_t)fcl((deep_new - deep_old) * 10000.0f, -32000.0f, 32000.0f);
+        }
+      }
+    }
+    // write the manifold slot
+    d.man_cur.hdr[slot] = make_int4((int)pr.x, (int)pr.y, m.key, m.count);
+    d.man_cur.nrm[slot] = st3(n, __int_as_float(error));
+    d.man_cur.t0[slot] = st3(t0, 0.0f);
+    d.man_cur.t1[slot] = st3(t1, 0.0f);
+    d.man_cur.anca[slot] = st3(A.pos, 0.0f);
+    d.man_cur.ancb[slot] = st3(B.pos, 0.0f);
+    for (int i = 0; i < m.count; ++i) {
+      const uint32_t cs = slot * 8 + i;
+      d.man_cur.cpos[cs] = st3(c[i].pos, c[i].pen);
+      d.man_cur.clam[cs] = make_float4(c[i].ln, c[i].lt0, c[i].lt1, c[i].kcoef);
+      d.man_cur.cmas[cs] = make_float4(0.0f, c[i].tm0, c[i].tm1, 0.0f);
+      d.man_cur.cana[cs] = st3(c[i].anca, 0.0f);
+      d.man_cur.canb[cs] = st3(c[i].ancb, __uint_as_float(c[i].stick));
+      d.man_cur.cfp[cs] = make_uint4(c[i].fp[0], c[i].fp[1], c[i].fp[2], c[i].fp[3]);
+      d.man_cur.cws[cs] = c[i].ws;
+      if (c[i].ws == 0) ++fresh;
+      if (c[i].pen < 0.0f) {
+        uint32_t mm = (uint32_t)fmn(-c[i].pen * 1000.0f, 1.0e9f);
+        pen_mm = pen_mm > mm ? pen_mm : mm;
+      }
+    }
+    ccount += (uint32_t)m.count;
+    ++mcount;
+    atomicAdd(&d.deg[ga], 1u);
+    atomicAdd(&d.deg[gb], 1u);
+    const uint32_t ra = da ? d.props.active_rank[off + ida] : kMAX;
+    const uint32_t rb = db ? d.props.active_rank[off + idb] : kMAX;
+    if (ra < W.a && rb < W.a) uf_link(d.link + W.act_off, ra, rb);
+  }
+  stat_add(d, w, ST_FRESH, fresh);
+  stat_add(d, w, ST_C, ccount);
+  stat_add(d, w, ST_M, mcount);
+  if (pen_mm) stat_max(d, w, ST_PEN, pen_mm);
+}
+
+// ================================================================= k_post
+// RigidBody::advect / advance (src/shaders/extensions.slang:127-139); the
+// damping factor exp(-dt*0.1) is evaluated once on the host.
+BB_DI void advect_row(const Dev& d, uint32_t off, uint32_t i, const WorldInfo& W) {
+  const uint32_t g = off + i;
+  const uint32_t fl = d.cur.flags[g];
+  if (!((fl & BB_BODY_DYNAMIC) && !(fl & BB_BODY_SLEEPING))) return;
+  v3 v = ld3(d.cur.vel[g]), om = ld3(d.cur.omg[g]);
+  if (fl & BB_BODY_GRAVITY) {
+    const uint32_t id = d.cur.id[g];
+    float inv_mass = d.props.mass[off + id].y;
+    v3 impulse = V(0.0f, W.gravity, 0.0f) * W.dt;
+    v3 lin = impulse * inv_mass;
+    v = v + lin;
+    v3 p = ld3(d.cur.pos[g]);
+    v3 rr = p - p;
+    v3 torque = cross(rr, impulse);
+    mat3 I = mrows(ld3(d.props.ii0[off + id]), ld3(d.props.ii1[off + id]), ld3(d.props.ii2[off + id]));
+    om = om + mulvm(torque, I);
+  }
+  v = v * W.damp;
+  om = om * W.damp;
+  d.cur.vel[g] = st3(v, 0.0f);
+  d.cur.omg[g] = st3(om, 0.0f);
+}
+
+BB_DI uint32_t gc_hash(uint32_t x) {  // coloring.slang:114-122
+  x ^= x >> 16;
+  x *= 0x7feb352du;
+  x ^= x >> 15;
+  x *= 0x846ca68bu;
+  x ^= x >> 16;
+  return x;
+}
+BB_DI uint32_t gc_priority(uint32_t pid, uint32_t round) {  // coloring.slang:132-138
+  uint32_t h = gc_hash(pid + round * 0x85ebca6bu) & 0xFFFu;
+  if (h == 0u) h = 1u;
+  return (h << 20) | (pid & 0xFFFFFu);
+}
+
+template <int BLOCK>
+struct PostLds {
+  ScanLds<BLOCK> scan;
+  uint32_t carry;
+  uint32_t flag;
+  uint32_t bcount[kBuckets];
+  uint32_t bfill[kBuckets];
+};
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_post(Dev d) {
+  __shared__ PostLds<BLOCK> L;
+  const uint32_t w = blockIdx.x, tid = threadIdx.x;
+  const WorldInfo W = d.worlds[w];
+  const uint32_t N = W.n, off = W.body_off, A = W.a;
+  const uint32_t P = d.npairs_cur[w];
+  const uint32_t flags = W.flags;
+  const bool pgs = W.solver != BB_SOLVER_AVBD;
+  const uint32_t* link = d.link + W.act_off;
+  // ---- advect, PGS family only (RB_sim.slang:518-554).  With the parity
+  // quirk the dispatch covers ceil(A/32)*32 threads indexing rows < N.
+  if (pgs && (flags & BB_SIM_ADVECTION)) {
+    uint32_t lim = N;
+    if (W.quirks & BB_QUIRK_ACTIVE_DISPATCH_ROWS) {
+      uint32_t r = ((A + 31u) / 32u) * 32u;
+      lim = r < N ? r : N;
+    }
+    for (uint32_t i = tid; i < lim; i += BLOCK) advect_row(d, off, i, W);
+  }
+  // ---- islands (islands.slang:215-302): root = min active index; slots in
+  // ascending root order
+  if (tid == 0) L.carry = 0;
+  __syncthreads();
+  for (uint32_t c0 = 0; c0 < A; c0 += BLOCK) {
+    uint32_t k = c0 + tid;
+    uint32_t is_root = 0;
+    if (k < A) {
+      uint32_t x = k;
+      while (link[x] != x) x = link[x];
+      d.ascratch[W.act_off + k] = x;
+      is_root = x == k ? 1u : 0u;
+    }
+    uint32_t tot;
+    uint32_t ex = block_scan<BLOCK>(is_root, L.scan, tot);
+    if (k < A && is_root) d.poff[off + k] = L.carry + ex;  // slot of root k
+    __syncthreads();
+    if (tid == 0) L.carry += tot;
+    __syncthreads();
+  }
+  const uint32_t islands = L.carry;
+  for (uint32_t i = tid; i < N; i += BLOCK) {
+    d.island[off + i] = kMAX;
+    d.islot[off + i] = kMAX;
+  }
+  __syncthreads();
+  for (uint32_t k = tid; k < A; k += BLOCK) {
+    uint32_t id = d.props.active_ids[W.act_off + k];
+    uint32_t row = d.entries_cur[off + id];
+    uint32_t root = d.ascratch[W.act_off + k];
+    d.island[off + row] = root;
+    d.islot[off + row] = d.poff[off + root];
+  }
+  // contact islands: islands owning at least one manifold
+  for (uint32_t k = tid; k < A; k += BLOCK) d.pcount[off + k] = 0;
+  __syncthreads();
+  for (uint32_t p = tid; p < P; p += BLOCK) {
+    int4 h = d.man_cur.hdr[W.pair_off + p];
+    if (h.w <= 0) continue;
+    uint32_t ia = d.props.active_rank[off + d.cur.id[off + h.x]];
+    uint32_t ib = d.props.active_rank[off + d.cur.id[off + h.y]];
+    uint32_t mn = ia < ib ? ia : ib;
+    if (mn < A) d.pcount[off + d.ascratch[W.act_off + mn]] = 1;
+  }
+  __syncthreads();
+  {
+    uint32_t cnt = 0;
+    for (uint32_t k = tid; k < A; k += BLOCK) cnt += d.pcount[off + k];
+    if (tid == 0) L.flag = 0;
+    __syncthreads();
+    atomicAdd(&L.flag, cnt);
+    __syncthreads();
+    if (tid == 0) {
+      d.stats[w * ST_COUNT + ST_ISLANDS] = islands;
+      d.stats[w * ST_COUNT + ST_CISLANDS] = L.flag;
+    }
+    __syncthreads();
+  }
+  // ---- sleeping (RB_sim.slang:556-737)
+  const bool sleep_on = (flags & BB_SIM_SLEEPING_ENABLED) != 0;
+  for (uint32_t i = tid; i < N; i += BLOCK) {
+    const uint32_t g = off + i;
+    uint32_t fl = d.cur.flags[g];
+    if (!(fl & BB_BODY_DYNAMIC)) continue;
+    if (!sleep_on) {
+      d.cur.sleep[g] = 0;
+      d.cur.flags[g] = fl & ~(uint32_t)BB_BODY_SLEEPING;
+      continue;
+    }
+    uint32_t st = d.cur.sleep[g];
+    if (d.deg[g] == 0 && (fl & BB_BODY_SLEEPING)) {
+      uint32_t t = st & 0x7FFFFFFFu;
+      d.cur.sleep[g] = t >= 10u ? t - 10u : 0u;
+      continue;
+    }
+    v3 v = ld3(d.cur.vel[g]), om = ld3(d.cur.omg[g]);
+    float lin2 = dot(v, v), ang2 = dot(om, om);
+    uint32_t timer = st & 0x7FFFFFFFu;
+    bool calm = lin2 < 0.0144f && ang2 < 0.0225f;
+    d.cur.sleep[g] = calm ? (timer + 1u < 30u ? timer + 1u : 30u) : (timer >= 4u ? timer - 4u : 0u);
+  }
+  __syncthreads();
+  if (sleep_on) {
+    for (uint32_t p = tid; p < P; p += BLOCK) {
+      const uint32_t slot = W.pair_off + p;
+      int4 h = d.man_cur.hdr[slot];
+      if (h.w <= 0) continue;
+      const uint32_t ga = off + h.x, gb = off + h.y;
+      bool adyn = (d.cur.flags[ga] & BB_BODY_DYNAMIC) != 0, bdyn = (d.cur.flags[gb] & BB_BODY_DYNAMIC) != 0;
+      bool aloud = adyn && ((d.cur.sleep[ga] & 0x7FFFFFFFu) < 30u);
+      bool bloud = bdyn && ((d.cur.sleep[gb] & 0x7FFFFFFFu) < 30u);
+      int32_t err = __float_as_int(d.man_cur.nrm[slot].w);
+      bool stalled = err != 0x7FFFFFFF && (err < 0 ? -err : err) < 10;
+      bool deep = false;
+      if (!stalled)
+        for (int ci = 0; ci < h.w; ++ci)
+          if (d.man_cur.cpos[slot * 8 + ci].w < -0.05f) { deep = true; break; }
+      if ((aloud || deep) && bdyn) atomicOr(&d.cur.sleep[gb], 0x80000000u);
+      if ((bloud || deep) && adyn) atomicOr(&d.cur.sleep[ga], 0x80000000u);
+    }
+    __syncthreads();
+    uint32_t sl_count = 0;
+    for (uint32_t i = tid; i < N; i += BLOCK) {
+      const uint32_t g = off + i;
+      uint32_t fl = d.cur.flags[g];
+      if (!(fl & BB_BODY_DYNAMIC)) continue;
+      uint32_t st = d.cur.sleep[g];
+      bool vetoed = (st & 0x80000000u) != 0;
+      bool quiet = (st & 0x7FFFFFFFu) >= 30u;
+      bool sl = quiet && !vetoed;
+      bool was = (fl & BB_BODY_SLEEPING) != 0;
+      if (sl) {
+        if (!was) {
+          d.cur.vel[g] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+          d.cur.omg[g] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+          d.cur.flags[g] = fl | BB_BODY_SLEEPING;
+        }
+        ++sl_count;
+      } else if (was) {
+        d.cur.flags[g] = fl & ~(uint32_t)BB_BODY_SLEEPING;
+      }
+    }
+    stat_add(d, w, ST_SLEEPING, sl_count);
+  }
+  __syncthreads();
+  if (!pgs) return;
+  // ---- edge colouring (coloring.slang:44-292).  Phase 2 commits at the
+  // colour picked in phase 1; the owner cells touched in a round are cleared
+  // by a third phase so the table is zero between rounds.
+  uint32_t* color = d.color + W.pair_off;
+  uint32_t* pick = d.pick + W.pair_off;
+  for (uint32_t i = tid; i < N; i += BLOCK) d.mask[off + i] = 0;
+  for (uint32_t p = tid; p < P; p += BLOCK) color[p] = d.man_cur.hdr[W.pair_off + p].w > 0 ? kMAX : kMAX - 1;
+  __syncthreads();
+  for (uint32_t round = 1; round <= kColors; ++round) {
+    if (tid == 0) L.flag = 0;
+    __syncthreads();
+    for (uint32_t p = tid; p < P; p += BLOCK) {
+      pick[p] = kColors;
+      if (color[p] != kMAX) continue;
+      int4 h = d.man_cur.hdr[W.pair_off + p];
+      const uint32_t ga = off + h.x, gb = off + h.y;
+      bool sa = !(d.cur.flags[ga] & BB_BODY_DYNAMIC), sb = !(d.cur.flags[gb] & BB_BODY_DYNAMIC);
+      uint32_t avail = ~((sa ? 0u : d.mask[ga]) | (sb ? 0u : d.mask[gb]));
+      if (avail == 0) continue;
+      uint32_t cc = (uint32_t)(__ffs(avail) - 1);
+      pick[p] = cc;
+      uint32_t ida = d.cur.id[ga], idb = d.cur.id[gb];
+      uint32_t lo = ida < idb ? ida : idb, hi = ida < idb ? idb : ida;
+      uint32_t pid = gc_hash(lo * 0x9e3779b9u + hi * 0x85ebca6bu + (uint32_t)h.z * 0xc2b2ae35u);
+      uint32_t pr = gc_priority(pid, round);
+      if (!sa) atomicMax(&d.owner[ga * kColors + cc], pr);
+      if (!sb) atomicMax(&d.owner[gb * kColors + cc], pr);
+      L.flag = 1;
+    }
+    __syncthreads();
+    const bool any = L.flag != 0;
+    for (uint32_t p = tid; p < P; p += BLOCK) {
+      if (color[p] != kMAX || pick[p] >= kColors) continue;
+      int4 h = d.man_cur.hdr[W.pair_off + p];
+      const uint32_t ga = off + h.x, gb = off + h.y, cc = pick[p];
+      bool sa = !(d.cur.flags[ga] & BB_BODY_DYNAMIC), sb = !(d.cur.flags[gb] & BB_BODY_DYNAMIC);
+      uint32_t ida = d.cur.id[ga], idb = d.cur.id[gb];
+      uint32_t lo = ida < idb ? ida : idb, hi = ida < idb ? idb : ida;
+      uint32_t pid = gc_hash(lo * 0x9e3779b9u + hi * 0x85ebca6bu + (uint32_t)h.z * 0xc2b2ae35u);
+      uint32_t pr = gc_priority(pid, round);
+      bool wa = sa || d.owner[ga * kColors + cc] == pr;
+      bool wb = sb || d.owner[gb * kColors + cc] == pr;
+      if (wa && wb) {
+        color[p] = cc;
+        if (!sa) atomicOr(&d.mask[ga], 1u << cc);
+        if (!sb) atomicOr(&d.mask[gb], 1u << cc);
+      }
+    }
+    __syncthreads();
+    for (uint32_t p = tid; p < P; p += BLOCK) {
+      if (pick[p] >= kColors) continue;
+      int4 h = d.man_cur.hdr[W.pair_off + p];
+      d.owner[(off + h.x) * kColors + pick[p]] = 0;
+      d.owner[(off + h.y) * kColors + pick[p]] = 0;
+    }
+    __syncthreads();
+    if (!any) break;
+  }
+  // ---- validator (coloring.slang:214-292) and colour buckets
+  if (tid < kBuckets) L.bcount[tid] = 0;
+  __syncthreads();
+  uint32_t viol = 0, used = 0;
+  for (uint32_t p = tid; p < P; p += BLOCK) {
+    uint32_t cc = color[p];
+    if (cc == kMAX - 1) continue;  // no manifold in this slot
+    if (cc >= kColors) {
+      atomicAdd(&L.bcount[kColors], 1u);
+      continue;
+    }
+    atomicAdd(&L.bcount[cc], 1u);
+    used = used > cc + 1 ? used : cc + 1;
+    int4 h = d.man_cur.hdr[W.pair_off + p];
+    const uint32_t ga = off + h.x, gb = off + h.y;
+    if (d.cur.flags[ga] & BB_BODY_DYNAMIC)
+      if (atomicAdd(&d.owner[ga * kColors + cc], 1u) >= 1u) ++viol;
+    if (d.cur.flags[gb] & BB_BODY_DYNAMIC)
+      if (atomicAdd(&d.owner[gb * kColors + cc], 1u) >= 1u) ++viol;
+  }
+  stat_add(d, w, ST_GC_VIOL, viol);
+  if (used) stat_max(d, w, ST_GC_COUNT, used);
+  __syncthreads();
+  for (uint32_t p = tid; p < P; p += BLOCK) {
+    uint32_t cc = color[p];
+    if (cc >= kColors) continue;
+    int4 h = d.man_cur.hdr[W.pair_off + p];
+    d.owner[(off + h.x) * kColors + cc] = 0;
+    d.owner[(off + h.y) * kColors + cc] = 0;
+  }
+  if (tid == 0) {
+    uint32_t s = 0;
+    uint32_t* bo = d.bucket_off + w * (kBuckets + 1);
+    for (uint32_t c2 = 0; c2 < kBuckets; ++c2) {
+      bo[c2] = s;
+      L.bfill[c2] = s;
+      s += L.bcount[c2];
+    }
+    bo[kBuckets] = s;
+    d.stats[w * ST_COUNT + ST_GC_OVF] = L.bcount[kColors];
+  }
+  __syncthreads();
+  // colour buckets: order inside a colour is free (no shared dynamic body);
+  // the overflow bucket is filled in ascending slot order (serial solve).
+  uint32_t* bucket = d.bucket + W.pair_off;
+  for (uint32_t p = tid; p < P; p += BLOCK) {
+    uint32_t cc = color[p];
+    if (cc < kColors) bucket[atomicAdd(&L.bfill[cc], 1u)] = W.pair_off + p;
+  }
+  if (L.bcount[kColors]) {
+    if (tid == 0) L.carry = L.bfill[kColors];
+    __syncthreads();
+    for (uint32_t c0 = 0; c0 < P; c0 += BLOCK) {
+      uint32_t p = c0 + tid;
+      uint32_t f = (p < P && color[p] != kMAX - 1 && color[p] >= kColors) ? 1u : 0u;
+      uint32_t tot;
+      uint32_t ex = block_scan<BLOCK>(f, L.scan, tot);
+      if (f) bucket[L.carry + ex] = W.pair_off + p;
+      __syncthreads();
+      if (tid == 0) L.carry += tot;
+      __syncthreads();
+    }
+  }
+}
+
+// ================================================================== k_pgs
+// PGS_SOFT::pre_step / solve (src/shaders/simulation/solvers.slang:142-279),
+// colour by colour (RB_sim.slang:1408-1578); one thread per manifold, bodies
+// of one colour are disjoint, statics are never written.
+struct Pgs {
+  bool accum, friction;
+  float dt, inv_dt, hertz;
+};
+
+BB_DI mat3 iw_load(const Dev& d, uint32_t g) {
+  return mrows(ld3(d.iw[g * 3 + 0]), ld3(d.iw[g * 3 + 1]), ld3(d.iw[g * 3 + 2]));
+}
+
+BB_DI void pgs_manifold(const Dev& d, uint32_t off, uint32_t slot, const Pgs& s, int kind) {
+  int4 h = d.man_cur.hdr[slot];
+  const uint32_t ga = off + h.x, gb = off + h.y;
+  const uint32_t fa = d.cur.flags[ga], fb = d.cur.flags[gb];
+  const bool awa = (fa & BB_BODY_DYNAMIC) && !(fa & BB_BODY_SLEEPING);
+  const bool awb = (fb & BB_BODY_DYNAMIC) && !(fb & BB_BODY_SLEEPING);
+  if (!awa && !awb) return;
+  const bool da = (fa & BB_BODY_DYNAMIC) != 0, db = (fb & BB_BODY_DYNAMIC) != 0;
+  const uint32_t ida = d.cur.id[ga], idb = d.cur.id[gb];
+  const float4 ma = d.props.mass[off + ida], mb = d.props.mass[off + idb];
+  const float ima = ma.y, imb = mb.y;
+  const mat3 Ia = iw_load(d, ga), Ib = iw_load(d, gb);
+  v3 va = ld3(d.cur.vel[ga]), wa = ld3(d.cur.omg[ga]);
+  v3 vb = ld3(d.cur.vel[gb]), wb = ld3(d.cur.omg[gb]);
+  const v3 n = ld3(d.man_cur.nrm[slot]);
+  const v3 t0 = ld3(d.man_cur.t0[slot]), t1 = ld3(d.man_cur.t1[slot]);
+  const v3 pa = ld3(d.man_cur.anca[slot]), pb = ld3(d.man_cur.ancb[slot]);
+  if (kind == 0) {
+    for (int i = 0; i < h.w; ++i) {
+      const uint32_t cs = slot * 8 + i;
+      const float4 cp = d.man_cur.cpos[cs];
+      float4 lam = d.man_cur.clam[cs];
+      const v3 r1 = ld3(cp) - pa, r2 = ld3(cp) - pb;
+      v3 rn1 = cross(r1, n), rn2 = cross(r2, n);
+      float kn = ima + imb;
+      kn += dot(rn1, mulmv(Ia, rn1));
+      kn += dot(rn2, mulmv(Ib, rn2));
+      float nm = kn == 0.0f ? 0.0f : 1.0f / kn;
+      float tm[2];
+      for (int j = 0; j < 2; ++j) {
+        v3 tj = j == 0 ? t0 : t1;
+        v3 rt1 = cross(r1, tj), rt2 = cross(r2, tj);
+        float kt = ima + imb;
+        kt += dot(rt1, mulmv(Ia, rt1));
+        kt += dot(rt2, mulmv(Ib, rt2));
+        tm[j] = kt == 0.0f ? 0.0f : 1.0f / kt;
+      }
+      float ch = (ima == 0.0f || imb == 0.0f) ? 2.0f * s.hertz : s.hertz;
+      const float zeta = 10.0f;
+      const float omega = 2.0f * 3.14159265359f * ch;
+      const float cc = s.dt * omega * (2.0f * zeta + s.dt * omega);
+      float bias_factor = omega / (2.0f * zeta + s.dt * omega);
+      float impulse_coef = 1.0f / (1.0f + cc);
+      float mass_coef = cc * impulse_coef;
+      lam.w = mass_coef;
+      d.man_cur.clam[cs] = lam;
+      d.man_cur.cmas[cs] = make_float4(nm, tm[0], tm[1], bias_factor);
+      float4 ana = d.man_cur.cana[cs];
+      ana.w = impulse_coef;
+      d.man_cur.cana[cs] = ana;
+      if (s.accum) {
+        v3 P = n * lam.x;
+        if (s.friction) {
+          P = P + t0 * lam.y;
+          P = P + t1 * lam.z;
+        }
+        if (da) {
+          va = va - P * ima;
+          wa = wa - mulmv(Ia, cross(r1, P));
+        }
+        if (db) {
+          vb = vb + P * imb;
+          wb = wb + mulmv(Ib, cross(r2, P));
+        }
+      }
+    }
+  } else {
+    const bool use_bias = kind == 1;
+    const float fr = __builtin_sqrtf(ma.z * mb.z);
+    for (int k = 0; k < h.w; ++k) {
+      const uint32_t cs = slot * 8 + k;
+      const float4 cp = d.man_cur.cpos[cs];
+      float4 lam = d.man_cur.clam[cs];
+      const float4 cm = d.man_cur.cmas[cs];
+      const float impulse_coef = d.man_cur.cana[cs].w;
+      const v3 r1 = ld3(cp) - pa, r2 = ld3(cp) - pb;
+      v3 rv = ((vb + cross(wb, r2)) - va) - cross(wa, r1);
+      if (s.friction) {
+        for (int j = 0; j < 2; ++j) {
+          v3 tj = j == 0 ? t0 : t1;
+          float tmass = j == 0 ? cm.y : cm.z;
+          float tv = dot(rv, tj);
+          float ti = tmass * -tv;
+          float maxf = fr * lam.x;
+          if (s.accum) {
+            float old = j == 0 ? lam.y : lam.z;
+            float nw = fcl(old + ti, -maxf, maxf);
+            if (j == 0) lam.y = nw; else lam.z = nw;
+            ti = nw - old;
+          } else {
+            ti = fcl(ti, -maxf, maxf);
+          }
+          v3 tiv = tj * ti;
+          if (da) {
+            va = va - tiv * ima;
+            wa = wa - mulmv(Ia, cross(r1, tiv));
+          }
+          if (db) {
+            vb = vb + tiv * imb;
+            wb = wb + mulmv(Ib, cross(r2, tiv));
+          }
+        }
+      }
+      rv = ((vb + cross(wb, r2)) - va) - cross(wa, r1);
+      float nv = dot(rv, n);
+      float bias = 0.0f, mass_scale = 1.0f, impulse_scale = 0.0f;
+      if (cp.w > 0.0f) {
+        bias = cp.w * s.inv_dt;
+      } else if (use_bias) {
+        bias = fmx(cm.w * cp.w, -4.0f);
+        mass_scale = lam.w;
+        impulse_scale = impulse_coef;
+      }
+      float imp = -cm.x * mass_scale * (nv + bias) - impulse_scale * lam.x;
+      if (s.accum) {
+        float old = lam.x;
+        lam.x = fmx(old + imp, 0.0f);
+        imp = lam.x - old;
+      } else {
+        imp = fmx(imp, 0.0f);
+      }
+      v3 niv = n * imp;
+      if (da) {
+        va = va - niv * ima;
+        wa = wa - mulmv(Ia, cross(r1, niv));
+      }
+      if (db) {
+        vb = vb + niv * imb;
+        wb = wb + mulmv(Ib, cross(r2, niv));
+      }
+      d.man_cur.clam[cs] = lam;
+    }
+  }
+  if (da) {
+    d.cur.vel[ga] = st3(va, 0.0f);
+    d.cur.omg[ga] = st3(wa, 0.0f);
+  }
+  if (db) {
+    d.cur.vel[gb] = st3(vb, 0.0f);
+    d.cur.omg[gb] = st3(wb, 0.0f);
+  }
+}
+
+template <int BLOCK>
+BB_DI void iw_refresh(const Dev& d, const WorldInfo& W) {
+  const bool quirk = (W.quirks & BB_QUIRK_TRANSPOSED_WORLD_INERTIA) != 0;
+  for (uint32_t i = threadIdx.x; i < W.n; i += BLOCK) {
+    const uint32_t g = W.body_off + i;
+    mat3 M = world_inv_inertia(d, W.body_off + d.cur.id[g], ldq(d.cur.rot[g]), quirk);
+    d.iw[g * 3 + 0] = st3(mrow(M, 0), 0.0f);
+    d.iw[g * 3 + 1] = st3(mrow(M, 1), 0.0f);
+    d.iw[g * 3 + 2] = st3(mrow(M, 2), 0.0f);
+  }
+}
+
+// RigidBody::apply_angular_velocity (src/shaders/extensions.slang:148-166)
+BB_DI qt apply_angular_velocity(qt q, v3 om, float dt) {
+  float angle = vlen(om) * dt;
+  if (angle > 0.0001f) {
+    v3 axis = vnorm(om);
+    float half = angle / 2.0f;
+    float sh, ch;
+    sincos_c(half, &sh, &ch);
+    return qnorm(qmul(qt{axis * sh, ch}, q));
+  }
+  v3 dr = (om * dt) * 0.5f;
+  return qnorm(qmul(qt{dr, 1.0f}, q));
+}
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_pgs(Dev d) {
+  const uint32_t w = blockIdx.x, tid = threadIdx.x;
+  const WorldInfo W = d.worlds[w];
+  if (W.solver == BB_SOLVER_AVBD || W.n == 0) return;
+  const uint32_t off = W.body_off, N = W.n, A = W.a;
+  Pgs s;
+  s.accum = (W.flags & BB_SIM_ACCUM_IMPULSE) != 0;
+  s.friction = (W.flags & BB_SIM_FRICTION) != 0;
+  s.dt = W.dt;
+  s.inv_dt = W.dt > 0.0f ? 1.0f / W.dt : 0.0f;
+  s.hertz = fmn(30.0f, s.inv_dt * 0.3333333333333333f);
+  const uint32_t* bo = d.bucket_off + w * (kBuckets + 1);
+  const uint32_t* bucket = d.bucket + W.pair_off;
+  const uint32_t used = d.stats[w * ST_COUNT + ST_GC_COUNT];
+  const uint32_t ovf_lo = bo[kColors], ovf_hi = bo[kBuckets];
+  iw_refresh<BLOCK>(d, W);
+  __syncthreads();
+  auto sweep = [&](int kind) {
+    for (uint32_t c = 0; c < used; ++c) {
+      for (uint32_t k = bo[c] + tid; k < bo[c + 1]; k += BLOCK) pgs_manifold(d, off, bucket[k], s, kind);
+      __syncthreads();
+    }
+    if (ovf_hi > ovf_lo) {
+      if (tid == 0)
+        for (uint32_t k = ovf_lo; k < ovf_hi; ++k) pgs_manifold(d, off, bucket[k], s, kind);
+      __syncthreads();
+    }
+  };
+  sweep(0);
+  for (uint32_t it = 0; it < W.iterations; ++it) sweep(1);
+  // integrate positions (RB_sim.slang:1287-1326), same row quirk as advect
+  if (W.flags & BB_SIM_ADVECTION) {
+    uint32_t lim = N;
+    if (W.quirks & BB_QUIRK_ACTIVE_DISPATCH_ROWS) {
+      uint32_t r = ((A + 31u) / 32u) * 32u;
+      lim = r < N ? r : N;
+    }
+    for (uint32_t i = tid; i < lim; i += BLOCK) {
+      const uint32_t g = off + i;
+      uint32_t fl = d.cur.flags[g];
+      if (!((fl & BB_BODY_DYNAMIC) && !(fl & BB_BODY_SLEEPING))) continue;
+      v3 p = ld3(d.cur.pos[g]), v = ld3(d.cur.vel[g]);
+      d.cur.pos[g] = st3(p + v * W.dt, 0.0f);
+      d.cur.rot[g] = stq(apply_angular_velocity(ldq(d.cur.rot[g]), ld3(d.cur.omg[g]), W.dt));
+    }
+  }
+  __syncthreads();
+  iw_refresh<BLOCK>(d, W);
+  __syncthreads();
+  for (uint32_t it = 0; it < W.iterations; ++it) sweep(2);
+}
+
+// ============================================================== k_finish
+// End-of-step counters (RB_sim.slang:1655-1715): id checksum and max |v|.
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_finish(Dev d) {
+  const uint32_t w = blockIdx.x, tid = threadIdx.x;
+  const WorldInfo W = d.worlds[w];
+  uint32_t idsum = 0, maxv = 0;
+  for (uint32_t i = tid; i < W.n; i += BLOCK) {
+    const uint32_t g = W.body_off + i;
+    idsum += d.cur.id[g];
+    if (d.cur.flags[g] & BB_BODY_DYNAMIC) {
+      v3 v = ld3(d.cur.vel[g]);
+      uint32_t mv = (uint32_t)fmn(__builtin_sqrtf(dot(v, v)) * 1000.0f, 1.0e9f);
+      maxv = maxv > mv ? maxv : mv;
+    }
+  }
+  stat_add(d, w, ST_IDSUM, idsum);
+  if (maxv) stat_max(d, w, ST_MAXV, maxv);
+  if (tid == 0) {
+    d.stats[w * ST_COUNT + ST_N] = W.n;
+    d.stats[w * ST_COUNT + ST_A] = W.a;
+    d.stats[w * ST_COUNT + ST_NODES] = 2 * d.stats[w * ST_COUNT + ST_M];
+  }
+}
+
+}  // namespace bb
+
+// ============================================================ host launch
+namespace bb {
+#define BB_INSTANTIATE(B)                      \
+  template __global__ void k_build<B>(Dev);    \
+  template __global__ void k_broad<B>(Dev);    \
+  template __global__ void k_narrow<B>(Dev);   \
+  template __global__ void k_post<B>(Dev);     \
+  template __global__ void k_pgs<B>(Dev);      \
+  template __global__ void k_finish<B>(Dev);
+
+template <int B>
+static void launch_all(const Dev& d, hipStream_t st) {
+  dim3 g(d.W), b(B);
+  hipLaunchKernelGGL(k_build<B>, g, b, 0, st, d);
+  hipLaunchKernelGGL(k_broad<B>, g, b, 0, st, d);
+  hipLaunchKernelGGL(k_narrow<B>, g, b, 0, st, d);
+  hipLaunchKernelGGL(k_post<B>, g, b, 0, st, d);
+  hipLaunchKernelGGL(k_pgs<B>, g, b, 0, st, d);
+  hipLaunchKernelGGL(k_finish<B>, g, b, 0, st, d);
+}
+
+void launch_step(const Dev& d, hipStream_t st, uint32_t block) {
+  switch (block) {
+    case 512: launch_all<512>(d, st); break;
+    case 1024: launch_all<1024>(d, st); break;
+    default: launch_all<256>(d, st); break;
+  }
+}
+}  // namespace bb

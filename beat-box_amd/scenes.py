@@ -1,0 +1,211 @@
+"""Seeded synthetic scenes for the rigid-body step.
+
+Every draw comes from SplitMix64 with an explicit seed: output k uses the state
+seed + (k+1)*0x9E3779B97F4A7C15, so draws are a pure function of (seed, k) and
+are vectorised with numpy.  A float in [0, 1) is (u >> 40) * 2**-24.  Draws are
+made in body-id order, so CPU and GPU consume identical bytes.
+
+Scenes return a numpy structured array with the layout of `bb_body_desc`
+(include/bb_capi.h) plus the world descriptor fields to use.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+GOLDEN = np.uint64(0x9E3779B97F4A7C15)
+MIX1 = np.uint64(0xBF58476D1CE4E5B9)
+MIX2 = np.uint64(0x94D049BB133111EB)
+
+# bb_body_desc, include/bb_capi.h
+BODY_DESC = np.dtype(
+    [
+        ("shape", "<u4"),
+        ("flags", "<u4"),
+        ("position", "<f4", 3),
+        ("rotation", "<f4", 4),
+        ("minimum", "<f4", 3),
+        ("maximum", "<f4", 3),
+        ("radius", "<f4"),
+        ("mass", "<f4"),
+        ("velocity", "<f4", 3),
+        ("omega", "<f4", 3),
+        ("restitution", "<f4"),
+        ("friction", "<f4"),
+    ]
+)
+assert BODY_DESC.itemsize == 100
+
+SHAPE_BOX, SHAPE_SPHERE = 0, 1
+BODY_DYNAMIC, BODY_GRAVITY = 2, 4
+SOLVER_PGS, SOLVER_PGS_SOFT, SOLVER_AVBD = 0, 1, 2
+SIM_DEFAULT_FLAGS = 1 | 2 | 4 | 8 | 16 | 32 | 128
+QUIRKS_PARITY = 1 | 2 | 4
+TIME_STEP = np.float32(0.01667)
+
+
+def splitmix64(seed: int, counters: np.ndarray) -> np.ndarray:
+    """SplitMix64 outputs for the given counters (vectorised)."""
+    with np.errstate(over="ignore"):
+        z = np.uint64(seed) + (counters.astype(np.uint64) + np.uint64(1)) * GOLDEN
+        z = (z ^ (z >> np.uint64(30))) * MIX1
+        z = (z ^ (z >> np.uint64(27))) * MIX2
+        return z ^ (z >> np.uint64(31))
+
+
+def uniform(seed: int, counters: np.ndarray, lo: float, hi: float) -> np.ndarray:
+    u = (splitmix64(seed, counters) >> np.uint64(40)).astype(np.float64) * (2.0 ** -24)
+    return lo + (hi - lo) * u
+
+
+def world_desc(solver=SOLVER_PGS_SOFT, iterations=8, flags=SIM_DEFAULT_FLAGS,
+               dt=float(TIME_STEP), gravity=-9.81, morton_bits=30,
+               quirks=QUIRKS_PARITY, capacity=0) -> dict:
+    return dict(capacity_bodies=capacity, solver=solver, iterations=iterations,
+                flags=flags, dt=dt, gravity=gravity, morton_bits=morton_bits,
+                quirks=quirks)
+
+
+def _ground(body: np.ndarray, half: float = 50.0, friction: float = 0.5) -> None:
+    """Static ground box, centre (0,-half,0): the reference floor
+    (src/scene_manager.hpp:257)."""
+    body["shape"] = SHAPE_BOX
+    body["flags"] = 0
+    body["position"] = (0.0, -half, 0.0)
+    body["rotation"] = (0.0, 0.0, 0.0, 1.0)
+    body["minimum"] = (-half, -half, -half)
+    body["maximum"] = (half, half, half)
+    body["mass"] = 0.0
+    body["friction"] = friction
+    body["restitution"] = 0.0
+
+
+def yaw_quat(theta: np.ndarray) -> np.ndarray:
+    q = np.zeros((theta.shape[0], 4), np.float64)
+    q[:, 1] = np.sin(theta * 0.5)
+    q[:, 3] = np.cos(theta * 0.5)
+    return q.astype(np.float32)
+
+
+def cubes_grid(seed: int = 0xBB01, nx: int = 10, ny: int = 10, nz: int = 10,
+               spacing: float = 1.5, jitter: float = 0.1, base_y: float = 2.5):
+    """Plain method small: 1,000 unit cubes (density 1, friction 0.5,
+    restitution 0) on a 10x10x10 grid, 1.5 m spacing, jitter U(-0.1,0.1),
+    random yaw, dropped from 2 m onto the static ground.  Body 0 = ground."""
+    n = nx * ny * nz
+    bodies = np.zeros(n + 1, BODY_DESC)
+    _ground(bodies[0])
+    idx = np.arange(n)
+    i = idx % nx
+    k = (idx // nx) % nz
+    j = idx // (nx * nz)
+    F = 4
+    c = idx.astype(np.uint64) * np.uint64(F)
+    jx = uniform(seed, c + np.uint64(0), -jitter, jitter)
+    jy = uniform(seed, c + np.uint64(1), -jitter, jitter)
+    jz = uniform(seed, c + np.uint64(2), -jitter, jitter)
+    yaw = uniform(seed, c + np.uint64(3), 0.0, 2.0 * math.pi)
+    b = bodies[1:]
+    b["shape"] = SHAPE_BOX
+    b["flags"] = BODY_DYNAMIC | BODY_GRAVITY
+    b["position"][:, 0] = ((i - (nx - 1) / 2.0) * spacing + jx).astype(np.float32)
+    b["position"][:, 1] = (base_y + spacing * j + jy).astype(np.float32)
+    b["position"][:, 2] = ((k - (nz - 1) / 2.0) * spacing + jz).astype(np.float32)
+    b["rotation"] = yaw_quat(yaw)
+    b["minimum"] = -0.5
+    b["maximum"] = 0.5
+    b["mass"] = 1.0
+    b["friction"] = 0.5
+    b["restitution"] = 0.0
+    return bodies
+
+
+def random_boxes(seed: int, n: int, extent: float = 6.0, y0: float = 0.6,
+                 y1: float = 6.0, half: float = 0.5, with_ground: bool = True,
+                 tilt: bool = True):
+    """Small random scenes for parity tests: n cubes in a box region with a
+    random orientation (Shoemake) and the ground box."""
+    bodies = np.zeros(n + (1 if with_ground else 0), BODY_DESC)
+    off = 0
+    if with_ground:
+        _ground(bodies[0])
+        off = 1
+    idx = np.arange(n, dtype=np.uint64) * np.uint64(8)
+    b = bodies[off:]
+    b["shape"] = SHAPE_BOX
+    b["flags"] = BODY_DYNAMIC | BODY_GRAVITY
+    b["position"][:, 0] = uniform(seed, idx + np.uint64(0), -extent, extent)
+    b["position"][:, 1] = uniform(seed, idx + np.uint64(1), y0, y1)
+    b["position"][:, 2] = uniform(seed, idx + np.uint64(2), -extent, extent)
+    if tilt:
+        b["rotation"] = shoemake(seed, idx + np.uint64(3))
+    else:
+        b["rotation"] = yaw_quat(uniform(seed, idx + np.uint64(3), 0.0, 2 * math.pi))
+    b["minimum"] = -half
+    b["maximum"] = half
+    b["mass"] = 1.0
+    b["friction"] = 0.5
+    return bodies
+
+
+def shoemake(seed: int, c: np.ndarray) -> np.ndarray:
+    """Uniform random rotation (Shoemake 1992) from three draws c, c+1, c+2."""
+    u1 = uniform(seed, c, 0.0, 1.0)
+    u2 = uniform(seed, c + np.uint64(1), 0.0, 2 * math.pi)
+    u3 = uniform(seed, c + np.uint64(2), 0.0, 2 * math.pi)
+    a = np.sqrt(1.0 - u1)
+    bb = np.sqrt(u1)
+    q = np.stack([a * np.sin(u2), a * np.cos(u2), bb * np.sin(u3), bb * np.cos(u3)], 1)
+    return q.astype(np.float32)
+
+
+def batch_world_sizes(seed: int = 0xBB04, worlds: int = 4096, lo: int = 16, hi: int = 2048):
+    u = uniform(seed, np.arange(worlds, dtype=np.uint64) * np.uint64(4),
+                math.log(lo), math.log(hi))
+    return np.clip(np.rint(np.exp(u)), lo, hi).astype(np.int64)
+
+
+def batch_world(seed: int, w: int, n_boxes: int):
+    """One batched world: even w = planar box pyramid (rows L..1 truncated to
+    n_boxes), odd w = 4x4-footprint tower of ceil(n/16) layers; box half-extent
+    U(0.3, 0.7) per world; its own ground box (body 0)."""
+    h = float(uniform(seed, np.array([w * 4 + 1], np.uint64), 0.3, 0.7)[0])
+    bodies = np.zeros(n_boxes + 1, BODY_DESC)
+    _ground(bodies[0])
+    pos = np.zeros((n_boxes, 3), np.float64)
+    if w % 2 == 0:
+        L = 1
+        while L * (L + 1) // 2 < n_boxes:
+            L += 1
+        k = 0
+        gap = 2.0 * h * 1.04
+        for r in range(L):
+            cnt = L - r
+            for c in range(cnt):
+                if k >= n_boxes:
+                    break
+                pos[k] = ((c - (cnt - 1) / 2.0) * gap, h * 1.002 + r * 2.0 * h * 1.002, 0.0)
+                k += 1
+    else:
+        gap = 2.0 * h * 1.03
+        for k in range(n_boxes):
+            layer, cell = divmod(k, 16)
+            cx, cz = divmod(cell, 4)
+            pos[k] = ((cx - 1.5) * gap, h * 1.002 + layer * 2.0 * h * 1.002, (cz - 1.5) * gap)
+    b = bodies[1:]
+    b["shape"] = SHAPE_BOX
+    b["flags"] = BODY_DYNAMIC | BODY_GRAVITY
+    b["position"] = pos.astype(np.float32)
+    b["rotation"] = (0.0, 0.0, 0.0, 1.0)
+    b["minimum"] = -h
+    b["maximum"] = h
+    b["mass"] = np.float32((2 * h) ** 3)
+    b["friction"] = 0.5
+    return bodies
+
+
+def batch_worlds(seed: int = 0xBB04, worlds: int = 4096, lo: int = 16, hi: int = 2048):
+    """Batched worlds: body counts log-uniform in [lo, hi]."""
+    sizes = batch_world_sizes(seed, worlds, lo, hi)
+    return [batch_world(seed, w, int(n)) for w, n in enumerate(sizes)]

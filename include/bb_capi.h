@@ -1,0 +1,316 @@
+/*
+ * bb_capi.h — C-ABI of the MI355X-native rigid-body step (beat-box drop-in).
+ *
+ * Plain C: POD structs, opaque handles, status-code returns, no exceptions
+ * cross this boundary, no HIP/torch types in any signature (streams are passed
+ * as `void*`, NULL = the library's own stream).
+ *
+ * Each entry point replaces one piece of the reference's C++ surface
+ * (Jaisiero/beat-box).  The citation next to each declaration names the
+ * reference function whose meaning, argument semantics and failure behaviour it
+ * keeps.  The reference exposes no C ABI of its own; its callers are
+ * SceneManager::load_scene and RendererManager::render.
+ *
+ * Thread safety: one handle per thread, or callers serialise.  There is no
+ * global mutable state apart from per-device kernel modules.
+ * Memory: the library owns device memory; the caller owns every host buffer.
+ */
+#ifndef BB_CAPI_H
+#define BB_CAPI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status */
+typedef enum bb_status {
+  BB_OK = 0,
+  BB_ERR_INVALID = 1,   /* bad argument / malformed descriptor              */
+  BB_ERR_CAPACITY = 2,  /* body / world capacity exceeded (add-body refusal)*/
+  BB_ERR_DEVICE = 3,    /* HIP runtime error                                */
+  BB_ERR_NO_DEVICE = 4, /* no GPU visible: there is no CPU fallback         */
+  BB_ERR_OVERFLOW = 5,  /* device-side buffer overflow flagged during a step*/
+  BB_ERR_STATE = 6,     /* call not valid in the current state              */
+  BB_ERR_BUDGET = 7     /* serving: request does not fit the HBM budget     */
+} bb_status;
+
+/* --------------------------------------------------------- enums / flags */
+/* SimSolverType, include/shared.inl:210-216.  PGS (plain sequential impulse)
+ * exists in the reference only on the per-island path; with graph colouring on
+ * (the default) "PGS" runs as PGS_SOFT, which is what this library does. */
+typedef enum bb_solver {
+  BB_SOLVER_PGS = 0,
+  BB_SOLVER_PGS_SOFT = 1,
+  BB_SOLVER_AVBD = 2
+} bb_solver;
+
+/* SimFlag, include/shared.inl:150-161 (same bit values). */
+enum {
+  BB_SIM_ACCUM_IMPULSE = 1u << 0,
+  BB_SIM_FRICTION = 1u << 1,
+  BB_SIM_WARM_STARTING = 1u << 2,
+  BB_SIM_ADVECTION = 1u << 3,
+  BB_SIM_DEBUG_INFO = 1u << 4,
+  BB_SIM_USE_GRAPH_COLORING = 1u << 5,
+  BB_SIM_DEBUG_GRAPH_COLORS = 1u << 6,
+  BB_SIM_SLEEPING_ENABLED = 1u << 7
+};
+/* Default set, src/rigid_body_manager.hpp:140-147. */
+#define BB_SIM_DEFAULT_FLAGS                                                   \
+  (BB_SIM_ADVECTION | BB_SIM_DEBUG_INFO | BB_SIM_FRICTION |                    \
+   BB_SIM_ACCUM_IMPULSE | BB_SIM_WARM_STARTING | BB_SIM_USE_GRAPH_COLORING |   \
+   BB_SIM_SLEEPING_ENABLED)
+
+/* RigidBodyFlag, include/shared.inl:91-97 (same bit values). */
+enum {
+  BB_BODY_NONE = 0,
+  BB_BODY_COLLIDING = 1u << 0,
+  BB_BODY_DYNAMIC = 1u << 1,
+  BB_BODY_GRAVITY = 1u << 2,
+  BB_BODY_SLEEPING = 1u << 3
+};
+
+typedef enum bb_shape { BB_SHAPE_BOX = 0, BB_SHAPE_SPHERE = 1 } bb_shape;
+
+/* Reference-compatibility switches ("parity mode").  Each reproduces one
+ * documented behaviour of the reference so results can be compared with it. */
+enum {
+  /* inverse inertia built from inv_mass where mass is expected
+   * (src/scene_manager.hpp:861 -> include/math.hpp:106-130)              */
+  BB_QUIRK_INERTIA_FROM_INV_MASS = 1u << 0,
+  /* world inverse inertia as R^T I^-1 R (include/shared.inl:367-375,
+   * src/shaders/extensions.slang:184-188)                                */
+  BB_QUIRK_TRANSPOSED_WORLD_INERTIA = 1u << 1,
+  /* advect/integrate dispatched over ceil(A/32)*32 threads but indexing rows
+   * < N (src/rigid_body_manager.cpp advect/IP tasks use the ACTIVE dispatch) */
+  BB_QUIRK_ACTIVE_DISPATCH_ROWS = 1u << 2
+};
+#define BB_QUIRKS_PARITY                                                       \
+  (BB_QUIRK_INERTIA_FROM_INV_MASS | BB_QUIRK_TRANSPOSED_WORLD_INERTIA |        \
+   BB_QUIRK_ACTIVE_DISPATCH_ROWS)
+
+/* ------------------------------------------------------------ descriptors */
+/* World descriptor.  Defaults (bb_world_desc_default) follow the reference:
+ * dt = TIME_STEP 0.01667f, gravity = -GRAVITY, iteration_count = 5,
+ * solver = AVBD, flags = default set (include/defines.hpp:354-364,
+ * src/rigid_body_manager.hpp:140-147, 258-261). */
+typedef struct bb_world_desc {
+  uint32_t capacity_bodies; /* 0 = grow on demand                          */
+  uint32_t solver;          /* bb_solver                                    */
+  uint32_t iterations;      /* PGS iteration_count (create() argument)      */
+  uint32_t flags;           /* BB_SIM_* bitmask                             */
+  float dt;
+  float gravity; /* signed y acceleration, reference uses -9.81             */
+  /* 30 = reference keys: 30-bit Morton over the fixed box [-64,64]^3
+   * (src/shaders/simulation/RB_sim.slang:99-101); 63 = 21 bits/axis over
+   * the AABB of this world's body centres (extension, no body cap). */
+  uint32_t morton_bits;
+  uint32_t quirks; /* BB_QUIRK_* bitmask (BB_QUIRKS_PARITY for parity)     */
+} bb_world_desc;
+
+/* One body to add.  Boxes use local bounds minimum/maximum (reference
+ * RigidBody::minimum/maximum); spheres use `radius` (extension: the reference
+ * has no spheres).  mass == 0 means immovable, as in load_scene. */
+typedef struct bb_body_desc {
+  uint32_t shape; /* bb_shape                                             */
+  uint32_t flags; /* BB_BODY_* (DYNAMIC, GRAVITY)                         */
+  float position[3];
+  float rotation[4]; /* quaternion x, y, z, w                              */
+  float minimum[3];
+  float maximum[3];
+  float radius;
+  float mass;
+  float velocity[3];
+  float omega[3];
+  float restitution; /* stored, ignored like the reference (solvers.slang:49-56) */
+  float friction;
+} bb_body_desc;
+
+/* Readback of one body. */
+typedef struct bb_body_state {
+  uint32_t id;
+  uint32_t flags;
+  uint32_t island_index; /* MAX = not in an island                         */
+  uint32_t row;          /* current Morton-sorted row                      */
+  float position[3];
+  float rotation[4];
+  float velocity[3];
+  float omega[3];
+} bb_body_state;
+
+typedef enum bb_order { BB_ORDER_BY_ID = 0, BB_ORDER_BY_ROW = 1 } bb_order;
+
+/* SimConfig-equivalent per-step counters (include/shared.inl:571-666). */
+typedef struct bb_step_stats {
+  uint32_t rigid_body_count;            /* N                                */
+  uint32_t active_rigid_body_count;     /* A                                */
+  uint32_t broad_phase_collision_count; /* P                                */
+  uint32_t collision_count;             /* M (manifolds)                    */
+  uint32_t manifold_node_count;         /* 2M                               */
+  uint32_t contact_count;               /* C                                */
+  uint32_t island_count;
+  uint32_t contact_island_count;
+  uint32_t sleeping_count;
+  uint32_t graph_color_count;
+  uint32_t graph_color_violations;
+  uint32_t graph_color_overflow;
+  uint32_t avbd_color_count;
+  uint32_t avbd_violations;
+  uint32_t avbd_stick_count;
+  uint32_t dbg_fresh;  /* contacts with warm_start == 0                   */
+  uint32_t dbg_pen;    /* deepest penetration this step, integer mm       */
+  uint32_t dbg_id_sum; /* sum of ids over rows (permutation checksum)     */
+  uint32_t dbg_maxv;   /* max |v| over dynamic rows, integer mm/s         */
+  uint32_t overflow;   /* nonzero: a device-side capacity was exceeded    */
+  uint64_t step_index;
+} bb_step_stats;
+
+/* Debug taps: canonical intermediate outputs for the oracle comparison. */
+typedef enum bb_tap {
+  BB_TAP_MORTON = 0,      /* u32 (30-bit) or u64 (63-bit) per pre-sort row */
+  BB_TAP_SORTED = 1,      /* {u64 code, u32 index, u32 pad} in sorted order */
+  BB_TAP_LBVH = 2,        /* bb_lbvh_node[2N-1]                             */
+  BB_TAP_PAIRS = 3,       /* {u32 a, u32 b} broad-phase pairs, (a,b) order  */
+  BB_TAP_MANIFOLDS = 4,   /* bb_manifold_tap[M] after the solve             */
+  BB_TAP_ISLANDS = 5,     /* u32 per row: island root (min active index)    */
+  BB_TAP_EDGE_COLORS = 6, /* u32 per manifold                               */
+  BB_TAP_BODY_COLORS = 7, /* u32 per row (AVBD)                             */
+  BB_TAP_ROWS = 8         /* bb_body_state per row                          */
+} bb_tap;
+
+typedef struct bb_lbvh_node { /* LBVHNode, include/shared.inl:867-897     */
+  float minimum[3];
+  float maximum[3];
+  int32_t left;
+  int32_t right;
+  uint32_t primitive_idx;
+  uint32_t pad;
+} bb_lbvh_node;
+
+typedef struct bb_contact_tap {
+  float position[3];
+  float penetration;
+  float normal_impulse;
+  float tangent_impulse[2];
+  float mass_coefficient;
+  float tangent_mass[2];
+  float anchor_a[3];
+  float anchor_b[3];
+  uint32_t stick;
+  uint32_t fp[4];
+  uint32_t warm_start;
+} bb_contact_tap;
+
+typedef struct bb_manifold_tap {
+  uint32_t obb1_index;
+  uint32_t obb2_index;
+  int32_t key;
+  int32_t error;
+  int32_t contact_count;
+  uint32_t color;
+  float normal[3];
+  float tangent[2][3];
+  float anchor_a[3];
+  float anchor_b[3];
+  bb_contact_tap contacts[8];
+} bb_manifold_tap;
+
+/* ------------------------------------------------------- single world API */
+typedef struct bb_world bb_world;
+
+/* Fills *desc with the reference defaults. */
+void bb_world_desc_default(bb_world_desc* desc);
+
+/* RigidBodyManager::create(name, renderer, gui, iterations = 5),
+ * src/rigid_body_manager.hpp:17 / src/rigid_body_manager.cpp:121-1488. */
+bb_status bb_world_create(const bb_world_desc* desc, bb_world** out);
+
+/* SceneManager::load_scene body rules + AccelerationStructureManager::
+ * build_accel_structs/build_AS (src/scene_manager.hpp:850-887,
+ * src/acceleration_structure_manager.cpp:224-441).  Ids are assigned
+ * sequentially from 0; inverse mass/inertia follow the load_scene rules.
+ * Fails with BB_ERR_CAPACITY (nothing added) when the world is full, as
+ * build_accel_structs returns false when N > MAX_RIGID_BODY_COUNT. */
+bb_status bb_world_add_bodies(bb_world* w, const bb_body_desc* bodies,
+                              uint32_t n, uint32_t* out_ids);
+
+/* RigidBodyManager::set_sim_flags (ORs the bits in), clear_sim_flags, and
+ * get_sim_flags (src/rigid_body_manager.hpp:35-53).  Like the reference, a
+ * flag change takes effect on the next step and keeps warm-start state. */
+bb_status bb_world_set_flags(bb_world* w, uint32_t flags);
+bb_status bb_world_clear_flags(bb_world* w, uint32_t flags);
+uint32_t bb_world_get_flags(const bb_world* w);
+/* RigidBodyManager::set_sim_type (src/rigid_body_manager.hpp:55-60).  As in
+ * the reference (set_sim_type marks the sim dirty and update_sim() rewrites
+ * SimConfig, zeroing the previous manifold count), the next step finds no old
+ * manifolds to warm start from. */
+bb_status bb_world_set_solver(bb_world* w, uint32_t solver);
+/* create(..., iterations) argument (src/rigid_body_manager.hpp:17). */
+bb_status bb_world_set_iterations(bb_world* w, uint32_t iterations);
+/* RigidBodyManager::skip_warm_starting_once (src/rigid_body_manager.hpp:29-31):
+ * the next step runs with WARM_STARTING cleared. */
+bb_status bb_world_skip_warm_start_once(bb_world* w);
+
+/* RigidBodyManager::simulate() (src/rigid_body_manager.cpp:1606-1622), nsteps
+ * times, enqueued on `stream` (NULL = library stream).  Asynchronous. */
+bb_status bb_world_step(bb_world* w, uint32_t nsteps, void* stream);
+/* device.wait_idle() after a step (src/renderer_manager.cpp:254-257). */
+bb_status bb_world_sync(bb_world* w);
+
+/* Body readback.  BY_ID goes through the id -> row map (rigid_body_entries,
+ * src/shaders/simulation/RB_sim.slang:374-381); BY_ROW returns Morton rows.
+ * The reference has no host body readback (its consumer is the TLAS update,
+ * src/shaders/AS_mngr.slang:19-27); this is that consumer's read path. */
+bb_status bb_world_read_bodies(bb_world* w, bb_body_state* out, uint32_t n,
+                               bb_order order);
+/* RigidBodyManager::read_back_sim_config + get_sim_config_reference
+ * (src/rigid_body_manager.cpp:1681-1693, :79-82): counters of the last step. */
+bb_status bb_world_read_stats(bb_world* w, bb_step_stats* out);
+/* Canonical intermediate outputs of the last step (for parity tests). */
+bb_status bb_world_debug_tap(bb_world* w, bb_tap kind, void* dst, size_t cap,
+                             size_t* written);
+/* Body count of the world. */
+uint32_t bb_world_body_count(const bb_world* w);
+/* RigidBodyManager::destroy (src/rigid_body_manager.cpp:1550-1585). */
+void bb_world_destroy(bb_world* w);
+
+/* ------------------------------------------------------------- batch API */
+/* Many independent worlds of uneven size stepped together: segmented sorts,
+ * BVHs and colour sweeps, one workgroup per world.  No reference counterpart
+ * (the reference steps one world). */
+typedef struct bb_batch bb_batch;
+
+bb_status bb_batch_create(uint32_t capacity_worlds, uint32_t capacity_bodies,
+                          bb_batch** out);
+/* Adds a world with its bodies; returns its index in *out_world. */
+bb_status bb_batch_add_world(bb_batch* b, const bb_world_desc* desc,
+                             const bb_body_desc* bodies, uint32_t n,
+                             uint32_t* out_world);
+/* Steps every world nsteps times.  With use_graph != 0 the step sequence is
+ * captured once as a hipGraph and replayed. */
+bb_status bb_batch_step(bb_batch* b, uint32_t nsteps, void* stream,
+                        int use_graph);
+bb_status bb_batch_sync(bb_batch* b);
+bb_status bb_batch_read_world(bb_batch* b, uint32_t world, bb_body_state* out,
+                              uint32_t n, bb_order order);
+bb_status bb_batch_world_stats(bb_batch* b, uint32_t world,
+                               bb_step_stats* out);
+bb_status bb_batch_debug_tap(bb_batch* b, uint32_t world, bb_tap kind,
+                             void* dst, size_t cap, size_t* written);
+uint32_t bb_batch_world_count(const bb_batch* b);
+uint32_t bb_batch_body_count(const bb_batch* b);
+/* Block size used by the per-world kernels (autotuner hook); 0 = automatic. */
+bb_status bb_batch_set_block_size(bb_batch* b, uint32_t threads);
+void bb_batch_destroy(bb_batch* b);
+
+/* Library version string. */
+const char* bb_version(void);
+
+#ifdef __cplusplus
+} /* extern "C" */
+#endif
+
+#endif /* BB_CAPI_H */

@@ -1,0 +1,1979 @@
+// bb_oracle.cpp — TEST INFRASTRUCTURE ONLY: the CPU oracle.
+//
+// A sequential C++ restatement of the reference rigid-body step
+// (Jaisiero/beat-box, RigidBodyManager::simulate, src/rigid_body_manager.cpp:
+// 1606-1622, pass order :1284-1427).  Each function cites the reference
+// file:line it follows.  Only tests/, __graft_entry__.smoke() and bench.py's
+// cpu_baseline leg load this library, and only as the checker; the product
+// (beat-box_amd/) never links or calls it.
+//
+// Where the reference is nondeterministic (atomic append orders, linked lists,
+// a same-round colouring race) this restatement fixes one canonical order:
+//   * broad-phase pairs sorted by (row a, row b), a < b;
+//   * manifolds in pair order (scratch index = rank among contact pairs);
+//   * per-body adjacency in ascending manifold order (replaces the linked
+//     lists built by InterlockedExchange, collision_detection.slang:899);
+//   * the active list in ascending id (reference: unordered_map order,
+//     src/scene_manager.hpp:944-951);
+//   * island slots in ascending root order (reference: InterlockedAdd order,
+//     islands.slang:221);
+//   * edge colouring phase 2 commits at the colour picked in phase 1
+//     (equivalent to the reference except for 32-bit priority collisions);
+//   * AVBD Jones-Plassmann rounds and the support-depth relaxation read a
+//     snapshot taken at the start of the round/pass (reference reads values
+//     written in the same round: avbd.slang:880-891, :691-724).
+// The packed (island-sorted) manifold buffer of the reference is an atomic-
+// order permutation used only by the per-island solver path; here the packed
+// order is the scratch order (identity collision_map).
+//
+// Build: oracle/Makefile (g++ -O2 -ffp-contract=off -fno-fast-math).
+#include <algorithm>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "../include/bb_capi.h"
+#include "oracle_math.hpp"
+
+namespace oracle {
+
+static const uint32_t MAXU = 0xFFFFFFFFu;
+static const int MAX_CONTACTS = 8;  // MAX_CONTACT_POINT_COUNT, math.hpp:260
+static const uint32_t MAX_COLORS = 32;
+static const uint32_t GC_ROUNDS = 32;
+static const float SLEEP_LIN2 = 0.0144f, SLEEP_ANG2 = 0.0225f;  // shared.inl:789-790
+static const uint32_t SLEEP_STEPS = 30, VETO_BIT = 0x80000000u, TIMER_MASK = 0x7FFFFFFFu;
+static const float AVBD_BETA = 100000.0f, AVBD_GAMMA = 0.99f, AVBD_KMIN = 1.0f, AVBD_KMAX = 1000000000.0f;
+static const float AVBD_MARGIN = 0.0005f, AVBD_STICK_THRESH = 0.01f, AVBD_STICK_SLOP = 0.001f;
+static const uint32_t AVBD_ITERS = 10, AVBD_ROUNDS = 16, AVBD_COLORS = 32, AVBD_LAYERS = 12,
+                      AVBD_POST_STAB = 4;
+
+struct Body {  // RigidBody, include/shared.inl:335-476 (render-only fields dropped)
+  uint32_t id, flags, island_index, active_index, sleep_timer, shape;
+  f3 position;
+  quat rotation;
+  f3 minimum, maximum;
+  float radius;
+  float mass, inv_mass;
+  f3 velocity, omega, prev_velocity;
+  m3 inv_inertia;
+  float restitution, friction;
+};
+
+struct FP {
+  uint32_t in_ref, out_ref, in_inc, out_inc;
+};
+struct Contact {  // Contact, include/math.hpp:436-457
+  f3 position;
+  float penetration;
+  float normal_impulse;
+  float tangent_impulse[2];
+  float bias_factor, impulse_coefficient, mass_coefficient, normal_mass;
+  float tangent_mass[2];
+  f3 anchor_a, anchor_b;
+  uint32_t stick;
+  FP fp;
+  uint32_t warm_start;
+};
+struct Manifold {  // Manifold, include/math.hpp:459-479
+  uint32_t a, b;
+  f3 anchor_a, anchor_b;
+  int32_t key, error;
+  f3 normal;
+  f3 t[2];
+  int32_t count;
+  Contact c[MAX_CONTACTS];
+};
+
+struct Node {  // LBVHNode, include/shared.inl:867-897
+  f3 mn, mx;
+  int32_t left, right;
+  uint32_t prim;
+};
+
+struct AvbdState {  // AvbdBodyState, include/shared.inl:938-947
+  f3 pos_start;
+  quat rot_start;
+  f3 pos_tilde;
+  quat rot_tilde;
+  uint32_t depth;
+};
+
+struct World {
+  bb_world_desc desc;
+  bool skip_ws_once = false;
+  bool drop_prev = false;
+  std::vector<Body> rows;       // current rows (Morton order after a step)
+  std::vector<uint32_t> entries;  // id -> row
+  std::vector<uint32_t> active;   // active list: ids of DYNAMIC bodies, ascending
+  // previous step
+  std::vector<Body> prev_rows;
+  std::vector<uint32_t> prev_entries;
+  std::vector<Manifold> prev_manifolds;
+  uint32_t prev_count = 0;  // previous rigid_body_count
+  // taps of the last step
+  std::vector<uint64_t> tap_morton;
+  std::vector<std::pair<uint64_t, uint32_t>> tap_sorted;
+  std::vector<Node> tap_nodes;
+  std::vector<std::pair<uint32_t, uint32_t>> tap_pairs;
+  std::vector<Manifold> manifolds;
+  std::vector<uint32_t> tap_islands;
+  std::vector<uint32_t> edge_color;
+  std::vector<uint32_t> body_color;
+  bb_step_stats stats{};
+  uint64_t step_index = 0;
+};
+
+// ============================================================ body helpers
+static inline bool is_dyn(const Body& r) { return (r.flags & BB_BODY_DYNAMIC) != 0; }
+static inline bool awake_dyn(const Body& r) {
+  return (r.flags & BB_BODY_DYNAMIC) != 0 && (r.flags & BB_BODY_SLEEPING) == 0;
+}
+// RigidBody::get_rotation_matrix on the shader path returns transpose(R)
+// (include/shared.inl:367-375).
+static inline m3 rot_matrix(const Body& r) { return transpose(qto_matrix(r.rotation)); }
+// RigidBody::get_transformed_inertia (src/shaders/extensions.slang:184-188):
+// mul(Rt, mul(I, transpose(Rt))) with Rt = get_rotation_matrix().  Without the
+// quirk the shader convention is corrected to R I R^T.
+static inline m3 world_inv_inertia(const Body& r, bool quirk) {
+  m3 R = qto_matrix(r.rotation);
+  if (quirk) {
+    m3 Rt = transpose(R);
+    return mulmm(Rt, mulmm(r.inv_inertia, transpose(Rt)));
+  }
+  return mulmm(R, mulmm(r.inv_inertia, transpose(R)));
+}
+static inline f3 obj_to_world(const Body& r, f3 v) { return add(qrotate(r.rotation, v), r.position); }
+static inline f3 half_extent(const Body& r) { return muls(sub(r.maximum, r.minimum), 0.5f); }
+
+// RigidBody::get_AABB_from_OBB (src/shaders/extensions.slang:195-210); spheres
+// (extension) use centre +- radius.
+static void aabb_of(const Body& r, f3& mn, f3& mx) {
+  if (r.shape == BB_SHAPE_SPHERE) {
+    f3 rr = mk(r.radius, r.radius, r.radius);
+    mn = sub(r.position, rr);
+    mx = add(r.position, rr);
+    return;
+  }
+  mn = mk(3.402823466e+38f, 3.402823466e+38f, 3.402823466e+38f);
+  mx = mk(-3.402823466e+38f, -3.402823466e+38f, -3.402823466e+38f);
+  for (uint32_t i = 0; i < 8; ++i) {
+    f3 c;
+    c.x = (i & 1) == 0 ? r.minimum.x : r.maximum.x;
+    c.y = (i & 2) == 0 ? r.minimum.y : r.maximum.y;
+    c.z = (i & 4) == 0 ? r.minimum.z : r.maximum.z;
+    c = obj_to_world(r, c);
+    mn = minv(mn, c);
+    mx = maxv(mx, c);
+  }
+}
+
+// ================================================================== Morton
+// MortonCodeGenerator (src/shaders/simulation/LBVH.slang:4-32).
+static inline uint32_t expand10(uint32_t v) {
+  v = (v * 0x00010001u) & 0xFF0000FFu;
+  v = (v * 0x00000101u) & 0x0F00F00Fu;
+  v = (v * 0x00000011u) & 0xC30C30C3u;
+  v = (v * 0x00000005u) & 0x49249249u;
+  return v;
+}
+static inline uint32_t morton30(f3 p) {
+  float x = fminr(fmaxr(p.x * 1024.0f, 0.0f), 1023.0f);
+  float y = fminr(fmaxr(p.y * 1024.0f, 0.0f), 1023.0f);
+  float z = fminr(fmaxr(p.z * 1024.0f, 0.0f), 1023.0f);
+  return expand10((uint32_t)x) * 4 + expand10((uint32_t)y) * 2 + expand10((uint32_t)z);
+}
+// 63-bit extension: 21 bits per axis, same interleave order (x highest).
+static inline uint64_t expand21(uint64_t x) {
+  x &= 0x1fffffull;
+  x = (x | x << 32) & 0x1f00000000ffffull;
+  x = (x | x << 16) & 0x1f0000ff0000ffull;
+  x = (x | x << 8) & 0x100f00f00f00f00full;
+  x = (x | x << 4) & 0x10c30c30c30c30c3ull;
+  x = (x | x << 2) & 0x1249249249249249ull;
+  return x;
+}
+static inline uint64_t morton63(f3 p) {
+  float x = fminr(fmaxr(p.x * 2097152.0f, 0.0f), 2097151.0f);
+  float y = fminr(fmaxr(p.y * 2097152.0f, 0.0f), 2097151.0f);
+  float z = fminr(fmaxr(p.z * 2097152.0f, 0.0f), 2097151.0f);
+  return (expand21((uint64_t)x) << 2) | (expand21((uint64_t)y) << 1) | expand21((uint64_t)z);
+}
+
+// entry_generate_morton_codes (RB_sim.slang:75-111): p^ = (pos - (-64)) / 128.
+// 63-bit mode normalises to the AABB of this world's body centres.
+static void morton_codes(const World& w, const std::vector<Body>& rb, std::vector<uint64_t>& out) {
+  size_t n = rb.size();
+  out.resize(n);
+  if (w.desc.morton_bits == 30) {
+    const f3 lo = mk(-64.0f, -64.0f, -64.0f), hi = mk(64.0f, 64.0f, 64.0f);
+    f3 ext = sub(hi, lo);
+    for (size_t i = 0; i < n; ++i) {
+      f3 p = sub(rb[i].position, lo);
+      p = f3{p.x / ext.x, p.y / ext.y, p.z / ext.z};
+      out[i] = morton30(p);
+    }
+    return;
+  }
+  // world AABB of the centres through the order-preserving map (exact, order
+  // independent; -0 < +0)
+  uint32_t lo_u[3] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu}, hi_u[3] = {0u, 0u, 0u};
+  for (size_t i = 0; i < n; ++i) {
+    float c[3] = {rb[i].position.x, rb[i].position.y, rb[i].position.z};
+    for (int k = 0; k < 3; ++k) {
+      lo_u[k] = std::min(lo_u[k], order_map(c[k]));
+      hi_u[k] = std::max(hi_u[k], order_map(c[k]));
+    }
+  }
+  f3 lo = mk(order_unmap(lo_u[0]), order_unmap(lo_u[1]), order_unmap(lo_u[2]));
+  f3 hi = mk(order_unmap(hi_u[0]), order_unmap(hi_u[1]), order_unmap(hi_u[2]));
+  f3 ext = sub(hi, lo);
+  for (size_t i = 0; i < n; ++i) {
+    f3 d = sub(rb[i].position, lo);
+    f3 p;
+    p.x = ext.x > 0.0f ? d.x / ext.x : 0.0f;
+    p.y = ext.y > 0.0f ? d.y / ext.y : 0.0f;
+    p.z = ext.z > 0.0f ? d.z / ext.z : 0.0f;
+    out[i] = morton63(p);
+  }
+}
+
+// =================================================================== LBVH
+static inline int clz32(uint32_t v) { return v == 0 ? 32 : __builtin_clz(v); }
+static inline int clz64(uint64_t v) { return v == 0 ? 64 : __builtin_clzll(v); }
+
+struct Keys {
+  const std::vector<std::pair<uint64_t, uint32_t>>* s;
+  int bits;  // 30 or 63
+  int n;
+};
+// LBVH::delta (LBVH.slang:159-170): duplicates split by index.
+static inline int delta(const Keys& k, int i, uint64_t ci, int j) {
+  if (j < 0 || j > k.n - 1) return -1;
+  uint64_t cj = (*k.s)[j].first;
+  if (k.bits == 30) {
+    if (ci == cj) return 32 + clz32((uint32_t)(i ^ j));
+    return clz32((uint32_t)(ci ^ cj));
+  }
+  if (ci == cj) return 64 + clz32((uint32_t)(i ^ j));
+  return clz64(ci ^ cj);
+}
+// LBVH::determine_range (LBVH.slang:172-197)
+static void determine_range(const Keys& k, int index, int& first, int& last) {
+  uint64_t code = (*k.s)[index].first;
+  int dl = delta(k, index, code, index - 1);
+  int dr = delta(k, index, code, index + 1);
+  int d = (dr >= dl) ? 1 : -1;
+  int dmin = dl < dr ? dl : dr;
+  int lmax = 2;
+  while (delta(k, index, code, index + lmax * d) > dmin) lmax <<= 1;
+  int l = 0;
+  for (int t = lmax >> 1; t > 0; t >>= 1)
+    if (delta(k, index, code, index + (l + t) * d) > dmin) l += t;
+  int j = index + l * d;
+  first = index < j ? index : j;
+  last = index < j ? j : index;
+}
+// LBVH::find_split (LBVH.slang:200-221)
+static int find_split(const Keys& k, int first, int last) {
+  uint64_t fc = (*k.s)[first].first;
+  int common = delta(k, first, fc, last);
+  int split = first, step = last - first;
+  do {
+    step = (step + 1) >> 1;
+    int ns = split + step;
+    if (ns < last) {
+      if (delta(k, first, fc, ns) > common) split = ns;
+    }
+  } while (step > 1);
+  return split;
+}
+
+// entry_generate_hierarchy_linear_bvh + build/convert bounding boxes
+// (RB_sim.slang:206-349).  Bounds merge through the order-preserving map,
+// which makes the min/max exact and order independent.
+static void build_lbvh(const Keys& k, const std::vector<Body>& scratch, std::vector<Node>& nodes) {
+  int n = k.n;
+  nodes.assign(n > 0 ? 2 * n - 1 : 0, Node{});
+  if (n == 0) return;
+  std::vector<uint32_t> parent(2 * n - 1, 0);
+  int leaf_off = n - 1;
+  for (int i = 0; i < n; ++i) {
+    Node& L = nodes[leaf_off + i];
+    aabb_of(scratch[(*k.s)[i].second], L.mn, L.mx);
+    L.left = -1;
+    L.right = -1;
+    L.prim = (*k.s)[i].second;
+  }
+  for (int i = 0; i < n - 1; ++i) {
+    int first, last;
+    determine_range(k, i, first, last);
+    int split = find_split(k, first, last);
+    int ca = (split == first) ? leaf_off + split : split;
+    int cb = (split + 1 == last) ? leaf_off + split + 1 : split + 1;
+    nodes[i].left = ca;
+    nodes[i].right = cb;
+    nodes[i].prim = MAXU;
+    parent[ca] = i;
+    parent[cb] = i;
+  }
+  std::vector<uint32_t> bmin((size_t)(n - 1) * 3, 0xFFFFFFFFu), bmax((size_t)(n - 1) * 3, 0u);
+  for (int i = 0; i < n && n > 1; ++i) {
+    const Node& L = nodes[leaf_off + i];
+    uint32_t lmn[3] = {order_map(L.mn.x), order_map(L.mn.y), order_map(L.mn.z)};
+    uint32_t lmx[3] = {order_map(L.mx.x), order_map(L.mx.y), order_map(L.mx.z)};
+    uint32_t node = parent[leaf_off + i];
+    while (true) {
+      for (int c = 0; c < 3; ++c) {
+        bmin[node * 3 + c] = std::min(bmin[node * 3 + c], lmn[c]);
+        bmax[node * 3 + c] = std::max(bmax[node * 3 + c], lmx[c]);
+      }
+      if (node == 0) break;
+      node = parent[node];
+    }
+  }
+  for (int i = 0; i < n - 1; ++i) {
+    nodes[i].mn = mk(order_unmap(bmin[i * 3]), order_unmap(bmin[i * 3 + 1]), order_unmap(bmin[i * 3 + 2]));
+    nodes[i].mx = mk(order_unmap(bmax[i * 3]), order_unmap(bmax[i * 3 + 1]), order_unmap(bmax[i * 3 + 2]));
+  }
+}
+
+static inline bool overlap(f3 amn, f3 amx, f3 bmn, f3 bmx) {  // Aabb::overlap, math.hpp:213-217
+  return amn.x <= bmx.x && amx.x >= bmn.x && amn.y <= bmx.y && amx.y >= bmn.y && amn.z <= bmx.z &&
+         amx.z >= bmn.z;
+}
+
+// LBVH::broad_phase (LBVH.slang:223-286): stack traversal from the root,
+// pair (i, j) for every overlapping leaf with row j > i.
+static void broad_phase_query(const std::vector<Node>& nodes, uint32_t i, f3 qmn, f3 qmx,
+                              std::vector<std::pair<uint32_t, uint32_t>>& out) {
+  if (nodes.size() < 3) return;  // a single body has no internal root
+  uint32_t stack[256];
+  uint32_t sp = 0;
+  stack[sp++] = 0;
+  uint32_t node = 0;
+  do {
+    const Node& N = nodes[node];
+    const Node& cl = nodes[N.left];
+    const Node& cr = nodes[N.right];
+    bool ol = overlap(qmn, qmx, cl.mn, cl.mx);
+    bool orr = overlap(qmn, qmx, cr.mn, cr.mx);
+    bool ll = cl.prim != MAXU, rl = cr.prim != MAXU;
+    if (ol && ll && cl.prim != i && i < cl.prim) out.push_back({i, cl.prim});
+    if (orr && rl && cr.prim != i && i < cr.prim) out.push_back({i, cr.prim});
+    bool tl = ol && !ll, tr = orr && !rl;
+    if (!tl && !tr) {
+      sp--;
+      if (sp == 0) break;
+      node = stack[sp];
+    } else {
+      if (tl && tr) {
+        if (sp < 256) stack[sp++] = (uint32_t)N.right;
+      }
+      node = tl ? (uint32_t)N.left : (uint32_t)N.right;
+    }
+  } while (sp > 0);
+}
+
+// ======================================================= narrow phase (SAT)
+// Transform (include/math.hpp:383-422) + extension methods
+// (src/shaders/extensions.slang:214-259).
+struct Tx {
+  m3 R;  // rotation as stored (row-major)
+  f3 p;
+};
+static inline f3 tx_rotate(const Tx& t, f3 v) { return mulvm(v, t.R); }
+static inline f3 tx_rotate_T(const Tx& t, f3 v) { return mulvm(v, transpose(t.R)); }
+static inline f3 tx_transform(const Tx& t, f3 v) { return add(mulvm(v, t.R), t.p); }
+static inline f3 tx_transform_T(const Tx& t, f3 v) { return mulvm(sub(v, t.p), transpose(t.R)); }
+
+struct ClipV {
+  f3 v;
+  FP f;
+};
+
+enum Edge : uint32_t {  // OBBvsOBBCollision::Edge, collision_detection.slang:48-61
+  PXPY = 0, PXNY = 1, PXPZ = 2, PXNZ = 3, NXPY = 4, NXNY = 5, NXPZ = 6, NXNZ = 7,
+  PYPZ = 8, PYNZ = 9, NYPZ = 10, NYNZ = 11
+};
+
+// compute_incident_face (collision_detection.slang:124-264)
+static void incident_face(const Tx& itx, f3 e, f3 n, ClipV out[4]) {
+  n = neg(tx_rotate_T(itx, n));
+  f3 an = absv(n);
+  if (an.x > an.y && an.x > an.z) {
+    if (n.x > 0.0f) {
+      out[0].v = mk(e.x, e.y, -e.z); out[1].v = mk(e.x, e.y, e.z);
+      out[2].v = mk(e.x, -e.y, e.z); out[3].v = mk(e.x, -e.y, -e.z);
+      out[0].f.in_inc = PYNZ; out[0].f.out_inc = PXNY; out[1].f.in_inc = PXNY; out[1].f.out_inc = PYPZ;
+      out[2].f.in_inc = PYPZ; out[2].f.out_inc = NXNZ; out[3].f.in_inc = NXNZ; out[3].f.out_inc = PYNZ;
+    } else {
+      out[0].v = mk(-e.x, -e.y, e.z); out[1].v = mk(-e.x, e.y, e.z);
+      out[2].v = mk(-e.x, e.y, -e.z); out[3].v = mk(-e.x, -e.y, -e.z);
+      out[0].f.in_inc = NXNY; out[0].f.out_inc = NYNZ; out[1].f.in_inc = NYNZ; out[1].f.out_inc = PXNZ;
+      out[2].f.in_inc = PXNZ; out[2].f.out_inc = NYPZ; out[3].f.in_inc = NYPZ; out[3].f.out_inc = NXNY;
+    }
+  } else if (an.y > an.x && an.y > an.z) {
+    if (n.y > 0.0f) {
+      out[0].v = mk(-e.x, e.y, e.z); out[1].v = mk(e.x, e.y, e.z);
+      out[2].v = mk(e.x, e.y, -e.z); out[3].v = mk(-e.x, e.y, -e.z);
+      out[0].f.in_inc = PXNZ; out[0].f.out_inc = PXPY; out[1].f.in_inc = PXPY; out[1].f.out_inc = PXNY;
+      out[2].f.in_inc = PXNY; out[2].f.out_inc = PXPZ; out[3].f.in_inc = PXPZ; out[3].f.out_inc = PXNZ;
+    } else {
+      out[0].v = mk(e.x, -e.y, e.z); out[1].v = mk(-e.x, -e.y, e.z);
+      out[2].v = mk(-e.x, -e.y, -e.z); out[3].v = mk(e.x, -e.y, -e.z);
+      out[0].f.in_inc = NXNZ; out[0].f.out_inc = NXPY; out[1].f.in_inc = NXPY; out[1].f.out_inc = NXNY;
+      out[2].f.in_inc = NXNY; out[2].f.out_inc = NXPZ; out[3].f.in_inc = NXPZ; out[3].f.out_inc = NXNZ;
+    }
+  } else {
+    if (n.z > 0.0f) {
+      out[0].v = mk(-e.x, e.y, e.z); out[1].v = mk(-e.x, -e.y, e.z);
+      out[2].v = mk(e.x, -e.y, e.z); out[3].v = mk(e.x, e.y, e.z);
+      out[0].f.in_inc = PXPY; out[0].f.out_inc = NYNZ; out[1].f.in_inc = NYNZ; out[1].f.out_inc = NXPY;
+      out[2].f.in_inc = NXPY; out[2].f.out_inc = PYPZ; out[3].f.in_inc = PYPZ; out[3].f.out_inc = PXPY;
+    } else {
+      out[0].v = mk(e.x, -e.y, -e.z); out[1].v = mk(-e.x, -e.y, -e.z);
+      out[2].v = mk(-e.x, e.y, -e.z); out[3].v = mk(e.x, e.y, -e.z);
+      out[0].f.in_inc = PYNZ; out[0].f.out_inc = NXPZ; out[1].f.in_inc = NXPZ; out[1].f.out_inc = NYPZ;
+      out[2].f.in_inc = NYPZ; out[2].f.out_inc = PXPZ; out[3].f.in_inc = PXPZ; out[3].f.out_inc = PYNZ;
+    }
+  }
+  for (int i = 0; i < 4; ++i) {
+    out[i].v = tx_transform(itx, out[i].v);
+    out[i].f.in_ref = 0;
+    out[i].f.out_ref = 0;
+  }
+}
+
+// compute_reference_edges_and_basis (collision_detection.slang:266-336)
+static void reference_edges(f3 er, const Tx& rtx, f3 n, int axis, uint32_t ce[4], m3& basis, f3& e) {
+  n = tx_rotate_T(rtx, n);
+  if (axis > 2) axis -= 3;
+  const m3& r = rtx.R;
+  f3 r0 = row(r, 0), r1 = row(r, 1), r2 = row(r, 2);
+  if (axis == 0) {
+    if (n.x > 0.0f) {
+      ce[0] = PXNY; ce[1] = PYPZ; ce[2] = NXNZ; ce[3] = PYNZ;
+      e = mk(er.y, er.z, er.x); basis = rows(r1, r2, r0);
+    } else {
+      ce[0] = NYNZ; ce[1] = PXNZ; ce[2] = NYPZ; ce[3] = NXNY;
+      e = mk(er.z, er.y, er.x); basis = rows(r2, r1, neg(r0));
+    }
+  } else if (axis == 1) {
+    if (n.y > 0.0f) {
+      ce[0] = PXPY; ce[1] = PXNY; ce[2] = PXPZ; ce[3] = PXNZ;
+      e = mk(er.z, er.x, er.y); basis = rows(r2, r0, r1);
+    } else {
+      ce[0] = NXPY; ce[1] = NXNY; ce[2] = NXPZ; ce[3] = NXNZ;
+      e = mk(er.z, er.x, er.y); basis = rows(r2, neg(r0), neg(r1));
+    }
+  } else {
+    if (n.z > 0.0f) {
+      ce[0] = NYNZ; ce[1] = NXPY; ce[2] = PYPZ; ce[3] = PXPY;
+      e = mk(er.y, er.x, er.z); basis = rows(neg(r1), r0, r2);
+    } else {
+      ce[0] = NXPZ; ce[1] = NYPZ; ce[2] = PXPZ; ce[3] = PYNZ;
+      e = mk(er.y, er.x, er.z); basis = rows(neg(r1), neg(r0), neg(r2));
+    }
+  }
+}
+
+static const float CLIP_TH = 0.005f;
+static inline bool in_front(float v) { return v < 0.0f; }
+static inline bool behind(float v) { return v >= 0.0f; }
+static inline bool on_plane(float v) { return (v < CLIP_TH) && (v > -CLIP_TH); }
+
+// ortographic_clip (collision_detection.slang:352-388).  The output is capped
+// at MAX_CONTACT_POINT_COUNT entries (the reference writes past a fixed
+// 8-entry array in that case).
+static int ortho_clip(float sign, float e, int axis, uint32_t ce, const ClipV* in, int nin, ClipV* out) {
+  int nout = 0;
+  ClipV a = in[nin - 1];
+  for (int i = 0; i < nin; ++i) {
+    ClipV b = in[i];
+    float da = sign * comp(a.v, axis) - e;
+    float db = sign * comp(b.v, axis) - e;
+    if ((in_front(da) && in_front(db)) || on_plane(da) || on_plane(db)) {
+      if (nout < MAX_CONTACTS) out[nout++] = b;
+    } else if (in_front(da) && behind(db)) {
+      ClipV cv;
+      cv.f = b.f;
+      cv.v = add(a.v, muls(sub(b.v, a.v), da / (da - db)));
+      cv.f.out_ref = ce;
+      if (nout < MAX_CONTACTS) out[nout++] = cv;
+    } else if (behind(da) && in_front(db)) {
+      ClipV cv;
+      cv.f = a.f;
+      cv.v = add(a.v, muls(sub(b.v, a.v), da / (da - db)));
+      cv.f.in_ref = ce;
+      if (nout < MAX_CONTACTS) out[nout++] = cv;
+      if (nout < MAX_CONTACTS) out[nout++] = b;
+    }
+    a = b;
+  }
+  return nout;
+}
+
+// clip (collision_detection.slang:390-452)
+static int clip(const Tx& tx, f3 e, const uint32_t ce[4], const ClipV inc[4], ClipV* outv, float* depths) {
+  ClipV in[MAX_CONTACTS], out[MAX_CONTACTS];
+  for (int i = 0; i < 4; ++i) {
+    in[i].v = tx_transform_T(tx, inc[i].v);
+    in[i].f = inc[i].f;
+  }
+  int nout = ortho_clip(1.0f, e.x, 0, ce[0], in, 4, out);
+  if (nout == 0) return -1;
+  int nin = ortho_clip(1.0f, e.y, 1, ce[1], out, nout, in);
+  if (nin == 0) return -1;
+  nout = ortho_clip(-1.0f, e.x, 0, ce[2], in, nin, out);
+  if (nout == 0) return -1;
+  nin = ortho_clip(-1.0f, e.y, 1, ce[3], out, nout, in);
+  if (nin == 0) return -1;
+  nout = 0;
+  for (int i = 0; i < nin; ++i) {
+    float d = in[i].v.z - e.z;
+    if (d <= 0.0f) {
+      outv[nout].v = tx_transform(tx, in[i].v);
+      outv[nout].f = in[i].f;
+      depths[nout] = d;
+      ++nout;
+    }
+  }
+  return nout;
+}
+
+static inline float c_sign(float x) { return x < 0.0f ? -1.0f : 1.0f; }
+
+// support_edge (collision_detection.slang:459-500)
+static void support_edge(const Tx& tx, f3 e, f3 n, f3& ca, f3& cb) {
+  n = tx_rotate_T(tx, n);
+  f3 an = absv(n);
+  int axis;
+  if (an.x <= an.y && an.x <= an.z) axis = 0;
+  else if (an.y <= an.x && an.y <= an.z) axis = 1;
+  else axis = 2;
+  f3 v0 = mk(c_sign(n.x) * e.x, c_sign(n.y) * e.y, c_sign(n.z) * e.z);
+  f3 v1 = v0;
+  if (axis == 0) { v0.x = e.x; v1.x = -e.x; }
+  else if (axis == 1) { v0.y = e.y; v1.y = -e.y; }
+  else { v0.z = e.z; v1.z = -e.z; }
+  ca = tx_transform(tx, v0);
+  cb = tx_transform(tx, v1);
+}
+
+// edges_contact (collision_detection.slang:502-553)
+static void edges_contact(f3 PA, f3 QA, f3 PB, f3 QB, f3& CA, f3& CB) {
+  f3 DA = sub(QA, PA), DB = sub(QB, PB), r = sub(PA, PB);
+  float a = dot(DA, DA), e = dot(DB, DB), f = dot(DB, r), c = dot(DA, r), b = dot(DA, DB);
+  float denom = a * e - b * b;
+  if (denom == 0.0f) {
+    float t = dot(r, DA) / a;
+    t = clampf(t, 0.0f, 1.0f);
+    CA = add(PA, muls(DA, t));
+    f3 s = sub(CA, PB);
+    float u = dot(s, DB) / e;
+    u = clampf(u, 0.0f, 1.0f);
+    CB = add(PB, muls(DB, u));
+    return;
+  }
+  float TA = (b * f - c * e) / denom;
+  TA = clampf(TA, 0.0f, 1.0f);
+  float TB = (b * TA + f) / e;
+  if (TB < 0.0f) {
+    TB = 0.0f;
+    TA = clampf(-c / a, 0.0f, 1.0f);
+  } else if (TB > 1.0f) {
+    TB = 1.0f;
+    TA = clampf((b - c) / a, 0.0f, 1.0f);
+  }
+  CA = add(PA, muls(DA, TA));
+  CB = add(PB, muls(DB, TB));
+}
+
+static inline int edge_axis_enum(int n) { return 6 + n; }  // get_axis_enum, :107-120
+
+static inline void zero_contact(Contact& c) { std::memset(&c, 0, sizeof(Contact)); }
+
+// OBBvsOBBCollision::collision_detection (collision_detection.slang:568-877)
+static bool obb_obb(const Body& A, const Body& B, Manifold& m) {
+  m.count = 0;
+  m.key = -1;
+  m.error = 0;
+  Tx txA{rot_matrix(A), A.position}, txB{rot_matrix(B), B.position};
+  f3 eA = half_extent(A), eB = half_extent(B);
+  m3 C = mulmm(transpose(txA.R), txB.R);
+  m3 aC = absm(C);
+  bool parallel = false;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j)
+      if (aC.m[i][j] > 0.9998f) parallel = true;
+  f3 t = qrotate_inv(A.rotation, sub(B.position, A.position));
+  float a_max = -3.402823466e+38f, b_max = -3.402823466e+38f, e_max = -3.402823466e+38f;
+  int a_axis = -1, b_axis = -1, e_axis = -1;
+  f3 n_a = mk(0, 0, 0), n_b = mk(0, 0, 0), n_e = mk(0, 0, 0);
+  for (int i = 0; i < 3; ++i) {
+    float s = fabsr(comp(t, i)) - (comp(eA, i) + dot(col(aC, i), eB));
+    if (s > 0.0f) return false;
+    if (s > a_max) { a_max = s; a_axis = i; n_a = row(txA.R, i); }
+  }
+  for (int i = 0; i < 3; ++i) {
+    float proj = dot(t, row(C, i));
+    float s = fabsr(proj) - (comp(eB, i) + dot(row(aC, i), eA));
+    if (s > 0.0f) return false;
+    if (s > b_max) { b_max = s; b_axis = 3 + i; n_b = row(txB.R, i); }
+  }
+  if (!parallel) {
+    for (int i = 0; i < 3; ++i) {
+      for (int j = 0; j < 3; ++j) {
+        int i1 = (i + 1) % 3, i2 = (i + 2) % 3, j1 = (j + 1) % 3, j2 = (j + 2) % 3;
+        float rA = comp(eA, i1) * aC.m[j][i2] + comp(eA, i2) * aC.m[j][i1];
+        float rB = comp(eB, j1) * aC.m[j2][i] + comp(eB, j2) * aC.m[j1][i];
+        float s = fabsr(comp(t, i2) * C.m[j][i1] - comp(t, i1) * C.m[j][i2]) - (rA + rB);
+        f3 ax = mk(0, 0, 0);
+        setc(ax, i1, -C.m[j][i2]);
+        setc(ax, i2, C.m[j][i1]);
+        if (s > 0.0f) return false;
+        float l = 1.0f / length(ax);
+        s *= l;
+        if (s > e_max) { e_max = s; e_axis = edge_axis_enum(i * 3 + j); n_e = muls(ax, l); }
+      }
+    }
+  }
+  const float rel = 0.95f, abs_tol = 0.01f;
+  int axis;
+  float s_max;
+  f3 normal;
+  float face_max = fmaxr(a_max, b_max);
+  if (rel * e_max > face_max + abs_tol) { axis = e_axis; s_max = e_max; normal = n_e; }
+  else if (rel * b_max > a_max + abs_tol) { axis = b_axis; s_max = b_max; normal = n_b; }
+  else { axis = a_axis; s_max = a_max; normal = n_a; }
+  if (dot(normal, sub(B.position, A.position)) < 0.0f) normal = neg(normal);
+  if (axis == -1) return false;
+  if (axis <= 5) {
+    Tx rtx, itx;
+    f3 er, ei;
+    bool flip = false;
+    if (axis <= 2) { rtx = txA; itx = txB; er = eA; ei = eB; }
+    else { rtx = txB; itx = txA; er = eB; ei = eA; flip = true; normal = neg(normal); }
+    ClipV inc[4];
+    incident_face(itx, ei, normal, inc);
+    uint32_t ce[4];
+    m3 basis;
+    f3 e;
+    reference_edges(er, rtx, normal, axis, ce, basis, e);
+    ClipV outv[MAX_CONTACTS];
+    float depths[MAX_CONTACTS];
+    Tx tx{basis, rtx.p};
+    int nout = clip(tx, e, ce, inc, outv, depths);
+    if (nout < 0) { m.key = axis; m.count = 0; return false; }
+    if (nout > 0) {
+      m.count = nout;
+      m.normal = flip ? neg(normal) : normal;
+      for (int i = 0; i < nout; ++i) {
+        if (flip) {
+          std::swap(outv[i].f.in_ref, outv[i].f.in_inc);
+          std::swap(outv[i].f.out_ref, outv[i].f.out_inc);
+        }
+        zero_contact(m.c[i]);
+        m.c[i].fp = outv[i].f;
+        m.c[i].position = outv[i].v;
+        m.c[i].penetration = depths[i];
+      }
+    }
+  } else {
+    normal = tx_rotate(txA, normal);
+    if (dot(normal, sub(B.position, A.position)) < 0.0f) normal = neg(normal);
+    f3 PA, QA, PB, QB, CA, CB;
+    support_edge(txA, eA, normal, PA, QA);
+    support_edge(txB, eB, neg(normal), PB, QB);
+    edges_contact(PA, QA, PB, QB, CA, CB);
+    m.count = 1;
+    m.normal = normal;
+    zero_contact(m.c[0]);
+    m.c[0].position = muls(add(CA, CB), 0.5f);
+    m.c[0].penetration = s_max;
+    m.c[0].fp = FP{(uint32_t)axis, (uint32_t)axis, (uint32_t)axis, (uint32_t)axis};
+  }
+  m.key = axis;
+  return m.count > 0;
+}
+
+// ----------------------------------------------- sphere contacts (extension)
+// The reference has no spheres.  Closed forms: sphere-sphere along the centre
+// line; box-sphere through the closest point of the box (clamped local
+// coordinates) or, for a centre inside the box, the face of least
+// penetration.  One contact at the midpoint of the two surface points.
+static const int KEY_SPHERE_SPHERE = 20, KEY_BOX_SPHERE = 21;
+static bool sphere_sphere(const Body& A, const Body& B, Manifold& m) {
+  m.count = 0;
+  m.key = KEY_SPHERE_SPHERE;
+  m.error = 0;
+  f3 d = sub(B.position, A.position);
+  float dist = length(d);
+  float pen = dist - (A.radius + B.radius);
+  if (pen > 0.0f) return false;
+  f3 n = dist > 0.0f ? divs(d, dist) : mk(0.0f, 1.0f, 0.0f);
+  f3 pa = add(A.position, muls(n, A.radius));
+  f3 pb = sub(B.position, muls(n, B.radius));
+  m.count = 1;
+  m.normal = n;
+  zero_contact(m.c[0]);
+  m.c[0].position = muls(add(pa, pb), 0.5f);
+  m.c[0].penetration = pen;
+  uint32_t k = (uint32_t)KEY_SPHERE_SPHERE;
+  m.c[0].fp = FP{k, k, k, k};
+  return true;
+}
+// box = X, sphere = S; the returned normal points from X to S.
+static bool box_sphere_core(const Body& X, const Body& S, f3& normal, f3& pos, float& pen) {
+  f3 e = half_extent(X);
+  f3 local = qrotate_inv(X.rotation, sub(S.position, X.position));
+  f3 q = mk(clampf(local.x, -e.x, e.x), clampf(local.y, -e.y, e.y), clampf(local.z, -e.z, e.z));
+  f3 d = sub(local, q);
+  float dist2 = dot(d, d);
+  f3 nl, surf;
+  if (dist2 > 0.0f) {
+    float dist = std::sqrt(dist2);
+    pen = dist - S.radius;
+    if (pen > 0.0f) return false;
+    nl = divs(d, dist);
+    surf = q;
+  } else {
+    float dx = e.x - fabsr(local.x), dy = e.y - fabsr(local.y), dz = e.z - fabsr(local.z);
+    int ax = 0;
+    float dmin = dx;
+    if (dy < dmin) { dmin = dy; ax = 1; }
+    if (dz < dmin) { dmin = dz; ax = 2; }
+    nl = mk(0, 0, 0);
+    setc(nl, ax, comp(local, ax) < 0.0f ? -1.0f : 1.0f);
+    surf = local;
+    setc(surf, ax, comp(nl, ax) * comp(e, ax));
+    pen = -dmin - S.radius;
+  }
+  normal = qrotate(X.rotation, nl);
+  f3 surf_w = obj_to_world(X, surf);
+  f3 sph_w = sub(S.position, muls(normal, S.radius));
+  pos = muls(add(surf_w, sph_w), 0.5f);
+  return true;
+}
+static bool box_sphere(const Body& A, const Body& B, Manifold& m) {
+  m.count = 0;
+  m.key = KEY_BOX_SPHERE;
+  m.error = 0;
+  f3 n, p;
+  float pen;
+  bool a_is_box = A.shape == BB_SHAPE_BOX;
+  const Body& X = a_is_box ? A : B;
+  const Body& S = a_is_box ? B : A;
+  if (!box_sphere_core(X, S, n, p, pen)) return false;
+  m.count = 1;
+  m.normal = a_is_box ? n : neg(n);
+  zero_contact(m.c[0]);
+  m.c[0].position = p;
+  m.c[0].penetration = pen;
+  uint32_t k = (uint32_t)KEY_BOX_SPHERE;
+  m.c[0].fp = FP{k, k, k, k};
+  return true;
+}
+
+// Manifold::compute_basis (src/shaders/extensions.slang:262-272)
+static void compute_basis(Manifold& m) {
+  f3 n = m.normal;
+  if (fabsr(n.x) > fabsr(n.y)) m.t[0] = mk(n.z, 0.0f, -n.x);
+  else m.t[0] = mk(0.0f, -n.z, n.y);
+  m.t[0] = normalize(m.t[0]);
+  m.t[1] = cross(m.t[0], n);
+}
+
+// Manifold::update_from_old (src/shaders/extensions.slang:305-473); OBB
+// manifolds only (voxel keys 90-95 do not occur).
+static void update_from_old(Manifold& m, const Manifold& old, bool swapped) {
+  float sgn = swapped ? -1.0f : 1.0f;
+  bool cross_key = m.key != old.key || swapped;
+  if (cross_key) {
+    if (dot(m.normal, muls(old.normal, sgn)) <= 0.7f) return;
+  }
+  uint32_t used_old = 0, matched_new = 0;
+  for (int i = 0; !cross_key && i < m.count; ++i) {
+    for (int j = 0; j < old.count; ++j) {
+      if (used_old & (1u << j)) continue;
+      const Contact& oc = old.c[j];
+      f3 dp = sub(oc.position, m.c[i].position);
+      bool feat = oc.fp.in_inc == m.c[i].fp.in_inc && oc.fp.out_inc == m.c[i].fp.out_inc;
+      bool close = dot(dp, dp) < 0.1f * 0.1f;
+      if (feat && close) {
+        Contact& c = m.c[i];
+        c.normal_impulse = oc.normal_impulse;
+        c.tangent_impulse[0] = oc.tangent_impulse[0];
+        c.tangent_impulse[1] = oc.tangent_impulse[1];
+        c.mass_coefficient = oc.mass_coefficient;
+        c.tangent_mass[0] = oc.tangent_mass[0];
+        c.tangent_mass[1] = oc.tangent_mass[1];
+        c.stick = oc.stick;
+        if (oc.stick != 0) { c.anchor_a = oc.anchor_a; c.anchor_b = oc.anchor_b; }
+        f3 d2 = sub(c.position, oc.position);
+        if (dot(d2, d2) < 0.01f) c.position = oc.position;
+        c.warm_start = std::min<uint32_t>(255u, oc.warm_start + 1u);
+        used_old |= 1u << j;
+        matched_new |= 1u << i;
+        break;
+      }
+    }
+  }
+  const float radius2 = 0.1225f;
+  for (int round = 0; round < (m.count > 0 ? m.count : 0); ++round) {
+    float best = radius2;
+    int bi = -1, bj = -1;
+    for (int i = 0; i < m.count; ++i) {
+      if (matched_new & (1u << i)) continue;
+      for (int j = 0; j < old.count; ++j) {
+        if (used_old & (1u << j)) continue;
+        f3 dp = sub(m.c[i].position, old.c[j].position);
+        float d2 = dot(dp, dp);
+        if (d2 < best) { best = d2; bi = i; bj = j; }
+      }
+    }
+    if (bi < 0) break;
+    Contact& c = m.c[bi];
+    const Contact& oc = old.c[bj];
+    c.normal_impulse = oc.normal_impulse;
+    c.tangent_impulse[0] = oc.tangent_impulse[0] * sgn;
+    c.tangent_impulse[1] = oc.tangent_impulse[1];
+    c.mass_coefficient = oc.mass_coefficient;
+    c.tangent_mass[0] = oc.tangent_mass[0];
+    c.tangent_mass[1] = oc.tangent_mass[1];
+    c.stick = oc.stick;
+    if (oc.stick != 0) {
+      c.anchor_a = swapped ? oc.anchor_b : oc.anchor_a;
+      c.anchor_b = swapped ? oc.anchor_a : oc.anchor_b;
+    }
+    c.fp = oc.fp;
+    c.warm_start = std::min<uint32_t>(255u, oc.warm_start + 1u);
+    matched_new |= 1u << bi;
+    used_old |= 1u << bj;
+  }
+  float deep_new = 0.0f, deep_old = 0.0f;
+  for (int i = 0; i < (m.count > 0 ? m.count : 0); ++i) deep_new = fminr(deep_new, m.c[i].penetration);
+  for (int j = 0; j < (old.count > 0 ? old.count : 0); ++j) deep_old = fminr(deep_old, old.c[j].penetration);
+  m.error = (int32_t)clampf((deep_new - deep_old) * 10000.0f, -32000.0f, 32000.0f);
+}
+
+// ======================================================== PGS_SOFT solver
+// PGS_SOFT::pre_step / solve (src/shaders/simulation/solvers.slang:142-279)
+struct SolveCtx {
+  bool accum, friction, quirk_inertia;
+  float dt, inv_dt, hertz;
+};
+static void pgs_pre_step(Manifold& m, Body& A, Body& B, const SolveCtx& s) {
+  m3 Ia = world_inv_inertia(A, s.quirk_inertia), Ib = world_inv_inertia(B, s.quirk_inertia);
+  for (int i = 0; i < m.count; ++i) {
+    Contact& c = m.c[i];
+    f3 r1 = sub(c.position, m.anchor_a), r2 = sub(c.position, m.anchor_b);
+    f3 rn1 = cross(r1, m.normal), rn2 = cross(r2, m.normal);
+    float kn = A.inv_mass + B.inv_mass;
+    kn += dot(rn1, mulmv(Ia, rn1));
+    kn += dot(rn2, mulmv(Ib, rn2));
+    c.normal_mass = kn == 0.0f ? 0.0f : 1.0f / kn;
+    for (int j = 0; j < 2; ++j) {
+      f3 rt1 = cross(r1, m.t[j]), rt2 = cross(r2, m.t[j]);
+      float kt = A.inv_mass + B.inv_mass;
+      kt += dot(rt1, mulmv(Ia, rt1));
+      kt += dot(rt2, mulmv(Ib, rt2));
+      c.tangent_mass[j] = kt == 0.0f ? 0.0f : 1.0f / kt;
+    }
+    float ch = (A.inv_mass == 0.0f || B.inv_mass == 0.0f) ? 2.0f * s.hertz : s.hertz;
+    const float zeta = 10.0f;
+    const float omega = 2.0f * 3.14159265359f * ch;
+    const float cc = s.dt * omega * (2.0f * zeta + s.dt * omega);
+    c.bias_factor = omega / (2.0f * zeta + s.dt * omega);
+    c.impulse_coefficient = 1.0f / (1.0f + cc);
+    c.mass_coefficient = cc * c.impulse_coefficient;
+    if (s.accum) {
+      f3 P = muls(m.normal, c.normal_impulse);
+      if (s.friction) {
+        for (int j = 0; j < 2; ++j) P = add(P, muls(m.t[j], c.tangent_impulse[j]));
+      }
+      if (is_dyn(A)) {
+        A.velocity = sub(A.velocity, muls(P, A.inv_mass));
+        A.omega = sub(A.omega, mulmv(Ia, cross(r1, P)));
+      }
+      if (is_dyn(B)) {
+        B.velocity = add(B.velocity, muls(P, B.inv_mass));
+        B.omega = add(B.omega, mulmv(Ib, cross(r2, P)));
+      }
+    }
+  }
+}
+
+static void pgs_solve(Manifold& m, Body& A, Body& B, const SolveCtx& s, bool use_bias) {
+  m3 Ia = world_inv_inertia(A, s.quirk_inertia), Ib = world_inv_inertia(B, s.quirk_inertia);
+  bool da = is_dyn(A), db = is_dyn(B);
+  float fr = std::sqrt(A.friction * B.friction);
+  for (int k = 0; k < m.count; ++k) {
+    Contact& c = m.c[k];
+    f3 r1 = sub(c.position, m.anchor_a), r2 = sub(c.position, m.anchor_b);
+    f3 rv = sub(sub(add(B.velocity, cross(B.omega, r2)), A.velocity), cross(A.omega, r1));
+    if (s.friction) {
+      for (int j = 0; j < 2; ++j) {
+        float tv = dot(rv, m.t[j]);
+        float ti = c.tangent_mass[j] * -tv;
+        float maxf = fr * c.normal_impulse;
+        if (s.accum) {
+          float old = c.tangent_impulse[j];
+          c.tangent_impulse[j] = clampf(old + ti, -maxf, maxf);
+          ti = c.tangent_impulse[j] - old;
+        } else {
+          ti = clampf(ti, -maxf, maxf);
+        }
+        f3 tiv = muls(m.t[j], ti);
+        if (da) {
+          A.velocity = sub(A.velocity, muls(tiv, A.inv_mass));
+          A.omega = sub(A.omega, mulmv(Ia, cross(r1, tiv)));
+        }
+        if (db) {
+          B.velocity = add(B.velocity, muls(tiv, B.inv_mass));
+          B.omega = add(B.omega, mulmv(Ib, cross(r2, tiv)));
+        }
+      }
+      // both tangent rows use the relative velocity read before the friction
+      // loop (solvers.slang:207-214); it is re-read for the normal row below
+    }
+    rv = sub(sub(add(B.velocity, cross(B.omega, r2)), A.velocity), cross(A.omega, r1));
+    float nv = dot(rv, m.normal);
+    float bias = 0.0f, mass_scale = 1.0f, impulse_scale = 0.0f;
+    if (c.penetration > 0.0f) {
+      bias = c.penetration * s.inv_dt;
+    } else if (use_bias) {
+      bias = fmaxr(c.bias_factor * c.penetration, -4.0f);
+      mass_scale = c.mass_coefficient;
+      impulse_scale = c.impulse_coefficient;
+    }
+    float imp = -c.normal_mass * mass_scale * (nv + bias) - impulse_scale * c.normal_impulse;
+    if (s.accum) {
+      float old = c.normal_impulse;
+      c.normal_impulse = fmaxr(old + imp, 0.0f);
+      imp = c.normal_impulse - old;
+    } else {
+      imp = fmaxr(imp, 0.0f);
+    }
+    f3 niv = muls(m.normal, imp);
+    if (da) {
+      A.velocity = sub(A.velocity, muls(niv, A.inv_mass));
+      A.omega = sub(A.omega, mulmv(Ia, cross(r1, niv)));
+    }
+    if (db) {
+      B.velocity = add(B.velocity, muls(niv, B.inv_mass));
+      B.omega = add(B.omega, mulmv(Ib, cross(r2, niv)));
+    }
+  }
+}
+
+// ============================================ integrate / advect (extensions)
+// RigidBody::apply_angular_velocity (src/shaders/extensions.slang:148-166)
+static quat apply_angular_velocity(quat q, f3 omega, float dt) {
+  float angle = length(omega) * dt;
+  if (angle > 0.0001f) {
+    f3 axis = normalize(omega);
+    float half = angle / 2.0f;
+    float sh, ch;
+    sincos_cephes(half, &sh, &ch);
+    quat d{muls(axis, sh), ch};
+    return qnormalize(qmul(d, q));
+  }
+  f3 dr = muls(muls(omega, dt), 0.5f);
+  quat d{dr, 1.0f};
+  return qnormalize(qmul(d, q));
+}
+
+// RigidBody::advect (src/shaders/extensions.slang:127-139) with the damping
+// factor exp(-dt*0.1) evaluated by the caller.
+static void advect(Body& r, float dt, float gravity, float damp) {
+  if (r.flags & BB_BODY_GRAVITY) {
+    f3 impulse = muls(mk(0.0f, gravity, 0.0f), dt);
+    f3 lin = muls(impulse, r.inv_mass);
+    r.velocity = add(r.velocity, lin);
+    f3 rr = sub(r.position, r.position);
+    f3 torque = cross(rr, impulse);
+    f3 ang = mulvm(torque, r.inv_inertia);
+    r.omega = add(r.omega, ang);
+  }
+  r.velocity = muls(r.velocity, damp);
+  r.omega = muls(r.omega, damp);
+}
+
+// ============================================================== AVBD passes
+// src/shaders/simulation/avbd.slang
+static inline uint32_t avbd_priority(uint32_t body, uint32_t round) {
+  uint32_t x = body * 0x9e3779b9u + round * 0x85ebca6bu;
+  x ^= x >> 16; x *= 0x7feb352du; x ^= x >> 15; x *= 0x846ca68bu; x ^= x >> 16;
+  return ((x & 0xFFFu) << 20) | (body & 0xFFFFFu);
+}
+static inline f3 small_angle_delta(quat now, quat start) {
+  quat dq = qmul(now, quat{neg(start.v), start.w});
+  float sg = dq.w < 0.0f ? -1.0f : 1.0f;
+  return muls(dq.v, 2.0f * sg);
+}
+static inline f3 avbd_q_rotate(quat q, f3 u) {
+  return add(u, muls(cross(q.v, add(cross(q.v, u), muls(u, q.w))), 2.0f));
+}
+static m3 avbd_inverse3(const m3& a) {
+  f3 c0 = col(a, 0), c1 = col(a, 1), c2 = col(a, 2);
+  f3 r0 = cross(c1, c2), r1 = cross(c2, c0), r2 = cross(c0, c1);
+  float det = dot(c0, r0);
+  float inv = fabsr(det) > 1e-12f ? 1.0f / det : 0.0f;
+  m3 o;
+  o.m[0][0] = r0.x * inv; o.m[0][1] = r0.y * inv; o.m[0][2] = r0.z * inv;
+  o.m[1][0] = r1.x * inv; o.m[1][1] = r1.y * inv; o.m[1][2] = r1.z * inv;
+  o.m[2][0] = r2.x * inv; o.m[2][1] = r2.y * inv; o.m[2][2] = r2.z * inv;
+  return o;
+}
+// avbd_solve6 (avbd.slang:324-387)
+static void avbd_solve6(float H[6][6], float b[6], float x[6]) {
+  for (int jj = 0; jj < 6; ++jj) {
+    float d = H[jj][jj];
+    for (int kk = 0; kk < jj; ++kk) d -= H[jj][kk] * H[jj][kk] * H[kk][kk];
+    d = fmaxr(d, 1e-8f);
+    H[jj][jj] = d;
+    for (int ii = jj + 1; ii < 6; ++ii) {
+      float s = H[ii][jj];
+      for (int kk = 0; kk < jj; ++kk) s -= H[ii][kk] * H[jj][kk] * H[kk][kk];
+      H[ii][jj] = s / d;
+    }
+  }
+  for (int ii = 0; ii < 6; ++ii) {
+    float s = b[ii];
+    for (int kk = 0; kk < ii; ++kk) s -= H[ii][kk] * b[kk];
+    b[ii] = s;
+  }
+  for (int ii = 0; ii < 6; ++ii) b[ii] /= H[ii][ii];
+  for (int ii = 5; ii >= 0; --ii) {
+    float s = b[ii];
+    for (int kk = ii + 1; kk < 6; ++kk) s -= H[kk][ii] * x[kk];
+    x[ii] = s;
+  }
+}
+
+struct AvbdPair {
+  bool dyn1, dyn2;
+  f3 c1, c2;
+  quat q1, q2;
+};
+// avbd_contact_C (avbd.slang:232-258)
+static float avbd_contact_C(const Manifold& m, int j, const Body& r1, const Body& r2, const AvbdState& s1,
+                            const AvbdState& s2, bool dyn1, bool dyn2, float alpha, f3& rr1, f3& rr2) {
+  f3 n = m.normal, p = m.c[j].position;
+  f3 c1 = dyn1 ? s1.pos_start : r1.position;
+  f3 c2 = dyn2 ? s2.pos_start : r2.position;
+  rr1 = sub(p, c1);
+  rr2 = sub(p, c2);
+  float C0 = m.c[j].penetration + AVBD_MARGIN;
+  float C = C0 * (1.0f - alpha);
+  if (dyn1) {
+    f3 dp1 = sub(r1.position, s1.pos_start);
+    f3 dth1 = small_angle_delta(r1.rotation, s1.rot_start);
+    C -= dot(n, add(dp1, cross(dth1, rr1)));
+  }
+  if (dyn2) {
+    f3 dp2 = sub(r2.position, s2.pos_start);
+    f3 dth2 = small_angle_delta(r2.rotation, s2.rot_start);
+    C += dot(n, add(dp2, cross(dth2, rr2)));
+  }
+  return C;
+}
+// avbd_dir_C (avbd.slang:274-305)
+static float avbd_dir_C(f3 d, const Manifold& m, int j, const Body& r1, const Body& r2, const AvbdState& s1,
+                        const AvbdState& s2, bool dyn1, bool dyn2, float alpha, f3& ar1, f3& ar2, float& C0t) {
+  f3 c1 = dyn1 ? s1.pos_start : r1.position;
+  f3 c2 = dyn2 ? s2.pos_start : r2.position;
+  quat q1 = dyn1 ? s1.rot_start : r1.rotation;
+  quat q2 = dyn2 ? s2.rot_start : r2.rotation;
+  ar1 = avbd_q_rotate(q1, m.c[j].anchor_a);
+  ar2 = avbd_q_rotate(q2, m.c[j].anchor_b);
+  C0t = dot(d, sub(add(c2, ar2), add(c1, ar1)));
+  float c0e = signf(C0t) * fmaxr(0.0f, fabsr(C0t) - AVBD_STICK_SLOP);
+  float C = c0e * (1.0f - alpha);
+  if (dyn1) {
+    f3 dp1 = sub(r1.position, s1.pos_start);
+    f3 dth1 = small_angle_delta(r1.rotation, s1.rot_start);
+    C -= dot(d, add(dp1, cross(dth1, ar1)));
+  }
+  if (dyn2) {
+    f3 dp2 = sub(r2.position, s2.pos_start);
+    f3 dth2 = small_angle_delta(r2.rotation, s2.rot_start);
+    C += dot(d, add(dp2, cross(dth2, ar2)));
+  }
+  return C;
+}
+
+// ================================================================ the step
+struct StepCtx {
+  World* w;
+  uint32_t n, a;
+  uint32_t flags;
+  bool quirk_inertia, quirk_rows;
+  float dt, gravity, damp;
+};
+
+static void do_step(World& w) {
+  const uint32_t N = (uint32_t)w.rows.size();
+  bb_step_stats st{};
+  st.rigid_body_count = N;
+  st.active_rigid_body_count = (uint32_t)w.active.size();
+  const uint32_t A = st.active_rigid_body_count;
+  uint32_t flags = w.desc.flags;
+  if (w.skip_ws_once) { flags &= ~BB_SIM_WARM_STARTING; w.skip_ws_once = false; }
+  const bool quirk_inertia = (w.desc.quirks & BB_QUIRK_TRANSPOSED_WORLD_INERTIA) != 0;
+  const bool quirk_rows = (w.desc.quirks & BB_QUIRK_ACTIVE_DISPATCH_ROWS) != 0;
+  const float dt = w.desc.dt, gravity = w.desc.gravity;
+  const float damp = std::exp(-dt * 0.1f);
+  const uint32_t row_limit = quirk_rows ? std::min<uint32_t>(N, ((A + 31u) / 32u) * 32u) : N;
+  const uint32_t solver = w.desc.solver == BB_SOLVER_PGS ? (uint32_t)BB_SOLVER_PGS_SOFT : w.desc.solver;
+
+  // previous-step snapshot (the other parity)
+  if (w.drop_prev) { w.prev_manifolds.clear(); w.drop_prev = false; }
+  w.prev_rows = w.rows;
+  w.prev_entries = w.entries;
+  std::vector<Manifold> prev_manifolds = std::move(w.prev_manifolds);
+  w.prev_manifolds.clear();
+  const uint32_t prev_count = w.prev_count;
+
+  // copy bodies -> scratch (task_CRB, rigid_body_manager.cpp:354-364)
+  std::vector<Body> scratch = w.rows;
+  // Morton codes (RB_sim.slang:75-111); the debug flag clear of COLLIDING
+  for (auto& r : scratch) r.flags &= ~BB_BODY_COLLIDING;
+  std::vector<uint64_t> codes;
+  morton_codes(w, scratch, codes);
+  w.tap_morton = codes;
+  // stable LSD radix sort == stable sort by code (LBVH.slang:34-152)
+  std::vector<std::pair<uint64_t, uint32_t>> sorted(N);
+  for (uint32_t i = 0; i < N; ++i) sorted[i] = {codes[i], i};
+  std::stable_sort(sorted.begin(), sorted.end(),
+                   [](const std::pair<uint64_t, uint32_t>& x, const std::pair<uint64_t, uint32_t>& y) {
+                     return x.first < y.first;
+                   });
+  w.tap_sorted = sorted;
+  // LBVH (RB_sim.slang:206-349)
+  Keys keys{&sorted, (int)w.desc.morton_bits == 30 ? 30 : 63, (int)N};
+  std::vector<Node> nodes;
+  build_lbvh(keys, scratch, nodes);
+  // reorder (RB_sim.slang:351-386)
+  for (uint32_t i = 0; i < N; ++i) {
+    w.rows[i] = scratch[sorted[i].second];
+    w.entries[w.rows[i].id] = i;
+    nodes[(N - 1) + i].prim = i;
+  }
+  w.tap_nodes = nodes;
+  // reset body links (RB_sim.slang:388-411)
+  for (auto& r : w.rows) { r.active_index = MAXU; r.island_index = MAXU; }
+  std::vector<uint32_t> link(A);
+  for (uint32_t k = 0; k < A; ++k) {
+    w.rows[w.entries[w.active[k]]].active_index = k;
+    link[k] = k;
+  }
+  // broad phase (RB_sim.slang:413-437)
+  std::vector<std::pair<uint32_t, uint32_t>> pairs;
+  for (uint32_t i = 0; i < N; ++i) {
+    f3 mn, mx;
+    aabb_of(w.rows[i], mn, mx);
+    broad_phase_query(nodes, i, mn, mx, pairs);
+  }
+  std::sort(pairs.begin(), pairs.end());
+  w.tap_pairs = pairs;
+  st.broad_phase_collision_count = (uint32_t)pairs.size();
+
+  // narrow phase (collision_detection.slang:1514-1755)
+  std::vector<Manifold>& ms = w.manifolds;
+  ms.clear();
+  const bool warm = (flags & BB_SIM_WARM_STARTING) != 0;
+  uint32_t fresh = 0, pen_mm = 0;
+  // find_old_collision over the previous step's manifolds of the pair
+  // (collision_detection.slang:914-963): at most one manifold per pair exists,
+  // so exact / any-key / swapped reduce to a lookup by the unordered pair.
+  auto find_old = [&](uint32_t pa, uint32_t pb, bool& swapped) -> const Manifold* {
+    uint32_t lo = std::min(pa, pb), hi = std::max(pa, pb);
+    auto it = std::lower_bound(prev_manifolds.begin(), prev_manifolds.end(), lo,
+                               [&](const Manifold& m, uint32_t v) { return m.a < v || (m.a == v && m.b < hi); });
+    if (it == prev_manifolds.end() || it->a != lo || it->b != hi) return nullptr;
+    swapped = (pa != lo);
+    return &*it;
+  };
+  for (auto& pr : pairs) {
+    Body& r1 = w.rows[pr.first];
+    Body& r2 = w.rows[pr.second];
+    if (!is_dyn(r1) && !is_dyn(r2)) continue;
+    Manifold m;
+    std::memset(&m, 0, sizeof(m));
+    bool have;
+    if (r1.shape == BB_SHAPE_BOX && r2.shape == BB_SHAPE_BOX) {
+      have = obb_obb(r1, r2, m);
+      if (have)
+        for (int ci = 0; ci < m.count; ++ci) m.c[ci].penetration = fmaxr(m.c[ci].penetration, -0.25f);
+    } else if (r1.shape == BB_SHAPE_SPHERE && r2.shape == BB_SHAPE_SPHERE) {
+      have = sphere_sphere(r1, r2, m);
+    } else {
+      have = box_sphere(r1, r2, m);
+    }
+    if (!have) continue;
+    m.a = pr.first;
+    m.b = pr.second;
+    m.anchor_a = r1.position;
+    m.anchor_b = r2.position;
+    m.error = 0x7FFFFFFF;
+    compute_basis(m);
+    uint32_t pa = r1.id < w.prev_entries.size() ? w.prev_entries[r1.id] : MAXU;
+    uint32_t pb = r2.id < w.prev_entries.size() ? w.prev_entries[r2.id] : MAXU;
+    if (warm && pa < prev_count && pb < prev_count) {
+      bool swapped = false;
+      const Manifold* old = find_old(pa, pb, swapped);
+      if (old) update_from_old(m, *old, swapped);
+    }
+    for (int ci = 0; ci < m.count; ++ci) {
+      if (m.c[ci].warm_start == 0) ++fresh;
+      float cp = m.c[ci].penetration;
+      if (cp < 0.0f) pen_mm = std::max(pen_mm, (uint32_t)fminr(-cp * 1000.0f, 1.0e9f));
+    }
+    ms.push_back(m);
+    // IslandBuilder::link_bodies (islands.slang:153-213): root = min index
+    uint32_t f = is_dyn(r1) ? r1.active_index : MAXU;
+    uint32_t s = is_dyn(r2) ? r2.active_index : MAXU;
+    if (f < A && s < A) {
+      auto root = [&](uint32_t x) { while (link[x] != x) x = link[x]; return x; };
+      uint32_t ra = root(f), rb = root(s);
+      if (ra != rb) {
+        if (ra < rb) link[rb] = ra; else link[ra] = rb;
+      }
+      uint32_t lo = std::min(ra, rb);
+      link[f] = std::min(link[f], lo);
+      link[s] = std::min(link[s], lo);
+    }
+  }
+  const uint32_t M = (uint32_t)ms.size();
+  st.collision_count = M;
+  st.manifold_node_count = 2 * M;
+  st.dbg_fresh = fresh;
+  st.dbg_pen = pen_mm;
+  uint32_t ccount = 0;
+  for (auto& m : ms) ccount += (uint32_t)m.count;
+  st.contact_count = ccount;
+  // adjacency: per row, incident manifolds in ascending order
+  std::vector<std::vector<uint32_t>> adj(N);
+  for (uint32_t k = 0; k < M; ++k) {
+    adj[ms[k].a].push_back(k);
+    adj[ms[k].b].push_back(k);
+  }
+
+  // advect, PGS family only (RB_sim.slang:518-554)
+  if (solver != BB_SOLVER_AVBD) {
+    for (uint32_t i = 0; i < row_limit; ++i) {
+      Body& r = w.rows[i];
+      if (!awake_dyn(r)) continue;
+      if (flags & BB_SIM_ADVECTION) advect(r, dt, gravity, damp);
+    }
+  }
+
+  // islands (islands.slang:215-302): root = min active index
+  auto find_root = [&](uint32_t x) { while (link[x] != x) x = link[x]; return x; };
+  std::vector<uint32_t> roots(A);
+  for (uint32_t k = 0; k < A; ++k) roots[k] = find_root(k);
+  std::vector<uint32_t> slot(A, MAXU);
+  uint32_t island_count = 0;
+  for (uint32_t k = 0; k < A; ++k)
+    if (roots[k] == k) slot[k] = island_count++;
+  st.island_count = island_count;
+  w.tap_islands.assign(N, MAXU);
+  for (uint32_t k = 0; k < A; ++k) {
+    uint32_t row_i = w.entries[w.active[k]];
+    w.rows[row_i].island_index = slot[roots[k]];
+    w.tap_islands[row_i] = roots[k];
+  }
+  {
+    std::vector<uint8_t> has_m(island_count, 0);
+    for (auto& m : ms) {
+      uint32_t ia = w.rows[m.a].active_index, ib = w.rows[m.b].active_index;
+      uint32_t mn = std::min(ia, ib);
+      if (mn < A) has_m[slot[roots[mn]]] = 1;
+    }
+    uint32_t ci = 0;
+    for (auto h : has_m) ci += h;
+    st.contact_island_count = ci;
+  }
+
+  // sleeping (RB_sim.slang:556-737)
+  const bool sleep_on = (flags & BB_SIM_SLEEPING_ENABLED) != 0;
+  for (uint32_t i = 0; i < N; ++i) {
+    Body& r = w.rows[i];
+    if (!is_dyn(r)) continue;
+    if (!sleep_on) {
+      r.sleep_timer = 0;
+      r.flags &= ~BB_BODY_SLEEPING;
+      continue;
+    }
+    if (adj[i].empty() && (r.flags & BB_BODY_SLEEPING)) {
+      uint32_t t = r.sleep_timer & TIMER_MASK;
+      r.sleep_timer = t >= 10u ? t - 10u : 0u;
+      continue;
+    }
+    float lin2 = dot(r.velocity, r.velocity), ang2 = dot(r.omega, r.omega);
+    uint32_t timer = r.sleep_timer & TIMER_MASK;
+    bool calm = lin2 < SLEEP_LIN2 && ang2 < SLEEP_ANG2;
+    r.sleep_timer = calm ? std::min(timer + 1u, SLEEP_STEPS) : (timer >= 4u ? timer - 4u : 0u);
+  }
+  if (sleep_on) {
+    for (auto& m : ms) {
+      Body& ra = w.rows[m.a];
+      Body& rb = w.rows[m.b];
+      bool adyn = is_dyn(ra), bdyn = is_dyn(rb);
+      bool aloud = adyn && ((ra.sleep_timer & TIMER_MASK) < SLEEP_STEPS);
+      bool bloud = bdyn && ((rb.sleep_timer & TIMER_MASK) < SLEEP_STEPS);
+      bool deep = false;
+      bool stalled = m.error != 0x7FFFFFFF && std::abs(m.error) < 10;
+      if (!stalled)
+        for (int ci = 0; ci < m.count; ++ci)
+          if (m.c[ci].penetration < -0.05f) { deep = true; break; }
+      if ((aloud || deep) && bdyn) rb.sleep_timer |= VETO_BIT;
+      if ((bloud || deep) && adyn) ra.sleep_timer |= VETO_BIT;
+    }
+    uint32_t sleeping = 0;
+    for (uint32_t i = 0; i < N; ++i) {
+      Body& r = w.rows[i];
+      if (!is_dyn(r)) continue;
+      bool vetoed = (r.sleep_timer & VETO_BIT) != 0;
+      bool quiet = (r.sleep_timer & TIMER_MASK) >= SLEEP_STEPS;
+      bool sl = quiet && !vetoed;
+      bool was = (r.flags & BB_BODY_SLEEPING) != 0;
+      if (sl) {
+        if (!was) {
+          r.velocity = mk(0, 0, 0);
+          r.omega = mk(0, 0, 0);
+          r.flags |= BB_BODY_SLEEPING;
+        }
+        ++sleeping;
+      } else if (was) {
+        r.flags &= ~BB_BODY_SLEEPING;
+      }
+    }
+    st.sleeping_count = sleeping;
+  }
+
+  w.edge_color.assign(M, MAXU);
+  w.body_color.assign(N, MAXU);
+  SolveCtx sc;
+  sc.accum = (flags & BB_SIM_ACCUM_IMPULSE) != 0;
+  sc.friction = (flags & BB_SIM_FRICTION) != 0;
+  sc.quirk_inertia = quirk_inertia;
+  sc.dt = dt;
+  sc.inv_dt = dt > 0.0f ? 1.0f / dt : 0.0f;
+  sc.hertz = fminr(30.0f, sc.inv_dt * 0.3333333333333333f);
+
+  if (solver == BB_SOLVER_PGS_SOFT) {
+    // edge colouring (coloring.slang:44-292)
+    auto gc_hash = [](uint32_t x) {
+      x ^= x >> 16; x *= 0x7feb352du; x ^= x >> 15; x *= 0x846ca68bu; x ^= x >> 16;
+      return x;
+    };
+    std::vector<uint32_t> mask(N, 0), pick(M, 0), pair_id(M);
+    std::vector<uint32_t> owner((size_t)N * MAX_COLORS, 0);
+    for (uint32_t k = 0; k < M; ++k) {
+      uint32_t ida = w.rows[ms[k].a].id, idb = w.rows[ms[k].b].id;
+      uint32_t lo = std::min(ida, idb), hi = std::max(ida, idb);
+      pair_id[k] = gc_hash(lo * 0x9e3779b9u + hi * 0x85ebca6bu + (uint32_t)ms[k].key * 0xc2b2ae35u);
+    }
+    auto prio = [&](uint32_t pid, uint32_t round) {
+      uint32_t h = gc_hash(pid + round * 0x85ebca6bu) & 0xFFFu;
+      if (h == 0) h = 1;
+      return (h << 20) | (pid & 0xFFFFFu);
+    };
+    std::vector<uint32_t>& color = w.edge_color;
+    for (uint32_t round = 1; round <= GC_ROUNDS; ++round) {
+      std::fill(owner.begin(), owner.end(), 0u);
+      for (uint32_t k = 0; k < M; ++k) {
+        pick[k] = MAX_COLORS;
+        if (color[k] != MAXU) continue;
+        uint32_t a = ms[k].a, b = ms[k].b;
+        bool sa = !is_dyn(w.rows[a]), sb = !is_dyn(w.rows[b]);
+        uint32_t avail = ~((sa ? 0u : mask[a]) | (sb ? 0u : mask[b]));
+        if (avail == 0) continue;
+        uint32_t c = (uint32_t)__builtin_ctz(avail);
+        pick[k] = c;
+        uint32_t p = prio(pair_id[k], round);
+        if (!sa) owner[a * MAX_COLORS + c] = std::max(owner[a * MAX_COLORS + c], p);
+        if (!sb) owner[b * MAX_COLORS + c] = std::max(owner[b * MAX_COLORS + c], p);
+      }
+      for (uint32_t k = 0; k < M; ++k) {
+        if (color[k] != MAXU || pick[k] >= MAX_COLORS) continue;
+        uint32_t a = ms[k].a, b = ms[k].b, c = pick[k];
+        bool sa = !is_dyn(w.rows[a]), sb = !is_dyn(w.rows[b]);
+        uint32_t p = prio(pair_id[k], round);
+        bool wa = sa || owner[a * MAX_COLORS + c] == p;
+        bool wb = sb || owner[b * MAX_COLORS + c] == p;
+        if (wa && wb) {
+          color[k] = c;
+          if (!sa) mask[a] |= 1u << c;
+          if (!sb) mask[b] |= 1u << c;
+        }
+      }
+    }
+    // validator (coloring.slang:214-292)
+    uint32_t used = 0, viol = 0, ovf = 0;
+    std::fill(owner.begin(), owner.end(), 0u);
+    for (uint32_t k = 0; k < M; ++k) {
+      uint32_t c = color[k];
+      if (c == MAXU || c >= MAX_COLORS) { ++ovf; continue; }
+      used = std::max(used, c + 1);
+      uint32_t a = ms[k].a, b = ms[k].b;
+      if (is_dyn(w.rows[a]) && owner[a * MAX_COLORS + c]++ >= 1) ++viol;
+      if (is_dyn(w.rows[b]) && owner[b * MAX_COLORS + c]++ >= 1) ++viol;
+    }
+    st.graph_color_count = used;
+    st.graph_color_violations = viol;
+    st.graph_color_overflow = ovf;
+
+    // colour sweeps (RB_sim.slang:1408-1578, rigid_body_manager.cpp:1376-1389)
+    std::vector<std::vector<uint32_t>> bucket(MAX_COLORS + 1);
+    for (uint32_t k = 0; k < M; ++k) bucket[color[k] < MAX_COLORS ? color[k] : MAX_COLORS].push_back(k);
+    auto run = [&](int kind) {
+      for (uint32_t c = 0; c <= MAX_COLORS; ++c) {
+        for (uint32_t k : bucket[c]) {
+          Manifold& m = ms[k];
+          Body& A_ = w.rows[m.a];
+          Body& B_ = w.rows[m.b];
+          if (!awake_dyn(A_) && !awake_dyn(B_)) continue;
+          if (kind == 0) pgs_pre_step(m, A_, B_, sc);
+          else pgs_solve(m, A_, B_, sc, kind == 1);
+        }
+      }
+    };
+    run(0);
+    for (uint32_t it = 0; it < w.desc.iterations; ++it) run(1);
+    // integrate positions (RB_sim.slang:1287-1326)
+    for (uint32_t i = 0; i < row_limit; ++i) {
+      Body& r = w.rows[i];
+      if (!awake_dyn(r)) continue;
+      if (flags & BB_SIM_ADVECTION) {
+        r.position = add(r.position, muls(r.velocity, dt));
+        r.rotation = apply_angular_velocity(r.rotation, r.omega, dt);
+      }
+    }
+    for (uint32_t it = 0; it < w.desc.iterations; ++it) run(2);
+  } else {
+    // ---------------------------------------------------------------- AVBD
+    std::vector<uint32_t>& bcol = w.body_color;
+    for (uint32_t rd = 0; rd < AVBD_ROUNDS; ++rd) {
+      std::vector<uint32_t> snap = bcol;
+      for (uint32_t i = 0; i < N; ++i) {
+        if (snap[i] != MAXU) continue;
+        const Body& r = w.rows[i];
+        if (!is_dyn(r)) continue;
+        uint32_t my = avbd_priority(r.id, rd);
+        uint32_t used = 0;
+        bool lose = false;
+        for (uint32_t k : adj[i]) {
+          const Manifold& m = ms[k];
+          uint32_t p = m.a == i ? m.b : m.a;
+          if (p >= N || p == i) continue;
+          if (!is_dyn(w.rows[p])) continue;
+          uint32_t pc = snap[p];
+          if (pc == MAXU) {
+            if (avbd_priority(w.rows[p].id, rd) > my) { lose = true; break; }
+          } else if (pc < AVBD_COLORS) {
+            used |= 1u << pc;
+          }
+        }
+        if (lose) continue;
+        uint32_t avail = ~used;
+        if (avail == 0) continue;
+        uint32_t c = (uint32_t)__builtin_ctz(avail);
+        if (c >= AVBD_COLORS) continue;
+        bcol[i] = c;
+      }
+    }
+    {
+      uint32_t cnt = 0, viol = 0;
+      for (uint32_t i = 0; i < N; ++i) {
+        if (!is_dyn(w.rows[i])) continue;
+        bool has = false;
+        for (uint32_t k : adj[i]) {
+          const Manifold& m = ms[k];
+          uint32_t p = m.a == i ? m.b : m.a;
+          if (p >= N || p == i || !is_dyn(w.rows[p])) continue;
+          has = true;
+          if (bcol[i] != MAXU && bcol[p] == bcol[i] && p > i) ++viol;
+        }
+        if (bcol[i] == MAXU) { if (has) ++viol; continue; }
+        cnt = std::max(cnt, bcol[i] + 1);
+      }
+      st.avbd_color_count = cnt;
+      st.avbd_violations = viol;
+    }
+    std::vector<AvbdState> S(N);
+    // prepare (avbd.slang:70-128)
+    for (uint32_t i = 0; i < N; ++i) {
+      Body& r = w.rows[i];
+      if (!is_dyn(r) || (r.flags & BB_BODY_SLEEPING)) continue;
+      float om = length(r.omega);
+      if (om > 50.0f) r.omega = muls(r.omega, 50.0f / om);
+      float grav = (r.flags & BB_BODY_GRAVITY) ? gravity : 0.0f;
+      AvbdState& s = S[i];
+      s.pos_start = r.position;
+      s.rot_start = r.rotation;
+      s.pos_tilde = add(add(r.position, muls(r.velocity, dt)), muls(mk(0.0f, grav, 0.0f), dt * dt));
+      quat qt = apply_angular_velocity(r.rotation, r.omega, dt);
+      s.rot_tilde = qt;
+      float accel_w = 0.0f;
+      if (fabsr(grav) > 1e-6f) {
+        float accel_y = (r.velocity.y - r.prev_velocity.y) / dt;
+        accel_w = clampf(accel_y * signf(grav) / fabsr(grav), 0.0f, 1.0f);
+      }
+      r.position = add(add(r.position, muls(r.velocity, dt)), muls(mk(0.0f, grav, 0.0f), accel_w * dt * dt));
+      r.rotation = qt;
+    }
+    // warm start (avbd.slang:389-473)
+    uint32_t stick_count = 0;
+    auto dynawake = [&](const Body& r) { return is_dyn(r) && !(r.flags & BB_BODY_SLEEPING); };
+    for (uint32_t k = 0; k < M; ++k) {
+      Manifold& m = ms[k];
+      const Body& r1 = w.rows[m.a];
+      const Body& r2 = w.rows[m.b];
+      bool d1 = dynawake(r1), d2 = dynawake(r2);
+      f3 c1 = d1 ? S[m.a].pos_start : r1.position;
+      f3 c2 = d2 ? S[m.b].pos_start : r2.position;
+      quat q1c = d1 ? quat{neg(S[m.a].rot_start.v), S[m.a].rot_start.w} : quat{neg(r1.rotation.v), r1.rotation.w};
+      quat q2c = d2 ? quat{neg(S[m.b].rot_start.v), S[m.b].rot_start.w} : quat{neg(r2.rotation.v), r2.rotation.w};
+      for (int j = 0; j < m.count; ++j) {
+        Contact& c = m.c[j];
+        bool matched = c.warm_start != 0 && (flags & BB_SIM_WARM_STARTING);
+        if (!matched) {
+          c.normal_impulse = 0.0f;
+          c.mass_coefficient = AVBD_KMIN;
+          c.tangent_impulse[0] = c.tangent_impulse[1] = 0.0f;
+          c.tangent_mass[0] = c.tangent_mass[1] = AVBD_KMIN;
+          c.stick = 0;
+        } else {
+          c.mass_coefficient = clampf(c.mass_coefficient * AVBD_GAMMA, AVBD_KMIN, AVBD_KMAX);
+          c.normal_impulse = fminr(c.normal_impulse, 0.0f);
+          for (int t = 0; t < 2; ++t) c.tangent_mass[t] = clampf(c.tangent_mass[t] * AVBD_GAMMA, AVBD_KMIN, AVBD_KMAX);
+        }
+        bool sticking = matched && c.stick != 0;
+        if (!sticking) {
+          c.anchor_a = avbd_q_rotate(q1c, sub(c.position, c1));
+          c.anchor_b = avbd_q_rotate(q2c, sub(c.position, c2));
+        } else {
+          ++stick_count;
+        }
+      }
+    }
+    st.avbd_stick_count = stick_count;
+    // support depth (avbd.slang:667-724), pass-snapshot semantics
+    for (uint32_t i = 0; i < N; ++i) {
+      const Body& r = w.rows[i];
+      bool grounded = !is_dyn(r) || (r.flags & BB_BODY_SLEEPING);
+      S[i].depth = grounded ? 0u : MAXU;
+    }
+    for (uint32_t pass = 0; pass < AVBD_LAYERS; ++pass) {
+      std::vector<uint32_t> snap(N);
+      for (uint32_t i = 0; i < N; ++i) snap[i] = S[i].depth;
+      for (uint32_t k = 0; k < M; ++k) {
+        const Manifold& m = ms[k];
+        if (m.count <= 0) continue;
+        uint32_t d1 = snap[m.a], d2 = snap[m.b];
+        if (d2 != MAXU && d2 + 1u < d1) S[m.a].depth = std::min(S[m.a].depth, d2 + 1u);
+        if (d1 != MAXU && d1 + 1u < d2) S[m.b].depth = std::min(S[m.b].depth, d1 + 1u);
+      }
+    }
+    // primal (avbd.slang:475-658)
+    auto primal = [&](uint32_t color, float alpha, uint32_t ps_depth) {
+      const float inv_dt2 = dt > 0.0f ? 1.0f / (dt * dt) : 0.0f;
+      for (uint32_t i = 0; i < N; ++i) {
+        if (bcol[i] != color) continue;
+        Body& r = w.rows[i];
+        if (!is_dyn(r) || (r.flags & BB_BODY_SLEEPING)) continue;
+        AvbdState& s = S[i];
+        if (ps_depth != MAXU) {
+          uint32_t layer = std::min(s.depth, AVBD_LAYERS - 1u);
+          if (layer != ps_depth) continue;
+        }
+        float m_lin = r.mass * inv_dt2;
+        m3 Iw = avbd_inverse3(world_inv_inertia(r, quirk_inertia));
+        float H[6][6], g[6];
+        for (int a = 0; a < 6; ++a) { g[a] = 0.0f; for (int b = 0; b < 6; ++b) H[a][b] = 0.0f; }
+        H[0][0] = m_lin; H[1][1] = m_lin; H[2][2] = m_lin;
+        for (int a = 0; a < 3; ++a)
+          for (int b = 0; b < 3; ++b) H[3 + a][3 + b] = Iw.m[a][b] * inv_dt2;
+        f3 dpos = sub(r.position, s.pos_tilde);
+        f3 dth = small_angle_delta(r.rotation, s.rot_tilde);
+        g[0] = m_lin * dpos.x; g[1] = m_lin * dpos.y; g[2] = m_lin * dpos.z;
+        f3 ga = muls(mulmv(Iw, dth), inv_dt2);
+        g[3] = ga.x; g[4] = ga.y; g[5] = ga.z;
+        for (uint32_t k : adj[i]) {
+          Manifold& m = ms[k];
+          uint32_t b1 = m.a, b2 = m.b;
+          if (b1 >= N || b2 >= N || b1 == b2) continue;
+          bool i1 = (b1 == i);
+          const Body& r1 = w.rows[b1];
+          const Body& r2 = w.rows[b2];
+          bool d1 = dynawake(r1), d2 = dynawake(r2);
+          const AvbdState& s1 = S[b1];
+          const AvbdState& s2 = S[b2];
+          float mu = std::sqrt(fmaxr(r1.friction, 0.0f) * fmaxr(r2.friction, 0.0f));
+          for (int j = 0; j < m.count; ++j) {
+            f3 rr1, rr2;
+            float C = avbd_contact_C(m, j, r1, r2, s1, s2, d1, d2, alpha, rr1, rr2);
+            float lam = m.c[j].normal_impulse;
+            float kk = m.c[j].mass_coefficient;
+            float f = fminr(kk * C + lam, 0.0f);
+            f3 n = m.normal;
+            f3 Jl = i1 ? neg(n) : n;
+            f3 Ja = i1 ? neg(cross(rr1, n)) : cross(rr2, n);
+            g[0] += f * Jl.x; g[1] += f * Jl.y; g[2] += f * Jl.z;
+            g[3] += f * Ja.x; g[4] += f * Ja.y; g[5] += f * Ja.z;
+            float J[6] = {Jl.x, Jl.y, Jl.z, Ja.x, Ja.y, Ja.z};
+            for (int a = 0; a < 6; ++a)
+              for (int b = 0; b < 6; ++b) H[a][b] += kk * J[a] * J[b];
+            float bound = mu * fabsr(lam);
+            for (int t = 0; t < 2; ++t) {
+              f3 td = m.t[t];
+              f3 ar1, ar2;
+              float C0t;
+              float Ct = avbd_dir_C(td, m, j, r1, r2, s1, s2, d1, d2, alpha, ar1, ar2, C0t);
+              float lt = m.c[j].tangent_impulse[t];
+              float kt = m.c[j].tangent_mass[t];
+              float ft = clampf(kt * Ct + lt, -bound, bound);
+              f3 Jlt = i1 ? neg(td) : td;
+              f3 Jat = i1 ? neg(cross(ar1, td)) : cross(ar2, td);
+              g[0] += ft * Jlt.x; g[1] += ft * Jlt.y; g[2] += ft * Jlt.z;
+              g[3] += ft * Jat.x; g[4] += ft * Jat.y; g[5] += ft * Jat.z;
+              float Jt[6] = {Jlt.x, Jlt.y, Jlt.z, Jat.x, Jat.y, Jat.z};
+              for (int a = 0; a < 6; ++a)
+                for (int b = 0; b < 6; ++b) H[a][b] += kt * Jt[a] * Jt[b];
+            }
+          }
+        }
+        float rhs[6], dx[6];
+        for (int a = 0; a < 6; ++a) rhs[a] = -g[a];
+        avbd_solve6(H, rhs, dx);
+        r.position = add(r.position, mk(dx[0], dx[1], dx[2]));
+        f3 half = muls(mk(dx[3], dx[4], dx[5]), 0.5f);
+        r.rotation = qnormalize(qmul(quat{half, 1.0f}, r.rotation));
+      }
+    };
+    auto dual = [&]() {  // avbd.slang:726-808
+      for (uint32_t k = 0; k < M; ++k) {
+        Manifold& m = ms[k];
+        const Body& r1 = w.rows[m.a];
+        const Body& r2 = w.rows[m.b];
+        bool d1 = dynawake(r1), d2 = dynawake(r2);
+        if (!d1 && !d2) continue;
+        float mu = std::sqrt(fmaxr(r1.friction, 0.0f) * fmaxr(r2.friction, 0.0f));
+        for (int j = 0; j < m.count; ++j) {
+          f3 rr1, rr2;
+          float C = avbd_contact_C(m, j, r1, r2, S[m.a], S[m.b], d1, d2, 1.0f, rr1, rr2);
+          float lam = m.c[j].normal_impulse, kk = m.c[j].mass_coefficient;
+          float nl = fminr(kk * C + lam, 0.0f);
+          m.c[j].normal_impulse = nl;
+          if (nl < 0.0f) m.c[j].mass_coefficient = fminr(kk + AVBD_BETA * fabsr(C), AVBD_KMAX);
+          float bound = mu * fabsr(lam);
+          bool stick_ok = true;
+          float drift2 = 0.0f;
+          for (int t = 0; t < 2; ++t) {
+            f3 ar1, ar2;
+            float C0t;
+            float Ct = avbd_dir_C(m.t[t], m, j, r1, r2, S[m.a], S[m.b], d1, d2, 1.0f, ar1, ar2, C0t);
+            float lt = m.c[j].tangent_impulse[t], kt = m.c[j].tangent_mass[t];
+            float nlt = clampf(kt * Ct + lt, -bound, bound);
+            m.c[j].tangent_impulse[t] = nlt;
+            if (nlt > -bound && nlt < bound) m.c[j].tangent_mass[t] = fminr(kt + AVBD_BETA * fabsr(Ct), AVBD_KMAX);
+            else stick_ok = false;
+            drift2 += C0t * C0t;
+          }
+          m.c[j].stick = (stick_ok && drift2 < AVBD_STICK_THRESH * AVBD_STICK_THRESH) ? 1u : 0u;
+        }
+      }
+    };
+    for (uint32_t it = 0; it < AVBD_ITERS; ++it) {
+      for (uint32_t c = 0; c < AVBD_COLORS; ++c) primal(c, 1.0f, MAXU);
+      dual();
+    }
+    // finalize (avbd.slang:130-202)
+    const float inv_dt = dt > 0.0f ? 1.0f / dt : 0.0f;
+    for (uint32_t i = 0; i < N; ++i) {
+      Body& r = w.rows[i];
+      if (!is_dyn(r) || (r.flags & BB_BODY_SLEEPING)) continue;
+      AvbdState& s = S[i];
+      r.prev_velocity = r.velocity;
+      r.velocity = muls(sub(r.position, s.pos_start), inv_dt);
+      quat dq = qmul(r.rotation, quat{neg(s.rot_start.v), s.rot_start.w});
+      float sg = dq.w < 0.0f ? -1.0f : 1.0f;
+      r.omega = muls(dq.v, 2.0f * sg * inv_dt);
+      float v2 = dot(r.velocity, r.velocity), w2 = dot(r.omega, r.omega);
+      if (v2 < 0.09f && w2 < 0.16f && adj[i].size() >= 3) {
+        r.velocity = muls(r.velocity, 0.9f);
+        r.omega = muls(r.omega, 0.9f);
+      }
+    }
+    // post-stabilisation cascade + plain sweeps (rigid_body_manager.cpp:1401-1423)
+    for (uint32_t d = 0; d < AVBD_LAYERS; ++d)
+      for (uint32_t c = 0; c < AVBD_COLORS; ++c) primal(c, 0.0f, d);
+    for (uint32_t ps = 1; ps < AVBD_POST_STAB; ++ps)
+      for (uint32_t c = 0; c < AVBD_COLORS; ++c) primal(c, 0.0f, MAXU);
+  }
+
+  // end of step (RB_sim.slang:1655-1715)
+  uint32_t idsum = 0, maxv = 0;
+  for (uint32_t i = 0; i < N; ++i) {
+    const Body& r = w.rows[i];
+    idsum += r.id;
+    if (is_dyn(r)) {
+      float v2 = dot(r.velocity, r.velocity);
+      maxv = std::max(maxv, (uint32_t)fminr(std::sqrt(v2) * 1000.0f, 1.0e9f));
+    }
+  }
+  st.dbg_id_sum = idsum;
+  st.dbg_maxv = maxv;
+  st.step_index = w.step_index++;
+  w.stats = st;
+  w.prev_manifolds = ms;
+  w.prev_count = N;
+}
+
+}  // namespace oracle
+
+// ================================================================== C API
+using namespace oracle;
+
+extern "C" {
+
+void* oracle_create(const bb_world_desc* d) {
+  World* w = new World();
+  w->desc = *d;
+  if (w->desc.morton_bits != 63) w->desc.morton_bits = 30;
+  return w;
+}
+
+void oracle_destroy(void* h) { delete (World*)h; }
+
+// SceneManager::load_scene body rules (src/scene_manager.hpp:850-887) and
+// cuboid_get_inverse_intertia (include/math.hpp:106-130).
+int oracle_add_bodies(void* h, const bb_body_desc* b, uint32_t n) {
+  World* w = (World*)h;
+  for (uint32_t i = 0; i < n; ++i) {
+    Body r;
+    std::memset(&r, 0, sizeof(r));
+    r.id = (uint32_t)w->rows.size();
+    r.flags = b[i].flags & (BB_BODY_DYNAMIC | BB_BODY_GRAVITY);
+    r.island_index = MAXU;
+    r.active_index = MAXU;
+    r.shape = b[i].shape;
+    r.position = mk(b[i].position[0], b[i].position[1], b[i].position[2]);
+    r.rotation = quat{mk(b[i].rotation[0], b[i].rotation[1], b[i].rotation[2]), b[i].rotation[3]};
+    r.radius = b[i].radius;
+    if (r.shape == BB_SHAPE_SPHERE) {
+      r.minimum = mk(-r.radius, -r.radius, -r.radius);
+      r.maximum = mk(r.radius, r.radius, r.radius);
+    } else {
+      r.minimum = mk(b[i].minimum[0], b[i].minimum[1], b[i].minimum[2]);
+      r.maximum = mk(b[i].maximum[0], b[i].maximum[1], b[i].maximum[2]);
+    }
+    r.mass = b[i].mass;
+    r.inv_mass = r.mass == 0.0f ? 0.0f : 1.0f / r.mass;
+    r.velocity = mk(b[i].velocity[0], b[i].velocity[1], b[i].velocity[2]);
+    r.omega = mk(b[i].omega[0], b[i].omega[1], b[i].omega[2]);
+    r.restitution = b[i].restitution;
+    r.friction = b[i].friction;
+    float m_in = (w->desc.quirks & BB_QUIRK_INERTIA_FROM_INV_MASS) ? r.inv_mass : r.mass;
+    std::memset(&r.inv_inertia, 0, sizeof(m3));
+    if (m_in != 0.0f) {
+      if (r.shape == BB_SHAPE_SPHERE) {
+        float I = (2.0f / 5.0f) * m_in * (r.radius * r.radius);
+        float inv = 1.0f / I;
+        r.inv_inertia.m[0][0] = inv; r.inv_inertia.m[1][1] = inv; r.inv_inertia.m[2][2] = inv;
+      } else {
+        f3 size = sub(r.maximum, r.minimum);
+        float ww = size.x, hh = size.y, dd = size.z;
+        float Ixx = (1.0f / 12.0f) * m_in * (hh * hh + dd * dd);
+        float Iyy = (1.0f / 12.0f) * m_in * (ww * ww + dd * dd);
+        float Izz = (1.0f / 12.0f) * m_in * (ww * ww + hh * hh);
+        r.inv_inertia.m[0][0] = 1.0f / Ixx; r.inv_inertia.m[1][1] = 1.0f / Iyy; r.inv_inertia.m[2][2] = 1.0f / Izz;
+      }
+    }
+    w->rows.push_back(r);
+    w->entries.push_back((uint32_t)w->rows.size() - 1);
+    if (r.flags & BB_BODY_DYNAMIC) w->active.push_back(r.id);
+  }
+  w->prev_count = (uint32_t)w->rows.size();
+  return 0;
+}
+
+int oracle_step(void* h, uint32_t nsteps) {
+  World* w = (World*)h;
+  for (uint32_t s = 0; s < nsteps; ++s) do_step(*w);
+  return 0;
+}
+
+void oracle_set_flags(void* h, uint32_t flags) { ((World*)h)->desc.flags = flags; }
+void oracle_set_solver(void* h, uint32_t solver) {
+  World* w = (World*)h;
+  w->desc.solver = solver;
+  w->drop_prev = true;
+}
+void oracle_set_iterations(void* h, uint32_t it) { ((World*)h)->desc.iterations = it; }
+void oracle_skip_warm_start_once(void* h) { ((World*)h)->skip_ws_once = true; }
+
+static void fill_state(const Body& r, uint32_t row, bb_body_state& o) {
+  o.id = r.id;
+  o.flags = r.flags;
+  o.island_index = r.island_index;
+  o.row = row;
+  o.position[0] = r.position.x; o.position[1] = r.position.y; o.position[2] = r.position.z;
+  o.rotation[0] = r.rotation.v.x; o.rotation[1] = r.rotation.v.y; o.rotation[2] = r.rotation.v.z;
+  o.rotation[3] = r.rotation.w;
+  o.velocity[0] = r.velocity.x; o.velocity[1] = r.velocity.y; o.velocity[2] = r.velocity.z;
+  o.omega[0] = r.omega.x; o.omega[1] = r.omega.y; o.omega[2] = r.omega.z;
+}
+
+int oracle_read_bodies(void* h, bb_body_state* out, uint32_t n, int order) {
+  World* w = (World*)h;
+  uint32_t N = (uint32_t)w->rows.size();
+  if (n < N) return BB_ERR_INVALID;
+  for (uint32_t k = 0; k < N; ++k) {
+    uint32_t row = order == BB_ORDER_BY_ID ? w->entries[k] : k;
+    fill_state(w->rows[row], row, out[k]);
+  }
+  return 0;
+}
+
+int oracle_read_stats(void* h, bb_step_stats* out) {
+  *out = ((World*)h)->stats;
+  return 0;
+}
+
+static void manifold_tap(const Manifold& m, uint32_t color, bb_manifold_tap& t) {
+  std::memset(&t, 0, sizeof(t));
+  t.obb1_index = m.a; t.obb2_index = m.b; t.key = m.key; t.error = m.error; t.contact_count = m.count;
+  t.color = color;
+  t.normal[0] = m.normal.x; t.normal[1] = m.normal.y; t.normal[2] = m.normal.z;
+  for (int k = 0; k < 2; ++k) { t.tangent[k][0] = m.t[k].x; t.tangent[k][1] = m.t[k].y; t.tangent[k][2] = m.t[k].z; }
+  t.anchor_a[0] = m.anchor_a.x; t.anchor_a[1] = m.anchor_a.y; t.anchor_a[2] = m.anchor_a.z;
+  t.anchor_b[0] = m.anchor_b.x; t.anchor_b[1] = m.anchor_b.y; t.anchor_b[2] = m.anchor_b.z;
+  for (int i = 0; i < m.count && i < MAX_CONTACTS; ++i) {
+    const Contact& c = m.c[i];
+    bb_contact_tap& o = t.contacts[i];
+    o.position[0] = c.position.x; o.position[1] = c.position.y; o.position[2] = c.position.z;
+    o.penetration = c.penetration;
+    o.normal_impulse = c.normal_impulse;
+    o.tangent_impulse[0] = c.tangent_impulse[0]; o.tangent_impulse[1] = c.tangent_impulse[1];
+    o.mass_coefficient = c.mass_coefficient;
+    o.tangent_mass[0] = c.tangent_mass[0]; o.tangent_mass[1] = c.tangent_mass[1];
+    o.anchor_a[0] = c.anchor_a.x; o.anchor_a[1] = c.anchor_a.y; o.anchor_a[2] = c.anchor_a.z;
+    o.anchor_b[0] = c.anchor_b.x; o.anchor_b[1] = c.anchor_b.y; o.anchor_b[2] = c.anchor_b.z;
+    o.stick = c.stick;
+    o.fp[0] = c.fp.in_ref; o.fp[1] = c.fp.out_ref; o.fp[2] = c.fp.in_inc; o.fp[3] = c.fp.out_inc;
+    o.warm_start = c.warm_start;
+  }
+}
+
+int oracle_tap(void* h, int kind, void* dst, size_t cap, size_t* written) {
+  World* w = (World*)h;
+  size_t need = 0;
+  auto put = [&](const void* src, size_t bytes) -> int {
+    need = bytes;
+    if (written) *written = bytes;
+    if (bytes > cap) return BB_ERR_INVALID;
+    if (bytes) std::memcpy(dst, src, bytes);
+    return 0;
+  };
+  switch (kind) {
+    case BB_TAP_MORTON: {
+      if (w->desc.morton_bits == 30) {
+        std::vector<uint32_t> v(w->tap_morton.size());
+        for (size_t i = 0; i < v.size(); ++i) v[i] = (uint32_t)w->tap_morton[i];
+        return put(v.data(), v.size() * 4);
+      }
+      return put(w->tap_morton.data(), w->tap_morton.size() * 8);
+    }
+    case BB_TAP_SORTED: {
+      std::vector<uint32_t> v(w->tap_sorted.size() * 4);
+      for (size_t i = 0; i < w->tap_sorted.size(); ++i) {
+        uint64_t c = w->tap_sorted[i].first;
+        v[i * 4 + 0] = (uint32_t)c; v[i * 4 + 1] = (uint32_t)(c >> 32);
+        v[i * 4 + 2] = w->tap_sorted[i].second; v[i * 4 + 3] = 0;
+      }
+      return put(v.data(), v.size() * 4);
+    }
+    case BB_TAP_LBVH: {
+      std::vector<bb_lbvh_node> v(w->tap_nodes.size());
+      for (size_t i = 0; i < v.size(); ++i) {
+        const Node& s = w->tap_nodes[i];
+        v[i].minimum[0] = s.mn.x; v[i].minimum[1] = s.mn.y; v[i].minimum[2] = s.mn.z;
+        v[i].maximum[0] = s.mx.x; v[i].maximum[1] = s.mx.y; v[i].maximum[2] = s.mx.z;
+        v[i].left = s.left; v[i].right = s.right; v[i].primitive_idx = s.prim; v[i].pad = 0;
+      }
+      return put(v.data(), v.size() * sizeof(bb_lbvh_node));
+    }
+    case BB_TAP_PAIRS: {
+      std::vector<uint32_t> v(w->tap_pairs.size() * 2);
+      for (size_t i = 0; i < w->tap_pairs.size(); ++i) { v[2 * i] = w->tap_pairs[i].first; v[2 * i + 1] = w->tap_pairs[i].second; }
+      return put(v.data(), v.size() * 4);
+    }
+    case BB_TAP_MANIFOLDS: {
+      std::vector<bb_manifold_tap> v(w->manifolds.size());
+      for (size_t i = 0; i < v.size(); ++i)
+        manifold_tap(w->manifolds[i], i < w->edge_color.size() ? w->edge_color[i] : MAXU, v[i]);
+      return put(v.data(), v.size() * sizeof(bb_manifold_tap));
+    }
+    case BB_TAP_ISLANDS: return put(w->tap_islands.data(), w->tap_islands.size() * 4);
+    case BB_TAP_EDGE_COLORS: return put(w->edge_color.data(), w->edge_color.size() * 4);
+    case BB_TAP_BODY_COLORS: return put(w->body_color.data(), w->body_color.size() * 4);
+    case BB_TAP_ROWS: {
+      std::vector<bb_body_state> v(w->rows.size());
+      for (size_t i = 0; i < v.size(); ++i) fill_state(w->rows[i], (uint32_t)i, v[i]);
+      return put(v.data(), v.size() * sizeof(bb_body_state));
+    }
+  }
+  (void)need;
+  return BB_ERR_INVALID;
+}
+
+// ------------------------------------------------------ unit-level pinning
+uint32_t oracle_morton30(float x, float y, float z) { return morton30(mk(x, y, z)); }
+uint64_t oracle_morton63(float x, float y, float z) { return morton63(mk(x, y, z)); }
+uint32_t oracle_expand10(uint32_t v) { return expand10(v); }
+uint64_t oracle_expand21(uint64_t v) { return expand21(v); }
+void oracle_sincos(float x, float* s, float* c) { sincos_cephes(x, s, c); }
+uint32_t oracle_order_map(float v) { return order_map(v); }
+float oracle_order_unmap(uint32_t v) { return order_unmap(v); }
+void oracle_solve6(const float* H36, const float* b6, float* x6) {
+  float H[6][6], b[6];
+  for (int i = 0; i < 6; ++i) { b[i] = b6[i]; for (int j = 0; j < 6; ++j) H[i][j] = H36[i * 6 + j]; }
+  avbd_solve6(H, b, x6);
+}
+// Contact generation between two bodies given as descriptors (A = the lower
+// row), without warm start; returns 1 and fills *out when a manifold exists.
+int oracle_collide(const bb_world_desc* d, const bb_body_desc* da, const bb_body_desc* db, bb_manifold_tap* out) {
+  World* w = (World*)oracle_create(d);
+  oracle_add_bodies(w, da, 1);
+  oracle_add_bodies(w, db, 1);
+  Body& A = w->rows[0];
+  Body& B = w->rows[1];
+  Manifold m;
+  std::memset(&m, 0, sizeof(m));
+  bool have;
+  if (A.shape == BB_SHAPE_BOX && B.shape == BB_SHAPE_BOX) {
+    have = obb_obb(A, B, m);
+    if (have)
+      for (int ci = 0; ci < m.count; ++ci) m.c[ci].penetration = fmaxr(m.c[ci].penetration, -0.25f);
+  } else if (A.shape == BB_SHAPE_SPHERE && B.shape == BB_SHAPE_SPHERE) {
+    have = sphere_sphere(A, B, m);
+  } else {
+    have = box_sphere(A, B, m);
+  }
+  if (have) {
+    m.a = 0; m.b = 1; m.anchor_a = A.position; m.anchor_b = B.position; m.error = 0x7FFFFFFF;
+    compute_basis(m);
+    manifold_tap(m, MAXU, *out);
+  }
+  delete w;
+  return have ? 1 : 0;
+}
+
+}  // extern "C"
