@@ -1664,6 +1664,583 @@ __global__ __launch_bounds__(BLOCK) void k_pgs(Dev d) {
   for (uint32_t it = 0; it < W.iterations; ++it) sweep(2);
 }
 
+// ================================================================== AVBD
+// src/shaders/simulation/avbd.slang.  Adjacency = per-row list of incident
+// manifold slots in ascending order (replaces the linked lists).
+static constexpr float AV_BETA = 100000.0f, AV_GAMMA = 0.99f, AV_KMIN = 1.0f, AV_KMAX = 1000000000.0f;
+static constexpr float AV_MARGIN = 0.0005f, AV_STICK_THRESH = 0.01f, AV_STICK_SLOP = 0.001f;
+static constexpr uint32_t AV_ITERS = 10, AV_ROUNDS = 16, AV_POST_STAB = 4;
+static constexpr uint32_t AV_NB = kAvbdColors * kAvbdLayers;
+
+BB_DI uint32_t avbd_priority(uint32_t body, uint32_t round) {  // avbd.slang:58-63
+  uint32_t x = body * 0x9e3779b9u + round * 0x85ebca6bu;
+  x ^= x >> 16;
+  x *= 0x7feb352du;
+  x ^= x >> 15;
+  x *= 0x846ca68bu;
+  x ^= x >> 16;
+  return ((x & 0xFFFu) << 20) | (body & 0xFFFFFu);
+}
+BB_DI v3 small_angle_delta(qt now, qt start) {
+  qt dq = qmul(now, qt{-start.v, start.w});
+  float sg = dq.w < 0.0f ? -1.0f : 1.0f;
+  return dq.v * (2.0f * sg);
+}
+BB_DI v3 avbd_q_rotate(qt q, v3 u) { return u + cross(q.v, cross(q.v, u) + u * q.w) * 2.0f; }
+BB_DI mat3 avbd_inverse3(const mat3& a) {
+  v3 c0 = mcol(a, 0), c1 = mcol(a, 1), c2 = mcol(a, 2);
+  v3 r0 = cross(c1, c2), r1 = cross(c2, c0), r2 = cross(c0, c1);
+  float det = dot(c0, r0);
+  float inv = fab(det) > 1e-12f ? 1.0f / det : 0.0f;
+  return mrows(r0 * inv, r1 * inv, r2 * inv);
+}
+// avbd_solve6 (avbd.slang:324-387): in-place LDL^T, diagonal >= 1e-8
+BB_DI void avbd_solve6(float H[6][6], float b[6], float x[6]) {
+#pragma unroll
+  for (int jj = 0; jj < 6; ++jj) {
+    float dd = H[jj][jj];
+#pragma unroll
+    for (int kk = 0; kk < jj; ++kk) dd -= H[jj][kk] * H[jj][kk] * H[kk][kk];
+    dd = fmx(dd, 1e-8f);
+    H[jj][jj] = dd;
+#pragma unroll
+    for (int ii = jj + 1; ii < 6; ++ii) {
+      float s = H[ii][jj];
+#pragma unroll
+      for (int kk = 0; kk < jj; ++kk) s -= H[ii][kk] * H[jj][kk] * H[kk][kk];
+      H[ii][jj] = s / dd;
+    }
+  }
+#pragma unroll
+  for (int ii = 0; ii < 6; ++ii) {
+    float s = b[ii];
+#pragma unroll
+    for (int kk = 0; kk < ii; ++kk) s -= H[ii][kk] * b[kk];
+    b[ii] = s;
+  }
+#pragma unroll
+  for (int ii = 0; ii < 6; ++ii) b[ii] /= H[ii][ii];
+#pragma unroll
+  for (int ii = 5; ii >= 0; --ii) {
+    float s = b[ii];
+#pragma unroll
+    for (int kk = ii + 1; kk < 6; ++kk) s -= H[kk][ii] * x[kk];
+    x[ii] = s;
+  }
+}
+
+struct AvBody {  // pose and step-start state of one side of a contact
+  bool dyn;
+  v3 pos;
+  qt rot;
+  v3 p0;
+  qt q0;
+};
+BB_DI AvBody av_body(const Dev& d, uint32_t g) {
+  AvBody b;
+  uint32_t fl = d.cur.flags[g];
+  b.dyn = (fl & BB_BODY_DYNAMIC) && !(fl & BB_BODY_SLEEPING);
+  b.pos = ld3(d.cur.pos[g]);
+  b.rot = ldq(d.cur.rot[g]);
+  if (b.dyn) {
+    b.p0 = ld3(d.s_pos0[g]);
+    b.q0 = ldq(d.s_rot0[g]);
+  } else {
+    b.p0 = b.pos;
+    b.q0 = b.rot;
+  }
+  return b;
+}
+// avbd_contact_C (avbd.slang:232-258)
+BB_DI float av_contact_C(v3 n, v3 p, float pen, const AvBody& b1, const AvBody& b2, float alpha, v3& rr1,
+                         v3& rr2) {
+  rr1 = p - b1.p0;
+  rr2 = p - b2.p0;
+  float C0 = pen + AV_MARGIN;
+  float C = C0 * (1.0f - alpha);
+  if (b1.dyn) {
+    v3 dp1 = b1.pos - b1.p0;
+    v3 dth1 = small_angle_delta(b1.rot, b1.q0);
+    C -= dot(n, dp1 + cross(dth1, rr1));
+  }
+  if (b2.dyn) {
+    v3 dp2 = b2.pos - b2.p0;
+    v3 dth2 = small_angle_delta(b2.rot, b2.q0);
+    C += dot(n, dp2 + cross(dth2, rr2));
+  }
+  return C;
+}
+// avbd_dir_C (avbd.slang:274-305)
+BB_DI float av_dir_C(v3 dir, v3 anc_a, v3 anc_b, const AvBody& b1, const AvBody& b2, float alpha, v3& ar1,
+                     v3& ar2, float& C0t) {
+  ar1 = avbd_q_rotate(b1.q0, anc_a);
+  ar2 = avbd_q_rotate(b2.q0, anc_b);
+  C0t = dot(dir, (b2.p0 + ar2) - (b1.p0 + ar1));
+  float c0e = fsign(C0t) * fmx(0.0f, fab(C0t) - AV_STICK_SLOP);
+  float C = c0e * (1.0f - alpha);
+  if (b1.dyn) {
+    v3 dp1 = b1.pos - b1.p0;
+    v3 dth1 = small_angle_delta(b1.rot, b1.q0);
+    C -= dot(dir, dp1 + cross(dth1, ar1));
+  }
+  if (b2.dyn) {
+    v3 dp2 = b2.pos - b2.p0;
+    v3 dth2 = small_angle_delta(b2.rot, b2.q0);
+    C += dot(dir, dp2 + cross(dth2, ar2));
+  }
+  return C;
+}
+
+// entry_avbd_primal (avbd.slang:475-658) for row i
+BB_DI void av_primal(const Dev& d, const WorldInfo& W, uint32_t i, float alpha) {
+  const uint32_t off = W.body_off, g = off + i;
+  const uint32_t id = d.cur.id[g];
+  const bool quirk = (W.quirks & BB_QUIRK_TRANSPOSED_WORLD_INERTIA) != 0;
+  const float dt = W.dt;
+  const float inv_dt2 = dt > 0.0f ? 1.0f / (dt * dt) : 0.0f;
+  v3 pos = ld3(d.cur.pos[g]);
+  qt rot = ldq(d.cur.rot[g]);
+  const float m_lin = d.props.mass[off + id].x * inv_dt2;
+  const mat3 Iw = avbd_inverse3(world_inv_inertia(d, off + id, rot, quirk));
+  float H[6][6], gr[6];
+#pragma unroll
+  for (int a = 0; a < 6; ++a) {
+    gr[a] = 0.0f;
+#pragma unroll
+    for (int b = 0; b < 6; ++b) H[a][b] = 0.0f;
+  }
+  H[0][0] = m_lin; H[1][1] = m_lin; H[2][2] = m_lin;
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int b = 0; b < 3; ++b) H[3 + a][3 + b] = Iw.m[a][b] * inv_dt2;
+  const v3 dpos = pos - ld3(d.s_post[g]);
+  const v3 dth = small_angle_delta(rot, ldq(d.s_rott[g]));
+  gr[0] = m_lin * dpos.x; gr[1] = m_lin * dpos.y; gr[2] = m_lin * dpos.z;
+  const v3 ga = mulmv(Iw, dth) * inv_dt2;
+  gr[3] = ga.x; gr[4] = ga.y; gr[5] = ga.z;
+  const uint32_t a0 = 2 * W.pair_off + d.adj_off[g], a1 = a0 + d.deg[g];
+  for (uint32_t e = a0; e < a1; ++e) {
+    const uint32_t slot = d.adj[e];
+    const int4 h = d.man_cur.hdr[slot];
+    const uint32_t b1 = (uint32_t)h.x, b2 = (uint32_t)h.y;
+    const bool i1 = b1 == i;
+    const AvBody B1 = av_body(d, off + b1), B2 = av_body(d, off + b2);
+    const float f1 = d.props.mass[off + d.cur.id[off + b1]].z, f2 = d.props.mass[off + d.cur.id[off + b2]].z;
+    const float mu = __builtin_sqrtf(fmx(f1, 0.0f) * fmx(f2, 0.0f));
+    const v3 n = ld3(d.man_cur.nrm[slot]);
+    const v3 tdir[2] = {ld3(d.man_cur.t0[slot]), ld3(d.man_cur.t1[slot])};
+    for (int j = 0; j < h.w; ++j) {
+      const uint32_t cs = slot * 8 + j;
+      const float4 cp = d.man_cur.cpos[cs];
+      const float4 lm = d.man_cur.clam[cs];
+      const float4 cmv = d.man_cur.cmas[cs];
+      const v3 anc_a = ld3(d.man_cur.cana[cs]), anc_b = ld3(d.man_cur.canb[cs]);
+      v3 rr1, rr2;
+      float C = av_contact_C(n, ld3(cp), cp.w, B1, B2, alpha, rr1, rr2);
+      float lam = lm.x, kk = lm.w;
+      float f = fmn(kk * C + lam, 0.0f);
+      v3 Jl = i1 ? -n : n;
+      v3 Ja = i1 ? -cross(rr1, n) : cross(rr2, n);
+      gr[0] += f * Jl.x; gr[1] += f * Jl.y; gr[2] += f * Jl.z;
+      gr[3] += f * Ja.x; gr[4] += f * Ja.y; gr[5] += f * Ja.z;
+      float J[6] = {Jl.x, Jl.y, Jl.z, Ja.x, Ja.y, Ja.z};
+#pragma unroll
+      for (int a = 0; a < 6; ++a)
+#pragma unroll
+        for (int b = 0; b < 6; ++b) H[a][b] += kk * J[a] * J[b];
+      const float bound = mu * fab(lam);
+#pragma unroll
+      for (int t = 0; t < 2; ++t) {
+        const v3 td = tdir[t];
+        v3 ar1, ar2;
+        float C0t;
+        float Ct = av_dir_C(td, anc_a, anc_b, B1, B2, alpha, ar1, ar2, C0t);
+        float lt = t == 0 ? lm.y : lm.z;
+        float kt = t == 0 ? cmv.y : cmv.z;
+        float ft = fcl(kt * Ct + lt, -bound, bound);
+        v3 Jlt = i1 ? -td : td;
+        v3 Jat = i1 ? -cross(ar1, td) : cross(ar2, td);
+        gr[0] += ft * Jlt.x; gr[1] += ft * Jlt.y; gr[2] += ft * Jlt.z;
+        gr[3] += ft * Jat.x; gr[4] += ft * Jat.y; gr[5] += ft * Jat.z;
+        float Jt[6] = {Jlt.x, Jlt.y, Jlt.z, Jat.x, Jat.y, Jat.z};
+#pragma unroll
+        for (int a = 0; a < 6; ++a)
+#pragma unroll
+          for (int b = 0; b < 6; ++b) H[a][b] += kt * Jt[a] * Jt[b];
+      }
+    }
+  }
+  float rhs[6], dx[6];
+#pragma unroll
+  for (int a = 0; a < 6; ++a) rhs[a] = -gr[a];
+  avbd_solve6(H, rhs, dx);
+  pos = pos + V(dx[0], dx[1], dx[2]);
+  v3 half = V(dx[3], dx[4], dx[5]) * 0.5f;
+  rot = qnorm(qmul(qt{half, 1.0f}, rot));
+  d.cur.pos[g] = st3(pos, 0.0f);
+  d.cur.rot[g] = stq(rot);
+}
+
+// entry_avbd_dual (avbd.slang:726-808) for one manifold slot
+BB_DI void av_dual(const Dev& d, const WorldInfo& W, uint32_t slot) {
+  const uint32_t off = W.body_off;
+  const int4 h = d.man_cur.hdr[slot];
+  const AvBody B1 = av_body(d, off + h.x), B2 = av_body(d, off + h.y);
+  if (!B1.dyn && !B2.dyn) return;
+  const float f1 = d.props.mass[off + d.cur.id[off + h.x]].z, f2 = d.props.mass[off + d.cur.id[off + h.y]].z;
+  const float mu = __builtin_sqrtf(fmx(f1, 0.0f) * fmx(f2, 0.0f));
+  const v3 n = ld3(d.man_cur.nrm[slot]);
+  const v3 tdir[2] = {ld3(d.man_cur.t0[slot]), ld3(d.man_cur.t1[slot])};
+  for (int j = 0; j < h.w; ++j) {
+    const uint32_t cs = slot * 8 + j;
+    const float4 cp = d.man_cur.cpos[cs];
+    float4 lm = d.man_cur.clam[cs];
+    float4 cmv = d.man_cur.cmas[cs];
+    float4 cb = d.man_cur.canb[cs];
+    const v3 anc_a = ld3(d.man_cur.cana[cs]), anc_b = ld3(cb);
+    v3 rr1, rr2;
+    float C = av_contact_C(n, ld3(cp), cp.w, B1, B2, 1.0f, rr1, rr2);
+    const float lam = lm.x, kk = lm.w;
+    const float nl = fmn(kk * C + lam, 0.0f);
+    lm.x = nl;
+    if (nl < 0.0f) lm.w = fmn(kk + AV_BETA * fab(C), AV_KMAX);
+    const float bound = mu * fab(lam);
+    bool stick_ok = true;
+    float drift2 = 0.0f;
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      v3 ar1, ar2;
+      float C0t;
+      float Ct = av_dir_C(tdir[t], anc_a, anc_b, B1, B2, 1.0f, ar1, ar2, C0t);
+      float lt = t == 0 ? lm.y : lm.z;
+      float kt = t == 0 ? cmv.y : cmv.z;
+      float nlt = fcl(kt * Ct + lt, -bound, bound);
+      if (t == 0) lm.y = nlt; else lm.z = nlt;
+      if (nlt > -bound && nlt < bound) {
+        float nk = fmn(kt + AV_BETA * fab(Ct), AV_KMAX);
+        if (t == 0) cmv.y = nk; else cmv.z = nk;
+      } else {
+        stick_ok = false;
+      }
+      drift2 += C0t * C0t;
+    }
+    cb.w = __uint_as_float((stick_ok && drift2 < AV_STICK_THRESH * AV_STICK_THRESH) ? 1u : 0u);
+    d.man_cur.clam[cs] = lm;
+    d.man_cur.cmas[cs] = cmv;
+    d.man_cur.canb[cs] = cb;
+  }
+}
+
+template <int BLOCK>
+struct AvbdLds {
+  ScanLds<BLOCK> scan;
+  uint32_t carry;
+  uint32_t flag;
+  uint32_t used;
+  uint32_t bcount[AV_NB];
+  uint32_t boff[AV_NB + 1];
+};
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_avbd(Dev d) {
+  __shared__ AvbdLds<BLOCK> L;
+  const uint32_t w = blockIdx.x, tid = threadIdx.x;
+  const WorldInfo W = d.worlds[w];
+  if (W.solver != BB_SOLVER_AVBD || W.n == 0) return;
+  const uint32_t N = W.n, off = W.body_off, P = d.npairs_cur[w];
+  const float dt = W.dt;
+  // ---- adjacency (CSR, ascending slot per row) for dynamic rows
+  if (tid == 0) L.carry = 0;
+  __syncthreads();
+  for (uint32_t c0 = 0; c0 < N; c0 += BLOCK) {
+    uint32_t i = c0 + tid;
+    uint32_t v = i < N ? d.deg[off + i] : 0u;
+    uint32_t tot;
+    uint32_t ex = block_scan<BLOCK>(v, L.scan, tot);
+    if (i < N) {
+      d.adj_off[off + i] = L.carry + ex;
+      d.pcount[off + i] = 0;  // fill cursor
+    }
+    __syncthreads();
+    if (tid == 0) L.carry += tot;
+    __syncthreads();
+  }
+  uint32_t* adj = d.adj + 2 * W.pair_off;
+  for (uint32_t p = tid; p < P; p += BLOCK) {
+    const uint32_t slot = W.pair_off + p;
+    int4 h = d.man_cur.hdr[slot];
+    if (h.w <= 0) continue;
+    const uint32_t ga = off + h.x, gb = off + h.y;
+    if (d.cur.flags[ga] & BB_BODY_DYNAMIC) adj[d.adj_off[ga] + atomicAdd(&d.pcount[ga], 1u)] = slot;
+    if (d.cur.flags[gb] & BB_BODY_DYNAMIC) adj[d.adj_off[gb] + atomicAdd(&d.pcount[gb], 1u)] = slot;
+  }
+  __syncthreads();
+  for (uint32_t i = tid; i < N; i += BLOCK) {
+    const uint32_t g = off + i;
+    if (!(d.cur.flags[g] & BB_BODY_DYNAMIC)) continue;
+    uint32_t* l = adj + d.adj_off[g];
+    const uint32_t cnt = d.deg[g];
+    for (uint32_t a = 1; a < cnt; ++a) {
+      uint32_t x = l[a], b = a;
+      while (b > 0 && l[b - 1] > x) {
+        l[b] = l[b - 1];
+        --b;
+      }
+      l[b] = x;
+    }
+    d.bcolor[g] = kMAX;
+  }
+  for (uint32_t i = tid; i < N; i += BLOCK)
+    if (!(d.cur.flags[off + i] & BB_BODY_DYNAMIC)) d.bcolor[off + i] = kMAX;
+  __syncthreads();
+  // ---- Jones-Plassmann body colouring, round-snapshot semantics
+  // (entry_avbd_color_round, avbd.slang:846-911)
+  for (uint32_t rd = 0; rd < AV_ROUNDS; ++rd) {
+    if (tid == 0) L.flag = 0;
+    for (uint32_t i = tid; i < N; i += BLOCK) {
+      uint32_t c = d.bcolor[off + i];
+      d.bcolor_snap[off + i] = c;
+      if (c == kMAX && (d.cur.flags[off + i] & BB_BODY_DYNAMIC)) L.flag = 1;
+    }
+    __syncthreads();
+    if (L.flag == 0) break;
+    for (uint32_t i = tid; i < N; i += BLOCK) {
+      const uint32_t g = off + i;
+      if (d.bcolor_snap[g] != kMAX || !(d.cur.flags[g] & BB_BODY_DYNAMIC)) continue;
+      const uint32_t my = avbd_priority(d.cur.id[g], rd);
+      uint32_t used = 0;
+      bool lose = false;
+      const uint32_t* l = adj + d.adj_off[g];
+      for (uint32_t e = 0; e < d.deg[g]; ++e) {
+        int4 h = d.man_cur.hdr[l[e]];
+        uint32_t pr = (uint32_t)h.x == i ? (uint32_t)h.y : (uint32_t)h.x;
+        if (pr >= N || pr == i) continue;
+        if (!(d.cur.flags[off + pr] & BB_BODY_DYNAMIC)) continue;
+        uint32_t pc = d.bcolor_snap[off + pr];
+        if (pc == kMAX) {
+          if (avbd_priority(d.cur.id[off + pr], rd) > my) {
+            lose = true;
+            break;
+          }
+        } else if (pc < kAvbdColors) {
+          used |= 1u << pc;
+        }
+      }
+      if (lose) continue;
+      uint32_t avail = ~used;
+      if (avail == 0) continue;
+      uint32_t c = (uint32_t)(__ffs(avail) - 1);
+      if (c < kAvbdColors) d.bcolor[g] = c;
+    }
+    __syncthreads();
+  }
+  // validator (entry_avbd_color_validate, avbd.slang:913-981)
+  {
+    uint32_t viol = 0, cnt = 0;
+    for (uint32_t i = tid; i < N; i += BLOCK) {
+      const uint32_t g = off + i;
+      if (!(d.cur.flags[g] & BB_BODY_DYNAMIC)) continue;
+      const uint32_t mc = d.bcolor[g];
+      bool has = false;
+      const uint32_t* l = adj + d.adj_off[g];
+      for (uint32_t e = 0; e < d.deg[g]; ++e) {
+        int4 h = d.man_cur.hdr[l[e]];
+        uint32_t pr = (uint32_t)h.x == i ? (uint32_t)h.y : (uint32_t)h.x;
+        if (pr >= N || pr == i || !(d.cur.flags[off + pr] & BB_BODY_DYNAMIC)) continue;
+        has = true;
+        if (mc != kMAX && d.bcolor[off + pr] == mc && pr > i) ++viol;
+      }
+      if (mc == kMAX) {
+        if (has) ++viol;
+        continue;
+      }
+      cnt = cnt > mc + 1 ? cnt : mc + 1;
+    }
+    stat_add(d, w, ST_AV_VIOL, viol);
+    if (cnt) stat_max(d, w, ST_AV_COUNT, cnt);
+  }
+  // ---- prepare (avbd.slang:70-128)
+  for (uint32_t i = tid; i < N; i += BLOCK) {
+    const uint32_t g = off + i;
+    const uint32_t fl = d.cur.flags[g];
+    if (!(fl & BB_BODY_DYNAMIC) || (fl & BB_BODY_SLEEPING)) continue;
+    v3 om = ld3(d.cur.omg[g]);
+    float oml = vlen(om);
+    if (oml > 50.0f) {
+      om = om * (50.0f / oml);
+      d.cur.omg[g] = st3(om, 0.0f);
+    }
+    const float grav = (fl & BB_BODY_GRAVITY) ? W.gravity : 0.0f;
+    const v3 p = ld3(d.cur.pos[g]), v = ld3(d.cur.vel[g]), pv = ld3(d.cur.pvel[g]);
+    const qt q = ldq(d.cur.rot[g]);
+    d.s_pos0[g] = st3(p, 0.0f);
+    d.s_rot0[g] = stq(q);
+    d.s_post[g] = st3((p + v * dt) + V(0.0f, grav, 0.0f) * (dt * dt), 0.0f);
+    qt qtl = apply_angular_velocity(q, om, dt);
+    d.s_rott[g] = stq(qtl);
+    float accel_w = 0.0f;
+    if (fab(grav) > 1e-6f) {
+      float accel_y = (v.y - pv.y) / dt;
+      accel_w = fcl(accel_y * fsign(grav) / fab(grav), 0.0f, 1.0f);
+    }
+    d.cur.pos[g] = st3((p + v * dt) + V(0.0f, grav, 0.0f) * (accel_w * dt * dt), 0.0f);
+    d.cur.rot[g] = stq(qtl);
+  }
+  __syncthreads();
+  // ---- warm start (avbd.slang:389-473)
+  {
+    uint32_t sticks = 0;
+    const bool ws_on = (W.flags & BB_SIM_WARM_STARTING) != 0;
+    for (uint32_t p = tid; p < P; p += BLOCK) {
+      const uint32_t slot = W.pair_off + p;
+      int4 h = d.man_cur.hdr[slot];
+      if (h.w <= 0) continue;
+      const AvBody B1 = av_body(d, off + h.x), B2 = av_body(d, off + h.y);
+      const qt q1c = qconj(B1.q0), q2c = qconj(B2.q0);
+      for (int j = 0; j < h.w; ++j) {
+        const uint32_t cs = slot * 8 + j;
+        float4 lm = d.man_cur.clam[cs];
+        float4 cmv = d.man_cur.cmas[cs];
+        float4 ca = d.man_cur.cana[cs];
+        float4 cb = d.man_cur.canb[cs];
+        const bool matched = d.man_cur.cws[cs] != 0 && ws_on;
+        uint32_t stick = __float_as_uint(cb.w);
+        if (!matched) {
+          lm.x = 0.0f;
+          lm.w = AV_KMIN;
+          lm.y = lm.z = 0.0f;
+          cmv.y = cmv.z = AV_KMIN;
+          stick = 0;
+        } else {
+          lm.w = fcl(lm.w * AV_GAMMA, AV_KMIN, AV_KMAX);
+          lm.x = fmn(lm.x, 0.0f);
+          cmv.y = fcl(cmv.y * AV_GAMMA, AV_KMIN, AV_KMAX);
+          cmv.z = fcl(cmv.z * AV_GAMMA, AV_KMIN, AV_KMAX);
+        }
+        const bool sticking = matched && stick != 0;
+        if (!sticking) {
+          const v3 pp = ld3(d.man_cur.cpos[cs]);
+          ca = st3(avbd_q_rotate(q1c, pp - B1.p0), ca.w);
+          cb = st3(avbd_q_rotate(q2c, pp - B2.p0), cb.w);
+        } else {
+          ++sticks;
+        }
+        cb.w = __uint_as_float(stick);
+        d.man_cur.clam[cs] = lm;
+        d.man_cur.cmas[cs] = cmv;
+        d.man_cur.cana[cs] = ca;
+        d.man_cur.canb[cs] = cb;
+      }
+    }
+    stat_add(d, w, ST_AV_STICK, sticks);
+  }
+  // ---- support depth (avbd.slang:667-724), pass-snapshot semantics
+  for (uint32_t i = tid; i < N; i += BLOCK) {
+    const uint32_t fl = d.cur.flags[off + i];
+    const bool grounded = !(fl & BB_BODY_DYNAMIC) || (fl & BB_BODY_SLEEPING);
+    d.depth[off + i] = grounded ? 0u : kMAX;
+  }
+  __syncthreads();
+  for (uint32_t pass = 0; pass < kAvbdLayers; ++pass) {
+    for (uint32_t i = tid; i < N; i += BLOCK) d.depth_snap[off + i] = d.depth[off + i];
+    __syncthreads();
+    for (uint32_t p = tid; p < P; p += BLOCK) {
+      int4 h = d.man_cur.hdr[W.pair_off + p];
+      if (h.w <= 0) continue;
+      const uint32_t d1 = d.depth_snap[off + h.x], d2 = d.depth_snap[off + h.y];
+      if (d2 != kMAX && d2 + 1u < d1) atomicMin(&d.depth[off + h.x], d2 + 1u);
+      if (d1 != kMAX && d1 + 1u < d2) atomicMin(&d.depth[off + h.y], d1 + 1u);
+    }
+    __syncthreads();
+  }
+  // ---- bodies grouped by (colour, layer) for the primal sweeps
+  for (uint32_t k = tid; k < AV_NB; k += BLOCK) L.bcount[k] = 0;
+  if (tid == 0) L.used = 0;
+  __syncthreads();
+  for (uint32_t i = tid; i < N; i += BLOCK) {
+    const uint32_t g = off + i;
+    const uint32_t fl = d.cur.flags[g], c = d.bcolor[g];
+    if (!(fl & BB_BODY_DYNAMIC) || (fl & BB_BODY_SLEEPING) || c >= kAvbdColors) continue;
+    uint32_t layer = d.depth[g] < kAvbdLayers - 1 ? d.depth[g] : kAvbdLayers - 1;
+    atomicAdd(&L.bcount[c * kAvbdLayers + layer], 1u);
+    atomicMax(&L.used, c + 1);
+  }
+  __syncthreads();
+  if (tid == 0) {
+    uint32_t s = 0;
+    for (uint32_t k = 0; k < AV_NB; ++k) {
+      L.boff[k] = s;
+      s += L.bcount[k];
+      L.bcount[k] = L.boff[k];  // fill cursor
+    }
+    L.boff[AV_NB] = s;
+  }
+  __syncthreads();
+  uint32_t* bb = d.bbucket + off;
+  for (uint32_t i = tid; i < N; i += BLOCK) {
+    const uint32_t g = off + i;
+    const uint32_t fl = d.cur.flags[g], c = d.bcolor[g];
+    if (!(fl & BB_BODY_DYNAMIC) || (fl & BB_BODY_SLEEPING) || c >= kAvbdColors) continue;
+    uint32_t layer = d.depth[g] < kAvbdLayers - 1 ? d.depth[g] : kAvbdLayers - 1;
+    bb[atomicAdd(&L.bcount[c * kAvbdLayers + layer], 1u)] = i;
+  }
+  __syncthreads();
+  const uint32_t used = L.used;
+  // ---- main sweeps: 10 x (colours, dual) (rigid_body_manager.cpp:1366-1373)
+  for (uint32_t it = 0; it < AV_ITERS; ++it) {
+    for (uint32_t c = 0; c < used; ++c) {
+      for (uint32_t k = L.boff[c * kAvbdLayers] + tid; k < L.boff[(c + 1) * kAvbdLayers]; k += BLOCK)
+        av_primal(d, W, bb[k], 1.0f);
+      __syncthreads();
+    }
+    for (uint32_t p = tid; p < P; p += BLOCK)
+      if (d.man_cur.hdr[W.pair_off + p].w > 0) av_dual(d, W, W.pair_off + p);
+    __syncthreads();
+  }
+  // ---- finalize (avbd.slang:130-202)
+  {
+    const float inv_dt = dt > 0.0f ? 1.0f / dt : 0.0f;
+    for (uint32_t i = tid; i < N; i += BLOCK) {
+      const uint32_t g = off + i;
+      const uint32_t fl = d.cur.flags[g];
+      if (!(fl & BB_BODY_DYNAMIC) || (fl & BB_BODY_SLEEPING)) continue;
+      const v3 p = ld3(d.cur.pos[g]);
+      const qt q = ldq(d.cur.rot[g]);
+      const qt q0 = ldq(d.s_rot0[g]);
+      d.cur.pvel[g] = d.cur.vel[g];
+      v3 v = (p - ld3(d.s_pos0[g])) * inv_dt;
+      qt dq = qmul(q, qt{-q0.v, q0.w});
+      float sg = dq.w < 0.0f ? -1.0f : 1.0f;
+      v3 om = dq.v * (2.0f * sg * inv_dt);
+      float v2 = dot(v, v), w2 = dot(om, om);
+      if (v2 < 0.09f && w2 < 0.16f && d.deg[g] >= 3u) {
+        v = v * 0.9f;
+        om = om * 0.9f;
+      }
+      d.cur.vel[g] = st3(v, 0.0f);
+      d.cur.omg[g] = st3(om, 0.0f);
+    }
+  }
+  __syncthreads();
+  // ---- post-stabilisation: one cascade by support layer, then plain sweeps
+  // (rigid_body_manager.cpp:1391-1423)
+  for (uint32_t layer = 0; layer < kAvbdLayers; ++layer) {
+    for (uint32_t c = 0; c < used; ++c) {
+      const uint32_t k0 = L.boff[c * kAvbdLayers + layer], k1 = L.boff[c * kAvbdLayers + layer + 1];
+      for (uint32_t k = k0 + tid; k < k1; k += BLOCK) av_primal(d, W, bb[k], 0.0f);
+      __syncthreads();
+    }
+  }
+  for (uint32_t ps = 1; ps < AV_POST_STAB; ++ps) {
+    for (uint32_t c = 0; c < used; ++c) {
+      for (uint32_t k = L.boff[c * kAvbdLayers] + tid; k < L.boff[(c + 1) * kAvbdLayers]; k += BLOCK)
+        av_primal(d, W, bb[k], 0.0f);
+      __syncthreads();
+    }
+  }
+}
+
 // ============================================================== k_finish
 // End-of-step counters (RB_sim.slang:1655-1715): id checksum and max |v|.
 template <int BLOCK>
@@ -1699,6 +2276,7 @@ namespace bb {
   template __global__ void k_narrow<B>(Dev);   \
   template __global__ void k_post<B>(Dev);     \
   template __global__ void k_pgs<B>(Dev);      \
+  template __global__ void k_avbd<B>(Dev);     \
   template __global__ void k_finish<B>(Dev);
 
 template <int B>
@@ -1709,6 +2287,7 @@ static void launch_all(const Dev& d, hipStream_t st) {
   hipLaunchKernelGGL(k_narrow<B>, g, b, 0, st, d);
   hipLaunchKernelGGL(k_post<B>, g, b, 0, st, d);
   hipLaunchKernelGGL(k_pgs<B>, g, b, 0, st, d);
+  hipLaunchKernelGGL(k_avbd<B>, g, b, 0, st, d);
   hipLaunchKernelGGL(k_finish<B>, g, b, 0, st, d);
 }
 
