@@ -91,6 +91,8 @@ MANIFOLD_TAP = np.dtype([
 TAP_MORTON, TAP_SORTED, TAP_LBVH, TAP_PAIRS, TAP_MANIFOLDS = 0, 1, 2, 3, 4
 TAP_ISLANDS, TAP_EDGE_COLORS, TAP_BODY_COLORS, TAP_ROWS = 5, 6, 7, 8
 ORDER_BY_ID, ORDER_BY_ROW = 0, 1
+KERNEL_NAMES = ("k_build", "k_broad", "k_narrow", "k_post", "k_pgs", "k_avbd", "k_finish")
+KERNEL_COUNT = len(KERNEL_NAMES)
 
 STATUS = {0: "BB_OK", 1: "BB_ERR_INVALID", 2: "BB_ERR_CAPACITY", 3: "BB_ERR_DEVICE",
           4: "BB_ERR_NO_DEVICE", 5: "BB_ERR_OVERFLOW", 6: "BB_ERR_STATE", 7: "BB_ERR_BUDGET"}
@@ -141,6 +143,8 @@ EXPORTS = {
     "bb_batch_world_count": (C.c_uint32, [C.c_void_p]),
     "bb_batch_body_count": (C.c_uint32, [C.c_void_p]),
     "bb_batch_set_block_size": (C.c_int, [C.c_void_p, C.c_uint32]),
+    "bb_batch_profile": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p]),
+    "bb_batch_device_bytes": (C.c_uint64, [C.c_void_p]),
     "bb_batch_destroy": (None, [C.c_void_p]),
     "bb_version": (C.c_char_p, []),
 }
@@ -276,6 +280,19 @@ class Batch:
 
     def sync(self) -> None:
         _check(lib().bb_batch_sync(self._h), "bb_batch_sync")
+
+    def profile(self, nsteps: int):
+        """Eager steps timed with hipEvents on the library stream: returns
+        (per-step ms array, per-stage summed ms array)."""
+        step_ms = np.zeros(nsteps, np.float32)
+        kern_ms = np.zeros(KERNEL_COUNT, np.float32)
+        _check(lib().bb_batch_profile(self._h, nsteps, step_ms.ctypes.data_as(C.c_void_p),
+                                      kern_ms.ctypes.data_as(C.c_void_p)), "bb_batch_profile")
+        return step_ms, kern_ms
+
+    @property
+    def device_bytes(self) -> int:
+        return lib().bb_batch_device_bytes(self._h)
 
     def set_block_size(self, threads: int) -> None:
         _check(lib().bb_batch_set_block_size(self._h, threads), "bb_batch_set_block_size")

@@ -18,6 +18,7 @@
 
 namespace bb {
 void launch_step(const Dev& d, hipStream_t st, uint32_t block);
+void launch_step_timed(const Dev& d, hipStream_t st, uint32_t block, hipEvent_t* ev);
 }  // namespace bb
 
 using namespace bb;
@@ -43,11 +44,13 @@ struct HostWorld {
 
 struct Buffers {
   std::vector<void*> ptrs;
+  uint64_t bytes = 0;
   template <class T>
   T* alloc(size_t count) {
     void* p = nullptr;
-    size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
-    if (hipMalloc(&p, bytes) != hipSuccess) p = nullptr;
+    size_t n = std::max<size_t>(count, 1) * sizeof(T);
+    if (hipMalloc(&p, n) != hipSuccess) p = nullptr;
+    else bytes += n;
     ptrs.push_back(p);
     return (T*)p;
   }
@@ -55,6 +58,7 @@ struct Buffers {
     for (void* p : ptrs)
       if (p) (void)hipFree(p);
     ptrs.clear();
+    bytes = 0;
   }
 };
 
@@ -265,6 +269,15 @@ bb_status build(bb_batch* b) {
   d.bucket = A.alloc<uint32_t>(NP);
   d.bucket_off = A.alloc<uint32_t>((size_t)W * (kBuckets + 1));
   d.iw = A.alloc<float4>((size_t)NB * 3);
+  d.np = NP;
+  d.sv_hdr = A.alloc<int4>(NP);
+  d.sv_n = A.alloc<float4>(NP);
+  d.sv_t0 = A.alloc<float4>(NP);
+  d.sv_t1 = A.alloc<float4>(NP);
+  d.sv_r1 = A.alloc<float4>((size_t)NP * 8);
+  d.sv_r2 = A.alloc<float4>((size_t)NP * 8);
+  d.sv_l = A.alloc<float4>((size_t)NP * 8);
+  d.sv_tm1 = A.alloc<float>((size_t)NP * 8);
   d.s_pos0 = A.alloc<float4>(NB);
   d.s_rot0 = A.alloc<float4>(NB);
   d.s_post = A.alloc<float4>(NB);
@@ -481,6 +494,42 @@ bb_status step_batch(bb_batch* b, uint32_t nsteps, bool use_graph) {
     st = launch_one(b);
     if (st != BB_OK) return st;
   }
+  return BB_OK;
+}
+
+bb_status profile_batch(bb_batch* b, uint32_t nsteps, float* step_ms, float* kernel_ms) {
+  if (b->worlds.empty()) return BB_ERR_STATE;
+  bb_status st = build(b);
+  if (st != BB_OK) return st;
+  while (any_one_shot(b)) {  // the one-shot step runs untimed
+    st = step_batch(b, 1, false);
+    if (st != BB_OK) return st;
+  }
+  if (kernel_ms)
+    for (int k = 0; k < BB_KERNEL_COUNT; ++k) kernel_ms[k] = 0.0f;
+  std::vector<hipEvent_t> ev((size_t)nsteps * 8);
+  for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+  for (uint32_t s = 0; s < nsteps; ++s) {
+    const int cur = b->latest ^ 1;
+    launch_step_timed(dev_for(b, cur), b->stream, pick_block(b), &ev[(size_t)s * 8]);
+    b->latest = cur;
+    b->stepped = true;
+    ++b->step_index;
+  }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(b->stream));
+  for (uint32_t s = 0; s < nsteps; ++s) {
+    hipEvent_t* e = &ev[(size_t)s * 8];
+    float total = 0.0f;
+    HIPCHK(hipEventElapsedTime(&total, e[0], e[7]));
+    if (step_ms) step_ms[s] = total;
+    for (int k = 0; k < BB_KERNEL_COUNT && kernel_ms; ++k) {
+      float t = 0.0f;
+      HIPCHK(hipEventElapsedTime(&t, e[k], e[k + 1]));
+      kernel_ms[k] += t;
+    }
+  }
+  for (auto& e : ev) (void)hipEventDestroy(e);
   return BB_OK;
 }
 
@@ -842,6 +891,12 @@ bb_status bb_batch_world_stats(bb_batch* b, uint32_t w, bb_step_stats* out) {
 bb_status bb_batch_debug_tap(bb_batch* b, uint32_t w, bb_tap kind, void* dst, size_t cap, size_t* written) {
   return b ? tap_world(b, w, kind, dst, cap, written) : BB_ERR_INVALID;
 }
+
+bb_status bb_batch_profile(bb_batch* b, uint32_t nsteps, float* step_ms, float* kernel_ms) {
+  return b ? profile_batch(b, nsteps, step_ms, kernel_ms) : BB_ERR_INVALID;
+}
+
+uint64_t bb_batch_device_bytes(const bb_batch* b) { return b ? b->buf.bytes : 0; }
 
 uint32_t bb_batch_world_count(const bb_batch* b) { return b ? (uint32_t)b->worlds.size() : 0; }
 

@@ -113,7 +113,17 @@ struct Dev {
   uint32_t* owner;    // [body*32], zero between rounds
   uint32_t* bucket;   // [pair slot] manifold slots grouped by colour
   uint32_t* bucket_off;  // [W * (kBuckets+1)]
-  float4* iw;         // [body*3] cached world inverse inertia rows
+  float4* iw;         // [body*3] cached world inverse inertia rows, row0.w = inv_mass
+  // PGS solver rows (colour-sorted, [pair slot]; contact c of row k at c*np+k)
+  uint32_t np;        // pair slots in the batch
+  int4* sv_hdr;       // a, b, count | dynamic bits, mass coefficient bits
+  float4* sv_n;       // normal, friction
+  float4* sv_t0;      // tangent 0, bias factor
+  float4* sv_t1;      // tangent 1, impulse coefficient
+  float4* sv_r1;      // [8*np] arm a, penetration
+  float4* sv_r2;      // [8*np] arm b, normal mass
+  float4* sv_l;       // [8*np] normal/tangent impulses, tangent mass 0
+  float* sv_tm1;      // [8*np] tangent mass 1
   // AVBD
   float4* s_pos0;     // [body] pos_start, w = support depth bits
   float4* s_rot0;

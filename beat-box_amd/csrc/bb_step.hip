@@ -373,10 +373,10 @@ __global__ __launch_bounds__(BLOCK) void k_build(Dev d) {
     for (uint32_t i = tid; i < N; i += BLOCK) {
       uint32_t p = d.parent[nb + leaf_off + i];
       while (p + 1 < N) {  // p is an internal node index in [0, N-2]
-        __threadfence();
+        __threadfence_block();
         uint32_t old = atomicAdd(&d.visit[nb + p], 1u);
         if (old == 0) break;
-        __threadfence();
+        __threadfence_block();
         float4 a0 = d.nmin[nb + p], a1 = d.nmax[nb + p];
         int l = __float_as_int(a0.w), r = __float_as_int(a1.w);
         if (l < 0 || r < 0 || l >= (int)(2 * N - 1) || r >= (int)(2 * N - 1)) break;
@@ -1152,6 +1152,11 @@ BB_DI uint32_t gc_priority(uint32_t pid, uint32_t round) {  // coloring.slang:13
 }
 
 template <int BLOCK>
+BB_DI void iw_refresh(const Dev& d, const WorldInfo& W);
+BB_DI bool pgs_solvable(const Dev& d, uint32_t off, int4 h);
+BB_DI void pgs_compact(const Dev& d, const WorldInfo& W, uint32_t k, uint32_t slot, float hertz);
+
+template <int BLOCK>
 struct PostLds {
   ScanLds<BLOCK> scan;
   uint32_t carry;
@@ -1394,31 +1399,50 @@ __global__ __launch_bounds__(BLOCK) void k_post(Dev d) {
     d.owner[(off + h.x) * kColors + cc] = 0;
     d.owner[(off + h.y) * kColors + cc] = 0;
   }
-  if (tid == 0) {
-    uint32_t s = 0;
-    uint32_t* bo = d.bucket_off + w * (kBuckets + 1);
-    for (uint32_t c2 = 0; c2 < kBuckets; ++c2) {
-      bo[c2] = s;
-      L.bfill[c2] = s;
-      s += L.bcount[c2];
-    }
-    bo[kBuckets] = s;
-    d.stats[w * ST_COUNT + ST_GC_OVF] = L.bcount[kColors];
+  __syncthreads();
+  // ---- colour buckets of the solvable manifolds (at least one awake dynamic
+  // body; the others are skipped by every sweep, RB_sim.slang:1424).  Order
+  // inside a colour is free (no shared dynamic body); the overflow bucket is
+  // filled in ascending slot order for its serial solve.
+  if (tid == 0) d.stats[w * ST_COUNT + ST_GC_OVF] = L.bcount[kColors];
+  __syncthreads();
+  if (tid < kBuckets) L.bcount[tid] = 0;
+  __syncthreads();
+  for (uint32_t p = tid; p < P; p += BLOCK) {
+    uint32_t cc = color[p];
+    if (cc == kMAX - 1) continue;
+    int4 h = d.man_cur.hdr[W.pair_off + p];
+    if (!pgs_solvable(d, off, h)) continue;
+    atomicAdd(&L.bcount[cc < kColors ? cc : kColors], 1u);
   }
   __syncthreads();
-  // colour buckets: order inside a colour is free (no shared dynamic body);
-  // the overflow bucket is filled in ascending slot order (serial solve).
+  if (tid == 0) {
+    uint32_t sum = 0;
+    uint32_t* bo = d.bucket_off + w * (kBuckets + 1);
+    for (uint32_t c2 = 0; c2 < kBuckets; ++c2) {
+      bo[c2] = sum;
+      L.bfill[c2] = sum;
+      sum += L.bcount[c2];
+    }
+    bo[kBuckets] = sum;
+  }
+  __syncthreads();
   uint32_t* bucket = d.bucket + W.pair_off;
   for (uint32_t p = tid; p < P; p += BLOCK) {
     uint32_t cc = color[p];
-    if (cc < kColors) bucket[atomicAdd(&L.bfill[cc], 1u)] = W.pair_off + p;
+    if (cc >= kColors) continue;
+    int4 h = d.man_cur.hdr[W.pair_off + p];
+    if (!pgs_solvable(d, off, h)) continue;
+    bucket[atomicAdd(&L.bfill[cc], 1u)] = W.pair_off + p;
   }
   if (L.bcount[kColors]) {
     if (tid == 0) L.carry = L.bfill[kColors];
     __syncthreads();
     for (uint32_t c0 = 0; c0 < P; c0 += BLOCK) {
       uint32_t p = c0 + tid;
-      uint32_t f = (p < P && color[p] != kMAX - 1 && color[p] >= kColors) ? 1u : 0u;
+      uint32_t f = 0;
+      if (p < P && color[p] != kMAX - 1 && color[p] >= kColors)
+        f = pgs_solvable(d, off, d.man_cur.hdr[W.pair_off + p]) ? 1u : 0u;
       uint32_t tot;
       uint32_t ex = block_scan<BLOCK>(f, L.scan, tot);
       if (f) bucket[L.carry + ex] = W.pair_off + p;
@@ -1427,114 +1451,166 @@ __global__ __launch_bounds__(BLOCK) void k_post(Dev d) {
       __syncthreads();
     }
   }
+  // ---- world inverse inertia of every row, then the solver rows
+  iw_refresh<BLOCK>(d, W);
+  __syncthreads();
+  const uint32_t total = d.bucket_off[w * (kBuckets + 1) + kBuckets];
+  const float inv_dt = W.dt > 0.0f ? 1.0f / W.dt : 0.0f;
+  const float hertz = fmn(30.0f, inv_dt * 0.3333333333333333f);
+  for (uint32_t k = tid; k < total; k += BLOCK) pgs_compact(d, W, W.pair_off + k, bucket[k], hertz);
 }
 
 // ================================================================== k_pgs
 // PGS_SOFT::pre_step / solve (src/shaders/simulation/solvers.slang:142-279),
-// colour by colour (RB_sim.slang:1408-1578); one thread per manifold, bodies
-// of one colour are disjoint, statics are never written.
+// colour by colour (RB_sim.slang:1408-1578).  After colouring, the solvable
+// manifolds are compacted into colour-sorted solver rows: row k holds the
+// manifold constants, and contact c of row k lives at c*NP + k, so one sweep
+// over a colour streams contiguous memory.  The velocity-independent pre-step
+// quantities (arms, effective masses, soft coefficients) are computed once
+// at compaction; the warm-start impulses are then applied colour by colour.
+// Statics are never written; rows of one colour share no dynamic body.
 struct Pgs {
   bool accum, friction;
-  float dt, inv_dt, hertz;
+  float inv_dt;
 };
 
 BB_DI mat3 iw_load(const Dev& d, uint32_t g) {
   return mrows(ld3(d.iw[g * 3 + 0]), ld3(d.iw[g * 3 + 1]), ld3(d.iw[g * 3 + 2]));
 }
 
-BB_DI void pgs_manifold(const Dev& d, uint32_t off, uint32_t slot, const Pgs& s, int kind) {
-  int4 h = d.man_cur.hdr[slot];
-  const uint32_t ga = off + h.x, gb = off + h.y;
-  const uint32_t fa = d.cur.flags[ga], fb = d.cur.flags[gb];
+// Caches R*I^-1*R^T (or the quirk form) per row with inv_mass in row0.w.
+template <int BLOCK>
+BB_DI void iw_refresh(const Dev& d, const WorldInfo& W) {
+  const bool quirk = (W.quirks & BB_QUIRK_TRANSPOSED_WORLD_INERTIA) != 0;
+  for (uint32_t i = threadIdx.x; i < W.n; i += BLOCK) {
+    const uint32_t g = W.body_off + i;
+    const uint32_t id = d.cur.id[g];
+    mat3 M = world_inv_inertia(d, W.body_off + id, ldq(d.cur.rot[g]), quirk);
+    d.iw[g * 3 + 0] = st3(mrow(M, 0), d.props.mass[W.body_off + id].y);
+    d.iw[g * 3 + 1] = st3(mrow(M, 1), 0.0f);
+    d.iw[g * 3 + 2] = st3(mrow(M, 2), 0.0f);
+  }
+}
+
+BB_DI bool pgs_solvable(const Dev& d, uint32_t off, int4 h) {
+  const uint32_t fa = d.cur.flags[off + h.x], fb = d.cur.flags[off + h.y];
   const bool awa = (fa & BB_BODY_DYNAMIC) && !(fa & BB_BODY_SLEEPING);
   const bool awb = (fb & BB_BODY_DYNAMIC) && !(fb & BB_BODY_SLEEPING);
-  if (!awa && !awb) return;
-  const bool da = (fa & BB_BODY_DYNAMIC) != 0, db = (fb & BB_BODY_DYNAMIC) != 0;
-  const uint32_t ida = d.cur.id[ga], idb = d.cur.id[gb];
-  const float4 ma = d.props.mass[off + ida], mb = d.props.mass[off + idb];
-  const float ima = ma.y, imb = mb.y;
+  return awa || awb;
+}
+
+// Solver row k from manifold slot: the constants of PGS_SOFT::pre_step.
+BB_DI void pgs_compact(const Dev& d, const WorldInfo& W, uint32_t k, uint32_t slot, float hertz) {
+  const uint32_t off = W.body_off, NP = d.np;
+  const int4 h = d.man_cur.hdr[slot];
+  const uint32_t ga = off + h.x, gb = off + h.y;
+  const bool da = (d.cur.flags[ga] & BB_BODY_DYNAMIC) != 0, db = (d.cur.flags[gb] & BB_BODY_DYNAMIC) != 0;
+  const float4 ia0 = d.iw[ga * 3], ib0 = d.iw[gb * 3];
+  const float ima = ia0.w, imb = ib0.w;
   const mat3 Ia = iw_load(d, ga), Ib = iw_load(d, gb);
-  v3 va = ld3(d.cur.vel[ga]), wa = ld3(d.cur.omg[ga]);
-  v3 vb = ld3(d.cur.vel[gb]), wb = ld3(d.cur.omg[gb]);
+  const float fa = d.props.mass[off + d.cur.id[ga]].z, fb = d.props.mass[off + d.cur.id[gb]].z;
   const v3 n = ld3(d.man_cur.nrm[slot]);
   const v3 t0 = ld3(d.man_cur.t0[slot]), t1 = ld3(d.man_cur.t1[slot]);
   const v3 pa = ld3(d.man_cur.anca[slot]), pb = ld3(d.man_cur.ancb[slot]);
+  const float ch = (ima == 0.0f || imb == 0.0f) ? 2.0f * hertz : hertz;
+  const float zeta = 10.0f;
+  const float omega = 2.0f * 3.14159265359f * ch;
+  const float cc = W.dt * omega * (2.0f * zeta + W.dt * omega);
+  const float bias_factor = omega / (2.0f * zeta + W.dt * omega);
+  const float impulse_coef = 1.0f / (1.0f + cc);
+  const float mass_coef = cc * impulse_coef;
+  for (int i = 0; i < h.w; ++i) {
+    const float4 cp = d.man_cur.cpos[slot * 8 + i];
+    const float4 lam = d.man_cur.clam[slot * 8 + i];
+    const v3 r1 = ld3(cp) - pa, r2 = ld3(cp) - pb;
+    const v3 rn1 = cross(r1, n), rn2 = cross(r2, n);
+    float kn = ima + imb;
+    kn += dot(rn1, mulmv(Ia, rn1));
+    kn += dot(rn2, mulmv(Ib, rn2));
+    const float nm = kn == 0.0f ? 0.0f : 1.0f / kn;
+    float tm[2];
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const v3 tj = j == 0 ? t0 : t1;
+      const v3 rt1 = cross(r1, tj), rt2 = cross(r2, tj);
+      float kt = ima + imb;
+      kt += dot(rt1, mulmv(Ia, rt1));
+      kt += dot(rt2, mulmv(Ib, rt2));
+      tm[j] = kt == 0.0f ? 0.0f : 1.0f / kt;
+    }
+    d.sv_r1[i * NP + k] = st3(r1, cp.w);
+    d.sv_r2[i * NP + k] = st3(r2, nm);
+    d.sv_l[i * NP + k] = make_float4(lam.x, lam.y, lam.z, tm[0]);
+    d.sv_tm1[i * NP + k] = tm[1];
+  }
+  d.sv_hdr[k] = make_int4(h.x, h.y, h.w | (da ? 256 : 0) | (db ? 512 : 0), __float_as_int(mass_coef));
+  d.sv_n[k] = st3(n, __builtin_sqrtf(fa * fb));
+  d.sv_t0[k] = st3(t0, bias_factor);
+  d.sv_t1[k] = st3(t1, impulse_coef);
+}
+
+// kind 0: warm-start impulses of pre_step; 1: solve with bias; 2: relax.
+BB_DI void pgs_row(const Dev& d, uint32_t off, uint32_t k, const Pgs& s, int kind) {
+  const uint32_t NP = d.np;
+  const int4 h = d.sv_hdr[k];
+  const int count = h.z & 255;
+  const bool da = (h.z & 256) != 0, db = (h.z & 512) != 0;
+  const uint32_t ga = off + h.x, gb = off + h.y;
+  const float4 ia0 = d.iw[ga * 3], ib0 = d.iw[gb * 3];
+  const float ima = ia0.w, imb = ib0.w;
+  const mat3 Ia = mrows(ld3(ia0), ld3(d.iw[ga * 3 + 1]), ld3(d.iw[ga * 3 + 2]));
+  const mat3 Ib = mrows(ld3(ib0), ld3(d.iw[gb * 3 + 1]), ld3(d.iw[gb * 3 + 2]));
+  v3 va = ld3(d.cur.vel[ga]), wa = ld3(d.cur.omg[ga]);
+  v3 vb = ld3(d.cur.vel[gb]), wb = ld3(d.cur.omg[gb]);
+  const float4 nn = d.sv_n[k], tt0 = d.sv_t0[k], tt1 = d.sv_t1[k];
+  const v3 n = ld3(nn), t0 = ld3(tt0), t1 = ld3(tt1);
   if (kind == 0) {
-    for (int i = 0; i < h.w; ++i) {
-      const uint32_t cs = slot * 8 + i;
-      const float4 cp = d.man_cur.cpos[cs];
-      float4 lam = d.man_cur.clam[cs];
-      const v3 r1 = ld3(cp) - pa, r2 = ld3(cp) - pb;
-      v3 rn1 = cross(r1, n), rn2 = cross(r2, n);
-      float kn = ima + imb;
-      kn += dot(rn1, mulmv(Ia, rn1));
-      kn += dot(rn2, mulmv(Ib, rn2));
-      float nm = kn == 0.0f ? 0.0f : 1.0f / kn;
-      float tm[2];
-      for (int j = 0; j < 2; ++j) {
-        v3 tj = j == 0 ? t0 : t1;
-        v3 rt1 = cross(r1, tj), rt2 = cross(r2, tj);
-        float kt = ima + imb;
-        kt += dot(rt1, mulmv(Ia, rt1));
-        kt += dot(rt2, mulmv(Ib, rt2));
-        tm[j] = kt == 0.0f ? 0.0f : 1.0f / kt;
+    if (!s.accum) return;
+    for (int i = 0; i < count; ++i) {
+      const float4 l = d.sv_l[i * NP + k];
+      const v3 r1 = ld3(d.sv_r1[i * NP + k]), r2 = ld3(d.sv_r2[i * NP + k]);
+      v3 P = n * l.x;
+      if (s.friction) {
+        P = P + t0 * l.y;
+        P = P + t1 * l.z;
       }
-      float ch = (ima == 0.0f || imb == 0.0f) ? 2.0f * s.hertz : s.hertz;
-      const float zeta = 10.0f;
-      const float omega = 2.0f * 3.14159265359f * ch;
-      const float cc = s.dt * omega * (2.0f * zeta + s.dt * omega);
-      float bias_factor = omega / (2.0f * zeta + s.dt * omega);
-      float impulse_coef = 1.0f / (1.0f + cc);
-      float mass_coef = cc * impulse_coef;
-      lam.w = mass_coef;
-      d.man_cur.clam[cs] = lam;
-      d.man_cur.cmas[cs] = make_float4(nm, tm[0], tm[1], bias_factor);
-      float4 ana = d.man_cur.cana[cs];
-      ana.w = impulse_coef;
-      d.man_cur.cana[cs] = ana;
-      if (s.accum) {
-        v3 P = n * lam.x;
-        if (s.friction) {
-          P = P + t0 * lam.y;
-          P = P + t1 * lam.z;
-        }
-        if (da) {
-          va = va - P * ima;
-          wa = wa - mulmv(Ia, cross(r1, P));
-        }
-        if (db) {
-          vb = vb + P * imb;
-          wb = wb + mulmv(Ib, cross(r2, P));
-        }
+      if (da) {
+        va = va - P * ima;
+        wa = wa - mulmv(Ia, cross(r1, P));
+      }
+      if (db) {
+        vb = vb + P * imb;
+        wb = wb + mulmv(Ib, cross(r2, P));
       }
     }
   } else {
     const bool use_bias = kind == 1;
-    const float fr = __builtin_sqrtf(ma.z * mb.z);
-    for (int k = 0; k < h.w; ++k) {
-      const uint32_t cs = slot * 8 + k;
-      const float4 cp = d.man_cur.cpos[cs];
-      float4 lam = d.man_cur.clam[cs];
-      const float4 cm = d.man_cur.cmas[cs];
-      const float impulse_coef = d.man_cur.cana[cs].w;
-      const v3 r1 = ld3(cp) - pa, r2 = ld3(cp) - pb;
+    const float fr = nn.w, bias_factor = tt0.w, impulse_coef = tt1.w;
+    const float mass_coef = __int_as_float(h.w);
+    for (int c = 0; c < count; ++c) {
+      const float4 c1 = d.sv_r1[c * NP + k], c2 = d.sv_r2[c * NP + k];
+      float4 l = d.sv_l[c * NP + k];
+      const float tm1 = d.sv_tm1[c * NP + k];
+      const v3 r1 = ld3(c1), r2 = ld3(c2);
+      const float pen = c1.w, nm = c2.w;
       v3 rv = ((vb + cross(wb, r2)) - va) - cross(wa, r1);
       if (s.friction) {
+#pragma unroll
         for (int j = 0; j < 2; ++j) {
-          v3 tj = j == 0 ? t0 : t1;
-          float tmass = j == 0 ? cm.y : cm.z;
-          float tv = dot(rv, tj);
+          const v3 tj = j == 0 ? t0 : t1;
+          const float tmass = j == 0 ? l.w : tm1;
+          const float tv = dot(rv, tj);
           float ti = tmass * -tv;
-          float maxf = fr * lam.x;
+          const float maxf = fr * l.x;
           if (s.accum) {
-            float old = j == 0 ? lam.y : lam.z;
-            float nw = fcl(old + ti, -maxf, maxf);
-            if (j == 0) lam.y = nw; else lam.z = nw;
+            const float old = j == 0 ? l.y : l.z;
+            const float nw = fcl(old + ti, -maxf, maxf);
+            if (j == 0) l.y = nw; else l.z = nw;
             ti = nw - old;
           } else {
             ti = fcl(ti, -maxf, maxf);
           }
-          v3 tiv = tj * ti;
+          const v3 tiv = tj * ti;
           if (da) {
             va = va - tiv * ima;
             wa = wa - mulmv(Ia, cross(r1, tiv));
@@ -1546,24 +1622,24 @@ BB_DI void pgs_manifold(const Dev& d, uint32_t off, uint32_t slot, const Pgs& s,
         }
       }
       rv = ((vb + cross(wb, r2)) - va) - cross(wa, r1);
-      float nv = dot(rv, n);
+      const float nv = dot(rv, n);
       float bias = 0.0f, mass_scale = 1.0f, impulse_scale = 0.0f;
-      if (cp.w > 0.0f) {
-        bias = cp.w * s.inv_dt;
+      if (pen > 0.0f) {
+        bias = pen * s.inv_dt;
       } else if (use_bias) {
-        bias = fmx(cm.w * cp.w, -4.0f);
-        mass_scale = lam.w;
+        bias = fmx(bias_factor * pen, -4.0f);
+        mass_scale = mass_coef;
         impulse_scale = impulse_coef;
       }
-      float imp = -cm.x * mass_scale * (nv + bias) - impulse_scale * lam.x;
+      float imp = -nm * mass_scale * (nv + bias) - impulse_scale * l.x;
       if (s.accum) {
-        float old = lam.x;
-        lam.x = fmx(old + imp, 0.0f);
-        imp = lam.x - old;
+        const float old = l.x;
+        l.x = fmx(old + imp, 0.0f);
+        imp = l.x - old;
       } else {
         imp = fmx(imp, 0.0f);
       }
-      v3 niv = n * imp;
+      const v3 niv = n * imp;
       if (da) {
         va = va - niv * ima;
         wa = wa - mulmv(Ia, cross(r1, niv));
@@ -1572,7 +1648,7 @@ BB_DI void pgs_manifold(const Dev& d, uint32_t off, uint32_t slot, const Pgs& s,
         vb = vb + niv * imb;
         wb = wb + mulmv(Ib, cross(r2, niv));
       }
-      d.man_cur.clam[cs] = lam;
+      d.sv_l[c * NP + k] = l;
     }
   }
   if (da) {
@@ -1582,18 +1658,6 @@ BB_DI void pgs_manifold(const Dev& d, uint32_t off, uint32_t slot, const Pgs& s,
   if (db) {
     d.cur.vel[gb] = st3(vb, 0.0f);
     d.cur.omg[gb] = st3(wb, 0.0f);
-  }
-}
-
-template <int BLOCK>
-BB_DI void iw_refresh(const Dev& d, const WorldInfo& W) {
-  const bool quirk = (W.quirks & BB_QUIRK_TRANSPOSED_WORLD_INERTIA) != 0;
-  for (uint32_t i = threadIdx.x; i < W.n; i += BLOCK) {
-    const uint32_t g = W.body_off + i;
-    mat3 M = world_inv_inertia(d, W.body_off + d.cur.id[g], ldq(d.cur.rot[g]), quirk);
-    d.iw[g * 3 + 0] = st3(mrow(M, 0), 0.0f);
-    d.iw[g * 3 + 1] = st3(mrow(M, 1), 0.0f);
-    d.iw[g * 3 + 2] = st3(mrow(M, 2), 0.0f);
   }
 }
 
@@ -1616,27 +1680,23 @@ __global__ __launch_bounds__(BLOCK) void k_pgs(Dev d) {
   const uint32_t w = blockIdx.x, tid = threadIdx.x;
   const WorldInfo W = d.worlds[w];
   if (W.solver == BB_SOLVER_AVBD || W.n == 0) return;
-  const uint32_t off = W.body_off, N = W.n, A = W.a;
+  const uint32_t off = W.body_off, N = W.n, A = W.a, NP = d.np;
   Pgs s;
   s.accum = (W.flags & BB_SIM_ACCUM_IMPULSE) != 0;
   s.friction = (W.flags & BB_SIM_FRICTION) != 0;
-  s.dt = W.dt;
   s.inv_dt = W.dt > 0.0f ? 1.0f / W.dt : 0.0f;
-  s.hertz = fmn(30.0f, s.inv_dt * 0.3333333333333333f);
   const uint32_t* bo = d.bucket_off + w * (kBuckets + 1);
-  const uint32_t* bucket = d.bucket + W.pair_off;
   const uint32_t used = d.stats[w * ST_COUNT + ST_GC_COUNT];
+  const uint32_t base = W.pair_off;
   const uint32_t ovf_lo = bo[kColors], ovf_hi = bo[kBuckets];
-  iw_refresh<BLOCK>(d, W);
-  __syncthreads();
   auto sweep = [&](int kind) {
     for (uint32_t c = 0; c < used; ++c) {
-      for (uint32_t k = bo[c] + tid; k < bo[c + 1]; k += BLOCK) pgs_manifold(d, off, bucket[k], s, kind);
+      for (uint32_t k = bo[c] + tid; k < bo[c + 1]; k += BLOCK) pgs_row(d, off, base + k, s, kind);
       __syncthreads();
     }
     if (ovf_hi > ovf_lo) {
       if (tid == 0)
-        for (uint32_t k = ovf_lo; k < ovf_hi; ++k) pgs_manifold(d, off, bucket[k], s, kind);
+        for (uint32_t k = ovf_lo; k < ovf_hi; ++k) pgs_row(d, off, base + k, s, kind);
       __syncthreads();
     }
   };
@@ -1662,6 +1722,23 @@ __global__ __launch_bounds__(BLOCK) void k_pgs(Dev d) {
   iw_refresh<BLOCK>(d, W);
   __syncthreads();
   for (uint32_t it = 0; it < W.iterations; ++it) sweep(2);
+  // impulses and pre-step quantities back into the manifold slots (warm start
+  // of the next step, taps)
+  const uint32_t* bucket = d.bucket + base;
+  for (uint32_t k = tid; k < ovf_hi; k += BLOCK) {
+    const uint32_t row = base + k, slot = bucket[k];
+    const int4 h = d.sv_hdr[row];
+    const float mass_coef = __int_as_float(h.w);
+    const float bias_factor = d.sv_t0[row].w, impulse_coef = d.sv_t1[row].w;
+    for (int c = 0; c < (h.z & 255); ++c) {
+      const float4 l = d.sv_l[c * NP + row];
+      d.man_cur.clam[slot * 8 + c] = make_float4(l.x, l.y, l.z, mass_coef);
+      d.man_cur.cmas[slot * 8 + c] = make_float4(d.sv_r2[c * NP + row].w, l.w, d.sv_tm1[c * NP + row], bias_factor);
+      float4 a = d.man_cur.cana[slot * 8 + c];
+      a.w = impulse_coef;
+      d.man_cur.cana[slot * 8 + c] = a;
+    }
+  }
 }
 
 // ================================================================== AVBD
@@ -2289,6 +2366,34 @@ static void launch_all(const Dev& d, hipStream_t st) {
   hipLaunchKernelGGL(k_pgs<B>, g, b, 0, st, d);
   hipLaunchKernelGGL(k_avbd<B>, g, b, 0, st, d);
   hipLaunchKernelGGL(k_finish<B>, g, b, 0, st, d);
+}
+
+template <int B>
+static void launch_all_timed(const Dev& d, hipStream_t st, hipEvent_t* ev) {
+  dim3 g(d.W), b(B);
+  (void)hipEventRecord(ev[0], st);
+  hipLaunchKernelGGL(k_build<B>, g, b, 0, st, d);
+  (void)hipEventRecord(ev[1], st);
+  hipLaunchKernelGGL(k_broad<B>, g, b, 0, st, d);
+  (void)hipEventRecord(ev[2], st);
+  hipLaunchKernelGGL(k_narrow<B>, g, b, 0, st, d);
+  (void)hipEventRecord(ev[3], st);
+  hipLaunchKernelGGL(k_post<B>, g, b, 0, st, d);
+  (void)hipEventRecord(ev[4], st);
+  hipLaunchKernelGGL(k_pgs<B>, g, b, 0, st, d);
+  (void)hipEventRecord(ev[5], st);
+  hipLaunchKernelGGL(k_avbd<B>, g, b, 0, st, d);
+  (void)hipEventRecord(ev[6], st);
+  hipLaunchKernelGGL(k_finish<B>, g, b, 0, st, d);
+  (void)hipEventRecord(ev[7], st);
+}
+
+void launch_step_timed(const Dev& d, hipStream_t st, uint32_t block, hipEvent_t* ev) {
+  switch (block) {
+    case 512: launch_all_timed<512>(d, st, ev); break;
+    case 1024: launch_all_timed<1024>(d, st, ev); break;
+    default: launch_all_timed<256>(d, st, ev); break;
+  }
 }
 
 void launch_step(const Dev& d, hipStream_t st, uint32_t block) {

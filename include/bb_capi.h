@@ -302,6 +302,25 @@ bb_status bb_batch_debug_tap(bb_batch* b, uint32_t world, bb_tap kind,
                              void* dst, size_t cap, size_t* written);
 uint32_t bb_batch_world_count(const bb_batch* b);
 uint32_t bb_batch_body_count(const bb_batch* b);
+/* Stage kernels of one step, in launch order (bb_batch_profile indices). */
+enum {
+  BB_KERNEL_BUILD = 0,  /* Morton keys, radix sort, LBVH, reorder          */
+  BB_KERNEL_BROAD = 1,  /* BVH traversal, pair lists                       */
+  BB_KERNEL_NARROW = 2, /* contacts, warm start, union-find                */
+  BB_KERNEL_POST = 3,   /* advect, islands, sleeping, edge colouring       */
+  BB_KERNEL_PGS = 4,    /* PGS_SOFT colour sweeps + integration            */
+  BB_KERNEL_AVBD = 5,   /* AVBD colouring and sweeps                       */
+  BB_KERNEL_FINISH = 6, /* end-of-step counters                            */
+  BB_KERNEL_COUNT = 7
+};
+/* Runs nsteps eager steps with hipEvents recorded on the library stream
+ * around every stage kernel.  step_ms[nsteps] receives each step's duration,
+ * kernel_ms[BB_KERNEL_COUNT] the summed duration of each stage over the steps.
+ * Either output may be NULL. */
+bb_status bb_batch_profile(bb_batch* b, uint32_t nsteps, float* step_ms, float* kernel_ms);
+/* Device bytes held by the batch. */
+uint64_t bb_batch_device_bytes(const bb_batch* b);
+
 /* Block size used by the per-world kernels (autotuner hook); 0 = automatic. */
 bb_status bb_batch_set_block_size(bb_batch* b, uint32_t threads);
 void bb_batch_destroy(bb_batch* b);

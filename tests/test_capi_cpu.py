@@ -1,0 +1,126 @@
+"""C-ABI checks that need no GPU: every symbol declared in include/bb_capi.h
+is exported, descriptors are validated, capacity is enforced, and stepping
+without a device fails loudly (there is no CPU fallback)."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+import beatbox
+import scenes
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_functions():
+    src = open(os.path.join(REPO, "include", "bb_capi.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(bb_[a-z0-9_]+)\s*\(", src)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = beatbox.lib()
+    names = header_functions()
+    assert len(names) >= 25
+    for n in names:
+        assert hasattr(lib, n), n
+    assert set(beatbox.EXPORTS) <= set(names)
+
+
+def test_struct_layouts_match_c_compiler(tmp_path):
+    """The ctypes/numpy mirrors of the header structs have the sizes and field
+    offsets gcc gives them."""
+    src = tmp_path / "sizes.c"
+    src.write_text('''#include <stdio.h>
+#include <stddef.h>
+#include "bb_capi.h"
+int main(void) {
+  printf("%zu %zu %zu %zu %zu %zu %zu\\n", sizeof(bb_world_desc), sizeof(bb_body_desc),
+         sizeof(bb_body_state), sizeof(bb_step_stats), sizeof(bb_lbvh_node),
+         sizeof(bb_manifold_tap), sizeof(bb_contact_tap));
+  printf("%zu %zu %zu\\n", offsetof(bb_body_desc, friction), offsetof(bb_manifold_tap, contacts),
+         offsetof(bb_step_stats, step_index));
+  return 0;
+}
+''')
+    exe = tmp_path / "sizes"
+    import subprocess
+    subprocess.run(["gcc", "-I", os.path.join(REPO, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()
+    sizes = [int(x) for x in out]
+    assert sizes[0] == C.sizeof(beatbox.WorldDesc)
+    assert sizes[1] == scenes.BODY_DESC.itemsize
+    assert sizes[2] == beatbox.BODY_STATE.itemsize == C.sizeof(beatbox.BodyState)
+    assert sizes[3] == C.sizeof(beatbox.StepStats)
+    assert sizes[4] == beatbox.LBVH_NODE.itemsize
+    assert sizes[5] == beatbox.MANIFOLD_TAP.itemsize
+    assert sizes[6] == beatbox.CONTACT_TAP.itemsize
+    assert sizes[7] == scenes.BODY_DESC.fields["friction"][1]
+    assert sizes[8] == beatbox.MANIFOLD_TAP.fields["contacts"][1]
+    assert sizes[9] == beatbox.StepStats.step_index.offset
+
+
+def test_defaults_follow_reference():
+    d = beatbox.default_desc()
+    assert abs(d.dt - 0.01667) < 1e-9 and abs(d.gravity + 9.81) < 1e-6
+    assert d.iterations == 5 and d.solver == 2 and d.morton_bits == 30
+    assert d.flags == (8 | 16 | 2 | 1 | 4 | 32 | 128)
+
+
+def test_invalid_descriptors_rejected():
+    w = beatbox.RigidBodyManager(beatbox.WorldDesc.make(**scenes.world_desc()))
+    bad = scenes.random_boxes(1, 2)
+    bad["maximum"][1] = bad["minimum"][1]  # degenerate box
+    with pytest.raises(beatbox.BBError) as e:
+        w.add_bodies(bad)
+    assert e.value.code == 1
+    nan = scenes.random_boxes(1, 2)
+    nan["position"][1][0] = np.nan
+    with pytest.raises(beatbox.BBError):
+        w.add_bodies(nan)
+    with pytest.raises(beatbox.BBError):
+        beatbox.RigidBodyManager(beatbox.WorldDesc.make(**scenes.world_desc(morton_bits=31)))
+    with pytest.raises(beatbox.BBError):
+        beatbox.RigidBodyManager(beatbox.WorldDesc.make(**scenes.world_desc(dt=0.0)))
+
+
+def test_parity_mode_keeps_reference_cap():
+    # MAX_RIGID_BODY_COUNT = 1024 (include/defines.hpp:357): build_accel_structs
+    # refuses larger scenes; nothing is added
+    w = beatbox.RigidBodyManager(beatbox.WorldDesc.make(**scenes.world_desc()))
+    ids = w.add_bodies(scenes.random_boxes(2, 1000))
+    assert len(ids) == 1001 and ids[0] == 0 and ids[-1] == 1000
+    with pytest.raises(beatbox.BBError) as e:
+        w.add_bodies(scenes.random_boxes(3, 30, with_ground=False))
+    assert e.value.code == 2
+    assert beatbox.lib().bb_world_body_count(w._h) == 1001
+    # extended mode (63-bit keys, quirks off) has no such cap
+    x = beatbox.RigidBodyManager(beatbox.WorldDesc.make(**scenes.world_desc(morton_bits=63, quirks=0)))
+    assert len(x.add_bodies(scenes.random_boxes(4, 3000))) == 3001
+
+
+def test_explicit_capacity():
+    w = beatbox.RigidBodyManager(beatbox.WorldDesc.make(**scenes.world_desc(capacity=10)))
+    with pytest.raises(beatbox.BBError) as e:
+        w.add_bodies(scenes.random_boxes(5, 20))
+    assert e.value.code == 2
+
+
+def test_flags_mirror_reference_setters():
+    w = beatbox.RigidBodyManager(beatbox.WorldDesc.make(**scenes.world_desc(flags=0)))
+    assert w.set_sim_flags(4 | 8) == 12
+    assert w.set_sim_flags(1) == 13
+    assert w.clear_sim_flags(4) == 9
+
+
+@pytest.mark.skipif(os.path.exists("/dev/kfd"), reason="a GPU is visible")
+def test_step_without_gpu_fails_loudly():
+    w = beatbox.RigidBodyManager(beatbox.WorldDesc.make(**scenes.world_desc()))
+    w.add_bodies(scenes.random_boxes(6, 5))
+    with pytest.raises(beatbox.BBError) as e:
+        w.simulate(1)
+    assert e.value.code == 4  # BB_ERR_NO_DEVICE
