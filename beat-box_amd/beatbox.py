@@ -147,6 +147,7 @@ EXPORTS = {
     "bb_batch_device_bytes": (C.c_uint64, [C.c_void_p]),
     "bb_batch_destroy": (None, [C.c_void_p]),
     "bb_version": (C.c_char_p, []),
+    "bb_copy_bandwidth": (C.c_int, [C.c_uint64, C.c_uint32, C.POINTER(C.c_float)]),
 }
 
 
@@ -164,6 +165,13 @@ def lib() -> C.CDLL:
             f.argtypes = args
         _LIB = L
     return _LIB
+
+
+def copy_bandwidth(nbytes: int, reps: int = 5) -> float:
+    """Mean duration (ms) of a device-to-device float4 copy of nbytes."""
+    ms = C.c_float(0.0)
+    _check(lib().bb_copy_bandwidth(nbytes, reps, C.byref(ms)), "bb_copy_bandwidth")
+    return ms.value
 
 
 def default_desc() -> WorldDesc:

@@ -19,6 +19,7 @@
 namespace bb {
 void launch_step(const Dev& d, hipStream_t st, uint32_t block);
 void launch_step_timed(const Dev& d, hipStream_t st, uint32_t block, hipEvent_t* ev);
+void launch_copy(const float4* src, float4* dst, size_t n, hipStream_t st);
 }  // namespace bb
 
 using namespace bb;
@@ -827,6 +828,36 @@ bool valid_body(const bb_body_desc& b) {
 extern "C" {
 
 const char* bb_version(void) { return "beatbox-amd 0.1 (gfx950)"; }
+
+bb_status bb_copy_bandwidth(uint64_t bytes, uint32_t reps, float* ms) {
+  if (!ms || bytes < 16 || reps == 0) return BB_ERR_INVALID;
+  if (!has_device()) return BB_ERR_NO_DEVICE;
+  const size_t n = bytes / 16;
+  float4 *src = nullptr, *dst = nullptr;
+  hipStream_t st = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  bb_status rc = BB_OK;
+  float total = 0.0f;
+  if (hipMalloc(&src, n * 16) != hipSuccess || hipMalloc(&dst, n * 16) != hipSuccess ||
+      hipMemset(src, 0, n * 16) != hipSuccess || hipStreamCreate(&st) != hipSuccess ||
+      hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+    rc = BB_ERR_DEVICE;
+  } else {
+    launch_copy(src, dst, n, st);  // warm-up
+    (void)hipEventRecord(e0, st);
+    for (uint32_t r = 0; r < reps; ++r) launch_copy(src, dst, n, st);
+    (void)hipEventRecord(e1, st);
+    if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&total, e0, e1) != hipSuccess)
+      rc = BB_ERR_DEVICE;
+  }
+  *ms = total / (float)reps;
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (st) (void)hipStreamDestroy(st);
+  if (src) (void)hipFree(src);
+  if (dst) (void)hipFree(dst);
+  return rc;
+}
 
 void bb_world_desc_default(bb_world_desc* d) {
   if (!d) return;

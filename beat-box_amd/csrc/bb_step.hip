@@ -2343,6 +2343,12 @@ __global__ __launch_bounds__(BLOCK) void k_finish(Dev d) {
   }
 }
 
+// Grid-stride float4 copy of known size (byte-counter calibration).
+__global__ __launch_bounds__(256) void k_copy_calibrate(const float4* __restrict__ src, float4* __restrict__ dst,
+                                                        size_t n) {
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) dst[i] = src[i];
+}
+
 }  // namespace bb
 
 // ============================================================ host launch
@@ -2394,6 +2400,10 @@ void launch_step_timed(const Dev& d, hipStream_t st, uint32_t block, hipEvent_t*
     case 1024: launch_all_timed<1024>(d, st, ev); break;
     default: launch_all_timed<256>(d, st, ev); break;
   }
+}
+
+void launch_copy(const float4* src, float4* dst, size_t n, hipStream_t st) {
+  hipLaunchKernelGGL(k_copy_calibrate, dim3(2048), dim3(256), 0, st, src, dst, n);
 }
 
 void launch_step(const Dev& d, hipStream_t st, uint32_t block) {

@@ -133,12 +133,24 @@ def main() -> int:
     ap.add_argument("--cpu-budget", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=6)
+    ap.add_argument("--plain-steps", type=int, default=0, help="0: max(20, steps/2)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run the copy kernel on MIB MiB (counter calibration) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: --gpus {args.gpus} is not supported (worlds shard "
               "across GPUs without exchange; this build measures one GPU's share)", file=sys.stderr)
         return 2
 
+    if args.copy_calibration:
+        nbytes = args.copy_calibration << 20
+        ms = beatbox.copy_bandwidth(nbytes, 5)
+        log(f"copy {args.copy_calibration} MiB: {ms:.3f} ms, {2 * nbytes / ms / 1e6:.1f} GB/s (read+write)")
+        return 0
+
+    copy_ms = beatbox.copy_bandwidth(1 << 30, 5)
+    copy_gbs = 2 * (1 << 30) / (copy_ms * 1e-3) / 1e9
+    log(f"copy kernel: {copy_gbs:.0f} GB/s (practical HBM ceiling)")
     batch, total = make_batch(args.worlds, args.seed, 16, 2048)
     t = time_batch(batch, total, args.steps, args.warmup)
     log(f"device bytes: {batch.device_bytes / 2**30:.2f} GiB")
@@ -160,7 +172,7 @@ def main() -> int:
               for n in beatbox.KERNEL_NAMES}
     log(f"stages: {stages}")
 
-    plain_bs, plain_ms = plain_world(max(20, args.steps // 2), 5)
+    plain_bs, plain_ms = plain_world(args.plain_steps or max(20, args.steps // 2), 5)
     log(f"plain: {plain_ms:.3f} ms/step, {plain_bs / 1e6:.3f} M body-steps/s, ratio {value / plain_bs:.1f}x")
 
     cpu = None if args.no_cpu else cpu_baseline(args.seed, args.cpu_budget)
@@ -192,6 +204,7 @@ def main() -> int:
             "batch_over_plain": value / plain_bs,
             "step_latency_ms_p50": p50,
             "step_latency_ms_p99": p99,
+            "copy_kernel_gbs": copy_gbs,
             "stages": stages,
         },
         "roofline": {

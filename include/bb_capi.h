@@ -325,6 +325,12 @@ uint64_t bb_batch_device_bytes(const bb_batch* b);
 bb_status bb_batch_set_block_size(bb_batch* b, uint32_t threads);
 void bb_batch_destroy(bb_batch* b);
 
+/* Measurement helper (no reference counterpart): copies `bytes` between two
+ * device buffers `reps` times with a grid-stride float4 copy kernel
+ * (k_copy_calibrate) and returns the mean kernel time in *ms.  Used to
+ * calibrate HBM byte counters and as the practical bandwidth ceiling. */
+bb_status bb_copy_bandwidth(uint64_t bytes, uint32_t reps, float* ms);
+
 /* Library version string. */
 const char* bb_version(void);
 
