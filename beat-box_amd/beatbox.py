@@ -205,6 +205,10 @@ class RigidBodyManager:
                "bb_world_add_bodies")
         return ids
 
+    # AccelerationStructureManager::build_accel_structs, the reference's name
+    # for the upload that load_scene ends with
+    build_accel_structs = add_bodies
+
     def simulate(self, nsteps: int = 1) -> None:
         _check(lib().bb_world_step(self._h, nsteps, None), "bb_world_step")
 
