@@ -93,6 +93,8 @@ TAP_ISLANDS, TAP_EDGE_COLORS, TAP_BODY_COLORS, TAP_ROWS = 5, 6, 7, 8
 ORDER_BY_ID, ORDER_BY_ROW = 0, 1
 KERNEL_NAMES = ("k_build", "k_broad", "k_narrow", "k_post", "k_pgs", "k_avbd", "k_finish")
 KERNEL_COUNT = len(KERNEL_NAMES)
+COUNT_NAMES = ("bodies", "active", "pairs", "manifolds", "contacts", "color_work", "max_pen_mm", "sleeping")
+COUNT_FIELDS = len(COUNT_NAMES)
 
 STATUS = {0: "BB_OK", 1: "BB_ERR_INVALID", 2: "BB_ERR_CAPACITY", 3: "BB_ERR_DEVICE",
           4: "BB_ERR_NO_DEVICE", 5: "BB_ERR_OVERFLOW", 6: "BB_ERR_STATE", 7: "BB_ERR_BUDGET"}
@@ -143,7 +145,9 @@ EXPORTS = {
     "bb_batch_world_count": (C.c_uint32, [C.c_void_p]),
     "bb_batch_body_count": (C.c_uint32, [C.c_void_p]),
     "bb_batch_set_block_size": (C.c_int, [C.c_void_p, C.c_uint32]),
-    "bb_batch_profile": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p]),
+    "bb_batch_profile": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "bb_batch_energy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32]),
+    "bb_world_energy": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
     "bb_batch_device_bytes": (C.c_uint64, [C.c_void_p]),
     "bb_batch_destroy": (None, [C.c_void_p]),
     "bb_version": (C.c_char_p, []),
@@ -250,6 +254,12 @@ class RigidBodyManager:
     def tap(self, kind: int, dtype) -> np.ndarray:
         return _tap(lambda dst, cap, wr: lib().bb_world_debug_tap(self._h, kind, dst, cap, wr), dtype)
 
+    def energy(self) -> float:
+        """Mechanical energy of the current state (float64)."""
+        e = C.c_double(0.0)
+        _check(lib().bb_world_energy(self._h, C.byref(e)), "bb_world_energy")
+        return e.value
+
     def destroy(self) -> None:
         if self._h:
             lib().bb_world_destroy(self._h)
@@ -293,14 +303,24 @@ class Batch:
     def sync(self) -> None:
         _check(lib().bb_batch_sync(self._h), "bb_batch_sync")
 
-    def profile(self, nsteps: int):
+    def profile(self, nsteps: int, counts: bool = False):
         """Eager steps timed with hipEvents on the library stream: returns
-        (per-step ms array, per-stage summed ms array)."""
+        (per-step ms array, per-stage summed ms array) and, with counts=True,
+        a (nsteps, COUNT_FIELDS) uint64 array of each step's batch totals."""
         step_ms = np.zeros(nsteps, np.float32)
         kern_ms = np.zeros(KERNEL_COUNT, np.float32)
+        cnt = np.zeros((nsteps, COUNT_FIELDS), np.uint64) if counts else None
         _check(lib().bb_batch_profile(self._h, nsteps, step_ms.ctypes.data_as(C.c_void_p),
-                                      kern_ms.ctypes.data_as(C.c_void_p)), "bb_batch_profile")
-        return step_ms, kern_ms
+                                      kern_ms.ctypes.data_as(C.c_void_p),
+                                      None if cnt is None else cnt.ctypes.data_as(C.c_void_p)),
+               "bb_batch_profile")
+        return (step_ms, kern_ms, cnt) if counts else (step_ms, kern_ms)
+
+    def energy(self) -> np.ndarray:
+        """Mechanical energy of each world's current state (float64)."""
+        out = np.zeros(max(self.world_count, 1), np.float64)
+        _check(lib().bb_batch_energy(self._h, out.ctypes.data_as(C.c_void_p), len(out)), "bb_batch_energy")
+        return out[: self.world_count]
 
     @property
     def device_bytes(self) -> int:

@@ -20,6 +20,8 @@ namespace bb {
 void launch_step(const Dev& d, hipStream_t st, uint32_t block);
 void launch_step_timed(const Dev& d, hipStream_t st, uint32_t block, hipEvent_t* ev);
 void launch_copy(const float4* src, float4* dst, size_t n, hipStream_t st);
+void launch_sum_counts(const Dev& d, unsigned long long* out, hipStream_t st);
+void launch_energy(const Dev& d, double* out, hipStream_t st);
 }  // namespace bb
 
 using namespace bb;
@@ -498,7 +500,7 @@ bb_status step_batch(bb_batch* b, uint32_t nsteps, bool use_graph) {
   return BB_OK;
 }
 
-bb_status profile_batch(bb_batch* b, uint32_t nsteps, float* step_ms, float* kernel_ms) {
+bb_status profile_batch(bb_batch* b, uint32_t nsteps, float* step_ms, float* kernel_ms, uint64_t* counts) {
   if (b->worlds.empty()) return BB_ERR_STATE;
   bb_status st = build(b);
   if (st != BB_OK) return st;
@@ -508,11 +510,17 @@ bb_status profile_batch(bb_batch* b, uint32_t nsteps, float* step_ms, float* ker
   }
   if (kernel_ms)
     for (int k = 0; k < BB_KERNEL_COUNT; ++k) kernel_ms[k] = 0.0f;
+  unsigned long long* dcounts = nullptr;
+  if (counts && nsteps &&
+      hipMalloc(&dcounts, (size_t)nsteps * BB_COUNT_FIELDS * sizeof(unsigned long long)) != hipSuccess)
+    return BB_ERR_DEVICE;
   std::vector<hipEvent_t> ev((size_t)nsteps * 8);
   for (auto& e : ev) HIPCHK(hipEventCreate(&e));
   for (uint32_t s = 0; s < nsteps; ++s) {
     const int cur = b->latest ^ 1;
-    launch_step_timed(dev_for(b, cur), b->stream, pick_block(b), &ev[(size_t)s * 8]);
+    const Dev d = dev_for(b, cur);
+    launch_step_timed(d, b->stream, pick_block(b), &ev[(size_t)s * 8]);
+    if (dcounts) launch_sum_counts(d, dcounts + (size_t)s * BB_COUNT_FIELDS, b->stream);
     b->latest = cur;
     b->stepped = true;
     ++b->step_index;
@@ -531,7 +539,28 @@ bb_status profile_batch(bb_batch* b, uint32_t nsteps, float* step_ms, float* ker
     }
   }
   for (auto& e : ev) (void)hipEventDestroy(e);
+  if (dcounts) {
+    const hipError_t rc =
+        hipMemcpy(counts, dcounts, (size_t)nsteps * BB_COUNT_FIELDS * sizeof(uint64_t), hipMemcpyDeviceToHost);
+    (void)hipFree(dcounts);
+    if (rc != hipSuccess) return BB_ERR_DEVICE;
+  }
   return BB_OK;
+}
+
+bb_status energy_batch(bb_batch* b, double* out, uint32_t n) {
+  if (!out || n < b->worlds.size()) return BB_ERR_INVALID;
+  if (b->worlds.empty()) return BB_OK;
+  bb_status st = build(b);
+  if (st != BB_OK) return st;
+  double* dout = nullptr;
+  if (hipMalloc(&dout, b->worlds.size() * sizeof(double)) != hipSuccess) return BB_ERR_DEVICE;
+  launch_energy(dev_for(b, b->latest), dout, b->stream);
+  hipError_t rc = hipGetLastError();
+  if (rc == hipSuccess) rc = hipStreamSynchronize(b->stream);
+  if (rc == hipSuccess) rc = hipMemcpy(out, dout, b->worlds.size() * sizeof(double), hipMemcpyDeviceToHost);
+  (void)hipFree(dout);
+  return rc == hipSuccess ? BB_OK : BB_ERR_DEVICE;
 }
 
 bb_status sync_batch(bb_batch* b) {
@@ -923,8 +952,12 @@ bb_status bb_batch_debug_tap(bb_batch* b, uint32_t w, bb_tap kind, void* dst, si
   return b ? tap_world(b, w, kind, dst, cap, written) : BB_ERR_INVALID;
 }
 
-bb_status bb_batch_profile(bb_batch* b, uint32_t nsteps, float* step_ms, float* kernel_ms) {
-  return b ? profile_batch(b, nsteps, step_ms, kernel_ms) : BB_ERR_INVALID;
+bb_status bb_batch_profile(bb_batch* b, uint32_t nsteps, float* step_ms, float* kernel_ms, uint64_t* counts) {
+  return b ? profile_batch(b, nsteps, step_ms, kernel_ms, counts) : BB_ERR_INVALID;
+}
+
+bb_status bb_batch_energy(bb_batch* b, double* per_world, uint32_t n) {
+  return b ? energy_batch(b, per_world, n) : BB_ERR_INVALID;
 }
 
 uint64_t bb_batch_device_bytes(const bb_batch* b) { return b ? b->buf.bytes : 0; }
@@ -1040,6 +1073,10 @@ bb_status bb_world_read_bodies(bb_world* w, bb_body_state* out, uint32_t n, bb_o
 
 bb_status bb_world_read_stats(bb_world* w, bb_step_stats* out) {
   return w ? world_stats(w->b, 0, out) : BB_ERR_INVALID;
+}
+
+bb_status bb_world_energy(bb_world* w, double* out) {
+  return w ? energy_batch(w->b, out, 1) : BB_ERR_INVALID;
 }
 
 bb_status bb_world_debug_tap(bb_world* w, bb_tap kind, void* dst, size_t cap, size_t* written) {

@@ -2343,6 +2343,87 @@ __global__ __launch_bounds__(BLOCK) void k_finish(Dev d) {
   }
 }
 
+// Batch totals of one step's counters, one workgroup: out[0..7] = sum N,
+// sum A, sum P, sum M, sum C, sum over worlds of (colours used x M), max
+// penetration (mm), sum sleeping.
+__global__ __launch_bounds__(1024) void k_sum_counts(const uint32_t* __restrict__ stats, uint32_t nw,
+                                                     unsigned long long* __restrict__ out) {
+  __shared__ unsigned long long red[8][1024 / 64];
+  unsigned long long acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (uint32_t w = threadIdx.x; w < nw; w += 1024) {
+    const uint32_t* s = stats + (size_t)w * ST_COUNT;
+    acc[0] += s[ST_N];
+    acc[1] += s[ST_A];
+    acc[2] += s[ST_P];
+    acc[3] += s[ST_M];
+    acc[4] += s[ST_C];
+    acc[5] += (unsigned long long)s[ST_GC_COUNT] * s[ST_M];
+    acc[6] = acc[6] > s[ST_PEN] ? acc[6] : s[ST_PEN];
+    acc[7] += s[ST_SLEEPING];
+  }
+  const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int f = 0; f < 8; ++f) {
+    unsigned long long v = acc[f];
+    for (int o = 32; o > 0; o >>= 1) {
+      const unsigned long long u = __shfl_down(v, o, 64);
+      v = f == 6 ? (v > u ? v : u) : v + u;
+    }
+    if (lane == 0) red[f][wv] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 8) {
+    const int f = threadIdx.x;
+    unsigned long long v = 0;
+    for (int k = 0; k < 1024 / 64; ++k) v = f == 6 ? (v > red[f][k] ? v : red[f][k]) : v + red[f][k];
+    out[f] = v;
+  }
+}
+
+// Mechanical energy per world of the current rows, in double:
+// sum over dynamic bodies of m|v|^2/2 + w_b.I_b.w_b/2 - m*g*y, with I_b the
+// physical inertia of the shape (box: m/12 diag(ey^2+ez^2, ...) over full
+// extents; sphere: 2/5 m r^2) and w_b the angular velocity in the body frame.
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_energy(Dev d, double* __restrict__ out) {
+  __shared__ double red[BLOCK / 64];
+  const uint32_t w = blockIdx.x, tid = threadIdx.x;
+  const WorldInfo W = d.worlds[w];
+  double e = 0.0;
+  for (uint32_t i = tid; i < W.n; i += BLOCK) {
+    const uint32_t g = W.body_off + i;
+    if (!(d.cur.flags[g] & BB_BODY_DYNAMIC)) continue;
+    const uint32_t pid = W.body_off + d.cur.id[g];
+    const float4 mp = d.props.mass[pid], lo = d.props.pmin[pid], hi = d.props.pmax[pid];
+    const double m = mp.x;
+    double ix, iy, iz;
+    if (hi.w != 0.0f) {
+      ix = iy = iz = 0.4 * m * (double)lo.w * (double)lo.w;
+    } else {
+      const double ex = (double)hi.x - lo.x, ey = (double)hi.y - lo.y, ez = (double)hi.z - lo.z;
+      ix = m / 12.0 * (ey * ey + ez * ez);
+      iy = m / 12.0 * (ex * ex + ez * ez);
+      iz = m / 12.0 * (ex * ex + ey * ey);
+    }
+    const float4 p = d.cur.pos[g], q = d.cur.rot[g], v = d.cur.vel[g], o = d.cur.omg[g];
+    // body-frame angular velocity: conj(q) * w * q
+    const double qx = q.x, qy = q.y, qz = q.z, qw = q.w, ox = o.x, oy = o.y, oz = o.z;
+    const double tx = 2.0 * (qy * oz - qz * oy), ty = 2.0 * (qz * ox - qx * oz), tz = 2.0 * (qx * oy - qy * ox);
+    const double bx = ox - qw * tx + (qy * tz - qz * ty);
+    const double by = oy - qw * ty + (qz * tx - qx * tz);
+    const double bz = oz - qw * tz + (qx * ty - qy * tx);
+    e += 0.5 * m * ((double)v.x * v.x + (double)v.y * v.y + (double)v.z * v.z) +
+         0.5 * (ix * bx * bx + iy * by * by + iz * bz * bz) - m * (double)W.gravity * (double)p.y;
+  }
+  for (int o = 32; o > 0; o >>= 1) e += __shfl_down(e, o, 64);
+  if ((tid & 63) == 0) red[tid >> 6] = e;
+  __syncthreads();
+  if (tid == 0) {
+    double s = 0.0;
+    for (int k = 0; k < BLOCK / 64; ++k) s += red[k];
+    out[w] = s;
+  }
+}
+
 // Grid-stride float4 copy of known size (byte-counter calibration).
 __global__ __launch_bounds__(256) void k_copy_calibrate(const float4* __restrict__ src, float4* __restrict__ dst,
                                                         size_t n) {
@@ -2404,6 +2485,14 @@ void launch_step_timed(const Dev& d, hipStream_t st, uint32_t block, hipEvent_t*
 
 void launch_copy(const float4* src, float4* dst, size_t n, hipStream_t st) {
   hipLaunchKernelGGL(k_copy_calibrate, dim3(2048), dim3(256), 0, st, src, dst, n);
+}
+
+void launch_sum_counts(const Dev& d, unsigned long long* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_sum_counts, dim3(1), dim3(1024), 0, st, d.stats, d.W, out);
+}
+
+void launch_energy(const Dev& d, double* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_energy<256>, dim3(d.W), dim3(256), 0, st, d, out);
 }
 
 void launch_step(const Dev& d, hipStream_t st, uint32_t block) {

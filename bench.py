@@ -3,12 +3,16 @@
 Workload (the configuration the headline metric is quoted on): 4,096
 independent worlds in one batch, body counts log-uniform in [16, 2048]
 (seed 0xBB04), each a box pyramid (even) or 4x4 tower (odd) on its own ground
-box, PGS_SOFT with 8 iterations, FP32, 63-bit Morton keys, hipGraph replay.
-Beside it: the plain single world (1,000 cubes + ground, PGS_SOFT I=8, parity
-mode) on the same GPU, and the CPU oracle on a bounded sample of the batch.
+box, PGS_SOFT with 8 iterations, FP32, 63-bit Morton keys, hipGraph replay,
+240 timed steps after 10 untimed ones.  Beside it: per-stage kernel durations
+over the same step window (hipEvents on the library stream, eager launches of
+a fresh batch that reaches the same states), the plain single world (1,000
+cubes + ground, PGS_SOFT I=8, parity mode, 200 steps) on the same GPU, the
+batch energy change and maximum penetration, and the CPU oracle on a bounded
+sample of the batch.
 
-Prints one JSON line (see the contract in the repository's driver notes):
-value = batch body-steps/s with all state resident in HBM.
+Prints one JSON line; value = batch body-steps/s with all state resident in
+HBM.
 """
 from __future__ import annotations
 
@@ -34,27 +38,24 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def stage_bytes(stats: list, iters: int, kw: int) -> dict:
-    """Algorithmic (compulsory) HBM bytes per step of each stage kernel, from
-    the per-unit figures of the byte model: N bodies, A active, P pairs,
-    M manifolds, C contacts, S = 84-byte compact state row, kw = key bytes."""
-    N = sum(s["rigid_body_count"] for s in stats)
-    A = sum(s["active_rigid_body_count"] for s in stats)
-    P = sum(s["broad_phase_collision_count"] for s in stats)
-    M = sum(s["collision_count"] for s in stats)
-    Cc = sum(s["contact_count"] for s in stats)
-    rounds = max((s["graph_color_count"] for s in stats), default=0)
+def stage_bytes(counts: np.ndarray, iters: int, kw: int) -> dict:
+    """Algorithmic (compulsory) HBM bytes of each stage kernel, summed over
+    the profiled steps, from the per-unit figures of the byte model and each
+    step's device-reported batch totals: N bodies, A active, P pairs,
+    M manifolds, C contacts, G = sum over worlds of colours used x M,
+    S = 84-byte compact state row, kw = key bytes."""
+    c = counts.astype(np.float64)
+    N, A, P, M, Cc, G = (c[:, i].sum() for i in range(6))
     S = 84
     passes = 4 if kw == 4 else 8
     return {
-        "counts": dict(N=N, A=A, P=P, M=M, C=Cc),
         "k_build": N * (12 + kw + 4) + passes * N * (3 * kw + 8) + N * (40 + 2 * kw)
-        + (2 * N - 1) * 36 + N * 72 + 2 * N * S,
+        + (2 * N - len(c)) * 36 + N * 72 + 2 * N * S,
         "k_broad": N * 24 + P * 8,
         "k_narrow": P * 96 + M * 48 + Cc * 112,
-        "k_post": M * 8 + A * 16 + rounds * M * 20 + N * 4,
+        "k_post": M * 8 + A * 16 + G * 20 + N * 4,
         "k_pgs": (1 + 2 * iters) * (M * (36 + 2 * S) + Cc * 60) + A * 112,
-        "k_avbd": 0,
+        "k_avbd": 0.0,
         "k_finish": N * 16,
     }
 
@@ -73,13 +74,17 @@ def make_batch(worlds: int, seed: int, lo: int, hi: int):
     return b, total
 
 
-def time_batch(b, total: int, steps: int, warmup: int) -> float:
+def time_batch(b, steps: int, warmup: int):
+    """Graph replay of `steps` steps after `warmup` untimed ones; returns the
+    wall time and the batch energy before and after the timed steps."""
     b.step(warmup, use_graph=True)
     b.sync()
+    e0 = float(b.energy().sum())
     t0 = time.perf_counter()
     b.step(steps, use_graph=True)
     b.sync()
-    return time.perf_counter() - t0
+    dt = time.perf_counter() - t0
+    return dt, e0, float(b.energy().sum())
 
 
 def plain_world(steps: int, warmup: int):
@@ -126,14 +131,14 @@ def cpu_baseline(seed: int, budget_s: float):
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=60)
+    ap.add_argument("--steps", type=int, default=240)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--worlds", type=int, default=4096)
     ap.add_argument("--seed", type=int, default=0xBB04)
     ap.add_argument("--cpu-budget", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--profile-steps", type=int, default=6)
-    ap.add_argument("--plain-steps", type=int, default=0, help="0: max(20, steps/2)")
+    ap.add_argument("--profile-steps", type=int, default=0, help="0: the timed steps")
+    ap.add_argument("--plain-steps", type=int, default=200)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run the copy kernel on MIB MiB (counter calibration) and exit")
     args = ap.parse_args()
@@ -151,28 +156,42 @@ def main() -> int:
     copy_ms = beatbox.copy_bandwidth(1 << 30, 5)
     copy_gbs = 2 * (1 << 30) / (copy_ms * 1e-3) / 1e9
     log(f"copy kernel: {copy_gbs:.0f} GB/s (practical HBM ceiling)")
+
+    # 1) throughput: hipGraph replay of the timed steps
     batch, total = make_batch(args.worlds, args.seed, 16, 2048)
-    t = time_batch(batch, total, args.steps, args.warmup)
-    log(f"device bytes: {batch.device_bytes / 2**30:.2f} GiB")
+    t, e0, e1 = time_batch(batch, args.steps, args.warmup)
+    dev_gib = batch.device_bytes / 2**30
+    batch.destroy()
     value = total * args.steps / t
     ms_per_step = t / args.steps * 1e3
-    log(f"batch: {ms_per_step:.3f} ms/step, {value / 1e6:.2f} M body-steps/s")
+    drift = (e1 - e0) / abs(e0) if e0 else 0.0
+    log(f"batch: {ms_per_step:.3f} ms/step, {value / 1e6:.2f} M body-steps/s, {dev_gib:.2f} GiB, "
+        f"energy {e0:.4e} -> {e1:.4e} J (rel. change {drift:+.3e})")
 
-    # per-step latency and per-stage durations (hipEvents on the library stream)
-    step_ms, kern_ms = batch.profile(args.profile_steps)
+    # 2) per-stage durations over the same step window: a fresh batch of the
+    # same seed reaches the same states (the step is deterministic), then
+    # eager steps are timed with hipEvents on the library stream
+    batch, _ = make_batch(args.worlds, args.seed, 16, 2048)
+    batch.step(args.warmup, use_graph=True)
+    batch.sync()
+    psteps = args.profile_steps or args.steps
+    step_ms, kern_ms, counts = batch.profile(psteps, counts=True)
+    batch.destroy()
     p50 = float(np.percentile(step_ms, 50))
     p99 = float(np.percentile(step_ms, 99))
-    stats = [batch.stats(w) for w in range(batch.world_count)]
-    sb = stage_bytes(stats, 8, 8)
-    per_launch_ms = {n: float(kern_ms[i]) / args.profile_steps for i, n in enumerate(beatbox.KERNEL_NAMES)}
+    sb = stage_bytes(counts, 8, 8)
+    per_launch_ms = {n: float(kern_ms[i]) / psteps for i, n in enumerate(beatbox.KERNEL_NAMES)}
     dominant = max(per_launch_ms, key=per_launch_ms.get)
-    achieved = sb[dominant] / (per_launch_ms[dominant] * 1e-3) / 1e9
+    achieved = sb[dominant] / (float(kern_ms[beatbox.KERNEL_NAMES.index(dominant)]) * 1e-3) / 1e9
     stages = {n: dict(ms=round(per_launch_ms[n], 4),
-                      gbs=round(sb[n] / (per_launch_ms[n] * 1e-3) / 1e9, 1) if per_launch_ms[n] > 0 else None)
-              for n in beatbox.KERNEL_NAMES}
-    log(f"stages: {stages}")
+                      gbs=round(float(sb[n]) / (float(kern_ms[i]) * 1e-3) / 1e9, 1) if kern_ms[i] > 0 else None)
+              for i, n in enumerate(beatbox.KERNEL_NAMES)}
+    avg = {n: float(counts[:, i].mean()) for i, n in enumerate(beatbox.COUNT_NAMES)}
+    max_pen = int(counts[:, beatbox.COUNT_NAMES.index("max_pen_mm")].max())
+    log(f"stages (avg per launch over {psteps} eager steps): {stages}")
+    log(f"eager step p50 {p50:.3f} ms, p99 {p99:.3f} ms; avg counts {avg}")
 
-    plain_bs, plain_ms = plain_world(args.plain_steps or max(20, args.steps // 2), 5)
+    plain_bs, plain_ms = plain_world(args.plain_steps, args.warmup)
     log(f"plain: {plain_ms:.3f} ms/step, {plain_bs / 1e6:.3f} M body-steps/s, ratio {value / plain_bs:.1f}x")
 
     cpu = None if args.no_cpu else cpu_baseline(args.seed, args.cpu_budget)
@@ -193,17 +212,20 @@ def main() -> int:
         "dtype": "f32",
         "data": "synthetic (seeded box pyramids and towers)",
         "config": {
-            "workload": "batched-worlds: 4096 worlds, bodies log-uniform [16,2048], PGS_SOFT I=8, hipGraph",
+            "workload": f"batched-worlds: {args.worlds} worlds, bodies log-uniform [16,2048], box pyramids/towers, PGS_SOFT I=8, hipGraph, steps {args.warmup}..{args.warmup + args.steps}",
             "worlds": args.worlds,
             "bodies": total,
-            "pairs": sb["counts"]["P"],
-            "manifolds": sb["counts"]["M"],
-            "contacts": sb["counts"]["C"],
+            "avg_pairs": avg["pairs"],
+            "avg_manifolds": avg["manifolds"],
+            "avg_contacts": avg["contacts"],
+            "max_penetration_mm": max_pen,
+            "energy_rel_change": drift,
+            "device_gib": dev_gib,
             "plain_body_steps_per_s": plain_bs,
             "plain_ms_per_step": plain_ms,
             "batch_over_plain": value / plain_bs,
-            "step_latency_ms_p50": p50,
-            "step_latency_ms_p99": p99,
+            "eager_step_ms_p50": p50,
+            "eager_step_ms_p99": p99,
             "copy_kernel_gbs": copy_gbs,
             "stages": stages,
         },

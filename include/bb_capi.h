@@ -274,6 +274,8 @@ bb_status bb_world_debug_tap(bb_world* w, bb_tap kind, void* dst, size_t cap,
                              size_t* written);
 /* Body count of the world. */
 uint32_t bb_world_body_count(const bb_world* w);
+/* Mechanical energy of the world's current state (see bb_batch_energy). */
+bb_status bb_world_energy(bb_world* w, double* out);
 /* RigidBodyManager::destroy (src/rigid_body_manager.cpp:1550-1585). */
 void bb_world_destroy(bb_world* w);
 
@@ -313,11 +315,31 @@ enum {
   BB_KERNEL_FINISH = 6, /* end-of-step counters                            */
   BB_KERNEL_COUNT = 7
 };
-/* Runs nsteps eager steps with hipEvents recorded on the library stream
- * around every stage kernel.  step_ms[nsteps] receives each step's duration,
- * kernel_ms[BB_KERNEL_COUNT] the summed duration of each stage over the steps.
- * Either output may be NULL. */
-bb_status bb_batch_profile(bb_batch* b, uint32_t nsteps, float* step_ms, float* kernel_ms);
+/* Batch totals of one step's counters (bb_batch_profile `counts` fields). */
+enum {
+  BB_COUNT_BODIES = 0,     /* sum N                                        */
+  BB_COUNT_ACTIVE = 1,     /* sum A                                        */
+  BB_COUNT_PAIRS = 2,      /* sum P                                        */
+  BB_COUNT_MANIFOLDS = 3,  /* sum M                                        */
+  BB_COUNT_CONTACTS = 4,   /* sum C                                        */
+  BB_COUNT_COLOR_WORK = 5, /* sum over worlds of edge colours used x M     */
+  BB_COUNT_MAX_PEN_MM = 6, /* max penetration over worlds (dbg_pen, mm)    */
+  BB_COUNT_SLEEPING = 7,   /* sum sleeping bodies                          */
+  BB_COUNT_FIELDS = 8
+};
+/* Measurement helper (no reference counterpart; the reference prints the
+ * wait-bracketed step time, src/renderer_manager.cpp:254-260).  Runs nsteps
+ * eager steps with hipEvents recorded on the library stream around every
+ * stage kernel.  step_ms[nsteps] receives each step's duration,
+ * kernel_ms[BB_KERNEL_COUNT] the summed duration of each stage over the steps,
+ * counts[nsteps * BB_COUNT_FIELDS] each step's batch totals.  Any output may
+ * be NULL. */
+bb_status bb_batch_profile(bb_batch* b, uint32_t nsteps, float* step_ms, float* kernel_ms,
+                           uint64_t* counts);
+/* Mechanical energy of each world's current state, in double (no reference
+ * counterpart): sum over dynamic bodies of m|v|^2/2 + w.I.w/2 - m*g*y with
+ * the physical inertia of the shape.  per_world[n], n >= world count. */
+bb_status bb_batch_energy(bb_batch* b, double* per_world, uint32_t n);
 /* Device bytes held by the batch. */
 uint64_t bb_batch_device_bytes(const bb_batch* b);
 

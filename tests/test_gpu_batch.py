@@ -65,3 +65,80 @@ def test_batch_mixed_solvers_and_graph_replay():
         for f in ("position", "rotation", "velocity", "omega"):
             assert a[f].tobytes() == g[f].tobytes(), (w, f)
         assert eager.stats(w)["collision_count"] == graph.stats(w)["collision_count"]
+
+
+def _quat_matrix(q):
+    x, y, z, w = (float(v) for v in q)
+    return np.array([
+        [1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+        [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+        [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
+
+
+def test_energy_matches_closed_form():
+    """Energy report before any step: potential m*g*y of a pile, and the
+    kinetic energy of spinning, moving boxes and spheres evaluated as
+    m|v|^2/2 + w.(R I_b R^T).w/2 in matrix form."""
+    pile = scenes.batch_world(0xBB04, 2, 40)
+    rng = np.random.default_rng(7)
+    spin = np.zeros(6, scenes.BODY_DESC)
+    for i in range(6):
+        b = spin[i]
+        b["shape"] = scenes.SHAPE_SPHERE if i % 3 == 2 else scenes.SHAPE_BOX
+        b["flags"] = scenes.BODY_DYNAMIC | scenes.BODY_GRAVITY
+        b["position"] = (4.0 * i, 3.0 + i, -2.0)
+        q = rng.normal(size=4)
+        b["rotation"] = q / np.linalg.norm(q)
+        half = np.array([0.3, 0.5, 0.9]) * (1 + 0.1 * i)
+        b["minimum"] = -half
+        b["maximum"] = half
+        b["radius"] = 0.4 + 0.05 * i
+        b["mass"] = 1.5 + i
+        b["velocity"] = rng.normal(size=3)
+        b["omega"] = rng.normal(size=3) * 2.0
+        b["friction"] = 0.5
+    desc = beatbox.WorldDesc.make(**scenes.world_desc(iterations=8, morton_bits=63, quirks=0))
+    bt = beatbox.Batch()
+    bt.add_world(desc, pile)
+    bt.add_world(desc, spin)
+    e = bt.energy()
+    g = -float(desc.gravity)
+    dyn = (pile["flags"] & scenes.BODY_DYNAMIC) != 0
+    e_pile = float(np.sum(pile["mass"][dyn].astype(np.float64) * g * pile["position"][dyn, 1]))
+    assert abs(e[0] - e_pile) <= 1e-9 * abs(e_pile)
+    e_spin = 0.0
+    for b in spin:
+        m = float(b["mass"])
+        if b["shape"] == scenes.SHAPE_SPHERE:
+            ib = np.eye(3) * 0.4 * m * float(b["radius"]) ** 2
+        else:
+            ext = (b["maximum"] - b["minimum"]).astype(np.float64)
+            ib = np.diag([m / 12 * (ext[1] ** 2 + ext[2] ** 2), m / 12 * (ext[0] ** 2 + ext[2] ** 2),
+                          m / 12 * (ext[0] ** 2 + ext[1] ** 2)])
+        r = _quat_matrix(b["rotation"])
+        w = b["omega"].astype(np.float64)
+        v = b["velocity"].astype(np.float64)
+        e_spin += 0.5 * m * v @ v + 0.5 * w @ (r @ ib @ r.T) @ w + m * g * float(b["position"][1])
+    assert abs(e[1] - e_spin) <= 1e-6 * abs(e_spin)
+
+
+def test_profile_counts_equal_world_stats():
+    sizes = [30, 200, 64, 500]
+    worlds = _worlds(0xBB04, sizes)
+    desc = beatbox.WorldDesc.make(**scenes.world_desc(iterations=8, morton_bits=63, quirks=0))
+    b = beatbox.Batch()
+    for bodies in worlds:
+        b.add_world(desc, bodies)
+    b.step(5)
+    step_ms, kern_ms, counts = b.profile(4, counts=True)
+    assert np.all(step_ms > 0) and np.all(kern_ms[[0, 1, 2, 3, 4, 6]] > 0)
+    st = [b.stats(w) for w in range(len(sizes))]
+    last = counts[-1]
+    assert last[0] == sum(s["rigid_body_count"] for s in st)
+    assert last[1] == sum(s["active_rigid_body_count"] for s in st)
+    assert last[2] == sum(s["broad_phase_collision_count"] for s in st)
+    assert last[3] == sum(s["collision_count"] for s in st)
+    assert last[4] == sum(s["contact_count"] for s in st)
+    assert last[5] == sum(s["graph_color_count"] * s["collision_count"] for s in st)
+    assert last[6] == max(s["dbg_pen"] for s in st)
+    assert last[7] == sum(s["sleeping_count"] for s in st)
