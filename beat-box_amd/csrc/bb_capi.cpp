@@ -78,6 +78,7 @@ struct bb_batch {
   uint32_t* entries[2] = {nullptr, nullptr};
   uint2* pairs[2] = {nullptr, nullptr};
   uint32_t* npairs[2] = {nullptr, nullptr};
+  uint32_t* pstart[2] = {nullptr, nullptr};
   Manifolds man[2];
   int latest = 0;  // buffer set holding the newest state
   uint32_t NB = 0, NA = 0, NP = 0;
@@ -226,6 +227,7 @@ bb_status build(bb_batch* b) {
     b->entries[s] = A.alloc<uint32_t>(NB);
     b->pairs[s] = A.alloc<uint2>(NP);
     b->npairs[s] = A.alloc<uint32_t>(W);
+    b->pstart[s] = A.alloc<uint32_t>(NB);
     Manifolds& M = b->man[s];
     M.hdr = A.alloc<int4>(NP);
     M.nrm = A.alloc<float4>(NP);
@@ -355,6 +357,7 @@ bb_status build(bb_batch* b) {
     HIPCHK(up(R.pvel, z4.data(), NB * 16));
     HIPCHK(up(b->entries[s], ent.data(), NB * 4));
     HIPCHK(hipMemset(b->npairs[s], 0, W * 4));
+    HIPCHK(hipMemset(b->pstart[s], 0, NB * 4));
   }
   HIPCHK(up(P.pmin, pmin.data(), NB * 16));
   HIPCHK(up(P.pmax, pmax.data(), NB * 16));
@@ -385,6 +388,8 @@ Dev dev_for(const bb_batch* b, int cur) {
   d.pairs_prev = b->pairs[prev];
   d.npairs_cur = b->npairs[cur];
   d.npairs_prev = b->npairs[prev];
+  d.pstart_cur = b->pstart[cur];
+  d.pstart_prev = b->pstart[prev];
   d.man_cur = b->man[cur];
   d.man_prev = b->man[prev];
   return d;

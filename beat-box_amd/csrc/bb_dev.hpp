@@ -95,6 +95,8 @@ struct Dev {
   // broad phase [body]
   uint32_t* pcount;
   uint32_t* poff;
+  uint32_t* pstart_cur;   // [body] first pair of each row (uncapped offset), kept
+  uint32_t* pstart_prev;  // for the next step's warm-start lookup
   // pairs / manifolds [pair slot]
   uint2* pairs_cur;
   uint2* pairs_prev;

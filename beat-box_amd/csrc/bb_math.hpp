@@ -12,6 +12,22 @@
 
 #define BB_DI __device__ __forceinline__
 
+#include <utility>
+
+namespace bb {
+// Compile-time loop: f(std::integral_constant<int, I>) for I = 0..N-1, so
+// array indices inside f are constants from the start and small private
+// arrays stay in registers.
+template <class F, int... I>
+BB_DI void static_for_impl(F& f, std::integer_sequence<int, I...>) {
+  (f(std::integral_constant<int, I>{}), ...);
+}
+template <int N, class F>
+BB_DI void static_for(F&& f) {
+  static_for_impl(f, std::make_integer_sequence<int, N>{});
+}
+}  // namespace bb
+
 namespace bb {
 
 struct v3 {

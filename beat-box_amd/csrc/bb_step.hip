@@ -472,7 +472,10 @@ __global__ __launch_bounds__(BLOCK) void k_broad(Dev d) {
     }
     uint32_t total;
     uint32_t ex = block_scan<BLOCK>(cnt, S, total);
-    if (i < N) d.poff[off + i] = carry + ex;
+    if (i < N) {
+      d.poff[off + i] = carry + ex;
+      d.pstart_cur[off + i] = carry + ex;
+    }
     __syncthreads();
     if (tid == 0) carry += total;
     __syncthreads();
@@ -517,26 +520,28 @@ BB_DI v3 tx_rotT(const Tx& t, v3 v) { return mulvm(v, mT(t.R)); }
 BB_DI v3 tx_tf(const Tx& t, v3 v) { return mulvm(v, t.R) + t.p; }
 BB_DI v3 tx_tfT(const Tx& t, v3 v) { return mulvm(v - t.p, mT(t.R)); }
 
-struct ClipV {
-  v3 v;
-  uint32_t f[4];  // in_ref, out_ref, in_inc, out_inc
+// Clip vertex: position and the four feature ids (in_ref, out_ref, in_inc,
+// out_inc) packed one per byte.  Polygons live in fixed 8-entry register
+// arrays; every index is a compile-time constant after unrolling.
+struct CV {
+  float x, y, z;
+  uint32_t f;
 };
+BB_DI v3 cv_v(const CV& c) { return V(c.x, c.y, c.z); }
+BB_DI void cv_set(CV& c, v3 v) { c.x = v.x; c.y = v.y; c.z = v.z; }
+BB_DI uint32_t cv_fp(const CV& c, int k) { return (c.f >> (8 * k)) & 0xFFu; }
 
 enum : uint32_t {
   PXPY = 0, PXNY = 1, PXPZ = 2, PXNZ = 3, NXPY = 4, NXNY = 5, NXPZ = 6, NXNZ = 7,
   PYPZ = 8, PYNZ = 9, NYPZ = 10, NYNZ = 11
 };
 
-BB_DI void set_inc(ClipV* o, int i, v3 v, uint32_t in, uint32_t out) {
-  o[i].v = v;
-  o[i].f[0] = 0;
-  o[i].f[1] = 0;
-  o[i].f[2] = in;
-  o[i].f[3] = out;
+BB_DI void set_inc(CV (&o)[8], int i, v3 v, uint32_t in, uint32_t out) {
+  o[i] = CV{v.x, v.y, v.z, (in << 16) | (out << 24)};
 }
 
 // compute_incident_face (collision_detection.slang:124-264)
-BB_DI void incident_face(const Tx& itx, v3 e, v3 n, ClipV* o) {
+BB_DI void incident_face(const Tx& itx, v3 e, v3 n, CV (&o)[8]) {
   n = -tx_rotT(itx, n);
   v3 an = vabs(n);
   if (an.x > an.y && an.x > an.z) {
@@ -576,8 +581,10 @@ BB_DI void incident_face(const Tx& itx, v3 e, v3 n, ClipV* o) {
       set_inc(o, 3, V(e.x, e.y, -e.z), PXPZ, PYNZ);
     }
   }
-#pragma unroll
-  for (int i = 0; i < 4; ++i) o[i].v = tx_tf(itx, o[i].v);
+  static_for<4>([&](auto ic) {
+    constexpr int i = decltype(ic)::value;
+    cv_set(o[i], tx_tf(itx, cv_v(o[i])));
+  });
 }
 
 // compute_reference_edges_and_basis (collision_detection.slang:266-336)
@@ -614,30 +621,53 @@ BB_DI void reference_edges(v3 er, const Tx& rtx, v3 n, int axis, uint32_t* ce, m
 
 BB_DI bool clip_on(float v) { return (v < 0.005f) && (v > -0.005f); }
 
-// ortographic_clip (collision_detection.slang:352-388); output capped at 8.
-BB_DI int ortho_clip(float sign, float e, int axis, uint32_t ce, const ClipV* in, int nin, ClipV* out) {
+BB_DI CV cv_sel(bool c, const CV& x, const CV& y) {
+  return CV{c ? x.x : y.x, c ? x.y : y.y, c ? x.z : y.z, c ? x.f : y.f};
+}
+// Writes x into o[k] (k < 8) with a select per entry: no runtime index and
+// no conditional store, so the array stays in registers.
+BB_DI void cv_put(CV (&o)[8], int k, const CV& x) {
+  static_for<8>([&](auto jc) {
+    constexpr int j = decltype(jc)::value;
+    o[j] = cv_sel(j == k, x, o[j]);
+  });
+}
+BB_DI CV cv_lerp(const CV& a, const CV& b, float t) {
+  return CV{a.x + (b.x - a.x) * t, a.y + (b.y - a.y) * t, a.z + (b.z - a.z) * t, 0u};
+}
+
+// ortographic_clip (collision_detection.slang:352-388) against the side
+// plane sign * v[AX] = e; output capped at 8.
+template <int MAXIN, int AX, int SG>
+BB_DI int clip_pass(float e, uint32_t ce, const CV (&in)[8], int nin, CV (&out)[8]) {
+  const float sign = SG > 0 ? 1.0f : -1.0f;
+  CV a = in[0];
+  static_for<MAXIN>([&](auto kc) {
+    constexpr int k = decltype(kc)::value;
+    if (k > 0) a = cv_sel(k == nin - 1, in[k], a);
+  });
   int nout = 0;
-  ClipV a = in[nin - 1];
-  for (int i = 0; i < nin; ++i) {
-    ClipV b = in[i];
-    float da = sign * cmp(a.v, axis) - e;
-    float db = sign * cmp(b.v, axis) - e;
-    if ((da < 0.0f && db < 0.0f) || clip_on(da) || clip_on(db)) {
-      if (nout < 8) out[nout++] = b;
-    } else if (da < 0.0f && db >= 0.0f) {
-      ClipV cv = b;
-      cv.v = a.v + (b.v - a.v) * (da / (da - db));
-      cv.f[1] = ce;
-      if (nout < 8) out[nout++] = cv;
-    } else if (da >= 0.0f && db < 0.0f) {
-      ClipV cv = a;
-      cv.v = a.v + (b.v - a.v) * (da / (da - db));
-      cv.f[0] = ce;
-      if (nout < 8) out[nout++] = cv;
-      if (nout < 8) out[nout++] = b;
+  static_for<MAXIN>([&](auto ic) {
+    constexpr int i = decltype(ic)::value;
+    if (i < nin) {
+      const CV b = in[i];
+      const float da = sign * (AX ? a.y : a.x) - e;
+      const float db = sign * (AX ? b.y : b.x) - e;
+      if ((da < 0.0f && db < 0.0f) || clip_on(da) || clip_on(db)) {
+        if (nout < 8) cv_put(out, nout++, b);
+      } else if (da < 0.0f && db >= 0.0f) {
+        CV c = cv_lerp(a, b, da / (da - db));
+        c.f = (b.f & ~0x0000FF00u) | (ce << 8);
+        if (nout < 8) cv_put(out, nout++, c);
+      } else if (da >= 0.0f && db < 0.0f) {
+        CV c = cv_lerp(a, b, da / (da - db));
+        c.f = (a.f & ~0x000000FFu) | ce;
+        if (nout < 8) cv_put(out, nout++, c);
+        if (nout < 8) cv_put(out, nout++, b);
+      }
+      a = b;
     }
-    a = b;
-  }
+  });
   return nout;
 }
 
@@ -689,17 +719,6 @@ BB_DI void edges_contact(v3 PA, v3 QA, v3 PB, v3 QB, v3& CA, v3& CB) {
   CB = PB + DB * TB;
 }
 
-struct Contact {
-  v3 pos;
-  float pen;
-  uint32_t fp[4];
-};
-struct MDet {  // freshly detected manifold
-  int count, key;
-  v3 normal;
-  Contact c[8];
-};
-
 struct BodyView {
   v3 pos;
   qt rot;
@@ -708,10 +727,26 @@ struct BodyView {
   uint32_t kind;
 };
 
-// OBBvsOBBCollision::collision_detection (collision_detection.slang:568-877)
-BB_DI bool obb_obb(const BodyView& A, const BodyView& B, MDet& m) {
-  m.count = 0;
-  m.key = -1;
+// Detected contacts go straight into the output manifold slot: position +
+// penetration and feature pair of contact k at base + k.
+struct CSink {
+  float4* cpos;
+  uint4* cfp;
+  uint32_t base;
+  bool capped;  // box-box penetrations are floored at -0.25
+  float deep;   // min(0, penetrations written)
+  BB_DI void put(int k, v3 p, float pen, uint32_t f0, uint32_t f1, uint32_t f2, uint32_t f3) {
+    if (capped) pen = fmx(pen, -0.25f);
+    deep = fmn(deep, pen);
+    cpos[base + k] = st3(p, pen);
+    cfp[base + k] = make_uint4(f0, f1, f2, f3);
+  }
+};
+
+// OBBvsOBBCollision::collision_detection (collision_detection.slang:568-877).
+// Returns the contact count (0: no manifold); key and normal on output.
+BB_DI int obb_obb(const BodyView& A, const BodyView& B, CSink& sk, int& key, v3& normal_out) {
+  key = -1;
   Tx txA{mT(qmat(A.rot)), A.pos}, txB{mT(qmat(B.rot)), B.pos};
   v3 eA = (A.mx - A.mn) * 0.5f, eB = (B.mx - B.mn) * 0.5f;
   mat3 C = mmul(mT(txA.R), txB.R);
@@ -726,28 +761,32 @@ BB_DI bool obb_obb(const BodyView& A, const BodyView& B, MDet& m) {
   float a_max = -3.402823466e+38f, b_max = -3.402823466e+38f, e_max = -3.402823466e+38f;
   int a_axis = -1, b_axis = -1, e_axis = -1;
   v3 n_a = V(0, 0, 0), n_b = V(0, 0, 0), n_e = V(0, 0, 0);
+#pragma unroll
   for (int i = 0; i < 3; ++i) {
     float s = fab(cmp(t, i)) - (cmp(eA, i) + dot(mcol(aC, i), eB));
-    if (s > 0.0f) return false;
+    if (s > 0.0f) return 0;
     if (s > a_max) { a_max = s; a_axis = i; n_a = mrow(txA.R, i); }
   }
+#pragma unroll
   for (int i = 0; i < 3; ++i) {
     float proj = dot(t, mrow(C, i));
     float s = fab(proj) - (cmp(eB, i) + dot(mrow(aC, i), eA));
-    if (s > 0.0f) return false;
+    if (s > 0.0f) return 0;
     if (s > b_max) { b_max = s; b_axis = 3 + i; n_b = mrow(txB.R, i); }
   }
   if (!parallel) {
+#pragma unroll
     for (int i = 0; i < 3; ++i) {
+#pragma unroll
       for (int j = 0; j < 3; ++j) {
-        int i1 = (i + 1) % 3, i2 = (i + 2) % 3, j1 = (j + 1) % 3, j2 = (j + 2) % 3;
+        const int i1 = (i + 1) % 3, i2 = (i + 2) % 3, j1 = (j + 1) % 3, j2 = (j + 2) % 3;
         float rA = cmp(eA, i1) * aC.m[j][i2] + cmp(eA, i2) * aC.m[j][i1];
         float rB = cmp(eB, j1) * aC.m[j2][i] + cmp(eB, j2) * aC.m[j1][i];
         float s = fab(cmp(t, i2) * C.m[j][i1] - cmp(t, i1) * C.m[j][i2]) - (rA + rB);
         v3 ax = V(0, 0, 0);
         setcmp(ax, i1, -C.m[j][i2]);
         setcmp(ax, i2, C.m[j][i1]);
-        if (s > 0.0f) return false;
+        if (s > 0.0f) return 0;
         float l = 1.0f / vlen(ax);
         s *= l;
         if (s > e_max) { e_max = s; e_axis = 6 + i * 3 + j; n_e = ax * l; }
@@ -762,14 +801,15 @@ BB_DI bool obb_obb(const BodyView& A, const BodyView& B, MDet& m) {
   else if (0.95f * b_max > a_max + 0.01f) { axis = b_axis; s_max = b_max; normal = n_b; }
   else { axis = a_axis; s_max = a_max; normal = n_a; }
   if (dot(normal, B.pos - A.pos) < 0.0f) normal = -normal;
-  if (axis == -1) return false;
+  if (axis == -1) return 0;
+  int count;
   if (axis <= 5) {
     Tx rtx, itx;
     v3 er, ei;
     bool flip = false;
     if (axis <= 2) { rtx = txA; itx = txB; er = eA; ei = eB; }
     else { rtx = txB; itx = txA; er = eB; ei = eA; flip = true; normal = -normal; }
-    ClipV inc[8], tmp[8];
+    CV inc[8], tmp[8];
     incident_face(itx, ei, normal, inc);
     uint32_t ce[4];
     mat3 basis;
@@ -777,34 +817,35 @@ BB_DI bool obb_obb(const BodyView& A, const BodyView& B, MDet& m) {
     reference_edges(er, rtx, normal, axis, ce, basis, e);
     Tx tx{basis, rtx.p};
     // clip (collision_detection.slang:390-452)
-#pragma unroll
-    for (int i = 0; i < 4; ++i) inc[i].v = tx_tfT(tx, inc[i].v);
-    int nout = ortho_clip(1.0f, e.x, 0, ce[0], inc, 4, tmp);
-    if (nout == 0) return false;
-    int nin = ortho_clip(1.0f, e.y, 1, ce[1], tmp, nout, inc);
-    if (nin == 0) return false;
-    nout = ortho_clip(-1.0f, e.x, 0, ce[2], inc, nin, tmp);
-    if (nout == 0) return false;
-    nin = ortho_clip(-1.0f, e.y, 1, ce[3], tmp, nout, inc);
-    if (nin == 0) return false;
+    static_for<4>([&](auto ic) {
+      constexpr int i = decltype(ic)::value;
+      cv_set(inc[i], tx_tfT(tx, cv_v(inc[i])));
+    });
+    int nout = clip_pass<4, 0, 1>(e.x, ce[0], inc, 4, tmp);
+    if (nout == 0) return 0;
+    int nin = clip_pass<8, 1, 1>(e.y, ce[1], tmp, nout, inc);
+    if (nin == 0) return 0;
+    nout = clip_pass<8, 0, -1>(e.x, ce[2], inc, nin, tmp);
+    if (nout == 0) return 0;
+    nin = clip_pass<8, 1, -1>(e.y, ce[3], tmp, nout, inc);
+    if (nin == 0) return 0;
     int k = 0;
-    for (int i = 0; i < nin; ++i) {
-      float dz = inc[i].v.z - e.z;
-      if (dz <= 0.0f) {
-        Contact& c = m.c[k++];
-        c.pos = tx_tf(tx, inc[i].v);
-        c.pen = dz;
-        if (flip) {
-          c.fp[0] = inc[i].f[2]; c.fp[1] = inc[i].f[3]; c.fp[2] = inc[i].f[0]; c.fp[3] = inc[i].f[1];
-        } else {
-          c.fp[0] = inc[i].f[0]; c.fp[1] = inc[i].f[1]; c.fp[2] = inc[i].f[2]; c.fp[3] = inc[i].f[3];
+    static_for<8>([&](auto ic) {
+      constexpr int i = decltype(ic)::value;
+      if (i < nin) {
+        const float dz = inc[i].z - e.z;
+        if (dz <= 0.0f) {
+          const v3 pw = tx_tf(tx, cv_v(inc[i]));
+          const uint32_t f0 = cv_fp(inc[i], 0), f1 = cv_fp(inc[i], 1), f2 = cv_fp(inc[i], 2), f3 = cv_fp(inc[i], 3);
+          if (flip) sk.put(k, pw, dz, f2, f3, f0, f1);
+          else sk.put(k, pw, dz, f0, f1, f2, f3);
+          ++k;
         }
       }
-    }
-    if (k > 0) {
-      m.count = k;
-      m.normal = flip ? -normal : normal;
-    }
+    });
+    if (k == 0) return 0;
+    count = k;
+    normal_out = flip ? -normal : normal;
   } else {
     normal = tx_rot(txA, normal);
     if (dot(normal, B.pos - A.pos) < 0.0f) normal = -normal;
@@ -812,39 +853,33 @@ BB_DI bool obb_obb(const BodyView& A, const BodyView& B, MDet& m) {
     support_edge(txA, eA, normal, PA, QA);
     support_edge(txB, eB, -normal, PB, QB);
     edges_contact(PA, QA, PB, QB, CA, CB);
-    m.count = 1;
-    m.normal = normal;
-    m.c[0].pos = (CA + CB) * 0.5f;
-    m.c[0].pen = s_max;
-    m.c[0].fp[0] = m.c[0].fp[1] = m.c[0].fp[2] = m.c[0].fp[3] = (uint32_t)axis;
+    count = 1;
+    normal_out = normal;
+    sk.put(0, (CA + CB) * 0.5f, s_max, (uint32_t)axis, (uint32_t)axis, (uint32_t)axis, (uint32_t)axis);
   }
-  m.key = axis;
-  return m.count > 0;
+  key = axis;
+  return count;
 }
 
 // Sphere contacts (extension; the reference has no spheres): sphere-sphere
 // along the centre line, box-sphere through the clamped local closest point
 // or, for a centre inside the box, the face of least penetration.
 static constexpr int KEY_SPHERE_SPHERE = 20, KEY_BOX_SPHERE = 21;
-BB_DI bool sphere_sphere(const BodyView& A, const BodyView& B, MDet& m) {
-  m.count = 0;
-  m.key = KEY_SPHERE_SPHERE;
+BB_DI int sphere_sphere(const BodyView& A, const BodyView& B, CSink& sk, int& key, v3& normal_out) {
+  key = KEY_SPHERE_SPHERE;
   v3 dd = B.pos - A.pos;
   float dist = vlen(dd);
   float pen = dist - (A.radius + B.radius);
-  if (pen > 0.0f) return false;
+  if (pen > 0.0f) return 0;
   v3 n = dist > 0.0f ? vdiv(dd, dist) : V(0.0f, 1.0f, 0.0f);
   v3 pa = A.pos + n * A.radius, pb = B.pos - n * B.radius;
-  m.count = 1;
-  m.normal = n;
-  m.c[0].pos = (pa + pb) * 0.5f;
-  m.c[0].pen = pen;
-  m.c[0].fp[0] = m.c[0].fp[1] = m.c[0].fp[2] = m.c[0].fp[3] = (uint32_t)KEY_SPHERE_SPHERE;
-  return true;
+  normal_out = n;
+  const uint32_t k = (uint32_t)KEY_SPHERE_SPHERE;
+  sk.put(0, (pa + pb) * 0.5f, pen, k, k, k, k);
+  return 1;
 }
-BB_DI bool box_sphere(const BodyView& A, const BodyView& B, MDet& m) {
-  m.count = 0;
-  m.key = KEY_BOX_SPHERE;
+BB_DI int box_sphere(const BodyView& A, const BodyView& B, CSink& sk, int& key, v3& normal_out) {
+  key = KEY_BOX_SPHERE;
   const bool a_box = A.kind == BB_SHAPE_BOX;
   const BodyView& X = a_box ? A : B;
   const BodyView& S = a_box ? B : A;
@@ -858,7 +893,7 @@ BB_DI bool box_sphere(const BodyView& A, const BodyView& B, MDet& m) {
   if (dist2 > 0.0f) {
     float dist = __builtin_sqrtf(dist2);
     pen = dist - S.radius;
-    if (pen > 0.0f) return false;
+    if (pen > 0.0f) return 0;
     nl = vdiv(dd, dist);
     surf = q;
   } else {
@@ -876,12 +911,10 @@ BB_DI bool box_sphere(const BodyView& A, const BodyView& B, MDet& m) {
   v3 n = qrot(X.rot, nl);
   v3 surf_w = qrot(X.rot, surf) + X.pos;
   v3 sph_w = S.pos - n * S.radius;
-  m.count = 1;
-  m.normal = a_box ? n : -n;
-  m.c[0].pos = (surf_w + sph_w) * 0.5f;
-  m.c[0].pen = pen;
-  m.c[0].fp[0] = m.c[0].fp[1] = m.c[0].fp[2] = m.c[0].fp[3] = (uint32_t)KEY_BOX_SPHERE;
-  return true;
+  normal_out = a_box ? n : -n;
+  const uint32_t k = (uint32_t)KEY_BOX_SPHERE;
+  sk.put(0, (surf_w + sph_w) * 0.5f, pen, k, k, k, k);
+  return 1;
 }
 
 BB_DI BodyView body_view(const Dev& d, uint32_t off, uint32_t row) {
@@ -890,18 +923,6 @@ BB_DI BodyView body_view(const Dev& d, uint32_t off, uint32_t row) {
   float4 a = d.props.pmin[off + id], b = d.props.pmax[off + id];
   return BodyView{ld3(d.cur.pos[g]), ldq(d.cur.rot[g]), ld3(a), ld3(b), a.w, b.w != 0.0f ? 1u : 0u};
 }
-
-// Full contact record as it travels through warm start.
-struct CFull {
-  v3 pos;
-  float pen;
-  float ln, lt0, lt1, kcoef;
-  float tm0, tm1;
-  v3 anca, ancb;
-  uint32_t stick;
-  uint32_t fp[4];
-  uint32_t ws;
-};
 
 // IslandBuilder::link_bodies (src/shaders/simulation/islands.slang:153-213):
 // lock-free union-find, links always point to the lower index, so every
@@ -932,6 +953,135 @@ BB_DI void uf_link(uint32_t* link, uint32_t f, uint32_t s) {
   }
 }
 
+// Reads the previous step's manifold of this pair and carries its impulses
+// over to the new contacts already written at slot*8 (find_old_collision +
+// Manifold::update_from_old, collision_detection.slang:914-963,
+// src/shaders/extensions.slang:305-473).  Returns the bitmask of new
+// contacts that inherited state and sets `error`.  Old and new contact
+// positions and feature ids are held in registers (all indices are
+// compile-time constants), so the matching loops issue no memory traffic.
+BB_DI uint32_t warm_start(const Dev& d, uint32_t os, bool swapped, int key, int count, v3 n, uint32_t slot,
+                          float deep_new, int32_t& error) {
+  const int4 oh = d.man_prev.hdr[os];
+  if (oh.w <= 0) return 0;
+  const float sgn = swapped ? -1.0f : 1.0f;
+  const bool cross_key = key != oh.z || swapped;
+  const v3 on = ld3(d.man_prev.nrm[os]);
+  const int ocount = oh.w;
+  if (cross_key && !(dot(n, on * sgn) > 0.7f)) return 0;
+  const uint32_t nb = slot * 8, ob = os * 8;
+  float4 op[8], np[8];  // position + penetration
+  uint2 of[8], nf[8];   // feature ids in_inc, out_inc
+  static_for<8>([&](auto jc) {
+    constexpr int j = decltype(jc)::value;
+    float4 p4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    uint2 f2 = make_uint2(0u, 0u);
+    if (j < ocount) {
+      p4 = d.man_prev.cpos[ob + j];
+      const uint4 f = d.man_prev.cfp[ob + j];
+      f2 = make_uint2(f.z, f.w);
+    }
+    op[j] = p4;
+    of[j] = f2;
+    float4 q4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    uint2 g2 = make_uint2(0u, 0u);
+    if (j < count) {
+      q4 = d.man_cur.cpos[nb + j];
+      const uint4 f = d.man_cur.cfp[nb + j];
+      g2 = make_uint2(f.z, f.w);
+    }
+    np[j] = q4;
+    nf[j] = g2;
+  });
+  // match map: byte i = old index j (pass 1), 0x80 | j (pass 2), 0xFF none
+  uint64_t map = ~0ull;
+  uint32_t used_old = 0, matched_new = 0, snap = 0;
+  // pass 1: same feature pair within 0.1 m (same key, not swapped)
+  if (!cross_key) {
+    static_for<8>([&](auto ic) {
+      constexpr int i = decltype(ic)::value;
+      if (i < count) {
+        bool found = false;
+        static_for<8>([&](auto jc) {
+          constexpr int j = decltype(jc)::value;
+          if (j < ocount && !found && !((used_old >> j) & 1u)) {
+            const v3 opos = ld3(op[j]), npos = ld3(np[i]);
+            const v3 dp = opos - npos;
+            const bool feat = of[j].x == nf[i].x && of[j].y == nf[i].y;
+            const bool close = dot(dp, dp) < 0.1f * 0.1f;
+            if (feat && close) {
+              found = true;
+              used_old |= 1u << j;
+              matched_new |= 1u << i;
+              map = (map & ~(0xFFull << (8 * i))) | ((uint64_t)j << (8 * i));
+              const v3 d2 = npos - opos;
+              if (dot(d2, d2) < 0.01f) snap |= 1u << i;
+            }
+          }
+        });
+      }
+    });
+  }
+  // pass 2: nearest remaining pairs within 0.35 m, one pair per round
+  for (int round = 0; round < count; ++round) {
+    float best = 0.1225f;
+    int bi = -1, bj = -1;
+    static_for<8>([&](auto ic) {
+      constexpr int i = decltype(ic)::value;
+      if (i < count && !((matched_new >> i) & 1u)) {
+        static_for<8>([&](auto jc) {
+          constexpr int j = decltype(jc)::value;
+          if (j < ocount && !((used_old >> j) & 1u)) {
+            const v3 dp = ld3(np[i]) - ld3(op[j]);
+            const float dd2 = dot(dp, dp);
+            if (dd2 < best) { best = dd2; bi = i; bj = j; }
+          }
+        });
+      }
+    });
+    if (bi < 0) break;
+    map = (map & ~(0xFFull << (8 * bi))) | ((uint64_t)(0x80 | bj) << (8 * bi));
+    matched_new |= 1u << bi;
+    used_old |= 1u << bj;
+  }
+  // carry the matched contacts' state over
+  static_for<8>([&](auto ic) {
+    constexpr int i = decltype(ic)::value;
+    const uint32_t m = (uint32_t)(map >> (8 * i)) & 0xFFu;
+    if (i < count && m != 0xFFu) {
+      const bool p2 = (m & 0x80u) != 0;
+      const uint32_t oc = ob + (m & 0x7Fu), c = nb + i;
+      const float4 ol = d.man_prev.clam[oc], om = d.man_prev.cmas[oc];
+      const float4 oa = d.man_prev.cana[oc], obb = d.man_prev.canb[oc];
+      const uint32_t ost = __float_as_uint(obb.w);
+      d.man_cur.clam[c] = make_float4(ol.x, p2 ? ol.y * sgn : ol.y, ol.z, ol.w);
+      d.man_cur.cmas[c] = make_float4(0.0f, om.y, om.z, 0.0f);
+      if (ost != 0) {
+        const bool sw = p2 && swapped;
+        d.man_cur.cana[c] = st3(sw ? ld3(obb) : ld3(oa), 0.0f);
+        d.man_cur.canb[c] = st3(sw ? ld3(oa) : ld3(obb), __uint_as_float(ost));
+      } else {
+        d.man_cur.cana[c] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        d.man_cur.canb[c] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      }
+      if (p2) d.man_cur.cfp[c] = d.man_prev.cfp[oc];
+      if ((snap >> i) & 1u) {
+        const float4 o4 = d.man_prev.cpos[oc];
+        d.man_cur.cpos[c] = make_float4(o4.x, o4.y, o4.z, np[i].w);
+      }
+      const uint32_t ows = d.man_prev.cws[oc];
+      d.man_cur.cws[c] = ows + 1u < 255u ? ows + 1u : 255u;
+    }
+  });
+  float deep_old = 0.0f;
+  static_for<8>([&](auto jc) {
+    constexpr int j = decltype(jc)::value;
+    if (j < ocount) deep_old = fmn(deep_old, op[j].w);
+  });
+  error = (int32_t)fcl((deep_new - deep_old) * 10000.0f, -32000.0f, 32000.0f);
+  return matched_new;
+}
+
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_narrow(Dev d) {
   const uint32_t w = blockIdx.x, tid = threadIdx.x;
@@ -944,160 +1094,76 @@ __global__ __launch_bounds__(BLOCK) void k_narrow(Dev d) {
   uint32_t fresh = 0, pen_mm = 0, ccount = 0, mcount = 0;
   for (uint32_t p = tid; p < P; p += BLOCK) {
     const uint32_t slot = W.pair_off + p;
-    uint2 pr = pairs[p];
+    const uint2 pr = pairs[p];
     const uint32_t ga = off + pr.x, gb = off + pr.y;
     const uint32_t fa = d.cur.flags[ga], fb = d.cur.flags[gb];
     const bool da = (fa & BB_BODY_DYNAMIC) != 0, db = (fb & BB_BODY_DYNAMIC) != 0;
-    d.man_cur.hdr[slot] = make_int4((int)pr.x, (int)pr.y, -1, 0);
-    if (!da && !db) continue;
-    BodyView A = body_view(d, off, pr.x), B = body_view(d, off, pr.y);
-    MDet m;
-    bool have;
-    if (A.kind == BB_SHAPE_BOX && B.kind == BB_SHAPE_BOX) {
-      have = obb_obb(A, B, m);
-      if (have)
-        for (int ci = 0; ci < m.count; ++ci) m.c[ci].pen = fmx(m.c[ci].pen, -0.25f);
-    } else if (A.kind == BB_SHAPE_SPHERE && B.kind == BB_SHAPE_SPHERE) {
-      have = sphere_sphere(A, B, m);
-    } else {
-      have = box_sphere(A, B, m);
+    if (!da && !db) {
+      d.man_cur.hdr[slot] = make_int4((int)pr.x, (int)pr.y, -1, 0);
+      continue;
     }
-    if (!have) continue;
+    const BodyView A = body_view(d, off, pr.x), B = body_view(d, off, pr.y);
+    CSink sk{d.man_cur.cpos, d.man_cur.cfp, slot * 8, false, 0.0f};
+    int key, count;
+    v3 n;
+    if (A.kind == BB_SHAPE_BOX && B.kind == BB_SHAPE_BOX) {
+      sk.capped = true;
+      count = obb_obb(A, B, sk, key, n);
+    } else if (A.kind == BB_SHAPE_SPHERE && B.kind == BB_SHAPE_SPHERE) {
+      count = sphere_sphere(A, B, sk, key, n);
+    } else {
+      count = box_sphere(A, B, sk, key, n);
+    }
+    if (count == 0) {
+      d.man_cur.hdr[slot] = make_int4((int)pr.x, (int)pr.y, -1, 0);
+      continue;
+    }
     // Manifold::compute_basis (src/shaders/extensions.slang:262-272)
-    v3 n = m.normal, t0, t1;
+    v3 t0, t1;
     if (fab(n.x) > fab(n.y)) t0 = V(n.z, 0.0f, -n.x);
     else t0 = V(0.0f, -n.z, n.y);
     t0 = vnorm(t0);
     t1 = cross(t0, n);
-    CFull c[8];
-    for (int i = 0; i < m.count; ++i) {
-      c[i].pos = m.c[i].pos;
-      c[i].pen = m.c[i].pen;
-      c[i].ln = c[i].lt0 = c[i].lt1 = c[i].kcoef = c[i].tm0 = c[i].tm1 = 0.0f;
-      c[i].anca = c[i].ancb = V(0, 0, 0);
-      c[i].stick = 0;
-      c[i].fp[0] = m.c[i].fp[0]; c[i].fp[1] = m.c[i].fp[1]; c[i].fp[2] = m.c[i].fp[2]; c[i].fp[3] = m.c[i].fp[3];
-      c[i].ws = 0;
-    }
     int32_t error = 0x7FFFFFFF;
-    // warm start: the previous step's manifold of this pair
-    // (find_old_collision + Manifold::update_from_old, collision_detection.slang:
-    // 914-963, src/shaders/extensions.slang:305-473)
+    uint32_t matched = 0;
+    // warm start: the previous step's manifold of this pair, found through
+    // the previous pair list's per-row offsets (lists ascend by partner)
     const uint32_t ida = d.cur.id[ga], idb = d.cur.id[gb];
-    const uint32_t pa = d.entries_prev[off + ida], pb = d.entries_prev[off + idb];
-    if (warm && pa < W.prev_n && pb < W.prev_n) {
-      const uint32_t lo = pa < pb ? pa : pb, hi = pa < pb ? pb : pa;
-      uint32_t L0 = 0, L1 = Pprev;
-      while (L0 < L1) {
-        uint32_t mid = (L0 + L1) >> 1;
-        uint2 q = ppairs[mid];
-        if (q.x < lo || (q.x == lo && q.y < hi)) L0 = mid + 1;
-        else L1 = mid;
-      }
-      int4 oh = make_int4(0, 0, 0, 0);
-      uint32_t os = 0;
-      if (L0 < Pprev && ppairs[L0].x == lo && ppairs[L0].y == hi) {
-        os = W.pair_off + L0;
-        oh = d.man_prev.hdr[os];
-      }
-      if (oh.w > 0) {
-        const bool swapped = pa != lo;
-        const float sgn = swapped ? -1.0f : 1.0f;
-        const bool cross_key = m.key != oh.z || swapped;
-        const v3 on = ld3(d.man_prev.nrm[os]);
-        const int ocount = oh.w;
-        bool go = true;
-        if (cross_key) go = dot(n, on * sgn) > 0.7f;
-        if (go) {
-          uint32_t used_old = 0, matched_new = 0;
-          for (int i = 0; !cross_key && i < m.count; ++i) {
-            for (int j = 0; j < ocount; ++j) {
-              if (used_old & (1u << j)) continue;
-              const uint32_t oc = os * 8 + j;
-              uint4 ofp = d.man_prev.cfp[oc];
-              v3 opos = ld3(d.man_prev.cpos[oc]);
-              v3 dp = opos - c[i].pos;
-              bool feat = ofp.z == c[i].fp[2] && ofp.w == c[i].fp[3];
-              bool close = dot(dp, dp) < 0.1f * 0.1f;
-              if (feat && close) {
-                float4 ol = d.man_prev.clam[oc], om = d.man_prev.cmas[oc];
-                float4 oa = d.man_prev.cana[oc], ob = d.man_prev.canb[oc];
-                c[i].ln = ol.x; c[i].lt0 = ol.y; c[i].lt1 = ol.z; c[i].kcoef = ol.w;
-                c[i].tm0 = om.y; c[i].tm1 = om.z;
-                uint32_t ost = __float_as_uint(ob.w);
-                c[i].stick = ost;
-                if (ost != 0) { c[i].anca = ld3(oa); c[i].ancb = ld3(ob); }
-                v3 d2 = c[i].pos - opos;
-                if (dot(d2, d2) < 0.01f) c[i].pos = opos;
-                uint32_t ows = d.man_prev.cws[oc];
-                c[i].ws = ows + 1u < 255u ? ows + 1u : 255u;
-                used_old |= 1u << j;
-                matched_new |= 1u << i;
-                break;
-              }
-            }
-          }
-          for (int round = 0; round < m.count; ++round) {
-            float best = 0.1225f;
-            int bi = -1, bj = -1;
-            for (int i = 0; i < m.count; ++i) {
-              if (matched_new & (1u << i)) continue;
-              for (int j = 0; j < ocount; ++j) {
-                if (used_old & (1u << j)) continue;
-                v3 dp = c[i].pos - ld3(d.man_prev.cpos[os * 8 + j]);
-                float dd2 = dot(dp, dp);
-                if (dd2 < best) { best = dd2; bi = i; bj = j; }
-              }
-            }
-            if (bi < 0) break;
-            const uint32_t oc = os * 8 + bj;
-            float4 ol = d.man_prev.clam[oc], om = d.man_prev.cmas[oc];
-            float4 oa = d.man_prev.cana[oc], ob = d.man_prev.canb[oc];
-            c[bi].ln = ol.x; c[bi].lt0 = ol.y * sgn; c[bi].lt1 = ol.z; c[bi].kcoef = ol.w;
-            c[bi].tm0 = om.y; c[bi].tm1 = om.z;
-            uint32_t ost = __float_as_uint(ob.w);
-            c[bi].stick = ost;
-            if (ost != 0) {
-              c[bi].anca = swapped ? ld3(ob) : ld3(oa);
-              c[bi].ancb = swapped ? ld3(oa) : ld3(ob);
-            }
-            uint4 ofp = d.man_prev.cfp[oc];
-            c[bi].fp[0] = ofp.x; c[bi].fp[1] = ofp.y; c[bi].fp[2] = ofp.z; c[bi].fp[3] = ofp.w;
-            uint32_t ows = d.man_prev.cws[oc];
-            c[bi].ws = ows + 1u < 255u ? ows + 1u : 255u;
-            matched_new |= 1u << bi;
-            used_old |= 1u << bj;
-          }
-          float deep_new = 0.0f, deep_old = 0.0f;
-          for (int i = 0; i < m.count; ++i) deep_new = fmn(deep_new, c[i].pen);
-          for (int j = 0; j < ocount; ++j) deep_old = fmn(deep_old, d.man_prev.cpos[os * 8 + j].w);
-          error = (int32_t)fcl((deep_new - deep_old) * 10000.0f, -32000.0f, 32000.0f);
-        }
+    if (warm && Pprev > 0) {
+      const uint32_t pa = d.entries_prev[off + ida], pb = d.entries_prev[off + idb];
+      if (pa < W.prev_n && pb < W.prev_n) {
+        const uint32_t lo = pa < pb ? pa : pb, hi = pa < pb ? pb : pa;
+        uint32_t L0 = d.pstart_prev[off + lo];
+        const uint32_t L1 = lo + 1 < W.prev_n ? d.pstart_prev[off + lo + 1] : Pprev;
+        const uint32_t end = L1 < Pprev ? L1 : Pprev;
+        while (L0 < end && ppairs[L0].y < hi) ++L0;
+        if (L0 < end && ppairs[L0].y == hi)
+          matched = warm_start(d, W.pair_off + L0, pa != lo, key, count, n, slot, sk.deep, error);
       }
     }
-    // write the manifold slot
-    d.man_cur.hdr[slot] = make_int4((int)pr.x, (int)pr.y, m.key, m.count);
+    // fresh contacts start from zero impulses
+    for (int i = 0; i < count; ++i) {
+      if (matched & (1u << i)) continue;
+      const uint32_t c = slot * 8 + i;
+      const float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      d.man_cur.clam[c] = z;
+      d.man_cur.cmas[c] = z;
+      d.man_cur.cana[c] = z;
+      d.man_cur.canb[c] = z;
+      d.man_cur.cws[c] = 0u;
+      ++fresh;
+    }
+    d.man_cur.hdr[slot] = make_int4((int)pr.x, (int)pr.y, key, count);
     d.man_cur.nrm[slot] = st3(n, __int_as_float(error));
     d.man_cur.t0[slot] = st3(t0, 0.0f);
     d.man_cur.t1[slot] = st3(t1, 0.0f);
     d.man_cur.anca[slot] = st3(A.pos, 0.0f);
     d.man_cur.ancb[slot] = st3(B.pos, 0.0f);
-    for (int i = 0; i < m.count; ++i) {
-      const uint32_t cs = slot * 8 + i;
-      d.man_cur.cpos[cs] = st3(c[i].pos, c[i].pen);
-      d.man_cur.clam[cs] = make_float4(c[i].ln, c[i].lt0, c[i].lt1, c[i].kcoef);
-      d.man_cur.cmas[cs] = make_float4(0.0f, c[i].tm0, c[i].tm1, 0.0f);
-      d.man_cur.cana[cs] = st3(c[i].anca, 0.0f);
-      d.man_cur.canb[cs] = st3(c[i].ancb, __uint_as_float(c[i].stick));
-      d.man_cur.cfp[cs] = make_uint4(c[i].fp[0], c[i].fp[1], c[i].fp[2], c[i].fp[3]);
-      d.man_cur.cws[cs] = c[i].ws;
-      if (c[i].ws == 0) ++fresh;
-      if (c[i].pen < 0.0f) {
-        uint32_t mm = (uint32_t)fmn(-c[i].pen * 1000.0f, 1.0e9f);
-        pen_mm = pen_mm > mm ? pen_mm : mm;
-      }
+    if (sk.deep < 0.0f) {
+      const uint32_t mm = (uint32_t)fmn(-sk.deep * 1000.0f, 1.0e9f);
+      pen_mm = pen_mm > mm ? pen_mm : mm;
     }
-    ccount += (uint32_t)m.count;
+    ccount += (uint32_t)count;
     ++mcount;
     atomicAdd(&d.deg[ga], 1u);
     atomicAdd(&d.deg[gb], 1u);
