@@ -75,6 +75,29 @@ class StepStats(C.Structure):
         return {n: getattr(self, n) for n, _ in self._fields_}
 
 
+class ServerDesc(C.Structure):
+    _fields_ = [
+        ("hbm_budget_bytes", C.c_uint64),
+        ("max_worlds", C.c_uint32),
+        ("max_bodies", C.c_uint32),
+        ("max_resident_bodies", C.c_uint32),
+        ("block", C.c_uint32),
+    ]
+
+
+class ServerReport(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in (
+        "submitted", "admitted", "retired", "queued", "resident_worlds", "resident_bodies",
+        "bytes_in_use", "bytes_budget", "bytes_allocated", "server_steps", "world_steps",
+        "body_steps")] + [(n, C.c_double) for n in (
+        "device_ms", "step_ms_p50", "step_ms_p99", "queue_ms_p50", "queue_ms_p99")]
+
+    def as_dict(self) -> dict:
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_UNKNOWN = 0, 1, 2, 3
+
 LBVH_NODE = np.dtype([("minimum", "<f4", 3), ("maximum", "<f4", 3), ("left", "<i4"),
                       ("right", "<i4"), ("primitive_idx", "<u4"), ("pad", "<u4")])
 CONTACT_TAP = np.dtype([
@@ -95,6 +118,7 @@ KERNEL_NAMES = ("k_build", "k_broad", "k_narrow", "k_post", "k_pgs", "k_avbd", "
 KERNEL_COUNT = len(KERNEL_NAMES)
 COUNT_NAMES = ("bodies", "active", "pairs", "manifolds", "contacts", "color_work", "max_pen_mm", "sleeping")
 COUNT_FIELDS = len(COUNT_NAMES)
+PARAM_BLOCK, PARAM_PGS_LDS_ROWS = 0, 1
 
 STATUS = {0: "BB_OK", 1: "BB_ERR_INVALID", 2: "BB_ERR_CAPACITY", 3: "BB_ERR_DEVICE",
           4: "BB_ERR_NO_DEVICE", 5: "BB_ERR_OVERFLOW", 6: "BB_ERR_STATE", 7: "BB_ERR_BUDGET"}
@@ -145,11 +169,23 @@ EXPORTS = {
     "bb_batch_world_count": (C.c_uint32, [C.c_void_p]),
     "bb_batch_body_count": (C.c_uint32, [C.c_void_p]),
     "bb_batch_set_block_size": (C.c_int, [C.c_void_p, C.c_uint32]),
+    "bb_batch_set_param": (C.c_int, [C.c_void_p, C.c_int, C.c_uint32]),
+    "bb_batch_get_param": (C.c_uint32, [C.c_void_p, C.c_int]),
+    "bb_batch_autotune": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_float)]),
     "bb_batch_profile": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]),
     "bb_batch_energy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32]),
     "bb_world_energy": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
     "bb_batch_device_bytes": (C.c_uint64, [C.c_void_p]),
     "bb_batch_destroy": (None, [C.c_void_p]),
+    "bb_server_create": (C.c_int, [C.POINTER(ServerDesc), C.POINTER(C.c_void_p)]),
+    "bb_server_submit": (C.c_int, [C.c_void_p, C.POINTER(WorldDesc), C.c_void_p, C.c_uint32, C.c_uint32,
+                                   C.POINTER(C.c_uint64)]),
+    "bb_server_step": (C.c_int, [C.c_void_p, C.c_uint32]),
+    "bb_server_drain": (C.c_int, [C.c_void_p]),
+    "bb_server_poll": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(C.c_uint32), C.c_void_p, C.c_uint32,
+                                 C.POINTER(StepStats)]),
+    "bb_server_report_get": (C.c_int, [C.c_void_p, C.POINTER(ServerReport)]),
+    "bb_server_destroy": (None, [C.c_void_p]),
     "bb_version": (C.c_char_p, []),
     "bb_copy_bandwidth": (C.c_int, [C.c_uint64, C.c_uint32, C.POINTER(C.c_float)]),
 }
@@ -329,6 +365,19 @@ class Batch:
     def set_block_size(self, threads: int) -> None:
         _check(lib().bb_batch_set_block_size(self._h, threads), "bb_batch_set_block_size")
 
+    def set_param(self, p: int, value: int) -> None:
+        _check(lib().bb_batch_set_param(self._h, p, value), "bb_batch_set_param")
+
+    def get_param(self, p: int) -> int:
+        return lib().bb_batch_get_param(self._h, p)
+
+    def autotune(self, reps: int = 2) -> float:
+        """Times every launch-parameter candidate on replays of the next step
+        (state unchanged afterwards) and keeps the fastest; returns its ms."""
+        ms = C.c_float(0.0)
+        _check(lib().bb_batch_autotune(self._h, reps, C.byref(ms)), "bb_batch_autotune")
+        return ms.value
+
     @property
     def world_count(self) -> int:
         return lib().bb_batch_world_count(self._h)
@@ -355,6 +404,58 @@ class Batch:
     def destroy(self) -> None:
         if self._h:
             lib().bb_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+class Server:
+    """Serving loop over the C-ABI (bb_server_*): continuous batching of
+    worlds admitted by HBM budget, retired when their steps are done."""
+
+    def __init__(self, budget_bytes: int, max_worlds: int = 1024, max_bodies: int = 0,
+                 max_resident_bodies: int = 0, block: int = 0):
+        self._h = C.c_void_p()
+        d = ServerDesc(budget_bytes, max_worlds, max_bodies, max_resident_bodies, block)
+        _check(lib().bb_server_create(C.byref(d), C.byref(self._h)), "bb_server_create")
+
+    def submit(self, desc: WorldDesc, bodies: np.ndarray, steps: int) -> int:
+        arr, ptr = _bodies_ptr(bodies)
+        t = C.c_uint64(0)
+        _check(lib().bb_server_submit(self._h, C.byref(desc), ptr, len(arr), steps, C.byref(t)),
+               "bb_server_submit")
+        return t.value
+
+    def step(self, nsteps: int = 1) -> None:
+        _check(lib().bb_server_step(self._h, nsteps), "bb_server_step")
+
+    def drain(self) -> None:
+        _check(lib().bb_server_drain(self._h), "bb_server_drain")
+
+    def poll(self, ticket: int, n: int = 0):
+        """(state, bodies or None, stats or None); results are released once
+        returned."""
+        st = C.c_uint32(0)
+        out = np.zeros(max(n, 1), BODY_STATE)
+        stats = StepStats()
+        _check(lib().bb_server_poll(self._h, ticket, C.byref(st), out.ctypes.data_as(C.c_void_p), max(n, 1),
+                                    C.byref(stats)), "bb_server_poll")
+        if st.value == REQ_DONE:
+            return st.value, out[:n], stats.as_dict()
+        return st.value, None, None
+
+    def report(self) -> dict:
+        r = ServerReport()
+        _check(lib().bb_server_report_get(self._h, C.byref(r)), "bb_server_report_get")
+        return r.as_dict()
+
+    def destroy(self) -> None:
+        if self._h:
+            lib().bb_server_destroy(self._h)
             self._h = C.c_void_p()
 
     def __del__(self):

@@ -13,88 +13,17 @@
 #include <new>
 #include <vector>
 
-#include "../../include/bb_capi.h"
-#include "bb_dev.hpp"
-
-namespace bb {
-void launch_step(const Dev& d, hipStream_t st, uint32_t block);
-void launch_step_timed(const Dev& d, hipStream_t st, uint32_t block, hipEvent_t* ev);
-void launch_copy(const float4* src, float4* dst, size_t n, hipStream_t st);
-void launch_sum_counts(const Dev& d, unsigned long long* out, hipStream_t st);
-void launch_energy(const Dev& d, double* out, hipStream_t st);
-}  // namespace bb
+#include "bb_host.hpp"
 
 using namespace bb;
-
-namespace {
-
-#define HIPCHK(x)                                                         \
-  do {                                                                    \
-    hipError_t e_ = (x);                                                  \
-    if (e_ != hipSuccess) {                                               \
-      std::fprintf(stderr, "beatbox: %s failed: %s\n", #x, hipGetErrorString(e_)); \
-      return BB_ERR_DEVICE;                                               \
-    }                                                                     \
-  } while (0)
-
-struct HostWorld {
-  bb_world_desc desc;
-  std::vector<bb_body_desc> bodies;  // id order
-  bool skip_ws_once = false;
-  bool drop_prev = false;
-  uint32_t body_off = 0, act_off = 0, pair_off = 0, pair_cap = 0, n = 0, a = 0;
-};
-
-struct Buffers {
-  std::vector<void*> ptrs;
-  uint64_t bytes = 0;
-  template <class T>
-  T* alloc(size_t count) {
-    void* p = nullptr;
-    size_t n = std::max<size_t>(count, 1) * sizeof(T);
-    if (hipMalloc(&p, n) != hipSuccess) p = nullptr;
-    else bytes += n;
-    ptrs.push_back(p);
-    return (T*)p;
-  }
-  void release() {
-    for (void* p : ptrs)
-      if (p) (void)hipFree(p);
-    ptrs.clear();
-    bytes = 0;
-  }
-};
-
-}  // namespace
-
-struct bb_batch {
-  std::vector<HostWorld> worlds;
-  Buffers buf;
-  bool built = false;
-  bool dirty = true;
-  bool stepped = false;
-  Dev base{};  // pointers common to both parities
-  Rows rows[2];
-  uint32_t* entries[2] = {nullptr, nullptr};
-  uint2* pairs[2] = {nullptr, nullptr};
-  uint32_t* npairs[2] = {nullptr, nullptr};
-  uint32_t* pstart[2] = {nullptr, nullptr};
-  Manifolds man[2];
-  int latest = 0;  // buffer set holding the newest state
-  uint32_t NB = 0, NA = 0, NP = 0;
-  hipStream_t stream = nullptr;
-  uint32_t block = 0;
-  uint64_t step_index = 0;
-  std::vector<WorldInfo> infos;
-  hipGraphExec_t graph = nullptr;
-  int graph_parity = -1;
-};
+using namespace bb::host;
 
 struct bb_world {
   bb_batch* b;
 };
 
-namespace {
+namespace bb {
+namespace host {
 
 uint32_t pair_capacity(uint32_t n) { return 8u * n + 64u; }
 
@@ -179,37 +108,15 @@ bb_status pull_state(bb_batch* b) {
   return BB_OK;
 }
 
-bb_status build(bb_batch* b) {
-  if (!b->dirty) return BB_OK;
-  if (!has_device()) return BB_ERR_NO_DEVICE;
-  bb_status st = pull_state(b);
-  if (st != BB_OK) return st;
-  if (b->graph) {
-    (void)hipGraphExecDestroy(b->graph);
-    b->graph = nullptr;
-  }
-  if (!b->stream) HIPCHK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
-  b->buf.release();
-  // layout
-  uint32_t NB = 0, NA = 0, NP = 0;
-  for (auto& hw : b->worlds) {
-    hw.n = (uint32_t)hw.bodies.size();
-    hw.a = 0;
-    for (auto& bd : hw.bodies)
-      if (bd.flags & BB_BODY_DYNAMIC) ++hw.a;
-    hw.body_off = NB;
-    hw.act_off = NA;
-    hw.pair_off = NP;
-    hw.pair_cap = pair_capacity(hw.n);
-    NB += hw.n;
-    NA += hw.a;
-    NP += hw.pair_cap;
-  }
-  b->NB = NB;
-  b->NA = NA;
-  b->NP = NP;
-  const uint32_t W = (uint32_t)b->worlds.size();
-  Buffers& A = b->buf;
+// Rows per world whose velocities the PGS sweeps keep in LDS: the tuning
+// value, or the largest world when that is smaller.
+uint32_t pgs_lds_rows(const bb_batch* b) {
+  uint32_t mx = 0;
+  for (auto& hw : b->worlds) mx = std::max(mx, (uint32_t)hw.bodies.size());
+  return std::min(mx, b->pgs_lds);
+}
+
+bb_status alloc_arrays(bb_batch* b, Buffers& A, uint32_t W, uint32_t NB, uint32_t NA, uint32_t NP) {
   Dev& d = b->base;
   d = Dev{};
   d.W = W;
@@ -275,10 +182,9 @@ bb_status build(bb_batch* b) {
   d.bucket_off = A.alloc<uint32_t>((size_t)W * (kBuckets + 1));
   d.iw = A.alloc<float4>((size_t)NB * 3);
   d.np = NP;
+  d.pgs_lcap = 0;  // set after the layout (pgs_lds_rows)
   d.sv_hdr = A.alloc<int4>(NP);
   d.sv_n = A.alloc<float4>(NP);
-  d.sv_t0 = A.alloc<float4>(NP);
-  d.sv_t1 = A.alloc<float4>(NP);
   d.sv_r1 = A.alloc<float4>((size_t)NP * 8);
   d.sv_r2 = A.alloc<float4>((size_t)NP * 8);
   d.sv_l = A.alloc<float4>((size_t)NP * 8);
@@ -296,81 +202,188 @@ bb_status build(bb_batch* b) {
   d.bbucket = A.alloc<uint32_t>(NB);
   d.bbucket_off = A.alloc<uint32_t>((size_t)W * (kAvbdColors * kAvbdLayers + 1));
   d.stats = A.alloc<uint32_t>((size_t)W * ST_COUNT);
+  if (A.count_only) return BB_OK;
   if (std::find(A.ptrs.begin(), A.ptrs.end(), nullptr) != A.ptrs.end()) return BB_ERR_CAPACITY;
-  // host images
-  std::vector<uint32_t> id(NB), fl(NB), zero(NB, 0), ent(NB), rank(NB), aids(std::max(NA, 1u));
-  std::vector<float4> pos(NB), rot(NB), vel(NB), omg(NB), z4(NB, make_float4(0, 0, 0, 0));
-  std::vector<float4> pmin(NB), pmax(NB), mass(NB), i0(NB), i1(NB), i2(NB);
+  // tables that must start at zero: colour owner cells (zero between
+  // rounds), counters, previous pair counts and pair-row offsets
+  HIPCHK(hipMemset(d.owner, 0, (size_t)NB * kColors * 4));
+  HIPCHK(hipMemset(d.stats, 0, (size_t)W * ST_COUNT * 4));
+  for (int s = 0; s < 2; ++s) {
+    HIPCHK(hipMemset(b->npairs[s], 0, (size_t)W * 4));
+    HIPCHK(hipMemset(b->pstart[s], 0, (size_t)NB * 4));
+    HIPCHK(hipMemset(b->rows[s].sleep, 0, (size_t)NB * 4));
+    HIPCHK(hipMemset(b->rows[s].pvel, 0, (size_t)NB * 16));
+  }
+  return BB_OK;
+}
+
+uint64_t device_bytes(uint32_t W, uint32_t NB, uint32_t NA, uint32_t NP) {
+  bb_batch tmp;
+  Buffers A;
+  A.count_only = true;
+  (void)alloc_arrays(&tmp, A, W, NB, NA, NP);
+  return A.bytes;
+}
+
+Images carve_images(void* mem, uint32_t nb, uint32_t na) {
+  char* p = (char*)mem;
+  Images img;
+  auto f4 = [&](float4*& dst) {
+    dst = (float4*)p;
+    p += (size_t)nb * 16;
+  };
+  f4(img.pos);
+  f4(img.rot);
+  f4(img.vel);
+  f4(img.omg);
+  f4(img.pmin);
+  f4(img.pmax);
+  f4(img.mass);
+  f4(img.ii0);
+  f4(img.ii1);
+  f4(img.ii2);
+  auto u4 = [&](uint32_t*& dst, size_t n) {
+    dst = (uint32_t*)p;
+    p += n * 4;
+  };
+  u4(img.id, nb);
+  u4(img.flags, nb);
+  u4(img.entries, nb);
+  u4(img.rank, nb);
+  u4(img.active_ids, na);
+  return img;
+}
+
+void fill_images(const HostWorld& hw, uint32_t r0, uint32_t a0, const Images& img) {
+  uint32_t k = 0;
+  for (uint32_t i = 0; i < hw.n; ++i) {
+    const bb_body_desc& bd = hw.bodies[i];
+    const uint32_t g = r0 + i;
+    img.id[g] = i;
+    img.flags[g] = bd.flags & (BB_BODY_DYNAMIC | BB_BODY_GRAVITY);
+    img.entries[g] = i;
+    img.pos[g] = make_float4(bd.position[0], bd.position[1], bd.position[2], 0.0f);
+    img.rot[g] = make_float4(bd.rotation[0], bd.rotation[1], bd.rotation[2], bd.rotation[3]);
+    img.vel[g] = make_float4(bd.velocity[0], bd.velocity[1], bd.velocity[2], 0.0f);
+    img.omg[g] = make_float4(bd.omega[0], bd.omega[1], bd.omega[2], 0.0f);
+    const float r = bd.radius;
+    if (bd.shape == BB_SHAPE_SPHERE) {
+      img.pmin[g] = make_float4(-r, -r, -r, r);
+      img.pmax[g] = make_float4(r, r, r, 1.0f);
+    } else {
+      img.pmin[g] = make_float4(bd.minimum[0], bd.minimum[1], bd.minimum[2], r);
+      img.pmax[g] = make_float4(bd.maximum[0], bd.maximum[1], bd.maximum[2], 0.0f);
+    }
+    const float inv_mass = bd.mass == 0.0f ? 0.0f : 1.0f / bd.mass;
+    img.mass[g] = make_float4(bd.mass, inv_mass, bd.friction, bd.restitution);
+    float inv[3];
+    inertia_of(hw.desc, bd, inv);
+    img.ii0[g] = make_float4(inv[0], 0.0f, 0.0f, 0.0f);
+    img.ii1[g] = make_float4(0.0f, inv[1], 0.0f, 0.0f);
+    img.ii2[g] = make_float4(0.0f, 0.0f, inv[2], 0.0f);
+    if (bd.flags & BB_BODY_DYNAMIC) {
+      img.rank[g] = k;
+      img.active_ids[a0 + k] = i;
+      ++k;
+    } else {
+      img.rank[g] = kMAX;
+    }
+  }
+}
+
+bb_status upload_images(bb_batch* b, const Images& img, uint32_t body_lo, uint32_t nb, uint32_t act_lo,
+                        uint32_t na, hipStream_t st) {
+  if (nb == 0) return BB_OK;
+  auto up = [&](void* dst, const void* src, size_t bytes) {
+    return hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st);
+  };
+  const size_t o = body_lo;
+  for (int s = 0; s < 2; ++s) {
+    Rows& R = b->rows[s];
+    HIPCHK(up(R.id + o, img.id, nb * 4));
+    HIPCHK(up(R.flags + o, img.flags, nb * 4));
+    HIPCHK(up(R.pos + o, img.pos, nb * 16));
+    HIPCHK(up(R.rot + o, img.rot, nb * 16));
+    HIPCHK(up(R.vel + o, img.vel, nb * 16));
+    HIPCHK(up(R.omg + o, img.omg, nb * 16));
+    HIPCHK(up(b->entries[s] + o, img.entries, nb * 4));
+  }
+  Props& P = b->base.props;
+  HIPCHK(up(P.pmin + o, img.pmin, nb * 16));
+  HIPCHK(up(P.pmax + o, img.pmax, nb * 16));
+  HIPCHK(up(P.mass + o, img.mass, nb * 16));
+  HIPCHK(up(P.ii0 + o, img.ii0, nb * 16));
+  HIPCHK(up(P.ii1 + o, img.ii1, nb * 16));
+  HIPCHK(up(P.ii2 + o, img.ii2, nb * 16));
+  HIPCHK(up(P.active_rank + o, img.rank, nb * 4));
+  if (na) HIPCHK(up(P.active_ids + act_lo, img.active_ids, na * 4));
+  return BB_OK;
+}
+
+bb_status reset_world_state(bb_batch* b, uint32_t w, hipStream_t st) {
+  const HostWorld& hw = b->worlds[w];
+  for (int s = 0; s < 2; ++s) {
+    HIPCHK(hipMemsetAsync(b->npairs[s] + w, 0, 4, st));
+    if (hw.n) {
+      HIPCHK(hipMemsetAsync(b->pstart[s] + hw.body_off, 0, (size_t)hw.n * 4, st));
+      HIPCHK(hipMemsetAsync(b->rows[s].sleep + hw.body_off, 0, (size_t)hw.n * 4, st));
+      HIPCHK(hipMemsetAsync(b->rows[s].pvel + hw.body_off, 0, (size_t)hw.n * 16, st));
+    }
+  }
+  HIPCHK(hipMemsetAsync(b->base.stats + (size_t)w * ST_COUNT, 0, ST_COUNT * 4, st));
+  return BB_OK;
+}
+
+bb_status build(bb_batch* b) {
+  if (!b->dirty) return BB_OK;
+  if (!has_device()) return BB_ERR_NO_DEVICE;
+  bb_status st = pull_state(b);
+  if (st != BB_OK) return st;
+  if (b->graph) {
+    (void)hipGraphExecDestroy(b->graph);
+    b->graph = nullptr;
+  }
+  if (!b->stream) HIPCHK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  b->buf.release();
+  // layout: worlds back to back
+  uint32_t NB = 0, NA = 0, NP = 0;
+  for (auto& hw : b->worlds) {
+    hw.n = (uint32_t)hw.bodies.size();
+    hw.a = 0;
+    for (auto& bd : hw.bodies)
+      if (bd.flags & BB_BODY_DYNAMIC) ++hw.a;
+    hw.body_off = NB;
+    hw.act_off = NA;
+    hw.pair_off = NP;
+    hw.pair_cap = pair_capacity(hw.n);
+    NB += hw.n;
+    NA += hw.a;
+    NP += hw.pair_cap;
+  }
+  b->NB = NB;
+  b->NA = NA;
+  b->NP = NP;
+  const uint32_t W = (uint32_t)b->worlds.size();
+  st = alloc_arrays(b, b->buf, W, NB, NA, NP);
+  if (st != BB_OK) return st;
+  std::vector<char> mem(kImageBodyBytes * (size_t)std::max(NB, 1u) + kImageActiveBytes * std::max(NA, 1u) + 64);
+  void* base = (void*)(((uintptr_t)mem.data() + 15) & ~(uintptr_t)15);
+  const Images img = carve_images(base, std::max(NB, 1u), std::max(NA, 1u));
   b->infos.resize(W);
   for (uint32_t w = 0; w < W; ++w) {
     HostWorld& hw = b->worlds[w];
     WorldInfo wi = info_of(hw);
     wi.drop_prev = 1;  // fresh layout: no previous manifolds
     b->infos[w] = wi;
-    uint32_t k = 0;
-    for (uint32_t i = 0; i < hw.n; ++i) {
-      const bb_body_desc& bd = hw.bodies[i];
-      uint32_t g = hw.body_off + i;
-      id[g] = i;
-      fl[g] = bd.flags & (BB_BODY_DYNAMIC | BB_BODY_GRAVITY);
-      ent[g] = i;
-      pos[g] = make_float4(bd.position[0], bd.position[1], bd.position[2], 0.0f);
-      rot[g] = make_float4(bd.rotation[0], bd.rotation[1], bd.rotation[2], bd.rotation[3]);
-      vel[g] = make_float4(bd.velocity[0], bd.velocity[1], bd.velocity[2], 0.0f);
-      omg[g] = make_float4(bd.omega[0], bd.omega[1], bd.omega[2], 0.0f);
-      float r = bd.radius;
-      if (bd.shape == BB_SHAPE_SPHERE) {
-        pmin[g] = make_float4(-r, -r, -r, r);
-        pmax[g] = make_float4(r, r, r, 1.0f);
-      } else {
-        pmin[g] = make_float4(bd.minimum[0], bd.minimum[1], bd.minimum[2], r);
-        pmax[g] = make_float4(bd.maximum[0], bd.maximum[1], bd.maximum[2], 0.0f);
-      }
-      float inv_mass = bd.mass == 0.0f ? 0.0f : 1.0f / bd.mass;
-      mass[g] = make_float4(bd.mass, inv_mass, bd.friction, bd.restitution);
-      float inv[3];
-      inertia_of(hw.desc, bd, inv);
-      i0[g] = make_float4(inv[0], 0.0f, 0.0f, 0.0f);
-      i1[g] = make_float4(0.0f, inv[1], 0.0f, 0.0f);
-      i2[g] = make_float4(0.0f, 0.0f, inv[2], 0.0f);
-      if (bd.flags & BB_BODY_DYNAMIC) {
-        rank[g] = k;
-        aids[hw.act_off + k] = i;
-        ++k;
-      } else {
-        rank[g] = kMAX;
-      }
-    }
+    fill_images(hw, hw.body_off, hw.act_off, img);
   }
-  auto up = [&](void* dst, const void* src, size_t bytes) {
-    return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
-  };
-  for (int s = 0; s < 2; ++s) {
-    Rows& R = b->rows[s];
-    HIPCHK(up(R.id, id.data(), NB * 4));
-    HIPCHK(up(R.flags, fl.data(), NB * 4));
-    HIPCHK(up(R.sleep, zero.data(), NB * 4));
-    HIPCHK(up(R.pos, pos.data(), NB * 16));
-    HIPCHK(up(R.rot, rot.data(), NB * 16));
-    HIPCHK(up(R.vel, vel.data(), NB * 16));
-    HIPCHK(up(R.omg, omg.data(), NB * 16));
-    HIPCHK(up(R.pvel, z4.data(), NB * 16));
-    HIPCHK(up(b->entries[s], ent.data(), NB * 4));
-    HIPCHK(hipMemset(b->npairs[s], 0, W * 4));
-    HIPCHK(hipMemset(b->pstart[s], 0, NB * 4));
-  }
-  HIPCHK(up(P.pmin, pmin.data(), NB * 16));
-  HIPCHK(up(P.pmax, pmax.data(), NB * 16));
-  HIPCHK(up(P.mass, mass.data(), NB * 16));
-  HIPCHK(up(P.ii0, i0.data(), NB * 16));
-  HIPCHK(up(P.ii1, i1.data(), NB * 16));
-  HIPCHK(up(P.ii2, i2.data(), NB * 16));
-  HIPCHK(up(P.active_rank, rank.data(), NB * 4));
-  HIPCHK(up(P.active_ids, aids.data(), std::max(NA, 1u) * 4));
-  HIPCHK(hipMemset(d.owner, 0, (size_t)NB * kColors * 4));
-  HIPCHK(hipMemset(d.stats, 0, (size_t)W * ST_COUNT * 4));
-  HIPCHK(up(d.worlds, b->infos.data(), W * sizeof(WorldInfo)));
+  st = upload_images(b, img, 0, NB, 0, NA, b->stream);
+  if (st != BB_OK) return st;
+  HIPCHK(hipMemcpyAsync(b->base.worlds, b->infos.data(), W * sizeof(WorldInfo), hipMemcpyHostToDevice, b->stream));
+  HIPCHK(hipStreamSynchronize(b->stream));
   b->latest = 1;  // the first step reads set 1 and writes set 0
+  b->base.pgs_lcap = pgs_lds_rows(b);
   b->built = true;
   b->dirty = false;
   b->stepped = false;
@@ -568,6 +581,107 @@ bb_status energy_batch(bb_batch* b, double* out, uint32_t n) {
   return rc == hipSuccess ? BB_OK : BB_ERR_DEVICE;
 }
 
+bb_status set_param(bb_batch* b, bb_param p, uint32_t value) {
+  if (p == BB_PARAM_BLOCK) {
+    if (value != 0 && value != 256 && value != 512 && value != 1024) return BB_ERR_INVALID;
+    b->block = value;
+  } else if (p == BB_PARAM_PGS_LDS_ROWS) {
+    if (value > 2048) return BB_ERR_INVALID;
+    b->pgs_lds = value;
+    b->base.pgs_lcap = pgs_lds_rows(b);
+  } else {
+    return BB_ERR_INVALID;
+  }
+  if (b->graph) {  // kernel arguments and launch shapes are baked into the graph
+    (void)hipStreamSynchronize(b->stream);
+    (void)hipGraphExecDestroy(b->graph);
+    b->graph = nullptr;
+  }
+  return BB_OK;
+}
+
+// Every step reads only the newest parity (plus properties and the world
+// table) and writes the other one, so the same step can be replayed from the
+// same state: the tuner times each candidate on identical work and then
+// restores the parity, step counter and counters it started from.
+bb_status autotune_batch(bb_batch* b, uint32_t reps, float* best_ms) {
+  if (b->worlds.empty() || reps == 0) return BB_ERR_INVALID;
+  bb_status st = build(b);
+  if (st != BB_OK) return st;
+  while (any_one_shot(b)) {  // one-shot steps run for real first
+    st = step_batch(b, 1, false);
+    if (st != BB_OK) return st;
+  }
+  HIPCHK(hipStreamSynchronize(b->stream));
+  const int saved = b->latest;
+  const uint64_t saved_index = b->step_index;
+  const bool saved_stepped = b->stepped;
+  const uint32_t saved_block = b->block, saved_lds = b->pgs_lds;
+  const size_t stat_bytes = b->worlds.size() * ST_COUNT * sizeof(uint32_t);
+  uint32_t* stat_copy = nullptr;
+  HIPCHK(hipMalloc(&stat_copy, stat_bytes));
+  HIPCHK(hipMemcpyAsync(stat_copy, b->base.stats, stat_bytes, hipMemcpyDeviceToDevice, b->stream));
+  uint32_t mx = 0;
+  for (auto& hw : b->worlds) mx = std::max(mx, (uint32_t)hw.bodies.size());
+  std::vector<uint32_t> lds_opts;
+  for (uint32_t v : {0u, 256u, 512u, 1024u, 2048u}) {
+    lds_opts.push_back(v);
+    if (v >= mx) break;
+  }
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  float best = 3.4e38f;
+  uint32_t best_block = saved_block, best_lds = saved_lds;
+  bb_status rc = BB_OK;
+  for (uint32_t blk : {256u, 512u, 1024u}) {
+    for (uint32_t lds : lds_opts) {
+      b->block = blk;
+      b->pgs_lds = lds;
+      b->base.pgs_lcap = pgs_lds_rows(b);
+      b->latest = saved;
+      if ((rc = launch_one(b)) != BB_OK) break;  // untimed first launch
+      (void)hipEventRecord(e0, b->stream);
+      for (uint32_t r = 0; r < reps && rc == BB_OK; ++r) {
+        b->latest = saved;
+        rc = launch_one(b);
+      }
+      (void)hipEventRecord(e1, b->stream);
+      if (rc != BB_OK || hipEventSynchronize(e1) != hipSuccess) {
+        rc = rc != BB_OK ? rc : BB_ERR_DEVICE;
+        break;
+      }
+      float ms = 0.0f;
+      (void)hipEventElapsedTime(&ms, e0, e1);
+      ms /= (float)reps;
+      if (ms < best) {
+        best = ms;
+        best_block = blk;
+        best_lds = lds;
+      }
+    }
+    if (rc != BB_OK) break;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  b->latest = saved;
+  b->step_index = saved_index;
+  b->stepped = saved_stepped;
+  if (hipMemcpyAsync(b->base.stats, stat_copy, stat_bytes, hipMemcpyDeviceToDevice, b->stream) != hipSuccess ||
+      hipStreamSynchronize(b->stream) != hipSuccess)
+    rc = rc != BB_OK ? rc : BB_ERR_DEVICE;
+  (void)hipFree(stat_copy);
+  b->block = rc == BB_OK ? best_block : saved_block;
+  b->pgs_lds = rc == BB_OK ? best_lds : saved_lds;
+  b->base.pgs_lcap = pgs_lds_rows(b);
+  if (b->graph) {
+    (void)hipGraphExecDestroy(b->graph);
+    b->graph = nullptr;
+  }
+  if (best_ms && rc == BB_OK) *best_ms = best;
+  return rc;
+}
+
 bb_status sync_batch(bb_batch* b) {
   if (!b->stream) return BB_OK;
   HIPCHK(hipStreamSynchronize(b->stream));
@@ -620,6 +734,11 @@ bb_status world_stats(bb_batch* b, uint32_t w, bb_step_stats* out) {
   HIPCHK(hipStreamSynchronize(b->stream));
   uint32_t s[ST_COUNT];
   HIPCHK(hipMemcpy(s, b->base.stats + (size_t)w * ST_COUNT, sizeof(s), hipMemcpyDeviceToHost));
+  stats_from_raw(s, b->step_index ? b->step_index - 1 : 0, out);
+  return BB_OK;
+}
+
+void stats_from_raw(const uint32_t* s, uint64_t step_index, bb_step_stats* out) {
   out->rigid_body_count = s[ST_N];
   out->active_rigid_body_count = s[ST_A];
   out->broad_phase_collision_count = s[ST_P];
@@ -640,8 +759,7 @@ bb_status world_stats(bb_batch* b, uint32_t w, bb_step_stats* out) {
   out->dbg_id_sum = s[ST_IDSUM];
   out->dbg_maxv = s[ST_MAXV];
   out->overflow = s[ST_OVERFLOW];
-  out->step_index = b->step_index ? b->step_index - 1 : 0;
-  return BB_OK;
+  out->step_index = step_index;
 }
 
 template <class T>
@@ -842,6 +960,12 @@ bool valid_desc(const bb_world_desc* d) {
   return true;
 }
 
+bool over_reference_cap(const bb_world_desc& d, uint32_t n) {
+  // parity mode keeps the reference's cap (MAX_RIGID_BODY_COUNT = 1024,
+  // include/defines.hpp:357; build_accel_structs refuses larger scenes)
+  return d.morton_bits == 30 && n > 1024 && (d.quirks & BB_QUIRKS_PARITY) == BB_QUIRKS_PARITY;
+}
+
 bool valid_body(const bb_body_desc& b) {
   if (b.shape > BB_SHAPE_SPHERE) return false;
   if (!(b.mass >= 0.0f) || !std::isfinite(b.mass)) return false;
@@ -856,7 +980,8 @@ bool valid_body(const bb_body_desc& b) {
   return true;
 }
 
-}  // namespace
+}  // namespace host
+}  // namespace bb
 
 // =================================================================== C ABI
 extern "C" {
@@ -921,11 +1046,7 @@ bb_status bb_batch_add_world(bb_batch* b, const bb_world_desc* desc, const bb_bo
   for (uint32_t i = 0; i < n; ++i)
     if (!valid_body(bodies[i])) return BB_ERR_INVALID;
   if (desc->capacity_bodies && n > desc->capacity_bodies) return BB_ERR_CAPACITY;
-  if (desc->morton_bits == 30 && n > 1024 && (desc->quirks & BB_QUIRKS_PARITY) == BB_QUIRKS_PARITY) {
-    // parity mode keeps the reference's cap (MAX_RIGID_BODY_COUNT = 1024,
-    // include/defines.hpp:357; build_accel_structs refuses larger scenes)
-    return BB_ERR_CAPACITY;
-  }
+  if (over_reference_cap(*desc, n)) return BB_ERR_CAPACITY;
   bb_status st = pull_state(b);
   if (st != BB_OK) return st;
   HostWorld hw;
@@ -977,14 +1098,23 @@ uint32_t bb_batch_body_count(const bb_batch* b) {
 }
 
 bb_status bb_batch_set_block_size(bb_batch* b, uint32_t threads) {
+  return bb_batch_set_param(b, BB_PARAM_BLOCK, threads);
+}
+
+bb_status bb_batch_set_param(bb_batch* b, bb_param p, uint32_t value) {
   if (!b) return BB_ERR_INVALID;
-  if (threads != 0 && threads != 256 && threads != 512 && threads != 1024) return BB_ERR_INVALID;
-  b->block = threads;
-  if (b->graph) {
-    (void)hipGraphExecDestroy(b->graph);
-    b->graph = nullptr;
-  }
-  return BB_OK;
+  return set_param(b, p, value);
+}
+
+uint32_t bb_batch_get_param(const bb_batch* b, bb_param p) {
+  if (!b) return 0;
+  if (p == BB_PARAM_BLOCK) return pick_block(b);
+  if (p == BB_PARAM_PGS_LDS_ROWS) return pgs_lds_rows(b);
+  return 0;
+}
+
+bb_status bb_batch_autotune(bb_batch* b, uint32_t reps, float* best_ms) {
+  return b ? autotune_batch(b, reps, best_ms) : BB_ERR_INVALID;
 }
 
 void bb_batch_destroy(bb_batch* b) {
@@ -1020,8 +1150,7 @@ bb_status bb_world_add_bodies(bb_world* w, const bb_body_desc* bodies, uint32_t 
     if (!valid_body(bodies[i])) return BB_ERR_INVALID;
   size_t total = hw.bodies.size() + n;
   if (hw.desc.capacity_bodies && total > hw.desc.capacity_bodies) return BB_ERR_CAPACITY;
-  if (hw.desc.morton_bits == 30 && (hw.desc.quirks & BB_QUIRKS_PARITY) == BB_QUIRKS_PARITY && total > 1024)
-    return BB_ERR_CAPACITY;
+  if (over_reference_cap(hw.desc, (uint32_t)total)) return BB_ERR_CAPACITY;
   bb_status st = pull_state(w->b);
   if (st != BB_OK) return st;
   for (uint32_t i = 0; i < n; ++i) {

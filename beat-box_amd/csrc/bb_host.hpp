@@ -1,0 +1,159 @@
+// bb_host.hpp — host-side internals shared by the C-ABI (bb_capi.cpp) and
+// the serving loop (bb_server.cpp): the batch object, its device arrays, and
+// the helpers that lay worlds out in them and upload their initial state.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <vector>
+
+#include "../../include/bb_capi.h"
+#include "bb_dev.hpp"
+
+namespace bb {
+void launch_step(const Dev& d, hipStream_t st, uint32_t block);
+void launch_step_timed(const Dev& d, hipStream_t st, uint32_t block, hipEvent_t* ev);
+void launch_copy(const float4* src, float4* dst, size_t n, hipStream_t st);
+void launch_sum_counts(const Dev& d, unsigned long long* out, hipStream_t st);
+void launch_energy(const Dev& d, double* out, hipStream_t st);
+}  // namespace bb
+
+#define HIPCHK(x)                                                                  \
+  do {                                                                             \
+    hipError_t e_ = (x);                                                           \
+    if (e_ != hipSuccess) {                                                        \
+      std::fprintf(stderr, "beatbox: %s failed: %s\n", #x, hipGetErrorString(e_)); \
+      return BB_ERR_DEVICE;                                                        \
+    }                                                                              \
+  } while (0)
+
+namespace bb {
+namespace host {
+
+struct HostWorld {
+  bb_world_desc desc;
+  std::vector<bb_body_desc> bodies;  // id order
+  bool skip_ws_once = false;
+  bool drop_prev = false;
+  uint32_t body_off = 0, act_off = 0, pair_off = 0, pair_cap = 0, n = 0, a = 0;
+};
+
+// Device allocations of a batch.  With count_only set nothing is allocated
+// and `bytes` sums what would be (used to price a world for admission).
+struct Buffers {
+  std::vector<void*> ptrs;
+  uint64_t bytes = 0;
+  bool count_only = false;
+  template <class T>
+  T* alloc(size_t count) {
+    void* p = nullptr;
+    size_t n = std::max<size_t>(count, 1) * sizeof(T);
+    if (count_only) {
+      bytes += n;
+      return nullptr;
+    }
+    if (hipMalloc(&p, n) != hipSuccess) p = nullptr;
+    else bytes += n;
+    ptrs.push_back(p);
+    return (T*)p;
+  }
+  void release() {
+    for (void* p : ptrs)
+      if (p) (void)hipFree(p);
+    ptrs.clear();
+    bytes = 0;
+  }
+};
+
+// Host images of the initial state of a contiguous range of body rows (both
+// parities get the same rows) and active slots.  Pointers may address
+// pinned memory for asynchronous uploads.
+struct Images {
+  uint32_t* id;
+  uint32_t* flags;
+  uint32_t* entries;
+  uint32_t* rank;
+  uint32_t* active_ids;
+  float4* pos;
+  float4* rot;
+  float4* vel;
+  float4* omg;
+  float4* pmin;
+  float4* pmax;
+  float4* mass;
+  float4* ii0;
+  float4* ii1;
+  float4* ii2;
+};
+// Bytes of image data per body row and per active slot.
+constexpr size_t kImageBodyBytes = 4 * 4 + 10 * 16;
+constexpr size_t kImageActiveBytes = 4;
+// Carves an Images set for nb rows and na active slots out of `mem`
+// (kImageBodyBytes * nb + kImageActiveBytes * na bytes, 16-byte aligned).
+Images carve_images(void* mem, uint32_t nb, uint32_t na);
+
+}  // namespace host
+}  // namespace bb
+
+struct bb_batch {
+  std::vector<bb::host::HostWorld> worlds;
+  bb::host::Buffers buf;
+  bool built = false;
+  bool dirty = true;
+  bool stepped = false;
+  bb::Dev base{};  // pointers common to both parities
+  bb::Rows rows[2];
+  uint32_t* entries[2] = {nullptr, nullptr};
+  uint2* pairs[2] = {nullptr, nullptr};
+  uint32_t* npairs[2] = {nullptr, nullptr};
+  uint32_t* pstart[2] = {nullptr, nullptr};
+  bb::Manifolds man[2];
+  int latest = 0;  // buffer set holding the newest state
+  uint32_t NB = 0, NA = 0, NP = 0;
+  hipStream_t stream = nullptr;
+  uint32_t block = 0;
+  uint32_t pgs_lds = 1024;  // BB_PARAM_PGS_LDS_ROWS
+  uint64_t step_index = 0;
+  std::vector<bb::WorldInfo> infos;
+  hipGraphExec_t graph = nullptr;
+  int graph_parity = -1;
+};
+
+namespace bb {
+namespace host {
+
+uint32_t pair_capacity(uint32_t n);
+bool has_device();
+bool valid_desc(const bb_world_desc* d);
+bool valid_body(const bb_body_desc& b);
+// Parity mode keeps the reference cap of 1,024 bodies per world.
+bool over_reference_cap(const bb_world_desc& d, uint32_t n);
+WorldInfo info_of(const HostWorld& hw);
+// Allocates (or, with A.count_only, prices) every device array of a batch
+// with W world slots, NB body rows, NA active slots and NP pair slots, and
+// points b->base / rows / man at them.
+bb_status alloc_arrays(bb_batch* b, Buffers& A, uint32_t W, uint32_t NB, uint32_t NA, uint32_t NP);
+// Device bytes of a batch of these capacities.
+uint64_t device_bytes(uint32_t W, uint32_t NB, uint32_t NA, uint32_t NP);
+// Writes world hw's initial rows into img at row r0 and active slot a0 of
+// the image range (image index = device index - range start).
+void fill_images(const HostWorld& hw, uint32_t r0, uint32_t a0, const Images& img);
+// Copies an image range to the device: rows [body_lo, body_lo+nb) of both
+// parities and of the property arrays, active slots [act_lo, act_lo+na).
+bb_status upload_images(bb_batch* b, const Images& img, uint32_t body_lo, uint32_t nb, uint32_t act_lo,
+                        uint32_t na, hipStream_t st);
+// Clears the per-world device state a fresh world starts from: previous pair
+// counts (both parities), pair-row offsets, sleep timers, velocities of the
+// previous step, and the world's counters.
+bb_status reset_world_state(bb_batch* b, uint32_t w, hipStream_t st);
+uint32_t pgs_lds_rows(const bb_batch* b);
+// Counters of one world (raw device slots, ST_COUNT of them) as the C-ABI
+// struct.
+void stats_from_raw(const uint32_t* raw, uint64_t step_index, bb_step_stats* out);
+uint32_t pick_block(const bb_batch* b);
+Dev dev_for(const bb_batch* b, int cur);
+
+}  // namespace host
+}  // namespace bb

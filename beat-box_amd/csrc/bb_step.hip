@@ -283,9 +283,9 @@ __global__ __launch_bounds__(BLOCK) void k_build(Dev d) {
   const uint32_t w = blockIdx.x, tid = threadIdx.x;
   const WorldInfo W = d.worlds[w];
   const uint32_t N = W.n, off = W.body_off;
-  if (N == 0) return;
-  // per-step resets
+  // per-step resets (an empty world slot keeps zero counters)
   if (tid < ST_COUNT) d.stats[w * ST_COUNT + tid] = 0;
+  if (N == 0) return;
   for (uint32_t k = tid; k < W.a; k += BLOCK) d.link[W.act_off + k] = k;
   for (uint32_t i = tid; i < N; i += BLOCK) d.deg[off + i] = 0;
   // ---- Morton keys from the previous rows (RB_sim.slang:75-111)
@@ -411,46 +411,64 @@ BB_DI bool overlap(v3 amn, v3 amx, float4 bmn, float4 bmx) {  // Aabb::overlap, 
 }
 
 // LBVH::broad_phase (LBVH.slang:223-286): stack traversal from the root,
-// one pair (i, j) per overlapping leaf with row j > i.
-template <bool WRITE>
-BB_DI uint32_t traverse(const Dev& d, uint32_t nb, uint32_t N, uint32_t i, v3 qmn, v3 qmx, uint2* out,
-                        uint32_t cap_left) {
+// one pair (i, j) per overlapping leaf with row j > i.  Each node record
+// carries its children's indices (nmin.w, nmax.w), so descending needs one
+// dependent load per level and the stack holds child-index pairs.  emit(k, j)
+// receives the k-th partner j (traversal order); returns the partner count.
+template <class Emit>
+BB_DI uint32_t traverse(const Dev& d, uint32_t nb, uint32_t N, uint32_t i, v3 qmn, v3 qmx, Emit&& emit) {
   if (N < 2) return 0;
   // depth of a Karras tree is at most key bits + index bits (63 + 32 < 128)
   const int nn = (int)(2 * N - 1);
-  uint32_t stack[128];
+  int2 stack[128];
   uint32_t sp = 0;
-  stack[sp++] = 0;
-  uint32_t node = 0, cnt = 0;
-  do {
-    float4 pm = d.nmin[nb + node], px = d.nmax[nb + node];
-    int l = __float_as_int(pm.w), r = __float_as_int(px.w);
+  const float4 r0m = d.nmin[nb], r0x = d.nmax[nb];
+  int l = __float_as_int(r0m.w), r = __float_as_int(r0x.w);
+  uint32_t cnt = 0;
+  for (;;) {
     if (l < 0 || r < 0 || l >= nn || r >= nn) break;
-    float4 lm = d.nmin[nb + l], lx = d.nmax[nb + l], rm = d.nmin[nb + r], rx = d.nmax[nb + r];
-    bool ol = overlap(qmn, qmx, lm, lx), orr = overlap(qmn, qmx, rm, rx);
-    bool ll = __float_as_int(lm.w) < 0, rl = __float_as_int(rm.w) < 0;
-    uint32_t lp = (uint32_t)__float_as_int(lx.w), rp = (uint32_t)__float_as_int(rx.w);
-    if (ol && ll && i < lp) {
-      if (WRITE && cnt < cap_left) out[cnt] = make_uint2(i, lp);
-      ++cnt;
-    }
-    if (orr && rl && i < rp) {
-      if (WRITE && cnt < cap_left) out[cnt] = make_uint2(i, rp);
-      ++cnt;
-    }
-    bool tl = ol && !ll, tr = orr && !rl;
-    if (!tl && !tr) {
-      sp--;
-      if (sp == 0) break;
-      node = stack[sp];
+    const float4 lm = d.nmin[nb + l], lx = d.nmax[nb + l], rm = d.nmin[nb + r], rx = d.nmax[nb + r];
+    const bool ol = overlap(qmn, qmx, lm, lx), orr = overlap(qmn, qmx, rm, rx);
+    const bool ll = __float_as_int(lm.w) < 0, rl = __float_as_int(rm.w) < 0;
+    const uint32_t lp = (uint32_t)__float_as_int(lx.w), rp = (uint32_t)__float_as_int(rx.w);
+    if (ol && ll && i < lp) emit(cnt++, lp);
+    if (orr && rl && i < rp) emit(cnt++, rp);
+    const bool tl = ol && !ll, tr = orr && !rl;
+    if (tl) {
+      if (tr && sp < 128) stack[sp++] = make_int2(__float_as_int(rm.w), __float_as_int(rx.w));
+      l = __float_as_int(lm.w);
+      r = __float_as_int(lx.w);
+    } else if (tr) {
+      l = __float_as_int(rm.w);
+      r = __float_as_int(rx.w);
     } else {
-      if (tl && tr) {
-        if (sp < 128) stack[sp++] = (uint32_t)r;
-      }
-      node = tl ? (uint32_t)l : (uint32_t)r;
+      if (sp == 0) break;
+      const int2 c = stack[--sp];
+      l = c.x;
+      r = c.y;
     }
-  } while (sp > 0);
+  }
   return cnt;
+}
+
+// Rows keep up to kPairStage partners in a staging area (the colouring
+// scratch `pick`, whose 8n+64 slots per world hold 8 per row) during the
+// counting traversal; only rows with more partners traverse a second time.
+static constexpr uint32_t kPairStage = 8;
+
+BB_DI void cas_u32(uint32_t& a, uint32_t& b) {
+  const uint32_t lo = a < b ? a : b, hi = a < b ? b : a;
+  a = lo;
+  b = hi;
+}
+// Optimal 19-comparator sorting network for 8 keys (Knuth, TAOCP vol. 3).
+BB_DI void sort8(uint32_t (&k)[8]) {
+  cas_u32(k[0], k[2]); cas_u32(k[1], k[3]); cas_u32(k[4], k[6]); cas_u32(k[5], k[7]);
+  cas_u32(k[0], k[4]); cas_u32(k[1], k[5]); cas_u32(k[2], k[6]); cas_u32(k[3], k[7]);
+  cas_u32(k[0], k[1]); cas_u32(k[2], k[3]); cas_u32(k[4], k[5]); cas_u32(k[6], k[7]);
+  cas_u32(k[2], k[4]); cas_u32(k[3], k[5]);
+  cas_u32(k[1], k[4]); cas_u32(k[3], k[6]);
+  cas_u32(k[1], k[2]); cas_u32(k[3], k[4]); cas_u32(k[5], k[6]);
 }
 
 template <int BLOCK>
@@ -460,6 +478,7 @@ __global__ __launch_bounds__(BLOCK) void k_broad(Dev d) {
   const uint32_t w = blockIdx.x, tid = threadIdx.x;
   const WorldInfo W = d.worlds[w];
   const uint32_t N = W.n, off = W.body_off, nb = 2 * off;
+  uint32_t* stage = d.pick + W.pair_off;
   if (tid == 0) carry = 0;
   __syncthreads();
   for (uint32_t c0 = 0; c0 < N; c0 += BLOCK) {
@@ -467,7 +486,10 @@ __global__ __launch_bounds__(BLOCK) void k_broad(Dev d) {
     uint32_t cnt = 0;
     if (i < N) {
       float4 mn = d.nmin[nb + N - 1 + i], mx = d.nmax[nb + N - 1 + i];
-      cnt = traverse<false>(d, nb, N, i, ld3(mn), ld3(mx), nullptr, 0);
+      uint32_t* st = stage + i * kPairStage;
+      cnt = traverse(d, nb, N, i, ld3(mn), ld3(mx), [&](uint32_t k, uint32_t j) {
+        if (k < kPairStage) st[k] = j;
+      });
       d.pcount[off + i] = cnt;
     }
     uint32_t total;
@@ -489,21 +511,39 @@ __global__ __launch_bounds__(BLOCK) void k_broad(Dev d) {
   }
   uint2* pairs = d.pairs_cur + W.pair_off;
   for (uint32_t i = tid; i < N; i += BLOCK) {
-    uint32_t o = d.poff[off + i];
+    const uint32_t o = d.poff[off + i];
     if (o >= cap) continue;
-    uint32_t room = cap - o;
+    const uint32_t room = cap - o;
+    uint32_t cnt = d.pcount[off + i];
+    const uint32_t keep = cnt < room ? cnt : room;  // first `room` in traversal order
+    if (cnt <= kPairStage) {
+      // canonical order: ascending partner row
+      uint32_t k8[8];
+      static_for<8>([&](auto jc) {
+        constexpr int j = decltype(jc)::value;
+        k8[j] = (uint32_t)j < keep ? stage[i * kPairStage + j] : 0xFFFFFFFFu;
+      });
+      sort8(k8);
+      static_for<8>([&](auto jc) {
+        constexpr int j = decltype(jc)::value;
+        if ((uint32_t)j < keep) pairs[o + j] = make_uint2(i, k8[j]);
+      });
+      continue;
+    }
+    // more partners than the staging area holds: traverse again, writing the
+    // first `room` partners straight into the pair list, then sort them
     float4 mn = d.nmin[nb + N - 1 + i], mx = d.nmax[nb + N - 1 + i];
-    uint32_t cnt = traverse<true>(d, nb, N, i, ld3(mn), ld3(mx), pairs + o, room);
-    if (cnt > room) cnt = room;
-    // canonical order: ascending partner row (insertion sort; lists are short)
-    for (uint32_t a = 1; a < cnt; ++a) {
+    traverse(d, nb, N, i, ld3(mn), ld3(mx), [&](uint32_t k, uint32_t j) {
+      if (k < keep) pairs[o + k] = make_uint2(i, j);
+    });
+    for (uint32_t a = 1; a < keep; ++a) {
       uint2 x = pairs[o + a];
-      uint32_t b = a;
-      while (b > 0 && pairs[o + b - 1].y > x.y) {
-        pairs[o + b] = pairs[o + b - 1];
-        --b;
+      uint32_t b2 = a;
+      while (b2 > 0 && pairs[o + b2 - 1].y > x.y) {
+        pairs[o + b2] = pairs[o + b2 - 1];
+        --b2;
       }
-      pairs[o + b] = x;
+      pairs[o + b2] = x;
     }
   }
 }
@@ -1220,7 +1260,7 @@ BB_DI uint32_t gc_priority(uint32_t pid, uint32_t round) {  // coloring.slang:13
 template <int BLOCK>
 BB_DI void iw_refresh(const Dev& d, const WorldInfo& W);
 BB_DI bool pgs_solvable(const Dev& d, uint32_t off, int4 h);
-BB_DI void pgs_compact(const Dev& d, const WorldInfo& W, uint32_t k, uint32_t slot, float hertz);
+BB_DI void pgs_compact(const Dev& d, const WorldInfo& W, uint32_t k, uint32_t slot);
 
 template <int BLOCK>
 struct PostLds {
@@ -1521,9 +1561,7 @@ __global__ __launch_bounds__(BLOCK) void k_post(Dev d) {
   iw_refresh<BLOCK>(d, W);
   __syncthreads();
   const uint32_t total = d.bucket_off[w * (kBuckets + 1) + kBuckets];
-  const float inv_dt = W.dt > 0.0f ? 1.0f / W.dt : 0.0f;
-  const float hertz = fmn(30.0f, inv_dt * 0.3333333333333333f);
-  for (uint32_t k = tid; k < total; k += BLOCK) pgs_compact(d, W, W.pair_off + k, bucket[k], hertz);
+  for (uint32_t k = tid; k < total; k += BLOCK) pgs_compact(d, W, W.pair_off + k, bucket[k]);
 }
 
 // ================================================================== k_pgs
@@ -1535,10 +1573,35 @@ __global__ __launch_bounds__(BLOCK) void k_post(Dev d) {
 // quantities (arms, effective masses, soft coefficients) are computed once
 // at compaction; the warm-start impulses are then applied colour by colour.
 // Statics are never written; rows of one colour share no dynamic body.
+// Soft contact coefficients of PGS_SOFT::pre_step (solvers.slang:142-200)
+// for contact hertz ch: zeta = 10, omega = 2*pi*ch.
+struct Soft {
+  float bias_factor, impulse_coef, mass_coef;
+};
+BB_DI Soft soft_coefs(float dt, float ch) {
+  const float zeta = 10.0f;
+  const float omega = 2.0f * 3.14159265359f * ch;
+  const float cc = dt * omega * (2.0f * zeta + dt * omega);
+  Soft s;
+  s.bias_factor = omega / (2.0f * zeta + dt * omega);
+  s.impulse_coef = 1.0f / (1.0f + cc);
+  s.mass_coef = cc * s.impulse_coef;
+  return s;
+}
+
 struct Pgs {
   bool accum, friction;
   float inv_dt;
+  Soft soft[2];  // [0]: both bodies movable, [1]: one immovable (2x hertz)
 };
+
+// Manifold::compute_basis (src/shaders/extensions.slang:262-272)
+BB_DI void contact_basis(v3 n, v3& t0, v3& t1) {
+  if (fab(n.x) > fab(n.y)) t0 = V(n.z, 0.0f, -n.x);
+  else t0 = V(0.0f, -n.z, n.y);
+  t0 = vnorm(t0);
+  t1 = cross(t0, n);
+}
 
 BB_DI mat3 iw_load(const Dev& d, uint32_t g) {
   return mrows(ld3(d.iw[g * 3 + 0]), ld3(d.iw[g * 3 + 1]), ld3(d.iw[g * 3 + 2]));
@@ -1565,8 +1628,12 @@ BB_DI bool pgs_solvable(const Dev& d, uint32_t off, int4 h) {
   return awa || awb;
 }
 
-// Solver row k from manifold slot: the constants of PGS_SOFT::pre_step.
-BB_DI void pgs_compact(const Dev& d, const WorldInfo& W, uint32_t k, uint32_t slot, float hertz) {
+// Solver row k from manifold slot: the velocity-independent quantities of
+// PGS_SOFT::pre_step (arms, effective masses).  The row keeps the normal
+// only: the tangents are recomputed from it exactly as Manifold::compute_basis
+// made them, and the soft coefficients depend only on whether either body
+// is immovable (contact hertz doubled, bit 1024 of the count word).
+BB_DI void pgs_compact(const Dev& d, const WorldInfo& W, uint32_t k, uint32_t slot) {
   const uint32_t off = W.body_off, NP = d.np;
   const int4 h = d.man_cur.hdr[slot];
   const uint32_t ga = off + h.x, gb = off + h.y;
@@ -1578,13 +1645,7 @@ BB_DI void pgs_compact(const Dev& d, const WorldInfo& W, uint32_t k, uint32_t sl
   const v3 n = ld3(d.man_cur.nrm[slot]);
   const v3 t0 = ld3(d.man_cur.t0[slot]), t1 = ld3(d.man_cur.t1[slot]);
   const v3 pa = ld3(d.man_cur.anca[slot]), pb = ld3(d.man_cur.ancb[slot]);
-  const float ch = (ima == 0.0f || imb == 0.0f) ? 2.0f * hertz : hertz;
-  const float zeta = 10.0f;
-  const float omega = 2.0f * 3.14159265359f * ch;
-  const float cc = W.dt * omega * (2.0f * zeta + W.dt * omega);
-  const float bias_factor = omega / (2.0f * zeta + W.dt * omega);
-  const float impulse_coef = 1.0f / (1.0f + cc);
-  const float mass_coef = cc * impulse_coef;
+  const bool soft_static = ima == 0.0f || imb == 0.0f;
   for (int i = 0; i < h.w; ++i) {
     const float4 cp = d.man_cur.cpos[slot * 8 + i];
     const float4 lam = d.man_cur.clam[slot * 8 + i];
@@ -1609,14 +1670,38 @@ BB_DI void pgs_compact(const Dev& d, const WorldInfo& W, uint32_t k, uint32_t sl
     d.sv_l[i * NP + k] = make_float4(lam.x, lam.y, lam.z, tm[0]);
     d.sv_tm1[i * NP + k] = tm[1];
   }
-  d.sv_hdr[k] = make_int4(h.x, h.y, h.w | (da ? 256 : 0) | (db ? 512 : 0), __float_as_int(mass_coef));
-  d.sv_n[k] = st3(n, __builtin_sqrtf(fa * fb));
-  d.sv_t0[k] = st3(t0, bias_factor);
-  d.sv_t1[k] = st3(t1, impulse_coef);
+  d.sv_hdr[k] = make_int4(h.x, h.y, h.w | (da ? 256 : 0) | (db ? 512 : 0) | (soft_static ? 1024 : 0),
+                          __float_as_int(__builtin_sqrtf(fa * fb)));
+  d.sv_n[k] = st3(n, 0.0f);
 }
 
+// Body velocities during the PGS sweeps: rows below `lcap` live in LDS
+// (vel, omega as two float4 per row), the rest in HBM.
+struct VelCache {
+  float4* lds;
+  uint32_t off, lcap;
+  BB_DI void load(const Dev& d, uint32_t i, v3& v, v3& w) const {
+    if (i < lcap) {
+      v = ld3(lds[2 * i]);
+      w = ld3(lds[2 * i + 1]);
+    } else {
+      v = ld3(d.cur.vel[off + i]);
+      w = ld3(d.cur.omg[off + i]);
+    }
+  }
+  BB_DI void store(const Dev& d, uint32_t i, v3 v, v3 w) const {
+    if (i < lcap) {
+      lds[2 * i] = st3(v, 0.0f);
+      lds[2 * i + 1] = st3(w, 0.0f);
+    } else {
+      d.cur.vel[off + i] = st3(v, 0.0f);
+      d.cur.omg[off + i] = st3(w, 0.0f);
+    }
+  }
+};
+
 // kind 0: warm-start impulses of pre_step; 1: solve with bias; 2: relax.
-BB_DI void pgs_row(const Dev& d, uint32_t off, uint32_t k, const Pgs& s, int kind) {
+BB_DI void pgs_row(const Dev& d, const VelCache& vc, uint32_t off, uint32_t k, const Pgs& s, int kind) {
   const uint32_t NP = d.np;
   const int4 h = d.sv_hdr[k];
   const int count = h.z & 255;
@@ -1626,10 +1711,12 @@ BB_DI void pgs_row(const Dev& d, uint32_t off, uint32_t k, const Pgs& s, int kin
   const float ima = ia0.w, imb = ib0.w;
   const mat3 Ia = mrows(ld3(ia0), ld3(d.iw[ga * 3 + 1]), ld3(d.iw[ga * 3 + 2]));
   const mat3 Ib = mrows(ld3(ib0), ld3(d.iw[gb * 3 + 1]), ld3(d.iw[gb * 3 + 2]));
-  v3 va = ld3(d.cur.vel[ga]), wa = ld3(d.cur.omg[ga]);
-  v3 vb = ld3(d.cur.vel[gb]), wb = ld3(d.cur.omg[gb]);
-  const float4 nn = d.sv_n[k], tt0 = d.sv_t0[k], tt1 = d.sv_t1[k];
-  const v3 n = ld3(nn), t0 = ld3(tt0), t1 = ld3(tt1);
+  v3 va, wa, vb, wb;
+  vc.load(d, (uint32_t)h.x, va, wa);
+  vc.load(d, (uint32_t)h.y, vb, wb);
+  const v3 n = ld3(d.sv_n[k]);
+  v3 t0, t1;
+  contact_basis(n, t0, t1);
   if (kind == 0) {
     if (!s.accum) return;
     for (int i = 0; i < count; ++i) {
@@ -1651,8 +1738,9 @@ BB_DI void pgs_row(const Dev& d, uint32_t off, uint32_t k, const Pgs& s, int kin
     }
   } else {
     const bool use_bias = kind == 1;
-    const float fr = nn.w, bias_factor = tt0.w, impulse_coef = tt1.w;
-    const float mass_coef = __int_as_float(h.w);
+    const float fr = __int_as_float(h.w);
+    const Soft sc = (h.z & 1024) ? s.soft[1] : s.soft[0];
+    const float bias_factor = sc.bias_factor, impulse_coef = sc.impulse_coef, mass_coef = sc.mass_coef;
     for (int c = 0; c < count; ++c) {
       const float4 c1 = d.sv_r1[c * NP + k], c2 = d.sv_r2[c * NP + k];
       float4 l = d.sv_l[c * NP + k];
@@ -1717,14 +1805,8 @@ BB_DI void pgs_row(const Dev& d, uint32_t off, uint32_t k, const Pgs& s, int kin
       d.sv_l[c * NP + k] = l;
     }
   }
-  if (da) {
-    d.cur.vel[ga] = st3(va, 0.0f);
-    d.cur.omg[ga] = st3(wa, 0.0f);
-  }
-  if (db) {
-    d.cur.vel[gb] = st3(vb, 0.0f);
-    d.cur.omg[gb] = st3(wb, 0.0f);
-  }
+  if (da) vc.store(d, (uint32_t)h.x, va, wa);
+  if (db) vc.store(d, (uint32_t)h.y, vb, wb);
 }
 
 // RigidBody::apply_angular_velocity (src/shaders/extensions.slang:148-166)
@@ -1743,6 +1825,7 @@ BB_DI qt apply_angular_velocity(qt q, v3 om, float dt) {
 
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_pgs(Dev d) {
+  extern __shared__ float4 pgs_lds[];
   const uint32_t w = blockIdx.x, tid = threadIdx.x;
   const WorldInfo W = d.worlds[w];
   if (W.solver == BB_SOLVER_AVBD || W.n == 0) return;
@@ -1751,18 +1834,29 @@ __global__ __launch_bounds__(BLOCK) void k_pgs(Dev d) {
   s.accum = (W.flags & BB_SIM_ACCUM_IMPULSE) != 0;
   s.friction = (W.flags & BB_SIM_FRICTION) != 0;
   s.inv_dt = W.dt > 0.0f ? 1.0f / W.dt : 0.0f;
+  {
+    const float hertz = fmn(30.0f, s.inv_dt * 0.3333333333333333f);
+    s.soft[0] = soft_coefs(W.dt, hertz);
+    s.soft[1] = soft_coefs(W.dt, 2.0f * hertz);
+  }
   const uint32_t* bo = d.bucket_off + w * (kBuckets + 1);
   const uint32_t used = d.stats[w * ST_COUNT + ST_GC_COUNT];
   const uint32_t base = W.pair_off;
   const uint32_t ovf_lo = bo[kColors], ovf_hi = bo[kBuckets];
+  const VelCache vc{pgs_lds, off, N < d.pgs_lcap ? N : d.pgs_lcap};
+  for (uint32_t i = tid; i < vc.lcap; i += BLOCK) {
+    pgs_lds[2 * i] = d.cur.vel[off + i];
+    pgs_lds[2 * i + 1] = d.cur.omg[off + i];
+  }
+  __syncthreads();
   auto sweep = [&](int kind) {
     for (uint32_t c = 0; c < used; ++c) {
-      for (uint32_t k = bo[c] + tid; k < bo[c + 1]; k += BLOCK) pgs_row(d, off, base + k, s, kind);
+      for (uint32_t k = bo[c] + tid; k < bo[c + 1]; k += BLOCK) pgs_row(d, vc, off, base + k, s, kind);
       __syncthreads();
     }
     if (ovf_hi > ovf_lo) {
       if (tid == 0)
-        for (uint32_t k = ovf_lo; k < ovf_hi; ++k) pgs_row(d, off, base + k, s, kind);
+        for (uint32_t k = ovf_lo; k < ovf_hi; ++k) pgs_row(d, vc, off, base + k, s, kind);
       __syncthreads();
     }
   };
@@ -1779,23 +1873,29 @@ __global__ __launch_bounds__(BLOCK) void k_pgs(Dev d) {
       const uint32_t g = off + i;
       uint32_t fl = d.cur.flags[g];
       if (!((fl & BB_BODY_DYNAMIC) && !(fl & BB_BODY_SLEEPING))) continue;
-      v3 p = ld3(d.cur.pos[g]), v = ld3(d.cur.vel[g]);
+      v3 v, om;
+      vc.load(d, i, v, om);
+      const v3 p = ld3(d.cur.pos[g]);
       d.cur.pos[g] = st3(p + v * W.dt, 0.0f);
-      d.cur.rot[g] = stq(apply_angular_velocity(ldq(d.cur.rot[g]), ld3(d.cur.omg[g]), W.dt));
+      d.cur.rot[g] = stq(apply_angular_velocity(ldq(d.cur.rot[g]), om, W.dt));
     }
   }
   __syncthreads();
   iw_refresh<BLOCK>(d, W);
   __syncthreads();
   for (uint32_t it = 0; it < W.iterations; ++it) sweep(2);
+  for (uint32_t i = tid; i < vc.lcap; i += BLOCK) {
+    d.cur.vel[off + i] = pgs_lds[2 * i];
+    d.cur.omg[off + i] = pgs_lds[2 * i + 1];
+  }
   // impulses and pre-step quantities back into the manifold slots (warm start
   // of the next step, taps)
   const uint32_t* bucket = d.bucket + base;
   for (uint32_t k = tid; k < ovf_hi; k += BLOCK) {
     const uint32_t row = base + k, slot = bucket[k];
     const int4 h = d.sv_hdr[row];
-    const float mass_coef = __int_as_float(h.w);
-    const float bias_factor = d.sv_t0[row].w, impulse_coef = d.sv_t1[row].w;
+    const Soft sc = (h.z & 1024) ? s.soft[1] : s.soft[0];
+    const float mass_coef = sc.mass_coef, bias_factor = sc.bias_factor, impulse_coef = sc.impulse_coef;
     for (int c = 0; c < (h.z & 255); ++c) {
       const float4 l = d.sv_l[c * NP + row];
       d.man_cur.clam[slot * 8 + c] = make_float4(l.x, l.y, l.z, mass_coef);
@@ -2516,7 +2616,7 @@ static void launch_all(const Dev& d, hipStream_t st) {
   hipLaunchKernelGGL(k_broad<B>, g, b, 0, st, d);
   hipLaunchKernelGGL(k_narrow<B>, g, b, 0, st, d);
   hipLaunchKernelGGL(k_post<B>, g, b, 0, st, d);
-  hipLaunchKernelGGL(k_pgs<B>, g, b, 0, st, d);
+  hipLaunchKernelGGL(k_pgs<B>, g, b, (size_t)d.pgs_lcap * 32, st, d);
   hipLaunchKernelGGL(k_avbd<B>, g, b, 0, st, d);
   hipLaunchKernelGGL(k_finish<B>, g, b, 0, st, d);
 }
@@ -2533,7 +2633,7 @@ static void launch_all_timed(const Dev& d, hipStream_t st, hipEvent_t* ev) {
   (void)hipEventRecord(ev[3], st);
   hipLaunchKernelGGL(k_post<B>, g, b, 0, st, d);
   (void)hipEventRecord(ev[4], st);
-  hipLaunchKernelGGL(k_pgs<B>, g, b, 0, st, d);
+  hipLaunchKernelGGL(k_pgs<B>, g, b, (size_t)d.pgs_lcap * 32, st, d);
   (void)hipEventRecord(ev[5], st);
   hipLaunchKernelGGL(k_avbd<B>, g, b, 0, st, d);
   (void)hipEventRecord(ev[6], st);

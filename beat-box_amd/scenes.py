@@ -209,3 +209,95 @@ def batch_worlds(seed: int = 0xBB04, worlds: int = 4096, lo: int = 16, hi: int =
     """Batched worlds: body counts log-uniform in [lo, hi]."""
     sizes = batch_world_sizes(seed, worlds, lo, hi)
     return [batch_world(seed, w, int(n)) for w, n in enumerate(sizes)]
+
+
+def _mixed_pile(seed: int, n: int, ext_lo: float, ext_hi: float, x_half: float, y_lo: float, y_hi: float,
+                with_ground: bool = True):
+    """n bodies (even ids boxes with per-axis half-extents U(ext_lo, ext_hi),
+    odd ids spheres with radius U(ext_lo, ext_hi)), density 1, uniform random
+    orientation (Shoemake), centres x, z U(-x_half, x_half), y U(y_lo, y_hi),
+    zero velocity; body 0 = the static ground when with_ground."""
+    off = 1 if with_ground else 0
+    bodies = np.zeros(n + off, BODY_DESC)
+    if with_ground:
+        _ground(bodies[0])
+    F = np.uint64(12)
+    c = np.arange(n, dtype=np.uint64) * F
+    b = bodies[off:]
+    sphere = (np.arange(n) % 2) == 1
+    hx = uniform(seed, c + np.uint64(0), ext_lo, ext_hi)
+    hy = uniform(seed, c + np.uint64(1), ext_lo, ext_hi)
+    hz = uniform(seed, c + np.uint64(2), ext_lo, ext_hi)
+    b["shape"] = np.where(sphere, SHAPE_SPHERE, SHAPE_BOX)
+    b["flags"] = BODY_DYNAMIC | BODY_GRAVITY
+    b["position"][:, 0] = uniform(seed, c + np.uint64(3), -x_half, x_half)
+    b["position"][:, 1] = uniform(seed, c + np.uint64(4), y_lo, y_hi)
+    b["position"][:, 2] = uniform(seed, c + np.uint64(5), -x_half, x_half)
+    b["rotation"] = shoemake(seed, c + np.uint64(6))
+    r = hx
+    b["radius"] = np.where(sphere, r, 0.0).astype(np.float32)
+    b["minimum"] = np.where(sphere[:, None], -r[:, None], -np.stack([hx, hy, hz], 1)).astype(np.float32)
+    b["maximum"] = -b["minimum"]
+    vol = np.where(sphere, 4.0 / 3.0 * math.pi * r ** 3, 8.0 * hx * hy * hz)
+    b["mass"] = vol.astype(np.float32)
+    b["friction"] = 0.5
+    return bodies
+
+
+def large_pile(seed: int = 0xBB02, n: int = 100_000):
+    """Single large pile: 50/50 boxes and spheres, half-extents/radii
+    U(0.25, 1.0) m, centres uniform in a 60 x 200 x 60 m column above the
+    ground (y in [1, 201]), zero velocity.  Run with 63-bit keys and AVBD,
+    10 iterations."""
+    return _mixed_pile(seed, n, 0.25, 1.0, 30.0, 1.0, 201.0)
+
+
+def sphere_gas(seed: int = 0xBB03, n: int = 1_000_000, half: float = 500.0):
+    """Broad-phase stress: n spheres, radius U(0.1, 0.5) m, centres uniform
+    in a (2*half)^3 cube, velocity N(0, 2 m/s) per axis (Box-Muller), no
+    ground; run with gravity 0."""
+    bodies = np.zeros(n, BODY_DESC)
+    c = np.arange(n, dtype=np.uint64) * np.uint64(10)
+    r = uniform(seed, c, 0.1, 0.5)
+    bodies["shape"] = SHAPE_SPHERE
+    bodies["flags"] = BODY_DYNAMIC
+    for k in range(3):
+        bodies["position"][:, k] = uniform(seed, c + np.uint64(1 + k), -half, half)
+    bodies["rotation"] = (0.0, 0.0, 0.0, 1.0)
+    bodies["radius"] = r
+    bodies["minimum"] = -r[:, None]
+    bodies["maximum"] = r[:, None]
+    bodies["mass"] = (4.0 / 3.0 * math.pi * r ** 3).astype(np.float32)
+    for k in range(3):
+        u1 = uniform(seed, c + np.uint64(4 + 2 * k), 2.0 ** -24, 1.0)
+        u2 = uniform(seed, c + np.uint64(5 + 2 * k), 0.0, 2.0 * math.pi)
+        bodies["velocity"][:, k] = (2.0 * np.sqrt(-2.0 * np.log(u1)) * np.cos(u2)).astype(np.float32)
+    bodies["friction"] = 0.5
+    return bodies
+
+
+def serving_pile(seed: int, n: int):
+    """One serving request: a random pile of n bodies, 50/50 boxes and
+    spheres with sizes (edge / diameter) U(0.2, 1.0) m, centres in a column
+    whose footprint grows with n, on the ground."""
+    x_half = max(2.0, 0.45 * n ** (1.0 / 3.0))
+    height = max(4.0, n * 0.5 / (4.0 * x_half * x_half))
+    return _mixed_pile(seed, n, 0.1, 0.5, x_half, 0.6, 0.6 + height)
+
+
+def serving_requests(seed: int = 0xBB05, rate: float = 500.0, duration: float = 600.0,
+                     lo: int = 64, hi: int = 20_000, h_lo: int = 60, h_hi: int = 600):
+    """Serving stream: Poisson arrivals (exponential gaps at `rate` per s)
+    within `duration` seconds; each request has a body count log-uniform in
+    [lo, hi] and a horizon uniform in {h_lo..h_hi} steps.  Returns arrays
+    (arrival_s, bodies, horizon, scene_seed)."""
+    m = int(rate * duration * 1.2 + 64)
+    c = np.arange(m, dtype=np.uint64) * np.uint64(4)
+    gaps = -np.log(uniform(seed, c, 2.0 ** -24, 1.0)) / rate
+    t = np.cumsum(gaps)
+    keep = t < duration
+    nb = np.clip(np.rint(np.exp(uniform(seed, c + np.uint64(1), math.log(lo), math.log(hi)))), lo, hi)
+    hz = np.floor(uniform(seed, c + np.uint64(2), h_lo, h_hi + 1)).astype(np.int64)
+    hz = np.clip(hz, h_lo, h_hi)
+    sd = splitmix64(seed, c + np.uint64(3))
+    return t[keep], nb[keep].astype(np.int64), hz[keep], sd[keep]

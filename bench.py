@@ -74,17 +74,22 @@ def make_batch(worlds: int, seed: int, lo: int, hi: int):
     return b, total
 
 
-def time_batch(b, steps: int, warmup: int):
-    """Graph replay of `steps` steps after `warmup` untimed ones; returns the
-    wall time and the batch energy before and after the timed steps."""
+def time_batch(b, steps: int, warmup: int, tune: bool):
+    """Graph replay of `steps` steps after `warmup` untimed ones (and the
+    autotuner, which replays the next step per candidate and leaves the state
+    unchanged); returns the wall time, the batch energy before and after the
+    timed steps and the chosen launch parameters."""
     b.step(warmup, use_graph=True)
     b.sync()
+    if tune:
+        b.autotune(2)
+    params = (b.get_param(beatbox.PARAM_BLOCK), b.get_param(beatbox.PARAM_PGS_LDS_ROWS))
     e0 = float(b.energy().sum())
     t0 = time.perf_counter()
     b.step(steps, use_graph=True)
     b.sync()
     dt = time.perf_counter() - t0
-    return dt, e0, float(b.energy().sum())
+    return dt, e0, float(b.energy().sum()), params
 
 
 def plain_world(steps: int, warmup: int):
@@ -137,6 +142,7 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=0xBB04)
     ap.add_argument("--cpu-budget", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="default launch parameters, no autotuner")
     ap.add_argument("--profile-steps", type=int, default=0, help="0: the timed steps")
     ap.add_argument("--plain-steps", type=int, default=200)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -159,7 +165,8 @@ def main() -> int:
 
     # 1) throughput: hipGraph replay of the timed steps
     batch, total = make_batch(args.worlds, args.seed, 16, 2048)
-    t, e0, e1 = time_batch(batch, args.steps, args.warmup)
+    t, e0, e1, params = time_batch(batch, args.steps, args.warmup, not args.no_tune)
+    log(f"launch parameters: block {params[0]}, PGS LDS rows {params[1]}")
     dev_gib = batch.device_bytes / 2**30
     batch.destroy()
     value = total * args.steps / t
@@ -172,6 +179,8 @@ def main() -> int:
     # same seed reaches the same states (the step is deterministic), then
     # eager steps are timed with hipEvents on the library stream
     batch, _ = make_batch(args.worlds, args.seed, 16, 2048)
+    batch.set_param(beatbox.PARAM_BLOCK, params[0])
+    batch.set_param(beatbox.PARAM_PGS_LDS_ROWS, params[1])
     batch.step(args.warmup, use_graph=True)
     batch.sync()
     psteps = args.profile_steps or args.steps
@@ -227,6 +236,8 @@ def main() -> int:
             "eager_step_ms_p50": p50,
             "eager_step_ms_p99": p99,
             "copy_kernel_gbs": copy_gbs,
+            "block": params[0],
+            "pgs_lds_rows": params[1],
             "stages": stages,
         },
         "roofline": {

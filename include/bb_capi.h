@@ -345,7 +345,85 @@ uint64_t bb_batch_device_bytes(const bb_batch* b);
 
 /* Block size used by the per-world kernels (autotuner hook); 0 = automatic. */
 bb_status bb_batch_set_block_size(bb_batch* b, uint32_t threads);
+
+/* Launch parameters (no reference counterpart).  None of them changes any
+ * result: every world's trajectory is bit-identical for every setting. */
+typedef enum bb_param {
+  BB_PARAM_BLOCK = 0,        /* threads per world workgroup: 0 = automatic,
+                                256, 512 or 1024                            */
+  BB_PARAM_PGS_LDS_ROWS = 1  /* rows per world whose velocities the PGS
+                                sweeps keep in LDS (32 B each): 0..2048     */
+} bb_param;
+bb_status bb_batch_set_param(bb_batch* b, bb_param p, uint32_t value);
+/* Effective value (BB_PARAM_BLOCK resolves 0 to the automatic choice). */
+uint32_t bb_batch_get_param(const bb_batch* b, bb_param p);
+/* Autotuner: times `reps` replays of the next step for every candidate
+ * (block size x LDS rows), each replay restarting from the same state, and
+ * keeps the fastest.  The simulation state afterwards is exactly the state
+ * before the call.  *best_ms (may be NULL) receives the chosen step time. */
+bb_status bb_batch_autotune(bb_batch* b, uint32_t reps, float* best_ms);
 void bb_batch_destroy(bb_batch* b);
+
+/* ----------------------------------------------------------- serving loop */
+/* Continuous batching of independent worlds (no reference counterpart; the
+ * reference steps one scene).  Requests are queued on submit and admitted at
+ * step boundaries, first come first served, while the resident worlds'
+ * device state fits the HBM budget (and, optionally, a resident-body cap that
+ * bounds the step latency).  Every resident world advances one step per
+ * server step in one launch sequence; a world that has run its requested
+ * number of steps is retired: its final state is copied back and its device
+ * memory and slot are reused.  Each world's trajectory is bit-identical to
+ * stepping it alone (bb_world_*), wherever it sits in the device arrays and
+ * whatever else is resident. */
+typedef struct bb_server bb_server;
+
+typedef struct bb_server_desc {
+  uint64_t hbm_budget_bytes;    /* device bytes for resident state (admission) */
+  uint32_t max_worlds;          /* resident world slots                       */
+  uint32_t max_bodies;          /* body rows; 0 = as many as the budget holds */
+  uint32_t max_resident_bodies; /* 0 = no cap; else admit while the resident
+                                   bodies stay within it (latency bound)      */
+  uint32_t block;               /* threads per world workgroup, 0 = 256       */
+} bb_server_desc;
+
+typedef enum bb_request_state {
+  BB_REQ_QUEUED = 0,
+  BB_REQ_RUNNING = 1,
+  BB_REQ_DONE = 2,
+  BB_REQ_UNKNOWN = 3
+} bb_request_state;
+
+typedef struct bb_server_report {
+  uint64_t submitted, admitted, retired, queued;
+  uint64_t resident_worlds, resident_bodies;
+  uint64_t bytes_in_use, bytes_budget, bytes_allocated;
+  uint64_t server_steps, world_steps, body_steps;
+  double device_ms;                  /* first step start .. last completed step */
+  double step_ms_p50, step_ms_p99;   /* per world-step latency: completion of
+                                        step s - completion of step s-1 (or the
+                                        world's admission for s = 1)          */
+  double queue_ms_p50, queue_ms_p99; /* submit -> admission, host clock       */
+} bb_server_report;
+
+bb_status bb_server_create(const bb_server_desc* desc, bb_server** out);
+/* Queues a world (bodies follow the load_scene rules, as bb_world_add_bodies)
+ * that runs `steps` steps.  BB_ERR_BUDGET when the world alone exceeds the
+ * budget or the capacities; *ticket identifies the request. */
+bb_status bb_server_submit(bb_server* s, const bb_world_desc* desc, const bb_body_desc* bodies, uint32_t n,
+                           uint32_t steps, uint64_t* ticket);
+/* Runs nsteps server steps (admission, one step of every resident world,
+ * retirement).  Launches are asynchronous; at most two steps run ahead of
+ * the host. */
+bb_status bb_server_step(bb_server* s, uint32_t nsteps);
+/* Steps until every submitted request is done. */
+bb_status bb_server_drain(bb_server* s);
+/* State of a request; when done, its final bodies (by id, n >= body count)
+ * and last-step counters are copied to out / stats (either may be NULL) and
+ * the result is released.  *state receives a bb_request_state. */
+bb_status bb_server_poll(bb_server* s, uint64_t ticket, uint32_t* state, bb_body_state* out, uint32_t n,
+                         bb_step_stats* stats);
+bb_status bb_server_report_get(bb_server* s, bb_server_report* out);
+void bb_server_destroy(bb_server* s);
 
 /* Measurement helper (no reference counterpart): copies `bytes` between two
  * device buffers `reps` times with a grid-stride float4 copy kernel

@@ -44,6 +44,8 @@ int main(void) {
          sizeof(bb_manifold_tap), sizeof(bb_contact_tap));
   printf("%zu %zu %zu\\n", offsetof(bb_body_desc, friction), offsetof(bb_manifold_tap, contacts),
          offsetof(bb_step_stats, step_index));
+  printf("%zu %zu %zu\\n", sizeof(bb_server_desc), sizeof(bb_server_report),
+         offsetof(bb_server_report, device_ms));
   return 0;
 }
 ''')
@@ -62,6 +64,9 @@ int main(void) {
     assert sizes[7] == scenes.BODY_DESC.fields["friction"][1]
     assert sizes[8] == beatbox.MANIFOLD_TAP.fields["contacts"][1]
     assert sizes[9] == beatbox.StepStats.step_index.offset
+    assert sizes[10] == C.sizeof(beatbox.ServerDesc)
+    assert sizes[11] == C.sizeof(beatbox.ServerReport)
+    assert sizes[12] == beatbox.ServerReport.device_ms.offset
 
 
 def test_defaults_follow_reference():
@@ -124,3 +129,12 @@ def test_step_without_gpu_fails_loudly():
     with pytest.raises(beatbox.BBError) as e:
         w.simulate(1)
     assert e.value.code == 4  # BB_ERR_NO_DEVICE
+
+
+def test_server_without_gpu_fails_loudly():
+    with pytest.raises(beatbox.BBError) as e:
+        beatbox.Server(budget_bytes=1 << 30, max_worlds=8)
+    assert e.value.code == 4  # BB_ERR_NO_DEVICE
+    with pytest.raises(beatbox.BBError) as e:
+        beatbox.Server(budget_bytes=0, max_worlds=8)
+    assert e.value.code == 1  # BB_ERR_INVALID

@@ -142,3 +142,64 @@ def test_profile_counts_equal_world_stats():
     assert last[5] == sum(s["graph_color_count"] * s["collision_count"] for s in st)
     assert last[6] == max(s["dbg_pen"] for s in st)
     assert last[7] == sum(s["sleeping_count"] for s in st)
+
+
+def _bodies_equal(a, b, n_worlds, sizes):
+    for w in range(n_worlds):
+        x = a.read_world(w, sizes[w])
+        y = b.read_world(w, sizes[w])
+        for f in ("position", "rotation", "velocity", "omega", "flags"):
+            assert x[f].tobytes() == y[f].tobytes(), (w, f)
+
+
+def test_launch_params_do_not_change_results():
+    """Block size and the LDS velocity cache are pure launch choices: every
+    setting gives bit-identical trajectories (worlds larger and smaller than
+    the LDS row count, PGS and AVBD)."""
+    sizes = [700, 40, 300, 1200]
+    worlds = _worlds(0xBB04, sizes)
+    descs = [beatbox.WorldDesc.make(**scenes.world_desc(
+        solver=scenes.SOLVER_AVBD if w == 1 else scenes.SOLVER_PGS_SOFT,
+        iterations=8, morton_bits=63, quirks=0)) for w in range(len(sizes))]
+    runs = []
+    for blk, lds in ((256, 0), (512, 256), (1024, 2048), (256, 1024)):
+        b = beatbox.Batch()
+        for d, bodies in zip(descs, worlds):
+            b.add_world(d, bodies)
+        b.set_param(beatbox.PARAM_BLOCK, blk)
+        b.set_param(beatbox.PARAM_PGS_LDS_ROWS, lds)
+        b.step(15, use_graph=True)
+        b.sync()
+        assert b.get_param(beatbox.PARAM_BLOCK) == blk
+        assert b.get_param(beatbox.PARAM_PGS_LDS_ROWS) == min(lds, max(sizes) + 1)
+        runs.append(b)
+    for r in runs[1:]:
+        _bodies_equal(runs[0], r, len(sizes), [s + 1 for s in sizes])
+
+
+def test_autotune_leaves_state_unchanged():
+    """The autotuner replays the next step from the same state; afterwards
+    the batch continues exactly as an untuned batch, and the counters of the
+    last real step are restored."""
+    sizes = [400, 64, 900, 150]
+    worlds = _worlds(0xBB04, sizes)
+    desc = beatbox.WorldDesc.make(**scenes.world_desc(iterations=8, morton_bits=63, quirks=0))
+    tuned, plain = beatbox.Batch(), beatbox.Batch()
+    for bodies in worlds:
+        tuned.add_world(desc, bodies)
+        plain.add_world(desc, bodies)
+    tuned.step(6, use_graph=True)
+    plain.step(6, use_graph=True)
+    before = [tuned.stats(w) for w in range(len(sizes))]
+    ms = tuned.autotune(2)
+    assert ms > 0.0
+    assert [tuned.stats(w) for w in range(len(sizes))] == before
+    tuned.step(7, use_graph=True)
+    plain.step(7, use_graph=True)
+    tuned.sync()
+    plain.sync()
+    _bodies_equal(tuned, plain, len(sizes), [s + 1 for s in sizes])
+    for w in range(len(sizes)):
+        a, b = tuned.stats(w), plain.stats(w)
+        a.pop("step_index"), b.pop("step_index")
+        assert a == b

@@ -1,0 +1,100 @@
+"""Serving loop: continuous batching through the C-ABI.  Each world's final
+state equals the same world stepped alone (bit-exact: the step is a pure
+function of the world's own state, wherever it sits in the device arrays and
+whatever else is resident); admission respects the HBM budget and the
+resident-body cap; counters and latency percentiles are reported."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import beatbox
+import scenes
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("position", "rotation", "velocity", "omega", "flags")
+
+
+def _alone(desc, bodies, steps):
+    w = beatbox.RigidBodyManager(desc)
+    w.add_bodies(bodies)
+    w.simulate(steps)
+    out = w.read_bodies()
+    st = w.read_back_sim_config()
+    w.destroy()
+    return out, st
+
+
+def _requests():
+    pgs = beatbox.WorldDesc.make(**scenes.world_desc(iterations=8, morton_bits=63, quirks=0))
+    avbd = beatbox.WorldDesc.make(**scenes.world_desc(solver=scenes.SOLVER_AVBD, iterations=10,
+                                                      morton_bits=63, quirks=0))
+    parity = beatbox.WorldDesc.make(**scenes.world_desc(iterations=8))
+    reqs = [
+        (pgs, scenes.batch_world(0xBB04, 3, 120), 9),
+        (avbd, scenes.serving_pile(11, 150), 7),
+        (parity, scenes.random_boxes(0xBB11, 64, extent=3.0, y0=0.55, y1=5.0), 12),
+        (pgs, scenes.batch_world(0xBB04, 8, 300), 1),
+        (avbd, scenes.serving_pile(12, 90), 15),
+        (pgs, scenes.serving_pile(13, 200), 5),
+    ]
+    return reqs
+
+
+def test_server_results_equal_alone():
+    reqs = _requests()
+    srv = beatbox.Server(budget_bytes=1 << 30, max_worlds=4)
+    tickets = []
+    # staggered arrivals: two now, the rest while others are running
+    for d, bodies, steps in reqs[:2]:
+        tickets.append(srv.submit(d, bodies, steps))
+    srv.step(3)
+    for d, bodies, steps in reqs[2:]:
+        tickets.append(srv.submit(d, bodies, steps))
+        srv.step(2)
+    srv.drain()
+    rep = srv.report()
+    assert rep["submitted"] == rep["admitted"] == rep["retired"] == len(reqs)
+    assert rep["queued"] == 0 and rep["resident_worlds"] == 0 and rep["bytes_in_use"] == 0
+    assert rep["world_steps"] == sum(r[2] for r in reqs)
+    assert rep["body_steps"] == sum(len(r[1]) * r[2] for r in reqs)
+    assert rep["step_ms_p50"] > 0 and rep["step_ms_p99"] >= rep["step_ms_p50"]
+    for t, (d, bodies, steps) in zip(tickets, reqs):
+        state, got, stats = srv.poll(t, len(bodies))
+        assert state == beatbox.REQ_DONE, t
+        ref, ref_stats = _alone(d, bodies, steps)
+        for f in FIELDS:
+            assert got[f].tobytes() == ref[f].tobytes(), (t, f)
+        assert stats["collision_count"] == ref_stats["collision_count"]
+        assert stats["contact_count"] == ref_stats["contact_count"]
+        # released after the first poll
+        assert srv.poll(t, len(bodies))[0] == beatbox.REQ_UNKNOWN
+    srv.destroy()
+
+
+def test_server_admission_limits():
+    """A resident-body cap admits one world at a time; a budget too small for
+    a world refuses it at submit; every accepted request completes."""
+    pgs = beatbox.WorldDesc.make(**scenes.world_desc(iterations=8, morton_bits=63, quirks=0))
+    worlds = [scenes.batch_world(0xBB04, w, 100 + 30 * w) for w in range(4)]
+    srv = beatbox.Server(budget_bytes=1 << 30, max_worlds=8, max_resident_bodies=150)
+    ts = [srv.submit(pgs, b, 4) for b in worlds]
+    srv.step(1)
+    rep = srv.report()
+    assert rep["resident_worlds"] == 1 and rep["queued"] == 3
+    assert srv.poll(ts[1], len(worlds[1]))[0] == beatbox.REQ_QUEUED
+    srv.drain()
+    rep = srv.report()
+    assert rep["retired"] == 4 and rep["server_steps"] == 16
+    for t, b in zip(ts, worlds):
+        state, got, _ = srv.poll(t, len(b))
+        assert state == beatbox.REQ_DONE
+        ref, _ = _alone(pgs, b, 4)
+        assert got["position"].tobytes() == ref["position"].tobytes()
+    srv.destroy()
+    tiny = beatbox.Server(budget_bytes=4 << 20, max_worlds=2)
+    with pytest.raises(beatbox.BBError) as e:
+        tiny.submit(pgs, scenes.batch_world(0xBB04, 5, 2000), 3)
+    assert e.value.code == 7  # BB_ERR_BUDGET
+    tiny.destroy()
