@@ -73,7 +73,7 @@ WorldInfo info_of(const HostWorld& hw) {
   wi.dt = hw.desc.dt;
   wi.gravity = hw.desc.gravity;
   wi.damp = std::exp(-hw.desc.dt * 0.1f);
-  wi.drop_prev = 0;
+  wi.mode = 0;
   return wi;
 }
 
@@ -202,8 +202,15 @@ bb_status alloc_arrays(bb_batch* b, Buffers& A, uint32_t W, uint32_t NB, uint32_
   d.bbucket = A.alloc<uint32_t>(NB);
   d.bbucket_off = A.alloc<uint32_t>((size_t)W * (kAvbdColors * kAvbdLayers + 1));
   d.stats = A.alloc<uint32_t>((size_t)W * ST_COUNT);
+  d.teams = A.alloc<TeamTable>(1);
+  d.team_ctl = A.alloc<TeamCtl>(kMaxTeams);
+  d.team_hist = A.alloc<uint32_t>((size_t)kMaxTeamBlocks * 256);
+  d.team_shared = A.alloc<char>((size_t)kMaxTeams * kTeamSharedBytes);
+  d.team_grid = 0;
   if (A.count_only) return BB_OK;
   if (std::find(A.ptrs.begin(), A.ptrs.end(), nullptr) != A.ptrs.end()) return BB_ERR_CAPACITY;
+  HIPCHK(hipMemset(d.teams, 0, sizeof(TeamTable)));
+  HIPCHK(hipMemset(d.team_ctl, 0, sizeof(TeamCtl) * kMaxTeams));
   // tables that must start at zero: colour owner cells (zero between
   // rounds), counters, previous pair counts and pair-row offsets
   HIPCHK(hipMemset(d.owner, 0, (size_t)NB * kColors * 4));
@@ -334,6 +341,51 @@ bb_status reset_world_state(bb_batch* b, uint32_t w, hipStream_t st) {
   return BB_OK;
 }
 
+uint32_t assign_teams(bb_batch* b, uint32_t grid_cap) {
+  TeamTable& T = b->team_table;
+  T = TeamTable{};
+  for (auto& wi : b->infos) wi.mode &= ~kModeTeam;
+  if (b->team_bodies == 0) return 0;
+  grid_cap = std::min(grid_cap, kMaxTeamBlocks);
+  std::vector<uint32_t> cand;
+  for (uint32_t w = 0; w < (uint32_t)b->worlds.size(); ++w)
+    if (b->infos[w].n >= b->team_bodies && b->infos[w].n > 0) cand.push_back(w);
+  // largest worlds first; each team gets at least two workgroups
+  std::sort(cand.begin(), cand.end(), [&](uint32_t x, uint32_t y) {
+    return b->infos[x].n != b->infos[y].n ? b->infos[x].n > b->infos[y].n : x < y;
+  });
+  while (cand.size() > kMaxTeams || 2 * cand.size() > grid_cap) cand.pop_back();
+  if (cand.empty()) return 0;
+  std::sort(cand.begin(), cand.end());
+  uint64_t want_total = 0;
+  std::vector<uint32_t> want(cand.size());
+  for (size_t k = 0; k < cand.size(); ++k) {
+    want[k] = std::max(2u, (b->infos[cand[k]].n + 1023u) / 1024u);
+    want_total += want[k];
+  }
+  uint32_t first = 0;
+  T.count = (uint32_t)cand.size();
+  for (size_t k = 0; k < cand.size(); ++k) {
+    uint32_t nb = want[k];
+    if (want_total > grid_cap) nb = std::max(2u, (uint32_t)((uint64_t)want[k] * grid_cap / want_total));
+    T.first[k] = first;
+    T.world[k] = cand[k];
+    first += nb;
+    b->infos[cand[k]].mode |= kModeTeam;
+  }
+  T.first[cand.size()] = first;
+  return first;
+}
+
+bb_status upload_teams(bb_batch* b) {
+  const uint32_t cap = team_capacity(pick_block(b)) / 2;
+  b->base.team_grid = assign_teams(b, cap);
+  HIPCHK(hipMemcpy(b->base.teams, &b->team_table, sizeof(TeamTable), hipMemcpyHostToDevice));
+  HIPCHK(hipMemset(b->base.team_ctl, 0, sizeof(TeamCtl) * kMaxTeams));
+  HIPCHK(hipMemcpy(b->base.worlds, b->infos.data(), b->infos.size() * sizeof(WorldInfo), hipMemcpyHostToDevice));
+  return BB_OK;
+}
+
 bb_status build(bb_batch* b) {
   if (!b->dirty) return BB_OK;
   if (!has_device()) return BB_ERR_NO_DEVICE;
@@ -374,14 +426,15 @@ bb_status build(bb_batch* b) {
   for (uint32_t w = 0; w < W; ++w) {
     HostWorld& hw = b->worlds[w];
     WorldInfo wi = info_of(hw);
-    wi.drop_prev = 1;  // fresh layout: no previous manifolds
+    wi.mode |= kModeDropPrev;  // fresh layout: no previous manifolds
     b->infos[w] = wi;
     fill_images(hw, hw.body_off, hw.act_off, img);
   }
   st = upload_images(b, img, 0, NB, 0, NA, b->stream);
   if (st != BB_OK) return st;
-  HIPCHK(hipMemcpyAsync(b->base.worlds, b->infos.data(), W * sizeof(WorldInfo), hipMemcpyHostToDevice, b->stream));
   HIPCHK(hipStreamSynchronize(b->stream));
+  st = upload_teams(b);  // also uploads the world table
+  if (st != BB_OK) return st;
   b->latest = 1;  // the first step reads set 1 and writes set 0
   b->base.pgs_lcap = pgs_lds_rows(b);
   b->built = true;
@@ -434,7 +487,7 @@ bool any_one_shot(const bb_batch* b) {
   for (auto& hw : b->worlds)
     if (hw.skip_ws_once || hw.drop_prev) return true;
   for (auto& wi : b->infos)
-    if (wi.drop_prev) return true;
+    if (wi.mode & kModeDropPrev) return true;
   return false;
 }
 
@@ -444,9 +497,9 @@ bb_status upload_infos(bb_batch* b, bool one_shot) {
     HostWorld& hw = b->worlds[w];
     if (one_shot) {
       if (hw.skip_ws_once) v[w].flags &= ~(uint32_t)BB_SIM_WARM_STARTING;
-      if (hw.drop_prev) v[w].drop_prev = 1;
+      if (hw.drop_prev) v[w].mode |= kModeDropPrev;
     } else {
-      v[w].drop_prev = 0;
+      v[w].mode &= ~kModeDropPrev;
     }
   }
   HIPCHK(hipStreamSynchronize(b->stream));
@@ -454,7 +507,7 @@ bb_status upload_infos(bb_batch* b, bool one_shot) {
   if (one_shot) {
     for (auto& hw : b->worlds) hw.skip_ws_once = hw.drop_prev = false;
   } else {
-    for (auto& wi : b->infos) wi.drop_prev = 0;
+    for (auto& wi : b->infos) wi.mode &= ~kModeDropPrev;
   }
   return BB_OK;
 }
@@ -462,7 +515,7 @@ bb_status upload_infos(bb_batch* b, bool one_shot) {
 bb_status refresh_infos(bb_batch* b) {
   for (size_t w = 0; w < b->worlds.size(); ++w) {
     WorldInfo wi = info_of(b->worlds[w]);
-    wi.drop_prev = b->infos[w].drop_prev;
+    wi.mode = b->infos[w].mode;
     b->infos[w] = wi;
   }
   return BB_OK;
@@ -589,14 +642,17 @@ bb_status set_param(bb_batch* b, bb_param p, uint32_t value) {
     if (value > 2048) return BB_ERR_INVALID;
     b->pgs_lds = value;
     b->base.pgs_lcap = pgs_lds_rows(b);
+  } else if (p == BB_PARAM_TEAM_BODIES) {
+    b->team_bodies = value;
   } else {
     return BB_ERR_INVALID;
   }
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
   if (b->graph) {  // kernel arguments and launch shapes are baked into the graph
-    (void)hipStreamSynchronize(b->stream);
     (void)hipGraphExecDestroy(b->graph);
     b->graph = nullptr;
   }
+  if (b->built && (p == BB_PARAM_BLOCK || p == BB_PARAM_TEAM_BODIES)) return upload_teams(b);
   return BB_OK;
 }
 
@@ -639,6 +695,7 @@ bb_status autotune_batch(bb_batch* b, uint32_t reps, float* best_ms) {
       b->block = blk;
       b->pgs_lds = lds;
       b->base.pgs_lcap = pgs_lds_rows(b);
+      if ((rc = upload_teams(b)) != BB_OK) break;
       b->latest = saved;
       if ((rc = launch_one(b)) != BB_OK) break;  // untimed first launch
       (void)hipEventRecord(e0, b->stream);
@@ -674,6 +731,10 @@ bb_status autotune_batch(bb_batch* b, uint32_t reps, float* best_ms) {
   b->block = rc == BB_OK ? best_block : saved_block;
   b->pgs_lds = rc == BB_OK ? best_lds : saved_lds;
   b->base.pgs_lcap = pgs_lds_rows(b);
+  {
+    const bb_status tr = upload_teams(b);
+    if (rc == BB_OK) rc = tr;
+  }
   if (b->graph) {
     (void)hipGraphExecDestroy(b->graph);
     b->graph = nullptr;
@@ -1110,6 +1171,7 @@ uint32_t bb_batch_get_param(const bb_batch* b, bb_param p) {
   if (!b) return 0;
   if (p == BB_PARAM_BLOCK) return pick_block(b);
   if (p == BB_PARAM_PGS_LDS_ROWS) return pgs_lds_rows(b);
+  if (p == BB_PARAM_TEAM_BODIES) return b->team_bodies;
   return 0;
 }
 

@@ -34,7 +34,31 @@ struct WorldInfo {  // 64 bytes
   uint32_t pair_off, pair_cap, flags, solver;
   uint32_t iterations, morton_bits, quirks, prev_n;
   float dt, gravity, damp;
-  uint32_t drop_prev;
+  uint32_t mode;  // kModeDropPrev | kModeTeam
+};
+// WorldInfo::mode bits.  kModeDropPrev: the previous step left nothing to
+// warm start from.  kModeTeam: the world is stepped by a group of workgroups
+// in k_team instead of one workgroup in the per-world stage kernels.
+static constexpr uint32_t kModeDropPrev = 1u, kModeTeam = 2u;
+
+// Control block of a workgroup team (k_team): barrier arrival counter and
+// generation, an error flag (a barrier wait that ran out of spins), a
+// team-wide flag for the radix sort, and per-workgroup scan partials.
+static constexpr uint32_t kMaxTeamBlocks = 2048;
+struct TeamCtl {
+  uint32_t count;
+  uint32_t gen;
+  uint32_t error;
+  uint32_t diff;
+  uint32_t part[kMaxTeamBlocks];
+};
+// Team j steps world world[j] with workgroups [first[j], first[j+1]) of the
+// k_team launch; workgroups from first[count] on have nothing to do.
+static constexpr uint32_t kMaxTeams = 256, kTeamSharedBytes = 4096;
+struct TeamTable {
+  uint32_t count;
+  uint32_t first[kMaxTeams + 1];
+  uint32_t world[kMaxTeams];
 };
 
 struct Rows {  // one parity of body rows
@@ -139,6 +163,13 @@ struct Dev {
   uint32_t* bbucket;  // [body] rows grouped by (colour, layer)
   uint32_t* bbucket_off;  // [W * (32*12+1)]
   uint32_t* stats;    // [W * ST_COUNT]
+  // teams (k_team): table, per-team control blocks, radix histograms
+  // [launch workgroups * 256] and stage scalars [kMaxTeams * kTeamSharedBytes]
+  TeamTable* teams;
+  TeamCtl* team_ctl;
+  uint32_t* team_hist;
+  char* team_shared;
+  uint32_t team_grid;     // workgroups of the k_team launch
 };
 
 }  // namespace bb

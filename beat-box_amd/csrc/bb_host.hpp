@@ -18,6 +18,7 @@ void launch_step_timed(const Dev& d, hipStream_t st, uint32_t block, hipEvent_t*
 void launch_copy(const float4* src, float4* dst, size_t n, hipStream_t st);
 void launch_sum_counts(const Dev& d, unsigned long long* out, hipStream_t st);
 void launch_energy(const Dev& d, double* out, hipStream_t st);
+uint32_t team_capacity(uint32_t block);
 }  // namespace bb
 
 #define HIPCHK(x)                                                                  \
@@ -115,6 +116,8 @@ struct bb_batch {
   hipStream_t stream = nullptr;
   uint32_t block = 0;
   uint32_t pgs_lds = 1024;  // BB_PARAM_PGS_LDS_ROWS
+  uint32_t team_bodies = 8192;  // BB_PARAM_TEAM_BODIES: worlds this large run in k_team
+  bb::TeamTable team_table{};
   uint64_t step_index = 0;
   std::vector<bb::WorldInfo> infos;
   hipGraphExec_t graph = nullptr;
@@ -153,6 +156,11 @@ uint32_t pgs_lds_rows(const bb_batch* b);
 // struct.
 void stats_from_raw(const uint32_t* raw, uint64_t step_index, bb_step_stats* out);
 uint32_t pick_block(const bb_batch* b);
+// Chooses the worlds that run as workgroup teams in k_team (at least
+// team_bodies bodies, workgroups in proportion to size within what can be
+// resident at once), sets their kModeTeam bit in b->infos and fills
+// b->team_table.  Returns the workgroups of the k_team launch (0: none).
+uint32_t assign_teams(bb_batch* b, uint32_t grid_cap);
 Dev dev_for(const bb_batch* b, int cur);
 
 }  // namespace host

@@ -166,6 +166,7 @@ struct bb_server {
   uint64_t bytes_in_use = 0, resident_bodies = 0;
   uint64_t submitted = 0, admitted_n = 0, retired_n = 0;
   uint64_t world_steps = 0, body_steps = 0;
+  bool table_dirty = false;  // a retirement changed the world table
   hipEvent_t first = nullptr, last_done = nullptr;
   bool have_prev_done = false;
   std::vector<std::pair<float, uint64_t>> step_samples;  // (ms, world-steps)
@@ -323,7 +324,7 @@ bb_status server_step(bb_server* s) {
       stage_bytes += (size_t)sl.n * (4 * 3 + 16 * 4) + ST_COUNT * 4 + 128;
     }
   }
-  stage_bytes += (size_t)(admitted.size() + retiring.size()) * (sizeof(WorldInfo) + 16);
+  stage_bytes += sizeof(WorldInfo) * (size_t)s->W + sizeof(TeamTable) + 64;
   bb_status st = ch.reserve(stage_bytes + 4096);
   if (st != BB_OK) return st;
   ch.used = 0;
@@ -340,13 +341,25 @@ bb_status server_step(bb_server* s) {
     st = reset_world_state(b, w, stream);
     if (st != BB_OK) return st;
     WorldInfo wi = info_of(hw);
-    wi.drop_prev = 0;  // the cleared pair counts already give an empty previous step
+    wi.mode = 0;  // the cleared pair counts already give an empty previous step
     b->infos[w] = wi;
-    WorldInfo* e = (WorldInfo*)ch.take(sizeof(WorldInfo));
-    *e = wi;
-    HIPCHK(hipMemcpyAsync(b->base.worlds + w, e, sizeof(WorldInfo), hipMemcpyHostToDevice, stream));
     hw.bodies.clear();
     hw.bodies.shrink_to_fit();
+  }
+  // ---- world table: after an admission or a retirement, choose the team
+  // worlds again and upload the whole table (and the team table)
+  if (!admitted.empty() || s->table_dirty) {
+    if (b->base.team_grid) {
+      assign_teams(b, b->base.team_grid);
+      TeamTable* tt = (TeamTable*)ch.take(sizeof(TeamTable));
+      *tt = b->team_table;
+      HIPCHK(hipMemcpyAsync(b->base.teams, tt, sizeof(TeamTable), hipMemcpyHostToDevice, stream));
+      HIPCHK(hipMemsetAsync(b->base.team_ctl, 0, sizeof(TeamCtl) * kMaxTeams, stream));
+    }
+    WorldInfo* tab = (WorldInfo*)ch.take(sizeof(WorldInfo) * s->W);
+    std::memcpy(tab, b->infos.data(), sizeof(WorldInfo) * s->W);
+    HIPCHK(hipMemcpyAsync(b->base.worlds, tab, sizeof(WorldInfo) * s->W, hipMemcpyHostToDevice, stream));
+    s->table_dirty = false;
   }
   StepRecord rec;
   rec.chunk = chunk;
@@ -400,11 +413,8 @@ bb_status server_step(bb_server* s) {
     rec.readbacks.push_back(r);
     // the slot becomes empty for the next step; its ranges are free now
     // (later uploads into them are ordered after this read-back)
-    WorldInfo empty{};
-    b->infos[w] = empty;
-    WorldInfo* e = (WorldInfo*)ch.take(sizeof(WorldInfo));
-    *e = empty;
-    HIPCHK(hipMemcpyAsync(b->base.worlds + w, e, sizeof(WorldInfo), hipMemcpyHostToDevice, stream));
+    b->infos[w] = WorldInfo{};
+    s->table_dirty = true;
     s->rb.release(sl.body_off, sl.n);
     s->ra.release(sl.act_off, sl.a);
     s->rp.release(sl.pair_off, sl.pair_cap);
@@ -492,6 +502,9 @@ bb_status bb_server_create(const bb_server_desc* desc, bb_server** out) {
     b->worlds.resize(s->W);
     b->infos.assign(s->W, WorldInfo{});
     b->base.pgs_lcap = b->pgs_lds;
+    // the k_team launch keeps one shape (half of what can be resident) and
+    // the team table in device memory says which workgroups step which world
+    b->base.team_grid = b->team_bodies ? std::min(team_capacity(b->block) / 2, kMaxTeamBlocks) : 0;
     if (hipMemcpy(b->base.worlds, b->infos.data(), s->W * sizeof(WorldInfo), hipMemcpyHostToDevice) != hipSuccess ||
         hipEventCreate(&s->first) != hipSuccess)
       st = BB_ERR_DEVICE;

@@ -223,6 +223,182 @@ BB_DI bool block_radix_sort(uint64_t* ka, uint32_t* ia, uint64_t* kb, uint32_t* 
   return in_a;  // true: sorted data in (ka, ia)
 }
 
+// ================================================================== teams
+// A team of threads advances one world through the stage functions s_*.
+// BlockTeam is one workgroup (the per-world stage kernels, many worlds per
+// launch).  GroupTeam is a contiguous range of workgroups of k_team (large
+// worlds); its barrier is an arrival counter plus generation in global
+// memory, and its scan combines per-workgroup partials.  Team scalars of a
+// stage live in LDS for a BlockTeam and in global memory for a GroupTeam; the
+// stage code is the same.
+template <int BLOCK>
+struct BlockTeam {
+  ScanLds<BLOCK>* scan_lds;
+  SortLds<BLOCK>* sort_lds;
+  BB_DI uint32_t rank() const { return threadIdx.x; }
+  BB_DI uint32_t size() const { return BLOCK; }
+  BB_DI void sync() const { __syncthreads(); }
+  BB_DI void fence() const { __threadfence_block(); }
+  BB_DI uint32_t scan(uint32_t v, uint32_t& total) const { return block_scan<BLOCK>(v, *scan_lds, total); }
+  BB_DI bool radix_sort(uint64_t* ka, uint32_t* ia, uint64_t* kb, uint32_t* ib, uint32_t n, int passes) const {
+    return block_radix_sort<BLOCK>(ka, ia, kb, ib, n, passes, *sort_lds);
+  }
+};
+
+// Spins before a team barrier wait gives up (about a second): a wait that
+// runs out sets TeamCtl::error, and from then on the team's barriers no
+// longer wait, so every workgroup reaches the end of the kernel.
+static constexpr uint32_t kTeamSpinLimit = 1u << 24;
+
+template <int BLOCK>
+struct GroupTeam {
+  ScanLds<BLOCK>* scan_lds;
+  SortLds<BLOCK>* sort_lds;
+  TeamCtl* ctl;
+  uint32_t* hist;  // [nblocks * 256] this team's radix histograms / offsets
+  uint32_t blk, nblocks;
+  BB_DI uint32_t rank() const { return blk * BLOCK + threadIdx.x; }
+  BB_DI uint32_t size() const { return nblocks * BLOCK; }
+  BB_DI void fence() const { __threadfence(); }
+  BB_DI void sync() const {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const uint32_t g = __hip_atomic_load(&ctl->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __threadfence();
+      if (__hip_atomic_load(&ctl->error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u) {
+        if (atomicAdd(&ctl->count, 1u) == nblocks - 1u) {
+          __hip_atomic_store(&ctl->count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          __threadfence();
+          atomicAdd(&ctl->gen, 1u);
+        } else {
+          uint32_t spins = 0;
+          while (__hip_atomic_load(&ctl->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == g) {
+            if (++spins > kTeamSpinLimit) {
+              atomicOr(&ctl->error, 1u);
+              break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+          }
+        }
+      }
+      __threadfence();
+    }
+    __syncthreads();
+  }
+  // Exclusive scan over the whole team (rank order); two team barriers.
+  BB_DI uint32_t scan(uint32_t v, uint32_t& total) const {
+    uint32_t btot;
+    const uint32_t ex = block_scan<BLOCK>(v, *scan_lds, btot);
+    if (threadIdx.x == 0) ctl->part[blk] = btot;
+    sync();
+    if (threadIdx.x < WAVE) {
+      uint32_t pre = 0, all = 0;
+      for (uint32_t b = threadIdx.x; b < nblocks; b += WAVE) {
+        const uint32_t x = ctl->part[b];
+        all += x;
+        if (b < blk) pre += x;
+      }
+      for (int o = 32; o > 0; o >>= 1) {
+        pre += __shfl_down(pre, o, WAVE);
+        all += __shfl_down(all, o, WAVE);
+      }
+      if (threadIdx.x == 0) {
+        scan_lds->w[0] = pre;
+        scan_lds->w[1] = all;
+      }
+    }
+    __syncthreads();
+    const uint32_t pre = scan_lds->w[0];
+    total = scan_lds->w[1];
+    sync();
+    return pre + ex;
+  }
+  // Stable LSD radix sort over the team: per pass each workgroup histograms
+  // its contiguous chunk, one team scan over the (digit, workgroup) counts in
+  // digit-major order gives every workgroup its output offset per digit, and
+  // each workgroup scatters its chunk with the ballot ranking of the
+  // one-workgroup sort.  Passes whose digit is constant are skipped.
+  BB_DI bool radix_sort(uint64_t* ka, uint32_t* ia, uint64_t* kb, uint32_t* ib, uint32_t n, int passes) const {
+    SortLds<BLOCK>& L = *sort_lds;
+    const uint32_t tid = threadIdx.x, lane = tid & (WAVE - 1), wave = tid / WAVE;
+    const uint32_t chunk = (n + nblocks - 1) / nblocks;
+    const uint32_t lo = blk * chunk < n ? blk * chunk : n;
+    const uint32_t hi = lo + chunk < n ? lo + chunk : n;
+    for (uint32_t i = tid; i < (BLOCK / WAVE) * 256; i += BLOCK) L.wcnt[i] = 0;
+    bool in_a = true;
+    for (int pass = 0; pass < passes; ++pass) {
+      const uint32_t shift = 8u * pass;
+      uint64_t* kin = in_a ? ka : kb;
+      uint32_t* iin = in_a ? ia : ib;
+      uint64_t* kout = in_a ? kb : ka;
+      uint32_t* iout = in_a ? ib : ia;
+      for (uint32_t i = tid; i < 256; i += BLOCK) L.base[i] = 0;
+      if (blk == 0 && tid == 0) ctl->diff = 0;
+      sync();
+      const uint32_t d0 = n ? (uint32_t)(kin[0] >> shift) & 255u : 0u;
+      bool differs = false;
+      for (uint32_t i = lo + tid; i < hi; i += BLOCK) {
+        const uint32_t dg = (uint32_t)(kin[i] >> shift) & 255u;
+        atomicAdd(&L.base[dg], 1u);
+        differs |= dg != d0;
+      }
+      if (differs) atomicOr(&ctl->diff, 1u);
+      __syncthreads();
+      for (uint32_t i = tid; i < 256; i += BLOCK) hist[i * nblocks + blk] = L.base[i];
+      sync();
+      if (ctl->diff == 0u) continue;  // team-uniform: read after the barrier
+      // offsets: exclusive scan of hist in (digit, workgroup) order
+      uint32_t carry = 0;
+      for (uint32_t e0 = 0; e0 < 256u * nblocks; e0 += size()) {
+        const uint32_t e = e0 + rank();
+        const uint32_t v = e < 256u * nblocks ? hist[e] : 0u;
+        uint32_t tot;
+        const uint32_t ex = scan(v, tot);
+        if (e < 256u * nblocks) hist[e] = carry + ex;
+        carry += tot;
+      }
+      sync();
+      for (uint32_t i = tid; i < 256; i += BLOCK) L.base[i] = hist[i * nblocks + blk];
+      __syncthreads();
+      for (uint32_t c0 = lo; c0 < hi; c0 += BLOCK) {
+        const uint32_t i = c0 + tid;
+        const bool valid = i < hi;
+        const uint64_t key = valid ? kin[i] : 0ull;
+        const uint32_t idx = valid ? iin[i] : 0u;
+        const uint32_t dg = (uint32_t)(key >> shift) & 255u;
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+          const uint64_t m = __ballot((dg >> b) & 1u);
+          peers &= ((dg >> b) & 1u) ? m : ~m;
+        }
+        const uint32_t rk = __popcll(peers & ((1ull << lane) - 1ull));
+        const uint32_t cnt = __popcll(peers);
+        const bool leader = valid && lane == (uint32_t)(__ffsll((unsigned long long)peers) - 1);
+        if (leader) L.wcnt[wave * 256 + dg] = cnt;
+        __syncthreads();
+        if (valid) {
+          uint32_t o = L.base[dg] + rk;
+          for (uint32_t w2 = 0; w2 < wave; ++w2) o += L.wcnt[w2 * 256 + dg];
+          if (o < n) {  // always true unless a barrier wait ran out
+            kout[o] = key;
+            iout[o] = idx;
+          }
+        }
+        __syncthreads();
+        if (leader) {
+          atomicAdd(&L.base[dg], cnt);
+          L.wcnt[wave * 256 + dg] = 0;
+        }
+        __syncthreads();
+      }
+      sync();
+      in_a = !in_a;
+    }
+    return in_a;
+  }
+};
+
 // =================================================================== LBVH
 struct KeyView {
   const uint64_t* k;
@@ -271,30 +447,27 @@ BB_DI int lb_split(const KeyView& K, int first, int last) {
 }
 
 // ================================================================ k_build
-template <int BLOCK>
-struct BuildLds {
-  SortLds<BLOCK> sort;
+struct BuildShared {
   uint32_t lo[3], hi[3];
 };
 
-template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_build(Dev d) {
-  __shared__ BuildLds<BLOCK> L;
-  const uint32_t w = blockIdx.x, tid = threadIdx.x;
+template <class T>
+BB_DI void s_build(const Dev& d, uint32_t w, const T& t, BuildShared& L) {
+  const uint32_t tid = t.rank(), NT = t.size();
   const WorldInfo W = d.worlds[w];
   const uint32_t N = W.n, off = W.body_off;
   // per-step resets (an empty world slot keeps zero counters)
   if (tid < ST_COUNT) d.stats[w * ST_COUNT + tid] = 0;
   if (N == 0) return;
-  for (uint32_t k = tid; k < W.a; k += BLOCK) d.link[W.act_off + k] = k;
-  for (uint32_t i = tid; i < N; i += BLOCK) d.deg[off + i] = 0;
+  for (uint32_t k = tid; k < W.a; k += NT) d.link[W.act_off + k] = k;
+  for (uint32_t i = tid; i < N; i += NT) d.deg[off + i] = 0;
   // ---- Morton keys from the previous rows (RB_sim.slang:75-111)
   uint64_t* ka = d.keys_a + off;
   uint64_t* kb = d.keys_b + off;
   uint32_t* ia = d.idx_a + off;
   uint32_t* ib = d.idx_b + off;
   if (W.morton_bits == 30) {
-    for (uint32_t i = tid; i < N; i += BLOCK) {
+    for (uint32_t i = tid; i < N; i += NT) {
       v3 p = ld3(d.prev.pos[off + i]);
       v3 q = p - V(-64.0f, -64.0f, -64.0f);
       v3 ext = V(128.0f, 128.0f, 128.0f);
@@ -306,17 +479,17 @@ __global__ __launch_bounds__(BLOCK) void k_build(Dev d) {
       L.lo[tid] = 0xFFFFFFFFu;
       L.hi[tid] = 0u;
     }
-    __syncthreads();
-    for (uint32_t i = tid; i < N; i += BLOCK) {
+    t.sync();
+    for (uint32_t i = tid; i < N; i += NT) {
       float4 p = d.prev.pos[off + i];
       atomicMin(&L.lo[0], omap(p.x)); atomicMin(&L.lo[1], omap(p.y)); atomicMin(&L.lo[2], omap(p.z));
       atomicMax(&L.hi[0], omap(p.x)); atomicMax(&L.hi[1], omap(p.y)); atomicMax(&L.hi[2], omap(p.z));
     }
-    __syncthreads();
+    t.sync();
     v3 lo = V(ounmap(L.lo[0]), ounmap(L.lo[1]), ounmap(L.lo[2]));
     v3 hi = V(ounmap(L.hi[0]), ounmap(L.hi[1]), ounmap(L.hi[2]));
     v3 ext = hi - lo;
-    for (uint32_t i = tid; i < N; i += BLOCK) {
+    for (uint32_t i = tid; i < N; i += NT) {
       v3 dd = ld3(d.prev.pos[off + i]) - lo;
       v3 p;
       p.x = ext.x > 0.0f ? dd.x / ext.x : 0.0f;
@@ -326,25 +499,25 @@ __global__ __launch_bounds__(BLOCK) void k_build(Dev d) {
       ia[i] = i;
     }
   }
-  __syncthreads();
+  t.sync();
   // tap copy of the unsorted keys lives in keys_b until the sort overwrites it;
   // the host reads taps from a dedicated copy made by k_finish instead.
   // ---- stable LSD radix sort (LBVH.slang:34-152)
   const int passes = W.morton_bits == 30 ? 4 : 8;
-  bool in_a = block_radix_sort<BLOCK>(ka, ia, kb, ib, N, passes, L.sort);
-  __syncthreads();
+  bool in_a = t.radix_sort(ka, ia, kb, ib, N, passes);
+  t.sync();
   if (!in_a) {
-    for (uint32_t i = tid; i < N; i += BLOCK) {
+    for (uint32_t i = tid; i < N; i += NT) {
       ka[i] = kb[i];
       ia[i] = ib[i];
     }
-    __syncthreads();
+    t.sync();
   }
   // ---- LBVH hierarchy (RB_sim.slang:206-278)
   const uint32_t nb = 2 * off;
   KeyView K{ka, (int)N, W.morton_bits == 30 ? 30 : 63};
   const uint32_t leaf_off = N - 1;
-  for (uint32_t i = tid; i < N; i += BLOCK) {
+  for (uint32_t i = tid; i < N; i += NT) {
     uint32_t src = off + ia[i];
     uint32_t id = d.prev.id[src];
     Shape s = shape_of(d, off + id);
@@ -353,7 +526,7 @@ __global__ __launch_bounds__(BLOCK) void k_build(Dev d) {
     d.nmin[nb + leaf_off + i] = st3(mn, __int_as_float(-1));
     d.nmax[nb + leaf_off + i] = st3(mx, __int_as_float((int)i));
   }
-  for (uint32_t i = tid; i + 1 < N; i += BLOCK) {
+  for (uint32_t i = tid; i + 1 < N; i += NT) {
     int first, last;
     lb_range(K, (int)i, first, last);
     int split = lb_split(K, first, last);
@@ -365,18 +538,18 @@ __global__ __launch_bounds__(BLOCK) void k_build(Dev d) {
     d.parent[nb + cb] = i;
     d.visit[nb + i] = 0;
   }
-  __syncthreads();
+  t.sync();
   // ---- bounds, bottom-up: the second child to arrive merges (exact min/max
   // through the order-preserving map, same values as the reference's integer
   // atomics, RB_sim.slang:282-349)
   if (N > 1) {
-    for (uint32_t i = tid; i < N; i += BLOCK) {
+    for (uint32_t i = tid; i < N; i += NT) {
       uint32_t p = d.parent[nb + leaf_off + i];
       while (p + 1 < N) {  // p is an internal node index in [0, N-2]
-        __threadfence_block();
+        t.fence();
         uint32_t old = atomicAdd(&d.visit[nb + p], 1u);
         if (old == 0) break;
-        __threadfence_block();
+        t.fence();
         float4 a0 = d.nmin[nb + p], a1 = d.nmax[nb + p];
         int l = __float_as_int(a0.w), r = __float_as_int(a1.w);
         if (l < 0 || r < 0 || l >= (int)(2 * N - 1) || r >= (int)(2 * N - 1)) break;
@@ -389,7 +562,7 @@ __global__ __launch_bounds__(BLOCK) void k_build(Dev d) {
     }
   }
   // ---- reorder: rows[i] = prev rows[sorted index] (RB_sim.slang:351-386)
-  for (uint32_t i = tid; i < N; i += BLOCK) {
+  for (uint32_t i = tid; i < N; i += NT) {
     uint32_t src = off + ia[i], dst = off + i;
     uint32_t id = d.prev.id[src];
     d.cur.id[dst] = id;
@@ -402,6 +575,16 @@ __global__ __launch_bounds__(BLOCK) void k_build(Dev d) {
     d.cur.pvel[dst] = d.prev.pvel[src];
     d.entries_cur[off + id] = i;
   }
+}
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_build(Dev d) {
+  __shared__ BuildShared L;
+  __shared__ ScanLds<BLOCK> S;
+  __shared__ SortLds<BLOCK> Q;
+  if (d.worlds[blockIdx.x].mode & kModeTeam) return;
+  const BlockTeam<BLOCK> t{&S, &Q};
+  s_build(d, blockIdx.x, t, L);
 }
 
 // ================================================================ k_broad
@@ -471,17 +654,19 @@ BB_DI void sort8(uint32_t (&k)[8]) {
   cas_u32(k[1], k[2]); cas_u32(k[3], k[4]); cas_u32(k[5], k[6]);
 }
 
-template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_broad(Dev d) {
-  __shared__ ScanLds<BLOCK> S;
-  __shared__ uint32_t carry;
-  const uint32_t w = blockIdx.x, tid = threadIdx.x;
+struct BroadShared {
+  uint32_t carry;
+};
+
+template <class T>
+BB_DI void s_broad(const Dev& d, uint32_t w, const T& t, BroadShared& L) {
+  const uint32_t tid = t.rank(), NT = t.size();
   const WorldInfo W = d.worlds[w];
   const uint32_t N = W.n, off = W.body_off, nb = 2 * off;
   uint32_t* stage = d.pick + W.pair_off;
-  if (tid == 0) carry = 0;
-  __syncthreads();
-  for (uint32_t c0 = 0; c0 < N; c0 += BLOCK) {
+  if (tid == 0) L.carry = 0;
+  t.sync();
+  for (uint32_t c0 = 0; c0 < N; c0 += NT) {
     uint32_t i = c0 + tid;
     uint32_t cnt = 0;
     if (i < N) {
@@ -493,16 +678,16 @@ __global__ __launch_bounds__(BLOCK) void k_broad(Dev d) {
       d.pcount[off + i] = cnt;
     }
     uint32_t total;
-    uint32_t ex = block_scan<BLOCK>(cnt, S, total);
+    uint32_t ex = t.scan(cnt, total);
     if (i < N) {
-      d.poff[off + i] = carry + ex;
-      d.pstart_cur[off + i] = carry + ex;
+      d.poff[off + i] = L.carry + ex;
+      d.pstart_cur[off + i] = L.carry + ex;
     }
-    __syncthreads();
-    if (tid == 0) carry += total;
-    __syncthreads();
+    t.sync();
+    if (tid == 0) L.carry += total;
+    t.sync();
   }
-  const uint32_t P = carry;
+  const uint32_t P = L.carry;
   const uint32_t cap = W.pair_cap;
   if (tid == 0) {
     d.npairs_cur[w] = P < cap ? P : cap;
@@ -510,7 +695,7 @@ __global__ __launch_bounds__(BLOCK) void k_broad(Dev d) {
     if (P > cap) d.stats[w * ST_COUNT + ST_OVERFLOW] = 1;
   }
   uint2* pairs = d.pairs_cur + W.pair_off;
-  for (uint32_t i = tid; i < N; i += BLOCK) {
+  for (uint32_t i = tid; i < N; i += NT) {
     const uint32_t o = d.poff[off + i];
     if (o >= cap) continue;
     const uint32_t room = cap - o;
@@ -546,6 +731,15 @@ __global__ __launch_bounds__(BLOCK) void k_broad(Dev d) {
       pairs[o + b2] = x;
     }
   }
+}
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_broad(Dev d) {
+  __shared__ BroadShared L;
+  __shared__ ScanLds<BLOCK> S;
+  if (d.worlds[blockIdx.x].mode & kModeTeam) return;
+  const BlockTeam<BLOCK> t{&S, nullptr};
+  s_broad(d, blockIdx.x, t, L);
 }
 
 // ==================================================== narrow phase (SAT)
@@ -1122,17 +1316,17 @@ BB_DI uint32_t warm_start(const Dev& d, uint32_t os, bool swapped, int key, int 
   return matched_new;
 }
 
-template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_narrow(Dev d) {
-  const uint32_t w = blockIdx.x, tid = threadIdx.x;
+template <class T>
+BB_DI void s_narrow(const Dev& d, uint32_t w, const T& t) {
+  const uint32_t tid = t.rank(), NT = t.size();
   const WorldInfo W = d.worlds[w];
   const uint32_t off = W.body_off, P = d.npairs_cur[w];
   const uint2* pairs = d.pairs_cur + W.pair_off;
-  const bool warm = (W.flags & BB_SIM_WARM_STARTING) != 0 && W.drop_prev == 0;
+  const bool warm = (W.flags & BB_SIM_WARM_STARTING) != 0 && (W.mode & kModeDropPrev) == 0;
   const uint32_t Pprev = d.npairs_prev[w];
   const uint2* ppairs = d.pairs_prev + W.pair_off;
   uint32_t fresh = 0, pen_mm = 0, ccount = 0, mcount = 0;
-  for (uint32_t p = tid; p < P; p += BLOCK) {
+  for (uint32_t p = tid; p < P; p += NT) {
     const uint32_t slot = W.pair_off + p;
     const uint2 pr = pairs[p];
     const uint32_t ga = off + pr.x, gb = off + pr.y;
@@ -1217,6 +1411,13 @@ __global__ __launch_bounds__(BLOCK) void k_narrow(Dev d) {
   if (pen_mm) stat_max(d, w, ST_PEN, pen_mm);
 }
 
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_narrow(Dev d) {
+  if (d.worlds[blockIdx.x].mode & kModeTeam) return;
+  const BlockTeam<BLOCK> t{nullptr, nullptr};
+  s_narrow(d, blockIdx.x, t);
+}
+
 // ================================================================= k_post
 // RigidBody::advect / advance (src/shaders/extensions.slang:127-139); the
 // damping factor exp(-dt*0.1) is evaluated once on the host.
@@ -1257,24 +1458,21 @@ BB_DI uint32_t gc_priority(uint32_t pid, uint32_t round) {  // coloring.slang:13
   return (h << 20) | (pid & 0xFFFFFu);
 }
 
-template <int BLOCK>
-BB_DI void iw_refresh(const Dev& d, const WorldInfo& W);
+template <class T>
+BB_DI void iw_refresh(const Dev& d, const WorldInfo& W, const T& t);
 BB_DI bool pgs_solvable(const Dev& d, uint32_t off, int4 h);
 BB_DI void pgs_compact(const Dev& d, const WorldInfo& W, uint32_t k, uint32_t slot);
 
-template <int BLOCK>
-struct PostLds {
-  ScanLds<BLOCK> scan;
+struct PostShared {
   uint32_t carry;
   uint32_t flag;
   uint32_t bcount[kBuckets];
   uint32_t bfill[kBuckets];
 };
 
-template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_post(Dev d) {
-  __shared__ PostLds<BLOCK> L;
-  const uint32_t w = blockIdx.x, tid = threadIdx.x;
+template <class T>
+BB_DI void s_post(const Dev& d, uint32_t w, const T& t, PostShared& L) {
+  const uint32_t tid = t.rank(), NT = t.size();
   const WorldInfo W = d.worlds[w];
   const uint32_t N = W.n, off = W.body_off, A = W.a;
   const uint32_t P = d.npairs_cur[w];
@@ -1289,13 +1487,13 @@ __global__ __launch_bounds__(BLOCK) void k_post(Dev d) {
       uint32_t r = ((A + 31u) / 32u) * 32u;
       lim = r < N ? r : N;
     }
-    for (uint32_t i = tid; i < lim; i += BLOCK) advect_row(d, off, i, W);
+    for (uint32_t i = tid; i < lim; i += NT) advect_row(d, off, i, W);
   }
   // ---- islands (islands.slang:215-302): root = min active index; slots in
   // ascending root order
   if (tid == 0) L.carry = 0;
-  __syncthreads();
-  for (uint32_t c0 = 0; c0 < A; c0 += BLOCK) {
+  t.sync();
+  for (uint32_t c0 = 0; c0 < A; c0 += NT) {
     uint32_t k = c0 + tid;
     uint32_t is_root = 0;
     if (k < A) {
@@ -1305,19 +1503,19 @@ __global__ __launch_bounds__(BLOCK) void k_post(Dev d) {
       is_root = x == k ? 1u : 0u;
     }
     uint32_t tot;
-    uint32_t ex = block_scan<BLOCK>(is_root, L.scan, tot);
+    uint32_t ex = t.scan(is_root, tot);
     if (k < A && is_root) d.poff[off + k] = L.carry + ex;  // slot of root k
-    __syncthreads();
+    t.sync();
     if (tid == 0) L.carry += tot;
-    __syncthreads();
+    t.sync();
   }
   const uint32_t islands = L.carry;
-  for (uint32_t i = tid; i < N; i += BLOCK) {
+  for (uint32_t i = tid; i < N; i += NT) {
     d.island[off + i] = kMAX;
     d.islot[off + i] = kMAX;
   }
-  __syncthreads();
-  for (uint32_t k = tid; k < A; k += BLOCK) {
+  t.sync();
+  for (uint32_t k = tid; k < A; k += NT) {
     uint32_t id = d.props.active_ids[W.act_off + k];
     uint32_t row = d.entries_cur[off + id];
     uint32_t root = d.ascratch[W.act_off + k];
@@ -1325,9 +1523,9 @@ __global__ __launch_bounds__(BLOCK) void k_post(Dev d) {
     d.islot[off + row] = d.poff[off + root];
   }
   // contact islands: islands owning at least one manifold
-  for (uint32_t k = tid; k < A; k += BLOCK) d.pcount[off + k] = 0;
-  __syncthreads();
-  for (uint32_t p = tid; p < P; p += BLOCK) {
+  for (uint32_t k = tid; k < A; k += NT) d.pcount[off + k] = 0;
+  t.sync();
+  for (uint32_t p = tid; p < P; p += NT) {
     int4 h = d.man_cur.hdr[W.pair_off + p];
     if (h.w <= 0) continue;
     uint32_t ia = d.props.active_rank[off + d.cur.id[off + h.x]];
@@ -1335,23 +1533,23 @@ __global__ __launch_bounds__(BLOCK) void k_post(Dev d) {
     uint32_t mn = ia < ib ? ia : ib;
     if (mn < A) d.pcount[off + d.ascratch[W.act_off + mn]] = 1;
   }
-  __syncthreads();
+  t.sync();
   {
     uint32_t cnt = 0;
-    for (uint32_t k = tid; k < A; k += BLOCK) cnt += d.pcount[off + k];
+    for (uint32_t k = tid; k < A; k += NT) cnt += d.pcount[off + k];
     if (tid == 0) L.flag = 0;
-    __syncthreads();
+    t.sync();
     atomicAdd(&L.flag, cnt);
-    __syncthreads();
+    t.sync();
     if (tid == 0) {
       d.stats[w * ST_COUNT + ST_ISLANDS] = islands;
       d.stats[w * ST_COUNT + ST_CISLANDS] = L.flag;
     }
-    __syncthreads();
+    t.sync();
   }
   // ---- sleeping (RB_sim.slang:556-737)
   const bool sleep_on = (flags & BB_SIM_SLEEPING_ENABLED) != 0;
-  for (uint32_t i = tid; i < N; i += BLOCK) {
+  for (uint32_t i = tid; i < N; i += NT) {
     const uint32_t g = off + i;
     uint32_t fl = d.cur.flags[g];
     if (!(fl & BB_BODY_DYNAMIC)) continue;
@@ -1372,9 +1570,9 @@ __global__ __launch_bounds__(BLOCK) void k_post(Dev d) {
     bool calm = lin2 < 0.0144f && ang2 < 0.0225f;
     d.cur.sleep[g] = calm ? (timer + 1u < 30u ? timer + 1u : 30u) : (timer >= 4u ? timer - 4u : 0u);
   }
-  __syncthreads();
+  t.sync();
   if (sleep_on) {
-    for (uint32_t p = tid; p < P; p += BLOCK) {
+    for (uint32_t p = tid; p < P; p += NT) {
       const uint32_t slot = W.pair_off + p;
       int4 h = d.man_cur.hdr[slot];
       if (h.w <= 0) continue;
@@ -1391,9 +1589,9 @@ __global__ __launch_bounds__(BLOCK) void k_post(Dev d) {
       if ((aloud || deep) && bdyn) atomicOr(&d.cur.sleep[gb], 0x80000000u);
       if ((bloud || deep) && adyn) atomicOr(&d.cur.sleep[ga], 0x80000000u);
     }
-    __syncthreads();
+    t.sync();
     uint32_t sl_count = 0;
-    for (uint32_t i = tid; i < N; i += BLOCK) {
+    for (uint32_t i = tid; i < N; i += NT) {
       const uint32_t g = off + i;
       uint32_t fl = d.cur.flags[g];
       if (!(fl & BB_BODY_DYNAMIC)) continue;
@@ -1415,20 +1613,20 @@ __global__ __launch_bounds__(BLOCK) void k_post(Dev d) {
     }
     stat_add(d, w, ST_SLEEPING, sl_count);
   }
-  __syncthreads();
+  t.sync();
   if (!pgs) return;
   // ---- edge colouring (coloring.slang:44-292).  Phase 2 commits at the
   // colour picked in phase 1; the owner cells touched in a round are cleared
   // by a third phase so the table is zero between rounds.
   uint32_t* color = d.color + W.pair_off;
   uint32_t* pick = d.pick + W.pair_off;
-  for (uint32_t i = tid; i < N; i += BLOCK) d.mask[off + i] = 0;
-  for (uint32_t p = tid; p < P; p += BLOCK) color[p] = d.man_cur.hdr[W.pair_off + p].w > 0 ? kMAX : kMAX - 1;
-  __syncthreads();
+  for (uint32_t i = tid; i < N; i += NT) d.mask[off + i] = 0;
+  for (uint32_t p = tid; p < P; p += NT) color[p] = d.man_cur.hdr[W.pair_off + p].w > 0 ? kMAX : kMAX - 1;
+  t.sync();
   for (uint32_t round = 1; round <= kColors; ++round) {
     if (tid == 0) L.flag = 0;
-    __syncthreads();
-    for (uint32_t p = tid; p < P; p += BLOCK) {
+    t.sync();
+    for (uint32_t p = tid; p < P; p += NT) {
       pick[p] = kColors;
       if (color[p] != kMAX) continue;
       int4 h = d.man_cur.hdr[W.pair_off + p];
@@ -1446,9 +1644,9 @@ __global__ __launch_bounds__(BLOCK) void k_post(Dev d) {
       if (!sb) atomicMax(&d.owner[gb * kColors + cc], pr);
       L.flag = 1;
     }
-    __syncthreads();
+    t.sync();
     const bool any = L.flag != 0;
-    for (uint32_t p = tid; p < P; p += BLOCK) {
+    for (uint32_t p = tid; p < P; p += NT) {
       if (color[p] != kMAX || pick[p] >= kColors) continue;
       int4 h = d.man_cur.hdr[W.pair_off + p];
       const uint32_t ga = off + h.x, gb = off + h.y, cc = pick[p];
@@ -1465,21 +1663,21 @@ __global__ __launch_bounds__(BLOCK) void k_post(Dev d) {
         if (!sb) atomicOr(&d.mask[gb], 1u << cc);
       }
     }
-    __syncthreads();
-    for (uint32_t p = tid; p < P; p += BLOCK) {
+    t.sync();
+    for (uint32_t p = tid; p < P; p += NT) {
       if (pick[p] >= kColors) continue;
       int4 h = d.man_cur.hdr[W.pair_off + p];
       d.owner[(off + h.x) * kColors + pick[p]] = 0;
       d.owner[(off + h.y) * kColors + pick[p]] = 0;
     }
-    __syncthreads();
+    t.sync();
     if (!any) break;
   }
   // ---- validator (coloring.slang:214-292) and colour buckets
   if (tid < kBuckets) L.bcount[tid] = 0;
-  __syncthreads();
+  t.sync();
   uint32_t viol = 0, used = 0;
-  for (uint32_t p = tid; p < P; p += BLOCK) {
+  for (uint32_t p = tid; p < P; p += NT) {
     uint32_t cc = color[p];
     if (cc == kMAX - 1) continue;  // no manifold in this slot
     if (cc >= kColors) {
@@ -1497,31 +1695,31 @@ __global__ __launch_bounds__(BLOCK) void k_post(Dev d) {
   }
   stat_add(d, w, ST_GC_VIOL, viol);
   if (used) stat_max(d, w, ST_GC_COUNT, used);
-  __syncthreads();
-  for (uint32_t p = tid; p < P; p += BLOCK) {
+  t.sync();
+  for (uint32_t p = tid; p < P; p += NT) {
     uint32_t cc = color[p];
     if (cc >= kColors) continue;
     int4 h = d.man_cur.hdr[W.pair_off + p];
     d.owner[(off + h.x) * kColors + cc] = 0;
     d.owner[(off + h.y) * kColors + cc] = 0;
   }
-  __syncthreads();
+  t.sync();
   // ---- colour buckets of the solvable manifolds (at least one awake dynamic
   // body; the others are skipped by every sweep, RB_sim.slang:1424).  Order
   // inside a colour is free (no shared dynamic body); the overflow bucket is
   // filled in ascending slot order for its serial solve.
   if (tid == 0) d.stats[w * ST_COUNT + ST_GC_OVF] = L.bcount[kColors];
-  __syncthreads();
+  t.sync();
   if (tid < kBuckets) L.bcount[tid] = 0;
-  __syncthreads();
-  for (uint32_t p = tid; p < P; p += BLOCK) {
+  t.sync();
+  for (uint32_t p = tid; p < P; p += NT) {
     uint32_t cc = color[p];
     if (cc == kMAX - 1) continue;
     int4 h = d.man_cur.hdr[W.pair_off + p];
     if (!pgs_solvable(d, off, h)) continue;
     atomicAdd(&L.bcount[cc < kColors ? cc : kColors], 1u);
   }
-  __syncthreads();
+  t.sync();
   if (tid == 0) {
     uint32_t sum = 0;
     uint32_t* bo = d.bucket_off + w * (kBuckets + 1);
@@ -1532,9 +1730,9 @@ __global__ __launch_bounds__(BLOCK) void k_post(Dev d) {
     }
     bo[kBuckets] = sum;
   }
-  __syncthreads();
+  t.sync();
   uint32_t* bucket = d.bucket + W.pair_off;
-  for (uint32_t p = tid; p < P; p += BLOCK) {
+  for (uint32_t p = tid; p < P; p += NT) {
     uint32_t cc = color[p];
     if (cc >= kColors) continue;
     int4 h = d.man_cur.hdr[W.pair_off + p];
@@ -1543,25 +1741,34 @@ __global__ __launch_bounds__(BLOCK) void k_post(Dev d) {
   }
   if (L.bcount[kColors]) {
     if (tid == 0) L.carry = L.bfill[kColors];
-    __syncthreads();
-    for (uint32_t c0 = 0; c0 < P; c0 += BLOCK) {
+    t.sync();
+    for (uint32_t c0 = 0; c0 < P; c0 += NT) {
       uint32_t p = c0 + tid;
       uint32_t f = 0;
       if (p < P && color[p] != kMAX - 1 && color[p] >= kColors)
         f = pgs_solvable(d, off, d.man_cur.hdr[W.pair_off + p]) ? 1u : 0u;
       uint32_t tot;
-      uint32_t ex = block_scan<BLOCK>(f, L.scan, tot);
+      uint32_t ex = t.scan(f, tot);
       if (f) bucket[L.carry + ex] = W.pair_off + p;
-      __syncthreads();
+      t.sync();
       if (tid == 0) L.carry += tot;
-      __syncthreads();
+      t.sync();
     }
   }
   // ---- world inverse inertia of every row, then the solver rows
-  iw_refresh<BLOCK>(d, W);
-  __syncthreads();
+  iw_refresh(d, W, t);
+  t.sync();
   const uint32_t total = d.bucket_off[w * (kBuckets + 1) + kBuckets];
-  for (uint32_t k = tid; k < total; k += BLOCK) pgs_compact(d, W, W.pair_off + k, bucket[k]);
+  for (uint32_t k = tid; k < total; k += NT) pgs_compact(d, W, W.pair_off + k, bucket[k]);
+}
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_post(Dev d) {
+  __shared__ PostShared L;
+  __shared__ ScanLds<BLOCK> S;
+  if (d.worlds[blockIdx.x].mode & kModeTeam) return;
+  const BlockTeam<BLOCK> t{&S, nullptr};
+  s_post(d, blockIdx.x, t, L);
 }
 
 // ================================================================== k_pgs
@@ -1608,10 +1815,10 @@ BB_DI mat3 iw_load(const Dev& d, uint32_t g) {
 }
 
 // Caches R*I^-1*R^T (or the quirk form) per row with inv_mass in row0.w.
-template <int BLOCK>
-BB_DI void iw_refresh(const Dev& d, const WorldInfo& W) {
+template <class T>
+BB_DI void iw_refresh(const Dev& d, const WorldInfo& W, const T& t) {
   const bool quirk = (W.quirks & BB_QUIRK_TRANSPOSED_WORLD_INERTIA) != 0;
-  for (uint32_t i = threadIdx.x; i < W.n; i += BLOCK) {
+  for (uint32_t i = t.rank(); i < W.n; i += t.size()) {
     const uint32_t g = W.body_off + i;
     const uint32_t id = d.cur.id[g];
     mat3 M = world_inv_inertia(d, W.body_off + id, ldq(d.cur.rot[g]), quirk);
@@ -1823,10 +2030,9 @@ BB_DI qt apply_angular_velocity(qt q, v3 om, float dt) {
   return qnorm(qmul(qt{dr, 1.0f}, q));
 }
 
-template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_pgs(Dev d) {
-  extern __shared__ float4 pgs_lds[];
-  const uint32_t w = blockIdx.x, tid = threadIdx.x;
+template <class T>
+BB_DI void s_pgs(const Dev& d, uint32_t w, const T& t, float4* pgs_lds, uint32_t lcap) {
+  const uint32_t tid = t.rank(), NT = t.size();
   const WorldInfo W = d.worlds[w];
   if (W.solver == BB_SOLVER_AVBD || W.n == 0) return;
   const uint32_t off = W.body_off, N = W.n, A = W.a, NP = d.np;
@@ -1843,21 +2049,21 @@ __global__ __launch_bounds__(BLOCK) void k_pgs(Dev d) {
   const uint32_t used = d.stats[w * ST_COUNT + ST_GC_COUNT];
   const uint32_t base = W.pair_off;
   const uint32_t ovf_lo = bo[kColors], ovf_hi = bo[kBuckets];
-  const VelCache vc{pgs_lds, off, N < d.pgs_lcap ? N : d.pgs_lcap};
-  for (uint32_t i = tid; i < vc.lcap; i += BLOCK) {
+  const VelCache vc{pgs_lds, off, N < lcap ? N : lcap};
+  for (uint32_t i = tid; i < vc.lcap; i += NT) {
     pgs_lds[2 * i] = d.cur.vel[off + i];
     pgs_lds[2 * i + 1] = d.cur.omg[off + i];
   }
-  __syncthreads();
+  t.sync();
   auto sweep = [&](int kind) {
     for (uint32_t c = 0; c < used; ++c) {
-      for (uint32_t k = bo[c] + tid; k < bo[c + 1]; k += BLOCK) pgs_row(d, vc, off, base + k, s, kind);
-      __syncthreads();
+      for (uint32_t k = bo[c] + tid; k < bo[c + 1]; k += NT) pgs_row(d, vc, off, base + k, s, kind);
+      t.sync();
     }
     if (ovf_hi > ovf_lo) {
       if (tid == 0)
         for (uint32_t k = ovf_lo; k < ovf_hi; ++k) pgs_row(d, vc, off, base + k, s, kind);
-      __syncthreads();
+      t.sync();
     }
   };
   sweep(0);
@@ -1869,7 +2075,7 @@ __global__ __launch_bounds__(BLOCK) void k_pgs(Dev d) {
       uint32_t r = ((A + 31u) / 32u) * 32u;
       lim = r < N ? r : N;
     }
-    for (uint32_t i = tid; i < lim; i += BLOCK) {
+    for (uint32_t i = tid; i < lim; i += NT) {
       const uint32_t g = off + i;
       uint32_t fl = d.cur.flags[g];
       if (!((fl & BB_BODY_DYNAMIC) && !(fl & BB_BODY_SLEEPING))) continue;
@@ -1880,18 +2086,18 @@ __global__ __launch_bounds__(BLOCK) void k_pgs(Dev d) {
       d.cur.rot[g] = stq(apply_angular_velocity(ldq(d.cur.rot[g]), om, W.dt));
     }
   }
-  __syncthreads();
-  iw_refresh<BLOCK>(d, W);
-  __syncthreads();
+  t.sync();
+  iw_refresh(d, W, t);
+  t.sync();
   for (uint32_t it = 0; it < W.iterations; ++it) sweep(2);
-  for (uint32_t i = tid; i < vc.lcap; i += BLOCK) {
+  for (uint32_t i = tid; i < vc.lcap; i += NT) {
     d.cur.vel[off + i] = pgs_lds[2 * i];
     d.cur.omg[off + i] = pgs_lds[2 * i + 1];
   }
   // impulses and pre-step quantities back into the manifold slots (warm start
   // of the next step, taps)
   const uint32_t* bucket = d.bucket + base;
-  for (uint32_t k = tid; k < ovf_hi; k += BLOCK) {
+  for (uint32_t k = tid; k < ovf_hi; k += NT) {
     const uint32_t row = base + k, slot = bucket[k];
     const int4 h = d.sv_hdr[row];
     const Soft sc = (h.z & 1024) ? s.soft[1] : s.soft[0];
@@ -1905,6 +2111,14 @@ __global__ __launch_bounds__(BLOCK) void k_pgs(Dev d) {
       d.man_cur.cana[slot * 8 + c] = a;
     }
   }
+}
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_pgs(Dev d) {
+  extern __shared__ float4 pgs_lds[];
+  if (d.worlds[blockIdx.x].mode & kModeTeam) return;
+  const BlockTeam<BLOCK> t{nullptr, nullptr};
+  s_pgs(d, blockIdx.x, t, pgs_lds, d.pgs_lcap);
 }
 
 // ================================================================== AVBD
@@ -2175,9 +2389,7 @@ BB_DI void av_dual(const Dev& d, const WorldInfo& W, uint32_t slot) {
   }
 }
 
-template <int BLOCK>
-struct AvbdLds {
-  ScanLds<BLOCK> scan;
+struct AvbdShared {
   uint32_t carry;
   uint32_t flag;
   uint32_t used;
@@ -2185,32 +2397,31 @@ struct AvbdLds {
   uint32_t boff[AV_NB + 1];
 };
 
-template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_avbd(Dev d) {
-  __shared__ AvbdLds<BLOCK> L;
-  const uint32_t w = blockIdx.x, tid = threadIdx.x;
+template <class T>
+BB_DI void s_avbd(const Dev& d, uint32_t w, const T& t, AvbdShared& L) {
+  const uint32_t tid = t.rank(), NT = t.size();
   const WorldInfo W = d.worlds[w];
   if (W.solver != BB_SOLVER_AVBD || W.n == 0) return;
   const uint32_t N = W.n, off = W.body_off, P = d.npairs_cur[w];
   const float dt = W.dt;
   // ---- adjacency (CSR, ascending slot per row) for dynamic rows
   if (tid == 0) L.carry = 0;
-  __syncthreads();
-  for (uint32_t c0 = 0; c0 < N; c0 += BLOCK) {
+  t.sync();
+  for (uint32_t c0 = 0; c0 < N; c0 += NT) {
     uint32_t i = c0 + tid;
     uint32_t v = i < N ? d.deg[off + i] : 0u;
     uint32_t tot;
-    uint32_t ex = block_scan<BLOCK>(v, L.scan, tot);
+    uint32_t ex = t.scan(v, tot);
     if (i < N) {
       d.adj_off[off + i] = L.carry + ex;
       d.pcount[off + i] = 0;  // fill cursor
     }
-    __syncthreads();
+    t.sync();
     if (tid == 0) L.carry += tot;
-    __syncthreads();
+    t.sync();
   }
   uint32_t* adj = d.adj + 2 * W.pair_off;
-  for (uint32_t p = tid; p < P; p += BLOCK) {
+  for (uint32_t p = tid; p < P; p += NT) {
     const uint32_t slot = W.pair_off + p;
     int4 h = d.man_cur.hdr[slot];
     if (h.w <= 0) continue;
@@ -2218,8 +2429,8 @@ __global__ __launch_bounds__(BLOCK) void k_avbd(Dev d) {
     if (d.cur.flags[ga] & BB_BODY_DYNAMIC) adj[d.adj_off[ga] + atomicAdd(&d.pcount[ga], 1u)] = slot;
     if (d.cur.flags[gb] & BB_BODY_DYNAMIC) adj[d.adj_off[gb] + atomicAdd(&d.pcount[gb], 1u)] = slot;
   }
-  __syncthreads();
-  for (uint32_t i = tid; i < N; i += BLOCK) {
+  t.sync();
+  for (uint32_t i = tid; i < N; i += NT) {
     const uint32_t g = off + i;
     if (!(d.cur.flags[g] & BB_BODY_DYNAMIC)) continue;
     uint32_t* l = adj + d.adj_off[g];
@@ -2234,21 +2445,22 @@ __global__ __launch_bounds__(BLOCK) void k_avbd(Dev d) {
     }
     d.bcolor[g] = kMAX;
   }
-  for (uint32_t i = tid; i < N; i += BLOCK)
+  for (uint32_t i = tid; i < N; i += NT)
     if (!(d.cur.flags[off + i] & BB_BODY_DYNAMIC)) d.bcolor[off + i] = kMAX;
-  __syncthreads();
+  t.sync();
   // ---- Jones-Plassmann body colouring, round-snapshot semantics
   // (entry_avbd_color_round, avbd.slang:846-911)
   for (uint32_t rd = 0; rd < AV_ROUNDS; ++rd) {
     if (tid == 0) L.flag = 0;
-    for (uint32_t i = tid; i < N; i += BLOCK) {
+    t.sync();
+    for (uint32_t i = tid; i < N; i += NT) {
       uint32_t c = d.bcolor[off + i];
       d.bcolor_snap[off + i] = c;
       if (c == kMAX && (d.cur.flags[off + i] & BB_BODY_DYNAMIC)) L.flag = 1;
     }
-    __syncthreads();
+    t.sync();
     if (L.flag == 0) break;
-    for (uint32_t i = tid; i < N; i += BLOCK) {
+    for (uint32_t i = tid; i < N; i += NT) {
       const uint32_t g = off + i;
       if (d.bcolor_snap[g] != kMAX || !(d.cur.flags[g] & BB_BODY_DYNAMIC)) continue;
       const uint32_t my = avbd_priority(d.cur.id[g], rd);
@@ -2276,12 +2488,12 @@ __global__ __launch_bounds__(BLOCK) void k_avbd(Dev d) {
       uint32_t c = (uint32_t)(__ffs(avail) - 1);
       if (c < kAvbdColors) d.bcolor[g] = c;
     }
-    __syncthreads();
+    t.sync();
   }
   // validator (entry_avbd_color_validate, avbd.slang:913-981)
   {
     uint32_t viol = 0, cnt = 0;
-    for (uint32_t i = tid; i < N; i += BLOCK) {
+    for (uint32_t i = tid; i < N; i += NT) {
       const uint32_t g = off + i;
       if (!(d.cur.flags[g] & BB_BODY_DYNAMIC)) continue;
       const uint32_t mc = d.bcolor[g];
@@ -2304,7 +2516,7 @@ __global__ __launch_bounds__(BLOCK) void k_avbd(Dev d) {
     if (cnt) stat_max(d, w, ST_AV_COUNT, cnt);
   }
   // ---- prepare (avbd.slang:70-128)
-  for (uint32_t i = tid; i < N; i += BLOCK) {
+  for (uint32_t i = tid; i < N; i += NT) {
     const uint32_t g = off + i;
     const uint32_t fl = d.cur.flags[g];
     if (!(fl & BB_BODY_DYNAMIC) || (fl & BB_BODY_SLEEPING)) continue;
@@ -2330,12 +2542,12 @@ __global__ __launch_bounds__(BLOCK) void k_avbd(Dev d) {
     d.cur.pos[g] = st3((p + v * dt) + V(0.0f, grav, 0.0f) * (accel_w * dt * dt), 0.0f);
     d.cur.rot[g] = stq(qtl);
   }
-  __syncthreads();
+  t.sync();
   // ---- warm start (avbd.slang:389-473)
   {
     uint32_t sticks = 0;
     const bool ws_on = (W.flags & BB_SIM_WARM_STARTING) != 0;
-    for (uint32_t p = tid; p < P; p += BLOCK) {
+    for (uint32_t p = tid; p < P; p += NT) {
       const uint32_t slot = W.pair_off + p;
       int4 h = d.man_cur.hdr[slot];
       if (h.w <= 0) continue;
@@ -2379,29 +2591,29 @@ __global__ __launch_bounds__(BLOCK) void k_avbd(Dev d) {
     stat_add(d, w, ST_AV_STICK, sticks);
   }
   // ---- support depth (avbd.slang:667-724), pass-snapshot semantics
-  for (uint32_t i = tid; i < N; i += BLOCK) {
+  for (uint32_t i = tid; i < N; i += NT) {
     const uint32_t fl = d.cur.flags[off + i];
     const bool grounded = !(fl & BB_BODY_DYNAMIC) || (fl & BB_BODY_SLEEPING);
     d.depth[off + i] = grounded ? 0u : kMAX;
   }
-  __syncthreads();
+  t.sync();
   for (uint32_t pass = 0; pass < kAvbdLayers; ++pass) {
-    for (uint32_t i = tid; i < N; i += BLOCK) d.depth_snap[off + i] = d.depth[off + i];
-    __syncthreads();
-    for (uint32_t p = tid; p < P; p += BLOCK) {
+    for (uint32_t i = tid; i < N; i += NT) d.depth_snap[off + i] = d.depth[off + i];
+    t.sync();
+    for (uint32_t p = tid; p < P; p += NT) {
       int4 h = d.man_cur.hdr[W.pair_off + p];
       if (h.w <= 0) continue;
       const uint32_t d1 = d.depth_snap[off + h.x], d2 = d.depth_snap[off + h.y];
       if (d2 != kMAX && d2 + 1u < d1) atomicMin(&d.depth[off + h.x], d2 + 1u);
       if (d1 != kMAX && d1 + 1u < d2) atomicMin(&d.depth[off + h.y], d1 + 1u);
     }
-    __syncthreads();
+    t.sync();
   }
   // ---- bodies grouped by (colour, layer) for the primal sweeps
-  for (uint32_t k = tid; k < AV_NB; k += BLOCK) L.bcount[k] = 0;
+  for (uint32_t k = tid; k < AV_NB; k += NT) L.bcount[k] = 0;
   if (tid == 0) L.used = 0;
-  __syncthreads();
-  for (uint32_t i = tid; i < N; i += BLOCK) {
+  t.sync();
+  for (uint32_t i = tid; i < N; i += NT) {
     const uint32_t g = off + i;
     const uint32_t fl = d.cur.flags[g], c = d.bcolor[g];
     if (!(fl & BB_BODY_DYNAMIC) || (fl & BB_BODY_SLEEPING) || c >= kAvbdColors) continue;
@@ -2409,7 +2621,7 @@ __global__ __launch_bounds__(BLOCK) void k_avbd(Dev d) {
     atomicAdd(&L.bcount[c * kAvbdLayers + layer], 1u);
     atomicMax(&L.used, c + 1);
   }
-  __syncthreads();
+  t.sync();
   if (tid == 0) {
     uint32_t s = 0;
     for (uint32_t k = 0; k < AV_NB; ++k) {
@@ -2419,32 +2631,32 @@ __global__ __launch_bounds__(BLOCK) void k_avbd(Dev d) {
     }
     L.boff[AV_NB] = s;
   }
-  __syncthreads();
+  t.sync();
   uint32_t* bb = d.bbucket + off;
-  for (uint32_t i = tid; i < N; i += BLOCK) {
+  for (uint32_t i = tid; i < N; i += NT) {
     const uint32_t g = off + i;
     const uint32_t fl = d.cur.flags[g], c = d.bcolor[g];
     if (!(fl & BB_BODY_DYNAMIC) || (fl & BB_BODY_SLEEPING) || c >= kAvbdColors) continue;
     uint32_t layer = d.depth[g] < kAvbdLayers - 1 ? d.depth[g] : kAvbdLayers - 1;
     bb[atomicAdd(&L.bcount[c * kAvbdLayers + layer], 1u)] = i;
   }
-  __syncthreads();
+  t.sync();
   const uint32_t used = L.used;
   // ---- main sweeps: 10 x (colours, dual) (rigid_body_manager.cpp:1366-1373)
   for (uint32_t it = 0; it < AV_ITERS; ++it) {
     for (uint32_t c = 0; c < used; ++c) {
-      for (uint32_t k = L.boff[c * kAvbdLayers] + tid; k < L.boff[(c + 1) * kAvbdLayers]; k += BLOCK)
+      for (uint32_t k = L.boff[c * kAvbdLayers] + tid; k < L.boff[(c + 1) * kAvbdLayers]; k += NT)
         av_primal(d, W, bb[k], 1.0f);
-      __syncthreads();
+      t.sync();
     }
-    for (uint32_t p = tid; p < P; p += BLOCK)
+    for (uint32_t p = tid; p < P; p += NT)
       if (d.man_cur.hdr[W.pair_off + p].w > 0) av_dual(d, W, W.pair_off + p);
-    __syncthreads();
+    t.sync();
   }
   // ---- finalize (avbd.slang:130-202)
   {
     const float inv_dt = dt > 0.0f ? 1.0f / dt : 0.0f;
-    for (uint32_t i = tid; i < N; i += BLOCK) {
+    for (uint32_t i = tid; i < N; i += NT) {
       const uint32_t g = off + i;
       const uint32_t fl = d.cur.flags[g];
       if (!(fl & BB_BODY_DYNAMIC) || (fl & BB_BODY_SLEEPING)) continue;
@@ -2465,33 +2677,42 @@ __global__ __launch_bounds__(BLOCK) void k_avbd(Dev d) {
       d.cur.omg[g] = st3(om, 0.0f);
     }
   }
-  __syncthreads();
+  t.sync();
   // ---- post-stabilisation: one cascade by support layer, then plain sweeps
   // (rigid_body_manager.cpp:1391-1423)
   for (uint32_t layer = 0; layer < kAvbdLayers; ++layer) {
     for (uint32_t c = 0; c < used; ++c) {
       const uint32_t k0 = L.boff[c * kAvbdLayers + layer], k1 = L.boff[c * kAvbdLayers + layer + 1];
-      for (uint32_t k = k0 + tid; k < k1; k += BLOCK) av_primal(d, W, bb[k], 0.0f);
-      __syncthreads();
+      for (uint32_t k = k0 + tid; k < k1; k += NT) av_primal(d, W, bb[k], 0.0f);
+      t.sync();
     }
   }
   for (uint32_t ps = 1; ps < AV_POST_STAB; ++ps) {
     for (uint32_t c = 0; c < used; ++c) {
-      for (uint32_t k = L.boff[c * kAvbdLayers] + tid; k < L.boff[(c + 1) * kAvbdLayers]; k += BLOCK)
+      for (uint32_t k = L.boff[c * kAvbdLayers] + tid; k < L.boff[(c + 1) * kAvbdLayers]; k += NT)
         av_primal(d, W, bb[k], 0.0f);
-      __syncthreads();
+      t.sync();
     }
   }
 }
 
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_avbd(Dev d) {
+  __shared__ AvbdShared L;
+  __shared__ ScanLds<BLOCK> S;
+  if (d.worlds[blockIdx.x].mode & kModeTeam) return;
+  const BlockTeam<BLOCK> t{&S, nullptr};
+  s_avbd(d, blockIdx.x, t, L);
+}
+
 // ============================================================== k_finish
 // End-of-step counters (RB_sim.slang:1655-1715): id checksum and max |v|.
-template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_finish(Dev d) {
-  const uint32_t w = blockIdx.x, tid = threadIdx.x;
+template <class T>
+BB_DI void s_finish(const Dev& d, uint32_t w, const T& t) {
+  const uint32_t tid = t.rank(), NT = t.size();
   const WorldInfo W = d.worlds[w];
   uint32_t idsum = 0, maxv = 0;
-  for (uint32_t i = tid; i < W.n; i += BLOCK) {
+  for (uint32_t i = tid; i < W.n; i += NT) {
     const uint32_t g = W.body_off + i;
     idsum += d.cur.id[g];
     if (d.cur.flags[g] & BB_BODY_DYNAMIC) {
@@ -2507,6 +2728,56 @@ __global__ __launch_bounds__(BLOCK) void k_finish(Dev d) {
     d.stats[w * ST_COUNT + ST_A] = W.a;
     d.stats[w * ST_COUNT + ST_NODES] = 2 * d.stats[w * ST_COUNT + ST_M];
   }
+}
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_finish(Dev d) {
+  if (d.worlds[blockIdx.x].mode & kModeTeam) return;
+  const BlockTeam<BLOCK> t{nullptr, nullptr};
+  s_finish(d, blockIdx.x, t);
+}
+
+// ================================================================ k_team
+// Large worlds: every stage of one step for each team, with team barriers in
+// place of the kernel boundaries of the per-world path.
+union TeamShared {
+  BuildShared build;
+  BroadShared broad;
+  PostShared post;
+  AvbdShared avbd;
+};
+static_assert(sizeof(TeamShared) <= kTeamSharedBytes, "team scalars exceed their slot");
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_team(Dev d) {
+  __shared__ ScanLds<BLOCK> S;
+  __shared__ SortLds<BLOCK> Q;
+  const TeamTable& T = *d.teams;
+  const uint32_t nt = T.count < kMaxTeams ? T.count : kMaxTeams;
+  if (nt == 0 || blockIdx.x >= T.first[nt]) return;  // workgroup-uniform
+  uint32_t j = 0;
+  while (j + 1 < nt && blockIdx.x >= T.first[j + 1]) ++j;
+  const uint32_t b0 = T.first[j], nb = T.first[j + 1] - b0;
+  const GroupTeam<BLOCK> t{&S, &Q, d.team_ctl + j, d.team_hist + (size_t)b0 * 256, blockIdx.x - b0, nb};
+  TeamShared& G = *reinterpret_cast<TeamShared*>(d.team_shared + (size_t)j * kTeamSharedBytes);
+  const uint32_t w = T.world[j];
+  s_build(d, w, t, G.build);
+  t.sync();
+  s_broad(d, w, t, G.broad);
+  t.sync();
+  s_narrow(d, w, t);
+  t.sync();
+  s_post(d, w, t, G.post);
+  t.sync();
+  s_pgs(d, w, t, nullptr, 0u);
+  t.sync();
+  s_avbd(d, w, t, G.avbd);
+  t.sync();
+  s_finish(d, w, t);
+  t.sync();
+  // a barrier that ran out of spins: flag the world (overflow bit 1)
+  if (t.rank() == 0 && __hip_atomic_load(&t.ctl->error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u)
+    atomicOr(&d.stats[w * ST_COUNT + ST_OVERFLOW], 2u);
 }
 
 // Batch totals of one step's counters, one workgroup: out[0..7] = sum N,
@@ -2601,6 +2872,7 @@ __global__ __launch_bounds__(256) void k_copy_calibrate(const float4* __restrict
 // ============================================================ host launch
 namespace bb {
 #define BB_INSTANTIATE(B)                      \
+  template __global__ void k_team<B>(Dev);     \
   template __global__ void k_build<B>(Dev);    \
   template __global__ void k_broad<B>(Dev);    \
   template __global__ void k_narrow<B>(Dev);   \
@@ -2612,6 +2884,7 @@ namespace bb {
 template <int B>
 static void launch_all(const Dev& d, hipStream_t st) {
   dim3 g(d.W), b(B);
+  if (d.team_grid) hipLaunchKernelGGL(k_team<B>, dim3(d.team_grid), b, 0, st, d);
   hipLaunchKernelGGL(k_build<B>, g, b, 0, st, d);
   hipLaunchKernelGGL(k_broad<B>, g, b, 0, st, d);
   hipLaunchKernelGGL(k_narrow<B>, g, b, 0, st, d);
@@ -2625,6 +2898,7 @@ template <int B>
 static void launch_all_timed(const Dev& d, hipStream_t st, hipEvent_t* ev) {
   dim3 g(d.W), b(B);
   (void)hipEventRecord(ev[0], st);
+  if (d.team_grid) hipLaunchKernelGGL(k_team<B>, dim3(d.team_grid), b, 0, st, d);
   hipLaunchKernelGGL(k_build<B>, g, b, 0, st, d);
   (void)hipEventRecord(ev[1], st);
   hipLaunchKernelGGL(k_broad<B>, g, b, 0, st, d);
@@ -2659,6 +2933,21 @@ void launch_sum_counts(const Dev& d, unsigned long long* out, hipStream_t st) {
 
 void launch_energy(const Dev& d, double* out, hipStream_t st) {
   hipLaunchKernelGGL(k_energy<256>, dim3(d.W), dim3(256), 0, st, d, out);
+}
+
+// Workgroups of k_team<block> that can be resident at once on the device.
+uint32_t team_capacity(uint32_t block) {
+  int dev = 0, cus = 0, per = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+  hipError_t e;
+  switch (block) {
+    case 512: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_team<512>, 512, 0); break;
+    case 1024: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_team<1024>, 1024, 0); break;
+    default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_team<256>, 256, 0); break;
+  }
+  if (e != hipSuccess || per <= 0 || cus <= 0) return 0;
+  return (uint32_t)per * (uint32_t)cus;
 }
 
 void launch_step(const Dev& d, hipStream_t st, uint32_t block) {

@@ -351,8 +351,11 @@ bb_status bb_batch_set_block_size(bb_batch* b, uint32_t threads);
 typedef enum bb_param {
   BB_PARAM_BLOCK = 0,        /* threads per world workgroup: 0 = automatic,
                                 256, 512 or 1024                            */
-  BB_PARAM_PGS_LDS_ROWS = 1  /* rows per world whose velocities the PGS
+  BB_PARAM_PGS_LDS_ROWS = 1, /* rows per world whose velocities the PGS
                                 sweeps keep in LDS (32 B each): 0..2048     */
+  BB_PARAM_TEAM_BODIES = 2   /* worlds with at least this many bodies are
+                                stepped by a team of workgroups (k_team);
+                                0 = every world by one workgroup            */
 } bb_param;
 bb_status bb_batch_set_param(bb_batch* b, bb_param p, uint32_t value);
 /* Effective value (BB_PARAM_BLOCK resolves 0 to the automatic choice). */

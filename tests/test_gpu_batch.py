@@ -203,3 +203,37 @@ def test_autotune_leaves_state_unchanged():
         a, b = tuned.stats(w), plain.stats(w)
         a.pop("step_index"), b.pop("step_index")
         assert a == b
+
+
+def test_team_path_equals_workgroup_path():
+    """Worlds stepped by a team of workgroups (k_team: team barriers, team
+    scans, the multi-workgroup radix sort) give bit-identical trajectories to
+    the one-workgroup path, for PGS_SOFT and AVBD, 30- and 63-bit keys."""
+    pgs63 = beatbox.WorldDesc.make(**scenes.world_desc(iterations=8, morton_bits=63, quirks=0))
+    avbd = beatbox.WorldDesc.make(**scenes.world_desc(solver=scenes.SOLVER_AVBD, iterations=10,
+                                                      morton_bits=63, quirks=0))
+    parity = beatbox.WorldDesc.make(**scenes.world_desc(iterations=8))
+    worlds = [(pgs63, scenes.batch_world(0xBB04, 7, 3000)),
+              (avbd, scenes.serving_pile(21, 2600)),
+              (parity, scenes.cubes_grid()),
+              (pgs63, scenes.batch_world(0xBB04, 2, 64)),
+              (avbd, scenes.serving_pile(22, 700))]
+    runs = []
+    for team in (0, 600):
+        b = beatbox.Batch()
+        for d, bodies in worlds:
+            b.add_world(d, bodies)
+        b.set_param(beatbox.PARAM_TEAM_BODIES, team)
+        b.step(8, use_graph=True)
+        b.sync()
+        runs.append(b)
+    for w, (d, bodies) in enumerate(worlds):
+        x = runs[0].read_world(w, len(bodies))
+        y = runs[1].read_world(w, len(bodies))
+        for f in ("position", "rotation", "velocity", "omega", "flags", "island_index"):
+            assert x[f].tobytes() == y[f].tobytes(), (w, f)
+        sx, sy = runs[0].stats(w), runs[1].stats(w)
+        assert sy["overflow"] == 0, w
+        for k in ("broad_phase_collision_count", "collision_count", "contact_count", "island_count",
+                  "graph_color_count", "avbd_color_count", "dbg_id_sum"):
+            assert sx[k] == sy[k], (w, k)
