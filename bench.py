@@ -106,31 +106,48 @@ def plain_world(steps: int, warmup: int):
     return len(bodies) * steps / dt, dt / steps * 1e3
 
 
-def cpu_baseline(seed: int, budget_s: float):
-    """The CPU oracle (single thread, -O2) on a bounded sample of the batch:
-    the first worlds of the same seeded batch, stepped until the budget."""
+def cpu_baseline(seed: int, budget_s: float, threads: int):
+    """The CPU oracle (C++, -O3, one world per call) on a bounded sample of
+    the batch: the first 32*threads worlds of the same seeded batch, dealt
+    round-robin to `threads` host threads (the foreign calls release the
+    interpreter lock), each stepping its worlds from step 0 until the
+    budget."""
+    import threading
+
     sys.path.insert(0, os.path.join(REPO, "tests"))
     from oracle_bind import OracleWorld
 
     desc = beatbox.WorldDesc.make(**scenes.world_desc(iterations=8, morton_bits=63, quirks=0))
     sizes = scenes.batch_world_sizes(seed, 4096)
-    ws, nb = [], 0
-    for w in range(32):
+    nw = min(4096, 32 * threads)
+    groups = [[] for _ in range(threads)]
+    nb = 0
+    for w in range(nw):
         o = OracleWorld(desc)
         bodies = scenes.batch_world(seed, w, int(sizes[w]))
         o.add_bodies(bodies)
-        ws.append(o)
+        groups[w % threads].append((o, len(bodies)))
         nb += len(bodies)
-    steps = 0
+    done = [0] * threads
+    steps = [0] * threads
     t0 = time.perf_counter()
-    while time.perf_counter() - t0 < budget_s:
-        for o in ws:
-            o.step(1)
-        steps += 1
+
+    def run(k):
+        while time.perf_counter() - t0 < budget_s:
+            for o, n in groups[k]:
+                o.step(1)
+                done[k] += n
+            steps[k] += 1
+
+    ts = [threading.Thread(target=run, args=(k,)) for k in range(threads)]
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join()
     dt = time.perf_counter() - t0
-    return dict(value=nb * steps / dt, unit="body-steps/s", cores=1, kind="port",
-                sample=f"CPU oracle, 1 thread: worlds 0-31 of the seeded 4096-world batch "
-                       f"({nb} bodies) x {steps} steps in {dt:.1f}s")
+    return dict(value=sum(done) / dt, unit="body-steps/s", cores=threads, kind="port",
+                sample=f"CPU oracle, {threads} threads: worlds 0-{nw - 1} of the seeded 4096-world batch "
+                       f"({nb} bodies), {min(steps)}-{max(steps)} steps per thread from step 0, {dt:.1f}s")
 
 
 def main() -> int:
@@ -141,6 +158,7 @@ def main() -> int:
     ap.add_argument("--worlds", type=int, default=4096)
     ap.add_argument("--seed", type=int, default=0xBB04)
     ap.add_argument("--cpu-budget", type=float, default=10.0)
+    ap.add_argument("--cpu-threads", type=int, default=0, help="0: min(16, host cores)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="default launch parameters, no autotuner")
     ap.add_argument("--profile-steps", type=int, default=0, help="0: the timed steps")
@@ -203,7 +221,8 @@ def main() -> int:
     plain_bs, plain_ms = plain_world(args.plain_steps, args.warmup)
     log(f"plain: {plain_ms:.3f} ms/step, {plain_bs / 1e6:.3f} M body-steps/s, ratio {value / plain_bs:.1f}x")
 
-    cpu = None if args.no_cpu else cpu_baseline(args.seed, args.cpu_budget)
+    threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+    cpu = None if args.no_cpu else cpu_baseline(args.seed, args.cpu_budget, threads)
     if cpu:
         log(f"cpu baseline: {cpu['value'] / 1e6:.3f} M body-steps/s")
 
