@@ -1617,61 +1617,106 @@ BB_DI void s_post(const Dev& d, uint32_t w, const T& t, PostShared& L) {
   if (!pgs) return;
   // ---- edge colouring (coloring.slang:44-292).  Phase 2 commits at the
   // colour picked in phase 1; the owner cells touched in a round are cleared
-  // by a third phase so the table is zero between rounds.
+  // by a third phase so the table is zero between rounds.  Only uncoloured
+  // manifolds are visited: they sit in a pending list of records (the solver
+  // row headers are free until the rows are built), {a row | static << 31,
+  // b row | static << 31, pair hash, slot}, compacted after every round.
   uint32_t* color = d.color + W.pair_off;
   uint32_t* pick = d.pick + W.pair_off;
+  int4* rec = d.sv_hdr + W.pair_off;
   for (uint32_t i = tid; i < N; i += NT) d.mask[off + i] = 0;
-  for (uint32_t p = tid; p < P; p += NT) color[p] = d.man_cur.hdr[W.pair_off + p].w > 0 ? kMAX : kMAX - 1;
+  if (tid == 0) L.carry = 0;
   t.sync();
+  for (uint32_t c0 = 0; c0 < P; c0 += NT) {
+    const uint32_t p = c0 + tid;
+    int4 h = make_int4(0, 0, 0, 0);
+    uint32_t f = 0;
+    if (p < P) {
+      h = d.man_cur.hdr[W.pair_off + p];
+      f = h.w > 0 ? 1u : 0u;
+      color[p] = f ? kMAX : kMAX - 1;
+    }
+    uint32_t tot;
+    const uint32_t ex = t.scan(f, tot);
+    if (f) {
+      const uint32_t ga = off + h.x, gb = off + h.y;
+      const uint32_t sa = (d.cur.flags[ga] & BB_BODY_DYNAMIC) ? 0u : 0x80000000u;
+      const uint32_t sb = (d.cur.flags[gb] & BB_BODY_DYNAMIC) ? 0u : 0x80000000u;
+      const uint32_t ida = d.cur.id[ga], idb = d.cur.id[gb];
+      const uint32_t lo = ida < idb ? ida : idb, hi = ida < idb ? idb : ida;
+      const uint32_t pid = gc_hash(lo * 0x9e3779b9u + hi * 0x85ebca6bu + (uint32_t)h.z * 0xc2b2ae35u);
+      rec[L.carry + ex] = make_int4((int)(ga | sa), (int)(gb | sb), (int)pid, (int)p);
+    }
+    t.sync();
+    if (tid == 0) L.carry += tot;
+    t.sync();
+  }
   for (uint32_t round = 1; round <= kColors; ++round) {
+    const uint32_t npend = L.carry;
+    if (npend == 0) break;
     if (tid == 0) L.flag = 0;
     t.sync();
-    for (uint32_t p = tid; p < P; p += NT) {
-      pick[p] = kColors;
-      if (color[p] != kMAX) continue;
-      int4 h = d.man_cur.hdr[W.pair_off + p];
-      const uint32_t ga = off + h.x, gb = off + h.y;
-      bool sa = !(d.cur.flags[ga] & BB_BODY_DYNAMIC), sb = !(d.cur.flags[gb] & BB_BODY_DYNAMIC);
-      uint32_t avail = ~((sa ? 0u : d.mask[ga]) | (sb ? 0u : d.mask[gb]));
-      if (avail == 0) continue;
-      uint32_t cc = (uint32_t)(__ffs(avail) - 1);
-      pick[p] = cc;
-      uint32_t ida = d.cur.id[ga], idb = d.cur.id[gb];
-      uint32_t lo = ida < idb ? ida : idb, hi = ida < idb ? idb : ida;
-      uint32_t pid = gc_hash(lo * 0x9e3779b9u + hi * 0x85ebca6bu + (uint32_t)h.z * 0xc2b2ae35u);
-      uint32_t pr = gc_priority(pid, round);
+    for (uint32_t i = tid; i < npend; i += NT) {
+      const int4 r = rec[i];
+      const uint32_t ga = (uint32_t)r.x & 0x7FFFFFFFu, gb = (uint32_t)r.y & 0x7FFFFFFFu;
+      const bool sa = r.x < 0, sb = r.y < 0;
+      const uint32_t avail = ~((sa ? 0u : d.mask[ga]) | (sb ? 0u : d.mask[gb]));
+      if (avail == 0) {
+        pick[i] = kColors;
+        continue;
+      }
+      const uint32_t cc = (uint32_t)(__ffs(avail) - 1);
+      pick[i] = cc;
+      const uint32_t pr = gc_priority((uint32_t)r.z, round);
       if (!sa) atomicMax(&d.owner[ga * kColors + cc], pr);
       if (!sb) atomicMax(&d.owner[gb * kColors + cc], pr);
       L.flag = 1;
     }
     t.sync();
     const bool any = L.flag != 0;
-    for (uint32_t p = tid; p < P; p += NT) {
-      if (color[p] != kMAX || pick[p] >= kColors) continue;
-      int4 h = d.man_cur.hdr[W.pair_off + p];
-      const uint32_t ga = off + h.x, gb = off + h.y, cc = pick[p];
-      bool sa = !(d.cur.flags[ga] & BB_BODY_DYNAMIC), sb = !(d.cur.flags[gb] & BB_BODY_DYNAMIC);
-      uint32_t ida = d.cur.id[ga], idb = d.cur.id[gb];
-      uint32_t lo = ida < idb ? ida : idb, hi = ida < idb ? idb : ida;
-      uint32_t pid = gc_hash(lo * 0x9e3779b9u + hi * 0x85ebca6bu + (uint32_t)h.z * 0xc2b2ae35u);
-      uint32_t pr = gc_priority(pid, round);
-      bool wa = sa || d.owner[ga * kColors + cc] == pr;
-      bool wb = sb || d.owner[gb * kColors + cc] == pr;
+    for (uint32_t i = tid; i < npend; i += NT) {
+      const uint32_t cc = pick[i];
+      if (cc >= kColors) continue;
+      const int4 r = rec[i];
+      const uint32_t ga = (uint32_t)r.x & 0x7FFFFFFFu, gb = (uint32_t)r.y & 0x7FFFFFFFu;
+      const bool sa = r.x < 0, sb = r.y < 0;
+      const uint32_t pr = gc_priority((uint32_t)r.z, round);
+      const bool wa = sa || d.owner[ga * kColors + cc] == pr;
+      const bool wb = sb || d.owner[gb * kColors + cc] == pr;
       if (wa && wb) {
-        color[p] = cc;
+        color[(uint32_t)r.w] = cc;
         if (!sa) atomicOr(&d.mask[ga], 1u << cc);
         if (!sb) atomicOr(&d.mask[gb], 1u << cc);
       }
     }
     t.sync();
-    for (uint32_t p = tid; p < P; p += NT) {
-      if (pick[p] >= kColors) continue;
-      int4 h = d.man_cur.hdr[W.pair_off + p];
-      d.owner[(off + h.x) * kColors + pick[p]] = 0;
-      d.owner[(off + h.y) * kColors + pick[p]] = 0;
+    for (uint32_t i = tid; i < npend; i += NT) {
+      const uint32_t cc = pick[i];
+      if (cc >= kColors) continue;
+      const int4 r = rec[i];
+      d.owner[((uint32_t)r.x & 0x7FFFFFFFu) * kColors + cc] = 0;
+      d.owner[((uint32_t)r.y & 0x7FFFFFFFu) * kColors + cc] = 0;
     }
+    if (tid == 0) L.carry = 0;
     t.sync();
     if (!any) break;
+    // keep the still uncoloured records (in place: every record is read
+    // before the scan that orders the writes, and moves only to lower slots)
+    for (uint32_t c0 = 0; c0 < npend; c0 += NT) {
+      const uint32_t i = c0 + tid;
+      int4 r = make_int4(0, 0, 0, 0);
+      uint32_t f = 0;
+      if (i < npend) {
+        r = rec[i];
+        f = color[(uint32_t)r.w] == kMAX ? 1u : 0u;
+      }
+      uint32_t tot;
+      const uint32_t ex = t.scan(f, tot);
+      if (f) rec[L.carry + ex] = r;
+      t.sync();
+      if (tid == 0) L.carry += tot;
+      t.sync();
+    }
   }
   // ---- validator (coloring.slang:214-292) and colour buckets
   if (tid < kBuckets) L.bcount[tid] = 0;
