@@ -106,6 +106,66 @@ def plain_world(steps: int, warmup: int):
     return len(bodies) * steps / dt, dt / steps * 1e3
 
 
+def single_world(name: str, desc, bodies, steps: int, warmup: int, profile: int) -> dict:
+    """One large world stepped alone (hipGraph replay of `steps` steps after
+    `warmup`), then `profile` eager steps timed per stage; worlds at or above
+    the team threshold run as workgroup teams (k_team, timed with k_build)."""
+    t0 = time.time()
+    b = beatbox.Batch()
+    b.add_world(desc, bodies)
+    b.step(warmup, use_graph=True)
+    b.sync()
+    build_s = time.time() - t0
+    t0 = time.perf_counter()
+    b.step(steps, use_graph=True)
+    b.sync()
+    dt = time.perf_counter() - t0
+    st, km, cnt = b.profile(profile, counts=True)
+    s = b.stats(0)
+    team = len(bodies) >= b.get_param(beatbox.PARAM_TEAM_BODIES) > 0
+    out = dict(bodies=len(bodies), steps=steps, warmup=warmup, ms_per_step=dt / steps * 1e3,
+               body_steps_per_s=len(bodies) * steps / dt, team=team,
+               profiled_step_ms_p50=float(np.percentile(st, 50)),
+               stages_ms={n: round(float(km[i]) / profile, 3) for i, n in enumerate(beatbox.KERNEL_NAMES)},
+               manifolds=s["collision_count"], contacts=s["contact_count"], pairs=s["broad_phase_collision_count"],
+               max_penetration_mm=int(cnt[:, beatbox.COUNT_NAMES.index("max_pen_mm")].max()),
+               overflow=s["overflow"])
+    b.destroy()
+    log(f"{name}: {out['ms_per_step']:.2f} ms/step, {out['body_steps_per_s'] / 1e6:.2f} M body-steps/s "
+        f"(setup {build_s:.1f}s), {out}")
+    return out
+
+
+def serving(duration: float, budget_gib: float, resident_cap: int, seed: int = 0xBB05) -> dict:
+    """Serving stream: Poisson arrivals at 500 worlds/s for `duration` seconds
+    (random piles of 64-20,000 bodies, AVBD 10 iterations, horizons 60-600
+    steps), admitted by the HBM budget and a resident-body cap, stepped
+    back to back; reports the server's counters and latency percentiles at
+    the end of the arrival window."""
+    desc = beatbox.WorldDesc.make(**scenes.world_desc(solver=scenes.SOLVER_AVBD, iterations=10,
+                                                      morton_bits=63, quirks=0))
+    arr, nb, hz, sd = scenes.serving_requests(seed, duration=duration)
+    srv = beatbox.Server(budget_bytes=int(budget_gib * 2**30), max_worlds=4096,
+                         max_resident_bodies=resident_cap)
+    t0 = time.perf_counter()
+    k = 0
+    while True:
+        now = time.perf_counter() - t0
+        if now >= duration:
+            break
+        while k < len(arr) and arr[k] <= now:
+            srv.submit(desc, scenes.serving_pile(int(sd[k]) & 0xFFFFFFFF, int(nb[k])), int(hz[k]))
+            k += 1
+        srv.step(1)
+    rep = srv.report()
+    wall = time.perf_counter() - t0
+    srv.destroy()
+    rep.update(duration_s=duration, wall_s=wall, arrival_rate=500.0, resident_body_cap=resident_cap,
+               body_steps_per_s=rep["body_steps"] / (rep["device_ms"] * 1e-3) if rep["device_ms"] else 0.0)
+    log(f"serving: {rep}")
+    return rep
+
+
 def cpu_baseline(seed: int, budget_s: float, threads: int):
     """The CPU oracle (C++, -O3, one world per call) on a bounded sample of
     the batch: the first 32*threads worlds of the same seeded batch, dealt
@@ -161,6 +221,10 @@ def main() -> int:
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: min(16, host cores)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="default launch parameters, no autotuner")
+    ap.add_argument("--no-extra", action="store_true", help="skip the large-pile, sphere-gas and serving runs")
+    ap.add_argument("--serving-seconds", type=float, default=10.0)
+    ap.add_argument("--serving-budget-gib", type=float, default=200.0)
+    ap.add_argument("--serving-cap", type=int, default=200_000, help="resident-body cap of the serving run")
     ap.add_argument("--profile-steps", type=int, default=0, help="0: the timed steps")
     ap.add_argument("--plain-steps", type=int, default=200)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -221,6 +285,19 @@ def main() -> int:
     plain_bs, plain_ms = plain_world(args.plain_steps, args.warmup)
     log(f"plain: {plain_ms:.3f} ms/step, {plain_bs / 1e6:.3f} M body-steps/s, ratio {value / plain_bs:.1f}x")
 
+    extra = {}
+    if not args.no_extra:
+        avbd = beatbox.WorldDesc.make(**scenes.world_desc(solver=scenes.SOLVER_AVBD, iterations=10,
+                                                          morton_bits=63, quirks=0))
+        extra["large_pile"] = dict(workload="100,000 bodies (50/50 boxes/spheres, 0.25-1 m) falling in a "
+                                            "60x200x60 m column, AVBD 10 iterations, 63-bit keys",
+                                   **single_world("large pile", avbd, scenes.large_pile(), 300, 1, 5))
+        gas = beatbox.WorldDesc.make(**scenes.world_desc(iterations=8, morton_bits=63, quirks=0, gravity=0.0))
+        extra["sphere_gas"] = dict(workload="1,000,000 spheres (r 0.1-0.5 m) in a 1000 m cube, v ~ N(0, 2 m/s), "
+                                            "no gravity, PGS_SOFT 8 iterations, 63-bit keys",
+                                   **single_world("sphere gas", gas, scenes.sphere_gas(), 100, 1, 5))
+        extra["serving"] = serving(args.serving_seconds, args.serving_budget_gib, args.serving_cap)
+
     threads = args.cpu_threads or min(16, os.cpu_count() or 1)
     cpu = None if args.no_cpu else cpu_baseline(args.seed, args.cpu_budget, threads)
     if cpu:
@@ -257,6 +334,7 @@ def main() -> int:
             "copy_kernel_gbs": copy_gbs,
             "block": params[0],
             "pgs_lds_rows": params[1],
+            **extra,
             "stages": stages,
         },
         "roofline": {
