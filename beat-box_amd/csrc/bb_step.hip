@@ -1316,16 +1316,15 @@ BB_DI uint32_t warm_start(const Dev& d, uint32_t os, bool swapped, int key, int 
   return matched_new;
 }
 
+// Narrow phase, part 1: contact detection.  Writes the slot header {a, b,
+// key, count} (count 0: no manifold), the normal, the body centres and the
+// contacts' positions, penetrations and feature pairs.
 template <class T>
 BB_DI void s_narrow(const Dev& d, uint32_t w, const T& t) {
   const uint32_t tid = t.rank(), NT = t.size();
   const WorldInfo W = d.worlds[w];
   const uint32_t off = W.body_off, P = d.npairs_cur[w];
   const uint2* pairs = d.pairs_cur + W.pair_off;
-  const bool warm = (W.flags & BB_SIM_WARM_STARTING) != 0 && (W.mode & kModeDropPrev) == 0;
-  const uint32_t Pprev = d.npairs_prev[w];
-  const uint2* ppairs = d.pairs_prev + W.pair_off;
-  uint32_t fresh = 0, pen_mm = 0, ccount = 0, mcount = 0;
   for (uint32_t p = tid; p < P; p += NT) {
     const uint32_t slot = W.pair_off + p;
     const uint2 pr = pairs[p];
@@ -1352,12 +1351,47 @@ BB_DI void s_narrow(const Dev& d, uint32_t w, const T& t) {
       d.man_cur.hdr[slot] = make_int4((int)pr.x, (int)pr.y, -1, 0);
       continue;
     }
+    d.man_cur.hdr[slot] = make_int4((int)pr.x, (int)pr.y, key, count);
+    d.man_cur.nrm[slot] = st3(n, 0.0f);
+    d.man_cur.anca[slot] = st3(A.pos, 0.0f);
+    d.man_cur.ancb[slot] = st3(B.pos, 0.0f);
+  }
+}
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_narrow(Dev d) {
+  if (d.worlds[blockIdx.x].mode & kModeTeam) return;
+  const BlockTeam<BLOCK> t{nullptr, nullptr};
+  s_narrow(d, blockIdx.x, t);
+}
+
+// Narrow phase, part 2, per manifold: tangent basis, warm start from the
+// previous step's manifold of the pair, fresh contacts' zero impulses,
+// counters, contact degrees and the union-find link.
+template <class T>
+BB_DI void s_contacts(const Dev& d, uint32_t w, const T& t) {
+  const uint32_t tid = t.rank(), NT = t.size();
+  const WorldInfo W = d.worlds[w];
+  const uint32_t off = W.body_off, P = d.npairs_cur[w];
+  const bool warm = (W.flags & BB_SIM_WARM_STARTING) != 0 && (W.mode & kModeDropPrev) == 0;
+  const uint32_t Pprev = d.npairs_prev[w];
+  const uint2* ppairs = d.pairs_prev + W.pair_off;
+  uint32_t fresh = 0, pen_mm = 0, ccount = 0, mcount = 0;
+  for (uint32_t p = tid; p < P; p += NT) {
+    const uint32_t slot = W.pair_off + p;
+    const int4 h = d.man_cur.hdr[slot];
+    const int count = h.w, key = h.z;
+    if (count <= 0) continue;
+    const uint32_t ga = off + (uint32_t)h.x, gb = off + (uint32_t)h.y;
+    const v3 n = ld3(d.man_cur.nrm[slot]);
     // Manifold::compute_basis (src/shaders/extensions.slang:262-272)
     v3 t0, t1;
     if (fab(n.x) > fab(n.y)) t0 = V(n.z, 0.0f, -n.x);
     else t0 = V(0.0f, -n.z, n.y);
     t0 = vnorm(t0);
     t1 = cross(t0, n);
+    float deep = 0.0f;
+    for (int j = 0; j < count; ++j) deep = fmn(deep, d.man_cur.cpos[slot * 8 + j].w);
     int32_t error = 0x7FFFFFFF;
     uint32_t matched = 0;
     // warm start: the previous step's manifold of this pair, found through
@@ -1372,7 +1406,7 @@ BB_DI void s_narrow(const Dev& d, uint32_t w, const T& t) {
         const uint32_t end = L1 < Pprev ? L1 : Pprev;
         while (L0 < end && ppairs[L0].y < hi) ++L0;
         if (L0 < end && ppairs[L0].y == hi)
-          matched = warm_start(d, W.pair_off + L0, pa != lo, key, count, n, slot, sk.deep, error);
+          matched = warm_start(d, W.pair_off + L0, pa != lo, key, count, n, slot, deep, error);
       }
     }
     // fresh contacts start from zero impulses
@@ -1387,20 +1421,18 @@ BB_DI void s_narrow(const Dev& d, uint32_t w, const T& t) {
       d.man_cur.cws[c] = 0u;
       ++fresh;
     }
-    d.man_cur.hdr[slot] = make_int4((int)pr.x, (int)pr.y, key, count);
     d.man_cur.nrm[slot] = st3(n, __int_as_float(error));
     d.man_cur.t0[slot] = st3(t0, 0.0f);
     d.man_cur.t1[slot] = st3(t1, 0.0f);
-    d.man_cur.anca[slot] = st3(A.pos, 0.0f);
-    d.man_cur.ancb[slot] = st3(B.pos, 0.0f);
-    if (sk.deep < 0.0f) {
-      const uint32_t mm = (uint32_t)fmn(-sk.deep * 1000.0f, 1.0e9f);
+    if (deep < 0.0f) {
+      const uint32_t mm = (uint32_t)fmn(-deep * 1000.0f, 1.0e9f);
       pen_mm = pen_mm > mm ? pen_mm : mm;
     }
     ccount += (uint32_t)count;
     ++mcount;
     atomicAdd(&d.deg[ga], 1u);
     atomicAdd(&d.deg[gb], 1u);
+    const bool da = (d.cur.flags[ga] & BB_BODY_DYNAMIC) != 0, db = (d.cur.flags[gb] & BB_BODY_DYNAMIC) != 0;
     const uint32_t ra = da ? d.props.active_rank[off + ida] : kMAX;
     const uint32_t rb = db ? d.props.active_rank[off + idb] : kMAX;
     if (ra < W.a && rb < W.a) uf_link(d.link + W.act_off, ra, rb);
@@ -1412,10 +1444,10 @@ BB_DI void s_narrow(const Dev& d, uint32_t w, const T& t) {
 }
 
 template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_narrow(Dev d) {
+__global__ __launch_bounds__(BLOCK) void k_contacts(Dev d) {
   if (d.worlds[blockIdx.x].mode & kModeTeam) return;
   const BlockTeam<BLOCK> t{nullptr, nullptr};
-  s_narrow(d, blockIdx.x, t);
+  s_contacts(d, blockIdx.x, t);
 }
 
 // ================================================================= k_post
@@ -2293,8 +2325,71 @@ BB_DI float av_dir_C(v3 dir, v3 anc_a, v3 anc_b, const AvBody& b1, const AvBody&
   return C;
 }
 
-// entry_avbd_primal (avbd.slang:475-658) for row i
-BB_DI void av_primal(const Dev& d, const WorldInfo& W, uint32_t i, float alpha) {
+// entry_avbd_primal (avbd.slang:475-658) for row i, by a group of kAvG
+// lanes (lane = 0..kAvG-1).  The Hessian and gradient are the inertial terms
+// plus one partial sum per incident manifold, added in ascending manifold
+// order (the adjacency order); lane e % kAvG assembles manifold e, so the
+// dependent loads of kAvG manifolds are in flight together.  Only the lower
+// triangle of H is formed: avbd_solve6 reads no other entry.
+static constexpr uint32_t kAvG = 4;
+static constexpr int kAvH = 21;  // lower-triangle entries of the 6x6 H
+BB_DI void av_manifold_terms(const Dev& d, const WorldInfo& W, uint32_t i, uint32_t slot, float alpha,
+                             float (&He)[kAvH], float (&ge)[6]) {
+  const uint32_t off = W.body_off;
+  const int4 h = d.man_cur.hdr[slot];
+  const uint32_t b1 = (uint32_t)h.x, b2 = (uint32_t)h.y;
+  const bool i1 = b1 == i;
+  const AvBody B1 = av_body(d, off + b1), B2 = av_body(d, off + b2);
+  const float f1 = d.props.mass[off + d.cur.id[off + b1]].z, f2 = d.props.mass[off + d.cur.id[off + b2]].z;
+  const float mu = __builtin_sqrtf(fmx(f1, 0.0f) * fmx(f2, 0.0f));
+  const v3 n = ld3(d.man_cur.nrm[slot]);
+  const v3 tdir[2] = {ld3(d.man_cur.t0[slot]), ld3(d.man_cur.t1[slot])};
+#pragma unroll
+  for (int q = 0; q < kAvH; ++q) He[q] = 0.0f;
+#pragma unroll
+  for (int a = 0; a < 6; ++a) ge[a] = 0.0f;
+  auto add_row = [&](float k, float f, const float (&J)[6]) {
+#pragma unroll
+    for (int a = 0; a < 6; ++a) ge[a] += f * J[a];
+    int q = 0;
+#pragma unroll
+    for (int a = 0; a < 6; ++a)
+#pragma unroll
+      for (int b = 0; b <= a; ++b) He[q++] += k * J[a] * J[b];
+  };
+  for (int j = 0; j < h.w; ++j) {
+    const uint32_t cs = slot * 8 + j;
+    const float4 cp = d.man_cur.cpos[cs];
+    const float4 lm = d.man_cur.clam[cs];
+    const float4 cmv = d.man_cur.cmas[cs];
+    const v3 anc_a = ld3(d.man_cur.cana[cs]), anc_b = ld3(d.man_cur.canb[cs]);
+    v3 rr1, rr2;
+    const float C = av_contact_C(n, ld3(cp), cp.w, B1, B2, alpha, rr1, rr2);
+    const float lam = lm.x, kk = lm.w;
+    const float f = fmn(kk * C + lam, 0.0f);
+    const v3 Jl = i1 ? -n : n;
+    const v3 Ja = i1 ? -cross(rr1, n) : cross(rr2, n);
+    const float J[6] = {Jl.x, Jl.y, Jl.z, Ja.x, Ja.y, Ja.z};
+    add_row(kk, f, J);
+    const float bound = mu * fab(lam);
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      const v3 td = tdir[t];
+      v3 ar1, ar2;
+      float C0t;
+      const float Ct = av_dir_C(td, anc_a, anc_b, B1, B2, alpha, ar1, ar2, C0t);
+      const float lt = t == 0 ? lm.y : lm.z;
+      const float kt = t == 0 ? cmv.y : cmv.z;
+      const float ft = fcl(kt * Ct + lt, -bound, bound);
+      const v3 Jlt = i1 ? -td : td;
+      const v3 Jat = i1 ? -cross(ar1, td) : cross(ar2, td);
+      const float Jt[6] = {Jlt.x, Jlt.y, Jlt.z, Jat.x, Jat.y, Jat.z};
+      add_row(kt, ft, Jt);
+    }
+  }
+}
+
+BB_DI void av_primal(const Dev& d, const WorldInfo& W, uint32_t i, float alpha, uint32_t lane) {
   const uint32_t off = W.body_off, g = off + i;
   const uint32_t id = d.cur.id[g];
   const bool quirk = (W.quirks & BB_QUIRK_TRANSPOSED_WORLD_INERTIA) != 0;
@@ -2304,78 +2399,61 @@ BB_DI void av_primal(const Dev& d, const WorldInfo& W, uint32_t i, float alpha) 
   qt rot = ldq(d.cur.rot[g]);
   const float m_lin = d.props.mass[off + id].x * inv_dt2;
   const mat3 Iw = avbd_inverse3(world_inv_inertia(d, off + id, rot, quirk));
-  float H[6][6], gr[6];
+  // inertial terms; lane l keeps entries q = l, l + kAvG, ... of H and g
+  float Hi[kAvH], gi[6];
+  {
+    int q = 0;
 #pragma unroll
-  for (int a = 0; a < 6; ++a) {
-    gr[a] = 0.0f;
+    for (int a = 0; a < 6; ++a)
 #pragma unroll
-    for (int b = 0; b < 6; ++b) H[a][b] = 0.0f;
+      for (int b = 0; b <= a; ++b) {
+        float v = 0.0f;
+        if (a == b && a < 3) v = m_lin;
+        if (a >= 3 && b >= 3) v = Iw.m[a - 3][b - 3] * inv_dt2;
+        Hi[q++] = v;
+      }
+    const v3 dpos = pos - ld3(d.s_post[g]);
+    const v3 dth = small_angle_delta(rot, ldq(d.s_rott[g]));
+    const v3 ga = mulmv(Iw, dth) * inv_dt2;
+    gi[0] = m_lin * dpos.x; gi[1] = m_lin * dpos.y; gi[2] = m_lin * dpos.z;
+    gi[3] = ga.x; gi[4] = ga.y; gi[5] = ga.z;
   }
-  H[0][0] = m_lin; H[1][1] = m_lin; H[2][2] = m_lin;
+  const uint32_t a0 = 2 * W.pair_off + d.adj_off[g], deg = d.deg[g];
+  for (uint32_t e0 = 0; e0 < deg; e0 += kAvG) {
+    float He[kAvH], ge[6];
+    const uint32_t e = e0 + lane;
+    if (e < deg) av_manifold_terms(d, W, i, d.adj[a0 + e], alpha, He, ge);
+    // add the group's partial sums in manifold order
 #pragma unroll
-  for (int a = 0; a < 3; ++a)
+    for (uint32_t src = 0; src < kAvG; ++src) {
+      const bool has = e0 + src < deg;
 #pragma unroll
-    for (int b = 0; b < 3; ++b) H[3 + a][3 + b] = Iw.m[a][b] * inv_dt2;
-  const v3 dpos = pos - ld3(d.s_post[g]);
-  const v3 dth = small_angle_delta(rot, ldq(d.s_rott[g]));
-  gr[0] = m_lin * dpos.x; gr[1] = m_lin * dpos.y; gr[2] = m_lin * dpos.z;
-  const v3 ga = mulmv(Iw, dth) * inv_dt2;
-  gr[3] = ga.x; gr[4] = ga.y; gr[5] = ga.z;
-  const uint32_t a0 = 2 * W.pair_off + d.adj_off[g], a1 = a0 + d.deg[g];
-  for (uint32_t e = a0; e < a1; ++e) {
-    const uint32_t slot = d.adj[e];
-    const int4 h = d.man_cur.hdr[slot];
-    const uint32_t b1 = (uint32_t)h.x, b2 = (uint32_t)h.y;
-    const bool i1 = b1 == i;
-    const AvBody B1 = av_body(d, off + b1), B2 = av_body(d, off + b2);
-    const float f1 = d.props.mass[off + d.cur.id[off + b1]].z, f2 = d.props.mass[off + d.cur.id[off + b2]].z;
-    const float mu = __builtin_sqrtf(fmx(f1, 0.0f) * fmx(f2, 0.0f));
-    const v3 n = ld3(d.man_cur.nrm[slot]);
-    const v3 tdir[2] = {ld3(d.man_cur.t0[slot]), ld3(d.man_cur.t1[slot])};
-    for (int j = 0; j < h.w; ++j) {
-      const uint32_t cs = slot * 8 + j;
-      const float4 cp = d.man_cur.cpos[cs];
-      const float4 lm = d.man_cur.clam[cs];
-      const float4 cmv = d.man_cur.cmas[cs];
-      const v3 anc_a = ld3(d.man_cur.cana[cs]), anc_b = ld3(d.man_cur.canb[cs]);
-      v3 rr1, rr2;
-      float C = av_contact_C(n, ld3(cp), cp.w, B1, B2, alpha, rr1, rr2);
-      float lam = lm.x, kk = lm.w;
-      float f = fmn(kk * C + lam, 0.0f);
-      v3 Jl = i1 ? -n : n;
-      v3 Ja = i1 ? -cross(rr1, n) : cross(rr2, n);
-      gr[0] += f * Jl.x; gr[1] += f * Jl.y; gr[2] += f * Jl.z;
-      gr[3] += f * Ja.x; gr[4] += f * Ja.y; gr[5] += f * Ja.z;
-      float J[6] = {Jl.x, Jl.y, Jl.z, Ja.x, Ja.y, Ja.z};
+      for (int q = 0; q < kAvH; ++q) {
+        const float v = __shfl(He[q], (int)src, (int)kAvG);
+        if (has) Hi[q] += v;
+      }
 #pragma unroll
-      for (int a = 0; a < 6; ++a)
-#pragma unroll
-        for (int b = 0; b < 6; ++b) H[a][b] += kk * J[a] * J[b];
-      const float bound = mu * fab(lam);
-#pragma unroll
-      for (int t = 0; t < 2; ++t) {
-        const v3 td = tdir[t];
-        v3 ar1, ar2;
-        float C0t;
-        float Ct = av_dir_C(td, anc_a, anc_b, B1, B2, alpha, ar1, ar2, C0t);
-        float lt = t == 0 ? lm.y : lm.z;
-        float kt = t == 0 ? cmv.y : cmv.z;
-        float ft = fcl(kt * Ct + lt, -bound, bound);
-        v3 Jlt = i1 ? -td : td;
-        v3 Jat = i1 ? -cross(ar1, td) : cross(ar2, td);
-        gr[0] += ft * Jlt.x; gr[1] += ft * Jlt.y; gr[2] += ft * Jlt.z;
-        gr[3] += ft * Jat.x; gr[4] += ft * Jat.y; gr[5] += ft * Jat.z;
-        float Jt[6] = {Jlt.x, Jlt.y, Jlt.z, Jat.x, Jat.y, Jat.z};
-#pragma unroll
-        for (int a = 0; a < 6; ++a)
-#pragma unroll
-          for (int b = 0; b < 6; ++b) H[a][b] += kt * Jt[a] * Jt[b];
+      for (int a = 0; a < 6; ++a) {
+        const float v = __shfl(ge[a], (int)src, (int)kAvG);
+        if (has) gi[a] += v;
       }
     }
   }
-  float rhs[6], dx[6];
+  if (lane != 0) return;
+  float H[6][6], rhs[6], dx[6];
+  {
+    int q = 0;
 #pragma unroll
-  for (int a = 0; a < 6; ++a) rhs[a] = -gr[a];
+    for (int a = 0; a < 6; ++a)
+#pragma unroll
+      for (int b = 0; b < 6; ++b) H[a][b] = 0.0f;
+#pragma unroll
+    for (int a = 0; a < 6; ++a)
+#pragma unroll
+      for (int b = 0; b <= a; ++b) H[a][b] = Hi[q++];
+  }
+#pragma unroll
+  for (int a = 0; a < 6; ++a) rhs[a] = -gi[a];
   avbd_solve6(H, rhs, dx);
   pos = pos + V(dx[0], dx[1], dx[2]);
   v3 half = V(dx[3], dx[4], dx[5]) * 0.5f;
@@ -2690,8 +2768,8 @@ BB_DI void s_avbd(const Dev& d, uint32_t w, const T& t, AvbdShared& L) {
   // ---- main sweeps: 10 x (colours, dual) (rigid_body_manager.cpp:1366-1373)
   for (uint32_t it = 0; it < AV_ITERS; ++it) {
     for (uint32_t c = 0; c < used; ++c) {
-      for (uint32_t k = L.boff[c * kAvbdLayers] + tid; k < L.boff[(c + 1) * kAvbdLayers]; k += NT)
-        av_primal(d, W, bb[k], 1.0f);
+      for (uint32_t k = L.boff[c * kAvbdLayers] + tid / kAvG; k < L.boff[(c + 1) * kAvbdLayers]; k += NT / kAvG)
+        av_primal(d, W, bb[k], 1.0f, tid % kAvG);
       t.sync();
     }
     for (uint32_t p = tid; p < P; p += NT)
@@ -2728,14 +2806,14 @@ BB_DI void s_avbd(const Dev& d, uint32_t w, const T& t, AvbdShared& L) {
   for (uint32_t layer = 0; layer < kAvbdLayers; ++layer) {
     for (uint32_t c = 0; c < used; ++c) {
       const uint32_t k0 = L.boff[c * kAvbdLayers + layer], k1 = L.boff[c * kAvbdLayers + layer + 1];
-      for (uint32_t k = k0 + tid; k < k1; k += NT) av_primal(d, W, bb[k], 0.0f);
+      for (uint32_t k = k0 + tid / kAvG; k < k1; k += NT / kAvG) av_primal(d, W, bb[k], 0.0f, tid % kAvG);
       t.sync();
     }
   }
   for (uint32_t ps = 1; ps < AV_POST_STAB; ++ps) {
     for (uint32_t c = 0; c < used; ++c) {
-      for (uint32_t k = L.boff[c * kAvbdLayers] + tid; k < L.boff[(c + 1) * kAvbdLayers]; k += NT)
-        av_primal(d, W, bb[k], 0.0f);
+      for (uint32_t k = L.boff[c * kAvbdLayers] + tid / kAvG; k < L.boff[(c + 1) * kAvbdLayers]; k += NT / kAvG)
+        av_primal(d, W, bb[k], 0.0f, tid % kAvG);
       t.sync();
     }
   }
@@ -2811,6 +2889,8 @@ __global__ __launch_bounds__(BLOCK) void k_team(Dev d) {
   s_broad(d, w, t, G.broad);
   t.sync();
   s_narrow(d, w, t);
+  t.sync();
+  s_contacts(d, w, t);
   t.sync();
   s_post(d, w, t, G.post);
   t.sync();
@@ -2921,6 +3001,7 @@ namespace bb {
   template __global__ void k_build<B>(Dev);    \
   template __global__ void k_broad<B>(Dev);    \
   template __global__ void k_narrow<B>(Dev);   \
+  template __global__ void k_contacts<B>(Dev); \
   template __global__ void k_post<B>(Dev);     \
   template __global__ void k_pgs<B>(Dev);      \
   template __global__ void k_avbd<B>(Dev);     \
@@ -2933,6 +3014,7 @@ static void launch_all(const Dev& d, hipStream_t st) {
   hipLaunchKernelGGL(k_build<B>, g, b, 0, st, d);
   hipLaunchKernelGGL(k_broad<B>, g, b, 0, st, d);
   hipLaunchKernelGGL(k_narrow<B>, g, b, 0, st, d);
+  hipLaunchKernelGGL(k_contacts<B>, g, b, 0, st, d);
   hipLaunchKernelGGL(k_post<B>, g, b, 0, st, d);
   hipLaunchKernelGGL(k_pgs<B>, g, b, (size_t)d.pgs_lcap * 32, st, d);
   hipLaunchKernelGGL(k_avbd<B>, g, b, 0, st, d);
@@ -2949,6 +3031,7 @@ static void launch_all_timed(const Dev& d, hipStream_t st, hipEvent_t* ev) {
   hipLaunchKernelGGL(k_broad<B>, g, b, 0, st, d);
   (void)hipEventRecord(ev[2], st);
   hipLaunchKernelGGL(k_narrow<B>, g, b, 0, st, d);
+  hipLaunchKernelGGL(k_contacts<B>, g, b, 0, st, d);
   (void)hipEventRecord(ev[3], st);
   hipLaunchKernelGGL(k_post<B>, g, b, 0, st, d);
   (void)hipEventRecord(ev[4], st);

@@ -57,6 +57,8 @@ def main() -> None:
     ap.add_argument("--cal-write", required=True)
     ap.add_argument("--cal-mib", type=int, default=256)
     ap.add_argument("--out", default=None)
+    ap.add_argument("--json-out", default=None, help="per-kernel figures as JSON (read by bench.py)")
+    ap.add_argument("--command", default="", help="command lines that produced the databases (recorded)")
     a = ap.parse_args()
     dur, meta = trace_rows(a.trace)
     fetch = counter_rows(a.fetch, "FETCH_SIZE")
@@ -69,6 +71,12 @@ def main() -> None:
     ff = statistics.mean(cal_f) / kib if cal_f else float("nan")
     fw = statistics.mean(cal_w) / kib if cal_w else float("nan")
     lines = []
+    if a.command:
+        lines.append("Commands:")
+        lines.append("")
+        for c in a.command.split(";;"):
+            lines.append(f"    {c.strip()}")
+        lines.append("")
     lines.append(f"Copy-kernel calibration ({a.cal_mib} MiB read + {a.cal_mib} MiB written per dispatch): "
                  f"FETCH_SIZE reads {ff:.3f}x, WRITE_SIZE reads {fw:.3f}x the true bytes; the HBM "
                  f"columns below are divided by these factors.")
@@ -76,6 +84,7 @@ def main() -> None:
     lines.append("| kernel | workgroups | calls | avg us | min us | max us | VGPR | AGPR | SGPR | LDS B | "
                  "scratch B | HBM read MB/dispatch | HBM write MB/dispatch | GB/s (read+write / avg) |")
     lines.append("|---|---|---|---|---|---|---|---|---|---|---|---|---|---|")
+    table = {}
     for key in sorted(dur, key=lambda k: -sum(dur[k])):
         ds = dur[key]
         avg = statistics.mean(ds)
@@ -83,11 +92,18 @@ def main() -> None:
         fr = statistics.mean(fetch[key]) / ff / 1024 if fetch.get(key) else None
         wr = statistics.mean(write[key]) / fw / 1024 if write.get(key) else None
         gbs = ((fr or 0) + (wr or 0)) * 1e6 / avg if (fr is not None and avg > 0) else None
+        table[f"{key[0]}@{key[1]}"] = dict(kernel=key[0], workgroups=key[1], calls=len(ds), avg_us=avg / 1e3,
+                                           read_bytes=None if fr is None else fr * 1e6,
+                                           write_bytes=None if wr is None else wr * 1e6)
         lines.append(f"| {key[0]} | {key[1]} | {len(ds)} | {avg / 1e3:.1f} | {min(ds) / 1e3:.1f} | "
                      f"{max(ds) / 1e3:.1f} | {vg} | {ag} | {sg} | {lds} | {scr} | "
                      f"{'' if fr is None else f'{fr:.2f}'} | {'' if wr is None else f'{wr:.2f}'} | "
                      f"{'' if gbs is None else f'{gbs:.0f}'} |")
     text = "\n".join(lines) + "\n"
+    if a.json_out:
+        import json
+        with open(a.json_out, "w") as f:
+            json.dump(dict(fetch_factor=ff, write_factor=fw, kernels=table), f, indent=1)
     if a.out:
         with open(a.out, "w") as f:
             f.write(text)

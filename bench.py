@@ -32,6 +32,7 @@ import beatbox  # noqa: E402
 import scenes  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(REPO, "profiles", "bench_batch4096_traffic.json")
 
 
 def log(msg: str) -> None:
@@ -58,6 +59,21 @@ def stage_bytes(counts: np.ndarray, iters: int, kw: int) -> dict:
         "k_avbd": 0.0,
         "k_finish": N * 16,
     }
+
+
+def profiled_traffic(kernel: str, worlds: int):
+    """HBM bytes per launch (read + write) of `kernel` over the whole batch,
+    from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs of this same
+    command (copy-kernel calibrated), or None when there is none."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            k = json.load(f)["kernels"]
+    except (OSError, ValueError, KeyError):
+        return None
+    for v in k.values():
+        if v["kernel"].startswith(f"bb::{kernel}<") and v["workgroups"] == worlds and v["read_bytes"] is not None:
+            return float(v["read_bytes"]) + float(v["write_bytes"] or 0.0)
+    return None
 
 
 def make_batch(worlds: int, seed: int, lo: int, hi: int):
@@ -344,7 +360,9 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": profiled_traffic(dominant, args.worlds),
+            "traffic_source": "rocprofv3 FETCH_SIZE/WRITE_SIZE of this command, calibrated; "
+                              "profiles/bench_batch4096_kernels.md",
         },
         "cpu_baseline": cpu,
     }

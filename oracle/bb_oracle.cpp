@@ -1616,6 +1616,11 @@ static void do_step(World& w) {
           const AvbdState& s1 = S[b1];
           const AvbdState& s2 = S[b2];
           float mu = std::sqrt(fmaxr(r1.friction, 0.0f) * fmaxr(r2.friction, 0.0f));
+          // this manifold's partial sums, added to H and g afterwards in
+          // adjacency order (the canonical grouping of the sum; the
+          // reference's list order is timing-dependent)
+          float He[6][6], ge[6];
+          for (int a = 0; a < 6; ++a) { ge[a] = 0.0f; for (int b = 0; b < 6; ++b) He[a][b] = 0.0f; }
           for (int j = 0; j < m.count; ++j) {
             f3 rr1, rr2;
             float C = avbd_contact_C(m, j, r1, r2, s1, s2, d1, d2, alpha, rr1, rr2);
@@ -1625,11 +1630,11 @@ static void do_step(World& w) {
             f3 n = m.normal;
             f3 Jl = i1 ? neg(n) : n;
             f3 Ja = i1 ? neg(cross(rr1, n)) : cross(rr2, n);
-            g[0] += f * Jl.x; g[1] += f * Jl.y; g[2] += f * Jl.z;
-            g[3] += f * Ja.x; g[4] += f * Ja.y; g[5] += f * Ja.z;
+            ge[0] += f * Jl.x; ge[1] += f * Jl.y; ge[2] += f * Jl.z;
+            ge[3] += f * Ja.x; ge[4] += f * Ja.y; ge[5] += f * Ja.z;
             float J[6] = {Jl.x, Jl.y, Jl.z, Ja.x, Ja.y, Ja.z};
             for (int a = 0; a < 6; ++a)
-              for (int b = 0; b < 6; ++b) H[a][b] += kk * J[a] * J[b];
+              for (int b = 0; b < 6; ++b) He[a][b] += kk * J[a] * J[b];
             float bound = mu * fabsr(lam);
             for (int t = 0; t < 2; ++t) {
               f3 td = m.t[t];
@@ -1641,12 +1646,16 @@ static void do_step(World& w) {
               float ft = clampf(kt * Ct + lt, -bound, bound);
               f3 Jlt = i1 ? neg(td) : td;
               f3 Jat = i1 ? neg(cross(ar1, td)) : cross(ar2, td);
-              g[0] += ft * Jlt.x; g[1] += ft * Jlt.y; g[2] += ft * Jlt.z;
-              g[3] += ft * Jat.x; g[4] += ft * Jat.y; g[5] += ft * Jat.z;
+              ge[0] += ft * Jlt.x; ge[1] += ft * Jlt.y; ge[2] += ft * Jlt.z;
+              ge[3] += ft * Jat.x; ge[4] += ft * Jat.y; ge[5] += ft * Jat.z;
               float Jt[6] = {Jlt.x, Jlt.y, Jlt.z, Jat.x, Jat.y, Jat.z};
               for (int a = 0; a < 6; ++a)
-                for (int b = 0; b < 6; ++b) H[a][b] += kt * Jt[a] * Jt[b];
+                for (int b = 0; b < 6; ++b) He[a][b] += kt * Jt[a] * Jt[b];
             }
+          }
+          for (int a = 0; a < 6; ++a) {
+            g[a] += ge[a];
+            for (int b = 0; b < 6; ++b) H[a][b] += He[a][b];
           }
         }
         float rhs[6], dx[6];
