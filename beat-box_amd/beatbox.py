@@ -82,6 +82,7 @@ class ServerDesc(C.Structure):
         ("max_bodies", C.c_uint32),
         ("max_resident_bodies", C.c_uint32),
         ("block", C.c_uint32),
+        ("team_bodies", C.c_uint32),
     ]
 
 
@@ -418,9 +419,9 @@ class Server:
     worlds admitted by HBM budget, retired when their steps are done."""
 
     def __init__(self, budget_bytes: int, max_worlds: int = 1024, max_bodies: int = 0,
-                 max_resident_bodies: int = 0, block: int = 0):
+                 max_resident_bodies: int = 0, block: int = 0, team_bodies: int = 0):
         self._h = C.c_void_p()
-        d = ServerDesc(budget_bytes, max_worlds, max_bodies, max_resident_bodies, block)
+        d = ServerDesc(budget_bytes, max_worlds, max_bodies, max_resident_bodies, block, team_bodies)
         _check(lib().bb_server_create(C.byref(d), C.byref(self._h)), "bb_server_create")
 
     def submit(self, desc: WorldDesc, bodies: np.ndarray, steps: int) -> int:

@@ -360,7 +360,7 @@ uint32_t assign_teams(bb_batch* b, uint32_t grid_cap) {
   uint64_t want_total = 0;
   std::vector<uint32_t> want(cand.size());
   for (size_t k = 0; k < cand.size(); ++k) {
-    want[k] = std::max(2u, (b->infos[cand[k]].n + 1023u) / 1024u);
+    want[k] = std::max(2u, (b->infos[cand[k]].n + kTeamBlock - 1u) / kTeamBlock);
     want_total += want[k];
   }
   uint32_t first = 0;
@@ -378,7 +378,7 @@ uint32_t assign_teams(bb_batch* b, uint32_t grid_cap) {
 }
 
 bb_status upload_teams(bb_batch* b) {
-  const uint32_t cap = team_capacity(pick_block(b)) / 2;
+  const uint32_t cap = team_capacity() / 2;
   b->base.team_grid = assign_teams(b, cap);
   HIPCHK(hipMemcpy(b->base.teams, &b->team_table, sizeof(TeamTable), hipMemcpyHostToDevice));
   HIPCHK(hipMemset(b->base.team_ctl, 0, sizeof(TeamCtl) * kMaxTeams));
@@ -463,8 +463,10 @@ Dev dev_for(const bb_batch* b, int cur) {
 
 uint32_t pick_block(const bb_batch* b) {
   if (b->block) return b->block;
+  // worlds that teams step do not count: k_team has its own workgroup size
   uint32_t mx = 0;
-  for (auto& hw : b->worlds) mx = std::max(mx, hw.n);
+  for (auto& hw : b->worlds)
+    if (b->team_bodies == 0 || hw.n < b->team_bodies) mx = std::max(mx, hw.n);
   if (mx >= 8192) return 1024;
   if (mx >= 2048 && b->worlds.size() < 64) return 512;
   return 256;
@@ -639,7 +641,7 @@ bb_status set_param(bb_batch* b, bb_param p, uint32_t value) {
     if (value != 0 && value != 256 && value != 512 && value != 1024) return BB_ERR_INVALID;
     b->block = value;
   } else if (p == BB_PARAM_PGS_LDS_ROWS) {
-    if (value > 2048) return BB_ERR_INVALID;
+    if (value > kMaxPgsLdsRows) return BB_ERR_INVALID;
     b->pgs_lds = value;
     b->base.pgs_lcap = pgs_lds_rows(b);
   } else if (p == BB_PARAM_TEAM_BODIES) {
@@ -680,7 +682,7 @@ bb_status autotune_batch(bb_batch* b, uint32_t reps, float* best_ms) {
   uint32_t mx = 0;
   for (auto& hw : b->worlds) mx = std::max(mx, (uint32_t)hw.bodies.size());
   std::vector<uint32_t> lds_opts;
-  for (uint32_t v : {0u, 256u, 512u, 1024u, 2048u}) {
+  for (uint32_t v : {0u, 256u, 512u, kMaxPgsLdsRows}) {
     lds_opts.push_back(v);
     if (v >= mx) break;
   }

@@ -45,6 +45,8 @@ static constexpr uint32_t kModeDropPrev = 1u, kModeTeam = 2u;
 // generation, an error flag (a barrier wait that ran out of spins), a
 // team-wide flag for the radix sort, and per-workgroup scan partials.
 static constexpr uint32_t kMaxTeamBlocks = 2048;
+// Threads per workgroup of k_team (independent of the per-world kernels').
+static constexpr uint32_t kTeamBlock = 512;
 struct TeamCtl {
   uint32_t count;
   uint32_t gen;

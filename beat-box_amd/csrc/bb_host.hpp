@@ -18,7 +18,7 @@ void launch_step_timed(const Dev& d, hipStream_t st, uint32_t block, hipEvent_t*
 void launch_copy(const float4* src, float4* dst, size_t n, hipStream_t st);
 void launch_sum_counts(const Dev& d, unsigned long long* out, hipStream_t st);
 void launch_energy(const Dev& d, double* out, hipStream_t st);
-uint32_t team_capacity(uint32_t block);
+uint32_t team_capacity();
 }  // namespace bb
 
 #define HIPCHK(x)                                                                  \
@@ -32,6 +32,10 @@ uint32_t team_capacity(uint32_t block);
 
 namespace bb {
 namespace host {
+
+// Rows per world whose body state k_pgs may keep in LDS (80 B each, within
+// 64 KB per workgroup).
+constexpr uint32_t kMaxPgsLdsRows = 768;
 
 struct HostWorld {
   bb_world_desc desc;
@@ -115,7 +119,7 @@ struct bb_batch {
   uint32_t NB = 0, NA = 0, NP = 0;
   hipStream_t stream = nullptr;
   uint32_t block = 0;
-  uint32_t pgs_lds = 1024;  // BB_PARAM_PGS_LDS_ROWS
+  uint32_t pgs_lds = 512;  // BB_PARAM_PGS_LDS_ROWS
   uint32_t team_bodies = 8192;  // BB_PARAM_TEAM_BODIES: worlds this large run in k_team
   bb::TeamTable team_table{};
   uint64_t step_index = 0;

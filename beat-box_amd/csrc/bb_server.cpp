@@ -492,6 +492,7 @@ bb_status bb_server_create(const bb_server_desc* desc, bb_server** out) {
   }
   bb_batch* b = s->b;
   b->block = desc->block ? desc->block : 256;
+  if (desc->team_bodies) b->team_bodies = desc->team_bodies == 0xFFFFFFFFu ? 0u : desc->team_bodies;
   bb_status st = BB_OK;
   if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) st = BB_ERR_DEVICE;
   if (st == BB_OK) st = alloc_arrays(b, b->buf, s->W, s->NB, s->NA, s->NP);
@@ -504,7 +505,7 @@ bb_status bb_server_create(const bb_server_desc* desc, bb_server** out) {
     b->base.pgs_lcap = b->pgs_lds;
     // the k_team launch keeps one shape (half of what can be resident) and
     // the team table in device memory says which workgroups step which world
-    b->base.team_grid = b->team_bodies ? std::min(team_capacity(b->block) / 2, kMaxTeamBlocks) : 0;
+    b->base.team_grid = b->team_bodies ? std::min(team_capacity() / 2, kMaxTeamBlocks) : 0;
     if (hipMemcpy(b->base.worlds, b->infos.data(), s->W * sizeof(WorldInfo), hipMemcpyHostToDevice) != hipSuccess ||
         hipEventCreate(&s->first) != hipSuccess)
       st = BB_ERR_DEVICE;

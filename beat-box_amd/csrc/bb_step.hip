@@ -1959,15 +1959,17 @@ BB_DI void pgs_compact(const Dev& d, const WorldInfo& W, uint32_t k, uint32_t sl
   d.sv_n[k] = st3(n, 0.0f);
 }
 
-// Body velocities during the PGS sweeps: rows below `lcap` live in LDS
-// (vel, omega as two float4 per row), the rest in HBM.
+// Body state during the PGS sweeps: rows below `lcap` keep velocity, angular
+// velocity and the three cached world inverse-inertia rows (inv_mass in
+// row0.w) in LDS, kBodyLds float4 per row; the rest stay in HBM.
+static constexpr uint32_t kBodyLds = 5;
 struct VelCache {
   float4* lds;
   uint32_t off, lcap;
   BB_DI void load(const Dev& d, uint32_t i, v3& v, v3& w) const {
     if (i < lcap) {
-      v = ld3(lds[2 * i]);
-      w = ld3(lds[2 * i + 1]);
+      v = ld3(lds[kBodyLds * i]);
+      w = ld3(lds[kBodyLds * i + 1]);
     } else {
       v = ld3(d.cur.vel[off + i]);
       w = ld3(d.cur.omg[off + i]);
@@ -1975,11 +1977,23 @@ struct VelCache {
   }
   BB_DI void store(const Dev& d, uint32_t i, v3 v, v3 w) const {
     if (i < lcap) {
-      lds[2 * i] = st3(v, 0.0f);
-      lds[2 * i + 1] = st3(w, 0.0f);
+      lds[kBodyLds * i] = st3(v, 0.0f);
+      lds[kBodyLds * i + 1] = st3(w, 0.0f);
     } else {
       d.cur.vel[off + i] = st3(v, 0.0f);
       d.cur.omg[off + i] = st3(w, 0.0f);
+    }
+  }
+  BB_DI void load_iw(const Dev& d, uint32_t i, float4& r0, float4& r1, float4& r2) const {
+    if (i < lcap) {
+      r0 = lds[kBodyLds * i + 2];
+      r1 = lds[kBodyLds * i + 3];
+      r2 = lds[kBodyLds * i + 4];
+    } else {
+      const uint32_t g = off + i;
+      r0 = d.iw[g * 3];
+      r1 = d.iw[g * 3 + 1];
+      r2 = d.iw[g * 3 + 2];
     }
   }
 };
@@ -1991,10 +2005,12 @@ BB_DI void pgs_row(const Dev& d, const VelCache& vc, uint32_t off, uint32_t k, c
   const int count = h.z & 255;
   const bool da = (h.z & 256) != 0, db = (h.z & 512) != 0;
   const uint32_t ga = off + h.x, gb = off + h.y;
-  const float4 ia0 = d.iw[ga * 3], ib0 = d.iw[gb * 3];
+  float4 ia0, ia1, ia2, ib0, ib1, ib2;
+  vc.load_iw(d, (uint32_t)h.x, ia0, ia1, ia2);
+  vc.load_iw(d, (uint32_t)h.y, ib0, ib1, ib2);
   const float ima = ia0.w, imb = ib0.w;
-  const mat3 Ia = mrows(ld3(ia0), ld3(d.iw[ga * 3 + 1]), ld3(d.iw[ga * 3 + 2]));
-  const mat3 Ib = mrows(ld3(ib0), ld3(d.iw[gb * 3 + 1]), ld3(d.iw[gb * 3 + 2]));
+  const mat3 Ia = mrows(ld3(ia0), ld3(ia1), ld3(ia2));
+  const mat3 Ib = mrows(ld3(ib0), ld3(ib1), ld3(ib2));
   v3 va, wa, vb, wb;
   vc.load(d, (uint32_t)h.x, va, wa);
   vc.load(d, (uint32_t)h.y, vb, wb);
@@ -2128,8 +2144,12 @@ BB_DI void s_pgs(const Dev& d, uint32_t w, const T& t, float4* pgs_lds, uint32_t
   const uint32_t ovf_lo = bo[kColors], ovf_hi = bo[kBuckets];
   const VelCache vc{pgs_lds, off, N < lcap ? N : lcap};
   for (uint32_t i = tid; i < vc.lcap; i += NT) {
-    pgs_lds[2 * i] = d.cur.vel[off + i];
-    pgs_lds[2 * i + 1] = d.cur.omg[off + i];
+    const uint32_t g = off + i;
+    pgs_lds[kBodyLds * i] = d.cur.vel[g];
+    pgs_lds[kBodyLds * i + 1] = d.cur.omg[g];
+    pgs_lds[kBodyLds * i + 2] = d.iw[g * 3];
+    pgs_lds[kBodyLds * i + 3] = d.iw[g * 3 + 1];
+    pgs_lds[kBodyLds * i + 4] = d.iw[g * 3 + 2];
   }
   t.sync();
   auto sweep = [&](int kind) {
@@ -2166,10 +2186,17 @@ BB_DI void s_pgs(const Dev& d, uint32_t w, const T& t, float4* pgs_lds, uint32_t
   t.sync();
   iw_refresh(d, W, t);
   t.sync();
+  for (uint32_t i = tid; i < vc.lcap; i += NT) {
+    const uint32_t g = off + i;
+    pgs_lds[kBodyLds * i + 2] = d.iw[g * 3];
+    pgs_lds[kBodyLds * i + 3] = d.iw[g * 3 + 1];
+    pgs_lds[kBodyLds * i + 4] = d.iw[g * 3 + 2];
+  }
+  t.sync();
   for (uint32_t it = 0; it < W.iterations; ++it) sweep(2);
   for (uint32_t i = tid; i < vc.lcap; i += NT) {
-    d.cur.vel[off + i] = pgs_lds[2 * i];
-    d.cur.omg[off + i] = pgs_lds[2 * i + 1];
+    d.cur.vel[off + i] = pgs_lds[kBodyLds * i];
+    d.cur.omg[off + i] = pgs_lds[kBodyLds * i + 1];
   }
   // impulses and pre-step quantities back into the manifold slots (warm start
   // of the next step, taps)
@@ -2806,6 +2833,7 @@ BB_DI void s_avbd(const Dev& d, uint32_t w, const T& t, AvbdShared& L) {
   for (uint32_t layer = 0; layer < kAvbdLayers; ++layer) {
     for (uint32_t c = 0; c < used; ++c) {
       const uint32_t k0 = L.boff[c * kAvbdLayers + layer], k1 = L.boff[c * kAvbdLayers + layer + 1];
+      if (k1 == k0) continue;  // team-uniform: an empty bucket needs no barrier
       for (uint32_t k = k0 + tid / kAvG; k < k1; k += NT / kAvG) av_primal(d, W, bb[k], 0.0f, tid % kAvG);
       t.sync();
     }
@@ -2862,7 +2890,13 @@ __global__ __launch_bounds__(BLOCK) void k_finish(Dev d) {
 
 // ================================================================ k_team
 // Large worlds: every stage of one step for each team, with team barriers in
-// place of the kernel boundaries of the per-world path.
+// place of the kernel boundaries of the per-world path.  Teams always use
+// kTeamBlock-thread workgroups, whatever the per-world workgroup size: the
+// 1,024-thread build of this kernel is held to 128 VGPRs and spills heavily
+// (about 2,400 scratch accesses), and it faulted the GPU on a 20,000-body
+// AVBD world whose emulated run (every workgroup a host thread, every
+// work-item a fiber, under AddressSanitizer) stays in bounds; that build is
+// not used.
 union TeamShared {
   BuildShared build;
   BroadShared broad;
@@ -2996,8 +3030,8 @@ __global__ __launch_bounds__(256) void k_copy_calibrate(const float4* __restrict
 
 // ============================================================ host launch
 namespace bb {
+template __global__ void k_team<kTeamBlock>(Dev);
 #define BB_INSTANTIATE(B)                      \
-  template __global__ void k_team<B>(Dev);     \
   template __global__ void k_build<B>(Dev);    \
   template __global__ void k_broad<B>(Dev);    \
   template __global__ void k_narrow<B>(Dev);   \
@@ -3010,13 +3044,13 @@ namespace bb {
 template <int B>
 static void launch_all(const Dev& d, hipStream_t st) {
   dim3 g(d.W), b(B);
-  if (d.team_grid) hipLaunchKernelGGL(k_team<B>, dim3(d.team_grid), b, 0, st, d);
+  if (d.team_grid) hipLaunchKernelGGL(k_team<kTeamBlock>, dim3(d.team_grid), dim3(kTeamBlock), 0, st, d);
   hipLaunchKernelGGL(k_build<B>, g, b, 0, st, d);
   hipLaunchKernelGGL(k_broad<B>, g, b, 0, st, d);
   hipLaunchKernelGGL(k_narrow<B>, g, b, 0, st, d);
   hipLaunchKernelGGL(k_contacts<B>, g, b, 0, st, d);
   hipLaunchKernelGGL(k_post<B>, g, b, 0, st, d);
-  hipLaunchKernelGGL(k_pgs<B>, g, b, (size_t)d.pgs_lcap * 32, st, d);
+  hipLaunchKernelGGL(k_pgs<B>, g, b, (size_t)d.pgs_lcap * 16 * kBodyLds, st, d);
   hipLaunchKernelGGL(k_avbd<B>, g, b, 0, st, d);
   hipLaunchKernelGGL(k_finish<B>, g, b, 0, st, d);
 }
@@ -3025,7 +3059,7 @@ template <int B>
 static void launch_all_timed(const Dev& d, hipStream_t st, hipEvent_t* ev) {
   dim3 g(d.W), b(B);
   (void)hipEventRecord(ev[0], st);
-  if (d.team_grid) hipLaunchKernelGGL(k_team<B>, dim3(d.team_grid), b, 0, st, d);
+  if (d.team_grid) hipLaunchKernelGGL(k_team<kTeamBlock>, dim3(d.team_grid), dim3(kTeamBlock), 0, st, d);
   hipLaunchKernelGGL(k_build<B>, g, b, 0, st, d);
   (void)hipEventRecord(ev[1], st);
   hipLaunchKernelGGL(k_broad<B>, g, b, 0, st, d);
@@ -3035,7 +3069,7 @@ static void launch_all_timed(const Dev& d, hipStream_t st, hipEvent_t* ev) {
   (void)hipEventRecord(ev[3], st);
   hipLaunchKernelGGL(k_post<B>, g, b, 0, st, d);
   (void)hipEventRecord(ev[4], st);
-  hipLaunchKernelGGL(k_pgs<B>, g, b, (size_t)d.pgs_lcap * 32, st, d);
+  hipLaunchKernelGGL(k_pgs<B>, g, b, (size_t)d.pgs_lcap * 16 * kBodyLds, st, d);
   (void)hipEventRecord(ev[5], st);
   hipLaunchKernelGGL(k_avbd<B>, g, b, 0, st, d);
   (void)hipEventRecord(ev[6], st);
@@ -3063,17 +3097,12 @@ void launch_energy(const Dev& d, double* out, hipStream_t st) {
   hipLaunchKernelGGL(k_energy<256>, dim3(d.W), dim3(256), 0, st, d, out);
 }
 
-// Workgroups of k_team<block> that can be resident at once on the device.
-uint32_t team_capacity(uint32_t block) {
+// Workgroups of k_team that can be resident at once on the device.
+uint32_t team_capacity() {
   int dev = 0, cus = 0, per = 0;
   if (hipGetDevice(&dev) != hipSuccess) return 0;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
-  hipError_t e;
-  switch (block) {
-    case 512: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_team<512>, 512, 0); break;
-    case 1024: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_team<1024>, 1024, 0); break;
-    default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_team<256>, 256, 0); break;
-  }
+  const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_team<kTeamBlock>, kTeamBlock, 0);
   if (e != hipSuccess || per <= 0 || cus <= 0) return 0;
   return (uint32_t)per * (uint32_t)cus;
 }

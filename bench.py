@@ -152,7 +152,8 @@ def single_world(name: str, desc, bodies, steps: int, warmup: int, profile: int)
     return out
 
 
-def serving(duration: float, budget_gib: float, resident_cap: int, seed: int = 0xBB05) -> dict:
+def serving(duration: float, budget_gib: float, resident_cap: int, team_bodies: int,
+            seed: int = 0xBB05) -> dict:
     """Serving stream: Poisson arrivals at 500 worlds/s for `duration` seconds
     (random piles of 64-20,000 bodies, AVBD 10 iterations, horizons 60-600
     steps), admitted by the HBM budget and a resident-body cap, stepped
@@ -162,7 +163,7 @@ def serving(duration: float, budget_gib: float, resident_cap: int, seed: int = 0
                                                       morton_bits=63, quirks=0))
     arr, nb, hz, sd = scenes.serving_requests(seed, duration=duration)
     srv = beatbox.Server(budget_bytes=int(budget_gib * 2**30), max_worlds=4096,
-                         max_resident_bodies=resident_cap)
+                         max_resident_bodies=resident_cap, team_bodies=team_bodies)
     t0 = time.perf_counter()
     k = 0
     while True:
@@ -177,6 +178,7 @@ def serving(duration: float, budget_gib: float, resident_cap: int, seed: int = 0
     wall = time.perf_counter() - t0
     srv.destroy()
     rep.update(duration_s=duration, wall_s=wall, arrival_rate=500.0, resident_body_cap=resident_cap,
+               team_bodies=team_bodies,
                body_steps_per_s=rep["body_steps"] / (rep["device_ms"] * 1e-3) if rep["device_ms"] else 0.0)
     log(f"serving: {rep}")
     return rep
@@ -241,6 +243,8 @@ def main() -> int:
     ap.add_argument("--serving-seconds", type=float, default=10.0)
     ap.add_argument("--serving-budget-gib", type=float, default=200.0)
     ap.add_argument("--serving-cap", type=int, default=200_000, help="resident-body cap of the serving run")
+    ap.add_argument("--serving-team-bodies", type=int, default=2048,
+                    help="worlds this large are stepped as workgroup teams in the serving run")
     ap.add_argument("--profile-steps", type=int, default=0, help="0: the timed steps")
     ap.add_argument("--plain-steps", type=int, default=200)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -312,7 +316,8 @@ def main() -> int:
         extra["sphere_gas"] = dict(workload="1,000,000 spheres (r 0.1-0.5 m) in a 1000 m cube, v ~ N(0, 2 m/s), "
                                             "no gravity, PGS_SOFT 8 iterations, 63-bit keys",
                                    **single_world("sphere gas", gas, scenes.sphere_gas(), 100, 1, 5))
-        extra["serving"] = serving(args.serving_seconds, args.serving_budget_gib, args.serving_cap)
+        extra["serving"] = serving(args.serving_seconds, args.serving_budget_gib, args.serving_cap,
+                                   args.serving_team_bodies)
 
     threads = args.cpu_threads or min(16, os.cpu_count() or 1)
     cpu = None if args.no_cpu else cpu_baseline(args.seed, args.cpu_budget, threads)

@@ -351,8 +351,9 @@ bb_status bb_batch_set_block_size(bb_batch* b, uint32_t threads);
 typedef enum bb_param {
   BB_PARAM_BLOCK = 0,        /* threads per world workgroup: 0 = automatic,
                                 256, 512 or 1024                            */
-  BB_PARAM_PGS_LDS_ROWS = 1, /* rows per world whose velocities the PGS
-                                sweeps keep in LDS (32 B each): 0..2048     */
+  BB_PARAM_PGS_LDS_ROWS = 1, /* rows per world whose velocities and world
+                                inverse inertia the PGS sweeps keep in LDS
+                                (80 B each): 0..768                         */
   BB_PARAM_TEAM_BODIES = 2   /* worlds with at least this many bodies are
                                 stepped by a team of workgroups (k_team);
                                 0 = every world by one workgroup            */
@@ -387,6 +388,8 @@ typedef struct bb_server_desc {
   uint32_t max_resident_bodies; /* 0 = no cap; else admit while the resident
                                    bodies stay within it (latency bound)      */
   uint32_t block;               /* threads per world workgroup, 0 = 256       */
+  uint32_t team_bodies;         /* worlds this large run as workgroup teams;
+                                   0 = 8192, 0xFFFFFFFF = never              */
 } bb_server_desc;
 
 typedef enum bb_request_state {
