@@ -2219,7 +2219,7 @@ BB_DI void s_pgs(const Dev& d, uint32_t w, const T& t, float4* pgs_lds, uint32_t
 
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_pgs(Dev d) {
-  extern __shared__ float4 pgs_lds[];
+  HIP_DYNAMIC_SHARED(float4, pgs_lds)
   if (d.worlds[blockIdx.x].mode & kModeTeam) return;
   const BlockTeam<BLOCK> t{nullptr, nullptr};
   s_pgs(d, blockIdx.x, t, pgs_lds, d.pgs_lcap);

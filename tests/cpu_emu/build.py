@@ -1,0 +1,43 @@
+"""Builds the CPU emulation of the step (tests/cpu_emu/_build/emu_main).
+
+The kernels of beat-box_amd/csrc/bb_step.hip and the host code of the C-ABI
+(bb_capi.cpp, bb_server.cpp) are compiled unchanged by g++, against the HIP
+emulation header in tests/cpu_emu/hip/, with AddressSanitizer.  emu_main
+(main.cpp) steps the worlds of a scene file through the C-ABI.
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+from concurrent.futures import ThreadPoolExecutor
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(os.path.dirname(HERE))
+CSRC = os.path.join(REPO, "beat-box_amd", "csrc")
+OUT_DIR = os.path.join(HERE, "_build")
+FLAGS = ["-std=c++17", "-O1", "-g", "-fno-omit-frame-pointer", "-fsanitize=address",
+         "-ffp-contract=off", "-w", "-I", HERE]
+UNITS = [(os.path.join(CSRC, "bb_step.hip"), True), (os.path.join(CSRC, "bb_capi.cpp"), False),
+         (os.path.join(CSRC, "bb_server.cpp"), False), (os.path.join(HERE, "emu.cpp"), False),
+         (os.path.join(HERE, "main.cpp"), False)]
+
+
+def build() -> str:
+    os.makedirs(OUT_DIR, exist_ok=True)
+    exe = os.path.join(OUT_DIR, "emu_main")
+
+    def compile_one(unit):
+        src, as_cxx = unit
+        obj = os.path.join(OUT_DIR, os.path.basename(src) + ".o")
+        cmd = ["g++", *FLAGS, "-c"] + (["-x", "c++"] if as_cxx else []) + [src, "-o", obj]
+        subprocess.run(cmd, check=True)
+        return obj
+
+    with ThreadPoolExecutor(max_workers=len(UNITS)) as ex:
+        objs = list(ex.map(compile_one, UNITS))
+    subprocess.run(["g++", *FLAGS, *objs, "-o", exe, "-lpthread"], check=True)
+    return exe
+
+
+if __name__ == "__main__":
+    print(build())
