@@ -121,6 +121,7 @@ bb_status alloc_arrays(bb_batch* b, Buffers& A, uint32_t W, uint32_t NB, uint32_
   d = Dev{};
   d.W = W;
   d.worlds = A.alloc<WorldInfo>(W);
+  d.order = A.alloc<uint32_t>(W);
   for (int s = 0; s < 2; ++s) {
     Rows& R = b->rows[s];
     R.id = A.alloc<uint32_t>(NB);
@@ -377,12 +378,22 @@ uint32_t assign_teams(bb_batch* b, uint32_t grid_cap) {
   return first;
 }
 
+void world_order(const std::vector<WorldInfo>& infos, uint32_t* out) {
+  const uint32_t W = (uint32_t)infos.size();
+  for (uint32_t w = 0; w < W; ++w) out[w] = w;
+  auto key = [&](uint32_t w) { return (infos[w].mode & kModeTeam) ? 0u : infos[w].n; };
+  std::stable_sort(out, out + W, [&](uint32_t x, uint32_t y) { return key(x) > key(y); });
+}
+
 bb_status upload_teams(bb_batch* b) {
   const uint32_t cap = team_capacity() / 2;
   b->base.team_grid = assign_teams(b, cap);
   HIPCHK(hipMemcpy(b->base.teams, &b->team_table, sizeof(TeamTable), hipMemcpyHostToDevice));
   HIPCHK(hipMemset(b->base.team_ctl, 0, sizeof(TeamCtl) * kMaxTeams));
   HIPCHK(hipMemcpy(b->base.worlds, b->infos.data(), b->infos.size() * sizeof(WorldInfo), hipMemcpyHostToDevice));
+  std::vector<uint32_t> order(b->infos.size());
+  world_order(b->infos, order.data());
+  HIPCHK(hipMemcpy((void*)b->base.order, order.data(), order.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
   return BB_OK;
 }
 

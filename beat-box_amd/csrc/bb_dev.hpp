@@ -172,6 +172,9 @@ struct Dev {
   uint32_t* team_hist;
   char* team_shared;
   uint32_t team_grid;     // workgroups of the k_team launch
+  // [W] world stepped by workgroup b of the per-world kernels: worlds in
+  // descending body count, so the longest workgroups start first
+  const uint32_t* order;
 };
 
 }  // namespace bb

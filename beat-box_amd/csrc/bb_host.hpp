@@ -165,6 +165,9 @@ uint32_t pick_block(const bb_batch* b);
 // resident at once), sets their kModeTeam bit in b->infos and fills
 // b->team_table.  Returns the workgroups of the k_team launch (0: none).
 uint32_t assign_teams(bb_batch* b, uint32_t grid_cap);
+// Workgroup-to-world order of the per-world kernels: descending body count
+// (ties by index), team worlds last (their workgroups exit at once).
+void world_order(const std::vector<WorldInfo>& infos, uint32_t* out);
 Dev dev_for(const bb_batch* b, int cur);
 
 }  // namespace host

@@ -324,7 +324,7 @@ bb_status server_step(bb_server* s) {
       stage_bytes += (size_t)sl.n * (4 * 3 + 16 * 4) + ST_COUNT * 4 + 128;
     }
   }
-  stage_bytes += sizeof(WorldInfo) * (size_t)s->W + sizeof(TeamTable) + 64;
+  stage_bytes += (sizeof(WorldInfo) + sizeof(uint32_t)) * (size_t)s->W + sizeof(TeamTable) + 96;
   bb_status st = ch.reserve(stage_bytes + 4096);
   if (st != BB_OK) return st;
   ch.used = 0;
@@ -359,6 +359,9 @@ bb_status server_step(bb_server* s) {
     WorldInfo* tab = (WorldInfo*)ch.take(sizeof(WorldInfo) * s->W);
     std::memcpy(tab, b->infos.data(), sizeof(WorldInfo) * s->W);
     HIPCHK(hipMemcpyAsync(b->base.worlds, tab, sizeof(WorldInfo) * s->W, hipMemcpyHostToDevice, stream));
+    uint32_t* ord = (uint32_t*)ch.take(sizeof(uint32_t) * s->W);
+    world_order(b->infos, ord);
+    HIPCHK(hipMemcpyAsync((void*)b->base.order, ord, sizeof(uint32_t) * s->W, hipMemcpyHostToDevice, stream));
     s->table_dirty = false;
   }
   StepRecord rec;
@@ -506,7 +509,10 @@ bb_status bb_server_create(const bb_server_desc* desc, bb_server** out) {
     // the k_team launch keeps one shape (half of what can be resident) and
     // the team table in device memory says which workgroups step which world
     b->base.team_grid = b->team_bodies ? std::min(team_capacity() / 2, kMaxTeamBlocks) : 0;
+    std::vector<uint32_t> order(s->W);
+    world_order(b->infos, order.data());
     if (hipMemcpy(b->base.worlds, b->infos.data(), s->W * sizeof(WorldInfo), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy((void*)b->base.order, order.data(), s->W * sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess ||
         hipEventCreate(&s->first) != hipSuccess)
       st = BB_ERR_DEVICE;
     b->latest = 1;

@@ -26,6 +26,8 @@
 namespace bb {
 
 #define WAVE 64
+// Lanes per body in the AVBD primal sweeps of the per-world kernels.
+static constexpr uint32_t kAvG = 4;
 
 // ============================================================ block helpers
 template <int BLOCK>
@@ -235,6 +237,9 @@ template <int BLOCK>
 struct BlockTeam {
   ScanLds<BLOCK>* scan_lds;
   SortLds<BLOCK>* sort_lds;
+  // AVBD primal: kAvG lanes per body; empty cascade buckets skip their barrier
+  static constexpr uint32_t kAvGroup = kAvG;
+  static constexpr bool kSkipEmpty = true;
   BB_DI uint32_t rank() const { return threadIdx.x; }
   BB_DI uint32_t size() const { return BLOCK; }
   BB_DI void sync() const { __syncthreads(); }
@@ -257,6 +262,9 @@ struct GroupTeam {
   TeamCtl* ctl;
   uint32_t* hist;  // [nblocks * 256] this team's radix histograms / offsets
   uint32_t blk, nblocks;
+  // AVBD primal: one lane per body, a barrier after every cascade bucket
+  static constexpr uint32_t kAvGroup = 1;
+  static constexpr bool kSkipEmpty = false;
   BB_DI uint32_t rank() const { return blk * BLOCK + threadIdx.x; }
   BB_DI uint32_t size() const { return nblocks * BLOCK; }
   BB_DI void fence() const { __threadfence(); }
@@ -582,9 +590,10 @@ __global__ __launch_bounds__(BLOCK) void k_build(Dev d) {
   __shared__ BuildShared L;
   __shared__ ScanLds<BLOCK> S;
   __shared__ SortLds<BLOCK> Q;
-  if (d.worlds[blockIdx.x].mode & kModeTeam) return;
+  const uint32_t w = d.order[blockIdx.x];
+  if (d.worlds[w].mode & kModeTeam) return;
   const BlockTeam<BLOCK> t{&S, &Q};
-  s_build(d, blockIdx.x, t, L);
+  s_build(d, w, t, L);
 }
 
 // ================================================================ k_broad
@@ -737,9 +746,10 @@ template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_broad(Dev d) {
   __shared__ BroadShared L;
   __shared__ ScanLds<BLOCK> S;
-  if (d.worlds[blockIdx.x].mode & kModeTeam) return;
+  const uint32_t w = d.order[blockIdx.x];
+  if (d.worlds[w].mode & kModeTeam) return;
   const BlockTeam<BLOCK> t{&S, nullptr};
-  s_broad(d, blockIdx.x, t, L);
+  s_broad(d, w, t, L);
 }
 
 // ==================================================== narrow phase (SAT)
@@ -1360,9 +1370,10 @@ BB_DI void s_narrow(const Dev& d, uint32_t w, const T& t) {
 
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_narrow(Dev d) {
-  if (d.worlds[blockIdx.x].mode & kModeTeam) return;
+  const uint32_t w = d.order[blockIdx.x];
+  if (d.worlds[w].mode & kModeTeam) return;
   const BlockTeam<BLOCK> t{nullptr, nullptr};
-  s_narrow(d, blockIdx.x, t);
+  s_narrow(d, w, t);
 }
 
 // Narrow phase, part 2, per manifold: tangent basis, warm start from the
@@ -1445,9 +1456,10 @@ BB_DI void s_contacts(const Dev& d, uint32_t w, const T& t) {
 
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_contacts(Dev d) {
-  if (d.worlds[blockIdx.x].mode & kModeTeam) return;
+  const uint32_t w = d.order[blockIdx.x];
+  if (d.worlds[w].mode & kModeTeam) return;
   const BlockTeam<BLOCK> t{nullptr, nullptr};
-  s_contacts(d, blockIdx.x, t);
+  s_contacts(d, w, t);
 }
 
 // ================================================================= k_post
@@ -1843,9 +1855,10 @@ template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_post(Dev d) {
   __shared__ PostShared L;
   __shared__ ScanLds<BLOCK> S;
-  if (d.worlds[blockIdx.x].mode & kModeTeam) return;
+  const uint32_t w = d.order[blockIdx.x];
+  if (d.worlds[w].mode & kModeTeam) return;
   const BlockTeam<BLOCK> t{&S, nullptr};
-  s_post(d, blockIdx.x, t, L);
+  s_post(d, w, t, L);
 }
 
 // ================================================================== k_pgs
@@ -2220,9 +2233,10 @@ BB_DI void s_pgs(const Dev& d, uint32_t w, const T& t, float4* pgs_lds, uint32_t
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_pgs(Dev d) {
   HIP_DYNAMIC_SHARED(float4, pgs_lds)
-  if (d.worlds[blockIdx.x].mode & kModeTeam) return;
+  const uint32_t w = d.order[blockIdx.x];
+  if (d.worlds[w].mode & kModeTeam) return;
   const BlockTeam<BLOCK> t{nullptr, nullptr};
-  s_pgs(d, blockIdx.x, t, pgs_lds, d.pgs_lcap);
+  s_pgs(d, w, t, pgs_lds, d.pgs_lcap);
 }
 
 // ================================================================== AVBD
@@ -2352,13 +2366,13 @@ BB_DI float av_dir_C(v3 dir, v3 anc_a, v3 anc_b, const AvBody& b1, const AvBody&
   return C;
 }
 
-// entry_avbd_primal (avbd.slang:475-658) for row i, by a group of kAvG
-// lanes (lane = 0..kAvG-1).  The Hessian and gradient are the inertial terms
-// plus one partial sum per incident manifold, added in ascending manifold
-// order (the adjacency order); lane e % kAvG assembles manifold e, so the
-// dependent loads of kAvG manifolds are in flight together.  Only the lower
-// triangle of H is formed: avbd_solve6 reads no other entry.
-static constexpr uint32_t kAvG = 4;
+// entry_avbd_primal (avbd.slang:475-658) for row i, by a group of G lanes
+// (lane = 0..G-1; G = kAvG in the per-world kernels, 1 in teams).  The
+// Hessian and gradient are the inertial terms plus one partial sum per
+// incident manifold, added in ascending manifold order (the adjacency
+// order); lane e % G assembles manifold e, so the dependent loads of G
+// manifolds are in flight together.  Only the lower triangle of H is formed:
+// avbd_solve6 reads no other entry.
 static constexpr int kAvH = 21;  // lower-triangle entries of the 6x6 H
 BB_DI void av_manifold_terms(const Dev& d, const WorldInfo& W, uint32_t i, uint32_t slot, float alpha,
                              float (&He)[kAvH], float (&ge)[6]) {
@@ -2416,6 +2430,7 @@ BB_DI void av_manifold_terms(const Dev& d, const WorldInfo& W, uint32_t i, uint3
   }
 }
 
+template <uint32_t G>
 BB_DI void av_primal(const Dev& d, const WorldInfo& W, uint32_t i, float alpha, uint32_t lane) {
   const uint32_t off = W.body_off, g = off + i;
   const uint32_t id = d.cur.id[g];
@@ -2426,7 +2441,7 @@ BB_DI void av_primal(const Dev& d, const WorldInfo& W, uint32_t i, float alpha, 
   qt rot = ldq(d.cur.rot[g]);
   const float m_lin = d.props.mass[off + id].x * inv_dt2;
   const mat3 Iw = avbd_inverse3(world_inv_inertia(d, off + id, rot, quirk));
-  // inertial terms; lane l keeps entries q = l, l + kAvG, ... of H and g
+  // inertial terms (every lane of the group holds all of H and g)
   float Hi[kAvH], gi[6];
   {
     int q = 0;
@@ -2446,23 +2461,30 @@ BB_DI void av_primal(const Dev& d, const WorldInfo& W, uint32_t i, float alpha, 
     gi[3] = ga.x; gi[4] = ga.y; gi[5] = ga.z;
   }
   const uint32_t a0 = 2 * W.pair_off + d.adj_off[g], deg = d.deg[g];
-  for (uint32_t e0 = 0; e0 < deg; e0 += kAvG) {
+  for (uint32_t e0 = 0; e0 < deg; e0 += G) {
     float He[kAvH], ge[6];
     const uint32_t e = e0 + lane;
     if (e < deg) av_manifold_terms(d, W, i, d.adj[a0 + e], alpha, He, ge);
-    // add the group's partial sums in manifold order
+    if constexpr (G == 1) {
 #pragma unroll
-    for (uint32_t src = 0; src < kAvG; ++src) {
-      const bool has = e0 + src < deg;
+      for (int q = 0; q < kAvH; ++q) Hi[q] += He[q];
 #pragma unroll
-      for (int q = 0; q < kAvH; ++q) {
-        const float v = __shfl(He[q], (int)src, (int)kAvG);
-        if (has) Hi[q] += v;
-      }
+      for (int a = 0; a < 6; ++a) gi[a] += ge[a];
+    } else {
+      // add the group's partial sums in manifold order
 #pragma unroll
-      for (int a = 0; a < 6; ++a) {
-        const float v = __shfl(ge[a], (int)src, (int)kAvG);
-        if (has) gi[a] += v;
+      for (uint32_t src = 0; src < G; ++src) {
+        const bool has = e0 + src < deg;
+#pragma unroll
+        for (int q = 0; q < kAvH; ++q) {
+          const float v = __shfl(He[q], (int)src, (int)G);
+          if (has) Hi[q] += v;
+        }
+#pragma unroll
+        for (int a = 0; a < 6; ++a) {
+          const float v = __shfl(ge[a], (int)src, (int)G);
+          if (has) gi[a] += v;
+        }
       }
     }
   }
@@ -2550,6 +2572,7 @@ struct AvbdShared {
 template <class T>
 BB_DI void s_avbd(const Dev& d, uint32_t w, const T& t, AvbdShared& L) {
   const uint32_t tid = t.rank(), NT = t.size();
+  constexpr uint32_t G = T::kAvGroup;  // lanes per body in the primal sweeps
   const WorldInfo W = d.worlds[w];
   if (W.solver != BB_SOLVER_AVBD || W.n == 0) return;
   const uint32_t N = W.n, off = W.body_off, P = d.npairs_cur[w];
@@ -2795,8 +2818,8 @@ BB_DI void s_avbd(const Dev& d, uint32_t w, const T& t, AvbdShared& L) {
   // ---- main sweeps: 10 x (colours, dual) (rigid_body_manager.cpp:1366-1373)
   for (uint32_t it = 0; it < AV_ITERS; ++it) {
     for (uint32_t c = 0; c < used; ++c) {
-      for (uint32_t k = L.boff[c * kAvbdLayers] + tid / kAvG; k < L.boff[(c + 1) * kAvbdLayers]; k += NT / kAvG)
-        av_primal(d, W, bb[k], 1.0f, tid % kAvG);
+      for (uint32_t k = L.boff[c * kAvbdLayers] + tid / G; k < L.boff[(c + 1) * kAvbdLayers]; k += NT / G)
+        av_primal<G>(d, W, bb[k], 1.0f, tid % G);
       t.sync();
     }
     for (uint32_t p = tid; p < P; p += NT)
@@ -2833,15 +2856,15 @@ BB_DI void s_avbd(const Dev& d, uint32_t w, const T& t, AvbdShared& L) {
   for (uint32_t layer = 0; layer < kAvbdLayers; ++layer) {
     for (uint32_t c = 0; c < used; ++c) {
       const uint32_t k0 = L.boff[c * kAvbdLayers + layer], k1 = L.boff[c * kAvbdLayers + layer + 1];
-      if (k1 == k0) continue;  // team-uniform: an empty bucket needs no barrier
-      for (uint32_t k = k0 + tid / kAvG; k < k1; k += NT / kAvG) av_primal(d, W, bb[k], 0.0f, tid % kAvG);
+      if (T::kSkipEmpty && k1 == k0) continue;  // an empty bucket needs no barrier
+      for (uint32_t k = k0 + tid / G; k < k1; k += NT / G) av_primal<G>(d, W, bb[k], 0.0f, tid % G);
       t.sync();
     }
   }
   for (uint32_t ps = 1; ps < AV_POST_STAB; ++ps) {
     for (uint32_t c = 0; c < used; ++c) {
-      for (uint32_t k = L.boff[c * kAvbdLayers] + tid / kAvG; k < L.boff[(c + 1) * kAvbdLayers]; k += NT / kAvG)
-        av_primal(d, W, bb[k], 0.0f, tid % kAvG);
+      for (uint32_t k = L.boff[c * kAvbdLayers] + tid / G; k < L.boff[(c + 1) * kAvbdLayers]; k += NT / G)
+        av_primal<G>(d, W, bb[k], 0.0f, tid % G);
       t.sync();
     }
   }
@@ -2851,9 +2874,10 @@ template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_avbd(Dev d) {
   __shared__ AvbdShared L;
   __shared__ ScanLds<BLOCK> S;
-  if (d.worlds[blockIdx.x].mode & kModeTeam) return;
+  const uint32_t w = d.order[blockIdx.x];
+  if (d.worlds[w].mode & kModeTeam) return;
   const BlockTeam<BLOCK> t{&S, nullptr};
-  s_avbd(d, blockIdx.x, t, L);
+  s_avbd(d, w, t, L);
 }
 
 // ============================================================== k_finish
@@ -2883,9 +2907,10 @@ BB_DI void s_finish(const Dev& d, uint32_t w, const T& t) {
 
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_finish(Dev d) {
-  if (d.worlds[blockIdx.x].mode & kModeTeam) return;
+  const uint32_t w = d.order[blockIdx.x];
+  if (d.worlds[w].mode & kModeTeam) return;
   const BlockTeam<BLOCK> t{nullptr, nullptr};
-  s_finish(d, blockIdx.x, t);
+  s_finish(d, w, t);
 }
 
 // ================================================================ k_team
