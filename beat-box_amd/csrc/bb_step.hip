@@ -2011,13 +2011,32 @@ struct VelCache {
   }
 };
 
+// Contacts per row whose solver data pgs_row loads up front.
+static constexpr int kPgsPre = 4;
 // kind 0: warm-start impulses of pre_step; 1: solve with bias; 2: relax.
 BB_DI void pgs_row(const Dev& d, const VelCache& vc, uint32_t off, uint32_t k, const Pgs& s, int kind) {
   const uint32_t NP = d.np;
   const int4 h = d.sv_hdr[k];
   const int count = h.z & 255;
+  if (kind == 0 && !s.accum) return;
   const bool da = (h.z & 256) != 0, db = (h.z & 512) != 0;
-  const uint32_t ga = off + h.x, gb = off + h.y;
+  // the first kPgsPre contacts' solver data are loaded together, before the
+  // body state: a row costs two dependent global round trips (header, then
+  // contacts) instead of one per contact
+  float4 pr1[kPgsPre], pr2[kPgsPre], pl[kPgsPre];
+  float ptm[kPgsPre];
+#pragma unroll
+  for (int c = 0; c < kPgsPre; ++c) {
+    pr1[c] = pr2[c] = pl[c] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    ptm[c] = 0.0f;
+    if (c < count) {
+      pr1[c] = d.sv_r1[c * NP + k];
+      pr2[c] = d.sv_r2[c * NP + k];
+      pl[c] = d.sv_l[c * NP + k];
+      if (kind != 0) ptm[c] = d.sv_tm1[c * NP + k];
+    }
+  }
+  const v3 n = ld3(d.sv_n[k]);
   float4 ia0, ia1, ia2, ib0, ib1, ib2;
   vc.load_iw(d, (uint32_t)h.x, ia0, ia1, ia2);
   vc.load_iw(d, (uint32_t)h.y, ib0, ib1, ib2);
@@ -2027,14 +2046,11 @@ BB_DI void pgs_row(const Dev& d, const VelCache& vc, uint32_t off, uint32_t k, c
   v3 va, wa, vb, wb;
   vc.load(d, (uint32_t)h.x, va, wa);
   vc.load(d, (uint32_t)h.y, vb, wb);
-  const v3 n = ld3(d.sv_n[k]);
   v3 t0, t1;
   contact_basis(n, t0, t1);
   if (kind == 0) {
-    if (!s.accum) return;
-    for (int i = 0; i < count; ++i) {
-      const float4 l = d.sv_l[i * NP + k];
-      const v3 r1 = ld3(d.sv_r1[i * NP + k]), r2 = ld3(d.sv_r2[i * NP + k]);
+    auto warm = [&](float4 l, float4 c1, float4 c2) {
+      const v3 r1 = ld3(c1), r2 = ld3(c2);
       v3 P = n * l.x;
       if (s.friction) {
         P = P + t0 * l.y;
@@ -2048,16 +2064,17 @@ BB_DI void pgs_row(const Dev& d, const VelCache& vc, uint32_t off, uint32_t k, c
         vb = vb + P * imb;
         wb = wb + mulmv(Ib, cross(r2, P));
       }
-    }
+    };
+#pragma unroll
+    for (int c = 0; c < kPgsPre; ++c)
+      if (c < count) warm(pl[c], pr1[c], pr2[c]);
+    for (int c = kPgsPre; c < count; ++c) warm(d.sv_l[c * NP + k], d.sv_r1[c * NP + k], d.sv_r2[c * NP + k]);
   } else {
     const bool use_bias = kind == 1;
     const float fr = __int_as_float(h.w);
     const Soft sc = (h.z & 1024) ? s.soft[1] : s.soft[0];
     const float bias_factor = sc.bias_factor, impulse_coef = sc.impulse_coef, mass_coef = sc.mass_coef;
-    for (int c = 0; c < count; ++c) {
-      const float4 c1 = d.sv_r1[c * NP + k], c2 = d.sv_r2[c * NP + k];
-      float4 l = d.sv_l[c * NP + k];
-      const float tm1 = d.sv_tm1[c * NP + k];
+    auto solve = [&](int c, float4 c1, float4 c2, float4 l, float tm1) {
       const v3 r1 = ld3(c1), r2 = ld3(c2);
       const float pen = c1.w, nm = c2.w;
       v3 rv = ((vb + cross(wb, r2)) - va) - cross(wa, r1);
@@ -2116,7 +2133,12 @@ BB_DI void pgs_row(const Dev& d, const VelCache& vc, uint32_t off, uint32_t k, c
         wb = wb + mulmv(Ib, cross(r2, niv));
       }
       d.sv_l[c * NP + k] = l;
-    }
+    };
+#pragma unroll
+    for (int c = 0; c < kPgsPre; ++c)
+      if (c < count) solve(c, pr1[c], pr2[c], pl[c], ptm[c]);
+    for (int c = kPgsPre; c < count; ++c)
+      solve(c, d.sv_r1[c * NP + k], d.sv_r2[c * NP + k], d.sv_l[c * NP + k], d.sv_tm1[c * NP + k]);
   }
   if (da) vc.store(d, (uint32_t)h.x, va, wa);
   if (db) vc.store(d, (uint32_t)h.y, vb, wb);

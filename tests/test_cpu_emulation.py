@@ -80,3 +80,34 @@ def test_large_workgroups_in_bounds(emu_main, tmp_path):
     a, x = _run(emu_main, scene, str(tmp_path / "a.bin"), 4, 0, 1024, graph=0)
     b, y = _run(emu_main, scene, str(tmp_path / "b.bin"), 4, 0, 256, graph=0)
     assert a == b and x == y
+
+
+def test_emulated_step_equals_oracle(emu_main, tmp_path):
+    """The emulated kernels reproduce the oracle bit for bit (poses,
+    velocities, flags) on a PGS_SOFT pile, an AVBD pile and a parity-mode
+    world stepped together in one batch, so changes to the kernels can be
+    checked against the oracle on the host."""
+    from oracle_bind import OracleWorld
+
+    avbd = beatbox.WorldDesc.make(**scenes.world_desc(solver=scenes.SOLVER_AVBD, iterations=10,
+                                                      morton_bits=63, quirks=0))
+    pgs = beatbox.WorldDesc.make(**scenes.world_desc(iterations=8, morton_bits=63, quirks=0))
+    parity = beatbox.WorldDesc.make(**scenes.world_desc(iterations=8))
+    worlds = [(pgs, scenes.serving_pile(31, 300)), (avbd, scenes.serving_pile(32, 250)),
+              (parity, scenes.random_boxes(0xBB12, 96, extent=3.0, y0=0.55, y1=5.0))]
+    scene = str(tmp_path / "scene.bin")
+    _scene(scene, worlds)
+    steps = 8
+    _, raw = _run(emu_main, scene, str(tmp_path / "o.bin"), steps, 0, 0)
+    got = np.frombuffer(raw, dtype=beatbox.BODY_STATE)
+    at = 0
+    for d, bodies in worlds:
+        o = OracleWorld(d)
+        o.add_bodies(bodies)
+        o.step(steps)
+        ref = o.read_bodies()
+        mine = got[at:at + len(bodies)]
+        at += len(bodies)
+        for f in ("position", "rotation", "velocity", "omega", "flags", "id"):
+            assert mine[f].tobytes() == ref[f].tobytes(), f
+    assert at == len(got)
