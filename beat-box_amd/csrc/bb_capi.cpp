@@ -172,6 +172,8 @@ bb_status alloc_arrays(bb_batch* b, Buffers& A, uint32_t W, uint32_t NB, uint32_
   d.poff = A.alloc<uint32_t>(NB);
   d.link = A.alloc<uint32_t>(NA);
   d.deg = A.alloc<uint32_t>(NB);
+  d.row_prev = A.alloc<uint32_t>(NB);
+  d.row_rank = A.alloc<uint32_t>(NB);
   d.island = A.alloc<uint32_t>(NB);
   d.islot = A.alloc<uint32_t>(NB);
   d.ascratch = A.alloc<uint32_t>(NA);

@@ -132,6 +132,8 @@ struct Dev {
   // islands / sleep / colouring
   uint32_t* link;     // [active slot]
   uint32_t* deg;      // [body] manifolds per row
+  uint32_t* row_prev; // [body] previous step's row of this row's body (entries_prev by id)
+  uint32_t* row_rank; // [body] active rank of this row's body (kMAX: not dynamic)
   uint32_t* island;   // [body] island root (tap)
   uint32_t* islot;    // [body] island slot (ascending root order)
   uint32_t* ascratch; // [active slot] per-island scratch

@@ -582,6 +582,11 @@ BB_DI void s_build(const Dev& d, uint32_t w, const T& t, BuildShared& L) {
     d.cur.omg[dst] = d.prev.omg[src];
     d.cur.pvel[dst] = d.prev.pvel[src];
     d.entries_cur[off + id] = i;
+    // per-row copies of the id-indexed lookups the narrow phase and the
+    // islands make per manifold (one gather per body here instead of two per
+    // manifold there)
+    d.row_prev[dst] = d.entries_prev[off + id];
+    d.row_rank[dst] = d.props.active_rank[off + id];
   }
 }
 
@@ -1327,7 +1332,8 @@ BB_DI uint32_t warm_start(const Dev& d, uint32_t os, bool swapped, int key, int 
 }
 
 // Narrow phase, part 1: contact detection.  Writes the slot header {a, b,
-// key, count} (count 0: no manifold), the normal, the body centres and the
+// key, count} (count 0: no manifold), the normal, the body centres (the
+// first one's w holds the deepest penetration, min(0, pen)) and the
 // contacts' positions, penetrations and feature pairs.
 template <class T>
 BB_DI void s_narrow(const Dev& d, uint32_t w, const T& t) {
@@ -1363,7 +1369,7 @@ BB_DI void s_narrow(const Dev& d, uint32_t w, const T& t) {
     }
     d.man_cur.hdr[slot] = make_int4((int)pr.x, (int)pr.y, key, count);
     d.man_cur.nrm[slot] = st3(n, 0.0f);
-    d.man_cur.anca[slot] = st3(A.pos, 0.0f);
+    d.man_cur.anca[slot] = st3(A.pos, sk.deep);  // w: deepest penetration, min(0, pen)
     d.man_cur.ancb[slot] = st3(B.pos, 0.0f);
   }
 }
@@ -1401,15 +1407,13 @@ BB_DI void s_contacts(const Dev& d, uint32_t w, const T& t) {
     else t0 = V(0.0f, -n.z, n.y);
     t0 = vnorm(t0);
     t1 = cross(t0, n);
-    float deep = 0.0f;
-    for (int j = 0; j < count; ++j) deep = fmn(deep, d.man_cur.cpos[slot * 8 + j].w);
+    const float deep = d.man_cur.anca[slot].w;
     int32_t error = 0x7FFFFFFF;
     uint32_t matched = 0;
     // warm start: the previous step's manifold of this pair, found through
     // the previous pair list's per-row offsets (lists ascend by partner)
-    const uint32_t ida = d.cur.id[ga], idb = d.cur.id[gb];
     if (warm && Pprev > 0) {
-      const uint32_t pa = d.entries_prev[off + ida], pb = d.entries_prev[off + idb];
+      const uint32_t pa = d.row_prev[ga], pb = d.row_prev[gb];
       if (pa < W.prev_n && pb < W.prev_n) {
         const uint32_t lo = pa < pb ? pa : pb, hi = pa < pb ? pb : pa;
         uint32_t L0 = d.pstart_prev[off + lo];
@@ -1444,8 +1448,8 @@ BB_DI void s_contacts(const Dev& d, uint32_t w, const T& t) {
     atomicAdd(&d.deg[ga], 1u);
     atomicAdd(&d.deg[gb], 1u);
     const bool da = (d.cur.flags[ga] & BB_BODY_DYNAMIC) != 0, db = (d.cur.flags[gb] & BB_BODY_DYNAMIC) != 0;
-    const uint32_t ra = da ? d.props.active_rank[off + ida] : kMAX;
-    const uint32_t rb = db ? d.props.active_rank[off + idb] : kMAX;
+    const uint32_t ra = da ? d.row_rank[ga] : kMAX;
+    const uint32_t rb = db ? d.row_rank[gb] : kMAX;
     if (ra < W.a && rb < W.a) uf_link(d.link + W.act_off, ra, rb);
   }
   stat_add(d, w, ST_FRESH, fresh);
@@ -1572,8 +1576,8 @@ BB_DI void s_post(const Dev& d, uint32_t w, const T& t, PostShared& L) {
   for (uint32_t p = tid; p < P; p += NT) {
     int4 h = d.man_cur.hdr[W.pair_off + p];
     if (h.w <= 0) continue;
-    uint32_t ia = d.props.active_rank[off + d.cur.id[off + h.x]];
-    uint32_t ib = d.props.active_rank[off + d.cur.id[off + h.y]];
+    uint32_t ia = d.row_rank[off + h.x];
+    uint32_t ib = d.row_rank[off + h.y];
     uint32_t mn = ia < ib ? ia : ib;
     if (mn < A) d.pcount[off + d.ascratch[W.act_off + mn]] = 1;
   }
@@ -1626,10 +1630,8 @@ BB_DI void s_post(const Dev& d, uint32_t w, const T& t, PostShared& L) {
       bool bloud = bdyn && ((d.cur.sleep[gb] & 0x7FFFFFFFu) < 30u);
       int32_t err = __float_as_int(d.man_cur.nrm[slot].w);
       bool stalled = err != 0x7FFFFFFF && (err < 0 ? -err : err) < 10;
-      bool deep = false;
-      if (!stalled)
-        for (int ci = 0; ci < h.w; ++ci)
-          if (d.man_cur.cpos[slot * 8 + ci].w < -0.05f) { deep = true; break; }
+      // a contact deeper than 5 cm (the narrow phase keeps min(0, pen) in anca.w)
+      const bool deep = !stalled && d.man_cur.anca[slot].w < -0.05f;
       if ((aloud || deep) && bdyn) atomicOr(&d.cur.sleep[gb], 0x80000000u);
       if ((bloud || deep) && adyn) atomicOr(&d.cur.sleep[ga], 0x80000000u);
     }
