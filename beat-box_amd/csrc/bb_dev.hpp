@@ -179,4 +179,11 @@ struct Dev {
   const uint32_t* order;
 };
 
+// One launch lane of the per-world kernels: `grid` workgroups of `block`
+// threads stepping worlds d.order[0..grid).
+struct Lane {
+  Dev d;
+  uint32_t block, grid;
+};
+
 }  // namespace bb

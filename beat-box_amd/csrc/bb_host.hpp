@@ -13,8 +13,11 @@
 #include "bb_dev.hpp"
 
 namespace bb {
-void launch_step(const Dev& d, hipStream_t st, uint32_t block);
-void launch_step_timed(const Dev& d, hipStream_t st, uint32_t block, hipEvent_t* ev);
+// One step of a lane: k_team first when `team`, then the per-world stages.
+void launch_step(const Lane& l, hipStream_t st, bool team);
+// One step of up to two lanes on one stream, stage by stage (both lanes'
+// kernels of a stage back to back), with events ev[0..7] around the stages.
+void launch_step_timed(const Lane* lanes, int nl, hipStream_t st, hipEvent_t* ev);
 void launch_copy(const float4* src, float4* dst, size_t n, hipStream_t st);
 void launch_sum_counts(const Dev& d, unsigned long long* out, hipStream_t st);
 void launch_energy(const Dev& d, double* out, hipStream_t st);
@@ -121,6 +124,13 @@ struct bb_batch {
   uint32_t block = 0;
   uint32_t pgs_lds = 512;  // BB_PARAM_PGS_LDS_ROWS
   uint32_t team_bodies = 8192;  // BB_PARAM_TEAM_BODIES: worlds this large run in k_team
+  // BB_PARAM_LANE_BODIES / _LANE_BLOCK: worlds this large (0: none) run the
+  // per-world kernels with lane_block threads on a second stream
+  uint32_t lane_bodies = 512;
+  uint32_t lane_block = 512;
+  uint32_t n_wide = 0;  // worlds in that lane: the first n_wide of the order table
+  hipStream_t stream2 = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   bb::TeamTable team_table{};
   uint64_t step_index = 0;
   std::vector<bb::WorldInfo> infos;
@@ -160,6 +170,9 @@ uint32_t pgs_lds_rows(const bb_batch* b);
 // struct.
 void stats_from_raw(const uint32_t* raw, uint64_t step_index, bb_step_stats* out);
 uint32_t pick_block(const bb_batch* b);
+// The lanes of one step over the device view d: the wide lane (when any world
+// is in it) and the rest.  Returns the lane count (1 or 2).
+int lanes_of(const bb_batch* b, const Dev& d, Lane* out);
 // Chooses the worlds that run as workgroup teams in k_team (at least
 // team_bodies bodies, workgroups in proportion to size within what can be
 // resident at once), sets their kModeTeam bit in b->infos and fills

@@ -242,7 +242,8 @@ bb_status ensure_graphs(bb_server* s) {
     if (s->graph[p]) continue;
     hipGraph_t g;
     HIPCHK(hipStreamBeginCapture(s->b->stream, hipStreamCaptureModeThreadLocal));
-    launch_step(dev_for(s->b, p), s->b->stream, s->b->block);
+    const Dev d = dev_for(s->b, p);
+    launch_step(Lane{d, s->b->block, d.W}, s->b->stream, true);
     HIPCHK(hipStreamEndCapture(s->b->stream, &g));
     HIPCHK(hipGraphInstantiate(&s->graph[p], g, nullptr, nullptr, 0));
     HIPCHK(hipGraphDestroy(g));

@@ -2255,7 +2255,7 @@ BB_DI void s_pgs(const Dev& d, uint32_t w, const T& t, float4* pgs_lds, uint32_t
 }
 
 template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_pgs(Dev d) {
+__global__ __launch_bounds__(BLOCK, 4) void k_pgs(Dev d) {
   HIP_DYNAMIC_SHARED(float4, pgs_lds)
   const uint32_t w = d.order[blockIdx.x];
   if (d.worlds[w].mode & kModeTeam) return;
@@ -3090,47 +3090,51 @@ template __global__ void k_team<kTeamBlock>(Dev);
   template __global__ void k_avbd<B>(Dev);     \
   template __global__ void k_finish<B>(Dev);
 
+// Stages of the per-world path, in launch order (k_team, when there is one,
+// goes before the first).  Stage k of the timed launch is bracketed by events
+// ev[k] and ev[k + 1]; the narrow phase's two kernels share stage 2.
 template <int B>
-static void launch_all(const Dev& d, hipStream_t st) {
-  dim3 g(d.W), b(B);
-  if (d.team_grid) hipLaunchKernelGGL(k_team<kTeamBlock>, dim3(d.team_grid), dim3(kTeamBlock), 0, st, d);
-  hipLaunchKernelGGL(k_build<B>, g, b, 0, st, d);
-  hipLaunchKernelGGL(k_broad<B>, g, b, 0, st, d);
-  hipLaunchKernelGGL(k_narrow<B>, g, b, 0, st, d);
-  hipLaunchKernelGGL(k_contacts<B>, g, b, 0, st, d);
-  hipLaunchKernelGGL(k_post<B>, g, b, 0, st, d);
-  hipLaunchKernelGGL(k_pgs<B>, g, b, (size_t)d.pgs_lcap * 16 * kBodyLds, st, d);
-  hipLaunchKernelGGL(k_avbd<B>, g, b, 0, st, d);
-  hipLaunchKernelGGL(k_finish<B>, g, b, 0, st, d);
+static void launch_stage(int stage, const Dev& d, hipStream_t st, uint32_t grid) {
+  const dim3 g(grid), b(B);
+  switch (stage) {
+    case 0: hipLaunchKernelGGL(k_build<B>, g, b, 0, st, d); break;
+    case 1: hipLaunchKernelGGL(k_broad<B>, g, b, 0, st, d); break;
+    case 2:
+      hipLaunchKernelGGL(k_narrow<B>, g, b, 0, st, d);
+      hipLaunchKernelGGL(k_contacts<B>, g, b, 0, st, d);
+      break;
+    case 3: hipLaunchKernelGGL(k_post<B>, g, b, 0, st, d); break;
+    case 4: hipLaunchKernelGGL(k_pgs<B>, g, b, (size_t)d.pgs_lcap * 16 * kBodyLds, st, d); break;
+    case 5: hipLaunchKernelGGL(k_avbd<B>, g, b, 0, st, d); break;
+    default: hipLaunchKernelGGL(k_finish<B>, g, b, 0, st, d); break;
+  }
+}
+static constexpr int kStages = 7;
+
+static void launch_stage(int stage, const Lane& l, hipStream_t st) {
+  if (l.grid == 0) return;
+  switch (l.block) {
+    case 512: launch_stage<512>(stage, l.d, st, l.grid); break;
+    case 1024: launch_stage<1024>(stage, l.d, st, l.grid); break;
+    default: launch_stage<256>(stage, l.d, st, l.grid); break;
+  }
 }
 
-template <int B>
-static void launch_all_timed(const Dev& d, hipStream_t st, hipEvent_t* ev) {
-  dim3 g(d.W), b(B);
+static void launch_team(const Dev& d, hipStream_t st) {
+  if (d.team_grid) hipLaunchKernelGGL(k_team<kTeamBlock>, dim3(d.team_grid), dim3(kTeamBlock), 0, st, d);
+}
+
+void launch_step(const Lane& l, hipStream_t st, bool team) {
+  if (team) launch_team(l.d, st);
+  for (int k = 0; k < kStages; ++k) launch_stage(k, l, st);
+}
+
+void launch_step_timed(const Lane* lanes, int nl, hipStream_t st, hipEvent_t* ev) {
   (void)hipEventRecord(ev[0], st);
-  if (d.team_grid) hipLaunchKernelGGL(k_team<kTeamBlock>, dim3(d.team_grid), dim3(kTeamBlock), 0, st, d);
-  hipLaunchKernelGGL(k_build<B>, g, b, 0, st, d);
-  (void)hipEventRecord(ev[1], st);
-  hipLaunchKernelGGL(k_broad<B>, g, b, 0, st, d);
-  (void)hipEventRecord(ev[2], st);
-  hipLaunchKernelGGL(k_narrow<B>, g, b, 0, st, d);
-  hipLaunchKernelGGL(k_contacts<B>, g, b, 0, st, d);
-  (void)hipEventRecord(ev[3], st);
-  hipLaunchKernelGGL(k_post<B>, g, b, 0, st, d);
-  (void)hipEventRecord(ev[4], st);
-  hipLaunchKernelGGL(k_pgs<B>, g, b, (size_t)d.pgs_lcap * 16 * kBodyLds, st, d);
-  (void)hipEventRecord(ev[5], st);
-  hipLaunchKernelGGL(k_avbd<B>, g, b, 0, st, d);
-  (void)hipEventRecord(ev[6], st);
-  hipLaunchKernelGGL(k_finish<B>, g, b, 0, st, d);
-  (void)hipEventRecord(ev[7], st);
-}
-
-void launch_step_timed(const Dev& d, hipStream_t st, uint32_t block, hipEvent_t* ev) {
-  switch (block) {
-    case 512: launch_all_timed<512>(d, st, ev); break;
-    case 1024: launch_all_timed<1024>(d, st, ev); break;
-    default: launch_all_timed<256>(d, st, ev); break;
+  launch_team(lanes[0].d, st);
+  for (int k = 0; k < kStages; ++k) {
+    for (int i = 0; i < nl; ++i) launch_stage(k, lanes[i], st);
+    (void)hipEventRecord(ev[k + 1], st);
   }
 }
 
@@ -3156,11 +3160,4 @@ uint32_t team_capacity() {
   return (uint32_t)per * (uint32_t)cus;
 }
 
-void launch_step(const Dev& d, hipStream_t st, uint32_t block) {
-  switch (block) {
-    case 512: launch_all<512>(d, st); break;
-    case 1024: launch_all<1024>(d, st); break;
-    default: launch_all<256>(d, st); break;
-  }
-}
 }  // namespace bb

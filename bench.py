@@ -62,18 +62,26 @@ def stage_bytes(counts: np.ndarray, iters: int, kw: int) -> dict:
 
 
 def profiled_traffic(kernel: str, worlds: int):
-    """HBM bytes per launch (read + write) of `kernel` over the whole batch,
+    """HBM bytes per step (read + write) of `kernel` over the whole batch,
     from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs of this same
-    command (copy-kernel calibrated), or None when there is none."""
+    command (copy-kernel calibrated), or None when there is none.  The step
+    launches the kernel once per lane; the lanes' dispatches are the ones
+    that ran every step (the autotuner's candidates ran a few times each),
+    and their workgroups add up to the batch's worlds."""
     try:
         with open(TRAFFIC_FILE) as f:
             k = json.load(f)["kernels"]
     except (OSError, ValueError, KeyError):
         return None
-    for v in k.values():
-        if v["kernel"].startswith(f"bb::{kernel}<") and v["workgroups"] == worlds and v["read_bytes"] is not None:
-            return float(v["read_bytes"]) + float(v["write_bytes"] or 0.0)
-    return None
+    rows = [v for v in k.values() if v["kernel"].startswith(f"bb::{kernel}<") and v["read_bytes"] is not None
+            and v["workgroups"] > 1]
+    if not rows:
+        return None
+    most = max(v["calls"] for v in rows)
+    lanes = [v for v in rows if v["calls"] * 2 >= most]
+    if sum(v["workgroups"] for v in lanes) != worlds:
+        return None
+    return sum(float(v["read_bytes"]) + float(v["write_bytes"] or 0.0) for v in lanes)
 
 
 def make_batch(worlds: int, seed: int, lo: int, hi: int):
@@ -99,7 +107,8 @@ def time_batch(b, steps: int, warmup: int, tune: bool):
     b.sync()
     if tune:
         b.autotune(2)
-    params = (b.get_param(beatbox.PARAM_BLOCK), b.get_param(beatbox.PARAM_PGS_LDS_ROWS))
+    params = (b.get_param(beatbox.PARAM_BLOCK), b.get_param(beatbox.PARAM_PGS_LDS_ROWS),
+              b.get_param(beatbox.PARAM_LANE_BODIES), b.get_param(beatbox.PARAM_LANE_BLOCK))
     e0 = float(b.energy().sum())
     t0 = time.perf_counter()
     b.step(steps, use_graph=True)
@@ -355,6 +364,8 @@ def main() -> int:
             "copy_kernel_gbs": copy_gbs,
             "block": params[0],
             "pgs_lds_rows": params[1],
+            "lane_bodies": params[2],
+            "lane_block": params[3],
             **extra,
             "stages": stages,
         },

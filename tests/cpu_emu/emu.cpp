@@ -281,7 +281,10 @@ void launch(dim3 grid, dim3 block, size_t shmem, std::function<void()> fn) {
 struct emu_stream {
   bool capturing = false;
   std::vector<std::function<void()>> cap;
+  emu_stream* redirect = nullptr;  // joined the capture of another stream
 };
+static emu_stream* g_capture_of = nullptr;
+static std::vector<emu_stream*> g_joined;
 struct emu_graph {
   std::vector<std::function<void()>> ops;
 };
@@ -296,6 +299,7 @@ static double now_ms() {
 
 namespace emu {
 void submit(hipStream_t st, std::function<void()> fn) {
+  if (st && st->redirect) st = st->redirect;
   if (st && st->capturing) st->cap.push_back(std::move(fn));
   else fn();
 }
@@ -345,12 +349,16 @@ hipError_t hipDeviceSynchronize() { return hipSuccess; }
 hipError_t hipStreamBeginCapture(hipStream_t s, hipStreamCaptureMode) {
   s->capturing = true;
   s->cap.clear();
+  g_capture_of = s;
   return hipSuccess;
 }
 hipError_t hipStreamEndCapture(hipStream_t s, hipGraph_t* g) {
   *g = new emu_graph{std::move(s->cap)};
   s->cap.clear();
   s->capturing = false;
+  g_capture_of = nullptr;
+  for (emu_stream* j : g_joined) j->redirect = nullptr;
+  g_joined.clear();
   return hipSuccess;
 }
 hipError_t hipGraphInstantiate(hipGraphExec_t* e, hipGraph_t g, hipGraphNode_t*, char*, size_t) {
@@ -377,6 +385,16 @@ hipError_t hipEventCreate(hipEvent_t* e) {
   return hipSuccess;
 }
 hipError_t hipEventCreateWithFlags(hipEvent_t* e, unsigned) { return hipEventCreate(e); }
+// Work runs when it is submitted (or replayed), so a cross-stream wait has
+// nothing to wait for; inside a capture, the second stream's work is recorded
+// into the capturing stream's graph.
+hipError_t hipStreamWaitEvent(hipStream_t s, hipEvent_t, unsigned) {
+  if (s && g_capture_of && s != g_capture_of && !s->redirect) {
+    s->redirect = g_capture_of;
+    g_joined.push_back(s);
+  }
+  return hipSuccess;
+}
 hipError_t hipEventDestroy(hipEvent_t e) {
   delete e;
   return hipSuccess;

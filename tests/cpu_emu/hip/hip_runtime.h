@@ -155,6 +155,7 @@ enum hipDeviceAttribute_t { hipDeviceAttributeMultiprocessorCount = 1 };
 #define hipStreamDefault 0u
 #define hipHostMallocDefault 0u
 #define hipEventDefault 0u
+#define hipEventDisableTiming 2u
 
 hipError_t hipMalloc(void** p, size_t n);
 template <class T>
@@ -181,6 +182,7 @@ hipError_t hipGraphExecDestroy(hipGraphExec_t e);
 hipError_t hipGraphDestroy(hipGraph_t g);
 hipError_t hipEventCreate(hipEvent_t* e);
 hipError_t hipEventCreateWithFlags(hipEvent_t* e, unsigned f);
+hipError_t hipStreamWaitEvent(hipStream_t s, hipEvent_t e, unsigned flags);
 hipError_t hipEventDestroy(hipEvent_t e);
 hipError_t hipEventRecord(hipEvent_t e, hipStream_t s = nullptr);
 hipError_t hipEventSynchronize(hipEvent_t e);

@@ -1,5 +1,5 @@
 // Emulation driver: steps the worlds of a scene file through the C-ABI.
-// usage: emu_main scene.bin steps team_bodies block graph out.bin
+// usage: emu_main scene.bin steps team_bodies block graph out.bin [lane_bodies]
 #include <cstdio>
 #include <cstdlib>
 #include <vector>
@@ -31,6 +31,7 @@ int main(int argc, char** argv) {
   bb_batch_set_param(b, BB_PARAM_TEAM_BODIES, (uint32_t)std::strtoul(argv[3], nullptr, 0));
   bb_batch_set_param(b, BB_PARAM_BLOCK, (uint32_t)std::atoi(argv[4]));
   const int graph = std::atoi(argv[5]);
+  if (argc > 7) bb_batch_set_param(b, BB_PARAM_LANE_BODIES, (uint32_t)std::strtoul(argv[7], nullptr, 0));
   for (int s = 0; s < steps; ++s) {
     int st = bb_batch_step(b, 1, nullptr, graph);
     if (st) { std::fprintf(stderr, "step %d\n", st); return 5; }

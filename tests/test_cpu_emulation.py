@@ -43,8 +43,8 @@ def _scene(path, worlds):
             f.write(arr.tobytes())
 
 
-def _run(exe, scene, out, steps, team_bodies, block, graph=1):
-    r = subprocess.run([exe, scene, str(steps), str(team_bodies), str(block), str(graph), out],
+def _run(exe, scene, out, steps, team_bodies, block, graph=1, lane_bodies=0):
+    r = subprocess.run([exe, scene, str(steps), str(team_bodies), str(block), str(graph), out, str(lane_bodies)],
                        capture_output=True, text=True, env=ENV, timeout=600)
     assert r.returncode == 0, r.stderr[-3000:]
     assert "AddressSanitizer" not in r.stderr, r.stderr[-3000:]
@@ -111,3 +111,19 @@ def test_emulated_step_equals_oracle(emu_main, tmp_path):
         for f in ("position", "rotation", "velocity", "omega", "flags", "id"):
             assert mine[f].tobytes() == ref[f].tobytes(), f
     assert at == len(got)
+
+
+def test_two_lanes_equal_one_lane(emu_main, tmp_path):
+    """Worlds above the lane threshold run the per-world kernels with
+    512-thread workgroups on a second stream (captured into the same graph);
+    the trajectories equal the one-lane step's."""
+    pgs = beatbox.WorldDesc.make(**scenes.world_desc(iterations=8, morton_bits=63, quirks=0))
+    avbd = beatbox.WorldDesc.make(**scenes.world_desc(solver=scenes.SOLVER_AVBD, iterations=10,
+                                                      morton_bits=63, quirks=0))
+    scene = str(tmp_path / "scene.bin")
+    _scene(scene, [(pgs, scenes.batch_world(0xBB04, 3, 60)), (pgs, scenes.batch_world(0xBB04, 8, 400)),
+                   (avbd, scenes.serving_pile(33, 350)), (pgs, scenes.serving_pile(34, 120))])
+    for graph in (1, 0):
+        a, x = _run(emu_main, scene, str(tmp_path / "a.bin"), 5, 0, 0, graph=graph, lane_bodies=0)
+        b, y = _run(emu_main, scene, str(tmp_path / "b.bin"), 5, 0, 0, graph=graph, lane_bodies=300)
+        assert a == b and x == y, graph

@@ -237,3 +237,35 @@ def test_team_path_equals_workgroup_path():
         for k in ("broad_phase_collision_count", "collision_count", "contact_count", "island_count",
                   "graph_color_count", "avbd_color_count", "dbg_id_sum"):
             assert sx[k] == sy[k], (w, k)
+
+
+def test_two_lanes_equal_one_lane():
+    """Worlds at or above BB_PARAM_LANE_BODIES run the per-world kernels with
+    BB_PARAM_LANE_BLOCK threads on a second stream, concurrently with the
+    rest (graph and eager steps); every world's trajectory and counters are
+    bit-identical to the one-lane step's."""
+    pgs = beatbox.WorldDesc.make(**scenes.world_desc(iterations=8, morton_bits=63, quirks=0))
+    avbd = beatbox.WorldDesc.make(**scenes.world_desc(solver=scenes.SOLVER_AVBD, iterations=10,
+                                                      morton_bits=63, quirks=0))
+    worlds = [(pgs, scenes.batch_world(0xBB04, w, n)) for w, n in ((1, 1500), (2, 90), (3, 1100), (4, 700))]
+    worlds += [(avbd, scenes.serving_pile(41, 1200)), (avbd, scenes.serving_pile(42, 300))]
+    runs = []
+    for lane, lblock in ((0, 512), (1024, 512), (600, 1024)):
+        b = beatbox.Batch()
+        for d, bodies in worlds:
+            b.add_world(d, bodies)
+        b.set_param(beatbox.PARAM_LANE_BODIES, lane)
+        b.set_param(beatbox.PARAM_LANE_BLOCK, lblock)
+        b.step(6, use_graph=True)
+        b.step(3, use_graph=False)
+        b.sync()
+        runs.append(b)
+    for w, (d, bodies) in enumerate(worlds):
+        ref = runs[0].read_world(w, len(bodies))
+        for r in runs[1:]:
+            got = r.read_world(w, len(bodies))
+            for f in ("position", "rotation", "velocity", "omega", "flags"):
+                assert got[f].tobytes() == ref[f].tobytes(), (w, f)
+            assert r.stats(w)["contact_count"] == runs[0].stats(w)["contact_count"]
+    for r in runs:
+        r.destroy()
