@@ -219,7 +219,9 @@ def test_team_path_equals_workgroup_path():
               (pgs63, scenes.batch_world(0xBB04, 2, 64)),
               (avbd, scenes.serving_pile(22, 700))]
     runs = []
-    for team in (0, 600):
+    # 0: every world by one workgroup (the two largest in the 512-thread lane);
+    # 600: four worlds as teams; 2000: two teams running beside both lanes
+    for team in (0, 600, 2000):
         b = beatbox.Batch()
         for d, bodies in worlds:
             b.add_world(d, bodies)
@@ -229,14 +231,16 @@ def test_team_path_equals_workgroup_path():
         runs.append(b)
     for w, (d, bodies) in enumerate(worlds):
         x = runs[0].read_world(w, len(bodies))
-        y = runs[1].read_world(w, len(bodies))
-        for f in ("position", "rotation", "velocity", "omega", "flags", "island_index"):
-            assert x[f].tobytes() == y[f].tobytes(), (w, f)
-        sx, sy = runs[0].stats(w), runs[1].stats(w)
-        assert sy["overflow"] == 0, w
-        for k in ("broad_phase_collision_count", "collision_count", "contact_count", "island_count",
-                  "graph_color_count", "avbd_color_count", "dbg_id_sum"):
-            assert sx[k] == sy[k], (w, k)
+        sx = runs[0].stats(w)
+        for r in runs[1:]:
+            y = r.read_world(w, len(bodies))
+            for f in ("position", "rotation", "velocity", "omega", "flags", "island_index"):
+                assert x[f].tobytes() == y[f].tobytes(), (w, f)
+            sy = r.stats(w)
+            assert sy["overflow"] == 0, w
+            for k in ("broad_phase_collision_count", "collision_count", "contact_count", "island_count",
+                      "graph_color_count", "avbd_color_count", "dbg_id_sum"):
+                assert sx[k] == sy[k], (w, k)
 
 
 def test_two_lanes_equal_one_lane():
