@@ -101,15 +101,15 @@ def make_batch(worlds: int, seed: int, lo: int, hi: int):
 def time_batch(b, steps: int, warmup: int, tune: bool):
     """Graph replay of `steps` steps after `warmup` untimed ones (and the
     autotuner, which replays the next step per candidate and leaves the state
-    unchanged); returns the wall time, the batch energy before and after the
-    timed steps and the chosen launch parameters."""
+    unchanged); returns the wall time, the batch energy at step 0 and after
+    the timed steps and the chosen launch parameters."""
+    e0 = float(b.energy().sum())
     b.step(warmup, use_graph=True)
     b.sync()
     if tune:
         b.autotune(2)
     params = (b.get_param(beatbox.PARAM_BLOCK), b.get_param(beatbox.PARAM_PGS_LDS_ROWS),
               b.get_param(beatbox.PARAM_LANE_BODIES), b.get_param(beatbox.PARAM_LANE_BLOCK))
-    e0 = float(b.energy().sum())
     t0 = time.perf_counter()
     b.step(steps, use_graph=True)
     b.sync()
@@ -284,7 +284,7 @@ def main() -> int:
     ms_per_step = t / args.steps * 1e3
     drift = (e1 - e0) / abs(e0) if e0 else 0.0
     log(f"batch: {ms_per_step:.3f} ms/step, {value / 1e6:.2f} M body-steps/s, {dev_gib:.2f} GiB, "
-        f"energy {e0:.4e} -> {e1:.4e} J (rel. change {drift:+.3e})")
+        f"energy {e0:.4e} J at step 0 -> {e1:.4e} J (rel. change {drift:+.3e})")
 
     # 2) per-stage durations over the same step window: a fresh batch of the
     # same seed reaches the same states (the step is deterministic), then
