@@ -119,7 +119,8 @@ KERNEL_NAMES = ("k_build", "k_broad", "k_narrow", "k_post", "k_pgs", "k_avbd", "
 KERNEL_COUNT = len(KERNEL_NAMES)
 COUNT_NAMES = ("bodies", "active", "pairs", "manifolds", "contacts", "color_work", "max_pen_mm", "sleeping")
 COUNT_FIELDS = len(COUNT_NAMES)
-PARAM_BLOCK, PARAM_PGS_LDS_ROWS, PARAM_TEAM_BODIES, PARAM_LANE_BODIES, PARAM_LANE_BLOCK = 0, 1, 2, 3, 4
+PARAM_BLOCK, PARAM_PGS_LDS_ROWS, PARAM_TEAM_BODIES, PARAM_LANE_BODIES, PARAM_LANE_BLOCK, PARAM_LANE_LDS_ROWS = \
+    0, 1, 2, 3, 4, 5
 
 STATUS = {0: "BB_OK", 1: "BB_ERR_INVALID", 2: "BB_ERR_CAPACITY", 3: "BB_ERR_DEVICE",
           4: "BB_ERR_NO_DEVICE", 5: "BB_ERR_OVERFLOW", 6: "BB_ERR_STATE", 7: "BB_ERR_BUDGET"}

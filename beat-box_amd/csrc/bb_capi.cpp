@@ -497,7 +497,13 @@ uint32_t pick_block(const bb_batch* b) {
 int lanes_of(const bb_batch* b, const Dev& d, Lane* out) {
   const uint32_t nw = std::min(b->n_wide, d.W);
   int nl = 0;
-  if (nw) out[nl++] = Lane{d, b->lane_block, nw};
+  if (nw) {
+    Dev dw = d;
+    uint32_t mx = 0;
+    for (auto& hw : b->worlds) mx = std::max(mx, (uint32_t)hw.bodies.size());
+    dw.pgs_lcap = std::min(mx, b->lane_lds);
+    out[nl++] = Lane{dw, b->lane_block, nw};
+  }
   Dev rest = d;
   rest.order = d.order + nw;
   out[nl++] = Lane{rest, pick_block(b), d.W - nw};
@@ -710,6 +716,9 @@ bb_status set_param(bb_batch* b, bb_param p, uint32_t value) {
   } else if (p == BB_PARAM_LANE_BLOCK) {
     if (value != 256 && value != 512 && value != 1024) return BB_ERR_INVALID;
     b->lane_block = value;
+  } else if (p == BB_PARAM_LANE_LDS_ROWS) {
+    if (value > kMaxPgsLdsRows) return BB_ERR_INVALID;
+    b->lane_lds = value;
   } else {
     return BB_ERR_INVALID;
   }
@@ -747,7 +756,7 @@ bb_status autotune_batch(bb_batch* b, uint32_t reps, float* best_ms) {
   uint32_t mx = 0;
   for (auto& hw : b->worlds) mx = std::max(mx, (uint32_t)hw.bodies.size());
   std::vector<uint32_t> lds_opts;
-  for (uint32_t v : {0u, 256u, 512u, kMaxPgsLdsRows}) {
+  for (uint32_t v : {0u, 256u, 512u, 768u}) {
     lds_opts.push_back(v);
     if (v >= mx) break;
   }
@@ -1252,6 +1261,7 @@ uint32_t bb_batch_get_param(const bb_batch* b, bb_param p) {
   if (p == BB_PARAM_TEAM_BODIES) return b->team_bodies;
   if (p == BB_PARAM_LANE_BODIES) return b->lane_bodies;
   if (p == BB_PARAM_LANE_BLOCK) return b->lane_block;
+  if (p == BB_PARAM_LANE_LDS_ROWS) return b->lane_lds;
   return 0;
 }
 

@@ -146,7 +146,7 @@ struct Dev {
   float4* iw;         // [body*3] cached world inverse inertia rows, row0.w = inv_mass
   // PGS solver rows (colour-sorted, [pair slot]; contact c of row k at c*np+k)
   uint32_t np;        // pair slots in the batch
-  uint32_t pgs_lcap;  // rows per world whose velocities k_pgs keeps in LDS (32 B each)
+  uint32_t pgs_lcap;  // rows per world whose body state k_pgs keeps in LDS (80 B each)
   int4* sv_hdr;       // a, b, count | 256 a dynamic | 512 b dynamic | 1024 one immovable, friction bits
   float4* sv_n;       // normal
   float4* sv_r1;      // [8*np] arm a, penetration

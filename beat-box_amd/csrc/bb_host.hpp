@@ -36,9 +36,9 @@ uint32_t team_capacity();
 namespace bb {
 namespace host {
 
-// Rows per world whose body state k_pgs may keep in LDS (80 B each, within
-// 64 KB per workgroup).
-constexpr uint32_t kMaxPgsLdsRows = 768;
+// Rows per world whose body state k_pgs may keep in LDS: 80 B each, within
+// the 160 KB a gfx950 workgroup can allocate.
+constexpr uint32_t kMaxPgsLdsRows = 2048;
 
 struct HostWorld {
   bb_world_desc desc;
@@ -128,6 +128,7 @@ struct bb_batch {
   // per-world kernels with lane_block threads on a second stream
   uint32_t lane_bodies = 512;
   uint32_t lane_block = 512;
+  uint32_t lane_lds = 1024;  // BB_PARAM_LANE_LDS_ROWS: LDS rows of k_pgs in that lane
   uint32_t n_wide = 0;  // worlds in that lane: the first n_wide of the order table
   hipStream_t stream2 = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;

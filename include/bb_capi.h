@@ -353,7 +353,7 @@ typedef enum bb_param {
                                 256, 512 or 1024                            */
   BB_PARAM_PGS_LDS_ROWS = 1, /* rows per world whose velocities and world
                                 inverse inertia the PGS sweeps keep in LDS
-                                (80 B each): 0..768                         */
+                                (80 B each): 0..2048                        */
   BB_PARAM_TEAM_BODIES = 2,  /* worlds with at least this many bodies are
                                 stepped by a team of workgroups (k_team);
                                 0 = every world by one workgroup            */
@@ -361,8 +361,10 @@ typedef enum bb_param {
                                 below the team threshold) run the per-world
                                 kernels in a second lane, concurrently with
                                 the rest; 0 = one lane.  Default 512        */
-  BB_PARAM_LANE_BLOCK = 4    /* threads per workgroup in that lane: 256, 512
+  BB_PARAM_LANE_BLOCK = 4,   /* threads per workgroup in that lane: 256, 512
                                 or 1024.  Default 512                       */
+  BB_PARAM_LANE_LDS_ROWS = 5 /* BB_PARAM_PGS_LDS_ROWS of that lane: 0..2048.
+                                Default 1,024                               */
 } bb_param;
 bb_status bb_batch_set_param(bb_batch* b, bb_param p, uint32_t value);
 /* Effective value (BB_PARAM_BLOCK resolves 0 to the automatic choice). */

@@ -162,7 +162,7 @@ def test_launch_params_do_not_change_results():
         solver=scenes.SOLVER_AVBD if w == 1 else scenes.SOLVER_PGS_SOFT,
         iterations=8, morton_bits=63, quirks=0)) for w in range(len(sizes))]
     runs = []
-    for blk, lds in ((256, 0), (512, 256), (1024, 768), (256, 512)):
+    for blk, lds in ((256, 0), (512, 256), (1024, 768), (256, 512), (512, 2048)):
         b = beatbox.Batch()
         for d, bodies in zip(descs, worlds):
             b.add_world(d, bodies)
