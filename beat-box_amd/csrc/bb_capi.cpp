@@ -748,7 +748,7 @@ bb_status autotune_batch(bb_batch* b, uint32_t reps, float* best_ms) {
   const int saved = b->latest;
   const uint64_t saved_index = b->step_index;
   const bool saved_stepped = b->stepped;
-  const uint32_t saved_block = b->block, saved_lds = b->pgs_lds, saved_lane = b->lane_bodies;
+  const uint32_t saved_block = b->block, saved_lds = b->pgs_lds;
   const size_t stat_bytes = b->worlds.size() * ST_COUNT * sizeof(uint32_t);
   uint32_t* stat_copy = nullptr;
   HIPCHK(hipMalloc(&stat_copy, stat_bytes));
@@ -764,44 +764,37 @@ bb_status autotune_batch(bb_batch* b, uint32_t reps, float* best_ms) {
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
   float best = 3.4e38f;
-  uint32_t best_block = saved_block, best_lds = saved_lds, best_lane = saved_lane;
+  uint32_t best_block = saved_block, best_lds = saved_lds;
   bb_status rc = BB_OK;
-  // The wide lane's threshold is not tuned here: one replayed step early in a
-  // run does not show how much the large worlds come to dominate later (over
-  // the 240-step batch the lane gains 11% while the first steps favour one
-  // lane), so it keeps its setting.
-  const uint32_t lane_opts[] = {saved_lane};
-  for (uint32_t lane : lane_opts) {
-    for (uint32_t blk : {256u, 512u, 1024u}) {
-      for (uint32_t lds : lds_opts) {
-        b->block = blk;
-        b->pgs_lds = lds;
-        b->lane_bodies = lane;
-        b->base.pgs_lcap = pgs_lds_rows(b);
-        if ((rc = upload_teams(b)) != BB_OK) break;
+  // The wide lane is not tuned here: one replayed step early in a run does
+  // not show how much the large worlds come to dominate later (over the
+  // 240-step batch the lane gains 11% while the first steps favour one lane).
+  for (uint32_t blk : {256u, 512u, 1024u}) {
+    for (uint32_t lds : lds_opts) {
+      b->block = blk;
+      b->pgs_lds = lds;
+      b->base.pgs_lcap = pgs_lds_rows(b);
+      if ((rc = upload_teams(b)) != BB_OK) break;
+      b->latest = saved;
+      if ((rc = launch_one(b)) != BB_OK) break;  // untimed first launch
+      (void)hipEventRecord(e0, b->stream);
+      for (uint32_t r = 0; r < reps && rc == BB_OK; ++r) {
         b->latest = saved;
-        if ((rc = launch_one(b)) != BB_OK) break;  // untimed first launch
-        (void)hipEventRecord(e0, b->stream);
-        for (uint32_t r = 0; r < reps && rc == BB_OK; ++r) {
-          b->latest = saved;
-          rc = launch_one(b);
-        }
-        (void)hipEventRecord(e1, b->stream);
-        if (rc != BB_OK || hipEventSynchronize(e1) != hipSuccess) {
-          rc = rc != BB_OK ? rc : BB_ERR_DEVICE;
-          break;
-        }
-        float ms = 0.0f;
-        (void)hipEventElapsedTime(&ms, e0, e1);
-        ms /= (float)reps;
-        if (ms < best) {
-          best = ms;
-          best_block = blk;
-          best_lds = lds;
-          best_lane = lane;
-        }
+        rc = launch_one(b);
       }
-      if (rc != BB_OK) break;
+      (void)hipEventRecord(e1, b->stream);
+      if (rc != BB_OK || hipEventSynchronize(e1) != hipSuccess) {
+        rc = rc != BB_OK ? rc : BB_ERR_DEVICE;
+        break;
+      }
+      float ms = 0.0f;
+      (void)hipEventElapsedTime(&ms, e0, e1);
+      ms /= (float)reps;
+      if (ms < best) {
+        best = ms;
+        best_block = blk;
+        best_lds = lds;
+      }
     }
     if (rc != BB_OK) break;
   }
@@ -816,7 +809,6 @@ bb_status autotune_batch(bb_batch* b, uint32_t reps, float* best_ms) {
   (void)hipFree(stat_copy);
   b->block = rc == BB_OK ? best_block : saved_block;
   b->pgs_lds = rc == BB_OK ? best_lds : saved_lds;
-  b->lane_bodies = rc == BB_OK ? best_lane : saved_lane;
   b->base.pgs_lcap = pgs_lds_rows(b);
   {
     const bb_status tr = upload_teams(b);
