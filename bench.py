@@ -252,7 +252,7 @@ def main() -> int:
     ap.add_argument("--serving-seconds", type=float, default=10.0)
     ap.add_argument("--serving-budget-gib", type=float, default=200.0)
     ap.add_argument("--serving-cap", type=int, default=200_000, help="resident-body cap of the serving run")
-    ap.add_argument("--serving-team-bodies", type=int, default=2048,
+    ap.add_argument("--serving-team-bodies", type=int, default=1024,
                     help="worlds this large are stepped as workgroup teams in the serving run")
     ap.add_argument("--profile-steps", type=int, default=0, help="0: the timed steps")
     ap.add_argument("--plain-steps", type=int, default=200)
