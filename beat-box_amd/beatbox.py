@@ -177,6 +177,8 @@ EXPORTS = {
     "bb_batch_profile": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]),
     "bb_batch_energy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32]),
     "bb_world_energy": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
+    "bb_world_instance_transforms": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_int]),
+    "bb_batch_instance_transforms": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_int]),
     "bb_batch_device_bytes": (C.c_uint64, [C.c_void_p]),
     "bb_batch_destroy": (None, [C.c_void_p]),
     "bb_server_create": (C.c_int, [C.POINTER(ServerDesc), C.POINTER(C.c_void_p)]),
@@ -298,6 +300,19 @@ class RigidBodyManager:
         _check(lib().bb_world_energy(self._h, C.byref(e)), "bb_world_energy")
         return e.value
 
+    def instance_transforms(self, n: int, device_ptr: int | None = None):
+        """Per body id, the 3x4 affine transform [R | t] of its pose (TLAS
+        instance layout).  Returns a (n, 3, 4) float32 array, or, with a device
+        pointer to n*12 floats, queues the write there and returns None."""
+        if device_ptr is not None:
+            _check(lib().bb_world_instance_transforms(self._h, C.c_void_p(device_ptr), n, 1),
+                   "bb_world_instance_transforms")
+            return None
+        out = np.zeros((n, 3, 4), np.float32)
+        _check(lib().bb_world_instance_transforms(self._h, out.ctypes.data_as(C.c_void_p), n, 0),
+               "bb_world_instance_transforms")
+        return out
+
     def destroy(self) -> None:
         if self._h:
             lib().bb_world_destroy(self._h)
@@ -359,6 +374,17 @@ class Batch:
         out = np.zeros(max(self.world_count, 1), np.float64)
         _check(lib().bb_batch_energy(self._h, out.ctypes.data_as(C.c_void_p), len(out)), "bb_batch_energy")
         return out[: self.world_count]
+
+    def instance_transforms(self, w: int, n: int, device_ptr: int | None = None):
+        """RigidBodyManager.instance_transforms for world w."""
+        if device_ptr is not None:
+            _check(lib().bb_batch_instance_transforms(self._h, w, C.c_void_p(device_ptr), n, 1),
+                   "bb_batch_instance_transforms")
+            return None
+        out = np.zeros((n, 3, 4), np.float32)
+        _check(lib().bb_batch_instance_transforms(self._h, w, out.ctypes.data_as(C.c_void_p), n, 0),
+               "bb_batch_instance_transforms")
+        return out
 
     @property
     def device_bytes(self) -> int:

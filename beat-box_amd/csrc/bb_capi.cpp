@@ -701,6 +701,28 @@ bb_status energy_batch(bb_batch* b, double* out, uint32_t n) {
   return rc == hipSuccess ? BB_OK : BB_ERR_DEVICE;
 }
 
+bb_status instances_batch(bb_batch* b, uint32_t w, float* dst, uint32_t n, int on_device) {
+  if (!dst || w >= b->worlds.size()) return BB_ERR_INVALID;
+  const uint32_t N = (uint32_t)b->worlds[w].bodies.size();
+  if (n < N) return BB_ERR_CAPACITY;
+  bb_status st = build(b);
+  if (st != BB_OK) return st;
+  const Dev d = dev_for(b, b->latest);
+  if (on_device) {  // stream-ordered after the steps; bb_batch_sync waits for it
+    launch_instances(d, w, N, dst, b->stream);
+    HIPCHK(hipGetLastError());
+    return BB_OK;
+  }
+  float* tmp = nullptr;
+  if (hipMalloc(&tmp, (size_t)std::max(N, 1u) * 12 * sizeof(float)) != hipSuccess) return BB_ERR_DEVICE;
+  launch_instances(d, w, N, tmp, b->stream);
+  hipError_t rc = hipGetLastError();
+  if (rc == hipSuccess) rc = hipStreamSynchronize(b->stream);
+  if (rc == hipSuccess) rc = hipMemcpy(dst, tmp, (size_t)N * 12 * sizeof(float), hipMemcpyDeviceToHost);
+  (void)hipFree(tmp);
+  return rc == hipSuccess ? BB_OK : BB_ERR_DEVICE;
+}
+
 bb_status set_param(bb_batch* b, bb_param p, uint32_t value) {
   if (p == BB_PARAM_BLOCK) {
     if (value != 0 && value != 256 && value != 512 && value != 1024) return BB_ERR_INVALID;
@@ -1226,6 +1248,10 @@ bb_status bb_batch_energy(bb_batch* b, double* per_world, uint32_t n) {
   return b ? energy_batch(b, per_world, n) : BB_ERR_INVALID;
 }
 
+bb_status bb_batch_instance_transforms(bb_batch* b, uint32_t world, float* dst, uint32_t n, int dst_on_device) {
+  return b ? instances_batch(b, world, dst, n, dst_on_device) : BB_ERR_INVALID;
+}
+
 uint64_t bb_batch_device_bytes(const bb_batch* b) { return b ? b->buf.bytes : 0; }
 
 uint32_t bb_batch_world_count(const bb_batch* b) { return b ? (uint32_t)b->worlds.size() : 0; }
@@ -1359,6 +1385,10 @@ bb_status bb_world_read_stats(bb_world* w, bb_step_stats* out) {
 
 bb_status bb_world_energy(bb_world* w, double* out) {
   return w ? energy_batch(w->b, out, 1) : BB_ERR_INVALID;
+}
+
+bb_status bb_world_instance_transforms(bb_world* w, float* dst, uint32_t n, int dst_on_device) {
+  return w ? instances_batch(w->b, 0, dst, n, dst_on_device) : BB_ERR_INVALID;
 }
 
 bb_status bb_world_debug_tap(bb_world* w, bb_tap kind, void* dst, size_t cap, size_t* written) {

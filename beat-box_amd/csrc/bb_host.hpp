@@ -21,6 +21,7 @@ void launch_step_timed(const Lane* lanes, int nl, hipStream_t st, hipEvent_t* ev
 void launch_copy(const float4* src, float4* dst, size_t n, hipStream_t st);
 void launch_sum_counts(const Dev& d, unsigned long long* out, hipStream_t st);
 void launch_energy(const Dev& d, double* out, hipStream_t st);
+void launch_instances(const Dev& d, uint32_t w, uint32_t n, float* dst, hipStream_t st);
 uint32_t team_capacity();
 }  // namespace bb
 

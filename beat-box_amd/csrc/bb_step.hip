@@ -2988,6 +2988,25 @@ __global__ __launch_bounds__(BLOCK) void k_team(Dev d) {
     atomicOr(&d.stats[w * ST_COUNT + ST_OVERFLOW], 2u);
 }
 
+// Render hand-off (entry_update_acceleration_structures, src/shaders/
+// AS_mngr.slang:5-36, with RigidBody::get_instance_transform,
+// include/shared.inl:377-419): for every body of world w, at its id, the
+// 3x4 row-major affine transform [R | t] of its pose (the layout of a
+// Vulkan TLAS instance), R from Quaternion::to_matrix (include/math.hpp:
+// 292-309).
+__global__ __launch_bounds__(256) void k_instances(Dev d, uint32_t w, float* __restrict__ dst) {
+  const WorldInfo W = d.worlds[w];
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  if (i >= W.n) return;
+  const uint32_t g = W.body_off + i;
+  const v3 p = ld3(d.cur.pos[g]);
+  const mat3 R = qmat(ldq(d.cur.rot[g]));
+  float4* o = reinterpret_cast<float4*>(dst + (size_t)d.cur.id[g] * 12u);
+  o[0] = make_float4(R.m[0][0], R.m[0][1], R.m[0][2], p.x);
+  o[1] = make_float4(R.m[1][0], R.m[1][1], R.m[1][2], p.y);
+  o[2] = make_float4(R.m[2][0], R.m[2][1], R.m[2][2], p.z);
+}
+
 // Batch totals of one step's counters, one workgroup: out[0..7] = sum N,
 // sum A, sum P, sum M, sum C, sum over worlds of (colours used x M), max
 // penetration (mm), sum sleeping.
@@ -3144,6 +3163,10 @@ void launch_copy(const float4* src, float4* dst, size_t n, hipStream_t st) {
 
 void launch_sum_counts(const Dev& d, unsigned long long* out, hipStream_t st) {
   hipLaunchKernelGGL(k_sum_counts, dim3(1), dim3(1024), 0, st, d.stats, d.W, out);
+}
+
+void launch_instances(const Dev& d, uint32_t w, uint32_t n, float* dst, hipStream_t st) {
+  if (n) hipLaunchKernelGGL(k_instances, dim3((n + 255u) / 256u), dim3(256), 0, st, d, w, dst);
 }
 
 void launch_energy(const Dev& d, double* out, hipStream_t st) {

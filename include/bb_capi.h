@@ -276,6 +276,19 @@ bb_status bb_world_debug_tap(bb_world* w, bb_tap kind, void* dst, size_t cap,
 uint32_t bb_world_body_count(const bb_world* w);
 /* Mechanical energy of the world's current state (see bb_batch_energy). */
 bb_status bb_world_energy(bb_world* w, double* out);
+/* Render hand-off: replaces entry_update_acceleration_structures
+ * (src/shaders/AS_mngr.slang:5-36) with RigidBody::get_instance_transform
+ * (include/shared.inl:377-419), which fill the TLAS instance buffer per
+ * body id.  Writes, for body id i, 12 floats at dst + 12*i: the 3x4
+ * row-major affine transform [R | t] of the current pose (the
+ * VkTransformMatrixKHR layout), R by Quaternion::to_matrix
+ * (include/math.hpp:292-309).  n = capacity of dst in bodies (at least the
+ * world's body count, else BB_ERR_CAPACITY).  With dst_on_device != 0, dst
+ * is device memory and the write is queued on the world's stream after the
+ * steps (bb_world_sync waits for it); otherwise dst is host memory and the
+ * call returns when it is filled. */
+bb_status bb_world_instance_transforms(bb_world* w, float* dst, uint32_t n,
+                                       int dst_on_device);
 /* RigidBodyManager::destroy (src/rigid_body_manager.cpp:1550-1585). */
 void bb_world_destroy(bb_world* w);
 
@@ -340,6 +353,9 @@ bb_status bb_batch_profile(bb_batch* b, uint32_t nsteps, float* step_ms, float* 
  * counterpart): sum over dynamic bodies of m|v|^2/2 + w.I.w/2 - m*g*y with
  * the physical inertia of the shape.  per_world[n], n >= world count. */
 bb_status bb_batch_energy(bb_batch* b, double* per_world, uint32_t n);
+/* bb_world_instance_transforms for one world of a batch. */
+bb_status bb_batch_instance_transforms(bb_batch* b, uint32_t world, float* dst,
+                                       uint32_t n, int dst_on_device);
 /* Device bytes held by the batch. */
 uint64_t bb_batch_device_bytes(const bb_batch* b);
 

@@ -76,6 +76,14 @@ class RigidBodyManager {
     last_ = bb_world_read_bodies(world_, out.data(), (uint32_t)out.size(), order);
     return last_ == BB_OK;
   }
+  // entry_update_acceleration_structures (src/shaders/AS_mngr.slang:5-36):
+  // per body id, the 3x4 [R | t] instance transform, 12 floats each, into
+  // device memory (queued on the world's stream) or host memory.
+  bool update_instances(float* dst, uint32_t capacity_bodies, bool dst_on_device) {
+    if (!world_) return false;
+    last_ = bb_world_instance_transforms(world_, dst, capacity_bodies, dst_on_device ? 1 : 0);
+    return last_ == BB_OK;
+  }
 
   uint32_t get_sim_flags() const { return world_ ? bb_world_get_flags(world_) : desc_.flags; }
   uint32_t set_sim_flags(uint32_t flags) {
