@@ -178,6 +178,8 @@ EXPORTS = {
     "bb_batch_energy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32]),
     "bb_world_energy": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
     "bb_world_instance_transforms": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_int]),
+    "bb_world_snapshot": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.POINTER(C.c_size_t)]),
+    "bb_world_restore": (C.c_int, [C.c_void_p, C.c_size_t, C.c_int, C.POINTER(C.c_void_p)]),
     "bb_batch_instance_transforms": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_int]),
     "bb_batch_device_bytes": (C.c_uint64, [C.c_void_p]),
     "bb_batch_destroy": (None, [C.c_void_p]),
@@ -299,6 +301,22 @@ class RigidBodyManager:
         e = C.c_double(0.0)
         _check(lib().bb_world_energy(self._h, C.byref(e)), "bb_world_energy")
         return e.value
+
+    def snapshot(self) -> bytes:
+        """Checkpoint of the world (bb_world_snapshot)."""
+        size = C.c_size_t(0)
+        _check(lib().bb_world_snapshot(self._h, None, 0, C.byref(size)), "bb_world_snapshot")
+        buf = C.create_string_buffer(size.value)
+        _check(lib().bb_world_snapshot(self._h, buf, size.value, C.byref(size)), "bb_world_snapshot")
+        return buf.raw[: size.value]
+
+    @classmethod
+    def restore(cls, blob: bytes, warm_start: bool = True) -> "RigidBodyManager":
+        """A world created from a checkpoint (bb_world_restore)."""
+        w = cls.__new__(cls)
+        w._h = C.c_void_p()
+        _check(lib().bb_world_restore(blob, len(blob), 1 if warm_start else 0, C.byref(w._h)), "bb_world_restore")
+        return w
 
     def instance_transforms(self, n: int, device_ptr: int | None = None):
         """Per body id, the 3x4 affine transform [R | t] of its pose (TLAS

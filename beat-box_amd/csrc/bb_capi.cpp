@@ -723,6 +723,139 @@ bb_status instances_batch(bb_batch* b, uint32_t w, float* dst, uint32_t n, int o
   return rc == hipSuccess ? BB_OK : BB_ERR_DEVICE;
 }
 
+// ---- checkpoints.  A world's blob holds what its next step reads from the
+// device (the newest parity's body rows in row order, the id -> row map, the
+// pair list with its per-row offsets, the manifold slots that warm start the
+// next step) plus its counters, island slots (for the read-back), step index
+// and the world and body descriptors the device layout is rebuilt from.
+struct SnapHeader {
+  char magic[8];
+  uint32_t version, n, npairs, warm;
+  uint64_t step_index;
+  bb_world_desc desc;
+};
+constexpr char kSnapMagic[8] = {'B', 'B', 'S', 'N', 'A', 'P', '0', '1'};
+constexpr uint32_t kSnapVersion = 1;
+// bytes of the arrays of n rows and p manifold slots
+size_t snap_bytes(uint32_t n, uint32_t p) {
+  return sizeof(SnapHeader) + (size_t)n * sizeof(bb_body_desc) + (size_t)n * (6 * 4 + 5 * 16) +
+         (size_t)p * (8 + 6 * 16) + (size_t)p * 8 * (6 * 16 + 4) + ST_COUNT * 4;
+}
+
+// The blob's arrays in order, each with its device source (world slice of the
+// newest parity) and size; `fn(dev, bytes)` sees them in blob order.
+template <class F>
+void snap_arrays(bb_batch* b, uint32_t n, uint32_t p, F&& fn) {
+  const HostWorld& hw = b->worlds[0];
+  const uint32_t g0 = hw.body_off, s0 = hw.pair_off;
+  const int L = b->latest;
+  const Rows& R = b->rows[L];
+  const Manifolds& M = b->man[L];
+  fn((void*)(R.id + g0), (size_t)n * 4);
+  fn((void*)(R.flags + g0), (size_t)n * 4);
+  fn((void*)(R.sleep + g0), (size_t)n * 4);
+  fn((void*)(b->entries[L] + g0), (size_t)n * 4);
+  fn((void*)(b->pstart[L] + g0), (size_t)n * 4);
+  fn((void*)(b->base.islot + g0), (size_t)n * 4);
+  fn((void*)(R.pos + g0), (size_t)n * 16);
+  fn((void*)(R.rot + g0), (size_t)n * 16);
+  fn((void*)(R.vel + g0), (size_t)n * 16);
+  fn((void*)(R.omg + g0), (size_t)n * 16);
+  fn((void*)(R.pvel + g0), (size_t)n * 16);
+  fn((void*)(b->pairs[L] + s0), (size_t)p * 8);
+  for (float4* a : {M.nrm, M.t0, M.t1, M.anca, M.ancb}) fn((void*)(a + s0), (size_t)p * 16);
+  fn((void*)(M.hdr + s0), (size_t)p * 16);
+  for (float4* a : {M.cpos, M.clam, M.cmas, M.cana, M.canb}) fn((void*)(a + (size_t)s0 * 8), (size_t)p * 8 * 16);
+  fn((void*)(M.cfp + (size_t)s0 * 8), (size_t)p * 8 * 16);
+  fn((void*)(M.cws + (size_t)s0 * 8), (size_t)p * 8 * 4);
+  fn((void*)(b->base.stats), (size_t)ST_COUNT * 4);
+  (void)hw;
+}
+
+bb_status snapshot_world(bb_batch* b, void* dst, size_t cap, size_t* size) {
+  if (!size || b->worlds.size() != 1) return BB_ERR_INVALID;
+  bb_status st = build(b);
+  if (st != BB_OK) return st;
+  HIPCHK(hipStreamSynchronize(b->stream));
+  const HostWorld& hw = b->worlds[0];
+  const uint32_t n = hw.n;
+  uint32_t p = 0;
+  // warm-start state exists once a step has run and no one-shot drop is pending
+  const bool warm = b->stepped && !hw.drop_prev && !(b->infos[0].mode & kModeDropPrev);
+  if (warm) HIPCHK(hipMemcpy(&p, b->npairs[b->latest], 4, hipMemcpyDeviceToHost));
+  *size = snap_bytes(n, p);
+  if (!dst) return BB_OK;
+  if (cap < *size) return BB_ERR_CAPACITY;
+  SnapHeader h{};
+  std::memcpy(h.magic, kSnapMagic, 8);
+  h.version = kSnapVersion;
+  h.n = n;
+  h.npairs = p;
+  h.warm = (warm ? 1u : 0u) | (hw.skip_ws_once ? 2u : 0u);
+  h.step_index = b->step_index;
+  h.desc = hw.desc;
+  char* o = (char*)dst;
+  std::memcpy(o, &h, sizeof(h));
+  o += sizeof(h);
+  std::memcpy(o, hw.bodies.data(), (size_t)n * sizeof(bb_body_desc));
+  o += (size_t)n * sizeof(bb_body_desc);
+  hipError_t rc = hipSuccess;
+  snap_arrays(b, n, p, [&](void* src, size_t bytes) {
+    if (rc == hipSuccess && bytes) rc = hipMemcpy(o, src, bytes, hipMemcpyDeviceToHost);
+    o += bytes;
+  });
+  return rc == hipSuccess ? BB_OK : BB_ERR_DEVICE;
+}
+
+bb_status restore_world(const void* src, size_t size, int warm_start, bb_world** out) {
+  if (!src || !out || size < sizeof(SnapHeader)) return BB_ERR_INVALID;
+  SnapHeader h;
+  std::memcpy(&h, src, sizeof(h));
+  if (std::memcmp(h.magic, kSnapMagic, 8) != 0 || h.version != kSnapVersion) return BB_ERR_INVALID;
+  if (size != snap_bytes(h.n, h.npairs) || h.npairs > pair_capacity(h.n)) return BB_ERR_INVALID;
+  const char* in = (const char*)src + sizeof(h);
+  bb_world* w = nullptr;
+  bb_status st = bb_world_create(&h.desc, &w);
+  if (st != BB_OK) return st;
+  auto fail = [&](bb_status s) {
+    bb_world_destroy(w);
+    return s;
+  };
+  if (h.n) {
+    st = bb_world_add_bodies(w, (const bb_body_desc*)in, h.n, nullptr);
+    if (st != BB_OK) return fail(st);
+  }
+  in += (size_t)h.n * sizeof(bb_body_desc);
+  bb_batch* b = w->b;
+  st = build(b);
+  if (st != BB_OK) return fail(st);
+  HIPCHK(hipStreamSynchronize(b->stream));
+  hipError_t rc = hipSuccess;
+  snap_arrays(b, h.n, h.npairs, [&](void* dst, size_t bytes) {
+    if (rc == hipSuccess && bytes) rc = hipMemcpy(dst, in, bytes, hipMemcpyHostToDevice);
+    in += bytes;
+  });
+  if (rc != hipSuccess) return fail(BB_ERR_DEVICE);
+  b->step_index = h.step_index;
+  b->stepped = true;
+  HostWorld& hw = b->worlds[0];
+  const bool warm = warm_start && (h.warm & 1u);
+  if (warm) {
+    // the restored pairs and manifolds warm start the next step
+    const uint32_t np = h.npairs;
+    if (hipMemcpy(b->npairs[b->latest], &np, 4, hipMemcpyHostToDevice) != hipSuccess) return fail(BB_ERR_DEVICE);
+    b->infos[0].mode &= ~kModeDropPrev;
+    if (hipMemcpy(b->base.worlds, b->infos.data(), sizeof(WorldInfo), hipMemcpyHostToDevice) != hipSuccess)
+      return fail(BB_ERR_DEVICE);
+    hw.skip_ws_once = (h.warm & 2u) != 0;
+  } else {
+    // like a resume after a pause: the next step starts without warm start
+    hw.skip_ws_once = true;
+  }
+  *out = w;
+  return BB_OK;
+}
+
 bb_status set_param(bb_batch* b, bb_param p, uint32_t value) {
   if (p == BB_PARAM_BLOCK) {
     if (value != 0 && value != 256 && value != 512 && value != 1024) return BB_ERR_INVALID;
@@ -1389,6 +1522,15 @@ bb_status bb_world_energy(bb_world* w, double* out) {
 
 bb_status bb_world_instance_transforms(bb_world* w, float* dst, uint32_t n, int dst_on_device) {
   return w ? instances_batch(w->b, 0, dst, n, dst_on_device) : BB_ERR_INVALID;
+}
+
+bb_status bb_world_snapshot(bb_world* w, void* dst, size_t cap, size_t* size) {
+  return w ? snapshot_world(w->b, dst, cap, size) : BB_ERR_INVALID;
+}
+
+
+bb_status bb_world_restore(const void* src, size_t size, int warm_start, bb_world** out) {
+  return restore_world(src, size, warm_start, out);
 }
 
 bb_status bb_world_debug_tap(bb_world* w, bb_tap kind, void* dst, size_t cap, size_t* written) {

@@ -289,6 +289,25 @@ bb_status bb_world_energy(bb_world* w, double* out);
  * call returns when it is filled. */
 bb_status bb_world_instance_transforms(bb_world* w, float* dst, uint32_t n,
                                        int dst_on_device);
+/* Checkpoint of a world for a later bb_world_restore, in this process or
+ * another.  The reference keeps a paused simulation's buffers only in memory
+ * (StatusManager::switch_simulating, src/status_manager.hpp:133-152, with the
+ * current -> previous copies of AccelerationStructureManager::
+ * record_update_AS_buffers_tasks, src/acceleration_structure_manager.cpp:
+ * 813-948); this blob holds everything the world's next step reads: the
+ * newest body rows in row order, the id -> row map, the pair list with its
+ * per-row offsets, the manifold slots that warm start the next step, the
+ * counters, the step index and the world and body descriptors.  With
+ * dst == NULL, *size receives the bytes needed; otherwise cap must be at
+ * least that (BB_ERR_CAPACITY). */
+bb_status bb_world_snapshot(bb_world* w, void* dst, size_t cap, size_t* size);
+/* Creates a world from a checkpoint.  warm_start != 0: the world continues
+ * bit-identically to the one that wrote the blob.  warm_start == 0: like the
+ * reference's resume after a pause (skip_warm_starting_once,
+ * src/status_manager.hpp:140-146), the next step starts without warm start.
+ * BB_ERR_INVALID for a blob that is not a checkpoint of this version. */
+bb_status bb_world_restore(const void* src, size_t size, int warm_start,
+                           bb_world** out);
 /* RigidBodyManager::destroy (src/rigid_body_manager.cpp:1550-1585). */
 void bb_world_destroy(bb_world* w);
 

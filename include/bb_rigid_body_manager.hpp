@@ -84,6 +84,23 @@ class RigidBodyManager {
     last_ = bb_world_instance_transforms(world_, dst, capacity_bodies, dst_on_device ? 1 : 0);
     return last_ == BB_OK;
   }
+  // Checkpoint of the simulation (bb_world_snapshot) and a manager resumed
+  // from one (bb_world_restore): warm_start keeps the exact continuation,
+  // otherwise the next step starts cold as after the reference's pause.
+  bool snapshot(std::vector<char>& blob) {
+    if (!world_) return false;
+    size_t size = 0;
+    last_ = bb_world_snapshot(world_, nullptr, 0, &size);
+    if (last_ != BB_OK) return false;
+    blob.resize(size);
+    last_ = bb_world_snapshot(world_, blob.data(), blob.size(), &size);
+    return last_ == BB_OK;
+  }
+  bool restore(const std::vector<char>& blob, bool warm_start = true) {
+    destroy();
+    last_ = bb_world_restore(blob.data(), blob.size(), warm_start ? 1 : 0, &world_);
+    return last_ == BB_OK;
+  }
 
   uint32_t get_sim_flags() const { return world_ ? bb_world_get_flags(world_) : desc_.flags; }
   uint32_t set_sim_flags(uint32_t flags) {
