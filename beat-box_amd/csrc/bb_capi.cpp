@@ -65,7 +65,10 @@ WorldInfo info_of(const HostWorld& hw) {
   wi.pair_off = hw.pair_off;
   wi.pair_cap = hw.pair_cap;
   wi.flags = hw.desc.flags;
-  wi.solver = hw.desc.solver == BB_SOLVER_PGS ? (uint32_t)BB_SOLVER_PGS_SOFT : hw.desc.solver;
+  // the colour sweeps implement PGS_SOFT only (RB_sim.slang:1440-1441,
+  // :1462-1463); the per-island path (USE_GRAPH_COLORING off) runs plain PGS
+  const bool coloring = (hw.desc.flags & BB_SIM_USE_GRAPH_COLORING) != 0;
+  wi.solver = (hw.desc.solver == BB_SOLVER_PGS && coloring) ? (uint32_t)BB_SOLVER_PGS_SOFT : hw.desc.solver;
   wi.iterations = hw.desc.iterations;
   wi.morton_bits = hw.desc.morton_bits == 63 ? 63u : 30u;
   wi.quirks = hw.desc.quirks;

@@ -1511,6 +1511,56 @@ BB_DI void iw_refresh(const Dev& d, const WorldInfo& W, const T& t);
 BB_DI bool pgs_solvable(const Dev& d, uint32_t off, int4 h);
 BB_DI void pgs_compact(const Dev& d, const WorldInfo& W, uint32_t k, uint32_t slot);
 
+// Per-row adjacency of a world in CSR form: the manifold slots of each
+// dynamic row in ascending order at adj[2*pair_off + adj_off[row]], deg[row]
+// entries (deg also counts a row's static partners' manifolds; static rows'
+// lists are left unfilled).  `carry` is a team-shared scalar.
+template <class T>
+BB_DI void build_adj(const Dev& d, const WorldInfo& W, uint32_t P, const T& t, uint32_t& carry) {
+  const uint32_t tid = t.rank(), NT = t.size();
+  const uint32_t N = W.n, off = W.body_off;
+  if (tid == 0) carry = 0;
+  t.sync();
+  for (uint32_t c0 = 0; c0 < N; c0 += NT) {
+    uint32_t i = c0 + tid;
+    uint32_t v = i < N ? d.deg[off + i] : 0u;
+    uint32_t tot;
+    uint32_t ex = t.scan(v, tot);
+    if (i < N) {
+      d.adj_off[off + i] = carry + ex;
+      d.pcount[off + i] = 0;  // fill cursor
+    }
+    t.sync();
+    if (tid == 0) carry += tot;
+    t.sync();
+  }
+  uint32_t* adj = d.adj + 2 * W.pair_off;
+  for (uint32_t p = tid; p < P; p += NT) {
+    const uint32_t slot = W.pair_off + p;
+    int4 h = d.man_cur.hdr[slot];
+    if (h.w <= 0) continue;
+    const uint32_t ga = off + h.x, gb = off + h.y;
+    if (d.cur.flags[ga] & BB_BODY_DYNAMIC) adj[d.adj_off[ga] + atomicAdd(&d.pcount[ga], 1u)] = slot;
+    if (d.cur.flags[gb] & BB_BODY_DYNAMIC) adj[d.adj_off[gb] + atomicAdd(&d.pcount[gb], 1u)] = slot;
+  }
+  t.sync();
+  for (uint32_t i = tid; i < N; i += NT) {
+    const uint32_t g = off + i;
+    if (!(d.cur.flags[g] & BB_BODY_DYNAMIC)) continue;
+    uint32_t* l = adj + d.adj_off[g];
+    const uint32_t cnt = d.deg[g];
+    for (uint32_t a = 1; a < cnt; ++a) {
+      uint32_t x = l[a], b = a;
+      while (b > 0 && l[b - 1] > x) {
+        l[b] = l[b - 1];
+        --b;
+      }
+      l[b] = x;
+    }
+  }
+  t.sync();
+}
+
 struct PostShared {
   uint32_t carry;
   uint32_t flag;
@@ -1795,16 +1845,66 @@ BB_DI void s_post(const Dev& d, uint32_t w, const T& t, PostShared& L) {
     d.owner[(off + h.y) * kColors + cc] = 0;
   }
   t.sync();
+  // ---- per-island mode (USE_GRAPH_COLORING off, rigid_body_manager.cpp:
+  // 1390-1399): the reference walks each contact island's manifolds serially
+  // in packed order (= slot order).  That order is kept exactly by parallel
+  // sweeps over dependency levels: a manifold's level is one more than the
+  // level of the previous manifold (lower slot) on either of its dynamic rows,
+  // so the manifolds of one level share no dynamic row and any two that share
+  // one are solved in slot order.  Levels are capped at kColors; the capped
+  // manifolds form the overflow bucket, solved serially in ascending slot
+  // order after the levels (a manifold after an overflow one on a shared row
+  // is itself overflow).  The levels are the longest-path fixpoint over the
+  // rows' ascending slot lists, reached after at most kColors + 1 rounds
+  // (each round settles one more step of every path); they live in `pick`,
+  // free after the colouring.
+  const bool island = (flags & BB_SIM_USE_GRAPH_COLORING) == 0;
+  uint32_t* key = island ? pick : color;
+  if (island) {
+    build_adj(d, W, P, t, L.carry);
+    for (uint32_t p = tid; p < P; p += NT) key[p] = color[p] == kMAX - 1 ? kMAX - 1 : 0u;
+    t.sync();
+    const uint32_t* adj = d.adj + 2 * W.pair_off;
+    uint32_t* lev = d.pick;  // indexed by absolute slot, as adj holds them
+    for (uint32_t round = 0; round <= kColors + 1; ++round) {
+      if (tid == 0) L.flag = 0;
+      t.sync();
+      for (uint32_t i = tid; i < N; i += NT) {
+        const uint32_t g = off + i;
+        const uint32_t cnt = d.deg[g];
+        if (!(d.cur.flags[g] & BB_BODY_DYNAMIC) || cnt < 2) continue;
+        const uint32_t* l = adj + d.adj_off[g];
+        uint32_t lv = __hip_atomic_load(&lev[l[0]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        for (uint32_t k = 1; k < cnt; ++k) {
+          const uint32_t cand = lv + 1 < kColors ? lv + 1 : kColors;
+          const uint32_t old = atomicMax(&lev[l[k]], cand);
+          if (old < cand) {
+            L.flag = 1;
+            lv = cand;
+          } else {
+            lv = old;
+          }
+        }
+      }
+      t.sync();
+      const bool more = L.flag != 0;
+      t.sync();
+      if (!more) break;
+    }
+  }
+  auto bkey = [&](uint32_t p) -> uint32_t {
+    return island ? __hip_atomic_load(&key[p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : color[p];
+  };
   // ---- colour buckets of the solvable manifolds (at least one awake dynamic
   // body; the others are skipped by every sweep, RB_sim.slang:1424).  Order
-  // inside a colour is free (no shared dynamic body); the overflow bucket is
-  // filled in ascending slot order for its serial solve.
+  // inside a colour (level) is free (no shared dynamic body); the overflow
+  // bucket is filled in ascending slot order for its serial solve.
   if (tid == 0) d.stats[w * ST_COUNT + ST_GC_OVF] = L.bcount[kColors];
   t.sync();
   if (tid < kBuckets) L.bcount[tid] = 0;
   t.sync();
   for (uint32_t p = tid; p < P; p += NT) {
-    uint32_t cc = color[p];
+    uint32_t cc = bkey(p);
     if (cc == kMAX - 1) continue;
     int4 h = d.man_cur.hdr[W.pair_off + p];
     if (!pgs_solvable(d, off, h)) continue;
@@ -1824,7 +1924,7 @@ BB_DI void s_post(const Dev& d, uint32_t w, const T& t, PostShared& L) {
   t.sync();
   uint32_t* bucket = d.bucket + W.pair_off;
   for (uint32_t p = tid; p < P; p += NT) {
-    uint32_t cc = color[p];
+    uint32_t cc = bkey(p);
     if (cc >= kColors) continue;
     int4 h = d.man_cur.hdr[W.pair_off + p];
     if (!pgs_solvable(d, off, h)) continue;
@@ -1836,7 +1936,8 @@ BB_DI void s_post(const Dev& d, uint32_t w, const T& t, PostShared& L) {
     for (uint32_t c0 = 0; c0 < P; c0 += NT) {
       uint32_t p = c0 + tid;
       uint32_t f = 0;
-      if (p < P && color[p] != kMAX - 1 && color[p] >= kColors)
+      const uint32_t cc = p < P ? bkey(p) : kMAX - 1;
+      if (cc != kMAX - 1 && cc >= kColors)
         f = pgs_solvable(d, off, d.man_cur.hdr[W.pair_off + p]) ? 1u : 0u;
       uint32_t tot;
       uint32_t ex = t.scan(f, tot);
@@ -1890,9 +1991,17 @@ BB_DI Soft soft_coefs(float dt, float ch) {
 
 struct Pgs {
   bool accum, friction;
+  bool plain;  // PGS (per-island path only), else PGS_SOFT
   float inv_dt;
   Soft soft[2];  // [0]: both bodies movable, [1]: one immovable (2x hertz)
 };
+
+// PGS::pre_step's Baumgarte bias (solvers.slang:28) from the contact's
+// penetration: PENETRATION_FACTOR 0.01 m allowed, BIAS_FACTOR 0.2
+// (shared.inl:82-83).
+BB_DI float plain_bias(float pen, float inv_dt) {
+  return -0.2f * inv_dt * fmn(pen + 0.01f, 0.0f);
+}
 
 // Manifold::compute_basis (src/shaders/extensions.slang:262-272)
 BB_DI void contact_basis(v3 n, v3& t0, v3& t1) {
@@ -2109,15 +2218,21 @@ BB_DI void pgs_row(const Dev& d, const VelCache& vc, uint32_t off, uint32_t k, c
       }
       rv = ((vb + cross(wb, r2)) - va) - cross(wa, r1);
       const float nv = dot(rv, n);
-      float bias = 0.0f, mass_scale = 1.0f, impulse_scale = 0.0f;
-      if (pen > 0.0f) {
-        bias = pen * s.inv_dt;
-      } else if (use_bias) {
-        bias = fmx(bias_factor * pen, -4.0f);
-        mass_scale = mass_coef;
-        impulse_scale = impulse_coef;
+      float imp;
+      if (s.plain) {
+        // PGS::solve (solvers.slang:106-117)
+        imp = nm * (-nv + plain_bias(pen, s.inv_dt));
+      } else {
+        float bias = 0.0f, mass_scale = 1.0f, impulse_scale = 0.0f;
+        if (pen > 0.0f) {
+          bias = pen * s.inv_dt;
+        } else if (use_bias) {
+          bias = fmx(bias_factor * pen, -4.0f);
+          mass_scale = mass_coef;
+          impulse_scale = impulse_coef;
+        }
+        imp = -nm * mass_scale * (nv + bias) - impulse_scale * l.x;
       }
-      float imp = -nm * mass_scale * (nv + bias) - impulse_scale * l.x;
       if (s.accum) {
         const float old = l.x;
         l.x = fmx(old + imp, 0.0f);
@@ -2169,6 +2284,7 @@ BB_DI void s_pgs(const Dev& d, uint32_t w, const T& t, float4* pgs_lds, uint32_t
   Pgs s;
   s.accum = (W.flags & BB_SIM_ACCUM_IMPULSE) != 0;
   s.friction = (W.flags & BB_SIM_FRICTION) != 0;
+  s.plain = W.solver == BB_SOLVER_PGS;
   s.inv_dt = W.dt > 0.0f ? 1.0f / W.dt : 0.0f;
   {
     const float hertz = fmn(30.0f, s.inv_dt * 0.3333333333333333f);
@@ -2176,7 +2292,10 @@ BB_DI void s_pgs(const Dev& d, uint32_t w, const T& t, float4* pgs_lds, uint32_t
     s.soft[1] = soft_coefs(W.dt, 2.0f * hertz);
   }
   const uint32_t* bo = d.bucket_off + w * (kBuckets + 1);
-  const uint32_t used = d.stats[w * ST_COUNT + ST_GC_COUNT];
+  // colours in use, or every level bucket in the per-island mode (empty
+  // buckets are skipped)
+  const uint32_t used =
+      (W.flags & BB_SIM_USE_GRAPH_COLORING) ? d.stats[w * ST_COUNT + ST_GC_COUNT] : kColors;
   const uint32_t base = W.pair_off;
   const uint32_t ovf_lo = bo[kColors], ovf_hi = bo[kBuckets];
   const VelCache vc{pgs_lds, off, N < lcap ? N : lcap};
@@ -2191,6 +2310,7 @@ BB_DI void s_pgs(const Dev& d, uint32_t w, const T& t, float4* pgs_lds, uint32_t
   t.sync();
   auto sweep = [&](int kind) {
     for (uint32_t c = 0; c < used; ++c) {
+      if (bo[c] == bo[c + 1]) continue;
       for (uint32_t k = bo[c] + tid; k < bo[c + 1]; k += NT) pgs_row(d, vc, off, base + k, s, kind);
       t.sync();
     }
@@ -2230,7 +2350,9 @@ BB_DI void s_pgs(const Dev& d, uint32_t w, const T& t, float4* pgs_lds, uint32_t
     pgs_lds[kBodyLds * i + 4] = d.iw[g * 3 + 2];
   }
   t.sync();
-  for (uint32_t it = 0; it < W.iterations; ++it) sweep(2);
+  // relaxation sweeps: PGS_SOFT only (RB_sim.slang:1398-1402)
+  if (!s.plain)
+    for (uint32_t it = 0; it < W.iterations; ++it) sweep(2);
   for (uint32_t i = tid; i < vc.lcap; i += NT) {
     d.cur.vel[off + i] = pgs_lds[kBodyLds * i];
     d.cur.omg[off + i] = pgs_lds[kBodyLds * i + 1];
@@ -2245,8 +2367,17 @@ BB_DI void s_pgs(const Dev& d, uint32_t w, const T& t, float4* pgs_lds, uint32_t
     const float mass_coef = sc.mass_coef, bias_factor = sc.bias_factor, impulse_coef = sc.impulse_coef;
     for (int c = 0; c < (h.z & 255); ++c) {
       const float4 l = d.sv_l[c * NP + row];
+      const float nm = d.sv_r2[c * NP + row].w, tm1 = d.sv_tm1[c * NP + row];
+      if (s.plain) {
+        // PGS::pre_step leaves the soft coefficients as the narrow phase set
+        // them
+        const float mc = d.man_cur.clam[slot * 8 + c].w;
+        d.man_cur.clam[slot * 8 + c] = make_float4(l.x, l.y, l.z, mc);
+        d.man_cur.cmas[slot * 8 + c] = make_float4(nm, l.w, tm1, plain_bias(d.sv_r1[c * NP + row].w, s.inv_dt));
+        continue;
+      }
       d.man_cur.clam[slot * 8 + c] = make_float4(l.x, l.y, l.z, mass_coef);
-      d.man_cur.cmas[slot * 8 + c] = make_float4(d.sv_r2[c * NP + row].w, l.w, d.sv_tm1[c * NP + row], bias_factor);
+      d.man_cur.cmas[slot * 8 + c] = make_float4(nm, l.w, tm1, bias_factor);
       float4 a = d.man_cur.cana[slot * 8 + c];
       a.w = impulse_coef;
       d.man_cur.cana[slot * 8 + c] = a;
@@ -2602,48 +2733,9 @@ BB_DI void s_avbd(const Dev& d, uint32_t w, const T& t, AvbdShared& L) {
   const uint32_t N = W.n, off = W.body_off, P = d.npairs_cur[w];
   const float dt = W.dt;
   // ---- adjacency (CSR, ascending slot per row) for dynamic rows
-  if (tid == 0) L.carry = 0;
-  t.sync();
-  for (uint32_t c0 = 0; c0 < N; c0 += NT) {
-    uint32_t i = c0 + tid;
-    uint32_t v = i < N ? d.deg[off + i] : 0u;
-    uint32_t tot;
-    uint32_t ex = t.scan(v, tot);
-    if (i < N) {
-      d.adj_off[off + i] = L.carry + ex;
-      d.pcount[off + i] = 0;  // fill cursor
-    }
-    t.sync();
-    if (tid == 0) L.carry += tot;
-    t.sync();
-  }
+  build_adj(d, W, P, t, L.carry);
   uint32_t* adj = d.adj + 2 * W.pair_off;
-  for (uint32_t p = tid; p < P; p += NT) {
-    const uint32_t slot = W.pair_off + p;
-    int4 h = d.man_cur.hdr[slot];
-    if (h.w <= 0) continue;
-    const uint32_t ga = off + h.x, gb = off + h.y;
-    if (d.cur.flags[ga] & BB_BODY_DYNAMIC) adj[d.adj_off[ga] + atomicAdd(&d.pcount[ga], 1u)] = slot;
-    if (d.cur.flags[gb] & BB_BODY_DYNAMIC) adj[d.adj_off[gb] + atomicAdd(&d.pcount[gb], 1u)] = slot;
-  }
-  t.sync();
-  for (uint32_t i = tid; i < N; i += NT) {
-    const uint32_t g = off + i;
-    if (!(d.cur.flags[g] & BB_BODY_DYNAMIC)) continue;
-    uint32_t* l = adj + d.adj_off[g];
-    const uint32_t cnt = d.deg[g];
-    for (uint32_t a = 1; a < cnt; ++a) {
-      uint32_t x = l[a], b = a;
-      while (b > 0 && l[b - 1] > x) {
-        l[b] = l[b - 1];
-        --b;
-      }
-      l[b] = x;
-    }
-    d.bcolor[g] = kMAX;
-  }
-  for (uint32_t i = tid; i < N; i += NT)
-    if (!(d.cur.flags[off + i] & BB_BODY_DYNAMIC)) d.bcolor[off + i] = kMAX;
+  for (uint32_t i = tid; i < N; i += NT) d.bcolor[off + i] = kMAX;
   t.sync();
   // ---- Jones-Plassmann body colouring, round-snapshot semantics
   // (entry_avbd_color_round, avbd.slang:846-911)

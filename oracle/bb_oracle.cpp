@@ -24,7 +24,8 @@
 //     written in the same round: avbd.slang:880-891, :691-724).
 // The packed (island-sorted) manifold buffer of the reference is an atomic-
 // order permutation used only by the per-island solver path; here the packed
-// order is the scratch order (identity collision_map).
+// order is the scratch order (identity collision_map), so each island walks
+// its manifolds in ascending scratch order.
 //
 // Build: oracle/Makefile (g++ -O2 -ffp-contract=off -fno-fast-math).
 #include <algorithm>
@@ -969,6 +970,105 @@ static void pgs_solve(Manifold& m, Body& A, Body& B, const SolveCtx& s, bool use
   }
 }
 
+// ============================================================ PGS (plain)
+// PGS::pre_step / solve (src/shaders/simulation/solvers.slang:3-138), used by
+// the per-island path only (RB_sim.slang:1176-1183, :1272-1280).  Arms are
+// taken from the body positions; the bias is Baumgarte's with an allowed
+// penetration.  The reference also writes static bodies; with a static's
+// inverse mass and inertia zero that write leaves a +0 or nonzero velocity as
+// it is, so only dynamic bodies are written here (the reference's own island
+// threads race on a shared static otherwise).
+static const float PENETRATION_FACTOR = 0.01f, BIAS_FACTOR = 0.2f;  // shared.inl:82-83
+static void pgs_plain_pre_step(Manifold& m, Body& A, Body& B, const SolveCtx& s) {
+  m3 Ia = world_inv_inertia(A, s.quirk_inertia), Ib = world_inv_inertia(B, s.quirk_inertia);
+  for (int i = 0; i < m.count; ++i) {
+    Contact& c = m.c[i];
+    f3 r1 = sub(c.position, A.position), r2 = sub(c.position, B.position);
+    f3 rn1 = cross(r1, m.normal), rn2 = cross(r2, m.normal);
+    float kn = A.inv_mass + B.inv_mass;
+    kn += dot(rn1, mulmv(Ia, rn1));
+    kn += dot(rn2, mulmv(Ib, rn2));
+    c.normal_mass = kn == 0.0f ? 0.0f : 1.0f / kn;
+    for (int j = 0; j < 2; ++j) {
+      f3 rt1 = cross(r1, m.t[j]), rt2 = cross(r2, m.t[j]);
+      float kt = A.inv_mass + B.inv_mass;
+      kt += dot(rt1, mulmv(Ia, rt1));
+      kt += dot(rt2, mulmv(Ib, rt2));
+      c.tangent_mass[j] = kt == 0.0f ? 0.0f : 1.0f / kt;
+    }
+    c.bias_factor = -BIAS_FACTOR * s.inv_dt * fminr(c.penetration + PENETRATION_FACTOR, 0.0f);
+    if (s.accum) {
+      f3 P = muls(m.normal, c.normal_impulse);
+      if (s.friction) {
+        for (int j = 0; j < 2; ++j) P = add(P, muls(m.t[j], c.tangent_impulse[j]));
+      }
+      if (is_dyn(A)) {
+        A.velocity = sub(A.velocity, muls(P, A.inv_mass));
+        A.omega = sub(A.omega, mulmv(Ia, cross(r1, P)));
+      }
+      if (is_dyn(B)) {
+        B.velocity = add(B.velocity, muls(P, B.inv_mass));
+        B.omega = add(B.omega, mulmv(Ib, cross(r2, P)));
+      }
+    }
+  }
+}
+
+static void pgs_plain_solve(Manifold& m, Body& A, Body& B, const SolveCtx& s) {
+  m3 Ia = world_inv_inertia(A, s.quirk_inertia), Ib = world_inv_inertia(B, s.quirk_inertia);
+  bool da = is_dyn(A), db = is_dyn(B);
+  float fr = std::sqrt(A.friction * B.friction);
+  for (int k = 0; k < m.count; ++k) {
+    Contact& c = m.c[k];
+    f3 r1 = sub(c.position, A.position), r2 = sub(c.position, B.position);
+    f3 rv = sub(sub(add(B.velocity, cross(B.omega, r2)), A.velocity), cross(A.omega, r1));
+    if (s.friction) {
+      // both tangent rows read the relative velocity taken before the loop
+      // (solvers.slang:75-106)
+      for (int j = 0; j < 2; ++j) {
+        float tv = dot(rv, m.t[j]);
+        float ti = c.tangent_mass[j] * -tv;
+        float maxf = fr * c.normal_impulse;
+        if (s.accum) {
+          float old = c.tangent_impulse[j];
+          c.tangent_impulse[j] = clampf(old + ti, -maxf, maxf);
+          ti = c.tangent_impulse[j] - old;
+        } else {
+          ti = clampf(ti, -maxf, maxf);
+        }
+        f3 tiv = muls(m.t[j], ti);
+        if (da) {
+          A.velocity = sub(A.velocity, muls(tiv, A.inv_mass));
+          A.omega = sub(A.omega, mulmv(Ia, cross(r1, tiv)));
+        }
+        if (db) {
+          B.velocity = add(B.velocity, muls(tiv, B.inv_mass));
+          B.omega = add(B.omega, mulmv(Ib, cross(r2, tiv)));
+        }
+      }
+    }
+    rv = sub(sub(add(B.velocity, cross(B.omega, r2)), A.velocity), cross(A.omega, r1));
+    float nv = dot(rv, m.normal);
+    float imp = c.normal_mass * (-nv + c.bias_factor);
+    if (s.accum) {
+      float old = c.normal_impulse;
+      c.normal_impulse = fmaxr(old + imp, 0.0f);
+      imp = c.normal_impulse - old;
+    } else {
+      imp = fmaxr(imp, 0.0f);
+    }
+    f3 niv = muls(m.normal, imp);
+    if (da) {
+      A.velocity = sub(A.velocity, muls(niv, A.inv_mass));
+      A.omega = sub(A.omega, mulmv(Ia, cross(r1, niv)));
+    }
+    if (db) {
+      B.velocity = add(B.velocity, muls(niv, B.inv_mass));
+      B.omega = add(B.omega, mulmv(Ib, cross(r2, niv)));
+    }
+  }
+}
+
 // ============================================ integrate / advect (extensions)
 // RigidBody::apply_angular_velocity (src/shaders/extensions.slang:148-166)
 static quat apply_angular_velocity(quat q, f3 omega, float dt) {
@@ -1128,7 +1228,11 @@ static void do_step(World& w) {
   const float dt = w.desc.dt, gravity = w.desc.gravity;
   const float damp = std::exp(-dt * 0.1f);
   const uint32_t row_limit = quirk_rows ? std::min<uint32_t>(N, ((A + 31u) / 32u) * 32u) : N;
-  const uint32_t solver = w.desc.solver == BB_SOLVER_PGS ? (uint32_t)BB_SOLVER_PGS_SOFT : w.desc.solver;
+  // The colour-sweep path implements PGS_SOFT only (RB_sim.slang:1440-1441,
+  // :1462-1463): there PGS runs as PGS_SOFT.  The per-island path runs both.
+  const bool coloring = (flags & BB_SIM_USE_GRAPH_COLORING) != 0;
+  const uint32_t solver =
+      (w.desc.solver == BB_SOLVER_PGS && coloring) ? (uint32_t)BB_SOLVER_PGS_SOFT : w.desc.solver;
 
   // previous-step snapshot (the other parity)
   if (w.drop_prev) { w.prev_manifolds.clear(); w.drop_prev = false; }
@@ -1366,8 +1470,9 @@ static void do_step(World& w) {
   sc.inv_dt = dt > 0.0f ? 1.0f / dt : 0.0f;
   sc.hertz = fminr(30.0f, sc.inv_dt * 0.3333333333333333f);
 
-  if (solver == BB_SOLVER_PGS_SOFT) {
-    // edge colouring (coloring.slang:44-292)
+  if (solver != BB_SOLVER_AVBD) {
+    // edge colouring (coloring.slang:44-292); recorded whether or not the
+    // sweeps use it (rigid_body_manager.cpp:1332-1342)
     auto gc_hash = [](uint32_t x) {
       x ^= x >> 16; x *= 0x7feb352du; x ^= x >> 15; x *= 0x846ca68bu; x ^= x >> 16;
       return x;
@@ -1430,17 +1535,35 @@ static void do_step(World& w) {
     st.graph_color_overflow = ovf;
 
     // colour sweeps (RB_sim.slang:1408-1578, rigid_body_manager.cpp:1376-1389)
+    // or, with USE_GRAPH_COLORING off, the per-island passes CPS / CS / CSR
+    // (RB_sim.slang:1122-1406, rigid_body_manager.cpp:1390-1399): one thread
+    // per contact island walks the island's manifolds in packed order.  Every
+    // manifold with a dynamic body belongs to that body's island, so islands
+    // share no dynamic body and the island-by-island walk equals one walk
+    // over all manifolds in packed (= scratch) order.
     std::vector<std::vector<uint32_t>> bucket(MAX_COLORS + 1);
-    for (uint32_t k = 0; k < M; ++k) bucket[color[k] < MAX_COLORS ? color[k] : MAX_COLORS].push_back(k);
+    if (coloring) {
+      for (uint32_t k = 0; k < M; ++k) bucket[color[k] < MAX_COLORS ? color[k] : MAX_COLORS].push_back(k);
+    } else {
+      for (uint32_t k = 0; k < M; ++k) bucket[0].push_back(k);
+    }
+    const bool plain = solver == BB_SOLVER_PGS;
     auto run = [&](int kind) {
+      if (plain && kind == 2) return;  // CSR relaxes PGS_SOFT only (RB_sim.slang:1398-1402)
       for (uint32_t c = 0; c <= MAX_COLORS; ++c) {
         for (uint32_t k : bucket[c]) {
           Manifold& m = ms[k];
           Body& A_ = w.rows[m.a];
           Body& B_ = w.rows[m.b];
           if (!awake_dyn(A_) && !awake_dyn(B_)) continue;
-          if (kind == 0) pgs_pre_step(m, A_, B_, sc);
-          else pgs_solve(m, A_, B_, sc, kind == 1);
+          if (plain) {
+            if (kind == 0) pgs_plain_pre_step(m, A_, B_, sc);
+            else pgs_plain_solve(m, A_, B_, sc);
+          } else if (kind == 0) {
+            pgs_pre_step(m, A_, B_, sc);
+          } else {
+            pgs_solve(m, A_, B_, sc, kind == 1);
+          }
         }
       }
     };

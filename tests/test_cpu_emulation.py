@@ -127,3 +127,36 @@ def test_two_lanes_equal_one_lane(emu_main, tmp_path):
         a, x = _run(emu_main, scene, str(tmp_path / "a.bin"), 5, 0, 0, graph=graph, lane_bodies=0)
         b, y = _run(emu_main, scene, str(tmp_path / "b.bin"), 5, 0, 0, graph=graph, lane_bodies=300)
         assert a == b and x == y, graph
+
+
+def test_emulated_island_path_equals_oracle(emu_main, tmp_path):
+    """USE_GRAPH_COLORING off: the per-island path (plain PGS and PGS_SOFT),
+    its dependency-level sweeps and serial overflow reproduce the oracle's
+    serial walk bit for bit, next to a colour-sweep world in the same batch."""
+    from oracle_bind import OracleWorld
+
+    island = scenes.SIM_DEFAULT_FLAGS & ~scenes.SIM_USE_GRAPH_COLORING
+    plain = beatbox.WorldDesc.make(**scenes.world_desc(solver=scenes.SOLVER_PGS, iterations=8, flags=island,
+                                                       morton_bits=63, quirks=0))
+    soft = beatbox.WorldDesc.make(**scenes.world_desc(iterations=6, flags=island, morton_bits=63, quirks=0))
+    parity = beatbox.WorldDesc.make(**scenes.world_desc(solver=scenes.SOLVER_PGS, iterations=8, flags=island))
+    colour = beatbox.WorldDesc.make(**scenes.world_desc(iterations=8, morton_bits=63, quirks=0))
+    worlds = [(plain, scenes.serving_pile(41, 300)), (soft, scenes.batch_world(0xBB04, 6, 400)),
+              (parity, scenes.random_boxes(0xBB13, 96, extent=3.0, y0=0.55, y1=5.0)),
+              (colour, scenes.serving_pile(42, 200))]
+    scene = str(tmp_path / "scene.bin")
+    _scene(scene, worlds)
+    steps = 8
+    for team in (0, 250):
+        _, raw = _run(emu_main, scene, str(tmp_path / "o.bin"), steps, team, 0)
+        got = np.frombuffer(raw, dtype=beatbox.BODY_STATE)
+        at = 0
+        for d, bodies in worlds:
+            o = OracleWorld(d)
+            o.add_bodies(bodies)
+            o.step(steps)
+            ref = o.read_bodies()
+            mine = got[at:at + len(bodies)]
+            at += len(bodies)
+            for f in ("position", "rotation", "velocity", "omega", "flags", "id"):
+                assert mine[f].tobytes() == ref[f].tobytes(), (team, f)

@@ -497,3 +497,126 @@ def test_avbd_colouring_valid():
         a, b = int(mm["obb1_index"]), int(mm["obb2_index"])
         if dyn[a] and dyn[b]:
             assert bc[a] != bc[b]
+
+
+# ------------------------------------------------- per-island path, plain PGS
+ISLAND_FLAGS = scenes.SIM_DEFAULT_FLAGS & ~scenes.SIM_USE_GRAPH_COLORING
+
+
+def _sphere_column(masses, speeds, overlap=0.03):
+    """Ground plus unit-diameter spheres stacked on the y axis, each pair and
+    the lowest sphere with the ground overlapping by `overlap`; no gravity,
+    vertical initial velocities.  Every contact normal is vertical and every
+    arm parallel to it, so the solve reduces to 1-D impulses."""
+    g = np.zeros(1, scenes.BODY_DESC)
+    scenes._ground(g[0])
+    parts = [g]
+    for k, (m, v) in enumerate(zip(masses, speeds)):
+        s = _sphere((0.0, 0.5 - overlap + k * (1.0 - overlap), 0.0), 0.5, mass=m)
+        s["flags"] = scenes.BODY_DYNAMIC
+        s["velocity"] = (0.0, v, 0.0)
+        parts.append(s)
+    return np.concatenate(parts)
+
+
+def _column_model(w, bodies, iterations, order, accum=True):
+    """Sequential impulses of PGS::pre_step / solve in 1-D over the
+    manifolds in the given order (first step, nothing to warm-start):
+    effective mass 1/(1/ma + 1/mb), Baumgarte bias 0.2/dt * (depth - 0.01),
+    accumulated impulses clamped at zero."""
+    m = w.tap(TAP_MANIFOLDS, MANIFOLD_TAP)
+    ids = w.read_bodies(order=beatbox.ORDER_BY_ROW)["id"]
+    dt = float(np.float32(0.01667))
+    damp = math.exp(-dt * 0.1)
+    v = {int(i): float(bodies["velocity"][i][1]) * damp for i in range(len(bodies))}
+    inv = {int(i): (1.0 / float(bodies["mass"][i]) if bodies["flags"][i] & scenes.BODY_DYNAMIC else 0.0)
+           for i in range(len(bodies))}
+    rows = []
+    for t in m:
+        assert t["contact_count"] == 1
+        n = t["normal"]
+        assert n[0] == 0.0 and n[2] == 0.0 and abs(n[1]) == 1.0
+        rows.append((int(ids[t["obb1_index"]]), int(ids[t["obb2_index"]]), float(n[1]),
+                     float(t["contacts"]["penetration"][0])))
+    lam = [0.0] * len(rows)
+    for _ in range(iterations):
+        for k in order:
+            a, b, s, pen = rows[k]
+            nm = 1.0 / (inv[a] + inv[b])
+            bias = -0.2 / dt * min(pen + 0.01, 0.0)
+            imp = nm * (-(s * (v[b] - v[a])) + bias)
+            if accum:
+                new = max(lam[k] + imp, 0.0)
+                imp, lam[k] = new - lam[k], new
+            else:
+                imp = max(imp, 0.0)
+            v[a] -= s * imp * inv[a]
+            v[b] += s * imp * inv[b]
+    return v, lam, m
+
+
+@pytest.mark.parametrize("accum", [True, False])
+def test_island_plain_pgs_is_serial_gauss_seidel(accum):
+    """USE_GRAPH_COLORING off with PGS: one step of a sphere column equals
+    1-D sequential impulses in packed (pair) order, and not in any other
+    order nor with PGS_SOFT's soft constraint, so the check sees both the
+    solver formula and the walk order."""
+    bodies = _sphere_column([1.0, 2.0, 0.5, 3.0, 1.0], [-1.0, 0.5, -2.0, 0.7, -1.5])
+    flags = ISLAND_FLAGS if accum else ISLAND_FLAGS & ~1
+    w = _world(bodies, solver=scenes.SOLVER_PGS, iterations=3, flags=flags, morton_bits=63, quirks=0)
+    w.step(1)
+    r = w.read_bodies()
+    M = len(w.tap(TAP_MANIFOLDS, MANIFOLD_TAP))
+    assert M == 5
+    v, lam, m = _column_model(w, bodies, 3, range(M), accum)
+    got = {int(i): float(r["velocity"][i][1]) for i in range(len(bodies))}
+    for i in range(1, len(bodies)):
+        assert abs(got[i] - v[i]) < 1e-4, (i, got[i], v[i])
+        assert r["velocity"][i][0] == 0.0 and r["velocity"][i][2] == 0.0
+    if accum:
+        assert np.allclose(m["contacts"]["normal_impulse"][:, 0], lam, atol=1e-4)
+    # another walk order gives a different answer
+    v2, _, _ = _column_model(w, bodies, 3, list(reversed(range(M))), accum)
+    assert max(abs(v2[i] - v[i]) for i in range(1, len(bodies))) > 1e-2
+    # and so does PGS_SOFT on the same path
+    s = _world(bodies, solver=scenes.SOLVER_PGS_SOFT, iterations=3, flags=flags, morton_bits=63, quirks=0)
+    s.step(1)
+    assert np.abs(s.read_bodies()["velocity"] - r["velocity"]).max() > 1e-2
+
+
+def test_colour_path_runs_pgs_as_pgs_soft_and_island_path_does_not():
+    """With colouring on, PGS is PGS_SOFT (the reference's colour sweeps
+    implement only that); with it off the two solvers differ."""
+    bodies = scenes.random_boxes(0xBB21, 48, extent=2.0, y0=0.55, y1=4.0)
+    out = {}
+    for solver in (scenes.SOLVER_PGS, scenes.SOLVER_PGS_SOFT):
+        for flags in (scenes.SIM_DEFAULT_FLAGS, ISLAND_FLAGS):
+            w = _world(bodies, solver=solver, iterations=8, flags=flags)
+            w.step(6)
+            out[solver, flags] = w.read_bodies()
+    a, b = out[scenes.SOLVER_PGS, scenes.SIM_DEFAULT_FLAGS], out[scenes.SOLVER_PGS_SOFT, scenes.SIM_DEFAULT_FLAGS]
+    assert a.tobytes() == b.tobytes()
+    c, d = out[scenes.SOLVER_PGS, ISLAND_FLAGS], out[scenes.SOLVER_PGS_SOFT, ISLAND_FLAGS]
+    assert np.abs(c["velocity"] - d["velocity"]).max() > 1e-3
+    assert np.abs(d["velocity"] - b["velocity"]).max() > 1e-3
+
+
+def test_island_path_islands_do_not_interact():
+    """Two sphere columns far apart in one world (two contact islands) step
+    exactly as each column alone: the walk over all manifolds in packed order
+    equals one serial walk per island."""
+    a = _sphere_column([1.0, 2.0, 0.5], [-1.0, 0.5, -2.0])
+    b = _sphere_column([3.0, 1.0], [0.7, -1.5])[1:]
+    b["position"][:, 0] += 20.0
+    both = np.concatenate([a, b])
+    w = _world(both, solver=scenes.SOLVER_PGS, iterations=3, flags=ISLAND_FLAGS, morton_bits=63, quirks=0)
+    w.step(3)
+    r = w.read_bodies()
+    for part, sl in ((a, slice(0, len(a))), (np.concatenate([a[:1], b]), None)):
+        x = _world(part, solver=scenes.SOLVER_PGS, iterations=3, flags=ISLAND_FLAGS, morton_bits=63, quirks=0)
+        x.step(3)
+        rx = x.read_bodies()
+        if sl is not None:
+            assert rx["velocity"].tobytes() == r["velocity"][sl].tobytes()
+        else:
+            assert rx["velocity"][1:].tobytes() == r["velocity"][len(a):].tobytes()
