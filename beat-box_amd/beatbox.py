@@ -99,6 +99,48 @@ class ServerReport(C.Structure):
 
 REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_UNKNOWN = 0, 1, 2, 3
 
+
+class VoxelShapeDesc(C.Structure):
+    _fields_ = [("dims", C.c_uint32 * 3), ("voxel_size", C.c_float), ("density", C.c_float),
+                ("occupancy", C.POINTER(C.c_uint32))]
+
+
+class VoxelShapeInfo(C.Structure):
+    _fields_ = [("shape_id", C.c_uint32), ("solid_count", C.c_uint32), ("surface_count", C.c_uint32),
+                ("mass", C.c_float), ("inv_inertia", C.c_float * 9), ("minimum", C.c_float * 3),
+                ("maximum", C.c_float * 3)]
+
+    def as_dict(self) -> dict:
+        return {"shape_id": self.shape_id, "solid_count": self.solid_count,
+                "surface_count": self.surface_count, "mass": self.mass,
+                "inv_inertia": np.array(self.inv_inertia, np.float32).reshape(3, 3),
+                "minimum": np.array(self.minimum, np.float32), "maximum": np.array(self.maximum, np.float32)}
+
+
+SHAPE_BOX, SHAPE_SPHERE, SHAPE_VOXEL = 0, 1, 2
+
+
+def voxel_shape_word(shape_id: int) -> int:
+    """The body shape word of voxel shape `shape_id` (BB_SHAPE_VOXEL_OF)."""
+    return SHAPE_VOXEL | (int(shape_id) << 8)
+
+
+def voxel_desc(solid: np.ndarray, voxel_size: float, density: float):
+    """A VoxelShapeDesc from a boolean grid solid[x, y, z]; returns the
+    descriptor and the occupancy words it points at (keep both alive)."""
+    solid = np.asarray(solid, bool)
+    dims = solid.shape
+    bits = solid.flatten(order="F")  # x fastest: bit x + y*dx + z*dx*dy
+    pad = (-len(bits)) % 32
+    bits = np.concatenate([bits, np.zeros(pad, bool)])
+    words = np.ascontiguousarray(np.packbits(bits, bitorder="little").view("<u4"))
+    d = VoxelShapeDesc()
+    d.dims[:] = [int(x) for x in dims]
+    d.voxel_size = voxel_size
+    d.density = density
+    d.occupancy = words.ctypes.data_as(C.POINTER(C.c_uint32))
+    return d, words
+
 LBVH_NODE = np.dtype([("minimum", "<f4", 3), ("maximum", "<f4", 3), ("left", "<i4"),
                       ("right", "<i4"), ("primitive_idx", "<u4"), ("pad", "<u4")])
 CONTACT_TAP = np.dtype([
@@ -146,6 +188,8 @@ EXPORTS = {
     "bb_world_create": (C.c_int, [C.POINTER(WorldDesc), C.POINTER(C.c_void_p)]),
     "bb_world_add_bodies": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]),
     "bb_world_set_flags": (C.c_int, [C.c_void_p, C.c_uint32]),
+    "bb_world_add_voxel_shape": (C.c_int, [C.c_void_p, C.POINTER(VoxelShapeDesc), C.POINTER(VoxelShapeInfo)]),
+    "bb_batch_add_voxel_shape": (C.c_int, [C.c_void_p, C.POINTER(VoxelShapeDesc), C.POINTER(VoxelShapeInfo)]),
     "bb_world_clear_flags": (C.c_int, [C.c_void_p, C.c_uint32]),
     "bb_world_get_flags": (C.c_uint32, [C.c_void_p]),
     "bb_world_set_solver": (C.c_int, [C.c_void_p, C.c_uint32]),
@@ -255,6 +299,15 @@ class RigidBodyManager:
     # for the upload that load_scene ends with
     build_accel_structs = add_bodies
 
+    def add_voxel_shape(self, solid: np.ndarray, voxel_size: float, density: float) -> dict:
+        """SceneManager::build_voxel_shape: solid[x, y, z] cells; returns the
+        shape id (bodies use shape = voxel_shape_word(id)) and mass data."""
+        d, words = voxel_desc(solid, voxel_size, density)
+        info = VoxelShapeInfo()
+        _check(lib().bb_world_add_voxel_shape(self._h, C.byref(d), C.byref(info)), "bb_world_add_voxel_shape")
+        del words
+        return info.as_dict()
+
     def simulate(self, nsteps: int = 1) -> None:
         _check(lib().bb_world_step(self._h, nsteps, None), "bb_world_step")
 
@@ -360,6 +413,14 @@ class Batch:
         self._h = C.c_void_p()
         _check(lib().bb_batch_create(capacity_worlds, capacity_bodies, C.byref(self._h)),
                "bb_batch_create")
+
+    def add_voxel_shape(self, solid: np.ndarray, voxel_size: float, density: float) -> dict:
+        """A voxel shape in the batch's pools (shared by all its worlds)."""
+        d, words = voxel_desc(solid, voxel_size, density)
+        info = VoxelShapeInfo()
+        _check(lib().bb_batch_add_voxel_shape(self._h, C.byref(d), C.byref(info)), "bb_batch_add_voxel_shape")
+        del words
+        return info.as_dict()
 
     def add_world(self, desc: WorldDesc, bodies: np.ndarray) -> int:
         arr, ptr = _bodies_ptr(bodies)

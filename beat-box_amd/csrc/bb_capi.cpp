@@ -76,7 +76,7 @@ WorldInfo info_of(const HostWorld& hw) {
   wi.dt = hw.desc.dt;
   wi.gravity = hw.desc.gravity;
   wi.damp = std::exp(-hw.desc.dt * 0.1f);
-  wi.mode = 0;
+  wi.mode = hw.has_voxel ? kModeVoxel : 0u;
   return wi;
 }
 
@@ -213,6 +213,13 @@ bb_status alloc_arrays(bb_batch* b, Buffers& A, uint32_t W, uint32_t NB, uint32_
   d.team_hist = A.alloc<uint32_t>((size_t)kMaxTeamBlocks * 256);
   d.team_shared = A.alloc<char>((size_t)kMaxTeams * kTeamSharedBytes);
   d.team_grid = 0;
+  b->d_vshape = A.alloc<VoxShape>(BB_MAX_VOXEL_SHAPES);
+  b->d_vocc = A.alloc<uint32_t>(BB_MAX_VOXEL_OCC_WORDS);
+  b->d_vsurf = A.alloc<uint32_t>(BB_MAX_VOXEL_SURFACE);
+  d.vshape = b->d_vshape;
+  d.vocc = b->d_vocc;
+  d.vsurf = b->d_vsurf;
+  b->vox_dirty = true;
   if (A.count_only) return BB_OK;
   if (std::find(A.ptrs.begin(), A.ptrs.end(), nullptr) != A.ptrs.end()) return BB_ERR_CAPACITY;
   HIPCHK(hipMemset(d.teams, 0, sizeof(TeamTable)));
@@ -267,7 +274,8 @@ Images carve_images(void* mem, uint32_t nb, uint32_t na) {
   return img;
 }
 
-void fill_images(const HostWorld& hw, uint32_t r0, uint32_t a0, const Images& img) {
+void fill_images(const HostWorld& hw, uint32_t r0, uint32_t a0, const Images& img,
+                 const std::vector<bb_voxel_shape_info>* vox) {
   uint32_t k = 0;
   for (uint32_t i = 0; i < hw.n; ++i) {
     const bb_body_desc& bd = hw.bodies[i];
@@ -280,6 +288,28 @@ void fill_images(const HostWorld& hw, uint32_t r0, uint32_t a0, const Images& im
     img.vel[g] = make_float4(bd.velocity[0], bd.velocity[1], bd.velocity[2], 0.0f);
     img.omg[g] = make_float4(bd.omega[0], bd.omega[1], bd.omega[2], 0.0f);
     const float r = bd.radius;
+    if (BB_SHAPE_KIND(bd.shape) == BB_SHAPE_VOXEL && vox) {
+      // push_voxel_body (src/scene_manager.hpp:568-572): the shape's bounds,
+      // mass and inverse inertia; immovable (mass 0) without DYNAMIC.  pmax.w
+      // carries the shape word (exact in float below 2^24).
+      const bb_voxel_shape_info& vi = (*vox)[BB_SHAPE_VOXEL_ID(bd.shape) - 1u];
+      const bool dyn = (bd.flags & BB_BODY_DYNAMIC) != 0;
+      img.pmin[g] = make_float4(vi.minimum[0], vi.minimum[1], vi.minimum[2], 0.0f);
+      img.pmax[g] = make_float4(vi.maximum[0], vi.maximum[1], vi.maximum[2], (float)bd.shape);
+      img.mass[g] = make_float4(dyn ? vi.mass : 0.0f, dyn ? 1.0f / vi.mass : 0.0f, bd.friction, bd.restitution);
+      const float* I = vi.inv_inertia;
+      img.ii0[g] = dyn ? make_float4(I[0], I[1], I[2], 0.0f) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      img.ii1[g] = dyn ? make_float4(I[3], I[4], I[5], 0.0f) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      img.ii2[g] = dyn ? make_float4(I[6], I[7], I[8], 0.0f) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      if (dyn) {
+        img.rank[g] = k;
+        img.active_ids[a0 + k] = i;
+        ++k;
+      } else {
+        img.rank[g] = kMAX;
+      }
+      continue;
+    }
     if (bd.shape == BB_SHAPE_SPHERE) {
       img.pmin[g] = make_float4(-r, -r, -r, r);
       img.pmax[g] = make_float4(r, r, r, 1.0f);
@@ -355,7 +385,8 @@ uint32_t assign_teams(bb_batch* b, uint32_t grid_cap) {
   grid_cap = std::min(grid_cap, kMaxTeamBlocks);
   std::vector<uint32_t> cand;
   for (uint32_t w = 0; w < (uint32_t)b->worlds.size(); ++w)
-    if (b->infos[w].n >= b->team_bodies && b->infos[w].n > 0) cand.push_back(w);
+    if (b->infos[w].n >= b->team_bodies && b->infos[w].n > 0 && !(b->infos[w].mode & kModeVoxel))
+      cand.push_back(w);  // voxel worlds run one workgroup per world
   // largest worlds first; each team gets at least two workgroups
   std::sort(cand.begin(), cand.end(), [&](uint32_t x, uint32_t y) {
     return b->infos[x].n != b->infos[y].n ? b->infos[x].n > b->infos[y].n : x < y;
@@ -407,7 +438,7 @@ bb_status upload_teams(bb_batch* b) {
 }
 
 bb_status build(bb_batch* b) {
-  if (!b->dirty) return BB_OK;
+  if (!b->dirty) return upload_voxels(b);
   if (!has_device()) return BB_ERR_NO_DEVICE;
   bb_status st = pull_state(b);
   if (st != BB_OK) return st;
@@ -431,7 +462,11 @@ bb_status build(bb_batch* b) {
     hw.body_off = NB;
     hw.act_off = NA;
     hw.pair_off = NP;
-    hw.pair_cap = pair_capacity(hw.n);
+    hw.has_voxel = false;
+    for (auto& bd : hw.bodies)
+      if (BB_SHAPE_KIND(bd.shape) == BB_SHAPE_VOXEL) hw.has_voxel = true;
+    // a pair with a voxel body takes six slots, one per contact-face code
+    hw.pair_cap = pair_capacity(hw.n) * (hw.has_voxel ? 6u : 1u);
     NB += hw.n;
     NA += hw.a;
     NP += hw.pair_cap;
@@ -451,11 +486,13 @@ bb_status build(bb_batch* b) {
     WorldInfo wi = info_of(hw);
     wi.mode |= kModeDropPrev;  // fresh layout: no previous manifolds
     b->infos[w] = wi;
-    fill_images(hw, hw.body_off, hw.act_off, img);
+    fill_images(hw, hw.body_off, hw.act_off, img, &b->vox_info);
   }
   st = upload_images(b, img, 0, NB, 0, NA, b->stream);
   if (st != BB_OK) return st;
   HIPCHK(hipStreamSynchronize(b->stream));
+  st = upload_voxels(b);
+  if (st != BB_OK) return st;
   st = upload_teams(b);  // also uploads the world table
   if (st != BB_OK) return st;
   b->latest = 1;  // the first step reads set 1 and writes set 0
@@ -1278,6 +1315,143 @@ bool valid_body(const bb_body_desc& b) {
   return true;
 }
 
+bool valid_body_with_voxels(const bb_body_desc& b, size_t shapes) {
+  if (BB_SHAPE_KIND(b.shape) != BB_SHAPE_VOXEL) return valid_body(b);
+  const uint32_t id = BB_SHAPE_VOXEL_ID(b.shape);
+  if (id == 0 || id > shapes) return false;
+  for (int k = 0; k < 3; ++k)
+    if (!std::isfinite(b.position[k]) || !std::isfinite(b.velocity[k]) || !std::isfinite(b.omega[k])) return false;
+  for (int k = 0; k < 4; ++k)
+    if (!std::isfinite(b.rotation[k])) return false;
+  return true;
+}
+
+// SceneManager::build_voxel_shape (src/scene_manager.hpp:478-566) with the
+// solid(x,y,z) predicate given as occupancy bits.  Centre of mass in double;
+// per-cell inertia (parallel-axis point term plus the cell's own cube term)
+// accumulated in float in z, y, x order; glm::inverse of the 3x3 (adjugate
+// over the determinant); the surface list keeps cells with an empty face
+// neighbour, coded with the first empty direction (-x,+x,-y,+y,-z,+z).
+bb_status add_voxel_shape(bb_batch* b, const bb_voxel_shape_desc* d, bb_voxel_shape_info* out) {
+  if (!d || !out || !d->occupancy) return BB_ERR_INVALID;
+  const uint32_t dx = d->dims[0], dy = d->dims[1], dz = d->dims[2];
+  if (dx == 0 || dy == 0 || dz == 0 || dx > 255 || dy > 255 || dz > 255) return BB_ERR_INVALID;
+  if (!(d->voxel_size > 0.0f) || !std::isfinite(d->voxel_size) || !(d->density > 0.0f) ||
+      !std::isfinite(d->density))
+    return BB_ERR_INVALID;
+  const uint32_t* occ = d->occupancy;
+  auto solid = [&](int x, int y, int z) -> bool {
+    if (x < 0 || y < 0 || z < 0 || x >= (int)dx || y >= (int)dy || z >= (int)dz) return false;
+    const uint32_t bit = (uint32_t)x + (uint32_t)y * dx + (uint32_t)z * dx * dy;
+    return ((occ[bit >> 5] >> (bit & 31u)) & 1u) != 0u;
+  };
+  const float vs = d->voxel_size;
+  double com[3] = {0.0, 0.0, 0.0};
+  uint32_t count = 0;
+  for (uint32_t z = 0; z < dz; ++z)
+    for (uint32_t y = 0; y < dy; ++y)
+      for (uint32_t x = 0; x < dx; ++x)
+        if (solid((int)x, (int)y, (int)z)) {
+          com[0] += x + 0.5;
+          com[1] += y + 0.5;
+          com[2] += z + 0.5;
+          ++count;
+        }
+  if (count == 0) return BB_ERR_INVALID;
+  for (int k = 0; k < 3; ++k) com[k] = com[k] / (double)count * (double)vs;
+  const float voxel_mass = d->density * vs * vs * vs;
+  const float mass = voxel_mass * (float)count;
+  const float org[3] = {-(float)com[0], -(float)com[1], -(float)com[2]};
+  const uint32_t cells = dx * dy * dz, words = (cells + 31u) / 32u;
+  std::vector<uint32_t> surf;
+  float I[3][3] = {{0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}};
+  const float own = voxel_mass * vs * vs / 6.0f;
+  static const int dirs[6][3] = {{-1, 0, 0}, {1, 0, 0}, {0, -1, 0}, {0, 1, 0}, {0, 0, -1}, {0, 0, 1}};
+  for (uint32_t z = 0; z < dz; ++z)
+    for (uint32_t y = 0; y < dy; ++y)
+      for (uint32_t x = 0; x < dx; ++x) {
+        if (!solid((int)x, (int)y, (int)z)) continue;
+        const float c[3] = {org[0] + ((float)x + 0.5f) * vs, org[1] + ((float)y + 0.5f) * vs,
+                            org[2] + ((float)z + 0.5f) * vs};
+        const float r2 = (c[0] * c[0] + c[1] * c[1]) + c[2] * c[2];
+        for (int i = 0; i < 3; ++i)
+          for (int j = 0; j < 3; ++j) I[i][j] = I[i][j] + voxel_mass * ((i == j ? r2 : 0.0f) - c[j] * c[i]);
+        for (int i = 0; i < 3; ++i)
+          for (int j = 0; j < 3; ++j) I[i][j] = I[i][j] + (i == j ? own : 0.0f);
+        for (uint32_t k = 0; k < 6; ++k)
+          if (!solid((int)x + dirs[k][0], (int)y + dirs[k][1], (int)z + dirs[k][2])) {
+            surf.push_back(x | (y << 8) | (z << 16) | (k << 24));
+            break;
+          }
+      }
+  if (b->vox_shapes.size() >= BB_MAX_VOXEL_SHAPES || b->vox_occ.size() + words > BB_MAX_VOXEL_OCC_WORDS ||
+      b->vox_surf.size() + surf.size() > BB_MAX_VOXEL_SURFACE)
+    return BB_ERR_CAPACITY;
+  // glm::inverse: m is column-major there; the inertia is symmetric, so its
+  // columns are its rows
+  const float(&m)[3][3] = I;
+  const float det = (m[0][0] * (m[1][1] * m[2][2] - m[2][1] * m[1][2]) -
+                     m[1][0] * (m[0][1] * m[2][2] - m[2][1] * m[0][2])) +
+                    m[2][0] * (m[0][1] * m[1][2] - m[1][1] * m[0][2]);
+  const float k1 = 1.0f / det;
+  float inv[3][3];  // [column][row]
+  inv[0][0] = (m[1][1] * m[2][2] - m[2][1] * m[1][2]) * k1;
+  inv[1][0] = -(m[1][0] * m[2][2] - m[2][0] * m[1][2]) * k1;
+  inv[2][0] = (m[1][0] * m[2][1] - m[2][0] * m[1][1]) * k1;
+  inv[0][1] = -(m[0][1] * m[2][2] - m[2][1] * m[0][2]) * k1;
+  inv[1][1] = (m[0][0] * m[2][2] - m[2][0] * m[0][2]) * k1;
+  inv[2][1] = -(m[0][0] * m[2][1] - m[2][0] * m[0][1]) * k1;
+  inv[0][2] = (m[0][1] * m[1][2] - m[1][1] * m[0][2]) * k1;
+  inv[1][2] = -(m[0][0] * m[1][2] - m[1][0] * m[0][2]) * k1;
+  inv[2][2] = (m[0][0] * m[1][1] - m[1][0] * m[0][1]) * k1;
+  VoxShape vsh{};
+  vsh.dims[0] = dx;
+  vsh.dims[1] = dy;
+  vsh.dims[2] = dz;
+  vsh.occ_off = (uint32_t)b->vox_occ.size();
+  vsh.vs = vs;
+  vsh.ox = org[0];
+  vsh.oy = org[1];
+  vsh.oz = org[2];
+  vsh.surf_off = (uint32_t)b->vox_surf.size();
+  vsh.surf_count = (uint32_t)surf.size();
+  const size_t o0 = b->vox_occ.size();
+  b->vox_occ.insert(b->vox_occ.end(), occ, occ + words);
+  if (cells % 32u) b->vox_occ[o0 + words - 1] &= (1u << (cells % 32u)) - 1u;
+  b->vox_surf.insert(b->vox_surf.end(), surf.begin(), surf.end());
+  b->vox_shapes.push_back(vsh);
+  bb_voxel_shape_info info{};
+  info.shape_id = (uint32_t)b->vox_shapes.size();
+  info.solid_count = count;
+  info.surface_count = vsh.surf_count;
+  info.mass = mass;
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) info.inv_inertia[r * 3 + c] = inv[r][c];
+  for (int k = 0; k < 3; ++k) {
+    info.minimum[k] = org[k];
+    info.maximum[k] = org[k] + (float)d->dims[k] * vs;
+  }
+  b->vox_info.push_back(info);
+  b->vox_dirty = true;
+  *out = info;
+  return BB_OK;
+}
+
+bb_status upload_voxels(bb_batch* b) {
+  if (!b->vox_dirty || !b->d_vshape) return BB_OK;
+  if (!b->vox_shapes.empty())
+    HIPCHK(hipMemcpyAsync(b->d_vshape, b->vox_shapes.data(), b->vox_shapes.size() * sizeof(VoxShape),
+                          hipMemcpyHostToDevice, b->stream));
+  if (!b->vox_occ.empty())
+    HIPCHK(hipMemcpyAsync(b->d_vocc, b->vox_occ.data(), b->vox_occ.size() * 4, hipMemcpyHostToDevice, b->stream));
+  if (!b->vox_surf.empty())
+    HIPCHK(hipMemcpyAsync(b->d_vsurf, b->vox_surf.data(), b->vox_surf.size() * 4, hipMemcpyHostToDevice,
+                          b->stream));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  b->vox_dirty = false;
+  return BB_OK;
+}
+
 }  // namespace host
 }  // namespace bb
 
@@ -1342,7 +1516,7 @@ bb_status bb_batch_add_world(bb_batch* b, const bb_world_desc* desc, const bb_bo
                              uint32_t* out_world) {
   if (!b || !valid_desc(desc) || (n && !bodies)) return BB_ERR_INVALID;
   for (uint32_t i = 0; i < n; ++i)
-    if (!valid_body(bodies[i])) return BB_ERR_INVALID;
+    if (!valid_body_with_voxels(bodies[i], b->vox_shapes.size())) return BB_ERR_INVALID;
   if (desc->capacity_bodies && n > desc->capacity_bodies) return BB_ERR_CAPACITY;
   if (over_reference_cap(*desc, n)) return BB_ERR_CAPACITY;
   bb_status st = pull_state(b);
@@ -1354,6 +1528,11 @@ bb_status bb_batch_add_world(bb_batch* b, const bb_world_desc* desc, const bb_bo
   b->dirty = true;
   if (out_world) *out_world = (uint32_t)b->worlds.size() - 1;
   return BB_OK;
+}
+
+bb_status bb_batch_add_voxel_shape(bb_batch* b, const bb_voxel_shape_desc* desc, bb_voxel_shape_info* out) {
+  if (!b) return BB_ERR_INVALID;
+  return add_voxel_shape(b, desc, out);
 }
 
 bb_status bb_batch_step(bb_batch* b, uint32_t nsteps, void* stream, int use_graph) {
@@ -1457,7 +1636,7 @@ bb_status bb_world_add_bodies(bb_world* w, const bb_body_desc* bodies, uint32_t 
   if (!w || (n && !bodies)) return BB_ERR_INVALID;
   HostWorld& hw = w->b->worlds[0];
   for (uint32_t i = 0; i < n; ++i)
-    if (!valid_body(bodies[i])) return BB_ERR_INVALID;
+    if (!valid_body_with_voxels(bodies[i], w->b->vox_shapes.size())) return BB_ERR_INVALID;
   size_t total = hw.bodies.size() + n;
   if (hw.desc.capacity_bodies && total > hw.desc.capacity_bodies) return BB_ERR_CAPACITY;
   if (over_reference_cap(hw.desc, (uint32_t)total)) return BB_ERR_CAPACITY;
@@ -1469,6 +1648,11 @@ bb_status bb_world_add_bodies(bb_world* w, const bb_body_desc* bodies, uint32_t 
   }
   w->b->dirty = true;
   return BB_OK;
+}
+
+bb_status bb_world_add_voxel_shape(bb_world* w, const bb_voxel_shape_desc* desc, bb_voxel_shape_info* out) {
+  if (!w) return BB_ERR_INVALID;
+  return add_voxel_shape(w->b, desc, out);
 }
 
 bb_status bb_world_set_flags(bb_world* w, uint32_t flags) {
@@ -1528,7 +1712,11 @@ bb_status bb_world_instance_transforms(bb_world* w, float* dst, uint32_t n, int 
 }
 
 bb_status bb_world_snapshot(bb_world* w, void* dst, size_t cap, size_t* size) {
-  return w ? snapshot_world(w->b, dst, cap, size) : BB_ERR_INVALID;
+  if (!w) return BB_ERR_INVALID;
+  // the blob carries no voxel shape pools
+  for (const bb_body_desc& bd : w->b->worlds[0].bodies)
+    if (BB_SHAPE_KIND(bd.shape) == BB_SHAPE_VOXEL) return BB_ERR_INVALID;
+  return snapshot_world(w->b, dst, cap, size);
 }
 
 

@@ -34,12 +34,25 @@ struct WorldInfo {  // 64 bytes
   uint32_t pair_off, pair_cap, flags, solver;
   uint32_t iterations, morton_bits, quirks, prev_n;
   float dt, gravity, damp;
-  uint32_t mode;  // kModeDropPrev | kModeTeam
+  uint32_t mode;  // kModeDropPrev | kModeTeam | kModeVoxel
 };
 // WorldInfo::mode bits.  kModeDropPrev: the previous step left nothing to
 // warm start from.  kModeTeam: the world is stepped by a group of workgroups
 // in k_team instead of one workgroup in the per-world stage kernels.
-static constexpr uint32_t kModeDropPrev = 1u, kModeTeam = 2u;
+// kModeVoxel: the world has voxel bodies; each pair with a voxel body takes
+// six consecutive pair slots, one per contact-face code (keys 90..95).
+static constexpr uint32_t kModeDropPrev = 1u, kModeTeam = 2u, kModeVoxel = 4u;
+
+// A voxel shape of the batch's pools (VoxelShape, include/shared.inl:
+// 938-952): occupancy bits at vocc[occ_off], bit x + y*dx + z*dx*dy, and
+// surface cells at vsurf[surf_off], x | y<<8 | z<<16 | normal code<<24.
+struct VoxShape {  // 48 bytes
+  uint32_t dims[3];
+  uint32_t occ_off;
+  float vs;
+  float ox, oy, oz;  // corner of cell (0,0,0) in the body frame
+  uint32_t surf_off, surf_count, pad0, pad1;
+};
 
 // Control block of a workgroup team (k_team): barrier arrival counter and
 // generation, an error flag (a barrier wait that ran out of spins), a
@@ -104,6 +117,10 @@ struct Manifolds {  // pair-indexed slots; contact arrays are [slot*8 + c]
 struct Dev {
   WorldInfo* worlds;
   uint32_t W;
+  // voxel shape pools, shared by the batch's worlds
+  const VoxShape* vshape;
+  const uint32_t* vocc;
+  const uint32_t* vsurf;
   Rows cur, prev;
   Props props;
   uint32_t* entries_cur;   // [body_off + id] = row

@@ -44,6 +44,7 @@ constexpr uint32_t kMaxPgsLdsRows = 2048;
 struct HostWorld {
   bb_world_desc desc;
   std::vector<bb_body_desc> bodies;  // id order
+  bool has_voxel = false;            // a body is a voxel shape (kModeVoxel)
   bool skip_ws_once = false;
   bool drop_prev = false;
   uint32_t body_off = 0, act_off = 0, pair_off = 0, pair_cap = 0, n = 0, a = 0;
@@ -138,6 +139,15 @@ struct bb_batch {
   std::vector<bb::WorldInfo> infos;
   hipGraphExec_t graph = nullptr;
   int graph_parity = -1;
+  // voxel shape pools shared by the worlds (host copies; the device arrays
+  // are allocated at their maximum sizes and refreshed when vox_dirty)
+  std::vector<bb::VoxShape> vox_shapes;
+  std::vector<bb_voxel_shape_info> vox_info;
+  std::vector<uint32_t> vox_occ, vox_surf;
+  bool vox_dirty = false;
+  bb::VoxShape* d_vshape = nullptr;
+  uint32_t* d_vocc = nullptr;
+  uint32_t* d_vsurf = nullptr;
 };
 
 namespace bb {
@@ -147,6 +157,12 @@ uint32_t pair_capacity(uint32_t n);
 bool has_device();
 bool valid_desc(const bb_world_desc* d);
 bool valid_body(const bb_body_desc& b);
+// valid_body, or a voxel body whose shape id names one of `shapes` shapes.
+bool valid_body_with_voxels(const bb_body_desc& b, size_t shapes);
+// SceneManager::build_voxel_shape into the batch's host pools.
+bb_status add_voxel_shape(bb_batch* b, const bb_voxel_shape_desc* d, bb_voxel_shape_info* out);
+// Copies the host voxel pools to the device when they changed.
+bb_status upload_voxels(bb_batch* b);
 // Parity mode keeps the reference cap of 1,024 bodies per world.
 bool over_reference_cap(const bb_world_desc& d, uint32_t n);
 WorldInfo info_of(const HostWorld& hw);
@@ -158,7 +174,8 @@ bb_status alloc_arrays(bb_batch* b, Buffers& A, uint32_t W, uint32_t NB, uint32_
 uint64_t device_bytes(uint32_t W, uint32_t NB, uint32_t NA, uint32_t NP);
 // Writes world hw's initial rows into img at row r0 and active slot a0 of
 // the image range (image index = device index - range start).
-void fill_images(const HostWorld& hw, uint32_t r0, uint32_t a0, const Images& img);
+void fill_images(const HostWorld& hw, uint32_t r0, uint32_t a0, const Images& img,
+                 const std::vector<bb_voxel_shape_info>* vox = nullptr);
 // Copies an image range to the device: rows [body_lo, body_lo+nb) of both
 // parities and of the property arrays, active slots [act_lo, act_lo+na).
 bb_status upload_images(bb_batch* b, const Images& img, uint32_t body_lo, uint32_t nb, uint32_t act_lo,

@@ -77,9 +77,12 @@ struct Shape {
   float radius;
   uint32_t kind;
 };
+// pmax.w holds the body's shape word as a float: 0 box, 1 sphere, or
+// BB_SHAPE_VOXEL | voxel shape id << 8.
+BB_DI uint32_t shape_word(float w) { return (uint32_t)w; }
 BB_DI Shape shape_of(const Dev& d, uint32_t gid) {
   float4 a = d.props.pmin[gid], b = d.props.pmax[gid];
-  return Shape{ld3(a), ld3(b), a.w, b.w != 0.0f ? 1u : 0u};
+  return Shape{ld3(a), ld3(b), a.w, shape_word(b.w) & 0xFFu};
 }
 
 // RigidBody::get_AABB_from_OBB (src/shaders/extensions.slang:195-210);
@@ -670,6 +673,7 @@ BB_DI void sort8(uint32_t (&k)[8]) {
 
 struct BroadShared {
   uint32_t carry;
+  uint32_t pairs;  // voxel worlds: pairs, as opposed to pair slots
 };
 
 template <class T>
@@ -678,24 +682,42 @@ BB_DI void s_broad(const Dev& d, uint32_t w, const T& t, BroadShared& L) {
   const WorldInfo W = d.worlds[w];
   const uint32_t N = W.n, off = W.body_off, nb = 2 * off;
   uint32_t* stage = d.pick + W.pair_off;
-  if (tid == 0) L.carry = 0;
+  // voxel worlds: a pair with a voxel body takes six slots (one per contact-
+  // face code), so the per-row offsets count slots, and the pair counter
+  // counts pairs
+  const bool vox = (W.mode & kModeVoxel) != 0;
+  auto is_vox = [&](uint32_t row) {
+    return (shape_word(d.props.pmax[off + d.cur.id[off + row]].w) & 0xFFu) == BB_SHAPE_VOXEL;
+  };
+  if (tid == 0) {
+    L.carry = 0;
+    L.pairs = 0;
+  }
   t.sync();
   for (uint32_t c0 = 0; c0 < N; c0 += NT) {
     uint32_t i = c0 + tid;
-    uint32_t cnt = 0;
+    uint32_t cnt = 0, slots = 0;
     if (i < N) {
       float4 mn = d.nmin[nb + N - 1 + i], mx = d.nmax[nb + N - 1 + i];
       uint32_t* st = stage + i * kPairStage;
+      const bool vi = vox && is_vox(i);
       cnt = traverse(d, nb, N, i, ld3(mn), ld3(mx), [&](uint32_t k, uint32_t j) {
         if (k < kPairStage) st[k] = j;
+        if (vox) slots += (vi || is_vox(j)) ? 6u : 1u;
       });
+      if (!vox) slots = cnt;
       d.pcount[off + i] = cnt;
     }
     uint32_t total;
-    uint32_t ex = t.scan(cnt, total);
+    uint32_t ex = t.scan(slots, total);
     if (i < N) {
       d.poff[off + i] = L.carry + ex;
       d.pstart_cur[off + i] = L.carry + ex;
+    }
+    if (vox) {
+      uint32_t tp;
+      (void)t.scan(cnt, tp);
+      if (tid == 0) L.pairs += tp;
     }
     t.sync();
     if (tid == 0) L.carry += total;
@@ -705,10 +727,70 @@ BB_DI void s_broad(const Dev& d, uint32_t w, const T& t, BroadShared& L) {
   const uint32_t cap = W.pair_cap;
   if (tid == 0) {
     d.npairs_cur[w] = P < cap ? P : cap;
-    d.stats[w * ST_COUNT + ST_P] = P;
+    d.stats[w * ST_COUNT + ST_P] = vox ? L.pairs : P;
     if (P > cap) d.stats[w * ST_COUNT + ST_OVERFLOW] = 1;
   }
   uint2* pairs = d.pairs_cur + W.pair_off;
+  if (vox) {
+    // partners in ascending row order, each written 1 or 6 times; slots past
+    // the capacity are dropped (the overflow flag is set)
+    for (uint32_t i = tid; i < N; i += NT) {
+      const uint32_t o = d.poff[off + i];
+      const uint32_t cnt = d.pcount[off + i];
+      if (cnt == 0 || o >= cap) continue;
+      const bool vi = is_vox(i);
+      uint32_t* st = stage + i * kPairStage;
+      uint32_t k8[8];
+      if (cnt <= kPairStage) {
+        static_for<8>([&](auto jc) {
+          constexpr int j = decltype(jc)::value;
+          k8[j] = (uint32_t)j < cnt ? st[j] : 0xFFFFFFFFu;
+        });
+        sort8(k8);
+      }
+      uint32_t at = o;
+      auto put = [&](uint32_t j) {
+        const uint32_t rep = (vi || is_vox(j)) ? 6u : 1u;
+        for (uint32_t r = 0; r < rep; ++r, ++at)
+          if (at < cap) pairs[at] = make_uint2(i, j);
+      };
+      if (cnt <= kPairStage) {
+        static_for<8>([&](auto jc) {
+          constexpr int j = decltype(jc)::value;
+          if ((uint32_t)j < cnt) put(k8[j]);
+        });
+        continue;
+      }
+      // more partners than the staging area: traverse again into the slot
+      // range (it holds at least cnt entries), sort there, then expand from
+      // the back (an entry only moves to a higher or equal slot)
+      const uint32_t room = cap - o;
+      const uint32_t keep = cnt < room ? cnt : room;
+      float4 mn = d.nmin[nb + N - 1 + i], mx = d.nmax[nb + N - 1 + i];
+      traverse(d, nb, N, i, ld3(mn), ld3(mx), [&](uint32_t k, uint32_t j) {
+        if (k < keep) pairs[o + k] = make_uint2(i, j);
+      });
+      for (uint32_t a = 1; a < keep; ++a) {
+        uint2 x = pairs[o + a];
+        uint32_t b2 = a;
+        while (b2 > 0 && pairs[o + b2 - 1].y > x.y) {
+          pairs[o + b2] = pairs[o + b2 - 1];
+          --b2;
+        }
+        pairs[o + b2] = x;
+      }
+      uint32_t end = o;
+      for (uint32_t a = 0; a < keep; ++a) end += (vi || is_vox(pairs[o + a].y)) ? 6u : 1u;
+      for (uint32_t a = keep; a-- > 0;) {
+        const uint32_t j = pairs[o + a].y;
+        const uint32_t rep = (vi || is_vox(j)) ? 6u : 1u;
+        end -= rep;
+        for (uint32_t r = 0; r < rep; ++r)
+          if (end + r < cap) pairs[end + r] = make_uint2(i, j);
+      }
+    }
+    return;
+  }
   for (uint32_t i = tid; i < N; i += NT) {
     const uint32_t o = d.poff[off + i];
     if (o >= cap) continue;
@@ -974,6 +1056,7 @@ struct BodyView {
   v3 mn, mx;
   float radius;
   uint32_t kind;
+  uint32_t vid;  // voxel shape id (1-based) of a voxel body
 };
 
 // Detected contacts go straight into the output manifold slot: position +
@@ -1170,8 +1253,301 @@ BB_DI BodyView body_view(const Dev& d, uint32_t off, uint32_t row) {
   uint32_t g = off + row;
   uint32_t id = d.cur.id[g];
   float4 a = d.props.pmin[off + id], b = d.props.pmax[off + id];
-  return BodyView{ld3(d.cur.pos[g]), ldq(d.cur.rot[g]), ld3(a), ld3(b), a.w, b.w != 0.0f ? 1u : 0u};
+  const uint32_t word = shape_word(b.w);
+  return BodyView{ld3(d.cur.pos[g]), ldq(d.cur.rot[g]), ld3(a), ld3(b), a.w, word & 0xFFu, word >> 8};
 }
+// ================================================== voxel narrow phase
+// Voxel-bitmask narrow phase (src/shaders/simulation/collision_detection.slang:
+// 1001-1512), one thread per (pair, contact-face code) slot: the surface
+// cells of the body with the shorter surface list are sampled against the
+// other body's occupancy (or its OBB: a box's bounds, a sphere's bounding
+// cube) along axis code/2 of the other body's frame, sign + for odd codes;
+// a four-way extreme reducer keeps at most four cells, emitted as at most
+// four contacts.  The reducer's arrays are indexed only by unrolled loop
+// counters so they stay in registers.
+struct VoxRed {
+  v3 pos[4];
+  float pen[4];
+  v3 nrm[4];
+  uint32_t fa[4], fb[4];
+  float score[4];
+  uint32_t samples, interior;
+};
+
+// voxel_occ_bit (collision_detection.slang:1009-1018)
+BB_DI bool vox_bit(const Dev& d, const VoxShape& s, int cx, int cy, int cz) {
+  if (cx < 0 || cy < 0 || cz < 0 || cx >= (int)s.dims[0] || cy >= (int)s.dims[1] || cz >= (int)s.dims[2])
+    return false;
+  const uint32_t bit = (uint32_t)cx + (uint32_t)cy * s.dims[0] + (uint32_t)cz * s.dims[0] * s.dims[1];
+  return ((d.vocc[s.occ_off + bit / 32u] >> (bit % 32u)) & 1u) != 0u;
+}
+
+BB_DI v3 vox_origin(const VoxShape& s) { return V(s.ox, s.oy, s.oz); }
+
+// VoxelManifoldReducer::add (collision_detection.slang:1063-1086): per
+// extreme direction the higher score bucket, then the deeper depth bucket,
+// then the lower cell id.
+BB_DI void vox_add(VoxRed& r, v3 p, float pe, v3 n, uint32_t fa, uint32_t fb, float su, float sv, float quant) {
+  const float sc4[4] = {su + sv, su - sv, -su + sv, -su - sv};
+  const int nb = (int)__builtin_floorf(pe / quant);
+#pragma unroll
+  for (int o = 0; o < 4; ++o) {
+    const bool empty = r.pen[o] > 0.5e9f;
+    const int nb_s = (int)__builtin_floorf(sc4[o] / quant);
+    const int ob_s = empty ? -2147483640 : (int)__builtin_floorf(r.score[o] / quant);
+    const int ob = empty ? 0 : (int)__builtin_floorf(r.pen[o] / quant);
+    if (empty || nb_s > ob_s || (nb_s == ob_s && (nb < ob || (nb == ob && fa < r.fa[o])))) {
+      r.pos[o] = p;
+      r.pen[o] = pe;
+      r.nrm[o] = n;
+      r.fa[o] = fa;
+      r.fb[o] = fb;
+      r.score[o] = sc4[o];
+    }
+  }
+  ++r.samples;
+}
+
+BB_DI int vox_ci(int x, int y, int z, uint32_t k) { return k == 0 ? x : (k == 1 ? y : z); }
+
+// voxel_sample_body (collision_detection.slang:1096-1266)
+BB_DI void vox_sample(const Dev& d, const BodyView& r1, const VoxShape& s1, const BodyView& r2,
+                      const VoxShape& s2, bool s2_vox, VoxRed& red, uint32_t code) {
+  const float vs1 = s1.vs;
+  const v3 a0 = qrot_inv(r2.rot, qrot(r1.rot, V(1.0f, 0.0f, 0.0f)));
+  const v3 a1 = qrot_inv(r2.rot, qrot(r1.rot, V(0.0f, 1.0f, 0.0f)));
+  const v3 a2 = qrot_inv(r2.rot, qrot(r1.rot, V(0.0f, 0.0f, 1.0f)));
+  const v3 sup = V((fab(a0.x) + fab(a1.x)) + fab(a2.x), (fab(a0.y) + fab(a1.y)) + fab(a2.y),
+                   (fab(a0.z) + fab(a1.z)) + fab(a2.z)) *
+                 (0.5f * vs1);
+  const uint32_t axk = code / 2u;
+  const float sgn_pass = (code & 1u) != 0u ? 1.0f : -1.0f;
+  const int isg = (int)sgn_pass;
+  const v3 o1 = vox_origin(s1), o2 = vox_origin(s2);
+  // the pass normal, outward of r2, as each branch of the reference forms it
+  const v3 n_pass = qrot(r2.rot, V(axk == 0u ? sgn_pass : 0.0f, axk == 1u ? sgn_pass : 0.0f,
+                                   axk == 2u ? sgn_pass : 0.0f));
+  const v3 n_pass_box = qrot(r2.rot, V(axk == 0u ? 1.0f : 0.0f, axk == 1u ? 1.0f : 0.0f, axk == 2u ? 1.0f : 0.0f) *
+                                         sgn_pass);
+  for (uint32_t i = 0; i < s1.surf_count; ++i) {
+    const uint32_t e = d.vsurf[s1.surf_off + i];
+    const v3 local = V(o1.x + ((float)(e & 0xFFu) + 0.5f) * vs1, o1.y + ((float)((e >> 8u) & 0xFFu) + 0.5f) * vs1,
+                       o1.z + ((float)((e >> 16u) & 0xFFu) + 0.5f) * vs1);
+    const v3 pw = r1.pos + qrot(r1.rot, local);
+    float pe, su, sv;
+    uint32_t fb;
+    if (s2_vox) {
+      const float vs2 = s2.vs;
+      const v3 l2 = qrot_inv(r2.rot, pw - r2.pos);
+      su = axk == 0u ? l2.y : l2.x;
+      sv = axk == 2u ? l2.y : l2.z;
+      const v3 g = vdiv(l2 - o2, vs2);
+      const int cx = (int)__builtin_floorf(g.x), cy = (int)__builtin_floorf(g.y), cz = (int)__builtin_floorf(g.z);
+      if (!vox_bit(d, s2, cx, cy, cz)) {
+        // hover band: the occupied cell just behind along the pass axis
+        const int ix = axk == 0u ? cx - isg : cx, iy = axk == 1u ? cy - isg : cy, iz = axk == 2u ? cz - isg : cz;
+        if (!vox_bit(d, s2, ix, iy, iz)) continue;
+        const float go_a = cmp(o2, (int)axk);
+        const float ci_a = (float)vox_ci(ix, iy, iz, axk);
+        const float dh = cmp(l2, (int)axk) - (go_a + (ci_a + 0.5f) * vs2);
+        const float ovh = (0.5f * vs2 + cmp(sup, (int)axk)) - sgn_pass * dh;
+        pe = -ovh;
+        fb = (uint32_t)(ix & 0xFF) | ((uint32_t)(iy & 0xFF) << 8u) | ((uint32_t)(iz & 0xFF) << 16u);
+      } else {
+        // cell-face contact: the smallest outward-feasible exit bounds the
+        // pass axis's dominance band
+        const v3 cc_l = o2 + (V((float)cx, (float)cy, (float)cz) + V(0.5f, 0.5f, 0.5f)) * vs2;
+        const v3 dl = l2 - cc_l;
+        float best = 1e9f;
+#pragma unroll
+        for (uint32_t ax = 0; ax < 3u; ++ax) {
+          const float di = cmp(dl, (int)ax);
+          const int sg = di >= 0.0f ? 1 : -1;
+          const int nx = ax == 0u ? cx + sg : cx, ny = ax == 1u ? cy + sg : cy, nz = ax == 2u ? cz + sg : cz;
+          if (vox_bit(d, s2, nx, ny, nz)) continue;
+          const float ov = (0.5f * vs2 + cmp(sup, (int)ax)) - fab(di);
+          if (ov < best) best = ov;
+        }
+        if (best > 0.5e9f) {
+          red.interior += 1u;
+          continue;
+        }
+        const int px = axk == 0u ? cx + isg : cx, py = axk == 1u ? cy + isg : cy, pz = axk == 2u ? cz + isg : cz;
+        if (vox_bit(d, s2, px, py, pz)) continue;
+        const float ovp = (0.5f * vs2 + cmp(sup, (int)axk)) - sgn_pass * cmp(dl, (int)axk);
+        if (ovp > best + 0.15f * vs2) continue;
+        pe = -ovp;
+        fb = (uint32_t)(cx & 0xFF) | ((uint32_t)(cy & 0xFF) << 8u) | ((uint32_t)(cz & 0xFF) << 16u);
+      }
+    } else {
+      // an OBB partner: the cell against the box faces
+      v3 l2 = qrot_inv(r2.rot, pw - r2.pos);
+      const v3 half = (r2.mx - r2.mn) * 0.5f;
+      const v3 center_off = (r2.mx + r2.mn) * 0.5f;
+      l2 = l2 - center_off;
+      su = axk == 0u ? l2.y : l2.x;
+      sv = axk == 2u ? l2.y : l2.z;
+      const v3 q = vabs(l2) - half;
+      if (q.x > sup.x || q.y > sup.y || q.z > sup.z) continue;
+      const v3 qs = q - sup;
+      const float best_q = fmx(qs.x, fmx(qs.y, qs.z));
+      const float q_pass = (sgn_pass * cmp(l2, (int)axk) - cmp(half, (int)axk)) - cmp(sup, (int)axk);
+      if (q_pass < best_q - 0.15f * vs1) continue;
+      pe = q_pass;
+      fb = axk;
+    }
+    if (pe >= 0.02f) continue;
+    pe = fmx(pe, s2_vox ? -0.5f * vs1 : -vs1);
+    vox_add(red, pw, pe, s2_vox ? n_pass : n_pass_box, e & 0xFFFFFFu, fb, su, sv, 0.5f * vs1);
+  }
+}
+
+// voxel_obb_prefilter (collision_detection.slang:1268-1294)
+BB_DI bool vox_prefilter(const BodyView& a, const BodyView& b) {
+  const v3 ha = (a.mx - a.mn) * 0.5f, hb = (b.mx - b.mn) * 0.5f;
+  const v3 ca = a.pos + qrot(a.rot, (a.mx + a.mn) * 0.5f);
+  const v3 cb = b.pos + qrot(b.rot, (b.mx + b.mn) * 0.5f);
+  const v3 dd = cb - ca;
+  const v3 ax[3] = {qrot(a.rot, V(1, 0, 0)), qrot(a.rot, V(0, 1, 0)), qrot(a.rot, V(0, 0, 1))};
+  const v3 bx[3] = {qrot(b.rot, V(1, 0, 0)), qrot(b.rot, V(0, 1, 0)), qrot(b.rot, V(0, 0, 1))};
+  const float haa[3] = {ha.x, ha.y, ha.z}, hbb[3] = {hb.x, hb.y, hb.z};
+  bool sep = false;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const float rb = (hbb[0] * fab(dot(ax[i], bx[0])) + hbb[1] * fab(dot(ax[i], bx[1]))) + hbb[2] * fab(dot(ax[i], bx[2]));
+    const float ra = (haa[0] * fab(dot(bx[i], ax[0])) + haa[1] * fab(dot(bx[i], ax[1]))) + haa[2] * fab(dot(bx[i], ax[2]));
+    sep = sep || fab(dot(dd, ax[i])) > (haa[i] + rb) + 0.01f || fab(dot(dd, bx[i])) > (hbb[i] + ra) + 0.01f;
+  }
+  return !sep;
+}
+
+// voxel_corner_supported (collision_detection.slang:1302-1322)
+BB_DI bool vox_corner_ok(const Dev& d, v3 p, v3 n_sep, const BodyView& r2, const VoxShape& s2, bool s2_vox,
+                         uint32_t axk) {
+  if (s2_vox) {
+    const float vs2 = s2.vs;
+    const v3 l2 = qrot_inv(r2.rot, (p - n_sep * (0.3f * vs2)) - r2.pos);
+    const v3 g = vdiv(l2 - vox_origin(s2), vs2);
+    return vox_bit(d, s2, (int)__builtin_floorf(g.x), (int)__builtin_floorf(g.y), (int)__builtin_floorf(g.z));
+  }
+  const v3 half = (r2.mx - r2.mn) * 0.5f;
+  const v3 co = (r2.mx + r2.mn) * 0.5f;
+  const v3 l2 = qrot_inv(r2.rot, p - r2.pos) - co;
+  const float lu = axk == 0u ? l2.y : l2.x, lv = axk == 2u ? l2.y : l2.z;
+  const float hu = axk == 0u ? half.y : half.x, hv = axk == 2u ? half.y : half.z;
+  return fab(lu) <= hu + 0.01f && fab(lv) <= hv + 0.01f;
+}
+
+// voxel_emit_manifold (collision_detection.slang:1324-1432): kept cells in
+// slot order; one cell gives its four face corners, two cells two each,
+// more their centres; unsupported corners retreat toward the cell centre.
+BB_DI int vox_emit(const Dev& d, const VoxRed& red, bool flip, v3 n_sep, float vs, const BodyView& r2,
+                   const VoxShape& s2, bool s2_vox, uint32_t axk, CSink& sk, v3& n_out) {
+  if (red.samples == 0u) return 0;
+  n_out = flip ? n_sep : -n_sep;
+  uint32_t keepm = 0u;
+  float align[4];
+#pragma unroll
+  for (int o = 0; o < 4; ++o) {
+    align[o] = dot(red.nrm[o], n_sep);
+    bool ok = red.pen[o] < 0.02f && align[o] > 0.1f;
+#pragma unroll
+    for (int q = 0; q < o; ++q)
+      if (((keepm >> q) & 1u) && red.fa[q] == red.fa[o]) ok = false;
+    if (ok) keepm |= 1u << o;
+  }
+  const uint32_t kn = (uint32_t)__popc(keepm);
+  const uint32_t reps = kn == 1u ? 4u : (kn == 2u ? 2u : 1u);
+  const v3 t1 = vnorm(cross(n_sep, fab(n_sep.y) < 0.9f ? V(0.0f, 1.0f, 0.0f) : V(1.0f, 0.0f, 0.0f)));
+  const v3 t2 = cross(n_sep, t1);
+  const v3 da = ((t1 + t2) * 0.318f) * vs;
+  const v3 db = ((t1 - t2) * 0.318f) * vs;
+  int out_n = 0;
+#pragma unroll
+  for (int o = 0; o < 4; ++o) {
+    if (!((keepm >> o) & 1u)) continue;
+    const uint32_t fa = red.fa[o];
+    const uint32_t vpar = (fa ^ (fa >> 8u) ^ (fa >> 16u)) & 1u;
+    v3 em[4];
+    uint32_t en = 0u;
+#pragma unroll
+    for (uint32_t c = 0; c < 4u; ++c) {
+      if (c >= reps) break;
+      v3 p = red.pos[o];
+      if (reps > 1u) {
+        const v3 offv = ((c < 2u) == (vpar == 0u)) ? da : db;
+        const v3 off = offv * ((c & 1u) == 0u ? 1.0f : -1.0f);
+        p = red.pos[o] + off;
+        if (!vox_corner_ok(d, p, n_sep, r2, s2, s2_vox, axk)) {
+          p = red.pos[o] + off * 0.5f;
+          if (!vox_corner_ok(d, p, n_sep, r2, s2, s2_vox, axk)) p = red.pos[o];
+        }
+      }
+      bool dup = false;
+#pragma unroll
+      for (uint32_t q = 0; q < 4u; ++q)
+        if (q < en) {
+          const v3 dp = p - em[q];
+          if (dot(dp, dp) < 1.0e-12f) dup = true;
+        }
+      if (dup) continue;
+#pragma unroll
+      for (uint32_t q = 0; q < 4u; ++q)
+        if (q == en) em[q] = p;
+      ++en;
+      sk.put(out_n, p, red.pen[o] * align[o], flip ? 1u : 0u, red.fb[o], fa | (c << 24u), fa | (c << 24u));
+      ++out_n;
+    }
+  }
+  return out_n;
+}
+
+// voxel_axis_collision_detection (collision_detection.slang:1446-1512):
+// the manifold of code `code`, key 90 + code.
+BB_DI int vox_collide(const Dev& d, const BodyView& A, const BodyView& B, uint32_t code, CSink& sk, int& key,
+                      v3& n) {
+  key = (int)(90u + code);
+  if (!vox_prefilter(A, B)) return 0;
+  const bool a_vox = A.kind == BB_SHAPE_VOXEL, b_vox = B.kind == BB_SHAPE_VOXEL;
+  const VoxShape sA = d.vshape[a_vox ? A.vid - 1u : 0u];
+  const VoxShape sB = d.vshape[b_vox ? B.vid - 1u : 0u];
+  const bool flip = !a_vox || (b_vox && sB.surf_count < sA.surf_count);
+  const BodyView& r1 = flip ? B : A;
+  const BodyView& r2 = flip ? A : B;
+  const VoxShape s1 = flip ? sB : sA;
+  const VoxShape s2 = flip ? sA : sB;
+  const bool s2_vox = flip ? a_vox : b_vox;
+  VoxRed red;
+#pragma unroll
+  for (int o = 0; o < 4; ++o) {
+    red.pen[o] = 1.0e9f;
+    red.fa[o] = 0u;
+    red.fb[o] = 0u;
+    red.score[o] = -1.0e30f;
+    red.pos[o] = V(0.0f, 0.0f, 0.0f);
+    red.nrm[o] = V(0.0f, 0.0f, 0.0f);
+  }
+  red.samples = 0u;
+  red.interior = 0u;
+  vox_sample(d, r1, s1, r2, s2, s2_vox, red, code);
+  if (red.samples == 0u) {
+    if (code == 0u && red.interior > 0u) {
+      // deep-interpenetration rescue: one centre-to-centre contact
+      const v3 dd = r1.pos - r2.pos;
+      const v3 n_res = dot(dd, dd) > 1e-8f ? vnorm(dd) : V(0.0f, 1.0f, 0.0f);
+      n = flip ? n_res : -n_res;
+      sk.put(0, (r1.pos + r2.pos) * 0.5f, -0.5f * s1.vs, 0xFFFFFFu, 0xFFFFFFu, 0xFFFFFFu, 0xFFFFFFu);
+      return 1;
+    }
+    return 0;
+  }
+  const uint32_t ax = code / 2u;
+  const float sgn = (code & 1u) != 0u ? 1.0f : -1.0f;
+  const v3 n_world = qrot(r2.rot, V(ax == 0u ? sgn : 0.0f, ax == 1u ? sgn : 0.0f, ax == 2u ? sgn : 0.0f));
+  return vox_emit(d, red, flip, n_world, s1.vs, r2, s2, s2_vox, ax, sk, n);
+}
+
+
 
 // IslandBuilder::link_bodies (src/shaders/simulation/islands.slang:153-213):
 // lock-free union-find, links always point to the lower index, so every
@@ -1355,7 +1731,12 @@ BB_DI void s_narrow(const Dev& d, uint32_t w, const T& t) {
     CSink sk{d.man_cur.cpos, d.man_cur.cfp, slot * 8, false, 0.0f};
     int key, count;
     v3 n;
-    if (A.kind == BB_SHAPE_BOX && B.kind == BB_SHAPE_BOX) {
+    if (A.kind == BB_SHAPE_VOXEL || B.kind == BB_SHAPE_VOXEL) {
+      // the pair's contact-face code is its slot's rank among the pair's six
+      uint32_t code = 0;
+      while (code < 5u && code < p && pairs[p - code - 1].x == pr.x && pairs[p - code - 1].y == pr.y) ++code;
+      count = vox_collide(d, A, B, code, sk, key, n);
+    } else if (A.kind == BB_SHAPE_BOX && B.kind == BB_SHAPE_BOX) {
       sk.capped = true;
       count = obb_obb(A, B, sk, key, n);
     } else if (A.kind == BB_SHAPE_SPHERE && B.kind == BB_SHAPE_SPHERE) {
@@ -1420,8 +1801,17 @@ BB_DI void s_contacts(const Dev& d, uint32_t w, const T& t) {
         const uint32_t L1 = lo + 1 < W.prev_n ? d.pstart_prev[off + lo + 1] : Pprev;
         const uint32_t end = L1 < Pprev ? L1 : Pprev;
         while (L0 < end && ppairs[L0].y < hi) ++L0;
-        if (L0 < end && ppairs[L0].y == hi)
+        if (key >= 90 && key <= 95) {
+          // voxel manifolds (keys 90..95) match only the same ordered pair
+          // under the same key (find_old_collision's exact_only,
+          // collision_detection.slang:914-963): the old pair's slot of the
+          // same contact-face code
+          const uint32_t os = L0 + (uint32_t)(key - 90);
+          if (pa == lo && os < end && ppairs[os].y == hi)
+            matched = warm_start(d, W.pair_off + os, false, key, count, n, slot, deep, error);
+        } else if (L0 < end && ppairs[L0].y == hi) {
           matched = warm_start(d, W.pair_off + L0, pa != lo, key, count, n, slot, deep, error);
+        }
       }
     }
     // fresh contacts start from zero impulses
@@ -3138,7 +3528,8 @@ __global__ __launch_bounds__(1024) void k_sum_counts(const uint32_t* __restrict_
 // Mechanical energy per world of the current rows, in double:
 // sum over dynamic bodies of m|v|^2/2 + w_b.I_b.w_b/2 - m*g*y, with I_b the
 // physical inertia of the shape (box: m/12 diag(ey^2+ez^2, ...) over full
-// extents; sphere: 2/5 m r^2) and w_b the angular velocity in the body frame.
+// extents; sphere: 2/5 m r^2; voxel shape: the inverse of the stored inverse
+// inertia) and w_b the angular velocity in the body frame.
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_energy(Dev d, double* __restrict__ out) {
   __shared__ double red[BLOCK / 64];
@@ -3151,24 +3542,42 @@ __global__ __launch_bounds__(BLOCK) void k_energy(Dev d, double* __restrict__ ou
     const uint32_t pid = W.body_off + d.cur.id[g];
     const float4 mp = d.props.mass[pid], lo = d.props.pmin[pid], hi = d.props.pmax[pid];
     const double m = mp.x;
-    double ix, iy, iz;
-    if (hi.w != 0.0f) {
-      ix = iy = iz = 0.4 * m * (double)lo.w * (double)lo.w;
+    const uint32_t kind = shape_word(hi.w) & 0xFFu;
+    // body-frame inertia I (voxel bodies: the inverse of their inverse
+    // inertia, a full symmetric 3x3)
+    double I[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    if (kind == BB_SHAPE_SPHERE) {
+      I[0][0] = I[1][1] = I[2][2] = 0.4 * m * (double)lo.w * (double)lo.w;
+    } else if (kind == BB_SHAPE_VOXEL) {
+      const float4 r0 = d.props.ii0[pid], r1 = d.props.ii1[pid], r2 = d.props.ii2[pid];
+      const double a[3][3] = {{r0.x, r0.y, r0.z}, {r1.x, r1.y, r1.z}, {r2.x, r2.y, r2.z}};
+      const double det = a[0][0] * (a[1][1] * a[2][2] - a[1][2] * a[2][1]) -
+                         a[0][1] * (a[1][0] * a[2][2] - a[1][2] * a[2][0]) +
+                         a[0][2] * (a[1][0] * a[2][1] - a[1][1] * a[2][0]);
+      if (det != 0.0) {
+        for (int r = 0; r < 3; ++r)
+          for (int c = 0; c < 3; ++c) {
+            const int r1i = (c + 1) % 3, r2i = (c + 2) % 3, c1 = (r + 1) % 3, c2 = (r + 2) % 3;
+            I[r][c] = (a[r1i][c1] * a[r2i][c2] - a[r1i][c2] * a[r2i][c1]) / det;
+          }
+      }
     } else {
       const double ex = (double)hi.x - lo.x, ey = (double)hi.y - lo.y, ez = (double)hi.z - lo.z;
-      ix = m / 12.0 * (ey * ey + ez * ez);
-      iy = m / 12.0 * (ex * ex + ez * ez);
-      iz = m / 12.0 * (ex * ex + ey * ey);
+      I[0][0] = m / 12.0 * (ey * ey + ez * ez);
+      I[1][1] = m / 12.0 * (ex * ex + ez * ez);
+      I[2][2] = m / 12.0 * (ex * ex + ey * ey);
     }
     const float4 p = d.cur.pos[g], q = d.cur.rot[g], v = d.cur.vel[g], o = d.cur.omg[g];
     // body-frame angular velocity: conj(q) * w * q
     const double qx = q.x, qy = q.y, qz = q.z, qw = q.w, ox = o.x, oy = o.y, oz = o.z;
     const double tx = 2.0 * (qy * oz - qz * oy), ty = 2.0 * (qz * ox - qx * oz), tz = 2.0 * (qx * oy - qy * ox);
-    const double bx = ox - qw * tx + (qy * tz - qz * ty);
-    const double by = oy - qw * ty + (qz * tx - qx * tz);
-    const double bz = oz - qw * tz + (qx * ty - qy * tx);
-    e += 0.5 * m * ((double)v.x * v.x + (double)v.y * v.y + (double)v.z * v.z) +
-         0.5 * (ix * bx * bx + iy * by * by + iz * bz * bz) - m * (double)W.gravity * (double)p.y;
+    const double bw[3] = {ox - qw * tx + (qy * tz - qz * ty), oy - qw * ty + (qz * tx - qx * tz),
+                          oz - qw * tz + (qx * ty - qy * tx)};
+    double rot = 0.0;
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) rot += bw[r] * I[r][c] * bw[c];
+    e += 0.5 * m * ((double)v.x * v.x + (double)v.y * v.y + (double)v.z * v.z) + 0.5 * rot -
+         m * (double)W.gravity * (double)p.y;
   }
   for (int o = 32; o > 0; o >>= 1) e += __shfl_down(e, o, 64);
   if ((tid & 63) == 0) red[tid >> 6] = e;

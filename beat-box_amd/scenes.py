@@ -302,3 +302,107 @@ def serving_requests(seed: int = 0xBB05, rate: float = 500.0, duration: float = 
     hz = np.clip(hz, h_lo, h_hi)
     sd = splitmix64(seed, c + np.uint64(3))
     return t[keep], nb[keep].astype(np.int64), hz[keep], sd[keep]
+
+
+# ------------------------------------------------------------ voxel bodies
+SHAPE_VOXEL = 2
+
+
+def voxel_shape_word(shape_id: int) -> int:
+    """Body shape word of voxel shape `shape_id` (1-based)."""
+    return SHAPE_VOXEL | (int(shape_id) << 8)
+
+
+def voxel_grid(dims, solid) -> np.ndarray:
+    """Boolean cells g[x, y, z] of a dims grid where solid(x, y, z) holds."""
+    x, y, z = np.meshgrid(*(np.arange(n) for n in dims), indexing="ij")
+    return np.asarray(solid(x, y, z), bool) & np.ones(dims, bool)
+
+
+# The concave shapes of the reference's voxel scene (src/scene_manager.hpp:
+# 626-638): an L, a cross and a square frame with a hole, 0.5 m cells,
+# density 2.
+VOXEL_SIZE, VOXEL_DENSITY = 0.5, 2.0
+VOXEL_SHAPES = {
+    "L": voxel_grid((6, 6, 2), lambda x, y, z: (y < 2) | (x < 2)),
+    "cross": voxel_grid((6, 6, 2), lambda x, y, z: ((x >= 2) & (x < 4)) | ((y >= 2) & (y < 4))),
+    "frame": voxel_grid((8, 8, 2), lambda x, y, z: ~((x >= 2) & (x < 6) & (y >= 2) & (y < 6))),
+}
+
+
+def voxel_body(shape_id: int, pos, rot=(0.0, 0.0, 0.0, 1.0), friction: float = 0.6,
+               dynamic: bool = True) -> np.ndarray:
+    b = np.zeros(1, BODY_DESC)
+    b["shape"] = voxel_shape_word(shape_id)
+    b["flags"] = (BODY_DYNAMIC | BODY_GRAVITY) if dynamic else 0
+    b["position"] = pos
+    b["rotation"] = rot
+    b["friction"] = friction
+    return b
+
+
+def _static_box(pos, half, friction=0.6) -> np.ndarray:
+    b = np.zeros(1, BODY_DESC)
+    b["shape"] = SHAPE_BOX
+    b["position"] = pos
+    b["rotation"] = (0.0, 0.0, 0.0, 1.0)
+    b["minimum"] = tuple(-h for h in half)
+    b["maximum"] = half
+    b["friction"] = friction
+    return b
+
+
+def voxel_pile():
+    """The reference's concave-body scene (src/scene_manager.hpp:614-655):
+    floor, a static panel and a static post, three Ls, three crosses and
+    three frames (the frames' holes fit the post).  Returns the shapes in id
+    order (name, cells) and the bodies; shape ids follow that order."""
+    shapes = [("L", VOXEL_SHAPES["L"]), ("cross", VOXEL_SHAPES["cross"]), ("frame", VOXEL_SHAPES["frame"])]
+    g = np.zeros(1, BODY_DESC)
+    _ground(g[0], friction=0.6)
+    q_id, q_x90 = (0.0, 0.0, 0.0, 1.0), (0.7071, 0.0, 0.0, 0.7071)
+    parts = [g, _static_box((0.0, 22.0, 14.0), (5.0, 0.2, 5.0)), _static_box((8.0, 4.0, 14.0), (0.35, 4.0, 0.35))]
+    for sid, pos, rot in [(1, (-6.0, 2.5, 13.0), q_id), (1, (-5.4, 5.5, 14.6), q_x90), (1, (-2.0, 8.0, 13.8), q_id),
+                          (2, (-3.5, 2.5, 14.8), q_id), (2, (-4.5, 11.0, 14.0), q_x90), (2, (0.5, 4.0, 14.2), q_id),
+                          (3, (-1.0, 12.5, 14.4), q_id), (3, (2.5, 7.0, 13.4), q_x90), (3, (8.0, 11.0, 14.0), q_x90)]:
+        parts.append(voxel_body(sid, pos, rot))
+    return shapes, np.concatenate(parts)
+
+
+def voxel_mixed_pile(seed: int, n: int):
+    """A random pile of n bodies on the ground: voxel Ls, crosses and frames
+    mixed with boxes and spheres, random yaw, dropped from a column.  Uses
+    shape ids 1..3 in VOXEL_SHAPES order, like voxel_pile."""
+    shapes = [("L", VOXEL_SHAPES["L"]), ("cross", VOXEL_SHAPES["cross"]), ("frame", VOXEL_SHAPES["frame"])]
+    c = np.arange(n, dtype=np.uint64) * np.uint64(8)
+    kind = np.floor(uniform(seed, c, 0.0, 5.0)).astype(np.int64)
+    half = max(4.0, 1.6 * n ** 0.5)
+    px = uniform(seed, c + np.uint64(1), -half, half)
+    pz = uniform(seed, c + np.uint64(2), -half, half)
+    py = 2.5 + np.arange(n) * 4.5 / max(1.0, n / 6.0)
+    yaw = uniform(seed, c + np.uint64(3), 0.0, 2.0 * math.pi)
+    q = yaw_quat(yaw)
+    g = np.zeros(1, BODY_DESC)
+    _ground(g[0], friction=0.6)
+    parts = [g]
+    for i in range(n):
+        rot = tuple(float(v) for v in q[i])
+        pos = (float(px[i]), float(py[i]), float(pz[i]))
+        if kind[i] < 3:
+            parts.append(voxel_body(int(kind[i]) + 1, pos, rot))
+        else:
+            b = np.zeros(1, BODY_DESC)
+            b["flags"] = BODY_DYNAMIC | BODY_GRAVITY
+            b["position"] = pos
+            b["rotation"] = rot
+            b["mass"] = 1.0
+            b["friction"] = 0.6
+            if kind[i] == 3:
+                b["shape"] = SHAPE_BOX
+                b["minimum"] = (-0.5, -0.5, -0.5)
+                b["maximum"] = (0.5, 0.5, 0.5)
+            else:
+                b["shape"] = SHAPE_SPHERE
+                b["radius"] = 0.5
+            parts.append(b)
+    return shapes, np.concatenate(parts)

@@ -73,7 +73,42 @@ enum {
   BB_BODY_SLEEPING = 1u << 3
 };
 
-typedef enum bb_shape { BB_SHAPE_BOX = 0, BB_SHAPE_SPHERE = 1 } bb_shape;
+typedef enum bb_shape { BB_SHAPE_BOX = 0, BB_SHAPE_SPHERE = 1, BB_SHAPE_VOXEL = 2 } bb_shape;
+
+/* A voxel body's shape word: BB_SHAPE_VOXEL in the low byte, the 1-based
+ * voxel shape id (RigidBody::shape_index, include/shared.inl:335-476) above. */
+#define BB_SHAPE_VOXEL_OF(id) ((uint32_t)BB_SHAPE_VOXEL | ((uint32_t)(id) << 8))
+#define BB_SHAPE_KIND(shape) ((uint32_t)(shape)&0xFFu)
+#define BB_SHAPE_VOXEL_ID(shape) ((uint32_t)(shape) >> 8)
+
+/* Voxel shape pools, include/shared.inl:772-774. */
+#define BB_MAX_VOXEL_SHAPES 64u
+#define BB_MAX_VOXEL_OCC_WORDS 16384u
+#define BB_MAX_VOXEL_SURFACE 16384u
+
+/* A concave voxel shape (VoxelShape, include/shared.inl:938-952): occupancy
+ * bits over a dims grid, bit x + y*dims[0] + z*dims[0]*dims[1] of the words
+ * (the solid(x,y,z) predicate of SceneManager::build_voxel_shape,
+ * src/scene_manager.hpp:478-566). */
+typedef struct bb_voxel_shape_desc {
+  uint32_t dims[3];           /* cells per axis, 1..255                     */
+  float voxel_size;           /* cell edge, > 0                             */
+  float density;              /* mass per unit volume, > 0                  */
+  const uint32_t* occupancy;  /* ceil(dims product / 32) words              */
+} bb_voxel_shape_desc;
+
+/* What build_voxel_shape returns (VoxelShapeBuild): the body origin is the
+ * centre of mass; bodies of this shape take mass, inverse inertia and local
+ * bounds from here (push_voxel_body, src/scene_manager.hpp:568-572). */
+typedef struct bb_voxel_shape_info {
+  uint32_t shape_id;      /* 1-based: body shape = BB_SHAPE_VOXEL_OF(id)     */
+  uint32_t solid_count;   /* solid cells                                     */
+  uint32_t surface_count; /* cells with an empty face neighbour              */
+  float mass;
+  float inv_inertia[9];   /* row-major, about the centre of mass             */
+  float minimum[3];       /* grid bounds in the body frame                   */
+  float maximum[3];
+} bb_voxel_shape_info;
 
 /* Reference-compatibility switches ("parity mode").  Each reproduces one
  * documented behaviour of the reference so results can be compared with it. */
@@ -113,7 +148,10 @@ typedef struct bb_world_desc {
 
 /* One body to add.  Boxes use local bounds minimum/maximum (reference
  * RigidBody::minimum/maximum); spheres use `radius` (extension: the reference
- * has no spheres).  mass == 0 means immovable, as in load_scene. */
+ * has no spheres).  mass == 0 means immovable, as in load_scene.  A voxel
+ * body (shape = BB_SHAPE_VOXEL_OF(id)) takes mass, inverse inertia and bounds
+ * from its shape and ignores minimum, maximum, radius and mass; without
+ * BB_BODY_DYNAMIC it is immovable (mass 0). */
 typedef struct bb_body_desc {
   uint32_t shape; /* bb_shape                                             */
   uint32_t flags; /* BB_BODY_* (DYNAMIC, GRAVITY)                         */
@@ -237,6 +275,14 @@ bb_status bb_world_create(const bb_world_desc* desc, bb_world** out);
 bb_status bb_world_add_bodies(bb_world* w, const bb_body_desc* bodies,
                               uint32_t n, uint32_t* out_ids);
 
+/* SceneManager::build_voxel_shape (src/scene_manager.hpp:478-566): appends a
+ * voxel shape to the world's pools (occupancy bits, surface-cell list, shape
+ * record) and returns its id and mass properties.  BB_ERR_INVALID for bad
+ * dims, sizes or an empty grid; BB_ERR_CAPACITY past the pool limits (the
+ * reference prints an error and overruns). */
+bb_status bb_world_add_voxel_shape(bb_world* w, const bb_voxel_shape_desc* desc,
+                                   bb_voxel_shape_info* out);
+
 /* RigidBodyManager::set_sim_flags (ORs the bits in), clear_sim_flags, and
  * get_sim_flags (src/rigid_body_manager.hpp:35-53).  Like the reference, a
  * flag change takes effect on the next step and keeps warm-start state. */
@@ -299,7 +345,8 @@ bb_status bb_world_instance_transforms(bb_world* w, float* dst, uint32_t n,
  * per-row offsets, the manifold slots that warm start the next step, the
  * counters, the step index and the world and body descriptors.  With
  * dst == NULL, *size receives the bytes needed; otherwise cap must be at
- * least that (BB_ERR_CAPACITY). */
+ * least that (BB_ERR_CAPACITY).  A world with voxel bodies is refused
+ * (BB_ERR_INVALID): the blob does not carry the voxel shape pools. */
 bb_status bb_world_snapshot(bb_world* w, void* dst, size_t cap, size_t* size);
 /* Creates a world from a checkpoint.  warm_start != 0: the world continues
  * bit-identically to the one that wrote the blob.  warm_start == 0: like the
@@ -323,6 +370,10 @@ bb_status bb_batch_create(uint32_t capacity_worlds, uint32_t capacity_bodies,
 bb_status bb_batch_add_world(bb_batch* b, const bb_world_desc* desc,
                              const bb_body_desc* bodies, uint32_t n,
                              uint32_t* out_world);
+/* bb_world_add_voxel_shape for a batch: the pools are shared by the batch's
+ * worlds, so a voxel body of any world may use any shape added before it. */
+bb_status bb_batch_add_voxel_shape(bb_batch* b, const bb_voxel_shape_desc* desc,
+                                   bb_voxel_shape_info* out);
 /* Steps every world nsteps times.  With use_graph != 0 the step sequence is
  * captured once as a hipGraph and replayed. */
 bb_status bb_batch_step(bb_batch* b, uint32_t nsteps, void* stream,

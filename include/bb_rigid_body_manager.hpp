@@ -69,6 +69,14 @@ class RigidBodyManager {
     if (ids) *ids = tmp;
     return last_ == BB_OK;
   }
+  // SceneManager::build_voxel_shape (src/scene_manager.hpp:478-566): a
+  // concave shape for voxel bodies (shape = BB_SHAPE_VOXEL_OF(info.shape_id),
+  // push_voxel_body's mass and inertia come from the shape).
+  bool build_voxel_shape(const bb_voxel_shape_desc& desc, bb_voxel_shape_info* info) {
+    if (!world_ || !info) return false;
+    last_ = bb_world_add_voxel_shape(world_, &desc, info);
+    return last_ == BB_OK;
+  }
   // Body readback through the id -> row map (the TLAS-update consumer).
   bool read_bodies(std::vector<bb_body_state>& out, bb_order order = BB_ORDER_BY_ID) {
     if (!world_) return false;

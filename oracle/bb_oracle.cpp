@@ -124,6 +124,16 @@ struct World {
   std::vector<uint32_t> body_color;
   bb_step_stats stats{};
   uint64_t step_index = 0;
+  // voxel shape pools (VoxelShape, include/shared.inl:938-952)
+  struct VShape {
+    uint32_t dims[3];
+    float vs;
+    f3 origin;
+    uint32_t occ_off, surf_off, surf_count;
+  };
+  std::vector<VShape> vshapes;
+  std::vector<bb_voxel_shape_info> vinfo;
+  std::vector<uint32_t> vocc, vsurf;
 };
 
 // ============================================================ body helpers
@@ -860,6 +870,317 @@ static void update_from_old(Manifold& m, const Manifold& old, bool swapped) {
   m.error = (int32_t)clampf((deep_new - deep_old) * 10000.0f, -32000.0f, 32000.0f);
 }
 
+// ======================================================= voxel narrow phase
+// Voxel-bitmask narrow phase (src/shaders/simulation/collision_detection.slang:
+// 1001-1512): the surface cells of one body are sampled against the other
+// body's solid cells (or its OBB), per contact-face axis of the other body;
+// the samples reduce to at most four extreme cells and at most four contacts.
+typedef World::VShape VShape;
+static inline uint32_t shape_kind(uint32_t s) { return s & 0xFFu; }
+static inline bool is_voxel(const Body& r) { return shape_kind(r.shape) == BB_SHAPE_VOXEL; }
+
+// voxel_occ_bit (collision_detection.slang:1009-1018)
+static bool vox_bit(const World& w, const VShape& s, int cx, int cy, int cz) {
+  if (cx < 0 || cy < 0 || cz < 0 || cx >= (int)s.dims[0] || cy >= (int)s.dims[1] || cz >= (int)s.dims[2])
+    return false;
+  uint32_t bit = (uint32_t)cx + (uint32_t)cy * s.dims[0] + (uint32_t)cz * s.dims[0] * s.dims[1];
+  return ((w.vocc[s.occ_off + bit / 32u] >> (bit % 32u)) & 1u) != 0u;
+}
+
+// VoxelManifoldReducer (collision_detection.slang:1033-1087): per extreme
+// direction of the contact-face plane, the sample with the highest score
+// bucket, then the deeper depth bucket, then the lowest cell id.  The
+// weighted normal sum of the reference feeds nothing downstream and is left
+// out.
+struct VoxReducer {
+  f3 pos[4];
+  float pen[4];
+  f3 nrm[4];
+  uint32_t fp_a[4], fp_b[4];
+  float score[4];
+  uint32_t sample_count, interior_hits;
+  void init() {
+    for (int i = 0; i < 4; ++i) { pen[i] = 1.0e9f; fp_a[i] = 0u; fp_b[i] = 0u; score[i] = -1.0e30f; }
+    sample_count = 0u;
+    interior_hits = 0u;
+  }
+  void add(f3 p, float pe, f3 n, uint32_t fa, uint32_t fb, float su, float sv, float quant) {
+    const float sc4[4] = {su + sv, su - sv, -su + sv, -su - sv};
+    const int nb = (int)std::floor(pe / quant);
+    for (int o = 0; o < 4; ++o) {
+      const bool empty = pen[o] > 0.5e9f;
+      const int nb_s = (int)std::floor(sc4[o] / quant);
+      const int ob_s = empty ? -2147483640 : (int)std::floor(score[o] / quant);
+      const int ob = empty ? 0 : (int)std::floor(pen[o] / quant);
+      if (empty || nb_s > ob_s || (nb_s == ob_s && (nb < ob || (nb == ob && fa < fp_a[o])))) {
+        pos[o] = p; pen[o] = pe; nrm[o] = n; fp_a[o] = fa; fp_b[o] = fb; score[o] = sc4[o];
+      }
+    }
+    ++sample_count;
+  }
+};
+
+static inline int comp_i(int x, int y, int z, uint32_t k) { return k == 0 ? x : (k == 1 ? y : z); }
+
+// voxel_sample_body (collision_detection.slang:1096-1266) for pass `code`
+// (axis code/2, sign + for odd codes) of r2's local frame.
+static void vox_sample(const World& w, const Body& r1, const VShape& s1, const Body& r2, const VShape& s2,
+                       bool s2_vox, VoxReducer& red, uint32_t code) {
+  const float vs1 = s1.vs;
+  const f3 a0 = qrotate_inv(r2.rotation, qrotate(r1.rotation, mk(1.0f, 0.0f, 0.0f)));
+  const f3 a1 = qrotate_inv(r2.rotation, qrotate(r1.rotation, mk(0.0f, 1.0f, 0.0f)));
+  const f3 a2 = qrotate_inv(r2.rotation, qrotate(r1.rotation, mk(0.0f, 0.0f, 1.0f)));
+  const f3 sup = muls(mk((fabsr(a0.x) + fabsr(a1.x)) + fabsr(a2.x), (fabsr(a0.y) + fabsr(a1.y)) + fabsr(a2.y),
+                         (fabsr(a0.z) + fabsr(a1.z)) + fabsr(a2.z)),
+                      0.5f * vs1);
+  const uint32_t axk = code / 2u;
+  const float sgn_pass = (code & 1u) != 0u ? 1.0f : -1.0f;
+  const int isg = (int)sgn_pass;
+  for (uint32_t i = 0; i < s1.surf_count; ++i) {
+    const uint32_t e = w.vsurf[s1.surf_off + i];
+    const f3 local = mk(s1.origin.x + ((float)(e & 0xFFu) + 0.5f) * vs1,
+                        s1.origin.y + ((float)((e >> 8u) & 0xFFu) + 0.5f) * vs1,
+                        s1.origin.z + ((float)((e >> 16u) & 0xFFu) + 0.5f) * vs1);
+    const f3 pw = add(r1.position, qrotate(r1.rotation, local));
+    float pe;
+    f3 n_out2;
+    uint32_t fb;
+    float su, sv;
+    if (s2_vox) {
+      const float vs2 = s2.vs;
+      const f3 l2 = qrotate_inv(r2.rotation, sub(pw, r2.position));
+      su = axk == 0u ? l2.y : l2.x;
+      sv = axk == 2u ? l2.y : l2.z;
+      const f3 g = divs(sub(l2, s2.origin), vs2);
+      const int cx = (int)std::floor(g.x), cy = (int)std::floor(g.y), cz = (int)std::floor(g.z);
+      if (!vox_bit(w, s2, cx, cy, cz)) {
+        // hover band: the occupied cell just behind along the pass axis
+        int ix = cx, iy = cy, iz = cz;
+        if (axk == 0u) ix -= isg;
+        else if (axk == 1u) iy -= isg;
+        else iz -= isg;
+        if (!vox_bit(w, s2, ix, iy, iz)) continue;
+        const float go_a = comp(s2.origin, (int)axk);
+        const float ci_a = (float)comp_i(ix, iy, iz, axk);
+        const float l2_a = comp(l2, (int)axk);
+        const float sup_h = comp(sup, (int)axk);
+        const float dh = l2_a - (go_a + (ci_a + 0.5f) * vs2);
+        const float ovh = (0.5f * vs2 + sup_h) - sgn_pass * dh;
+        n_out2 = qrotate(r2.rotation, mk(axk == 0u ? sgn_pass : 0.0f, axk == 1u ? sgn_pass : 0.0f,
+                                         axk == 2u ? sgn_pass : 0.0f));
+        pe = -ovh;
+        fb = (uint32_t)(ix & 0xFF) | ((uint32_t)(iy & 0xFF) << 8u) | ((uint32_t)(iz & 0xFF) << 16u);
+      } else {
+        // cell-face contact: smallest outward-feasible exit as the reference
+        // for the pass axis's dominance band
+        const f3 cc_l = add(s2.origin, muls(add(mk((float)cx, (float)cy, (float)cz), mk(0.5f, 0.5f, 0.5f)), vs2));
+        const f3 dl = sub(l2, cc_l);
+        float best = 1e9f;
+        for (uint32_t ax = 0; ax < 3u; ++ax) {
+          const float di = comp(dl, (int)ax);
+          const float sgn = di >= 0.0f ? 1.0f : -1.0f;
+          int nx = cx, ny = cy, nz = cz;
+          if (ax == 0u) nx += (int)sgn;
+          else if (ax == 1u) ny += (int)sgn;
+          else nz += (int)sgn;
+          if (vox_bit(w, s2, nx, ny, nz)) continue;
+          const float ov = (0.5f * vs2 + comp(sup, (int)ax)) - fabsr(di);
+          if (ov < best) best = ov;
+        }
+        if (best > 0.5e9f) {
+          red.interior_hits += 1u;
+          continue;
+        }
+        const float dla = comp(dl, (int)axk);
+        int px = cx, py = cy, pz = cz;
+        if (axk == 0u) px += isg;
+        else if (axk == 1u) py += isg;
+        else pz += isg;
+        if (vox_bit(w, s2, px, py, pz)) continue;
+        const float ovp = (0.5f * vs2 + comp(sup, (int)axk)) - sgn_pass * dla;
+        if (ovp > best + 0.15f * vs2) continue;
+        n_out2 = qrotate(r2.rotation, mk(axk == 0u ? sgn_pass : 0.0f, axk == 1u ? sgn_pass : 0.0f,
+                                         axk == 2u ? sgn_pass : 0.0f));
+        pe = -ovp;
+        fb = (uint32_t)(cx & 0xFF) | ((uint32_t)(cy & 0xFF) << 8u) | ((uint32_t)(cz & 0xFF) << 16u);
+      }
+    } else {
+      // r2 an OBB: the sample cell against the box faces
+      f3 l2 = qrotate_inv(r2.rotation, sub(pw, r2.position));
+      const f3 half = muls(sub(r2.maximum, r2.minimum), 0.5f);
+      const f3 center_off = muls(add(r2.maximum, r2.minimum), 0.5f);
+      l2 = sub(l2, center_off);
+      su = axk == 0u ? l2.y : l2.x;
+      sv = axk == 2u ? l2.y : l2.z;
+      const f3 q = sub(absv(l2), half);
+      if (q.x > sup.x || q.y > sup.y || q.z > sup.z) continue;
+      const f3 qs = sub(q, sup);
+      const float best_q = fmaxr(qs.x, fmaxr(qs.y, qs.z));
+      const float q_pass = sgn_pass * comp(l2, (int)axk) - comp(half, (int)axk) - comp(sup, (int)axk);
+      if (q_pass < best_q - 0.15f * vs1) continue;
+      const f3 axis = mk(axk == 0u ? 1.0f : 0.0f, axk == 1u ? 1.0f : 0.0f, axk == 2u ? 1.0f : 0.0f);
+      n_out2 = qrotate(r2.rotation, muls(axis, sgn_pass));
+      pe = q_pass;
+      fb = axk;
+    }
+    if (pe >= 0.02f) continue;
+    pe = fmaxr(pe, s2_vox ? -0.5f * vs1 : -vs1);
+    red.add(pw, pe, n_out2, e & 0xFFFFFFu, fb, su, sv, 0.5f * vs1);
+  }
+}
+
+// voxel_obb_prefilter (collision_detection.slang:1268-1294): face axes of
+// both bounding OBBs, 0.01 margin.
+static bool vox_prefilter(const Body& a, const Body& b) {
+  const f3 ha = muls(sub(a.maximum, a.minimum), 0.5f);
+  const f3 hb = muls(sub(b.maximum, b.minimum), 0.5f);
+  const f3 ca = add(a.position, qrotate(a.rotation, muls(add(a.maximum, a.minimum), 0.5f)));
+  const f3 cb = add(b.position, qrotate(b.rotation, muls(add(b.maximum, b.minimum), 0.5f)));
+  const f3 d = sub(cb, ca);
+  const f3 ax[3] = {qrotate(a.rotation, mk(1, 0, 0)), qrotate(a.rotation, mk(0, 1, 0)), qrotate(a.rotation, mk(0, 0, 1))};
+  const f3 bx[3] = {qrotate(b.rotation, mk(1, 0, 0)), qrotate(b.rotation, mk(0, 1, 0)), qrotate(b.rotation, mk(0, 0, 1))};
+  const float haa[3] = {ha.x, ha.y, ha.z}, hbb[3] = {hb.x, hb.y, hb.z};
+  for (int i = 0; i < 3; ++i) {
+    const float rb = (hbb[0] * fabsr(dot(ax[i], bx[0])) + hbb[1] * fabsr(dot(ax[i], bx[1]))) +
+                     hbb[2] * fabsr(dot(ax[i], bx[2]));
+    if (fabsr(dot(d, ax[i])) > (haa[i] + rb) + 0.01f) return false;
+    const float ra = (haa[0] * fabsr(dot(bx[i], ax[0])) + haa[1] * fabsr(dot(bx[i], ax[1]))) +
+                     haa[2] * fabsr(dot(bx[i], ax[2]));
+    if (fabsr(dot(d, bx[i])) > (hbb[i] + ra) + 0.01f) return false;
+  }
+  return true;
+}
+
+// voxel_corner_supported (collision_detection.slang:1302-1322)
+static bool vox_corner_supported(const World& w, f3 p, f3 n_sep, const Body& r2, const VShape& s2, bool s2_vox,
+                                 uint32_t axk) {
+  if (s2_vox) {
+    const float vs2 = s2.vs;
+    const f3 l2 = qrotate_inv(r2.rotation, sub(sub(p, muls(n_sep, 0.3f * vs2)), r2.position));
+    const f3 g = divs(sub(l2, s2.origin), vs2);
+    return vox_bit(w, s2, (int)std::floor(g.x), (int)std::floor(g.y), (int)std::floor(g.z));
+  }
+  const f3 half = muls(sub(r2.maximum, r2.minimum), 0.5f);
+  const f3 co = muls(add(r2.maximum, r2.minimum), 0.5f);
+  const f3 l2 = sub(qrotate_inv(r2.rotation, sub(p, r2.position)), co);
+  const float lu = axk == 0u ? l2.y : l2.x;
+  const float lv = axk == 2u ? l2.y : l2.z;
+  const float hu = axk == 0u ? half.y : half.x;
+  const float hv = axk == 2u ? half.y : half.z;
+  return fabsr(lu) <= hu + 0.01f && fabsr(lv) <= hv + 0.01f;
+}
+
+// voxel_emit_manifold (collision_detection.slang:1324-1432)
+static bool vox_emit(const World& w, const VoxReducer& red, bool flip, int key, f3 n_sep, float vs, const Body& r2,
+                     const VShape& s2, bool s2_vox, uint32_t axk, Manifold& m) {
+  m.count = 0;
+  m.key = key;
+  m.error = 0;
+  if (red.sample_count == 0u) return false;
+  m.normal = flip ? n_sep : neg(n_sep);
+  uint32_t kept[4];
+  float keep_align[4];
+  uint32_t kn = 0u;
+  for (uint32_t o = 0; o < 4u; ++o) {
+    if (red.pen[o] >= 0.02f) continue;
+    const float align = dot(red.nrm[o], n_sep);
+    if (align <= 0.1f) continue;
+    bool dup = false;
+    for (uint32_t q = 0; q < kn; ++q)
+      if (red.fp_a[kept[q]] == red.fp_a[o]) { dup = true; break; }
+    if (dup) continue;
+    kept[kn] = o;
+    keep_align[kn] = align;
+    ++kn;
+  }
+  const uint32_t reps = kn == 1u ? 4u : (kn == 2u ? 2u : 1u);
+  const f3 t1 = normalize(cross(n_sep, fabsr(n_sep.y) < 0.9f ? mk(0.0f, 1.0f, 0.0f) : mk(1.0f, 0.0f, 0.0f)));
+  const f3 t2 = cross(n_sep, t1);
+  const f3 da = muls(muls(add(t1, t2), 0.318f), vs);
+  const f3 db = muls(muls(sub(t1, t2), 0.318f), vs);
+  uint32_t out_n = 0u;
+  for (uint32_t k = 0; k < kn; ++k) {
+    const uint32_t o = kept[k];
+    const uint32_t vpar = (red.fp_a[o] ^ (red.fp_a[o] >> 8u) ^ (red.fp_a[o] >> 16u)) & 1u;
+    f3 emitted[4];
+    uint32_t emitted_n = 0u;
+    for (uint32_t c = 0; c < reps; ++c) {
+      f3 p = red.pos[o];
+      if (reps > 1u) {
+        const f3 offv = ((c < 2u) == (vpar == 0u)) ? da : db;
+        const f3 off = muls(offv, (c & 1u) == 0u ? 1.0f : -1.0f);
+        p = add(red.pos[o], off);
+        if (!vox_corner_supported(w, p, n_sep, r2, s2, s2_vox, axk)) {
+          p = add(red.pos[o], muls(off, 0.5f));
+          if (!vox_corner_supported(w, p, n_sep, r2, s2, s2_vox, axk)) p = red.pos[o];
+        }
+      }
+      bool dup_p = false;
+      for (uint32_t q = 0; q < emitted_n; ++q) {
+        const f3 dp = sub(p, emitted[q]);
+        if (dot(dp, dp) < 1.0e-12f) { dup_p = true; break; }
+      }
+      if (dup_p) continue;
+      emitted[emitted_n++] = p;
+      Contact& ct = m.c[out_n];
+      std::memset(&ct, 0, sizeof(ct));
+      ct.position = p;
+      ct.penetration = red.pen[o] * keep_align[k];
+      ct.fp.in_ref = flip ? 1u : 0u;
+      ct.fp.out_ref = red.fp_b[o];
+      ct.fp.in_inc = red.fp_a[o] | (c << 24u);
+      ct.fp.out_inc = red.fp_a[o] | (c << 24u);
+      ++out_n;
+    }
+  }
+  m.count = (int32_t)out_n;
+  return out_n > 0u;
+}
+
+// voxel_axis_collision_detection (collision_detection.slang:1446-1512): the
+// manifold of contact-face code `code` (key 90 + code); the body with the
+// smaller surface list is sampled.  A non-voxel partner is an OBB of its
+// local bounds (a sphere: its bounding cube; spheres are an extension).
+static bool vox_collide(const World& w, const Body& rA, const Body& rB, uint32_t code, Manifold& m) {
+  m.count = 0;
+  m.key = (int32_t)(90u + code);
+  m.error = 0;
+  if (!vox_prefilter(rA, rB)) return false;
+  const bool a_vox = is_voxel(rA), b_vox = is_voxel(rB);
+  const VShape& sA = w.vshapes[a_vox ? (rA.shape >> 8) - 1u : 0u];
+  const VShape& sB = w.vshapes[b_vox ? (rB.shape >> 8) - 1u : 0u];
+  const uint32_t sa_n = sA.surf_count, sb_n = sB.surf_count;
+  const bool flip = !a_vox || (b_vox && sb_n < sa_n);
+  const Body& r1 = flip ? rB : rA;
+  const VShape& s1 = flip ? sB : sA;
+  const Body& r2 = flip ? rA : rB;
+  const VShape& s2 = flip ? sA : sB;
+  const bool s2_vox = flip ? a_vox : b_vox;
+  VoxReducer red;
+  red.init();
+  vox_sample(w, r1, s1, r2, s2, s2_vox, red, code);
+  if (red.sample_count == 0u) {
+    if (code == 0u && red.interior_hits > 0u) {
+      // deep-interpenetration rescue: one centre-to-centre contact
+      const f3 d = sub(r1.position, r2.position);
+      const f3 n_res = dot(d, d) > 1e-8f ? normalize(d) : mk(0.0f, 1.0f, 0.0f);
+      m.normal = flip ? n_res : neg(n_res);
+      Contact& ct = m.c[0];
+      std::memset(&ct, 0, sizeof(ct));
+      ct.position = muls(add(r1.position, r2.position), 0.5f);
+      ct.penetration = -0.5f * s1.vs;
+      ct.fp.in_ref = ct.fp.out_ref = ct.fp.in_inc = ct.fp.out_inc = 0xFFFFFFu;
+      m.count = 1;
+      return true;
+    }
+    return false;
+  }
+  const uint32_t ax = code / 2u;
+  const float sgn = (code & 1u) != 0u ? 1.0f : -1.0f;
+  const f3 n_world = qrotate(r2.rotation, mk(ax == 0u ? sgn : 0.0f, ax == 1u ? sgn : 0.0f, ax == 2u ? sgn : 0.0f));
+  return vox_emit(w, red, flip, (int)(90u + code), n_world, s1.vs, r2, s2, s2_vox, ax, m);
+}
+
 // ======================================================== PGS_SOFT solver
 // PGS_SOFT::pre_step / solve (src/shaders/simulation/solvers.slang:142-279)
 struct SolveCtx {
@@ -1283,7 +1604,6 @@ static void do_step(World& w) {
     broad_phase_query(nodes, i, mn, mx, pairs);
   }
   std::sort(pairs.begin(), pairs.end());
-  w.tap_pairs = pairs;
   st.broad_phase_collision_count = (uint32_t)pairs.size();
 
   // narrow phase (collision_detection.slang:1514-1755)
@@ -1302,54 +1622,75 @@ static void do_step(World& w) {
     swapped = (pa != lo);
     return &*it;
   };
+  // voxel pairs: up to six manifolds, keys 90..95, warm-started only from the
+  // same ordered pair under the same key (find_old_collision's exact_only,
+  // collision_detection.slang:914-963)
+  auto find_old_key = [&](uint32_t pa, uint32_t pb, int32_t key) -> const Manifold* {
+    if (pa > pb) return nullptr;  // the roles swapped: no exact match
+    auto it = std::lower_bound(prev_manifolds.begin(), prev_manifolds.end(), pa,
+                               [&](const Manifold& m, uint32_t v) { return m.a < v || (m.a == v && m.b < pb); });
+    for (; it != prev_manifolds.end() && it->a == pa && it->b == pb; ++it)
+      if (it->key == key) return &*it;
+    return nullptr;
+  };
+  w.tap_pairs.clear();
   for (auto& pr : pairs) {
     Body& r1 = w.rows[pr.first];
     Body& r2 = w.rows[pr.second];
+    const bool vox = is_voxel(r1) || is_voxel(r2);
+    for (uint32_t code = 0; code < (vox ? 6u : 1u); ++code) w.tap_pairs.push_back(pr);
     if (!is_dyn(r1) && !is_dyn(r2)) continue;
-    Manifold m;
-    std::memset(&m, 0, sizeof(m));
-    bool have;
-    if (r1.shape == BB_SHAPE_BOX && r2.shape == BB_SHAPE_BOX) {
-      have = obb_obb(r1, r2, m);
-      if (have)
-        for (int ci = 0; ci < m.count; ++ci) m.c[ci].penetration = fmaxr(m.c[ci].penetration, -0.25f);
-    } else if (r1.shape == BB_SHAPE_SPHERE && r2.shape == BB_SHAPE_SPHERE) {
-      have = sphere_sphere(r1, r2, m);
-    } else {
-      have = box_sphere(r1, r2, m);
-    }
-    if (!have) continue;
-    m.a = pr.first;
-    m.b = pr.second;
-    m.anchor_a = r1.position;
-    m.anchor_b = r2.position;
-    m.error = 0x7FFFFFFF;
-    compute_basis(m);
-    uint32_t pa = r1.id < w.prev_entries.size() ? w.prev_entries[r1.id] : MAXU;
-    uint32_t pb = r2.id < w.prev_entries.size() ? w.prev_entries[r2.id] : MAXU;
-    if (warm && pa < prev_count && pb < prev_count) {
-      bool swapped = false;
-      const Manifold* old = find_old(pa, pb, swapped);
-      if (old) update_from_old(m, *old, swapped);
-    }
-    for (int ci = 0; ci < m.count; ++ci) {
-      if (m.c[ci].warm_start == 0) ++fresh;
-      float cp = m.c[ci].penetration;
-      if (cp < 0.0f) pen_mm = std::max(pen_mm, (uint32_t)fminr(-cp * 1000.0f, 1.0e9f));
-    }
-    ms.push_back(m);
-    // IslandBuilder::link_bodies (islands.slang:153-213): root = min index
-    uint32_t f = is_dyn(r1) ? r1.active_index : MAXU;
-    uint32_t s = is_dyn(r2) ? r2.active_index : MAXU;
-    if (f < A && s < A) {
-      auto root = [&](uint32_t x) { while (link[x] != x) x = link[x]; return x; };
-      uint32_t ra = root(f), rb = root(s);
-      if (ra != rb) {
-        if (ra < rb) link[rb] = ra; else link[ra] = rb;
+    bool linked = false;
+    for (uint32_t code = 0; code < (vox ? 6u : 1u); ++code) {
+      Manifold m;
+      std::memset(&m, 0, sizeof(m));
+      bool have;
+      if (vox) {
+        have = vox_collide(w, r1, r2, code, m);
+      } else if (r1.shape == BB_SHAPE_BOX && r2.shape == BB_SHAPE_BOX) {
+        have = obb_obb(r1, r2, m);
+        if (have)
+          for (int ci = 0; ci < m.count; ++ci) m.c[ci].penetration = fmaxr(m.c[ci].penetration, -0.25f);
+      } else if (r1.shape == BB_SHAPE_SPHERE && r2.shape == BB_SHAPE_SPHERE) {
+        have = sphere_sphere(r1, r2, m);
+      } else {
+        have = box_sphere(r1, r2, m);
       }
-      uint32_t lo = std::min(ra, rb);
-      link[f] = std::min(link[f], lo);
-      link[s] = std::min(link[s], lo);
+      if (!have) continue;
+      m.a = pr.first;
+      m.b = pr.second;
+      m.anchor_a = r1.position;
+      m.anchor_b = r2.position;
+      m.error = 0x7FFFFFFF;
+      compute_basis(m);
+      uint32_t pa = r1.id < w.prev_entries.size() ? w.prev_entries[r1.id] : MAXU;
+      uint32_t pb = r2.id < w.prev_entries.size() ? w.prev_entries[r2.id] : MAXU;
+      if (warm && pa < prev_count && pb < prev_count) {
+        bool swapped = false;
+        const Manifold* old = vox ? find_old_key(pa, pb, m.key) : find_old(pa, pb, swapped);
+        if (old) update_from_old(m, *old, swapped);
+      }
+      for (int ci = 0; ci < m.count; ++ci) {
+        if (m.c[ci].warm_start == 0) ++fresh;
+        float cp = m.c[ci].penetration;
+        if (cp < 0.0f) pen_mm = std::max(pen_mm, (uint32_t)fminr(-cp * 1000.0f, 1.0e9f));
+      }
+      ms.push_back(m);
+      if (linked) continue;
+      linked = true;
+      // IslandBuilder::link_bodies (islands.slang:153-213): root = min index
+      uint32_t f = is_dyn(r1) ? r1.active_index : MAXU;
+      uint32_t s = is_dyn(r2) ? r2.active_index : MAXU;
+      if (f < A && s < A) {
+        auto root = [&](uint32_t x) { while (link[x] != x) x = link[x]; return x; };
+        uint32_t ra = root(f), rb = root(s);
+        if (ra != rb) {
+          if (ra < rb) link[rb] = ra; else link[ra] = rb;
+        }
+        uint32_t lo = std::min(ra, rb);
+        link[f] = std::min(link[f], lo);
+        link[s] = std::min(link[s], lo);
+      }
     }
   }
   const uint32_t M = (uint32_t)ms.size();
@@ -1884,10 +2225,122 @@ void* oracle_create(const bb_world_desc* d) {
 
 void oracle_destroy(void* h) { delete (World*)h; }
 
+// SceneManager::build_voxel_shape (src/scene_manager.hpp:478-566) from an
+// occupancy bitmask: centre of mass in double, per-cell parallel-axis inertia
+// plus the cell's own cube term in float, glm::inverse of the 3x3 (adjugate
+// over the determinant), and the surface list (cells with an empty face
+// neighbour, first empty direction -x,+x,-y,+y,-z,+z as the normal code).
+int oracle_add_voxel_shape(void* h, const bb_voxel_shape_desc* d, bb_voxel_shape_info* out) {
+  World* w = (World*)h;
+  if (!d || !out || !d->occupancy) return -1;
+  const uint32_t dx = d->dims[0], dy = d->dims[1], dz = d->dims[2];
+  if (dx == 0 || dy == 0 || dz == 0 || dx > 255 || dy > 255 || dz > 255) return -1;
+  if (!(d->voxel_size > 0.0f) || !(d->density > 0.0f)) return -1;
+  auto solid = [&](int x, int y, int z) -> bool {
+    if (x < 0 || y < 0 || z < 0 || x >= (int)dx || y >= (int)dy || z >= (int)dz) return false;
+    const uint32_t bit = (uint32_t)x + (uint32_t)y * dx + (uint32_t)z * dx * dy;
+    return ((d->occupancy[bit / 32u] >> (bit % 32u)) & 1u) != 0u;
+  };
+  const float vs = d->voxel_size;
+  double cx = 0.0, cy = 0.0, cz = 0.0;
+  uint32_t count = 0;
+  for (uint32_t z = 0; z < dz; ++z)
+    for (uint32_t y = 0; y < dy; ++y)
+      for (uint32_t x = 0; x < dx; ++x)
+        if (solid((int)x, (int)y, (int)z)) {
+          cx += x + 0.5; cy += y + 0.5; cz += z + 0.5;
+          ++count;
+        }
+  if (count == 0) return -1;
+  cx = cx / (double)count * (double)vs;
+  cy = cy / (double)count * (double)vs;
+  cz = cz / (double)count * (double)vs;
+  const float voxel_mass = d->density * vs * vs * vs;
+  const float mass = voxel_mass * (float)count;
+  const f3 origin = mk(-(float)cx, -(float)cy, -(float)cz);
+  const uint32_t bits = dx * dy * dz;
+  const uint32_t words = (bits + 31u) / 32u;
+  const uint32_t occ_off = (uint32_t)w->vocc.size(), surf_off = (uint32_t)w->vsurf.size();
+  std::vector<uint32_t> surf;
+  float I[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+  const float self = voxel_mass * vs * vs / 6.0f;
+  static const int nbr[6][3] = {{-1, 0, 0}, {1, 0, 0}, {0, -1, 0}, {0, 1, 0}, {0, 0, -1}, {0, 0, 1}};
+  for (uint32_t z = 0; z < dz; ++z)
+    for (uint32_t y = 0; y < dy; ++y)
+      for (uint32_t x = 0; x < dx; ++x) {
+        if (!solid((int)x, (int)y, (int)z)) continue;
+        const f3 c = add(origin, muls(mk((float)x + 0.5f, (float)y + 0.5f, (float)z + 0.5f), vs));
+        const float cc[3] = {c.x, c.y, c.z};
+        const float r2 = (c.x * c.x + c.y * c.y) + c.z * c.z;
+        for (int i = 0; i < 3; ++i)
+          for (int j = 0; j < 3; ++j) {
+            const float e = (i == j ? r2 : 0.0f) - cc[j] * cc[i];
+            I[i][j] = I[i][j] + voxel_mass * e;
+          }
+        for (int i = 0; i < 3; ++i)
+          for (int j = 0; j < 3; ++j) I[i][j] = I[i][j] + (i == j ? self : 0.0f);
+        for (uint32_t k = 0; k < 6; ++k)
+          if (!solid((int)x + nbr[k][0], (int)y + nbr[k][1], (int)z + nbr[k][2])) {
+            surf.push_back(x | (y << 8) | (z << 16) | (k << 24));
+            break;
+          }
+      }
+  if (w->vshapes.size() >= BB_MAX_VOXEL_SHAPES || occ_off + words > BB_MAX_VOXEL_OCC_WORDS ||
+      surf_off + surf.size() > BB_MAX_VOXEL_SURFACE)
+    return -2;
+  // glm::inverse (column-major m[c][r]; the matrix is symmetric, so m[c][r] = I[c][r])
+  const float (&m)[3][3] = I;
+  const float det = (m[0][0] * (m[1][1] * m[2][2] - m[2][1] * m[1][2]) -
+                     m[1][0] * (m[0][1] * m[2][2] - m[2][1] * m[0][2])) +
+                    m[2][0] * (m[0][1] * m[1][2] - m[1][1] * m[0][2]);
+  const float k = 1.0f / det;
+  float inv[3][3];
+  inv[0][0] = (m[1][1] * m[2][2] - m[2][1] * m[1][2]) * k;
+  inv[1][0] = -(m[1][0] * m[2][2] - m[2][0] * m[1][2]) * k;
+  inv[2][0] = (m[1][0] * m[2][1] - m[2][0] * m[1][1]) * k;
+  inv[0][1] = -(m[0][1] * m[2][2] - m[2][1] * m[0][2]) * k;
+  inv[1][1] = (m[0][0] * m[2][2] - m[2][0] * m[0][2]) * k;
+  inv[2][1] = -(m[0][0] * m[2][1] - m[2][0] * m[0][1]) * k;
+  inv[0][2] = (m[0][1] * m[1][2] - m[1][1] * m[0][2]) * k;
+  inv[1][2] = -(m[0][0] * m[1][2] - m[1][0] * m[0][2]) * k;
+  inv[2][2] = (m[0][0] * m[1][1] - m[1][0] * m[0][1]) * k;
+  w->vocc.resize(occ_off + words, 0u);
+  for (uint32_t i = 0; i < words; ++i) w->vocc[occ_off + i] = d->occupancy[i];
+  if (bits % 32u) w->vocc[occ_off + words - 1] &= (1u << (bits % 32u)) - 1u;
+  w->vsurf.insert(w->vsurf.end(), surf.begin(), surf.end());
+  World::VShape s;
+  s.dims[0] = dx; s.dims[1] = dy; s.dims[2] = dz;
+  s.vs = vs;
+  s.origin = origin;
+  s.occ_off = occ_off;
+  s.surf_off = surf_off;
+  s.surf_count = (uint32_t)surf.size();
+  w->vshapes.push_back(s);
+  std::memset(out, 0, sizeof(*out));
+  out->shape_id = (uint32_t)w->vshapes.size();
+  out->solid_count = count;
+  out->surface_count = s.surf_count;
+  out->mass = mass;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) out->inv_inertia[i * 3 + j] = inv[j][i];  // row r = glm column r
+  out->minimum[0] = origin.x; out->minimum[1] = origin.y; out->minimum[2] = origin.z;
+  out->maximum[0] = origin.x + (float)dx * vs;
+  out->maximum[1] = origin.y + (float)dy * vs;
+  out->maximum[2] = origin.z + (float)dz * vs;
+  w->vinfo.push_back(*out);
+  return 0;
+}
+
 // SceneManager::load_scene body rules (src/scene_manager.hpp:850-887) and
 // cuboid_get_inverse_intertia (include/math.hpp:106-130).
 int oracle_add_bodies(void* h, const bb_body_desc* b, uint32_t n) {
   World* w = (World*)h;
+  for (uint32_t i = 0; i < n; ++i) {
+    const uint32_t k = shape_kind(b[i].shape), id = b[i].shape >> 8;
+    if (k > BB_SHAPE_VOXEL || (k != BB_SHAPE_VOXEL && id != 0) ||
+        (k == BB_SHAPE_VOXEL && (id == 0 || id > w->vshapes.size())))
+      return -1;
+  }
   for (uint32_t i = 0; i < n; ++i) {
     Body r;
     std::memset(&r, 0, sizeof(r));
@@ -1899,6 +2352,31 @@ int oracle_add_bodies(void* h, const bb_body_desc* b, uint32_t n) {
     r.position = mk(b[i].position[0], b[i].position[1], b[i].position[2]);
     r.rotation = quat{mk(b[i].rotation[0], b[i].rotation[1], b[i].rotation[2]), b[i].rotation[3]};
     r.radius = b[i].radius;
+    if (shape_kind(r.shape) == BB_SHAPE_VOXEL) {
+      // push_voxel_body (src/scene_manager.hpp:568-572): mass, inverse
+      // inertia and bounds of the shape; immovable without DYNAMIC
+      const World::VShape& vsh = w->vshapes[(r.shape >> 8) - 1u];
+      const bb_voxel_shape_info& vi = w->vinfo[(r.shape >> 8) - 1u];
+      (void)vsh;
+      r.radius = 0.0f;
+      r.minimum = mk(vi.minimum[0], vi.minimum[1], vi.minimum[2]);
+      r.maximum = mk(vi.maximum[0], vi.maximum[1], vi.maximum[2]);
+      const bool dyn = (r.flags & BB_BODY_DYNAMIC) != 0;
+      r.mass = dyn ? vi.mass : 0.0f;
+      r.inv_mass = dyn ? 1.0f / vi.mass : 0.0f;
+      std::memset(&r.inv_inertia, 0, sizeof(m3));
+      if (dyn)
+        for (int a = 0; a < 3; ++a)
+          for (int c = 0; c < 3; ++c) r.inv_inertia.m[a][c] = vi.inv_inertia[a * 3 + c];
+      r.velocity = mk(b[i].velocity[0], b[i].velocity[1], b[i].velocity[2]);
+      r.omega = mk(b[i].omega[0], b[i].omega[1], b[i].omega[2]);
+      r.restitution = b[i].restitution;
+      r.friction = b[i].friction;
+      w->rows.push_back(r);
+      w->entries.push_back((uint32_t)w->rows.size() - 1);
+      if (r.flags & BB_BODY_DYNAMIC) w->active.push_back(r.id);
+      continue;
+    }
     if (r.shape == BB_SHAPE_SPHERE) {
       r.minimum = mk(-r.radius, -r.radius, -r.radius);
       r.maximum = mk(r.radius, r.radius, r.radius);

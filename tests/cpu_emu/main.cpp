@@ -11,7 +11,27 @@ int main(int argc, char** argv) {
   uint32_t nw = 0;
   if (std::fread(&nw, 4, 1, f) != 1) return 3;
   bb_batch* b = nullptr;
+  const bool shapes = (nw & 0x80000000u) != 0;
+  nw &= 0x7FFFFFFFu;
   bb_batch_create(nw, 0, &b);
+  if (shapes) {
+    // voxel shapes: count, then dims[3], voxel size, density, word count, words
+    uint32_t ns = 0;
+    if (std::fread(&ns, 4, 1, f) != 1) return 3;
+    for (uint32_t k = 0; k < ns; ++k) {
+      bb_voxel_shape_desc vd{};
+      uint32_t nwords = 0;
+      if (std::fread(vd.dims, 4, 3, f) != 3 || std::fread(&vd.voxel_size, 4, 1, f) != 1 ||
+          std::fread(&vd.density, 4, 1, f) != 1 || std::fread(&nwords, 4, 1, f) != 1)
+        return 3;
+      std::vector<uint32_t> words(nwords);
+      if (nwords && std::fread(words.data(), 4, nwords, f) != nwords) return 3;
+      vd.occupancy = words.data();
+      bb_voxel_shape_info info{};
+      int st = bb_batch_add_voxel_shape(b, &vd, &info);
+      if (st) { std::fprintf(stderr, "add_voxel_shape %d\n", st); return 4; }
+    }
+  }
   std::vector<uint32_t> ns;
   for (uint32_t w = 0; w < nw; ++w) {
     uint32_t dsz = 0, n = 0;

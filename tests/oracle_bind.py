@@ -16,7 +16,8 @@ REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 sys.path.insert(0, os.path.join(REPO, "beat-box_amd"))
 
 import beatbox  # noqa: E402
-from beatbox import BODY_STATE, MANIFOLD_TAP, LBVH_NODE, StepStats, WorldDesc  # noqa: E402
+from beatbox import (BODY_STATE, MANIFOLD_TAP, LBVH_NODE, StepStats, VoxelShapeDesc, VoxelShapeInfo,  # noqa: E402
+                     WorldDesc, voxel_desc)
 from scenes import BODY_DESC  # noqa: E402
 
 ORACLE_SO = os.path.join(REPO, "oracle", "_build", "liboracle.so")
@@ -38,6 +39,7 @@ def olib() -> C.CDLL:
         L.oracle_create.restype = C.c_void_p
         L.oracle_create.argtypes = [C.POINTER(WorldDesc)]
         L.oracle_destroy.argtypes = [C.c_void_p]
+        L.oracle_add_bodies.restype = C.c_int
         L.oracle_add_bodies.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32]
         L.oracle_step.argtypes = [C.c_void_p, C.c_uint32]
         L.oracle_set_flags.argtypes = [C.c_void_p, C.c_uint32]
@@ -61,6 +63,8 @@ def olib() -> C.CDLL:
         L.oracle_order_unmap.restype = C.c_float
         L.oracle_order_unmap.argtypes = [C.c_uint32]
         L.oracle_solve6.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.oracle_add_voxel_shape.restype = C.c_int
+        L.oracle_add_voxel_shape.argtypes = [C.c_void_p, C.POINTER(VoxelShapeDesc), C.POINTER(VoxelShapeInfo)]
         L.oracle_collide.restype = C.c_int
         L.oracle_collide.argtypes = [C.POINTER(WorldDesc), C.c_void_p, C.c_void_p, C.c_void_p]
         _OL = L
@@ -74,7 +78,8 @@ class OracleWorld:
 
     def add_bodies(self, bodies: np.ndarray) -> None:
         arr = np.ascontiguousarray(bodies, dtype=BODY_DESC)
-        olib().oracle_add_bodies(self._h, arr.ctypes.data_as(C.c_void_p), len(arr))
+        if olib().oracle_add_bodies(self._h, arr.ctypes.data_as(C.c_void_p), len(arr)) != 0:
+            raise ValueError("oracle_add_bodies: invalid body")
         self.n = getattr(self, "n", 0) + len(arr)
 
     def step(self, n: int = 1) -> None:
@@ -82,6 +87,15 @@ class OracleWorld:
 
     def set_flags(self, f: int) -> None:
         olib().oracle_set_flags(self._h, f)
+
+    def add_voxel_shape(self, solid: np.ndarray, voxel_size: float, density: float) -> dict:
+        d, words = voxel_desc(solid, voxel_size, density)
+        info = VoxelShapeInfo()
+        rc = olib().oracle_add_voxel_shape(self._h, C.byref(d), C.byref(info))
+        del words
+        if rc != 0:
+            raise ValueError(f"oracle_add_voxel_shape: {rc}")
+        return info.as_dict()
 
     def set_solver(self, s: int) -> None:
         olib().oracle_set_solver(self._h, s)

@@ -1,0 +1,99 @@
+"""Voxel (concave) bodies on the GPU against the oracle, bit for bit: the
+reference's voxel scene (Ls, crosses and frames, src/scene_manager.hpp:
+614-655) with AVBD, and voxel shapes mixed with boxes and spheres under
+PGS_SOFT, every step's taps (expanded pair slots, manifolds with keys 90..95,
+islands, colours) and counters included; a voxel world in a batch equals the
+world alone; bad voxel bodies and checkpoints of voxel worlds are refused."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import beatbox
+import scenes
+from oracle_bind import OracleWorld
+from parity_util import compare_step
+
+pytestmark = pytest.mark.gpu
+
+
+def _pair(desc_kw, shapes, bodies):
+    desc = beatbox.WorldDesc.make(**desc_kw)
+    gpu = beatbox.RigidBodyManager(desc)
+    ora = OracleWorld(desc)
+    for _, cells in shapes:
+        a = gpu.add_voxel_shape(cells, scenes.VOXEL_SIZE, scenes.VOXEL_DENSITY)
+        b = ora.add_voxel_shape(cells, scenes.VOXEL_SIZE, scenes.VOXEL_DENSITY)
+        for k in ("shape_id", "solid_count", "surface_count", "mass"):
+            assert a[k] == b[k], k
+        for k in ("inv_inertia", "minimum", "maximum"):
+            assert a[k].tobytes() == b[k].tobytes(), k
+    gpu.add_bodies(bodies)
+    ora.add_bodies(bodies)
+    return desc, gpu, ora
+
+
+def _parity(desc_kw, shapes, bodies, steps):
+    desc, gpu, ora = _pair(desc_kw, shapes, bodies)
+    for s in range(steps):
+        gpu.simulate(1)
+        ora.step(1)
+        bad = compare_step(gpu, ora, desc.morton_bits, s)
+        assert not bad, "\n".join(bad[:20])
+    st = gpu.read_back_sim_config()
+    gpu.destroy()
+    return st
+
+
+def test_reference_voxel_scene_avbd_parity():
+    shapes, bodies = scenes.voxel_pile()
+    kw = scenes.world_desc(solver=scenes.SOLVER_AVBD, iterations=10, morton_bits=63, quirks=0)
+    st = _parity(kw, shapes, bodies, steps=120)
+    assert st["collision_count"] > 5
+
+
+def test_voxel_boxes_spheres_pgs_soft_parity():
+    shapes, bodies = scenes.voxel_mixed_pile(0xBB62, 40)
+    kw = scenes.world_desc(iterations=8, morton_bits=63, quirks=0)
+    st = _parity(kw, shapes, bodies, steps=80)
+    assert st["collision_count"] >= 5
+
+
+def test_voxel_world_in_a_batch_equals_alone():
+    shapes, vox = scenes.voxel_mixed_pile(0xBB63, 30)
+    desc_v = beatbox.WorldDesc.make(**scenes.world_desc(solver=scenes.SOLVER_AVBD, iterations=10,
+                                                        morton_bits=63, quirks=0))
+    desc_p = beatbox.WorldDesc.make(**scenes.world_desc(iterations=8, morton_bits=63, quirks=0))
+    worlds = [(desc_p, scenes.serving_pile(53, 300)), (desc_v, vox), (desc_p, scenes.batch_world(0xBB04, 2, 200))]
+    b = beatbox.Batch()
+    for _, cells in shapes:
+        b.add_voxel_shape(cells, scenes.VOXEL_SIZE, scenes.VOXEL_DENSITY)
+    for d, bodies in worlds:
+        b.add_world(d, bodies)
+    b.step(30, use_graph=True)
+    b.sync()
+    for k, (d, bodies) in enumerate(worlds):
+        w = beatbox.RigidBodyManager(d)
+        for _, cells in shapes:
+            w.add_voxel_shape(cells, scenes.VOXEL_SIZE, scenes.VOXEL_DENSITY)
+        w.add_bodies(bodies)
+        w.simulate(30)
+        x, y = b.read_world(k, len(bodies)), w.read_bodies()
+        for f in ("position", "rotation", "velocity", "omega", "flags"):
+            assert x[f].tobytes() == y[f].tobytes(), (k, f)
+        w.destroy()
+    b.destroy()
+
+
+def test_voxel_errors():
+    w = beatbox.RigidBodyManager(beatbox.WorldDesc.make(**scenes.world_desc(morton_bits=63, quirks=0)))
+    with pytest.raises(beatbox.BBError):
+        w.add_bodies(scenes.voxel_body(1, (0.0, 1.0, 0.0)))  # no shape yet
+    with pytest.raises(beatbox.BBError):
+        w.add_voxel_shape(np.zeros((2, 2, 2), bool), 0.5, 2.0)  # empty grid
+    info = w.add_voxel_shape(np.ones((2, 2, 2), bool), 0.5, 2.0)
+    w.add_bodies(scenes.voxel_body(info["shape_id"], (0.0, 1.0, 0.0)))
+    w.simulate(2)
+    with pytest.raises(beatbox.BBError):
+        w.snapshot()  # checkpoints do not carry voxel shape pools
+    w.destroy()
