@@ -497,6 +497,10 @@ bb_status build(bb_batch* b) {
   if (st != BB_OK) return st;
   b->latest = 1;  // the first step reads set 1 and writes set 0
   b->base.pgs_lcap = pgs_lds_rows(b);
+  b->base.any_voxel = 0;
+  for (auto& hw : b->worlds)
+    if (hw.has_voxel) b->base.any_voxel = 1;
+  b->base.any_plain = any_plain(b);
   b->built = true;
   b->dirty = false;
   b->stepped = false;
@@ -613,11 +617,29 @@ bb_status upload_infos(bb_batch* b, bool one_shot) {
   return BB_OK;
 }
 
+// Whether any world runs plain PGS on the per-island path (k_pgs_plain is
+// then part of the launch sequence).
+uint32_t any_plain(const bb_batch* b) {
+  for (const WorldInfo& wi : b->infos)
+    if (wi.solver == BB_SOLVER_PGS) return 1u;
+  return 0u;
+}
+
 bb_status refresh_infos(bb_batch* b) {
   for (size_t w = 0; w < b->worlds.size(); ++w) {
     WorldInfo wi = info_of(b->worlds[w]);
     wi.mode = b->infos[w].mode;
     b->infos[w] = wi;
+  }
+  // a solver or flag change can add or drop k_pgs_plain: a captured step
+  // graph holds the old launch sequence
+  const uint32_t p = any_plain(b);
+  if (p != b->base.any_plain) {
+    b->base.any_plain = p;
+    if (b->graph) {
+      (void)hipGraphExecDestroy(b->graph);
+      b->graph = nullptr;
+    }
   }
   return BB_OK;
 }

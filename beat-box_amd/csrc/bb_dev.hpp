@@ -117,7 +117,10 @@ struct Manifolds {  // pair-indexed slots; contact arrays are [slot*8 + c]
 struct Dev {
   WorldInfo* worlds;
   uint32_t W;
-  // voxel shape pools, shared by the batch's worlds
+  // voxel shape pools, shared by the batch's worlds; any_voxel: a world has
+  // voxel bodies (k_narrow_vox is launched); any_plain: a world runs PGS on
+  // the per-island path (k_pgs_plain is launched)
+  uint32_t any_voxel, any_plain;
   const VoxShape* vshape;
   const uint32_t* vocc;
   const uint32_t* vsurf;

@@ -1711,10 +1711,13 @@ BB_DI uint32_t warm_start(const Dev& d, uint32_t os, bool swapped, int key, int 
 // key, count} (count 0: no manifold), the normal, the body centres (the
 // first one's w holds the deepest penetration, min(0, pen)) and the
 // contacts' positions, penetrations and feature pairs.
-template <class T>
+// kVox: the instance for worlds with voxel bodies (k_narrow_vox); the other
+// instance skips them, so the box and sphere kernels carry no voxel code.
+template <bool kVox, class T>
 BB_DI void s_narrow(const Dev& d, uint32_t w, const T& t) {
   const uint32_t tid = t.rank(), NT = t.size();
   const WorldInfo W = d.worlds[w];
+  if (((W.mode & kModeVoxel) != 0) != kVox) return;
   const uint32_t off = W.body_off, P = d.npairs_cur[w];
   const uint2* pairs = d.pairs_cur + W.pair_off;
   for (uint32_t p = tid; p < P; p += NT) {
@@ -1731,7 +1734,7 @@ BB_DI void s_narrow(const Dev& d, uint32_t w, const T& t) {
     CSink sk{d.man_cur.cpos, d.man_cur.cfp, slot * 8, false, 0.0f};
     int key, count;
     v3 n;
-    if (A.kind == BB_SHAPE_VOXEL || B.kind == BB_SHAPE_VOXEL) {
+    if (kVox && (A.kind == BB_SHAPE_VOXEL || B.kind == BB_SHAPE_VOXEL)) {
       // the pair's contact-face code is its slot's rank among the pair's six
       uint32_t code = 0;
       while (code < 5u && code < p && pairs[p - code - 1].x == pr.x && pairs[p - code - 1].y == pr.y) ++code;
@@ -1756,11 +1759,19 @@ BB_DI void s_narrow(const Dev& d, uint32_t w, const T& t) {
 }
 
 template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_narrow_vox(Dev d) {
+  const uint32_t w = d.order[blockIdx.x];
+  if (d.worlds[w].mode & kModeTeam) return;
+  const BlockTeam<BLOCK> t{nullptr, nullptr};
+  s_narrow<true>(d, w, t);
+}
+
+template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_narrow(Dev d) {
   const uint32_t w = d.order[blockIdx.x];
   if (d.worlds[w].mode & kModeTeam) return;
   const BlockTeam<BLOCK> t{nullptr, nullptr};
-  s_narrow(d, w, t);
+  s_narrow<false>(d, w, t);
 }
 
 // Narrow phase, part 2, per manifold: tangent basis, warm start from the
@@ -2515,6 +2526,8 @@ struct VelCache {
 // Contacts per row whose solver data pgs_row loads up front.
 static constexpr int kPgsPre = 4;
 // kind 0: warm-start impulses of pre_step; 1: solve with bias; 2: relax.
+// kPlain: PGS (the per-island path) instead of PGS_SOFT.
+template <bool kPlain>
 BB_DI void pgs_row(const Dev& d, const VelCache& vc, uint32_t off, uint32_t k, const Pgs& s, int kind) {
   const uint32_t NP = d.np;
   const int4 h = d.sv_hdr[k];
@@ -2609,7 +2622,7 @@ BB_DI void pgs_row(const Dev& d, const VelCache& vc, uint32_t off, uint32_t k, c
       rv = ((vb + cross(wb, r2)) - va) - cross(wa, r1);
       const float nv = dot(rv, n);
       float imp;
-      if (s.plain) {
+      if (kPlain) {
         // PGS::solve (solvers.slang:106-117)
         imp = nm * (-nv + plain_bias(pen, s.inv_dt));
       } else {
@@ -2665,16 +2678,19 @@ BB_DI qt apply_angular_velocity(qt q, v3 om, float dt) {
   return qnorm(qmul(qt{dr, 1.0f}, q));
 }
 
-template <class T>
+// kPlain: the instance for PGS worlds (the per-island path, k_pgs_plain);
+// the other instance steps PGS_SOFT worlds and carries no PGS code.
+template <bool kPlain, class T>
 BB_DI void s_pgs(const Dev& d, uint32_t w, const T& t, float4* pgs_lds, uint32_t lcap) {
   const uint32_t tid = t.rank(), NT = t.size();
   const WorldInfo W = d.worlds[w];
   if (W.solver == BB_SOLVER_AVBD || W.n == 0) return;
+  if ((W.solver == BB_SOLVER_PGS) != kPlain) return;
   const uint32_t off = W.body_off, N = W.n, A = W.a, NP = d.np;
   Pgs s;
   s.accum = (W.flags & BB_SIM_ACCUM_IMPULSE) != 0;
   s.friction = (W.flags & BB_SIM_FRICTION) != 0;
-  s.plain = W.solver == BB_SOLVER_PGS;
+  s.plain = kPlain;
   s.inv_dt = W.dt > 0.0f ? 1.0f / W.dt : 0.0f;
   {
     const float hertz = fmn(30.0f, s.inv_dt * 0.3333333333333333f);
@@ -2698,15 +2714,16 @@ BB_DI void s_pgs(const Dev& d, uint32_t w, const T& t, float4* pgs_lds, uint32_t
     pgs_lds[kBodyLds * i + 4] = d.iw[g * 3 + 2];
   }
   t.sync();
+  auto row = [&](uint32_t k, int kind) { pgs_row<kPlain>(d, vc, off, base + k, s, kind); };
   auto sweep = [&](int kind) {
     for (uint32_t c = 0; c < used; ++c) {
       if (bo[c] == bo[c + 1]) continue;
-      for (uint32_t k = bo[c] + tid; k < bo[c + 1]; k += NT) pgs_row(d, vc, off, base + k, s, kind);
+      for (uint32_t k = bo[c] + tid; k < bo[c + 1]; k += NT) row(k, kind);
       t.sync();
     }
     if (ovf_hi > ovf_lo) {
       if (tid == 0)
-        for (uint32_t k = ovf_lo; k < ovf_hi; ++k) pgs_row(d, vc, off, base + k, s, kind);
+        for (uint32_t k = ovf_lo; k < ovf_hi; ++k) row(k, kind);
       t.sync();
     }
   };
@@ -2781,7 +2798,16 @@ __global__ __launch_bounds__(BLOCK, 4) void k_pgs(Dev d) {
   const uint32_t w = d.order[blockIdx.x];
   if (d.worlds[w].mode & kModeTeam) return;
   const BlockTeam<BLOCK> t{nullptr, nullptr};
-  s_pgs(d, w, t, pgs_lds, d.pgs_lcap);
+  s_pgs<false>(d, w, t, pgs_lds, d.pgs_lcap);
+}
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK, 4) void k_pgs_plain(Dev d) {
+  HIP_DYNAMIC_SHARED(float4, pgs_lds)
+  const uint32_t w = d.order[blockIdx.x];
+  if (d.worlds[w].mode & kModeTeam) return;
+  const BlockTeam<BLOCK> t{nullptr, nullptr};
+  s_pgs<true>(d, w, t, pgs_lds, d.pgs_lcap);
 }
 
 // ================================================================== AVBD
@@ -3453,13 +3479,14 @@ __global__ __launch_bounds__(BLOCK) void k_team(Dev d) {
   t.sync();
   s_broad(d, w, t, G.broad);
   t.sync();
-  s_narrow(d, w, t);
+  s_narrow<false>(d, w, t);  // voxel worlds never run as teams
   t.sync();
   s_contacts(d, w, t);
   t.sync();
   s_post(d, w, t, G.post);
   t.sync();
-  s_pgs(d, w, t, nullptr, 0u);
+  if (d.worlds[w].solver == BB_SOLVER_PGS) s_pgs<true>(d, w, t, nullptr, 0u);
+  else s_pgs<false>(d, w, t, nullptr, 0u);
   t.sync();
   s_avbd(d, w, t, G.avbd);
   t.sync();
@@ -3604,9 +3631,11 @@ template __global__ void k_team<kTeamBlock>(Dev);
   template __global__ void k_build<B>(Dev);    \
   template __global__ void k_broad<B>(Dev);    \
   template __global__ void k_narrow<B>(Dev);   \
+  template __global__ void k_narrow_vox<B>(Dev); \
   template __global__ void k_contacts<B>(Dev); \
   template __global__ void k_post<B>(Dev);     \
   template __global__ void k_pgs<B>(Dev);      \
+  template __global__ void k_pgs_plain<B>(Dev); \
   template __global__ void k_avbd<B>(Dev);     \
   template __global__ void k_finish<B>(Dev);
 
@@ -3621,10 +3650,14 @@ static void launch_stage(int stage, const Dev& d, hipStream_t st, uint32_t grid)
     case 1: hipLaunchKernelGGL(k_broad<B>, g, b, 0, st, d); break;
     case 2:
       hipLaunchKernelGGL(k_narrow<B>, g, b, 0, st, d);
+      if (d.any_voxel) hipLaunchKernelGGL(k_narrow_vox<B>, g, b, 0, st, d);
       hipLaunchKernelGGL(k_contacts<B>, g, b, 0, st, d);
       break;
     case 3: hipLaunchKernelGGL(k_post<B>, g, b, 0, st, d); break;
-    case 4: hipLaunchKernelGGL(k_pgs<B>, g, b, (size_t)d.pgs_lcap * 16 * kBodyLds, st, d); break;
+    case 4:
+      hipLaunchKernelGGL(k_pgs<B>, g, b, (size_t)d.pgs_lcap * 16 * kBodyLds, st, d);
+      if (d.any_plain) hipLaunchKernelGGL(k_pgs_plain<B>, g, b, (size_t)d.pgs_lcap * 16 * kBodyLds, st, d);
+      break;
     case 5: hipLaunchKernelGGL(k_avbd<B>, g, b, 0, st, d); break;
     default: hipLaunchKernelGGL(k_finish<B>, g, b, 0, st, d); break;
   }
