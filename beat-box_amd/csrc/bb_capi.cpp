@@ -795,13 +795,23 @@ struct SnapHeader {
   uint32_t version, n, npairs, warm;
   uint64_t step_index;
   bb_world_desc desc;
+  uint32_t nshapes, shape_words;  // voxel shapes and their occupancy words
 };
-constexpr char kSnapMagic[8] = {'B', 'B', 'S', 'N', 'A', 'P', '0', '1'};
-constexpr uint32_t kSnapVersion = 1;
-// bytes of the arrays of n rows and p manifold slots
-size_t snap_bytes(uint32_t n, uint32_t p) {
-  return sizeof(SnapHeader) + (size_t)n * sizeof(bb_body_desc) + (size_t)n * (6 * 4 + 5 * 16) +
-         (size_t)p * (8 + 6 * 16) + (size_t)p * 8 * (6 * 16 + 4) + ST_COUNT * 4;
+constexpr char kSnapMagic[8] = {'B', 'B', 'S', 'N', 'A', 'P', '0', '2'};
+constexpr uint32_t kSnapVersion = 2;
+// a voxel shape's record in the blob: dims, cell size, density, word count;
+// its occupancy words follow
+struct SnapShape {
+  uint32_t dims[3];
+  float voxel_size, density;
+  uint32_t words;
+};
+// bytes of the arrays of n rows and p manifold slots, after the header and
+// the voxel shape section
+size_t snap_bytes(uint32_t n, uint32_t p, uint32_t nshapes = 0, uint32_t shape_words = 0) {
+  return sizeof(SnapHeader) + (size_t)nshapes * sizeof(SnapShape) + (size_t)shape_words * 4 +
+         (size_t)n * sizeof(bb_body_desc) + (size_t)n * (6 * 4 + 5 * 16) + (size_t)p * (8 + 6 * 16) +
+         (size_t)p * 8 * (6 * 16 + 4) + ST_COUNT * 4;
 }
 
 // The blob's arrays in order, each with its device source (world slice of the
@@ -845,7 +855,8 @@ bb_status snapshot_world(bb_batch* b, void* dst, size_t cap, size_t* size) {
   // warm-start state exists once a step has run and no one-shot drop is pending
   const bool warm = b->stepped && !hw.drop_prev && !(b->infos[0].mode & kModeDropPrev);
   if (warm) HIPCHK(hipMemcpy(&p, b->npairs[b->latest], 4, hipMemcpyDeviceToHost));
-  *size = snap_bytes(n, p);
+  const uint32_t nshapes = (uint32_t)b->vox_shapes.size(), shape_words = (uint32_t)b->vox_occ.size();
+  *size = snap_bytes(n, p, nshapes, shape_words);
   if (!dst) return BB_OK;
   if (cap < *size) return BB_ERR_CAPACITY;
   SnapHeader h{};
@@ -856,9 +867,21 @@ bb_status snapshot_world(bb_batch* b, void* dst, size_t cap, size_t* size) {
   h.warm = (warm ? 1u : 0u) | (hw.skip_ws_once ? 2u : 0u);
   h.step_index = b->step_index;
   h.desc = hw.desc;
+  h.nshapes = nshapes;
+  h.shape_words = shape_words;
   char* o = (char*)dst;
   std::memcpy(o, &h, sizeof(h));
   o += sizeof(h);
+  // the voxel shapes, rebuilt on restore by the same build_voxel_shape
+  for (uint32_t k = 0; k < nshapes; ++k) {
+    const VoxShape& vs = b->vox_shapes[k];
+    const uint32_t cells = vs.dims[0] * vs.dims[1] * vs.dims[2];
+    SnapShape r{{vs.dims[0], vs.dims[1], vs.dims[2]}, vs.vs, b->vox_density[k], (cells + 31u) / 32u};
+    std::memcpy(o, &r, sizeof(r));
+    o += sizeof(r);
+    std::memcpy(o, b->vox_occ.data() + vs.occ_off, (size_t)r.words * 4);
+    o += (size_t)r.words * 4;
+  }
   std::memcpy(o, hw.bodies.data(), (size_t)n * sizeof(bb_body_desc));
   o += (size_t)n * sizeof(bb_body_desc);
   hipError_t rc = hipSuccess;
@@ -874,7 +897,9 @@ bb_status restore_world(const void* src, size_t size, int warm_start, bb_world**
   SnapHeader h;
   std::memcpy(&h, src, sizeof(h));
   if (std::memcmp(h.magic, kSnapMagic, 8) != 0 || h.version != kSnapVersion) return BB_ERR_INVALID;
-  if (size != snap_bytes(h.n, h.npairs) || h.npairs > pair_capacity(h.n)) return BB_ERR_INVALID;
+  if (h.nshapes > BB_MAX_VOXEL_SHAPES || h.shape_words > BB_MAX_VOXEL_OCC_WORDS) return BB_ERR_INVALID;
+  if (size != snap_bytes(h.n, h.npairs, h.nshapes, h.shape_words) || h.npairs > 6 * pair_capacity(h.n))
+    return BB_ERR_INVALID;
   const char* in = (const char*)src + sizeof(h);
   bb_world* w = nullptr;
   bb_status st = bb_world_create(&h.desc, &w);
@@ -883,6 +908,23 @@ bb_status restore_world(const void* src, size_t size, int warm_start, bb_world**
     bb_world_destroy(w);
     return s;
   };
+  uint32_t words_seen = 0;
+  for (uint32_t k = 0; k < h.nshapes; ++k) {
+    SnapShape r;
+    std::memcpy(&r, in, sizeof(r));
+    in += sizeof(r);
+    words_seen += r.words;
+    const uint64_t cells = (uint64_t)r.dims[0] * r.dims[1] * r.dims[2];
+    if (words_seen > h.shape_words || r.words != (cells + 31u) / 32u) return fail(BB_ERR_INVALID);
+    std::vector<uint32_t> occ(r.words);
+    std::memcpy(occ.data(), in, (size_t)r.words * 4);
+    in += (size_t)r.words * 4;
+    bb_voxel_shape_desc vd{{r.dims[0], r.dims[1], r.dims[2]}, r.voxel_size, r.density, occ.data()};
+    bb_voxel_shape_info info{};
+    st = bb_world_add_voxel_shape(w, &vd, &info);
+    if (st != BB_OK || info.shape_id != k + 1) return fail(st != BB_OK ? st : BB_ERR_INVALID);
+  }
+  if (words_seen != h.shape_words) return fail(BB_ERR_INVALID);
   if (h.n) {
     st = bb_world_add_bodies(w, (const bb_body_desc*)in, h.n, nullptr);
     if (st != BB_OK) return fail(st);
@@ -891,6 +933,7 @@ bb_status restore_world(const void* src, size_t size, int warm_start, bb_world**
   bb_batch* b = w->b;
   st = build(b);
   if (st != BB_OK) return fail(st);
+  if (h.npairs > b->worlds[0].pair_cap) return fail(BB_ERR_INVALID);
   HIPCHK(hipStreamSynchronize(b->stream));
   hipError_t rc = hipSuccess;
   snap_arrays(b, h.n, h.npairs, [&](void* dst, size_t bytes) {
@@ -1454,6 +1497,7 @@ bb_status add_voxel_shape(bb_batch* b, const bb_voxel_shape_desc* d, bb_voxel_sh
     info.maximum[k] = org[k] + (float)d->dims[k] * vs;
   }
   b->vox_info.push_back(info);
+  b->vox_density.push_back(d->density);
   b->vox_dirty = true;
   *out = info;
   return BB_OK;
@@ -1735,9 +1779,6 @@ bb_status bb_world_instance_transforms(bb_world* w, float* dst, uint32_t n, int 
 
 bb_status bb_world_snapshot(bb_world* w, void* dst, size_t cap, size_t* size) {
   if (!w) return BB_ERR_INVALID;
-  // the blob carries no voxel shape pools
-  for (const bb_body_desc& bd : w->b->worlds[0].bodies)
-    if (BB_SHAPE_KIND(bd.shape) == BB_SHAPE_VOXEL) return BB_ERR_INVALID;
   return snapshot_world(w->b, dst, cap, size);
 }
 

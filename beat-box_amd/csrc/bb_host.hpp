@@ -143,6 +143,7 @@ struct bb_batch {
   // are allocated at their maximum sizes and refreshed when vox_dirty)
   std::vector<bb::VoxShape> vox_shapes;
   std::vector<bb_voxel_shape_info> vox_info;
+  std::vector<float> vox_density;
   std::vector<uint32_t> vox_occ, vox_surf;
   bool vox_dirty = false;
   bb::VoxShape* d_vshape = nullptr;

@@ -345,8 +345,8 @@ bb_status bb_world_instance_transforms(bb_world* w, float* dst, uint32_t n,
  * per-row offsets, the manifold slots that warm start the next step, the
  * counters, the step index and the world and body descriptors.  With
  * dst == NULL, *size receives the bytes needed; otherwise cap must be at
- * least that (BB_ERR_CAPACITY).  A world with voxel bodies is refused
- * (BB_ERR_INVALID): the blob does not carry the voxel shape pools. */
+ * least that (BB_ERR_CAPACITY).  The world's voxel shapes travel in the
+ * blob and are rebuilt, with the same ids, on restore. */
 bb_status bb_world_snapshot(bb_world* w, void* dst, size_t cap, size_t* size);
 /* Creates a world from a checkpoint.  warm_start != 0: the world continues
  * bit-identically to the one that wrote the blob.  warm_start == 0: like the

@@ -3,7 +3,8 @@ reference's voxel scene (Ls, crosses and frames, src/scene_manager.hpp:
 614-655) with AVBD, and voxel shapes mixed with boxes and spheres under
 PGS_SOFT, every step's taps (expanded pair slots, manifolds with keys 90..95,
 islands, colours) and counters included; a voxel world in a batch equals the
-world alone; bad voxel bodies and checkpoints of voxel worlds are refused."""
+world alone; bad voxel bodies are refused; a voxel world's checkpoint
+continues bit for bit."""
 from __future__ import annotations
 
 import numpy as np
@@ -94,6 +95,27 @@ def test_voxel_errors():
     info = w.add_voxel_shape(np.ones((2, 2, 2), bool), 0.5, 2.0)
     w.add_bodies(scenes.voxel_body(info["shape_id"], (0.0, 1.0, 0.0)))
     w.simulate(2)
-    with pytest.raises(beatbox.BBError):
-        w.snapshot()  # checkpoints do not carry voxel shape pools
     w.destroy()
+
+
+def test_voxel_world_checkpoint_continues_bit_identically():
+    """The checkpoint of a voxel world carries its shapes: the restored
+    world continues exactly (warm start) like the one that wrote it."""
+    shapes, bodies = scenes.voxel_pile()
+    kw = scenes.world_desc(solver=scenes.SOLVER_AVBD, iterations=10, morton_bits=63, quirks=0)
+    desc = beatbox.WorldDesc.make(**kw)
+    a = beatbox.RigidBodyManager(desc)
+    for _, cells in shapes:
+        a.add_voxel_shape(cells, scenes.VOXEL_SIZE, scenes.VOXEL_DENSITY)
+    a.add_bodies(bodies)
+    a.simulate(40)
+    blob = a.snapshot()
+    b = beatbox.RigidBodyManager.restore(blob, warm_start=True)
+    a.simulate(30)
+    b.simulate(30)
+    x, y = a.read_bodies(), b.read_bodies()
+    for f in ("position", "rotation", "velocity", "omega", "flags"):
+        assert x[f].tobytes() == y[f].tobytes(), f
+    assert a.read_back_sim_config()["collision_count"] == b.read_back_sim_config()["collision_count"] > 0
+    a.destroy()
+    b.destroy()
