@@ -99,3 +99,27 @@ def test_island_worlds_in_a_batch_and_teams_equal_alone():
         assert b.stats(k)["collision_count"] == w.read_back_sim_config()["collision_count"] > 0
         w.destroy()
     b.destroy()
+
+
+def test_large_island_world_team_equals_workgroup():
+    """A 5,000-body plain-PGS world on the per-island path: the team path
+    (level fixpoint and sweeps over team barriers) equals one workgroup, bit
+    for bit."""
+    bodies = scenes.serving_pile(45, 5000)
+    desc = beatbox.WorldDesc.make(**scenes.world_desc(solver=scenes.SOLVER_PGS, iterations=8, flags=ISLAND,
+                                                      morton_bits=63, quirks=0))
+    runs = []
+    for team in (2048, 0):
+        b = beatbox.Batch()
+        b.add_world(desc, bodies)
+        b.set_param(beatbox.PARAM_TEAM_BODIES, team)
+        b.step(12)
+        b.sync()
+        runs.append((b.read_world(0, len(bodies)), b.stats(0)))
+        b.destroy()
+    (x, sx), (y, sy) = runs
+    for f in ("position", "rotation", "velocity", "omega", "flags"):
+        assert x[f].tobytes() == y[f].tobytes(), f
+    for k in ("collision_count", "contact_count", "island_count", "overflow"):
+        assert sx[k] == sy[k], k
+    assert sx["collision_count"] > 1000 and sx["overflow"] == 0

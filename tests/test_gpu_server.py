@@ -31,8 +31,13 @@ def _requests():
     avbd = beatbox.WorldDesc.make(**scenes.world_desc(solver=scenes.SOLVER_AVBD, iterations=10,
                                                       morton_bits=63, quirks=0))
     parity = beatbox.WorldDesc.make(**scenes.world_desc(iterations=8))
+    # plain PGS on the per-island path (USE_GRAPH_COLORING off)
+    island = beatbox.WorldDesc.make(**scenes.world_desc(
+        solver=scenes.SOLVER_PGS, iterations=8, flags=scenes.SIM_DEFAULT_FLAGS & ~scenes.SIM_USE_GRAPH_COLORING,
+        morton_bits=63, quirks=0))
     reqs = [
         (pgs, scenes.batch_world(0xBB04, 3, 120), 9),
+        (island, scenes.serving_pile(14, 160), 8),
         (avbd, scenes.serving_pile(11, 150), 7),
         (parity, scenes.random_boxes(0xBB11, 64, extent=3.0, y0=0.55, y1=5.0), 12),
         (pgs, scenes.batch_world(0xBB04, 8, 300), 1),

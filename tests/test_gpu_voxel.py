@@ -119,3 +119,30 @@ def test_voxel_world_checkpoint_continues_bit_identically():
     assert a.read_back_sim_config()["collision_count"] == b.read_back_sim_config()["collision_count"] > 0
     a.destroy()
     b.destroy()
+
+
+def test_large_voxel_pile_batch_equals_alone():
+    """600 bodies, two thirds of them concave voxel shapes: graph replay in a
+    batch next to a box world equals the world stepped alone, with no pair
+    overflow."""
+    shapes, vox = scenes.voxel_mixed_pile(0xBB64, 600)
+    desc = beatbox.WorldDesc.make(**scenes.world_desc(iterations=8, morton_bits=63, quirks=0))
+    b = beatbox.Batch()
+    for _, cells in shapes:
+        b.add_voxel_shape(cells, scenes.VOXEL_SIZE, scenes.VOXEL_DENSITY)
+    b.add_world(desc, scenes.batch_world(0xBB04, 5, 400))
+    b.add_world(desc, vox)
+    b.step(60, use_graph=True)
+    b.sync()
+    w = beatbox.RigidBodyManager(desc)
+    for _, cells in shapes:
+        w.add_voxel_shape(cells, scenes.VOXEL_SIZE, scenes.VOXEL_DENSITY)
+    w.add_bodies(vox)
+    w.simulate(60)
+    x, y = b.read_world(1, len(vox)), w.read_bodies()
+    for f in ("position", "rotation", "velocity", "omega", "flags"):
+        assert x[f].tobytes() == y[f].tobytes(), f
+    st = b.stats(1)
+    assert st["overflow"] == 0 and st["collision_count"] > 100
+    w.destroy()
+    b.destroy()
