@@ -186,6 +186,7 @@ bb_status alloc_arrays(bb_batch* b, Buffers& A, uint32_t W, uint32_t NB, uint32_
   d.owner = A.alloc<uint32_t>((size_t)NB * kColors);
   d.bucket = A.alloc<uint32_t>(NP);
   d.bucket_off = A.alloc<uint32_t>((size_t)W * (kBuckets + 1));
+  d.nlev = A.alloc<uint32_t>(W);
   d.iw = A.alloc<float4>((size_t)NB * 3);
   d.np = NP;
   d.pgs_lcap = 0;  // set after the layout (pgs_lds_rows)
@@ -500,7 +501,7 @@ bb_status build(bb_batch* b) {
   b->base.any_voxel = 0;
   for (auto& hw : b->worlds)
     if (hw.has_voxel) b->base.any_voxel = 1;
-  b->base.any_plain = any_plain(b);
+  b->base.any_island = any_island(b);
   b->built = true;
   b->dirty = false;
   b->stepped = false;
@@ -617,11 +618,11 @@ bb_status upload_infos(bb_batch* b, bool one_shot) {
   return BB_OK;
 }
 
-// Whether any world runs plain PGS on the per-island path (k_pgs_plain is
-// then part of the launch sequence).
-uint32_t any_plain(const bb_batch* b) {
+// Whether any world runs the per-island solver path (k_levels and
+// k_pgs_island are then part of the launch sequence).
+uint32_t any_island(const bb_batch* b) {
   for (const WorldInfo& wi : b->infos)
-    if (wi.solver == BB_SOLVER_PGS) return 1u;
+    if (wi.solver != BB_SOLVER_AVBD && !(wi.flags & BB_SIM_USE_GRAPH_COLORING)) return 1u;
   return 0u;
 }
 
@@ -631,11 +632,11 @@ bb_status refresh_infos(bb_batch* b) {
     wi.mode = b->infos[w].mode;
     b->infos[w] = wi;
   }
-  // a solver or flag change can add or drop k_pgs_plain: a captured step
-  // graph holds the old launch sequence
-  const uint32_t p = any_plain(b);
-  if (p != b->base.any_plain) {
-    b->base.any_plain = p;
+  // a solver or flag change can add or drop the per-island kernels: a
+  // captured step graph holds the old launch sequence
+  const uint32_t p = any_island(b);
+  if (p != b->base.any_island) {
+    b->base.any_island = p;
     if (b->graph) {
       (void)hipGraphExecDestroy(b->graph);
       b->graph = nullptr;

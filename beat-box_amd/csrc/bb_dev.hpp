@@ -118,9 +118,9 @@ struct Dev {
   WorldInfo* worlds;
   uint32_t W;
   // voxel shape pools, shared by the batch's worlds; any_voxel: a world has
-  // voxel bodies (k_narrow_vox is launched); any_plain: a world runs PGS on
-  // the per-island path (k_pgs_plain is launched)
-  uint32_t any_voxel, any_plain;
+  // voxel bodies (k_narrow_vox is launched); any_island: a world runs the
+  // per-island solver path (k_levels and k_pgs_island are launched)
+  uint32_t any_voxel, any_island;
   const VoxShape* vshape;
   const uint32_t* vocc;
   const uint32_t* vsurf;
@@ -163,6 +163,7 @@ struct Dev {
   uint32_t* owner;    // [body*32], zero between rounds
   uint32_t* bucket;   // [pair slot] manifold slots grouped by colour
   uint32_t* bucket_off;  // [W * (kBuckets+1)]
+  uint32_t* nlev;        // [W] exact dependency levels of the per-island path (0: none)
   float4* iw;         // [body*3] cached world inverse inertia rows, row0.w = inv_mass
   // PGS solver rows (colour-sorted, [pair slot]; contact c of row k at c*np+k)
   uint32_t np;        // pair slots in the batch

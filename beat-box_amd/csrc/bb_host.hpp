@@ -202,8 +202,8 @@ uint32_t assign_teams(bb_batch* b, uint32_t grid_cap);
 // (ties by index), team worlds last (their workgroups exit at once).
 void world_order(const std::vector<WorldInfo>& infos, uint32_t* out);
 Dev dev_for(const bb_batch* b, int cur);
-// 1 when a world of the batch runs plain PGS on the per-island path.
-uint32_t any_plain(const bb_batch* b);
+// 1 when a world of the batch runs the per-island solver path.
+uint32_t any_island(const bb_batch* b);
 
 }  // namespace host
 }  // namespace bb

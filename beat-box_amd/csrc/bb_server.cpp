@@ -507,9 +507,9 @@ bb_status bb_server_create(const bb_server_desc* desc, bb_server** out) {
     b->worlds.resize(s->W);
     b->infos.assign(s->W, WorldInfo{});
     b->base.pgs_lcap = b->pgs_lds;
-    // requests may run plain PGS on the per-island path; k_pgs_plain exits at
-    // once for every other world
-    b->base.any_plain = 1;
+    // requests may run the per-island solver path; its kernels exit at once
+    // for every other world
+    b->base.any_island = 1;
     // the k_team launch keeps one shape (half of what can be resident) and
     // the team table in device memory says which workgroups step which world
     b->base.team_grid = b->team_bodies ? std::min(team_capacity() / 2, kMaxTeamBlocks) : 0;

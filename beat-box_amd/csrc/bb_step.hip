@@ -1962,12 +1962,89 @@ BB_DI void build_adj(const Dev& d, const WorldInfo& W, uint32_t P, const T& t, u
   t.sync();
 }
 
+// Exact dependency levels of the per-island path (per-world kernels): one
+// thread walks the slots in order with each row's next free level in LDS;
+// the slots are staged kLvlChunk at a time so the walk touches LDS only.
+// Levels are below the slot count, which fits 16 bits for worlds of at most
+// kLvlRows rows whose slot count stays below 65,536.
+static constexpr uint32_t kLvlRows = 4096, kLvlChunk = 256;
+struct LevelLds {
+  uint16_t lvl[kLvlRows];  // 1 + level of the last manifold on the row (0: none)
+  uint32_t sa[kLvlChunk], sb[kLvlChunk];  // staged rows (kLvlSkip: not solved)
+  uint16_t out[kLvlChunk];
+  uint32_t top;
+};
+static constexpr uint32_t kLvlSkip = 0xFFFFFFFFu, kLvlStatic = 0x80000000u;
+
 struct PostShared {
   uint32_t carry;
   uint32_t flag;
   uint32_t bcount[kBuckets];
   uint32_t bfill[kBuckets];
 };
+
+// Buckets of the solvable manifolds (at least one awake dynamic body; the
+// others are skipped by every sweep, RB_sim.slang:1424) by bkey(p): a colour,
+// or a capped dependency level (kColors and above: the overflow bucket;
+// kMAX - 1: no manifold).  Order inside a bucket is free (no shared dynamic
+// body); the overflow bucket is filled in ascending slot order for its
+// serial solve.  Then the world inverse inertia and the solver rows.
+template <class T, class K>
+BB_DI void build_rows(const Dev& d, const WorldInfo& W, uint32_t w, const T& t, PostShared& L, K&& bkey) {
+  const uint32_t tid = t.rank(), NT = t.size();
+  const uint32_t off = W.body_off, P = d.npairs_cur[w];
+  if (tid < kBuckets) L.bcount[tid] = 0;
+  t.sync();
+  for (uint32_t p = tid; p < P; p += NT) {
+    uint32_t cc = bkey(p);
+    if (cc == kMAX - 1) continue;
+    int4 h = d.man_cur.hdr[W.pair_off + p];
+    if (!pgs_solvable(d, off, h)) continue;
+    atomicAdd(&L.bcount[cc < kColors ? cc : kColors], 1u);
+  }
+  t.sync();
+  if (tid == 0) {
+    uint32_t sum = 0;
+    uint32_t* bo = d.bucket_off + w * (kBuckets + 1);
+    for (uint32_t c2 = 0; c2 < kBuckets; ++c2) {
+      bo[c2] = sum;
+      L.bfill[c2] = sum;
+      sum += L.bcount[c2];
+    }
+    bo[kBuckets] = sum;
+  }
+  t.sync();
+  uint32_t* bucket = d.bucket + W.pair_off;
+  for (uint32_t p = tid; p < P; p += NT) {
+    uint32_t cc = bkey(p);
+    if (cc >= kColors) continue;
+    int4 h = d.man_cur.hdr[W.pair_off + p];
+    if (!pgs_solvable(d, off, h)) continue;
+    bucket[atomicAdd(&L.bfill[cc], 1u)] = W.pair_off + p;
+  }
+  if (L.bcount[kColors]) {
+    if (tid == 0) L.carry = L.bfill[kColors];
+    t.sync();
+    for (uint32_t c0 = 0; c0 < P; c0 += NT) {
+      uint32_t p = c0 + tid;
+      uint32_t f = 0;
+      const uint32_t cc = p < P ? bkey(p) : kMAX - 1;
+      if (cc != kMAX - 1 && cc >= kColors)
+        f = pgs_solvable(d, off, d.man_cur.hdr[W.pair_off + p]) ? 1u : 0u;
+      uint32_t tot;
+      uint32_t ex = t.scan(f, tot);
+      if (f) bucket[L.carry + ex] = W.pair_off + p;
+      t.sync();
+      if (tid == 0) L.carry += tot;
+      t.sync();
+    }
+  }
+  // ---- world inverse inertia of every row, then the solver rows
+  iw_refresh(d, W, t);
+  t.sync();
+  const uint32_t total = d.bucket_off[w * (kBuckets + 1) + kBuckets];
+  for (uint32_t k = tid; k < total; k += NT) pgs_compact(d, W, W.pair_off + k, bucket[k]);
+}
 
 template <class T>
 BB_DI void s_post(const Dev& d, uint32_t w, const T& t, PostShared& L) {
@@ -2246,22 +2323,92 @@ BB_DI void s_post(const Dev& d, uint32_t w, const T& t, PostShared& L) {
     d.owner[(off + h.y) * kColors + cc] = 0;
   }
   t.sync();
+  if (tid == 0) d.stats[w * ST_COUNT + ST_GC_OVF] = L.bcount[kColors];
+  t.sync();
+  // the per-island path (USE_GRAPH_COLORING off) builds its solver rows in
+  // s_levels
+  if ((flags & BB_SIM_USE_GRAPH_COLORING) == 0) return;
+  build_rows(d, W, w, t, L, [&](uint32_t p) { return color[p]; });
+}
+
+// Solver rows of the per-island path: dependency levels, level buckets and
+// the rows (after s_post; per-world kernels with LL in LDS, teams without).
+template <class T>
+BB_DI void s_levels(const Dev& d, uint32_t w, const T& t, PostShared& L, LevelLds* LL) {
+  const uint32_t tid = t.rank(), NT = t.size();
+  const WorldInfo W = d.worlds[w];
+  if (W.solver == BB_SOLVER_AVBD || W.n == 0 || (W.flags & BB_SIM_USE_GRAPH_COLORING)) return;
+  const uint32_t N = W.n, off = W.body_off, P = d.npairs_cur[w];
+  uint32_t* color = d.color + W.pair_off;
+  uint32_t* pick = d.pick + W.pair_off;
   // ---- per-island mode (USE_GRAPH_COLORING off, rigid_body_manager.cpp:
   // 1390-1399): the reference walks each contact island's manifolds serially
   // in packed order (= slot order).  That order is kept exactly by parallel
   // sweeps over dependency levels: a manifold's level is one more than the
-  // level of the previous manifold (lower slot) on either of its dynamic rows,
-  // so the manifolds of one level share no dynamic row and any two that share
-  // one are solved in slot order.  Levels are capped at kColors; the capped
-  // manifolds form the overflow bucket, solved serially in ascending slot
-  // order after the levels (a manifold after an overflow one on a shared row
-  // is itself overflow).  The levels are the longest-path fixpoint over the
-  // rows' ascending slot lists, reached after at most kColors + 1 rounds
-  // (each round settles one more step of every path); they live in `pick`,
-  // free after the colouring.
-  const bool island = (flags & BB_SIM_USE_GRAPH_COLORING) == 0;
-  uint32_t* key = island ? pick : color;
-  if (island) {
+  // level of the previous solvable manifold (lower slot) on either of its
+  // dynamic rows, so the manifolds of one level share no dynamic row and any
+  // two that share one are solved in slot order.  Manifolds no sweep solves
+  // (no awake dynamic body) get no level.
+  //  - Per-world kernels (LL, at most kLvlRows rows, fewer than 65,536
+  //    slots): exact levels from one in-order walk, then a counting sort of
+  //    the rows by level; every level is a sweep of its own (d.nlev[w]).
+  //  - Teams and larger worlds: levels capped at kColors, the longest-path
+  //    fixpoint over the rows' ascending slot lists (at most kColors + 1
+  //    rounds, each settling one more step of every path); the capped
+  //    manifolds form the overflow bucket, solved serially in ascending slot
+  //    order after the levels (a manifold after an overflow one on a shared
+  //    row is itself overflow).
+  // The levels live in `pick`, free after the colouring.
+  const bool exact = LL != nullptr && N <= kLvlRows && P < 65536u;
+  uint32_t* key = pick;
+  if (exact) {
+    for (uint32_t i = tid; i < N; i += NT) LL->lvl[i] = 0;
+    if (tid == 0) LL->top = 0;
+    t.sync();
+    for (uint32_t c0 = 0; c0 < P; c0 += kLvlChunk) {
+      const uint32_t nc = P - c0 < kLvlChunk ? P - c0 : kLvlChunk;
+      for (uint32_t q = tid; q < nc; q += NT) {
+        const uint32_t p = c0 + q;
+        uint32_t a = kLvlSkip, b = kLvlSkip;
+        if (color[p] != kMAX - 1) {
+          const int4 h = d.man_cur.hdr[W.pair_off + p];
+          if (pgs_solvable(d, off, h)) {
+            a = (uint32_t)h.x | ((d.cur.flags[off + h.x] & BB_BODY_DYNAMIC) ? 0u : kLvlStatic);
+            b = (uint32_t)h.y | ((d.cur.flags[off + h.y] & BB_BODY_DYNAMIC) ? 0u : kLvlStatic);
+          }
+        }
+        LL->sa[q] = a;
+        LL->sb[q] = b;
+      }
+      t.sync();
+      if (tid == 0) {
+        uint32_t top = LL->top;
+#pragma unroll 1
+        for (uint32_t q = 0; q < nc; ++q) {
+          const uint32_t a = LL->sa[q], b = LL->sb[q];
+          if (a == kLvlSkip) {
+            LL->out[q] = 0xFFFFu;
+            continue;
+          }
+          const bool da = (a & kLvlStatic) == 0, db = (b & kLvlStatic) == 0;
+          const uint32_t ra = a & ~kLvlStatic, rb = b & ~kLvlStatic;
+          uint32_t lv = da ? LL->lvl[ra] : 0u;
+          if (db) {
+            const uint32_t lb = LL->lvl[rb];
+            lv = lb > lv ? lb : lv;
+          }
+          LL->out[q] = (uint16_t)lv;
+          if (da) LL->lvl[ra] = (uint16_t)(lv + 1u);
+          if (db) LL->lvl[rb] = (uint16_t)(lv + 1u);
+          top = lv + 1u > top ? lv + 1u : top;
+        }
+        LL->top = top;
+      }
+      t.sync();
+      for (uint32_t q = tid; q < nc; q += NT) key[c0 + q] = LL->out[q] == 0xFFFFu ? kMAX - 1 : LL->out[q];
+      t.sync();
+    }
+  } else {
     build_adj(d, W, P, t, L.carry);
     for (uint32_t p = tid; p < P; p += NT) key[p] = color[p] == kMAX - 1 ? kMAX - 1 : 0u;
     t.sync();
@@ -2293,66 +2440,57 @@ BB_DI void s_post(const Dev& d, uint32_t w, const T& t, PostShared& L) {
       if (!more) break;
     }
   }
-  auto bkey = [&](uint32_t p) -> uint32_t {
-    return island ? __hip_atomic_load(&key[p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : color[p];
-  };
-  // ---- colour buckets of the solvable manifolds (at least one awake dynamic
-  // body; the others are skipped by every sweep, RB_sim.slang:1424).  Order
-  // inside a colour (level) is free (no shared dynamic body); the overflow
-  // bucket is filled in ascending slot order for its serial solve.
-  if (tid == 0) d.stats[w * ST_COUNT + ST_GC_OVF] = L.bcount[kColors];
-  t.sync();
-  if (tid < kBuckets) L.bcount[tid] = 0;
-  t.sync();
-  for (uint32_t p = tid; p < P; p += NT) {
-    uint32_t cc = bkey(p);
-    if (cc == kMAX - 1) continue;
-    int4 h = d.man_cur.hdr[W.pair_off + p];
-    if (!pgs_solvable(d, off, h)) continue;
-    atomicAdd(&L.bcount[cc < kColors ? cc : kColors], 1u);
-  }
-  t.sync();
-  if (tid == 0) {
-    uint32_t sum = 0;
-    uint32_t* bo = d.bucket_off + w * (kBuckets + 1);
-    for (uint32_t c2 = 0; c2 < kBuckets; ++c2) {
-      bo[c2] = sum;
-      L.bfill[c2] = sum;
-      sum += L.bcount[c2];
-    }
-    bo[kBuckets] = sum;
-  }
-  t.sync();
-  uint32_t* bucket = d.bucket + W.pair_off;
-  for (uint32_t p = tid; p < P; p += NT) {
-    uint32_t cc = bkey(p);
-    if (cc >= kColors) continue;
-    int4 h = d.man_cur.hdr[W.pair_off + p];
-    if (!pgs_solvable(d, off, h)) continue;
-    bucket[atomicAdd(&L.bfill[cc], 1u)] = W.pair_off + p;
-  }
-  if (L.bcount[kColors]) {
-    if (tid == 0) L.carry = L.bfill[kColors];
+  if (tid == 0) d.nlev[w] = 0u;
+  if (exact) {
+    // counting sort of the solved rows by level: level offsets in the
+    // world's adjacency area (2 words per slot, free in PGS worlds), fill
+    // cursors in the solver row headers (free until the rows are built)
+    const uint32_t nlev = LL->top;
+    uint32_t* lvoff = d.adj + 2 * W.pair_off;
+    uint32_t* cursor = reinterpret_cast<uint32_t*>(d.sv_hdr + W.pair_off);
+    uint32_t* bucket = d.bucket + W.pair_off;
+    for (uint32_t l = tid; l <= nlev; l += NT) lvoff[l] = 0;
     t.sync();
-    for (uint32_t c0 = 0; c0 < P; c0 += NT) {
-      uint32_t p = c0 + tid;
-      uint32_t f = 0;
-      const uint32_t cc = p < P ? bkey(p) : kMAX - 1;
-      if (cc != kMAX - 1 && cc >= kColors)
-        f = pgs_solvable(d, off, d.man_cur.hdr[W.pair_off + p]) ? 1u : 0u;
+    for (uint32_t p = tid; p < P; p += NT) {
+      const uint32_t k = key[p];
+      if (k != kMAX - 1) atomicAdd(&lvoff[k], 1u);
+    }
+    t.sync();
+    if (tid == 0) L.carry = 0;
+    t.sync();
+    for (uint32_t c0 = 0; c0 <= nlev; c0 += NT) {
+      const uint32_t l = c0 + tid;
+      const uint32_t v = l <= nlev ? lvoff[l] : 0u;
       uint32_t tot;
-      uint32_t ex = t.scan(f, tot);
-      if (f) bucket[L.carry + ex] = W.pair_off + p;
+      const uint32_t ex = t.scan(v, tot);
+      if (l <= nlev) {
+        lvoff[l] = L.carry + ex;
+        cursor[l] = L.carry + ex;
+      }
       t.sync();
       if (tid == 0) L.carry += tot;
       t.sync();
     }
+    for (uint32_t p = tid; p < P; p += NT) {
+      const uint32_t k = key[p];
+      if (k != kMAX - 1) bucket[atomicAdd(&cursor[k], 1u)] = W.pair_off + p;
+    }
+    t.sync();
+    const uint32_t total = lvoff[nlev];
+    if (tid == 0) {
+      uint32_t* bo = d.bucket_off + w * (kBuckets + 1);
+      for (uint32_t c2 = 0; c2 < kBuckets; ++c2) bo[c2] = 0;  // no colour buckets, no overflow
+      bo[kBuckets] = total;
+      d.nlev[w] = nlev;
+    }
+    iw_refresh(d, W, t);
+    t.sync();
+    for (uint32_t k = tid; k < total; k += NT) pgs_compact(d, W, W.pair_off + k, bucket[k]);
+    return;
   }
-  // ---- world inverse inertia of every row, then the solver rows
-  iw_refresh(d, W, t);
-  t.sync();
-  const uint32_t total = d.bucket_off[w * (kBuckets + 1) + kBuckets];
-  for (uint32_t k = tid; k < total; k += NT) pgs_compact(d, W, W.pair_off + k, bucket[k]);
+  build_rows(d, W, w, t, L, [&](uint32_t p) {
+    return __hip_atomic_load(&key[p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  });
 }
 
 template <int BLOCK>
@@ -2363,6 +2501,17 @@ __global__ __launch_bounds__(BLOCK) void k_post(Dev d) {
   if (d.worlds[w].mode & kModeTeam) return;
   const BlockTeam<BLOCK> t{&S, nullptr};
   s_post(d, w, t, L);
+}
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_levels(Dev d) {
+  __shared__ PostShared L;
+  __shared__ ScanLds<BLOCK> S;
+  __shared__ LevelLds LL;
+  const uint32_t w = d.order[blockIdx.x];
+  if (d.worlds[w].mode & kModeTeam) return;
+  const BlockTeam<BLOCK> t{&S, nullptr};
+  s_levels(d, w, t, L, &LL);
 }
 
 // ================================================================== k_pgs
@@ -2678,13 +2827,15 @@ BB_DI qt apply_angular_velocity(qt q, v3 om, float dt) {
   return qnorm(qmul(qt{dr, 1.0f}, q));
 }
 
-// kPlain: the instance for PGS worlds (the per-island path, k_pgs_plain);
-// the other instance steps PGS_SOFT worlds and carries no PGS code.
-template <bool kPlain, class T>
+// kIsland: the instances for the per-island path (k_pgs_island), which
+// sweep dependency levels; kPlain: PGS (per-island path only) rather than
+// PGS_SOFT.  k_pgs, the colour sweeps, carries neither.
+template <bool kIsland, bool kPlain, class T>
 BB_DI void s_pgs(const Dev& d, uint32_t w, const T& t, float4* pgs_lds, uint32_t lcap) {
   const uint32_t tid = t.rank(), NT = t.size();
   const WorldInfo W = d.worlds[w];
   if (W.solver == BB_SOLVER_AVBD || W.n == 0) return;
+  if (((W.flags & BB_SIM_USE_GRAPH_COLORING) == 0) != kIsland) return;
   if ((W.solver == BB_SOLVER_PGS) != kPlain) return;
   const uint32_t off = W.body_off, N = W.n, A = W.a, NP = d.np;
   Pgs s;
@@ -2698,10 +2849,9 @@ BB_DI void s_pgs(const Dev& d, uint32_t w, const T& t, float4* pgs_lds, uint32_t
     s.soft[1] = soft_coefs(W.dt, 2.0f * hertz);
   }
   const uint32_t* bo = d.bucket_off + w * (kBuckets + 1);
-  // colours in use, or every level bucket in the per-island mode (empty
-  // buckets are skipped)
-  const uint32_t used =
-      (W.flags & BB_SIM_USE_GRAPH_COLORING) ? d.stats[w * ST_COUNT + ST_GC_COUNT] : kColors;
+  // colours in use, or every capped level bucket of the per-island path
+  // (empty buckets are skipped)
+  const uint32_t used = kIsland ? kColors : d.stats[w * ST_COUNT + ST_GC_COUNT];
   const uint32_t base = W.pair_off;
   const uint32_t ovf_lo = bo[kColors], ovf_hi = bo[kBuckets];
   const VelCache vc{pgs_lds, off, N < lcap ? N : lcap};
@@ -2715,7 +2865,17 @@ BB_DI void s_pgs(const Dev& d, uint32_t w, const T& t, float4* pgs_lds, uint32_t
   }
   t.sync();
   auto row = [&](uint32_t k, int kind) { pgs_row<kPlain>(d, vc, off, base + k, s, kind); };
+  // exact dependency levels of the per-island path (0: buckets)
+  const uint32_t nlev = kIsland ? d.nlev[w] : 0u;
+  const uint32_t* lvoff = d.adj + 2 * W.pair_off;
   auto sweep = [&](int kind) {
+    if (nlev) {
+      for (uint32_t lv = 0; lv < nlev; ++lv) {
+        for (uint32_t k = lvoff[lv] + tid; k < lvoff[lv + 1]; k += NT) row(k, kind);
+        t.sync();
+      }
+      return;
+    }
     for (uint32_t c = 0; c < used; ++c) {
       if (bo[c] == bo[c + 1]) continue;
       for (uint32_t k = bo[c] + tid; k < bo[c + 1]; k += NT) row(k, kind);
@@ -2798,16 +2958,17 @@ __global__ __launch_bounds__(BLOCK, 4) void k_pgs(Dev d) {
   const uint32_t w = d.order[blockIdx.x];
   if (d.worlds[w].mode & kModeTeam) return;
   const BlockTeam<BLOCK> t{nullptr, nullptr};
-  s_pgs<false>(d, w, t, pgs_lds, d.pgs_lcap);
+  s_pgs<false, false>(d, w, t, pgs_lds, d.pgs_lcap);
 }
 
 template <int BLOCK>
-__global__ __launch_bounds__(BLOCK, 4) void k_pgs_plain(Dev d) {
+__global__ __launch_bounds__(BLOCK, 4) void k_pgs_island(Dev d) {
   HIP_DYNAMIC_SHARED(float4, pgs_lds)
   const uint32_t w = d.order[blockIdx.x];
   if (d.worlds[w].mode & kModeTeam) return;
   const BlockTeam<BLOCK> t{nullptr, nullptr};
-  s_pgs<true>(d, w, t, pgs_lds, d.pgs_lcap);
+  if (d.worlds[w].solver == BB_SOLVER_PGS) s_pgs<true, true>(d, w, t, pgs_lds, d.pgs_lcap);
+  else s_pgs<true, false>(d, w, t, pgs_lds, d.pgs_lcap);
 }
 
 // ================================================================== AVBD
@@ -3485,8 +3646,11 @@ __global__ __launch_bounds__(BLOCK) void k_team(Dev d) {
   t.sync();
   s_post(d, w, t, G.post);
   t.sync();
-  if (d.worlds[w].solver == BB_SOLVER_PGS) s_pgs<true>(d, w, t, nullptr, 0u);
-  else s_pgs<false>(d, w, t, nullptr, 0u);
+  s_levels(d, w, t, G.post, nullptr);
+  t.sync();
+  if (d.worlds[w].flags & BB_SIM_USE_GRAPH_COLORING) s_pgs<false, false>(d, w, t, nullptr, 0u);
+  else if (d.worlds[w].solver == BB_SOLVER_PGS) s_pgs<true, true>(d, w, t, nullptr, 0u);
+  else s_pgs<true, false>(d, w, t, nullptr, 0u);
   t.sync();
   s_avbd(d, w, t, G.avbd);
   t.sync();
@@ -3635,7 +3799,8 @@ template __global__ void k_team<kTeamBlock>(Dev);
   template __global__ void k_contacts<B>(Dev); \
   template __global__ void k_post<B>(Dev);     \
   template __global__ void k_pgs<B>(Dev);      \
-  template __global__ void k_pgs_plain<B>(Dev); \
+  template __global__ void k_pgs_island<B>(Dev); \
+  template __global__ void k_levels<B>(Dev);    \
   template __global__ void k_avbd<B>(Dev);     \
   template __global__ void k_finish<B>(Dev);
 
@@ -3655,8 +3820,9 @@ static void launch_stage(int stage, const Dev& d, hipStream_t st, uint32_t grid)
       break;
     case 3: hipLaunchKernelGGL(k_post<B>, g, b, 0, st, d); break;
     case 4:
+      if (d.any_island) hipLaunchKernelGGL(k_levels<B>, g, b, 0, st, d);
       hipLaunchKernelGGL(k_pgs<B>, g, b, (size_t)d.pgs_lcap * 16 * kBodyLds, st, d);
-      if (d.any_plain) hipLaunchKernelGGL(k_pgs_plain<B>, g, b, (size_t)d.pgs_lcap * 16 * kBodyLds, st, d);
+      if (d.any_island) hipLaunchKernelGGL(k_pgs_island<B>, g, b, (size_t)d.pgs_lcap * 16 * kBodyLds, st, d);
       break;
     case 5: hipLaunchKernelGGL(k_avbd<B>, g, b, 0, st, d); break;
     default: hipLaunchKernelGGL(k_finish<B>, g, b, 0, st, d); break;

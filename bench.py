@@ -161,6 +161,81 @@ def single_world(name: str, desc, bodies, steps: int, warmup: int, profile: int)
     return out
 
 
+def batch_run(name: str, worlds, shapes=(), steps: int = 60, warmup: int = 10, profile: int = 10) -> dict:
+    """A batch of (desc, bodies) worlds (voxel `shapes` added first): hipGraph
+    replay of `steps` steps after `warmup`, then `profile` eager steps timed
+    per stage."""
+    b = beatbox.Batch()
+    for cells in shapes:
+        b.add_voxel_shape(cells, scenes.VOXEL_SIZE, scenes.VOXEL_DENSITY)
+    total = 0
+    for desc, bodies in worlds:
+        b.add_world(desc, bodies)
+        total += len(bodies)
+    b.step(warmup, use_graph=True)
+    b.sync()
+    t0 = time.perf_counter()
+    b.step(steps, use_graph=True)
+    b.sync()
+    dt = time.perf_counter() - t0
+    _, km, cnt = b.profile(profile, counts=True)
+    manifolds = int(cnt[:, beatbox.COUNT_NAMES.index("manifolds")].mean())
+    overflow = sum(b.stats(w)["overflow"] for w in range(len(worlds)))
+    b.destroy()
+    out = dict(worlds=len(worlds), bodies=total, steps=steps, warmup=warmup, ms_per_step=dt / steps * 1e3,
+               body_steps_per_s=total * steps / dt, manifolds=manifolds, overflow=int(overflow),
+               stages_ms={n: round(float(km[i]) / profile, 3) for i, n in enumerate(beatbox.KERNEL_NAMES)})
+    log(f"{name}: {out['ms_per_step']:.2f} ms/step, {out['body_steps_per_s'] / 1e6:.2f} M body-steps/s, {out}")
+    return out
+
+
+def next_rows(seed: int) -> dict:
+    """Throughput of the paths beside the headline one: the per-island solver
+    (plain PGS, USE_GRAPH_COLORING off) against the colour sweeps on the same
+    1,024 worlds, voxel bodies, and checkpoint / render hand-off costs."""
+    out = {}
+    sizes = scenes.batch_world_sizes(seed, 4096, 16, 2048)[:1024]
+    bodies = [scenes.batch_world(seed, w, int(n)) for w, n in enumerate(sizes)]
+    colour = beatbox.WorldDesc.make(**scenes.world_desc(iterations=8, morton_bits=63, quirks=0))
+    island = beatbox.WorldDesc.make(**scenes.world_desc(
+        solver=scenes.SOLVER_PGS, iterations=8, morton_bits=63, quirks=0,
+        flags=scenes.SIM_DEFAULT_FLAGS & ~scenes.SIM_USE_GRAPH_COLORING))
+    c = batch_run("colour sweeps, 1,024 worlds", [(colour, x) for x in bodies])
+    i = batch_run("per-island plain PGS, 1,024 worlds", [(island, x) for x in bodies])
+    out["per_island"] = dict(workload="worlds 0-1023 of the headline batch, PGS I=8; plain PGS on the per-island "
+                                      "path against PGS_SOFT colour sweeps", colour=c, island=i,
+                             island_over_colour=i["body_steps_per_s"] / c["body_steps_per_s"])
+    shapes, _ = scenes.voxel_mixed_pile(0, 1)
+    avbd = beatbox.WorldDesc.make(**scenes.world_desc(solver=scenes.SOLVER_AVBD, iterations=10,
+                                                      morton_bits=63, quirks=0))
+    vworlds = [(avbd, scenes.voxel_mixed_pile(0xBB70 + k, 200)[1]) for k in range(64)]
+    out["voxel"] = dict(workload="64 worlds of 200 bodies (3/5 voxel Ls, crosses and frames, 1/5 boxes, 1/5 "
+                                 "spheres), AVBD 10 iterations",
+                        **batch_run("voxel piles", vworlds, shapes=[cl for _, cl in shapes]))
+    # checkpoint and render hand-off of one 20,000-body world
+    w = beatbox.RigidBodyManager(avbd)
+    pile = scenes.serving_pile(0xBB71, 20000)
+    w.add_bodies(pile)
+    w.simulate(30)
+    w.sync()
+    t0 = time.perf_counter()
+    blob = w.snapshot()
+    t1 = time.perf_counter()
+    r = beatbox.RigidBodyManager.restore(blob, warm_start=True)
+    r.sync()
+    t2 = time.perf_counter()
+    xf = w.instance_transforms(len(pile))
+    t3 = time.perf_counter()
+    out["checkpoint"] = dict(bodies=len(pile), blob_mb=len(blob) / 1e6, snapshot_ms=(t1 - t0) * 1e3,
+                             restore_ms=(t2 - t1) * 1e3)
+    out["handoff_host"] = dict(bodies=len(pile), bytes=int(xf.nbytes), ms=(t3 - t2) * 1e3,
+                               note="host destination: device kernel plus the PCIe copy")
+    r.destroy()
+    w.destroy()
+    log(f"checkpoint: {out['checkpoint']}, hand-off: {out['handoff_host']}")
+    return out
+
+
 def serving(duration: float, budget_gib: float, resident_cap: int, team_bodies: int,
             seed: int = 0xBB05) -> dict:
     """Serving stream: Poisson arrivals at 500 worlds/s for `duration` seconds
@@ -327,6 +402,7 @@ def main() -> int:
                                    **single_world("sphere gas", gas, scenes.sphere_gas(), 100, 1, 5))
         extra["serving"] = serving(args.serving_seconds, args.serving_budget_gib, args.serving_cap,
                                    args.serving_team_bodies)
+        extra["next_rows"] = next_rows(args.seed)
 
     threads = args.cpu_threads or min(16, os.cpu_count() or 1)
     cpu = None if args.no_cpu else cpu_baseline(args.seed, args.cpu_budget, threads)
