@@ -467,7 +467,7 @@ bb_status build(bb_batch* b) {
     for (auto& bd : hw.bodies)
       if (BB_SHAPE_KIND(bd.shape) == BB_SHAPE_VOXEL) hw.has_voxel = true;
     // a pair with a voxel body takes six slots, one per contact-face code
-    hw.pair_cap = pair_capacity(hw.n) * (hw.has_voxel ? 6u : 1u);
+    hw.pair_cap = pair_capacity(hw.n) * (hw.has_voxel ? 6u : 1u) * hw.pair_scale;
     NB += hw.n;
     NA += hw.a;
     NP += hw.pair_cap;
@@ -505,6 +505,73 @@ bb_status build(bb_batch* b) {
   b->built = true;
   b->dirty = false;
   b->stepped = false;
+  return BB_OK;
+}
+
+// The world's pair slots start at 0 in a single-world layout, so a larger
+// buffer set laid out the same way keeps every slot index: each array of
+// the old set is copied into the front of its counterpart (arrays sized by
+// the pair capacity grow, the rest are the same size), then the old set is
+// freed.  Arrays strided by the capacity (the PGS row scratch) land
+// elsewhere, but a step writes them before it reads them.
+bb_status grow_pairs(bb_batch* b) {
+  if (!b->built || b->dirty || b->worlds.size() != 1) return BB_ERR_STATE;
+  HostWorld& hw = b->worlds[0];
+  if (hw.pair_cap > (1u << 26)) return BB_ERR_CAPACITY;
+  HIPCHK(hipStreamSynchronize(b->stream));
+  HIPCHK(hipStreamSynchronize(b->stream2));
+  // alloc_arrays repoints these; they are put back if it fails
+  const Dev base0 = b->base;
+  Rows rows0[2] = {b->rows[0], b->rows[1]};
+  Manifolds man0[2] = {b->man[0], b->man[1]};
+  uint32_t* entries0[2] = {b->entries[0], b->entries[1]};
+  uint2* pairs0[2] = {b->pairs[0], b->pairs[1]};
+  uint32_t* npairs0[2] = {b->npairs[0], b->npairs[1]};
+  uint32_t* pstart0[2] = {b->pstart[0], b->pstart[1]};
+  VoxShape* vshape0 = b->d_vshape;
+  uint32_t *vocc0 = b->d_vocc, *vsurf0 = b->d_vsurf;
+  const bool vdirty0 = b->vox_dirty;
+  const uint32_t cap = hw.pair_cap * 2u;
+  Buffers nb;
+  bb_status st = alloc_arrays(b, nb, 1, b->NB, b->NA, cap);
+  hipError_t rc = hipSuccess;
+  if (st == BB_OK) {
+    for (size_t i = 0; i < nb.ptrs.size() && rc == hipSuccess; ++i)
+      rc = hipMemcpy(nb.ptrs[i], b->buf.ptrs[i], std::min(nb.sizes[i], b->buf.sizes[i]), hipMemcpyDeviceToDevice);
+  }
+  if (st != BB_OK || rc != hipSuccess) {
+    nb.release();
+    b->base = base0;
+    for (int s = 0; s < 2; ++s) {
+      b->rows[s] = rows0[s];
+      b->man[s] = man0[s];
+      b->entries[s] = entries0[s];
+      b->pairs[s] = pairs0[s];
+      b->npairs[s] = npairs0[s];
+      b->pstart[s] = pstart0[s];
+    }
+    b->d_vshape = vshape0;
+    b->d_vocc = vocc0;
+    b->d_vsurf = vsurf0;
+    b->vox_dirty = vdirty0;
+    return st != BB_OK ? st : BB_ERR_DEVICE;
+  }
+  b->buf.release();
+  b->buf = std::move(nb);
+  b->base.pgs_lcap = base0.pgs_lcap;
+  b->base.any_voxel = base0.any_voxel;
+  b->base.any_island = base0.any_island;
+  b->base.team_grid = base0.team_grid;
+  b->vox_dirty = false;  // the pools came over with the rest
+  b->NP = cap;
+  hw.pair_cap = cap;
+  hw.pair_scale *= 2u;
+  b->infos[0].pair_cap = cap;
+  HIPCHK(hipMemcpy(b->base.worlds, b->infos.data(), sizeof(WorldInfo), hipMemcpyHostToDevice));
+  if (b->graph) {
+    (void)hipGraphExecDestroy(b->graph);
+    b->graph = nullptr;
+  }
   return BB_OK;
 }
 
@@ -899,7 +966,9 @@ bb_status restore_world(const void* src, size_t size, int warm_start, bb_world**
   std::memcpy(&h, src, sizeof(h));
   if (std::memcmp(h.magic, kSnapMagic, 8) != 0 || h.version != kSnapVersion) return BB_ERR_INVALID;
   if (h.nshapes > BB_MAX_VOXEL_SHAPES || h.shape_words > BB_MAX_VOXEL_OCC_WORDS) return BB_ERR_INVALID;
-  if (size != snap_bytes(h.n, h.npairs, h.nshapes, h.shape_words) || h.npairs > 6 * pair_capacity(h.n))
+  // at most six slots per body pair (voxel pairs)
+  if (size != snap_bytes(h.n, h.npairs, h.nshapes, h.shape_words) ||
+      (uint64_t)h.npairs > 3ull * h.n * (h.n ? h.n - 1ull : 0ull))
     return BB_ERR_INVALID;
   const char* in = (const char*)src + sizeof(h);
   bb_world* w = nullptr;
@@ -932,6 +1001,8 @@ bb_status restore_world(const void* src, size_t size, int warm_start, bb_world**
   }
   in += (size_t)h.n * sizeof(bb_body_desc);
   bb_batch* b = w->b;
+  // a world whose pair capacity grew comes back with room for its pairs
+  while ((uint64_t)pair_capacity(h.n) * b->worlds[0].pair_scale < h.npairs) b->worlds[0].pair_scale *= 2u;
   st = build(b);
   if (st != BB_OK) return fail(st);
   if (h.npairs > b->worlds[0].pair_cap) return fail(BB_ERR_INVALID);
@@ -1755,9 +1826,43 @@ bb_status bb_world_skip_warm_start_once(bb_world* w) {
   return BB_OK;
 }
 
+// One step at a time, each checked for a pair overflow (counter bit 0: the
+// broad phase found more pairs than the world's pair capacity and dropped
+// the rest).  Such a step is run again with twice the capacity
+// (grow_pairs keeps every device array), from the state it read: a step
+// writes only the other parity, so that state, the manifolds it warm-starts
+// from included, is still there.  The replay sees the same world table and
+// one-shot settings as the first run, so the result equals a step with
+// enough capacity from the start, bit for bit.
 bb_status bb_world_step(bb_world* w, uint32_t nsteps, void* stream) {
   if (!w) return BB_ERR_INVALID;
-  return bb_batch_step(w->b, nsteps, stream, 0);
+  constexpr int kPairGrowths = 16;
+  bb_batch* b = w->b;
+  if (b->worlds.empty()) return BB_ERR_STATE;
+  for (uint32_t s = 0; s < nsteps; ++s) {
+    bb_status st = build(b);
+    if (st != BB_OK) return st;
+    HostWorld& hw = b->worlds[0];
+    const std::vector<WorldInfo> infos0 = b->infos;
+    const bool skip0 = hw.skip_ws_once, drop0 = hw.drop_prev;
+    st = bb_batch_step(b, 1, stream, 0);
+    for (int g = 0; st == BB_OK; ++g) {
+      uint32_t ovf = 0;
+      HIPCHK(hipStreamSynchronize(b->stream));
+      HIPCHK(hipMemcpy(&ovf, b->base.stats + ST_OVERFLOW, 4, hipMemcpyDeviceToHost));
+      if (!(ovf & 1u)) break;
+      if (g == kPairGrowths) return BB_ERR_CAPACITY;
+      b->latest ^= 1;
+      b->step_index -= 1;
+      b->infos = infos0;
+      hw.skip_ws_once = skip0;
+      hw.drop_prev = drop0;
+      st = grow_pairs(b);  // also uploads the world table
+      if (st == BB_OK) st = bb_batch_step(b, 1, stream, 0);
+    }
+    if (st != BB_OK) return st;
+  }
+  return BB_OK;
 }
 
 bb_status bb_world_sync(bb_world* w) { return w ? sync_batch(w->b) : BB_ERR_INVALID; }

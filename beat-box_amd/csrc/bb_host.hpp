@@ -45,6 +45,7 @@ struct HostWorld {
   bb_world_desc desc;
   std::vector<bb_body_desc> bodies;  // id order
   bool has_voxel = false;            // a body is a voxel shape (kModeVoxel)
+  uint32_t pair_scale = 1;           // pair capacity multiplier, doubled on overflow
   bool skip_ws_once = false;
   bool drop_prev = false;
   uint32_t body_off = 0, act_off = 0, pair_off = 0, pair_cap = 0, n = 0, a = 0;
@@ -54,6 +55,7 @@ struct HostWorld {
 // and `bytes` sums what would be (used to price a world for admission).
 struct Buffers {
   std::vector<void*> ptrs;
+  std::vector<size_t> sizes;  // bytes of each allocation, in ptrs order
   uint64_t bytes = 0;
   bool count_only = false;
   template <class T>
@@ -67,12 +69,14 @@ struct Buffers {
     if (hipMalloc(&p, n) != hipSuccess) p = nullptr;
     else bytes += n;
     ptrs.push_back(p);
+    sizes.push_back(n);
     return (T*)p;
   }
   void release() {
     for (void* p : ptrs)
       if (p) (void)hipFree(p);
     ptrs.clear();
+    sizes.clear();
     bytes = 0;
   }
 };
@@ -186,6 +190,9 @@ bb_status upload_images(bb_batch* b, const Images& img, uint32_t body_lo, uint32
 // previous step, and the world's counters.
 bb_status reset_world_state(bb_batch* b, uint32_t w, hipStream_t st);
 uint32_t pgs_lds_rows(const bb_batch* b);
+// Doubles the pair capacity of a built single-world batch, keeping every
+// device array's contents (both parities, the manifolds included).
+bb_status grow_pairs(bb_batch* b);
 // Counters of one world (raw device slots, ST_COUNT of them) as the C-ABI
 // struct.
 void stats_from_raw(const uint32_t* raw, uint64_t step_index, bb_step_stats* out);

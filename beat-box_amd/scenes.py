@@ -277,6 +277,30 @@ def sphere_gas(seed: int = 0xBB03, n: int = 1_000_000, half: float = 500.0):
     return bodies
 
 
+def dense_spheres(seed: int, side: int = 5, radius: float = 0.5, spacing: float = 0.45):
+    """Pair-capacity stress: side^3 spheres of one radius on a cubic grid
+    whose spacing is below the radius, so each sphere's box overlaps the
+    boxes of the spheres up to two grid steps away on every axis (about 60
+    pairs per sphere, well over 8 per body); centres jittered by
+    U(-0.02, 0.02), no ground; run with gravity 0."""
+    n = side ** 3
+    bodies = np.zeros(n, BODY_DESC)
+    idx = np.arange(n)
+    c = idx.astype(np.uint64) * np.uint64(3)
+    bodies["shape"] = SHAPE_SPHERE
+    bodies["flags"] = BODY_DYNAMIC
+    for k, g in enumerate((idx % side, (idx // side) % side, idx // (side * side))):
+        bodies["position"][:, k] = ((g - (side - 1) / 2.0) * spacing
+                                    + uniform(seed, c + np.uint64(k), -0.02, 0.02)).astype(np.float32)
+    bodies["rotation"] = (0.0, 0.0, 0.0, 1.0)
+    bodies["radius"] = radius
+    bodies["minimum"] = -radius
+    bodies["maximum"] = radius
+    bodies["mass"] = np.float32(4.0 / 3.0 * math.pi * radius ** 3)
+    bodies["friction"] = 0.5
+    return bodies
+
+
 def serving_pile(seed: int, n: int):
     """One serving request: a random pile of n bodies, 50/50 boxes and
     spheres with sizes (edge / diameter) U(0.2, 1.0) m, centres in a column

@@ -301,7 +301,13 @@ bb_status bb_world_set_iterations(bb_world* w, uint32_t iterations);
 bb_status bb_world_skip_warm_start_once(bb_world* w);
 
 /* RigidBodyManager::simulate() (src/rigid_body_manager.cpp:1606-1622), nsteps
- * times, enqueued on `stream` (NULL = library stream).  Asynchronous. */
+ * times, on the library stream (`stream` is reserved).  The reference's fixed
+ * pair capacity becomes a growing one: after each step the call reads the
+ * step's pair-overflow bit (one 4-byte read and a stream synchronisation per
+ * step), and a step that overflowed is run again, from the state it started
+ * from, with the world's pair capacity doubled (at most 16 times, else
+ * BB_ERR_CAPACITY).  The result equals a step with enough capacity from the
+ * start, bit for bit.  Returns when the steps are done. */
 bb_status bb_world_step(bb_world* w, uint32_t nsteps, void* stream);
 /* device.wait_idle() after a step (src/renderer_manager.cpp:254-257). */
 bb_status bb_world_sync(bb_world* w);
@@ -375,7 +381,10 @@ bb_status bb_batch_add_world(bb_batch* b, const bb_world_desc* desc,
 bb_status bb_batch_add_voxel_shape(bb_batch* b, const bb_voxel_shape_desc* desc,
                                    bb_voxel_shape_info* out);
 /* Steps every world nsteps times.  With use_graph != 0 the step sequence is
- * captured once as a hipGraph and replayed. */
+ * captured once as a hipGraph and replayed.  Asynchronous: a world whose
+ * pairs overflow its capacity keeps the pairs that fit and reports the
+ * overflow in its counters (bb_batch_world_stats); the replay-and-grow of
+ * bb_world_step is for single worlds. */
 bb_status bb_batch_step(bb_batch* b, uint32_t nsteps, void* stream,
                         int use_graph);
 bb_status bb_batch_sync(bb_batch* b);

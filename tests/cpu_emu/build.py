@@ -7,6 +7,7 @@ emulation header in tests/cpu_emu/hip/, with AddressSanitizer.  emu_main
 """
 from __future__ import annotations
 
+import fcntl
 import os
 import subprocess
 from concurrent.futures import ThreadPoolExecutor
@@ -23,7 +24,15 @@ UNITS = [(os.path.join(CSRC, "bb_step.hip"), True), (os.path.join(CSRC, "bb_capi
 
 
 def build() -> str:
+    """Builds emu_main; test processes running side by side (pytest -n) take
+    turns through a lock file, so no two write the same objects at once."""
     os.makedirs(OUT_DIR, exist_ok=True)
+    with open(os.path.join(OUT_DIR, ".lock"), "w") as lock:
+        fcntl.flock(lock, fcntl.LOCK_EX)
+        return _build()
+
+
+def _build() -> str:
     exe = os.path.join(OUT_DIR, "emu_main")
 
     def compile_one(unit):

@@ -51,8 +51,9 @@ def _scene(path, worlds, shapes=()):
             f.write(arr.tobytes())
 
 
-def _run(exe, scene, out, steps, team_bodies, block, graph=1, lane_bodies=0):
-    r = subprocess.run([exe, scene, str(steps), str(team_bodies), str(block), str(graph), out, str(lane_bodies)],
+def _run(exe, scene, out, steps, team_bodies, block, graph=1, lane_bodies=0, single=0):
+    r = subprocess.run([exe, scene, str(steps), str(team_bodies), str(block), str(graph), out, str(lane_bodies),
+                        str(single)],
                        capture_output=True, text=True, env=ENV, timeout=600)
     assert r.returncode == 0, r.stderr[-3000:]
     assert "AddressSanitizer" not in r.stderr, r.stderr[-3000:]
@@ -204,3 +205,34 @@ def test_emulated_voxel_worlds_equal_oracle(emu_main, tmp_path):
     # the voxel worlds made contact
     last = [l for l in log.splitlines() if l.startswith(f"step {steps - 1} world 1")][0].split()
     assert int(last[7]) > 0
+
+
+def test_emulated_pair_overflow_grows_and_replays_exactly(emu_main, tmp_path):
+    """125 spheres packed closer than their radius: about 3,400 box pairs
+    against the 8n+64 = 1,064 pair slots a world starts with.  Stepped as a
+    batch world the overflow is flagged and the surplus pairs dropped; the
+    single-world step doubles the capacity and runs the step again from the
+    state it read (twice here), so its trajectory equals the oracle's, which
+    has no pair capacity, bit for bit, and no step reports an overflow."""
+    from oracle_bind import OracleWorld
+
+    bodies = scenes.dense_spheres(0xBB71)
+    d = beatbox.WorldDesc.make(**scenes.world_desc(iterations=8, morton_bits=63, quirks=0, gravity=0.0))
+    scene = str(tmp_path / "scene.bin")
+    _scene(scene, [(d, bodies)])
+    steps = 6
+    log_b, raw_b = _run(emu_main, scene, str(tmp_path / "b.bin"), steps, 0, 0, graph=0)
+    first = log_b.splitlines()[0].split()
+    assert int(first[11]) & 1 and int(first[5]) > 3 * 1064
+    log_s, raw_s = _run(emu_main, scene, str(tmp_path / "s.bin"), steps, 0, 0, graph=0, single=1)
+    lines = [l.split() for l in log_s.splitlines()]
+    assert all(int(l[11]) == 0 for l in lines)
+    assert int(lines[0][5]) == int(first[5])
+    o = OracleWorld(d)
+    o.add_bodies(bodies)
+    o.step(steps)
+    ref = o.read_bodies()
+    got = np.frombuffer(raw_s, dtype=beatbox.BODY_STATE)
+    for f in ("position", "rotation", "velocity", "omega", "flags", "id"):
+        assert got[f].tobytes() == ref[f].tobytes(), f
+    assert raw_b != raw_s

@@ -1,0 +1,71 @@
+"""Pair capacity past the reference's fixed buffer caps (RigidBodyManager,
+src/rigid_body_manager.cpp:132-282): a world
+starts with 8n+64 pair slots; a step whose broad phase finds more pairs sets
+the overflow counter, and the single-world step then doubles the capacity and
+runs the step again from the state it read.  Spheres packed closer than their
+radius give 27 to 42 pairs per body, so the first step overflows; the grown
+world follows the oracle (which has no capacity) bit for bit, taps included,
+a batch world of the same scene reports the overflow instead, and a grown
+world's checkpoint restores with room for its pairs."""
+from __future__ import annotations
+
+import pytest
+
+import beatbox
+import scenes
+from oracle_bind import OracleWorld
+from parity_util import compare_step
+
+pytestmark = pytest.mark.gpu
+
+DESC = scenes.world_desc(iterations=8, morton_bits=63, quirks=0, gravity=0.0)
+
+
+@pytest.mark.parametrize("side", [5, 10])
+def test_pair_overflow_grows_and_matches_oracle(side):
+    bodies = scenes.dense_spheres(0xBB71 + side, side=side)
+    n = len(bodies)
+    desc = beatbox.WorldDesc.make(**DESC)
+    gpu = beatbox.RigidBodyManager(desc)
+    gpu.add_bodies(bodies)
+    ora = OracleWorld(desc)
+    ora.add_bodies(bodies)
+    for s in range(6):
+        gpu.simulate(1)
+        ora.step(1)
+        bad = compare_step(gpu, ora, desc.morton_bits, s)
+        assert not bad, "\n".join(bad[:20])
+        st = gpu.read_back_sim_config()
+        assert st["overflow"] == 0, s
+        if s == 0:
+            assert st["broad_phase_collision_count"] > 3 * (8 * n + 64)
+    gpu.destroy()
+
+
+def test_batch_world_reports_the_overflow():
+    bodies = scenes.dense_spheres(0xBB76)
+    b = beatbox.Batch()
+    b.add_world(beatbox.WorldDesc.make(**DESC), bodies)
+    b.step(1)
+    b.sync()
+    st = b.stats(0)
+    assert st["overflow"] & 1 and st["broad_phase_collision_count"] > 8 * len(bodies) + 64
+    b.destroy()
+
+
+def test_grown_world_checkpoint_continues_bit_identically():
+    bodies = scenes.dense_spheres(0xBB77)
+    desc = beatbox.WorldDesc.make(**DESC)
+    a = beatbox.RigidBodyManager(desc)
+    a.add_bodies(bodies)
+    a.simulate(2)
+    assert a.read_back_sim_config()["broad_phase_collision_count"] > 8 * len(bodies) + 64
+    blob = a.snapshot()
+    b = beatbox.RigidBodyManager.restore(blob, warm_start=True)
+    a.simulate(5)
+    b.simulate(5)
+    x, y = a.read_bodies(), b.read_bodies()
+    for f in ("position", "rotation", "velocity", "omega", "flags"):
+        assert x[f].tobytes() == y[f].tobytes(), f
+    a.destroy()
+    b.destroy()
