@@ -780,11 +780,24 @@ bb_status profile_batch(bb_batch* b, uint32_t nsteps, float* step_ms, float* ker
   if (counts && nsteps &&
       hipMalloc(&dcounts, (size_t)nsteps * BB_COUNT_FIELDS * sizeof(unsigned long long)) != hipSuccess)
     return BB_ERR_DEVICE;
+  // team stage clocks: the team kernel runs before the stage kernels in a
+  // timed step, so its whole time lands in the first stage's interval; the
+  // slowest team's own marks split it by stage
+  const size_t nclk = (size_t)kMaxTeams * kTeamMarks;
+  unsigned long long* dclock = nullptr;
+  if (kernel_ms && nsteps && b->base.team_grid) {
+    if (hipMalloc(&dclock, nsteps * nclk * sizeof(unsigned long long)) != hipSuccess) {
+      (void)hipFree(dcounts);
+      return BB_ERR_DEVICE;
+    }
+    HIPCHK(hipMemset(dclock, 0, nsteps * nclk * sizeof(unsigned long long)));
+  }
   std::vector<hipEvent_t> ev((size_t)nsteps * 8);
   for (auto& e : ev) HIPCHK(hipEventCreate(&e));
   for (uint32_t s = 0; s < nsteps; ++s) {
     const int cur = b->latest ^ 1;
-    const Dev d = dev_for(b, cur);
+    Dev d = dev_for(b, cur);
+    if (dclock) d.team_clock = dclock + s * nclk;
     Lane L[2];
     const int nl = lanes_of(b, d, L);
     launch_step_timed(L, nl, b->stream, &ev[(size_t)s * 8]);
@@ -807,6 +820,37 @@ bb_status profile_batch(bb_batch* b, uint32_t nsteps, float* step_ms, float* ker
     }
   }
   for (auto& e : ev) (void)hipEventDestroy(e);
+  if (dclock) {
+    std::vector<unsigned long long> clk(nsteps * nclk);
+    int khz = 0;
+    hipError_t rc = hipMemcpy(clk.data(), dclock, clk.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost);
+    (void)hipFree(dclock);
+    int dev = 0;
+    if (rc == hipSuccess) rc = hipGetDevice(&dev);
+    if (rc == hipSuccess) rc = hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev);
+    if (rc != hipSuccess || khz <= 0) {
+      (void)hipFree(dcounts);
+      return BB_ERR_DEVICE;
+    }
+    const uint32_t nt = std::min(b->team_table.count, kMaxTeams);
+    for (uint32_t s = 0; s < nsteps; ++s) {
+      const unsigned long long* c = &clk[s * nclk];
+      uint32_t slow = nt;
+      unsigned long long span = 0;
+      for (uint32_t j = 0; j < nt; ++j) {
+        const unsigned long long* m = c + (size_t)j * kTeamMarks;
+        if (m[kTeamMarks - 1] > m[0] && m[kTeamMarks - 1] - m[0] > span) {
+          span = m[kTeamMarks - 1] - m[0];
+          slow = j;
+        }
+      }
+      if (slow == nt) continue;
+      const unsigned long long* m = c + (size_t)slow * kTeamMarks;
+      const float ms_per_tick = 1.0f / (float)khz;
+      kernel_ms[0] = std::max(0.0f, kernel_ms[0] - (float)span * ms_per_tick);
+      for (uint32_t k = 0; k + 1 < kTeamMarks; ++k) kernel_ms[k] += (float)(m[k + 1] - m[k]) * ms_per_tick;
+    }
+  }
   if (dcounts) {
     const hipError_t rc =
         hipMemcpy(counts, dcounts, (size_t)nsteps * BB_COUNT_FIELDS * sizeof(uint64_t), hipMemcpyDeviceToHost);

@@ -70,6 +70,10 @@ struct TeamCtl {
 // Team j steps world world[j] with workgroups [first[j], first[j+1]) of the
 // k_team launch; workgroups from first[count] on have nothing to do.
 static constexpr uint32_t kMaxTeams = 256, kTeamSharedBytes = 4096;
+// Stage clock marks of a team step (profiling): start, then the end of the
+// build, broad, narrow (with contacts), post, PGS (with levels), AVBD and
+// finish stages.
+static constexpr uint32_t kTeamMarks = 8;
 struct TeamTable {
   uint32_t count;
   uint32_t first[kMaxTeams + 1];
@@ -195,6 +199,9 @@ struct Dev {
   uint32_t* team_hist;
   char* team_shared;
   uint32_t team_grid;     // workgroups of the k_team launch
+  // [kMaxTeams * kTeamMarks] wall-clock marks of each team's stages, written
+  // when non-null (profiling launches only)
+  unsigned long long* team_clock;
   // [W] world stepped by workgroup b of the per-world kernels: worlds in
   // descending body count, so the longest workgroups start first
   const uint32_t* order;

@@ -3636,26 +3636,40 @@ __global__ __launch_bounds__(BLOCK) void k_team(Dev d) {
   const GroupTeam<BLOCK> t{&S, &Q, d.team_ctl + j, d.team_hist + (size_t)b0 * 256, blockIdx.x - b0, nb};
   TeamShared& G = *reinterpret_cast<TeamShared*>(d.team_shared + (size_t)j * kTeamSharedBytes);
   const uint32_t w = T.world[j];
+  // stage boundaries: every workgroup has passed the barrier before rank 0
+  // stamps the clock
+  unsigned long long* clk = d.team_clock ? d.team_clock + (size_t)j * kTeamMarks : nullptr;
+  auto mark = [&](uint32_t m) {
+    if (clk && t.rank() == 0) clk[m] = wall_clock64();
+  };
+  mark(0);
   s_build(d, w, t, G.build);
   t.sync();
+  mark(1);
   s_broad(d, w, t, G.broad);
   t.sync();
+  mark(2);
   s_narrow<false>(d, w, t);  // voxel worlds never run as teams
   t.sync();
   s_contacts(d, w, t);
   t.sync();
+  mark(3);
   s_post(d, w, t, G.post);
   t.sync();
+  mark(4);
   s_levels(d, w, t, G.post, nullptr);
   t.sync();
   if (d.worlds[w].flags & BB_SIM_USE_GRAPH_COLORING) s_pgs<false, false>(d, w, t, nullptr, 0u);
   else if (d.worlds[w].solver == BB_SOLVER_PGS) s_pgs<true, true>(d, w, t, nullptr, 0u);
   else s_pgs<true, false>(d, w, t, nullptr, 0u);
   t.sync();
+  mark(5);
   s_avbd(d, w, t, G.avbd);
   t.sync();
+  mark(6);
   s_finish(d, w, t);
   t.sync();
+  mark(7);
   // a barrier that ran out of spins: flag the world (overflow bit 1)
   if (t.rank() == 0 && __hip_atomic_load(&t.ctl->error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u)
     atomicOr(&d.stats[w * ST_COUNT + ST_OVERFLOW], 2u);

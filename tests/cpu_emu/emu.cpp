@@ -420,7 +420,11 @@ hipError_t hipGetDevice(int* d) {
   return hipSuccess;
 }
 hipError_t hipSetDevice(int) { return hipSuccess; }
-hipError_t hipDeviceGetAttribute(int* v, hipDeviceAttribute_t, int) {
+hipError_t hipDeviceGetAttribute(int* v, hipDeviceAttribute_t a, int) {
+  if (a == hipDeviceAttributeWallClockRate) {
+    *v = 100000;  // kHz, the rate of wall_clock64 above
+    return hipSuccess;
+  }
   const char* cu = std::getenv("EMU_CUS");
   *v = cu ? std::atoi(cu) : 256;
   return hipSuccess;

@@ -13,6 +13,7 @@
 #include <string.h>
 #include <stddef.h>
 #include <math.h>
+#include <chrono>
 #include <functional>
 #include <utility>
 #include <tuple>
@@ -61,6 +62,11 @@ void launch(dim3 grid, dim3 block, size_t shmem, std::function<void()> fn);
 #define blockDim (emu::t_bdim)
 
 static inline void __syncthreads() { emu::barrier(); }
+// the device's constant-rate clock, here nanoseconds / 10 (100 MHz)
+static inline unsigned long long wall_clock64() {
+  return (unsigned long long)std::chrono::duration_cast<std::chrono::nanoseconds>(
+             std::chrono::steady_clock::now().time_since_epoch()).count() / 10ull;
+}
 static inline void __threadfence() { __atomic_thread_fence(__ATOMIC_SEQ_CST); }
 static inline void __threadfence_block() { __atomic_thread_fence(__ATOMIC_SEQ_CST); }
 #define __builtin_amdgcn_s_sleep(x) emu::yield()
@@ -150,7 +156,7 @@ typedef struct emu_graph* hipGraphExec_t;
 typedef struct emu_node* hipGraphNode_t;
 enum hipMemcpyKind { hipMemcpyHostToHost, hipMemcpyHostToDevice, hipMemcpyDeviceToHost, hipMemcpyDeviceToDevice, hipMemcpyDefault };
 enum hipStreamCaptureMode { hipStreamCaptureModeGlobal, hipStreamCaptureModeThreadLocal, hipStreamCaptureModeRelaxed };
-enum hipDeviceAttribute_t { hipDeviceAttributeMultiprocessorCount = 1 };
+enum hipDeviceAttribute_t { hipDeviceAttributeMultiprocessorCount = 1, hipDeviceAttributeWallClockRate = 2 };
 #define hipStreamNonBlocking 1u
 #define hipStreamDefault 0u
 #define hipHostMallocDefault 0u

@@ -144,6 +144,36 @@ def test_profile_counts_equal_world_stats():
     assert last[7] == sum(s["sleeping_count"] for s in st)
 
 
+def test_profile_splits_team_time_by_stage():
+    """A world stepped by a team runs as one kernel; the profile splits its
+    time by stage from the team's own clock marks: every stage of an AVBD
+    team world gets time, the first stage keeps only its share, and the
+    stages still add up to the steps.  Profiling does not change the
+    trajectory."""
+    bodies = scenes.serving_pile(46, 3000)
+    desc = beatbox.WorldDesc.make(**scenes.world_desc(solver=scenes.SOLVER_AVBD, iterations=10,
+                                                      morton_bits=63, quirks=0))
+    runs = []
+    for prof in (True, False):
+        b = beatbox.Batch()
+        b.add_world(desc, bodies)
+        b.set_param(beatbox.PARAM_TEAM_BODIES, 1024)
+        b.step(3)
+        if prof:
+            step_ms, kern_ms = b.profile(4)
+            assert np.all(kern_ms[[0, 1, 2, 3, 5, 6]] > 0), kern_ms
+            total = float(step_ms.sum())
+            assert abs(float(kern_ms.sum()) - total) < 0.05 * total, (kern_ms, total)
+            assert kern_ms[0] < 0.6 * total and kern_ms[5] > kern_ms[0], kern_ms
+        else:
+            b.step(4)
+        b.sync()
+        runs.append(b.read_world(0, len(bodies)))
+        b.destroy()
+    for f in ("position", "rotation", "velocity", "omega", "flags"):
+        assert runs[0][f].tobytes() == runs[1][f].tobytes(), f
+
+
 def _bodies_equal(a, b, n_worlds, sizes):
     for w in range(n_worlds):
         x = a.read_world(w, sizes[w])
