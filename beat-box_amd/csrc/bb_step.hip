@@ -247,6 +247,9 @@ struct BlockTeam {
   BB_DI uint32_t size() const { return BLOCK; }
   BB_DI void sync() const { __syncthreads(); }
   BB_DI void fence() const { __threadfence_block(); }
+  // every leaf range is this workgroup's (see GroupTeam::leaves_local)
+  BB_DI bool leaves_local(uint32_t, uint32_t) const { return true; }
+  BB_DI bool lead_group() const { return true; }
   BB_DI uint32_t scan(uint32_t v, uint32_t& total) const { return block_scan<BLOCK>(v, *scan_lds, total); }
   BB_DI bool radix_sort(uint64_t* ka, uint32_t* ia, uint64_t* kb, uint32_t* ib, uint32_t n, int passes) const {
     return block_radix_sort<BLOCK>(ka, ia, kb, ib, n, passes, *sort_lds);
@@ -271,6 +274,14 @@ struct GroupTeam {
   BB_DI uint32_t rank() const { return blk * BLOCK + threadIdx.x; }
   BB_DI uint32_t size() const { return nblocks * BLOCK; }
   BB_DI void fence() const { __threadfence(); }
+  // With leaf i handled by rank i mod size(), a tree node over leaves
+  // [first, last] inside one BLOCK-aligned run is touched by one workgroup
+  // alone: a bottom-up pass orders its steps there with workgroup scope
+  // instead of device-scope fences (an L2 writeback and invalidate each on
+  // a multi-XCD chip).
+  BB_DI bool leaves_local(uint32_t first, uint32_t last) const { return first / BLOCK == last / BLOCK; }
+  // the workgroup that finishes the nodes above the runs
+  BB_DI bool lead_group() const { return blk == 0; }
   BB_DI void sync() const {
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -460,6 +471,7 @@ BB_DI int lb_split(const KeyView& K, int first, int last) {
 // ================================================================ k_build
 struct BuildShared {
   uint32_t lo[3], hi[3];
+  uint32_t nroots;  // bounds pass: nodes handed from the runs to the lead workgroup
 };
 
 template <class T>
@@ -491,11 +503,29 @@ BB_DI void s_build(const Dev& d, uint32_t w, const T& t, BuildShared& L) {
       L.hi[tid] = 0u;
     }
     t.sync();
+    // per-thread, then per-wave extremes; one lane per wave merges them (a
+    // team's L lives in global memory, where a million same-address atomics
+    // would serialise)
+    uint32_t mlo[3] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu}, mhi[3] = {0u, 0u, 0u};
     for (uint32_t i = tid; i < N; i += NT) {
-      float4 p = d.prev.pos[off + i];
-      atomicMin(&L.lo[0], omap(p.x)); atomicMin(&L.lo[1], omap(p.y)); atomicMin(&L.lo[2], omap(p.z));
-      atomicMax(&L.hi[0], omap(p.x)); atomicMax(&L.hi[1], omap(p.y)); atomicMax(&L.hi[2], omap(p.z));
+      const float4 p = d.prev.pos[off + i];
+      const uint32_t m[3] = {omap(p.x), omap(p.y), omap(p.z)};
+      for (int k = 0; k < 3; ++k) {
+        mlo[k] = m[k] < mlo[k] ? m[k] : mlo[k];
+        mhi[k] = m[k] > mhi[k] ? m[k] : mhi[k];
+      }
     }
+    for (int o = 32; o > 0; o >>= 1)
+      for (int k = 0; k < 3; ++k) {
+        const uint32_t a = __shfl_down(mlo[k], o, 64), b = __shfl_down(mhi[k], o, 64);
+        mlo[k] = a < mlo[k] ? a : mlo[k];
+        mhi[k] = b > mhi[k] ? b : mhi[k];
+      }
+    if ((threadIdx.x & 63u) == 0u)
+      for (int k = 0; k < 3; ++k) {
+        atomicMin(&L.lo[k], mlo[k]);
+        atomicMax(&L.hi[k], mhi[k]);
+      }
     t.sync();
     v3 lo = V(ounmap(L.lo[0]), ounmap(L.lo[1]), ounmap(L.lo[2]));
     v3 hi = V(ounmap(L.hi[0]), ounmap(L.hi[1]), ounmap(L.hi[2]));
@@ -548,27 +578,55 @@ BB_DI void s_build(const Dev& d, uint32_t w, const T& t, BuildShared& L) {
     d.parent[nb + ca] = i;
     d.parent[nb + cb] = i;
     d.visit[nb + i] = 0;
+    kb[i] = (uint64_t)(uint32_t)first | ((uint64_t)(uint32_t)last << 32);  // leaf range (the sort is done with kb)
   }
+  if (tid == 0) L.nroots = 0;
   t.sync();
   // ---- bounds, bottom-up: the second child to arrive merges (exact min/max
   // through the order-preserving map, same values as the reference's integer
   // atomics, RB_sim.slang:282-349)
+  // Two phases: each workgroup climbs the nodes over its own runs of leaves
+  // (workgroup-scope ordering, see leaves_local) and hands the first node
+  // above them to the lead workgroup (ib, free after the sort); after a
+  // barrier the lead workgroup climbs the rest.  A node's bounds are the same
+  // whichever child arrives last.
+  auto merge = [&](uint32_t p) -> bool {
+    const float4 a0 = d.nmin[nb + p], a1 = d.nmax[nb + p];
+    const int l = __float_as_int(a0.w), r = __float_as_int(a1.w);
+    if (l < 0 || r < 0 || l >= (int)(2 * N - 1) || r >= (int)(2 * N - 1)) return false;
+    const float4 lm = d.nmin[nb + l], lx = d.nmax[nb + l], rm = d.nmin[nb + r], rx = d.nmax[nb + r];
+    d.nmin[nb + p] = make_float4(omin(lm.x, rm.x), omin(lm.y, rm.y), omin(lm.z, rm.z), a0.w);
+    d.nmax[nb + p] = make_float4(omax(lx.x, rx.x), omax(lx.y, rx.y), omax(lx.z, rx.z), a1.w);
+    return true;
+  };
   if (N > 1) {
     for (uint32_t i = tid; i < N; i += NT) {
       uint32_t p = d.parent[nb + leaf_off + i];
       while (p + 1 < N) {  // p is an internal node index in [0, N-2]
-        t.fence();
-        uint32_t old = atomicAdd(&d.visit[nb + p], 1u);
-        if (old == 0) break;
-        t.fence();
-        float4 a0 = d.nmin[nb + p], a1 = d.nmax[nb + p];
-        int l = __float_as_int(a0.w), r = __float_as_int(a1.w);
-        if (l < 0 || r < 0 || l >= (int)(2 * N - 1) || r >= (int)(2 * N - 1)) break;
-        float4 lm = d.nmin[nb + l], lx = d.nmax[nb + l], rm = d.nmin[nb + r], rx = d.nmax[nb + r];
-        d.nmin[nb + p] = make_float4(omin(lm.x, rm.x), omin(lm.y, rm.y), omin(lm.z, rm.z), a0.w);
-        d.nmax[nb + p] = make_float4(omax(lx.x, rx.x), omax(lx.y, rx.y), omax(lx.z, rx.z), a1.w);
-        if (p == 0) break;
+        const uint64_t rg = kb[p];
+        if (!t.leaves_local((uint32_t)rg, (uint32_t)(rg >> 32))) {
+          ib[atomicAdd(&L.nroots, 1u)] = p;
+          break;
+        }
+        __threadfence_block();
+        if (atomicAdd(&d.visit[nb + p], 1u) == 0u) break;
+        __threadfence_block();
+        if (!merge(p) || p == 0) break;
         p = d.parent[nb + p];
+      }
+    }
+    t.sync();
+    if (t.lead_group()) {
+      const uint32_t nr = L.nroots;
+      for (uint32_t k = threadIdx.x; k < nr; k += blockDim.x) {
+        uint32_t p = ib[k];
+        while (p + 1 < N) {
+          __threadfence_block();
+          if (atomicAdd(&d.visit[nb + p], 1u) == 0u) break;
+          __threadfence_block();
+          if (!merge(p) || p == 0) break;
+          p = d.parent[nb + p];
+        }
       }
     }
   }
@@ -3479,14 +3537,20 @@ BB_DI void s_avbd(const Dev& d, uint32_t w, const T& t, AvbdShared& L) {
   for (uint32_t k = tid; k < AV_NB; k += NT) L.bcount[k] = 0;
   if (tid == 0) L.used = 0;
   t.sync();
+  uint32_t used_t = 0;  // colours used: per thread, per wave, then one atomic per wave
   for (uint32_t i = tid; i < N; i += NT) {
     const uint32_t g = off + i;
     const uint32_t fl = d.cur.flags[g], c = d.bcolor[g];
     if (!(fl & BB_BODY_DYNAMIC) || (fl & BB_BODY_SLEEPING) || c >= kAvbdColors) continue;
     uint32_t layer = d.depth[g] < kAvbdLayers - 1 ? d.depth[g] : kAvbdLayers - 1;
     atomicAdd(&L.bcount[c * kAvbdLayers + layer], 1u);
-    atomicMax(&L.used, c + 1);
+    used_t = c + 1 > used_t ? c + 1 : used_t;
   }
+  for (int o = 32; o > 0; o >>= 1) {
+    const uint32_t u = __shfl_down(used_t, o, 64);
+    used_t = u > used_t ? u : used_t;
+  }
+  if ((threadIdx.x & 63u) == 0u && used_t) atomicMax(&L.used, used_t);
   t.sync();
   if (tid == 0) {
     uint32_t s = 0;
