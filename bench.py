@@ -134,7 +134,8 @@ def plain_world(steps: int, warmup: int):
 def single_world(name: str, desc, bodies, steps: int, warmup: int, profile: int) -> dict:
     """One large world stepped alone (hipGraph replay of `steps` steps after
     `warmup`), then `profile` eager steps timed per stage; worlds at or above
-    the team threshold run as workgroup teams (k_team, timed with k_build)."""
+    the team threshold run as workgroup teams (k_team, whose time the profile
+    splits by stage with the team's own clock marks)."""
     t0 = time.time()
     b = beatbox.Batch()
     b.add_world(desc, bodies)

@@ -423,7 +423,10 @@ enum {
  * wait-bracketed step time, src/renderer_manager.cpp:254-260).  Runs nsteps
  * eager steps with hipEvents recorded on the library stream around every
  * stage kernel.  step_ms[nsteps] receives each step's duration,
- * kernel_ms[BB_KERNEL_COUNT] the summed duration of each stage over the steps,
+ * kernel_ms[BB_KERNEL_COUNT] the summed duration of each stage over the steps
+ * (a world stepped by a workgroup team runs all stages in one kernel; its
+ * time is split by stage with the team's own clock marks, from the slowest
+ * team),
  * counts[nsteps * BB_COUNT_FIELDS] each step's batch totals.  Any output may
  * be NULL. */
 bb_status bb_batch_profile(bb_batch* b, uint32_t nsteps, float* step_ms, float* kernel_ms,
