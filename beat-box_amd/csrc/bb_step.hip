@@ -243,6 +243,7 @@ struct BlockTeam {
   // AVBD primal: kAvG lanes per body; empty cascade buckets skip their barrier
   static constexpr uint32_t kAvGroup = kAvG;
   static constexpr bool kSkipEmpty = true;
+  static constexpr bool kGlobalScalars = false;  // stage scalars in LDS
   BB_DI uint32_t rank() const { return threadIdx.x; }
   BB_DI uint32_t size() const { return BLOCK; }
   BB_DI void sync() const { __syncthreads(); }
@@ -271,6 +272,7 @@ struct GroupTeam {
   // AVBD primal: one lane per body, a barrier after every cascade bucket
   static constexpr uint32_t kAvGroup = 1;
   static constexpr bool kSkipEmpty = false;
+  static constexpr bool kGlobalScalars = true;  // stage scalars in a global slot
   BB_DI uint32_t rank() const { return blk * BLOCK + threadIdx.x; }
   BB_DI uint32_t size() const { return nblocks * BLOCK; }
   BB_DI void fence() const { __threadfence(); }
@@ -3036,6 +3038,8 @@ static constexpr float AV_BETA = 100000.0f, AV_GAMMA = 0.99f, AV_KMIN = 1.0f, AV
 static constexpr float AV_MARGIN = 0.0005f, AV_STICK_THRESH = 0.01f, AV_STICK_SLOP = 0.001f;
 static constexpr uint32_t AV_ITERS = 10, AV_ROUNDS = 16, AV_POST_STAB = 4;
 static constexpr uint32_t AV_NB = kAvbdColors * kAvbdLayers;
+// a team's per-workgroup bucket counts and bases fit the sort's staging area
+static_assert(2 * AV_NB <= (256 / 64) * 256, "AVBD bucket scratch exceeds SortLds::wcnt");
 
 BB_DI uint32_t avbd_priority(uint32_t body, uint32_t round) {  // avbd.slang:58-63
   uint32_t x = body * 0x9e3779b9u + round * 0x85ebca6bu;
@@ -3533,7 +3537,17 @@ BB_DI void s_avbd(const Dev& d, uint32_t w, const T& t, AvbdShared& L) {
     }
     t.sync();
   }
-  // ---- bodies grouped by (colour, layer) for the primal sweeps
+  // ---- bodies grouped by (colour, layer) for the primal sweeps.  A team's
+  // counters live in global memory: each workgroup counts its rows in LDS
+  // (the sort's staging area, lc[0, AV_NB)) and adds one total per bucket,
+  // then takes its base in each bucket (lc[AV_NB, 2 AV_NB)) with one atomic.
+  // Rows of a bucket are independent (one colour), so their order within
+  // it does not change the sweeps.
+  uint32_t* lc = nullptr;
+  if constexpr (T::kGlobalScalars) {
+    lc = t.sort_lds->wcnt;
+    for (uint32_t k = threadIdx.x; k < 2 * AV_NB; k += blockDim.x) lc[k] = 0;
+  }
   for (uint32_t k = tid; k < AV_NB; k += NT) L.bcount[k] = 0;
   if (tid == 0) L.used = 0;
   t.sync();
@@ -3543,8 +3557,13 @@ BB_DI void s_avbd(const Dev& d, uint32_t w, const T& t, AvbdShared& L) {
     const uint32_t fl = d.cur.flags[g], c = d.bcolor[g];
     if (!(fl & BB_BODY_DYNAMIC) || (fl & BB_BODY_SLEEPING) || c >= kAvbdColors) continue;
     uint32_t layer = d.depth[g] < kAvbdLayers - 1 ? d.depth[g] : kAvbdLayers - 1;
-    atomicAdd(&L.bcount[c * kAvbdLayers + layer], 1u);
+    atomicAdd(lc ? &lc[c * kAvbdLayers + layer] : &L.bcount[c * kAvbdLayers + layer], 1u);
     used_t = c + 1 > used_t ? c + 1 : used_t;
+  }
+  if constexpr (T::kGlobalScalars) {
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < AV_NB; k += blockDim.x)
+      if (lc[k]) atomicAdd(&L.bcount[k], lc[k]);
   }
   for (int o = 32; o > 0; o >>= 1) {
     const uint32_t u = __shfl_down(used_t, o, 64);
@@ -3562,13 +3581,21 @@ BB_DI void s_avbd(const Dev& d, uint32_t w, const T& t, AvbdShared& L) {
     L.boff[AV_NB] = s;
   }
   t.sync();
+  if constexpr (T::kGlobalScalars) {
+    for (uint32_t k = threadIdx.x; k < AV_NB; k += blockDim.x) {
+      if (lc[k]) lc[AV_NB + k] = atomicAdd(&L.bcount[k], lc[k]);
+      lc[k] = 0;  // now this workgroup's cursor in the bucket
+    }
+    __syncthreads();
+  }
   uint32_t* bb = d.bbucket + off;
   for (uint32_t i = tid; i < N; i += NT) {
     const uint32_t g = off + i;
     const uint32_t fl = d.cur.flags[g], c = d.bcolor[g];
     if (!(fl & BB_BODY_DYNAMIC) || (fl & BB_BODY_SLEEPING) || c >= kAvbdColors) continue;
     uint32_t layer = d.depth[g] < kAvbdLayers - 1 ? d.depth[g] : kAvbdLayers - 1;
-    bb[atomicAdd(&L.bcount[c * kAvbdLayers + layer], 1u)] = i;
+    const uint32_t k = c * kAvbdLayers + layer;
+    bb[lc ? lc[AV_NB + k] + atomicAdd(&lc[k], 1u) : atomicAdd(&L.bcount[k], 1u)] = i;
   }
   t.sync();
   const uint32_t used = L.used;
