@@ -91,13 +91,14 @@ class ServerReport(C.Structure):
         "submitted", "admitted", "retired", "queued", "resident_worlds", "resident_bodies",
         "bytes_in_use", "bytes_budget", "bytes_allocated", "server_steps", "world_steps",
         "body_steps")] + [(n, C.c_double) for n in (
-        "device_ms", "step_ms_p50", "step_ms_p99", "queue_ms_p50", "queue_ms_p99")]
+        "device_ms", "step_ms_p50", "step_ms_p99", "queue_ms_p50", "queue_ms_p99")] + [
+        ("reruns", C.c_uint64)]
 
     def as_dict(self) -> dict:
         return {n: getattr(self, n) for n, _ in self._fields_}
 
 
-REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_UNKNOWN = 0, 1, 2, 3
+REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_UNKNOWN, REQ_FAILED = 0, 1, 2, 3, 4
 
 
 class VoxelShapeDesc(C.Structure):
@@ -163,6 +164,7 @@ COUNT_NAMES = ("bodies", "active", "pairs", "manifolds", "contacts", "color_work
 COUNT_FIELDS = len(COUNT_NAMES)
 PARAM_BLOCK, PARAM_PGS_LDS_ROWS, PARAM_TEAM_BODIES, PARAM_LANE_BODIES, PARAM_LANE_BLOCK, PARAM_LANE_LDS_ROWS = \
     0, 1, 2, 3, 4, 5
+PARAM_TEAM_SPIN = 6
 
 STATUS = {0: "BB_OK", 1: "BB_ERR_INVALID", 2: "BB_ERR_CAPACITY", 3: "BB_ERR_DEVICE",
           4: "BB_ERR_NO_DEVICE", 5: "BB_ERR_OVERFLOW", 6: "BB_ERR_STATE", 7: "BB_ERR_BUDGET"}
@@ -197,6 +199,7 @@ EXPORTS = {
     "bb_world_skip_warm_start_once": (C.c_int, [C.c_void_p]),
     "bb_world_step": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p]),
     "bb_world_sync": (C.c_int, [C.c_void_p]),
+    "bb_world_set_param": (C.c_int, [C.c_void_p, C.c_int, C.c_uint32]),
     "bb_world_read_bodies": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_int]),
     "bb_world_read_stats": (C.c_int, [C.c_void_p, C.POINTER(StepStats)]),
     "bb_world_debug_tap": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_size_t,
@@ -208,6 +211,7 @@ EXPORTS = {
                                      C.c_uint32, C.POINTER(C.c_uint32)]),
     "bb_batch_step": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_int]),
     "bb_batch_sync": (C.c_int, [C.c_void_p]),
+    "bb_batch_overflow": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32)]),
     "bb_batch_read_world": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_int]),
     "bb_batch_world_stats": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(StepStats)]),
     "bb_batch_debug_tap": (C.c_int, [C.c_void_p, C.c_uint32, C.c_int, C.c_void_p, C.c_size_t,
@@ -340,6 +344,9 @@ class RigidBodyManager:
     def set_sim_type(self, solver: int) -> None:
         _check(lib().bb_world_set_solver(self._h, solver), "bb_world_set_solver")
 
+    def set_param(self, p: int, value: int) -> None:
+        _check(lib().bb_world_set_param(self._h, p, value), "bb_world_set_param")
+
     def set_iterations(self, it: int) -> None:
         _check(lib().bb_world_set_iterations(self._h, it), "bb_world_set_iterations")
 
@@ -432,8 +439,23 @@ class Batch:
     def step(self, nsteps: int = 1, use_graph: bool = False) -> None:
         _check(lib().bb_batch_step(self._h, nsteps, None, 1 if use_graph else 0), "bb_batch_step")
 
-    def sync(self) -> None:
-        _check(lib().bb_batch_sync(self._h), "bb_batch_sync")
+    def sync(self, allow_overflow: bool = False) -> None:
+        """Waits for the steps.  Raises BBError(BB_ERR_OVERFLOW) while a
+        world's overflow mark is uncollected, unless allow_overflow."""
+        rc = lib().bb_batch_sync(self._h)
+        if rc == 5 and allow_overflow:
+            return
+        _check(rc, "bb_batch_sync")
+
+    def overflow(self, cap: int | None = None) -> list:
+        """Indices of the worlds whose steps flagged an overflow since the
+        last call (bb_batch_overflow); clears the marks."""
+        cap = self.world_count() if cap is None else cap
+        out = np.zeros(max(cap, 1), np.uint32)
+        cnt = C.c_uint32(0)
+        _check(lib().bb_batch_overflow(self._h, out.ctypes.data_as(C.c_void_p), cap, C.byref(cnt)),
+               "bb_batch_overflow")
+        return out[:min(cnt.value, cap)].tolist()
 
     def profile(self, nsteps: int, counts: bool = False):
         """Eager steps timed with hipEvents on the library stream: returns
@@ -551,7 +573,7 @@ class Server:
         stats = StepStats()
         _check(lib().bb_server_poll(self._h, ticket, C.byref(st), out.ctypes.data_as(C.c_void_p), max(n, 1),
                                     C.byref(stats)), "bb_server_poll")
-        if st.value == REQ_DONE:
+        if st.value in (REQ_DONE, REQ_FAILED):
             return st.value, out[:n], stats.as_dict()
         return st.value, None, None
 

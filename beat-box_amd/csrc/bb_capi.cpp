@@ -209,6 +209,8 @@ bb_status alloc_arrays(bb_batch* b, Buffers& A, uint32_t W, uint32_t NB, uint32_
   d.bbucket = A.alloc<uint32_t>(NB);
   d.bbucket_off = A.alloc<uint32_t>((size_t)W * (kAvbdColors * kAvbdLayers + 1));
   d.stats = A.alloc<uint32_t>((size_t)W * ST_COUNT);
+  d.ovf_any = A.alloc<uint32_t>(1);
+  d.team_spin = b->team_spin;
   d.teams = A.alloc<TeamTable>(1);
   d.team_ctl = A.alloc<TeamCtl>(kMaxTeams);
   d.team_hist = A.alloc<uint32_t>((size_t)kMaxTeamBlocks * 256);
@@ -229,6 +231,7 @@ bb_status alloc_arrays(bb_batch* b, Buffers& A, uint32_t W, uint32_t NB, uint32_
   // rounds), counters, previous pair counts and pair-row offsets
   HIPCHK(hipMemset(d.owner, 0, (size_t)NB * kColors * 4));
   HIPCHK(hipMemset(d.stats, 0, (size_t)W * ST_COUNT * 4));
+  HIPCHK(hipMemset(d.ovf_any, 0, 4));
   for (int s = 0; s < 2; ++s) {
     HIPCHK(hipMemset(b->npairs[s], 0, (size_t)W * 4));
     HIPCHK(hipMemset(b->pstart[s], 0, (size_t)NB * 4));
@@ -386,7 +389,8 @@ uint32_t assign_teams(bb_batch* b, uint32_t grid_cap) {
   grid_cap = std::min(grid_cap, kMaxTeamBlocks);
   std::vector<uint32_t> cand;
   for (uint32_t w = 0; w < (uint32_t)b->worlds.size(); ++w)
-    if (b->infos[w].n >= b->team_bodies && b->infos[w].n > 0 && !(b->infos[w].mode & kModeVoxel))
+    if (b->infos[w].n >= b->team_bodies && b->infos[w].n > 0 && !(b->infos[w].mode & kModeVoxel) &&
+        !b->worlds[w].no_team)
       cand.push_back(w);  // voxel worlds run one workgroup per world
   // largest worlds first; each team gets at least two workgroups
   std::sort(cand.begin(), cand.end(), [&](uint32_t x, uint32_t y) {
@@ -1004,6 +1008,40 @@ bb_status snapshot_world(bb_batch* b, void* dst, size_t cap, size_t* size) {
   return rc == hipSuccess ? BB_OK : BB_ERR_DEVICE;
 }
 
+// Checks, on the host, every index of a blob's arrays (in = the first array,
+// after the body descriptors) that the next step uses to address device
+// memory: ids are a permutation of [0, n), the id -> row map agrees with
+// them, pair rows are below n, and every manifold slot with contacts names
+// rows below n and holds at most 8 contacts.  A blob that fails is refused
+// before anything reaches the device.
+bool valid_blob_indices(const char* in, uint32_t n, uint32_t p) {
+  auto u32 = [&](size_t byte_off, size_t i) {
+    uint32_t v;
+    std::memcpy(&v, in + byte_off + i * 4, 4);
+    return v;
+  };
+  const size_t o_id = 0, o_ent = (size_t)n * 12, o_pairs = (size_t)n * 104;
+  const size_t o_hdr = o_pairs + (size_t)p * 88;
+  std::vector<uint8_t> seen(n, 0);
+  for (uint32_t r = 0; r < n; ++r) {
+    const uint32_t id = u32(o_id, r);
+    if (id >= n || seen[id]) return false;
+    seen[id] = 1;
+  }
+  for (uint32_t r = 0; r < n; ++r) {
+    const uint32_t id = u32(o_id, r);
+    if (u32(o_ent, id) != r) return false;
+  }
+  for (uint32_t k = 0; k < p; ++k) {
+    if (u32(o_pairs, 2 * (size_t)k) >= n || u32(o_pairs, 2 * (size_t)k + 1) >= n) return false;
+    int32_t h[4];
+    std::memcpy(h, in + o_hdr + (size_t)k * 16, 16);
+    if (h[3] > (int32_t)kMaxContacts) return false;
+    if (h[3] > 0 && ((uint32_t)h[0] >= n || (uint32_t)h[1] >= n)) return false;
+  }
+  return true;
+}
+
 bb_status restore_world(const void* src, size_t size, int warm_start, bb_world** out) {
   if (!src || !out || size < sizeof(SnapHeader)) return BB_ERR_INVALID;
   SnapHeader h;
@@ -1044,6 +1082,7 @@ bb_status restore_world(const void* src, size_t size, int warm_start, bb_world**
     if (st != BB_OK) return fail(st);
   }
   in += (size_t)h.n * sizeof(bb_body_desc);
+  if (!valid_blob_indices(in, h.n, h.npairs)) return fail(BB_ERR_INVALID);
   bb_batch* b = w->b;
   // a world whose pair capacity grew comes back with room for its pairs
   while ((uint64_t)pair_capacity(h.n) * b->worlds[0].pair_scale < h.npairs) b->worlds[0].pair_scale *= 2u;
@@ -1095,6 +1134,9 @@ bb_status set_param(bb_batch* b, bb_param p, uint32_t value) {
   } else if (p == BB_PARAM_LANE_LDS_ROWS) {
     if (value > kMaxPgsLdsRows) return BB_ERR_INVALID;
     b->lane_lds = value;
+  } else if (p == BB_PARAM_TEAM_SPIN) {
+    b->team_spin = value;
+    b->base.team_spin = value;
   } else {
     return BB_ERR_INVALID;
   }
@@ -1127,8 +1169,9 @@ bb_status autotune_batch(bb_batch* b, uint32_t reps, float* best_ms) {
   const uint32_t saved_block = b->block, saved_lds = b->pgs_lds;
   const size_t stat_bytes = b->worlds.size() * ST_COUNT * sizeof(uint32_t);
   uint32_t* stat_copy = nullptr;
-  HIPCHK(hipMalloc(&stat_copy, stat_bytes));
+  HIPCHK(hipMalloc(&stat_copy, stat_bytes + 4));
   HIPCHK(hipMemcpyAsync(stat_copy, b->base.stats, stat_bytes, hipMemcpyDeviceToDevice, b->stream));
+  HIPCHK(hipMemcpyAsync((char*)stat_copy + stat_bytes, b->base.ovf_any, 4, hipMemcpyDeviceToDevice, b->stream));
   uint32_t mx = 0;
   for (auto& hw : b->worlds) mx = std::max(mx, (uint32_t)hw.bodies.size());
   std::vector<uint32_t> lds_opts;
@@ -1180,6 +1223,8 @@ bb_status autotune_batch(bb_batch* b, uint32_t reps, float* best_ms) {
   b->step_index = saved_index;
   b->stepped = saved_stepped;
   if (hipMemcpyAsync(b->base.stats, stat_copy, stat_bytes, hipMemcpyDeviceToDevice, b->stream) != hipSuccess ||
+      hipMemcpyAsync(b->base.ovf_any, (char*)stat_copy + stat_bytes, 4, hipMemcpyDeviceToDevice, b->stream) !=
+          hipSuccess ||
       hipStreamSynchronize(b->stream) != hipSuccess)
     rc = rc != BB_OK ? rc : BB_ERR_DEVICE;
   (void)hipFree(stat_copy);
@@ -1198,10 +1243,52 @@ bb_status autotune_batch(bb_batch* b, uint32_t reps, float* best_ms) {
   return rc;
 }
 
+bb_status read_overflow_any(bb_batch* b, uint32_t* out) {
+  *out = 0;
+  if (!b->built || !b->base.ovf_any) return BB_OK;
+  HIPCHK(hipStreamSynchronize(b->stream));
+  HIPCHK(hipMemcpy(out, b->base.ovf_any, 4, hipMemcpyDeviceToHost));
+  return BB_OK;
+}
+
+bb_status clear_overflow(bb_batch* b) {
+  if (!b->built || !b->base.ovf_any) return BB_OK;
+  HIPCHK(hipStreamSynchronize(b->stream));
+  const uint32_t W = (uint32_t)b->worlds.size();
+  for (uint32_t w = 0; w < W; ++w)
+    HIPCHK(hipMemsetAsync(b->base.stats + (size_t)w * ST_COUNT + ST_OVF_STICKY, 0, 4, b->stream));
+  HIPCHK(hipMemsetAsync(b->base.ovf_any, 0, 4, b->stream));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  return BB_OK;
+}
+
+// Waits for the batch's work; BB_ERR_OVERFLOW while a world's overflow
+// (pairs past its capacity, or a team that did not meet at a barrier) has
+// not been collected by bb_batch_overflow.
 bb_status sync_batch(bb_batch* b) {
   if (!b->stream) return BB_OK;
   HIPCHK(hipStreamSynchronize(b->stream));
-  return BB_OK;
+  uint32_t any = 0;
+  bb_status st = read_overflow_any(b, &any);
+  if (st != BB_OK) return st;
+  return any ? BB_ERR_OVERFLOW : BB_OK;
+}
+
+bb_status overflow_worlds(bb_batch* b, uint32_t* worlds, uint32_t cap, uint32_t* count) {
+  *count = 0;
+  if (!b->built || !b->stepped) return BB_OK;
+  HIPCHK(hipStreamSynchronize(b->stream));
+  const uint32_t W = (uint32_t)b->worlds.size();
+  std::vector<uint32_t> raw((size_t)W * ST_COUNT);
+  HIPCHK(hipMemcpy(raw.data(), b->base.stats, raw.size() * 4, hipMemcpyDeviceToHost));
+  uint32_t k = 0;
+  for (uint32_t w = 0; w < W; ++w)
+    if (raw[(size_t)w * ST_COUNT + ST_OVF_STICKY]) {
+      if (worlds && k < cap) worlds[k] = w;
+      ++k;
+    }
+  *count = k;
+  return clear_overflow(b);
 }
 
 bb_status read_world(bb_batch* b, uint32_t w, bb_body_state* out, uint32_t n, bb_order order) {
@@ -1725,6 +1812,11 @@ bb_status bb_batch_step(bb_batch* b, uint32_t nsteps, void* stream, int use_grap
 
 bb_status bb_batch_sync(bb_batch* b) { return b ? sync_batch(b) : BB_ERR_INVALID; }
 
+bb_status bb_batch_overflow(bb_batch* b, uint32_t* worlds, uint32_t cap, uint32_t* count) {
+  if (!b || !count || (cap && !worlds)) return BB_ERR_INVALID;
+  return overflow_worlds(b, worlds, cap, count);
+}
+
 bb_status bb_batch_read_world(bb_batch* b, uint32_t w, bb_body_state* out, uint32_t n, bb_order order) {
   return b ? read_world(b, w, out, n, order) : BB_ERR_INVALID;
 }
@@ -1777,6 +1869,7 @@ uint32_t bb_batch_get_param(const bb_batch* b, bb_param p) {
   if (p == BB_PARAM_LANE_BODIES) return b->lane_bodies;
   if (p == BB_PARAM_LANE_BLOCK) return b->lane_block;
   if (p == BB_PARAM_LANE_LDS_ROWS) return b->lane_lds;
+  if (p == BB_PARAM_TEAM_SPIN) return b->team_spin;
   return 0;
 }
 
@@ -1878,6 +1971,12 @@ bb_status bb_world_skip_warm_start_once(bb_world* w) {
 // from included, is still there.  The replay sees the same world table and
 // one-shot settings as the first run, so the result equals a step with
 // enough capacity from the start, bit for bit.
+//
+// A world stepped by a workgroup team whose workgroups did not all meet at a
+// barrier (counter bit 1: the team left the step unfinished, see k_team) is
+// stepped again the same way from the same state, by one workgroup: teams
+// are switched off for this world from then on.  The two paths give
+// bit-identical results.
 bb_status bb_world_step(bb_world* w, uint32_t nsteps, void* stream) {
   if (!w) return BB_ERR_INVALID;
   constexpr int kPairGrowths = 16;
@@ -1890,26 +1989,48 @@ bb_status bb_world_step(bb_world* w, uint32_t nsteps, void* stream) {
     const std::vector<WorldInfo> infos0 = b->infos;
     const bool skip0 = hw.skip_ws_once, drop0 = hw.drop_prev;
     st = bb_batch_step(b, 1, stream, 0);
-    for (int g = 0; st == BB_OK; ++g) {
+    int g = 0, team_retry = 0;
+    while (st == BB_OK) {
       uint32_t ovf = 0;
       HIPCHK(hipStreamSynchronize(b->stream));
       HIPCHK(hipMemcpy(&ovf, b->base.stats + ST_OVERFLOW, 4, hipMemcpyDeviceToHost));
-      if (!(ovf & 1u)) break;
-      if (g == kPairGrowths) return BB_ERR_CAPACITY;
+      if (!(ovf & 3u)) break;
+      // undo: the step wrote only the other parity
       b->latest ^= 1;
       b->step_index -= 1;
       b->infos = infos0;
       hw.skip_ws_once = skip0;
       hw.drop_prev = drop0;
-      st = grow_pairs(b);  // also uploads the world table
+      if (ovf & 2u) {
+        if (team_retry++) return BB_ERR_DEVICE;
+        b->team_bodies = 0;
+        for (auto& wi : b->infos) wi.mode &= ~kModeTeam;
+        st = upload_teams(b);  // also uploads the world table
+        if (b->graph) {
+          (void)hipGraphExecDestroy(b->graph);
+          b->graph = nullptr;
+        }
+      } else {
+        if (g++ == kPairGrowths) return BB_ERR_CAPACITY;
+        st = grow_pairs(b);  // also uploads the world table
+      }
       if (st == BB_OK) st = bb_batch_step(b, 1, stream, 0);
     }
     if (st != BB_OK) return st;
+    // the step that stands has no overflow; the marks of the replayed runs go
+    if (g || team_retry) {
+      st = clear_overflow(b);
+      if (st != BB_OK) return st;
+    }
   }
   return BB_OK;
 }
 
 bb_status bb_world_sync(bb_world* w) { return w ? sync_batch(w->b) : BB_ERR_INVALID; }
+
+bb_status bb_world_set_param(bb_world* w, bb_param p, uint32_t value) {
+  return w ? set_param(w->b, p, value) : BB_ERR_INVALID;
+}
 
 bb_status bb_world_read_bodies(bb_world* w, bb_body_state* out, uint32_t n, bb_order order) {
   return w ? read_world(w->b, 0, out, n, order) : BB_ERR_INVALID;

@@ -26,7 +26,10 @@ static constexpr uint32_t kMAX = 0xFFFFFFFFu;
 enum StatSlot : uint32_t {
   ST_N = 0, ST_A, ST_P, ST_M, ST_NODES, ST_C, ST_ISLANDS, ST_CISLANDS, ST_SLEEPING,
   ST_GC_COUNT, ST_GC_VIOL, ST_GC_OVF, ST_AV_COUNT, ST_AV_VIOL, ST_AV_STICK, ST_FRESH,
-  ST_PEN, ST_IDSUM, ST_MAXV, ST_OVERFLOW, ST_USED_COLORS, ST_COUNT
+  ST_PEN, ST_IDSUM, ST_MAXV, ST_OVERFLOW, ST_USED_COLORS,
+  // OR of ST_OVERFLOW over every step since the host last read and cleared it
+  // (the step's own reset leaves it alone)
+  ST_OVF_STICKY, ST_COUNT
 };
 
 struct WorldInfo {  // 64 bytes
@@ -56,7 +59,9 @@ struct VoxShape {  // 48 bytes
 
 // Control block of a workgroup team (k_team): barrier arrival counter and
 // generation, an error flag (a barrier wait that ran out of spins), a
-// team-wide flag for the radix sort, and per-workgroup scan partials.
+// team-wide flag for the radix sort, the count of workgroups that have left
+// the step (the last one resets count, error and exited for the next launch),
+// and per-workgroup scan partials.
 static constexpr uint32_t kMaxTeamBlocks = 2048;
 // Threads per workgroup of k_team (independent of the per-world kernels').
 static constexpr uint32_t kTeamBlock = 512;
@@ -65,6 +70,7 @@ struct TeamCtl {
   uint32_t gen;
   uint32_t error;
   uint32_t diff;
+  uint32_t exited;
   uint32_t part[kMaxTeamBlocks];
 };
 // Team j steps world world[j] with workgroups [first[j], first[j+1]) of the
@@ -199,6 +205,10 @@ struct Dev {
   uint32_t* team_hist;
   char* team_shared;
   uint32_t team_grid;     // workgroups of the k_team launch
+  uint32_t team_spin;     // spins before a team barrier wait gives up (0: default)
+  // set (bit 0) by any world whose step flags an overflow; the host reads
+  // and clears it
+  uint32_t* ovf_any;
   // [kMaxTeams * kTeamMarks] wall-clock marks of each team's stages, written
   // when non-null (profiling launches only)
   unsigned long long* team_clock;

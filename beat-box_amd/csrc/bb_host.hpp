@@ -46,6 +46,7 @@ struct HostWorld {
   std::vector<bb_body_desc> bodies;  // id order
   bool has_voxel = false;            // a body is a voxel shape (kModeVoxel)
   uint32_t pair_scale = 1;           // pair capacity multiplier, doubled on overflow
+  bool no_team = false;              // stepped by one workgroup whatever its size
   bool skip_ws_once = false;
   bool drop_prev = false;
   uint32_t body_off = 0, act_off = 0, pair_off = 0, pair_cap = 0, n = 0, a = 0;
@@ -135,6 +136,7 @@ struct bb_batch {
   uint32_t lane_bodies = 512;
   uint32_t lane_block = 512;
   uint32_t lane_lds = 1024;  // BB_PARAM_LANE_LDS_ROWS: LDS rows of k_pgs in that lane
+  uint32_t team_spin = 0;    // BB_PARAM_TEAM_SPIN: spins of a team barrier wait (0: default)
   uint32_t n_wide = 0;  // worlds in that lane: the first n_wide of the order table
   hipStream_t stream2 = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
@@ -193,6 +195,11 @@ uint32_t pgs_lds_rows(const bb_batch* b);
 // Doubles the pair capacity of a built single-world batch, keeping every
 // device array's contents (both parities, the manifolds included).
 bb_status grow_pairs(bb_batch* b);
+// Reads the batch's overflow word (set by any world whose step flagged an
+// overflow since it was last cleared) after the batch stream has finished.
+bb_status read_overflow_any(bb_batch* b, uint32_t* out);
+// Clears the sticky overflow words of every world and the batch's word.
+bb_status clear_overflow(bb_batch* b);
 // Counters of one world (raw device slots, ST_COUNT of them) as the C-ABI
 // struct.
 void stats_from_raw(const uint32_t* raw, uint64_t step_index, bb_step_stats* out);

@@ -111,7 +111,17 @@ struct Request {
   uint32_t active;
   uint64_t bytes;
   double submit_ms;
+  // a request runs again from its first step when a step of it overflowed:
+  // with twice the pair capacity (pairs past it were dropped), or by one
+  // workgroup instead of a team (the team left a step unfinished)
+  uint32_t pair_scale = 1;
+  bool no_team = false;
+  uint32_t reruns = 0;
 };
+
+// Pair-capacity doublings and team fallbacks a request may take before it
+// is reported as failed.
+constexpr uint32_t kMaxReruns = 17;
 
 struct Slot {
   bool used = false;
@@ -143,6 +153,7 @@ struct StepRecord {
 struct Result {
   std::vector<bb_body_state> bodies;
   bb_step_stats stats;
+  bool failed = false;
 };
 
 }  // namespace
@@ -158,6 +169,9 @@ struct bb_server {
   std::deque<Request> queue;
   std::unordered_map<uint64_t, Result> results;
   std::unordered_map<uint64_t, uint32_t> running;  // ticket -> slot
+  // requests on the device or in a read-back, kept to run again on overflow
+  std::unordered_map<uint64_t, Request> origin;
+  uint64_t reruns = 0;
   std::deque<StepRecord> inflight;
   std::vector<hipEvent_t> spare_events;
   Chunk chunks[kChunks];
@@ -187,8 +201,8 @@ hipEvent_t take_event(bb_server* s) {
   return e;
 }
 
-uint64_t price(const bb_server* s, uint32_t n, uint32_t a) {
-  return s->cW + s->cB * n + s->cA * a + s->cP * pair_capacity(n);
+uint64_t price(const bb_server* s, uint32_t n, uint32_t a, uint32_t pair_scale = 1) {
+  return s->cW + s->cB * n + s->cA * a + s->cP * (uint64_t)pair_capacity(n) * pair_scale;
 }
 
 void to_states(const Readback& r, Result& res) {
@@ -227,7 +241,34 @@ bb_status finish_oldest(bb_server* s) {
   float total = 0.0f;
   HIPCHK(hipEventElapsedTime(&total, s->first, rec.done));
   s->device_ms = total;
-  for (const Readback& r : rec.readbacks) to_states(r, s->results[r.ticket]);
+  for (const Readback& r : rec.readbacks) {
+    auto oit = s->origin.find(r.ticket);
+    const uint32_t ovf = r.stats[ST_OVF_STICKY];
+    if (ovf && oit != s->origin.end() && oit->second.reruns < kMaxReruns) {
+      // a step of this world dropped pairs or was left unfinished by its
+      // team: run the request again from its first step, ahead of the queue,
+      // if it fits the budget and the pair arena that way
+      const Request& o = oit->second;
+      const uint32_t scale = (ovf & 1u) ? o.pair_scale * 2u : o.pair_scale;
+      const uint32_t nb = (uint32_t)o.bodies.size();
+      const uint64_t bytes = price(s, nb, o.active, scale);
+      if (bytes <= s->desc.hbm_budget_bytes && (uint64_t)pair_capacity(nb) * scale <= s->NP) {
+        Request q = std::move(oit->second);
+        s->origin.erase(oit);
+        q.pair_scale = scale;
+        if (ovf & 2u) q.no_team = true;
+        ++q.reruns;
+        ++s->reruns;
+        q.bytes = bytes;
+        s->queue.push_front(std::move(q));
+        continue;
+      }
+    }
+    Result& res = s->results[r.ticket];
+    to_states(r, res);
+    res.failed = ovf != 0;
+    if (oit != s->origin.end()) s->origin.erase(oit);
+  }
   if (s->have_prev_done) s->spare_events.push_back(s->last_done);
   s->last_done = rec.done;
   s->have_prev_done = true;
@@ -277,7 +318,8 @@ bb_status server_step(bb_server* s) {
       s->rb.release(bo, n);
       break;
     }
-    if (!s->rp.alloc(pair_capacity(n), &po)) {
+    const uint32_t pcap = pair_capacity(n) * q.pair_scale;
+    if ((uint64_t)pcap > s->NP || !s->rp.alloc(pcap, &po)) {
       s->rb.release(bo, n);
       s->ra.release(ao, q.active);
       break;
@@ -296,11 +338,13 @@ bb_status server_step(bb_server* s) {
     sl.act_off = ao;
     sl.a = q.active;
     sl.pair_off = po;
-    sl.pair_cap = pair_capacity(n);
+    sl.pair_cap = pcap;
     HostWorld& hw = b->worlds[w];
     hw = HostWorld{};
     hw.desc = q.desc;
-    hw.bodies = std::move(q.bodies);
+    hw.bodies = q.bodies;
+    hw.pair_scale = q.pair_scale;
+    hw.no_team = q.no_team;
     hw.n = n;
     hw.a = q.active;
     hw.body_off = bo;
@@ -310,10 +354,14 @@ bb_status server_step(bb_server* s) {
     s->bytes_in_use += q.bytes;
     s->resident_bodies += n;
     s->running[q.ticket] = w;
-    s->queue_samples.push_back(t_now - q.submit_ms);
-    ++s->admitted_n;
+    if (!q.reruns) {
+      s->queue_samples.push_back(t_now - q.submit_ms);
+      ++s->admitted_n;
+    }
+    const uint64_t tk = q.ticket;
     stage_bytes += kImageBodyBytes * std::max(n, 1u) + kImageActiveBytes * std::max(q.active, 1u) + 64;
     admitted.push_back(w);
+    s->origin[tk] = std::move(q);
     s->queue.pop_front();
   }
   // ---- read-back sizes of the worlds that finish with this step
@@ -571,19 +619,23 @@ bb_status bb_server_step(bb_server* s, uint32_t nsteps) {
 
 bb_status bb_server_drain(bb_server* s) {
   if (!s) return BB_ERR_INVALID;
-  while (!idle(s)) {
-    if (s->running.empty() && !s->queue.empty() && s->free_slots.size() == s->W) {
-      // nothing resident yet the head of the queue does not fit: cannot happen
-      // for a request accepted by submit, but never spin on it
-      const Request& q = s->queue.front();
-      if (q.bytes > s->desc.hbm_budget_bytes) return BB_ERR_BUDGET;
+  // a read-back that completes may put its request back in the queue (a
+  // rerun after an overflow), so drain until nothing is left anywhere
+  while (!idle(s) || !s->inflight.empty()) {
+    while (!idle(s)) {
+      if (s->running.empty() && !s->queue.empty() && s->free_slots.size() == s->W) {
+        // nothing resident yet the head of the queue does not fit: cannot
+        // happen for a request accepted by submit, but never spin on it
+        const Request& q = s->queue.front();
+        if (q.bytes > s->desc.hbm_budget_bytes) return BB_ERR_BUDGET;
+      }
+      bb_status st = server_step(s);
+      if (st != BB_OK) return st;
     }
-    bb_status st = server_step(s);
-    if (st != BB_OK) return st;
-  }
-  while (!s->inflight.empty()) {
-    bb_status st = finish_oldest(s);
-    if (st != BB_OK) return st;
+    while (!s->inflight.empty()) {
+      bb_status st = finish_oldest(s);
+      if (st != BB_OK) return st;
+    }
   }
   return BB_OK;
 }
@@ -619,7 +671,7 @@ bb_status bb_server_poll(bb_server* s, uint64_t ticket, uint32_t* state, bb_body
   if (out && n < it->second.bodies.size()) return BB_ERR_INVALID;
   if (out) std::memcpy(out, it->second.bodies.data(), it->second.bodies.size() * sizeof(bb_body_state));
   if (stats) *stats = it->second.stats;
-  *state = BB_REQ_DONE;
+  *state = it->second.failed ? BB_REQ_FAILED : BB_REQ_DONE;
   s->results.erase(it);
   return BB_OK;
 }
@@ -647,6 +699,7 @@ bb_status bb_server_report_get(bb_server* s, bb_server_report* out) {
   for (double v : s->queue_samples) qs.emplace_back((float)v, 1);
   out->queue_ms_p50 = percentile(qs, 0.50);
   out->queue_ms_p99 = percentile(qs, 0.99);
+  out->reruns = s->reruns;
   return BB_OK;
 }
 

@@ -257,9 +257,12 @@ struct BlockTeam {
   }
 };
 
-// Spins before a team barrier wait gives up (about a second): a wait that
-// runs out sets TeamCtl::error, and from then on the team's barriers no
-// longer wait, so every workgroup reaches the end of the kernel.
+// Spins before a team barrier wait gives up (about a second).  A wait that
+// runs out sets TeamCtl::error.  Every workgroup that then reaches a team
+// barrier, or is still waiting at one, leaves the step at that barrier
+// (GroupTeam::abandon): no stage code runs on data the team did not finish,
+// so no index computed from it is ever used.  The world's overflow word gets
+// bit 1, and the host replays the step.
 static constexpr uint32_t kTeamSpinLimit = 1u << 24;
 
 template <int BLOCK>
@@ -269,6 +272,10 @@ struct GroupTeam {
   TeamCtl* ctl;
   uint32_t* hist;  // [nblocks * 256] this team's radix histograms / offsets
   uint32_t blk, nblocks;
+  uint32_t* dead;  // LDS word: thread 0's verdict of the last barrier
+  uint32_t* stats; // the world's counters (ST_COUNT words)
+  uint32_t* ovf_any;
+  uint32_t spin;   // spins before a wait gives up
   // AVBD primal: one lane per body, a barrier after every cascade bucket
   static constexpr uint32_t kAvGroup = 1;
   static constexpr bool kSkipEmpty = false;
@@ -284,12 +291,38 @@ struct GroupTeam {
   BB_DI bool leaves_local(uint32_t first, uint32_t last) const { return first / BLOCK == last / BLOCK; }
   // the workgroup that finishes the nodes above the runs
   BB_DI bool lead_group() const { return blk == 0; }
+  BB_DI uint32_t err() const { return __hip_atomic_load(&ctl->error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+  // Thread 0: this workgroup has left the step.  The last workgroup of the
+  // team to leave resets the control block for the next launch (every other
+  // member is past its last use of it).
+  BB_DI void leave() const {
+    __threadfence();
+    if (atomicAdd(&ctl->exited, 1u) == nblocks - 1u) {
+      __hip_atomic_store(&ctl->count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&ctl->error, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&ctl->exited, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __threadfence();
+    }
+  }
+  // The team did not meet at a barrier: flag the world (overflow bit 1, also
+  // in its sticky word and the batch's overflow word) and end every wave of
+  // this workgroup here.  Called by all threads, workgroup-uniformly.
+  BB_DI void abandon() const {
+    if (threadIdx.x == 0) {
+      atomicOr(&stats[ST_OVERFLOW], 2u);
+      atomicOr(&stats[ST_OVF_STICKY], 2u);
+      atomicOr(ovf_any, 1u);
+      leave();
+    }
+    __builtin_amdgcn_endpgm();
+  }
   BB_DI void sync() const {
     __syncthreads();
     if (threadIdx.x == 0) {
-      const uint32_t g = __hip_atomic_load(&ctl->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __threadfence();
-      if (__hip_atomic_load(&ctl->error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u) {
+      uint32_t bad = err();
+      if (bad == 0u) {
+        const uint32_t g = __hip_atomic_load(&ctl->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __threadfence();
         if (atomicAdd(&ctl->count, 1u) == nblocks - 1u) {
           __hip_atomic_store(&ctl->count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
           __threadfence();
@@ -297,17 +330,21 @@ struct GroupTeam {
         } else {
           uint32_t spins = 0;
           while (__hip_atomic_load(&ctl->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == g) {
-            if (++spins > kTeamSpinLimit) {
+            if (err() != 0u) break;
+            if (++spins > spin) {
               atomicOr(&ctl->error, 1u);
               break;
             }
             __builtin_amdgcn_s_sleep(1);
           }
         }
+        __threadfence();
+        bad = err();
       }
-      __threadfence();
+      *dead = bad;
     }
     __syncthreads();
+    if (*dead) abandon();
   }
   // Exclusive scan over the whole team (rank order); two team barriers.
   BB_DI uint32_t scan(uint32_t v, uint32_t& total) const {
@@ -482,7 +519,7 @@ BB_DI void s_build(const Dev& d, uint32_t w, const T& t, BuildShared& L) {
   const WorldInfo W = d.worlds[w];
   const uint32_t N = W.n, off = W.body_off;
   // per-step resets (an empty world slot keeps zero counters)
-  if (tid < ST_COUNT) d.stats[w * ST_COUNT + tid] = 0;
+  if (tid < ST_COUNT && tid != ST_OVF_STICKY) d.stats[w * ST_COUNT + tid] = 0;
   if (N == 0) return;
   for (uint32_t k = tid; k < W.a; k += NT) d.link[W.act_off + k] = k;
   for (uint32_t i = tid; i < N; i += NT) d.deg[off + i] = 0;
@@ -607,7 +644,8 @@ BB_DI void s_build(const Dev& d, uint32_t w, const T& t, BuildShared& L) {
       while (p + 1 < N) {  // p is an internal node index in [0, N-2]
         const uint64_t rg = kb[p];
         if (!t.leaves_local((uint32_t)rg, (uint32_t)(rg >> 32))) {
-          ib[atomicAdd(&L.nroots, 1u)] = p;
+          const uint32_t k = atomicAdd(&L.nroots, 1u);
+          if (k < N) ib[k] = p;  // at most one node per run, far fewer than N
           break;
         }
         __threadfence_block();
@@ -619,7 +657,7 @@ BB_DI void s_build(const Dev& d, uint32_t w, const T& t, BuildShared& L) {
     }
     t.sync();
     if (t.lead_group()) {
-      const uint32_t nr = L.nroots;
+      const uint32_t nr = L.nroots < N ? L.nroots : N;
       for (uint32_t k = threadIdx.x; k < nr; k += blockDim.x) {
         uint32_t p = ib[k];
         while (p + 1 < N) {
@@ -3595,7 +3633,8 @@ BB_DI void s_avbd(const Dev& d, uint32_t w, const T& t, AvbdShared& L) {
     if (!(fl & BB_BODY_DYNAMIC) || (fl & BB_BODY_SLEEPING) || c >= kAvbdColors) continue;
     uint32_t layer = d.depth[g] < kAvbdLayers - 1 ? d.depth[g] : kAvbdLayers - 1;
     const uint32_t k = c * kAvbdLayers + layer;
-    bb[lc ? lc[AV_NB + k] + atomicAdd(&lc[k], 1u) : atomicAdd(&L.bcount[k], 1u)] = i;
+    const uint32_t o = lc ? lc[AV_NB + k] + atomicAdd(&lc[k], 1u) : atomicAdd(&L.bcount[k], 1u);
+    if (o < N) bb[o] = i;  // the buckets hold at most the world's N rows
   }
   t.sync();
   const uint32_t used = L.used;
@@ -3686,6 +3725,13 @@ BB_DI void s_finish(const Dev& d, uint32_t w, const T& t) {
     d.stats[w * ST_COUNT + ST_N] = W.n;
     d.stats[w * ST_COUNT + ST_A] = W.a;
     d.stats[w * ST_COUNT + ST_NODES] = 2 * d.stats[w * ST_COUNT + ST_M];
+    // overflow is kept until the host reads it: the world's sticky word, and
+    // one word for the whole batch so that a check costs one 4-byte read
+    const uint32_t ovf = d.stats[w * ST_COUNT + ST_OVERFLOW];
+    if (ovf) {
+      atomicOr(&d.stats[w * ST_COUNT + ST_OVF_STICKY], ovf);
+      atomicOr(d.ovf_any, 1u);
+    }
   }
 }
 
@@ -3718,15 +3764,18 @@ template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_team(Dev d) {
   __shared__ ScanLds<BLOCK> S;
   __shared__ SortLds<BLOCK> Q;
+  __shared__ uint32_t dead;
   const TeamTable& T = *d.teams;
   const uint32_t nt = T.count < kMaxTeams ? T.count : kMaxTeams;
   if (nt == 0 || blockIdx.x >= T.first[nt]) return;  // workgroup-uniform
   uint32_t j = 0;
   while (j + 1 < nt && blockIdx.x >= T.first[j + 1]) ++j;
   const uint32_t b0 = T.first[j], nb = T.first[j + 1] - b0;
-  const GroupTeam<BLOCK> t{&S, &Q, d.team_ctl + j, d.team_hist + (size_t)b0 * 256, blockIdx.x - b0, nb};
-  TeamShared& G = *reinterpret_cast<TeamShared*>(d.team_shared + (size_t)j * kTeamSharedBytes);
   const uint32_t w = T.world[j];
+  const GroupTeam<BLOCK> t{&S, &Q, d.team_ctl + j, d.team_hist + (size_t)b0 * 256, blockIdx.x - b0, nb,
+                           &dead, d.stats + (size_t)w * ST_COUNT, d.ovf_any,
+                           d.team_spin ? d.team_spin : kTeamSpinLimit};
+  TeamShared& G = *reinterpret_cast<TeamShared*>(d.team_shared + (size_t)j * kTeamSharedBytes);
   // stage boundaries: every workgroup has passed the barrier before rank 0
   // stamps the clock
   unsigned long long* clk = d.team_clock ? d.team_clock + (size_t)j * kTeamMarks : nullptr;
@@ -3761,9 +3810,9 @@ __global__ __launch_bounds__(BLOCK) void k_team(Dev d) {
   s_finish(d, w, t);
   t.sync();
   mark(7);
-  // a barrier that ran out of spins: flag the world (overflow bit 1)
-  if (t.rank() == 0 && __hip_atomic_load(&t.ctl->error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u)
-    atomicOr(&d.stats[w * ST_COUNT + ST_OVERFLOW], 2u);
+  // every barrier of the step was met (a workgroup that saw one fail has
+  // left through GroupTeam::abandon)
+  if (threadIdx.x == 0) t.leave();
 }
 
 // Render hand-off (entry_update_acceleration_structures, src/shaders/

@@ -301,6 +301,17 @@ def dense_spheres(seed: int, side: int = 5, radius: float = 0.5, spacing: float 
     return bodies
 
 
+def expanding_spheres(seed: int, side: int = 5, rate: float = 100.0):
+    """Overflow in the first step only: the dense_spheres block moving
+    apart with velocity rate * position (a homologous expansion).  The first
+    broad phase sees every sphere's box overlap its neighbours' (several
+    times the 8n+64 pair slots); a step later the spacing has grown past the
+    diameter and the pairs are gone.  Run with gravity 0."""
+    bodies = dense_spheres(seed, side=side)
+    bodies["velocity"] = (bodies["position"].astype(np.float64) * rate).astype(np.float32)
+    return bodies
+
+
 def serving_pile(seed: int, n: int):
     """One serving request: a random pile of n bodies, 50/50 boxes and
     spheres with sizes (edge / diameter) U(0.2, 1.0) m, centres in a column

@@ -307,10 +307,14 @@ bb_status bb_world_skip_warm_start_once(bb_world* w);
  * step), and a step that overflowed is run again, from the state it started
  * from, with the world's pair capacity doubled (at most 16 times, else
  * BB_ERR_CAPACITY).  The result equals a step with enough capacity from the
- * start, bit for bit.  Returns when the steps are done. */
+ * start, bit for bit.  A world stepped by a workgroup team whose workgroups
+ * did not all meet at a barrier (overflow bit 1) is stepped again from the
+ * same state by one workgroup, with the same result, and stays on that path
+ * (BB_ERR_DEVICE if that fails too).  Returns when the steps are done. */
 bb_status bb_world_step(bb_world* w, uint32_t nsteps, void* stream);
 /* device.wait_idle() after a step (src/renderer_manager.cpp:254-257). */
 bb_status bb_world_sync(bb_world* w);
+
 
 /* Body readback.  BY_ID goes through the id -> row map (rigid_body_entries,
  * src/shaders/simulation/RB_sim.slang:374-381); BY_ROW returns Morton rows.
@@ -382,12 +386,23 @@ bb_status bb_batch_add_voxel_shape(bb_batch* b, const bb_voxel_shape_desc* desc,
                                    bb_voxel_shape_info* out);
 /* Steps every world nsteps times.  With use_graph != 0 the step sequence is
  * captured once as a hipGraph and replayed.  Asynchronous: a world whose
- * pairs overflow its capacity keeps the pairs that fit and reports the
- * overflow in its counters (bb_batch_world_stats); the replay-and-grow of
- * bb_world_step is for single worlds. */
+ * pairs overflow its capacity keeps the pairs that fit (the reference's early
+ * return on a full buffer, src/shaders/simulation/LBVH.slang:248) and flags
+ * the step: bit 0 of its `overflow` counter for that step, and a sticky mark
+ * that stays until bb_batch_overflow collects it, whichever later steps do.
+ * Bit 1 means a workgroup team did not meet at a barrier and left the step
+ * unfinished.  The replay-and-grow of bb_world_step is for single worlds. */
 bb_status bb_batch_step(bb_batch* b, uint32_t nsteps, void* stream,
                         int use_graph);
+/* Waits for the batch's steps.  BB_ERR_OVERFLOW while any world carries an
+ * uncollected overflow mark (the results are there, but such a world dropped
+ * pairs or left a step unfinished). */
 bb_status bb_batch_sync(bb_batch* b);
+/* Collects the overflow marks: the indices of the worlds whose steps flagged
+ * an overflow since the last call (at most cap of them written to worlds,
+ * *count receives how many there are), then clears every mark. */
+bb_status bb_batch_overflow(bb_batch* b, uint32_t* worlds, uint32_t cap,
+                            uint32_t* count);
 bb_status bb_batch_read_world(bb_batch* b, uint32_t world, bb_body_state* out,
                               uint32_t n, bb_order order);
 bb_status bb_batch_world_stats(bb_batch* b, uint32_t world,
@@ -461,12 +476,20 @@ typedef enum bb_param {
                                 the rest; 0 = one lane.  Default 512        */
   BB_PARAM_LANE_BLOCK = 4,   /* threads per workgroup in that lane: 256, 512
                                 or 1024.  Default 512                       */
-  BB_PARAM_LANE_LDS_ROWS = 5 /* BB_PARAM_PGS_LDS_ROWS of that lane: 0..2048.
+  BB_PARAM_LANE_LDS_ROWS = 5, /* BB_PARAM_PGS_LDS_ROWS of that lane: 0..2048.
                                 Default 1,024                               */
+  BB_PARAM_TEAM_SPIN = 6     /* spins of a team barrier wait before the team
+                                gives the step up (overflow bit 1); 0 = the
+                                default, about a second.  Small values force
+                                that failure path (tests): bb_world_step then
+                                replays the step by one workgroup, a batch
+                                flags the world                             */
 } bb_param;
 bb_status bb_batch_set_param(bb_batch* b, bb_param p, uint32_t value);
 /* Effective value (BB_PARAM_BLOCK resolves 0 to the automatic choice). */
 uint32_t bb_batch_get_param(const bb_batch* b, bb_param p);
+/* bb_batch_set_param for a single world (no reference counterpart). */
+bb_status bb_world_set_param(bb_world* w, bb_param p, uint32_t value);
 /* Autotuner: times `reps` replays of the next step for every candidate
  * (block size x LDS rows), each replay restarting from the same state, and
  * keeps the fastest.  The simulation state afterwards is exactly the state
@@ -502,7 +525,11 @@ typedef enum bb_request_state {
   BB_REQ_QUEUED = 0,
   BB_REQ_RUNNING = 1,
   BB_REQ_DONE = 2,
-  BB_REQ_UNKNOWN = 3
+  BB_REQ_UNKNOWN = 3,
+  /* done, but a step overflowed even after the request was run again from
+   * its first step 17 times (pair capacity doubled, or one workgroup in place
+   * of a team): the bodies are the last run's and may have dropped pairs   */
+  BB_REQ_FAILED = 4
 } bb_request_state;
 
 typedef struct bb_server_report {
@@ -515,6 +542,8 @@ typedef struct bb_server_report {
                                         step s - completion of step s-1 (or the
                                         world's admission for s = 1)          */
   double queue_ms_p50, queue_ms_p99; /* submit -> admission, host clock       */
+  uint64_t reruns;                   /* requests run again from their first
+                                        step after an overflow             */
 } bb_server_report;
 
 bb_status bb_server_create(const bb_server_desc* desc, bb_server** out);
@@ -531,7 +560,11 @@ bb_status bb_server_step(bb_server* s, uint32_t nsteps);
 bb_status bb_server_drain(bb_server* s);
 /* State of a request; when done, its final bodies (by id, n >= body count)
  * and last-step counters are copied to out / stats (either may be NULL) and
- * the result is released.  *state receives a bb_request_state. */
+ * the result is released.  *state receives a bb_request_state.  A request
+ * whose world overflowed in any step (pairs past its capacity, or a team
+ * that left a step unfinished) is run again from its first step with twice
+ * the pair capacity, or by one workgroup, before it is reported: its result
+ * equals the world stepped alone by bb_world_step, which grows the same way. */
 bb_status bb_server_poll(bb_server* s, uint64_t ticket, uint32_t* state, bb_body_state* out, uint32_t n,
                          bb_step_stats* stats);
 bb_status bb_server_report_get(bb_server* s, bb_server_report* out);

@@ -72,11 +72,7 @@ static void to_sched(Fiber* f) {
 #endif
 }
 
-static void fiber_main() {
-#if EMU_ASAN
-  __sanitizer_finish_switch_fiber(nullptr, &t_blk->sched_bottom, &t_blk->sched_size);
-#endif
-  (*t_blk->fn)();
+[[noreturn]] static void finish_fiber() {
   Fiber* f = t_cur;
   f->state = DONE;
 #if EMU_ASAN
@@ -85,6 +81,18 @@ static void fiber_main() {
   emu_swap(&f->sp, t_blk->sched_sp);
   std::abort();
 }
+
+static void fiber_main() {
+#if EMU_ASAN
+  __sanitizer_finish_switch_fiber(nullptr, &t_blk->sched_bottom, &t_blk->sched_size);
+#endif
+  (*t_blk->fn)();
+  finish_fiber();
+}
+
+// The kernel's frames on this fiber's stack are abandoned; the stack is
+// unpoisoned again when the next workgroup takes it.
+void end_program() { finish_fiber(); }
 
 void barrier() {
   Fiber* f = t_cur;
@@ -255,6 +263,22 @@ static void run_block(uint32_t bx, dim3 grid, dim3 block, size_t shmem, const st
 void launch(dim3 grid, dim3 block, size_t shmem, std::function<void()> fn) {
   const uint32_t G = grid.x;
   if (G == 0) return;
+  const char* mb = std::getenv("EMU_MAX_BLOCKS");
+  const uint32_t max_blocks = mb ? (uint32_t)std::atoi(mb) : 0u;
+  if (max_blocks) {  // at most K workgroups resident at a time
+    std::atomic<uint32_t> next{0};
+    std::vector<std::thread> th;
+    for (uint32_t k = 0; k < max_blocks && k < G; ++k)
+      th.emplace_back([&] {
+        for (;;) {
+          const uint32_t b = next.fetch_add(1);
+          if (b >= G) break;
+          run_block(b, grid, block, shmem, fn);
+        }
+      });
+    for (auto& t : th) t.join();
+    return;
+  }
   if (G <= 256) {
     std::vector<std::thread> th;
     th.reserve(G);

@@ -2,7 +2,9 @@
 // device code (beat-box_amd/csrc/bb_step.hip, unchanged) can be compiled by
 // g++ with AddressSanitizer and its indexing checked on the host.  Each
 // workgroup of a launch runs on its own OS thread (so a team of workgroups
-// really runs concurrently and its global barrier works); each work-item is a
+// really runs concurrently and its global barrier works; EMU_MAX_BLOCKS=K
+// runs at most K workgroups at a time, as a GPU that cannot hold a whole
+// team resident would); each work-item is a
 // fiber of that thread; __syncthreads, wave operations (shuffles, ballots)
 // and s_sleep switch fibers.  Wave operations are rendezvous of the lanes of
 // a width-aligned segment.  Atomics are sequentially consistent, so the
@@ -51,6 +53,7 @@ const dim3& tid_of(Fiber*);
 extern thread_local dim3 t_block, t_grid, t_bdim;
 void barrier();
 void yield();
+[[noreturn]] void end_program();
 uint64_t wave_op(int kind, uint64_t v, int arg, int width);
 void* dyn_lds();
 void launch(dim3 grid, dim3 block, size_t shmem, std::function<void()> fn);
@@ -70,6 +73,8 @@ static inline unsigned long long wall_clock64() {
 static inline void __threadfence() { __atomic_thread_fence(__ATOMIC_SEQ_CST); }
 static inline void __threadfence_block() { __atomic_thread_fence(__ATOMIC_SEQ_CST); }
 #define __builtin_amdgcn_s_sleep(x) emu::yield()
+// s_endpgm: the work-item's fiber ends here
+#define __builtin_amdgcn_endpgm() emu::end_program()
 
 enum { EMU_SHFL = 0, EMU_SHFL_UP = 1, EMU_SHFL_DOWN = 2, EMU_BALLOT = 3 };
 template <class T>

@@ -1,7 +1,10 @@
 // Emulation driver: steps the worlds of a scene file through the C-ABI.
-// usage: emu_main scene.bin steps team_bodies block graph out.bin [lane_bodies [single]]
+// usage: emu_main scene.bin steps team_bodies block graph out.bin [lane_bodies [single [team_spin]]]
 // With single = 1 the scene's one world runs through the single-world API
 // (bb_world_*: pair-capacity growth on overflow) instead of a batch.
+// team_spin sets BB_PARAM_TEAM_SPIN (spins before a team barrier wait gives
+// up).  The per-step line reports the step's overflow word; a batch's
+// overflow marks are collected after each step and listed.
 #include <cstdio>
 #include <cstdlib>
 #include <vector>
@@ -72,16 +75,32 @@ int main(int argc, char** argv) {
                     : bb_batch_add_world(b, &w.d, w.bodies.data(), n, &idx);
     if (st) { std::fprintf(stderr, "add_world %d\n", st); return 4; }
   }
+  const uint32_t spin = argc > 9 ? (uint32_t)std::strtoul(argv[9], nullptr, 0) : 0u;
   if (!single) {
     bb_batch_set_param(b, BB_PARAM_TEAM_BODIES, (uint32_t)std::strtoul(argv[3], nullptr, 0));
     bb_batch_set_param(b, BB_PARAM_BLOCK, (uint32_t)std::atoi(argv[4]));
     if (argc > 7) bb_batch_set_param(b, BB_PARAM_LANE_BODIES, (uint32_t)std::strtoul(argv[7], nullptr, 0));
+    if (spin) bb_batch_set_param(b, BB_PARAM_TEAM_SPIN, spin);
+  } else {
+    bb_world_set_param(sw, BB_PARAM_TEAM_BODIES, (uint32_t)std::strtoul(argv[3], nullptr, 0));
+    if (spin) bb_world_set_param(sw, BB_PARAM_TEAM_SPIN, spin);
   }
   const int graph = std::atoi(argv[5]);
   for (int s = 0; s < steps; ++s) {
     int st = single ? bb_world_step(sw, 1, nullptr) : bb_batch_step(b, 1, nullptr, graph);
     if (st) { std::fprintf(stderr, "step %d\n", st); return 5; }
-    if (!single) bb_batch_sync(b);
+    if (!single) {
+      st = bb_batch_sync(b);
+      if (st == BB_ERR_OVERFLOW) {
+        std::vector<uint32_t> ow(nw);
+        uint32_t cnt = 0;
+        if (bb_batch_overflow(b, ow.data(), nw, &cnt)) return 5;
+        for (uint32_t k = 0; k < cnt && k < nw; ++k) std::printf("step %d overflow world %u\n", s, ow[k]);
+      } else if (st) {
+        std::fprintf(stderr, "sync %d\n", st);
+        return 5;
+      }
+    }
     for (uint32_t w = 0; w < nw; ++w) {
       bb_step_stats ss{};
       if (single) bb_world_read_stats(sw, &ss);

@@ -138,3 +138,68 @@ def test_server_without_gpu_fails_loudly():
     with pytest.raises(beatbox.BBError) as e:
         beatbox.Server(budget_bytes=0, max_worlds=8)
     assert e.value.code == 1  # BB_ERR_INVALID
+
+
+def _blob(n=3, p=1, mutate=None):
+    """A checkpoint blob of n boxes and p manifold slots in the layout of
+    bb_world_snapshot (header, body descriptors, then the arrays), with
+    consistent indices unless `mutate` changes them."""
+    import struct
+    desc = beatbox.WorldDesc.make(**scenes.world_desc(iterations=8, morton_bits=63, quirks=0))
+    bodies = scenes.random_boxes(0xBB90, n - 1, extent=2.0, y0=0.55, y1=3.0)
+    assert len(bodies) == n
+    ids = np.arange(n, dtype=np.uint32)[::-1].copy()
+    entries = np.zeros(n, np.uint32)
+    entries[ids] = np.arange(n, dtype=np.uint32)
+    arrays = dict(id=ids, flags=np.zeros(n, np.uint32), sleep=np.zeros(n, np.uint32), entries=entries,
+                  pstart=np.zeros(n, np.uint32), islot=np.zeros(n, np.uint32),
+                  rows=np.zeros((5, n, 4), np.float32), pairs=np.array([[0, 1]] * p, np.uint32),
+                  geo=np.zeros((5, p, 4), np.float32), hdr=np.array([[0, 1, 5, 4]] * p, np.int32),
+                  contacts=np.zeros(p * 8 * (6 * 16 + 4), np.uint8), stats=np.zeros(22, np.uint32))
+    if mutate:
+        mutate(arrays)
+    head = b"BBSNAP02" + struct.pack("<4IQ", 2, n, p, 1, 0) + bytes(desc) + struct.pack("<2I", 0, 0)
+    body = np.ascontiguousarray(bodies, dtype=scenes.BODY_DESC).tobytes()
+    order = ("id", "flags", "sleep", "entries", "pstart", "islot", "rows", "pairs", "geo", "hdr", "contacts",
+             "stats")
+    return head + body + b"".join(arrays[k].tobytes() for k in order)
+
+
+def _restore_status(blob: bytes) -> int:
+    out = C.c_void_p()
+    buf = C.create_string_buffer(blob, len(blob))
+    rc = beatbox.lib().bb_world_restore(buf, len(blob), 1, C.byref(out))
+    if rc == 0:
+        beatbox.lib().bb_world_destroy(out)
+    return rc
+
+
+@pytest.mark.parametrize("what", ["ok", "id_range", "id_repeat", "entries", "pair_row", "hdr_row", "hdr_count"])
+def test_restore_refuses_blobs_with_bad_indices(what):
+    """bb_world_restore checks every index the next step would use to
+    address device memory before anything is uploaded: a consistent blob
+    gets as far as needing the device (BB_ERR_NO_DEVICE here, no GPU), a
+    blob with an id out of range or repeated, an id -> row map that
+    disagrees, a pair row or manifold row past the world, or more than 8
+    contacts in a slot is refused with BB_ERR_INVALID."""
+    def mutate(a):
+        if what == "id_range":
+            a["id"][1] = 7
+        elif what == "id_repeat":
+            a["id"][1] = a["id"][0]
+        elif what == "entries":
+            a["entries"][0] = 1
+        elif what == "pair_row":
+            a["pairs"][0, 1] = 3
+        elif what == "hdr_row":
+            a["hdr"][0, 0] = 9
+        elif what == "hdr_count":
+            a["hdr"][0, 3] = 9
+    blob = _blob(mutate=mutate)
+    layout_size = 72 + 3 * 100 + 3 * 104 + 1 * 88 + 16 + 8 * (6 * 16 + 4) + 22 * 4
+    assert len(blob) == layout_size
+    rc = _restore_status(blob)
+    if what == "ok":
+        assert rc in (0, 4)  # BB_OK with a GPU, BB_ERR_NO_DEVICE without
+    else:
+        assert rc == 1, what

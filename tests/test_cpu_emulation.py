@@ -51,13 +51,21 @@ def _scene(path, worlds, shapes=()):
             f.write(arr.tobytes())
 
 
-def _run(exe, scene, out, steps, team_bodies, block, graph=1, lane_bodies=0, single=0):
+def _run(exe, scene, out, steps, team_bodies, block, graph=1, lane_bodies=0, single=0, team_spin=0,
+         max_blocks=0):
+    env = dict(ENV, EMU_MAX_BLOCKS=str(max_blocks)) if max_blocks else ENV
     r = subprocess.run([exe, scene, str(steps), str(team_bodies), str(block), str(graph), out, str(lane_bodies),
-                        str(single)],
-                       capture_output=True, text=True, env=ENV, timeout=600)
+                        str(single), str(team_spin)],
+                       capture_output=True, text=True, env=env, timeout=600)
     assert r.returncode == 0, r.stderr[-3000:]
     assert "AddressSanitizer" not in r.stderr, r.stderr[-3000:]
     return r.stdout, open(out, "rb").read()
+
+
+def _world_lines(log: str) -> list:
+    """The per-world counter lines of the driver's log (without the lines
+    that list collected overflow marks)."""
+    return [l for l in log.splitlines() if " world " in l and " overflow world " not in l]
 
 
 def test_team_and_workgroup_paths_in_bounds_and_identical(emu_main, tmp_path):
@@ -222,7 +230,8 @@ def test_emulated_pair_overflow_grows_and_replays_exactly(emu_main, tmp_path):
     _scene(scene, [(d, bodies)])
     steps = 6
     log_b, raw_b = _run(emu_main, scene, str(tmp_path / "b.bin"), steps, 0, 0, graph=0)
-    first = log_b.splitlines()[0].split()
+    first = _world_lines(log_b)[0].split()
+    assert "step 0 overflow world 0" in log_b
     assert int(first[11]) & 1 and int(first[5]) > 3 * 1064
     log_s, raw_s = _run(emu_main, scene, str(tmp_path / "s.bin"), steps, 0, 0, graph=0, single=1)
     lines = [l.split() for l in log_s.splitlines()]
@@ -236,3 +245,72 @@ def test_emulated_pair_overflow_grows_and_replays_exactly(emu_main, tmp_path):
     for f in ("position", "rotation", "velocity", "omega", "flags", "id"):
         assert got[f].tobytes() == ref[f].tobytes(), f
     assert raw_b != raw_s
+
+
+def _team_scene(path):
+    """A 700-body AVBD pile (a team of two 512-thread workgroups at a
+    600-body threshold) and a small PGS world stepped beside it."""
+    avbd = beatbox.WorldDesc.make(**scenes.world_desc(solver=scenes.SOLVER_AVBD, iterations=10,
+                                                      morton_bits=63, quirks=0))
+    pgs = beatbox.WorldDesc.make(**scenes.world_desc(iterations=8, morton_bits=63, quirks=0))
+    worlds = [(avbd, scenes.serving_pile(22, 700)), (pgs, scenes.serving_pile(25, 150))]
+    _scene(path, worlds)
+    return worlds
+
+
+def test_team_that_cannot_meet_leaves_every_step_in_bounds(emu_main, tmp_path):
+    """One workgroup resident at a time (EMU_MAX_BLOCKS=1): a team's barrier
+    can never be met.  Each waiting workgroup gives up after the spin limit,
+    and every member leaves the step at its first barrier after that
+    (GroupTeam::abandon) without touching data the team did not finish:
+    AddressSanitizer sees every access in bounds, each step flags the team
+    world (overflow bit 1, collected as a mark) and the next step's team
+    starts from a reset control block.  The other world is stepped as usual
+    and equals its run with the team path off."""
+    scene = str(tmp_path / "scene.bin")
+    worlds = _team_scene(scene)
+    log, raw = _run(emu_main, scene, str(tmp_path / "t.bin"), 3, 600, 0, graph=0, team_spin=2000, max_blocks=1)
+    for s in range(3):
+        assert f"step {s} overflow world 0" in log, log
+        assert f"step {s} overflow world 1" not in log, log
+        w0 = [l for l in _world_lines(log) if l.startswith(f"step {s} world 0 ")][0].split()
+        assert int(w0[11]) & 2
+    ref_log, ref = _run(emu_main, scene, str(tmp_path / "r.bin"), 3, 0, 0, graph=0)
+    n0 = len(worlds[0][1])
+    got = np.frombuffer(raw, dtype=beatbox.BODY_STATE)[n0:]
+    want = np.frombuffer(ref, dtype=beatbox.BODY_STATE)[n0:]
+    assert got.tobytes() == want.tobytes()
+    assert [l for l in _world_lines(log) if " world 1 " in l] == [l for l in _world_lines(ref_log) if " world 1 " in l]
+
+
+def test_team_barrier_timeouts_at_any_stage_stay_in_bounds(emu_main, tmp_path):
+    """Every workgroup resident, but a spin limit so short that waits run
+    out now and then at whatever barrier a slow workgroup holds up: the
+    team leaves the step at that point, in any stage, and stays in bounds."""
+    scene = str(tmp_path / "scene.bin")
+    _team_scene(scene)
+    _run(emu_main, scene, str(tmp_path / "t.bin"), 4, 600, 0, graph=1, team_spin=40)
+
+
+def test_single_world_team_failure_replays_by_one_workgroup(emu_main, tmp_path):
+    """bb_world_step on a team world whose team cannot meet: the step is
+    undone and run again by one workgroup, so the trajectory equals the
+    oracle's bit for bit and no step reports an overflow."""
+    from oracle_bind import OracleWorld
+
+    avbd = beatbox.WorldDesc.make(**scenes.world_desc(solver=scenes.SOLVER_AVBD, iterations=10,
+                                                      morton_bits=63, quirks=0))
+    bodies = scenes.serving_pile(22, 700)
+    scene = str(tmp_path / "scene.bin")
+    _scene(scene, [(avbd, bodies)])
+    steps = 3
+    log, raw = _run(emu_main, scene, str(tmp_path / "s.bin"), steps, 600, 0, graph=0, single=1, team_spin=2000,
+                    max_blocks=1)
+    assert all(int(l.split()[11]) == 0 for l in _world_lines(log))
+    o = OracleWorld(avbd)
+    o.add_bodies(bodies)
+    o.step(steps)
+    ref = o.read_bodies()
+    got = np.frombuffer(raw, dtype=beatbox.BODY_STATE)
+    for f in ("position", "rotation", "velocity", "omega", "flags", "id"):
+        assert got[f].tobytes() == ref[f].tobytes(), f

@@ -47,10 +47,50 @@ def test_batch_world_reports_the_overflow():
     b = beatbox.Batch()
     b.add_world(beatbox.WorldDesc.make(**DESC), bodies)
     b.step(1)
-    b.sync()
+    with pytest.raises(beatbox.BBError) as e:
+        b.sync()
+    assert e.value.code == 5  # BB_ERR_OVERFLOW
     st = b.stats(0)
     assert st["overflow"] & 1 and st["broad_phase_collision_count"] > 8 * len(bodies) + 64
+    assert b.overflow() == [0]
+    b.sync()  # the mark is collected
     b.destroy()
+
+
+def test_overflow_in_an_early_step_is_kept_until_collected():
+    """Worlds 0 and 2 overflow in the first of three steps only (a block of
+    spheres flying apart); after the three steps their own counters say
+    nothing, but the marks stay until bb_batch_overflow collects them, and
+    the sync reports BB_ERR_OVERFLOW until then.  The single-world step of
+    the same world grows its capacity and replays, and equals the oracle."""
+    desc = beatbox.WorldDesc.make(**DESC)
+    boom = scenes.expanding_spheres(0xBB78)
+    calm = scenes.random_boxes(0xBB79, 40, extent=2.0, y0=0.55, y1=3.0)
+    b = beatbox.Batch()
+    for bodies in (boom, calm, scenes.expanding_spheres(0xBB7A)):
+        b.add_world(desc, bodies)
+    b.step(3, use_graph=True)
+    with pytest.raises(beatbox.BBError) as e:
+        b.sync()
+    assert e.value.code == 5
+    assert all(b.stats(w)["overflow"] == 0 for w in range(3))
+    assert b.overflow() == [0, 2]
+    b.sync()
+    assert b.overflow() == []
+    b.step(2)
+    b.sync()
+    b.destroy()
+    gpu = beatbox.RigidBodyManager(desc)
+    gpu.add_bodies(boom)
+    ora = OracleWorld(desc)
+    ora.add_bodies(boom)
+    for s in range(3):
+        gpu.simulate(1)
+        ora.step(1)
+        bad = compare_step(gpu, ora, desc.morton_bits, s)
+        assert not bad, "\n".join(bad[:20])
+    gpu.sync()
+    gpu.destroy()
 
 
 def test_grown_world_checkpoint_continues_bit_identically():

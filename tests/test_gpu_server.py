@@ -103,3 +103,30 @@ def test_server_admission_limits():
         tiny.submit(pgs, scenes.batch_world(0xBB04, 5, 2000), 3)
     assert e.value.code == 7  # BB_ERR_BUDGET
     tiny.destroy()
+
+
+def test_server_reruns_a_world_that_overflowed():
+    """A served world whose step drops pairs past its capacity is run again
+    from its first step with twice the pair capacity (as many times as it
+    takes) before its result is reported, so the result equals the world
+    stepped alone by bb_world_step, which grows and replays the same way.
+    One world overflows in its first step only (spheres flying apart), one in
+    every step (a dense block); a calm world beside them is unaffected."""
+    desc = beatbox.WorldDesc.make(**scenes.world_desc(iterations=8, morton_bits=63, quirks=0, gravity=0.0))
+    pgs = beatbox.WorldDesc.make(**scenes.world_desc(iterations=8, morton_bits=63, quirks=0))
+    reqs = [(desc, scenes.expanding_spheres(0xBB7B), 4), (pgs, scenes.batch_world(0xBB04, 9, 150), 6),
+            (desc, scenes.dense_spheres(0xBB7C), 5)]
+    srv = beatbox.Server(budget_bytes=1 << 30, max_worlds=4)
+    tickets = [srv.submit(d, b, s) for d, b, s in reqs]
+    srv.drain()
+    rep = srv.report()
+    assert rep["reruns"] >= 2
+    for t, (d, bodies, steps) in zip(tickets, reqs):
+        state, got, stats = srv.poll(t, len(bodies))
+        assert state == beatbox.REQ_DONE, t
+        ref, ref_stats = _alone(d, bodies, steps)
+        for f in FIELDS:
+            assert got[f].tobytes() == ref[f].tobytes(), (t, f)
+        assert stats["overflow"] == 0
+        assert stats["broad_phase_collision_count"] == ref_stats["broad_phase_collision_count"]
+    srv.destroy()
