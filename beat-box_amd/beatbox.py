@@ -226,6 +226,9 @@ EXPORTS = {
     "bb_batch_energy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32]),
     "bb_world_energy": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
     "bb_world_instance_transforms": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_int]),
+    "bb_world_lbvh_nodes": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32), C.c_int]),
+    "bb_batch_lbvh_nodes": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32),
+                                      C.c_int]),
     "bb_world_snapshot": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.POINTER(C.c_size_t)]),
     "bb_world_restore": (C.c_int, [C.c_void_p, C.c_size_t, C.c_int, C.POINTER(C.c_void_p)]),
     "bb_batch_instance_transforms": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_int]),
@@ -391,6 +394,22 @@ class RigidBodyManager:
                "bb_world_instance_transforms")
         return out
 
+    def lbvh_nodes(self, device_ptr: int | None = None, cap: int = 0):
+        """The last step's BVH in the reference's LBVHNode layout
+        (bb_world_lbvh_nodes): a LBVH_NODE array on the host, or, with a
+        device pointer to `cap` 40-byte records, the node count after
+        queueing the write there."""
+        cnt = C.c_uint32(0)
+        _check(lib().bb_world_lbvh_nodes(self._h, None, 0, C.byref(cnt), 0), "bb_world_lbvh_nodes")
+        if device_ptr is not None:
+            _check(lib().bb_world_lbvh_nodes(self._h, C.c_void_p(device_ptr), cap, C.byref(cnt), 1),
+                   "bb_world_lbvh_nodes")
+            return cnt.value
+        out = np.zeros(cnt.value, LBVH_NODE)
+        _check(lib().bb_world_lbvh_nodes(self._h, out.ctypes.data_as(C.c_void_p), cnt.value, C.byref(cnt), 0),
+               "bb_world_lbvh_nodes")
+        return out
+
     def destroy(self) -> None:
         if self._h:
             lib().bb_world_destroy(self._h)
@@ -450,7 +469,7 @@ class Batch:
     def overflow(self, cap: int | None = None) -> list:
         """Indices of the worlds whose steps flagged an overflow since the
         last call (bb_batch_overflow); clears the marks."""
-        cap = self.world_count() if cap is None else cap
+        cap = self.world_count if cap is None else cap
         out = np.zeros(max(cap, 1), np.uint32)
         cnt = C.c_uint32(0)
         _check(lib().bb_batch_overflow(self._h, out.ctypes.data_as(C.c_void_p), cap, C.byref(cnt)),

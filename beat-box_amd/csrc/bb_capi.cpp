@@ -901,6 +901,29 @@ bb_status instances_batch(bb_batch* b, uint32_t w, float* dst, uint32_t n, int o
   return rc == hipSuccess ? BB_OK : BB_ERR_DEVICE;
 }
 
+bb_status tap_world(bb_batch* b, uint32_t w, bb_tap kind, void* dst, size_t cap, size_t* written);
+
+// The BVH of world w's last step as 2n-1 LBVHNode records (the debug tap's
+// conversion, done by a kernel for a device destination).
+bb_status lbvh_batch(bb_batch* b, uint32_t w, bb_lbvh_node* dst, uint32_t cap, uint32_t* count,
+                     int on_device) {
+  if (w >= b->worlds.size() || !count) return BB_ERR_INVALID;
+  bb_status st = build(b);
+  if (st != BB_OK) return st;
+  const uint32_t N = b->worlds[w].n;
+  const uint32_t nodes = (b->stepped && N) ? 2 * N - 1 : 0;
+  *count = nodes;
+  if (!dst) return BB_OK;
+  if (cap < nodes) return BB_ERR_CAPACITY;
+  if (!on_device) {
+    size_t written = 0;
+    return tap_world(b, w, BB_TAP_LBVH, dst, (size_t)cap * sizeof(bb_lbvh_node), &written);
+  }
+  launch_lbvh_export(dev_for(b, b->latest), w, nodes, dst, b->stream);
+  HIPCHK(hipGetLastError());
+  return BB_OK;
+}
+
 // ---- checkpoints.  A world's blob holds what its next step reads from the
 // device (the newest parity's body rows in row order, the id -> row map, the
 // pair list with its per-row offsets, the manifold slots that warm start the
@@ -2047,6 +2070,16 @@ bb_status bb_world_energy(bb_world* w, double* out) {
 bb_status bb_world_instance_transforms(bb_world* w, float* dst, uint32_t n, int dst_on_device) {
   return w ? instances_batch(w->b, 0, dst, n, dst_on_device) : BB_ERR_INVALID;
 }
+
+bb_status bb_world_lbvh_nodes(bb_world* w, bb_lbvh_node* dst, uint32_t cap, uint32_t* count, int dst_on_device) {
+  return w ? lbvh_batch(w->b, 0, dst, cap, count, dst_on_device) : BB_ERR_INVALID;
+}
+
+bb_status bb_batch_lbvh_nodes(bb_batch* b, uint32_t world, bb_lbvh_node* dst, uint32_t cap, uint32_t* count,
+                              int dst_on_device) {
+  return b ? lbvh_batch(b, world, dst, cap, count, dst_on_device) : BB_ERR_INVALID;
+}
+
 
 bb_status bb_world_snapshot(bb_world* w, void* dst, size_t cap, size_t* size) {
   if (!w) return BB_ERR_INVALID;

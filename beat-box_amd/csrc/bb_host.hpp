@@ -22,6 +22,8 @@ void launch_copy(const float4* src, float4* dst, size_t n, hipStream_t st);
 void launch_sum_counts(const Dev& d, unsigned long long* out, hipStream_t st);
 void launch_energy(const Dev& d, double* out, hipStream_t st);
 void launch_instances(const Dev& d, uint32_t w, uint32_t n, float* dst, hipStream_t st);
+// World w's `nodes` (= 2n-1) BVH nodes as 40-byte LBVHNode records into dst.
+void launch_lbvh_export(const Dev& d, uint32_t w, uint32_t nodes, void* dst, hipStream_t st);
 uint32_t team_capacity();
 }  // namespace bb
 

@@ -3834,6 +3834,34 @@ __global__ __launch_bounds__(256) void k_instances(Dev d, uint32_t w, float* __r
   o[2] = make_float4(R.m[2][0], R.m[2][1], R.m[2][2], p.z);
 }
 
+// LBVH hand-off (RigidBodyManager::get_lbvh_node_buffer, src/
+// rigid_body_manager.cpp:1601-1604, consumed as procedural AABB geometry
+// with a 40-byte stride by src/acceleration_structure_manager.cpp:490-535):
+// the 2n-1 nodes of world w's last BVH in the reference's LBVHNode layout
+// (include/shared.inl:867-897): min xyz, max xyz, left, right, primitive
+// index (the leaf's row; 0xFFFFFFFF for an inner node) and a zero pad.  A
+// leaf has left = right = -1.
+__global__ __launch_bounds__(256) void k_lbvh_export(Dev d, uint32_t w, uint32_t* __restrict__ dst) {
+  const WorldInfo W = d.worlds[w];
+  const uint32_t nn = W.n ? 2u * W.n - 1u : 0u;
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  if (i >= nn) return;
+  const size_t g = 2 * (size_t)W.body_off + i;
+  const float4 mn = d.nmin[g], mx = d.nmax[g];
+  const int l = __float_as_int(mn.w), r = __float_as_int(mx.w);
+  uint32_t* o = dst + (size_t)i * 10u;
+  o[0] = __float_as_uint(mn.x);
+  o[1] = __float_as_uint(mn.y);
+  o[2] = __float_as_uint(mn.z);
+  o[3] = __float_as_uint(mx.x);
+  o[4] = __float_as_uint(mx.y);
+  o[5] = __float_as_uint(mx.z);
+  o[6] = l < 0 ? 0xFFFFFFFFu : (uint32_t)l;
+  o[7] = l < 0 ? 0xFFFFFFFFu : (uint32_t)r;
+  o[8] = l < 0 ? (uint32_t)r : kMAX;
+  o[9] = 0u;
+}
+
 // Batch totals of one step's counters, one workgroup: out[0..7] = sum N,
 // sum A, sum P, sum M, sum C, sum over worlds of (colours used x M), max
 // penetration (mm), sum sleeping.
@@ -4021,6 +4049,12 @@ void launch_sum_counts(const Dev& d, unsigned long long* out, hipStream_t st) {
 
 void launch_instances(const Dev& d, uint32_t w, uint32_t n, float* dst, hipStream_t st) {
   if (n) hipLaunchKernelGGL(k_instances, dim3((n + 255u) / 256u), dim3(256), 0, st, d, w, dst);
+}
+
+void launch_lbvh_export(const Dev& d, uint32_t w, uint32_t nodes, void* dst, hipStream_t st) {
+  if (nodes)
+    hipLaunchKernelGGL(k_lbvh_export, dim3((nodes + 255u) / 256u), dim3(256), 0, st, d, w,
+                       static_cast<uint32_t*>(dst));
 }
 
 void launch_energy(const Dev& d, double* out, hipStream_t st) {

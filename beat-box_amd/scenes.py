@@ -441,3 +441,123 @@ def voxel_mixed_pile(seed: int, n: int):
                 b["radius"] = 0.5
             parts.append(b)
     return shapes, np.concatenate(parts)
+
+
+# ---------------------------------------------------------------------------
+# The reference's own test scenes (src/scene_manager.hpp), as body lists in
+# load order (body id = position in the list).  They carry the reference's
+# known answers (towers stand, ramp stick/slide outcomes and resting friction
+# force, every cube of the box pool asleep) and are stepped in parity mode.
+# Floats are computed as the reference computes them, in float32.
+
+def _ref_body(pos, rot, half, mass, friction, dynamic):
+    b = np.zeros(1, BODY_DESC)[0]
+    b["shape"] = SHAPE_BOX
+    b["flags"] = (BODY_DYNAMIC | BODY_GRAVITY) if dynamic else 0
+    b["position"] = np.asarray(pos, np.float32)
+    b["rotation"] = np.asarray(rot, np.float32)
+    h = np.asarray(half, np.float32)
+    b["minimum"] = -h
+    b["maximum"] = h
+    b["mass"] = mass
+    b["friction"] = friction
+    return b
+
+
+def _ref_floor_and_panel(friction, panel_pos, panel_half):
+    return [_ref_body((0.0, -50.0, 0.0), (0, 0, 0, 1), (50.0, 50.0, 50.0), 0.0, friction, False),
+            _ref_body(panel_pos, (0, 0, 0, 1), panel_half, 0.0, friction, False)]
+
+
+def reference_towers():
+    """scene_3 (src/scene_manager.hpp:327-353): floor, a static panel and three
+    towers of 5, 8 and 12 unit cubes (mass 5, friction 0.5) at x = -6, 0, 6,
+    z = 14, spawned with a 1 mm gap per level.  Known answer: they stand."""
+    out = _ref_floor_and_panel(0.5, (0.0, 22.0, 14.0), (5.0, 0.2, 5.0))
+    for x, h in ((-6.0, 5), (0.0, 8), (6.0, 12)):
+        for i in range(h):
+            y = np.float32(0.5) + np.float32(i) * np.float32(1.001)
+            out.append(_ref_body((x, y, 14.0), (0, 0, 0, 1), (0.5, 0.5, 0.5), 5.0, 0.5, True))
+    return np.array(out, BODY_DESC)
+
+
+# scene_4 ramp angles and the indices of its cubes: (ramp, effective friction,
+# first body id); the 2-cube tower's top cube is listed last.
+REF_RAMP_DEG = (15.0, 30.0)
+
+
+def reference_ramps():
+    """scene_4 (src/scene_manager.hpp:355-467): floor and panel (friction 1),
+    a 15 deg ramp (downhill -x) and a 30 deg ramp (downhill +x), both static
+    with friction 1, and unit cubes of mass 5 with friction 0.04 / 0.25 / 1.0
+    (effective sqrt(mu * 1) = 0.2 / 0.5 / 1.0) parked on them, two of each
+    per ramp, plus a second cube stacked on the gentle ramp's first mu = 1
+    cube.  Returns (bodies, info): info[k] = (ramp index, mu_eff) of body
+    4 + k.  Known answers (Coulomb): on 15 deg (tan 0.268) mu 0.2 slides,
+    0.5 and 1.0 stick and the tower stands; on 30 deg (tan 0.577) 0.2 and
+    0.5 slide, 1.0 sticks; a resting cube's friction force is m g sin(theta)."""
+    f32 = np.float32
+    deg = f32(3.14159265) / f32(180.0)
+    th = (f32(15.0) * deg, f32(30.0) * deg)
+    qa = (0.0, 0.0, float(np.float32(np.sin(np.float64(th[0] * f32(0.5))))),
+          float(np.float32(np.cos(np.float64(th[0] * f32(0.5))))))
+    qb = (0.0, 0.0, -float(np.float32(np.sin(np.float64(th[1] * f32(0.5))))),
+          float(np.float32(np.cos(np.float64(th[1] * f32(0.5))))))
+    s = lambda x: f32(np.sin(np.float64(x)))  # noqa: E731
+    c = lambda x: f32(np.cos(np.float64(x)))  # noqa: E731
+    xl = ((c(th[0]), s(th[0])), (c(th[1]), -s(th[1])))
+    yl = ((-s(th[0]), c(th[0])), (s(th[1]), c(th[1])))
+    centre = ((f32(-7.0), f32(2.3), f32(14.0)), (f32(7.0), f32(3.9), f32(14.0)))
+    quat = (qa, qb)
+    out = _ref_floor_and_panel(1.0, (0.0, 20.0, 14.0), (5.0, 0.2, 5.0))
+    for k in range(2):
+        out.append(_ref_body(centre[k], quat[k], (7.0, 0.4, 4.5), 0.0, 1.0, False))
+    info = []
+
+    def cube(k, u, z, mu, lift):
+        cx, cy, _ = centre[k]
+        u, lift = f32(u), f32(lift)
+        p = (cx + u * xl[k][0] + lift * yl[k][0], cy + u * xl[k][1] + lift * yl[k][1], f32(z))
+        out.append(_ref_body(p, quat[k], (0.5, 0.5, 0.5), 5.0, mu, True))
+        info.append((k, float(np.float32(np.sqrt(np.float64(mu))))))
+
+    lift0 = f32(0.4) + f32(0.5) + f32(0.005)
+    for u, z, mu in ((3.5, 16.2, 0.04), (3.5, 17.8, 0.04), (1.75, 13.4, 0.25), (1.75, 15.0, 0.25),
+                     (0.0, 10.6, 1.0), (0.0, 12.2, 1.0)):
+        cube(0, u, z, mu, lift0)
+    cube(0, 0.0, 10.6, 1.0, lift0 + f32(1.005))
+    for u, z, mu in ((-3.5, 16.2, 0.04), (-3.5, 17.8, 0.04), (-1.75, 13.4, 0.25), (-1.75, 15.0, 0.25),
+                     (0.0, 10.6, 1.0), (0.0, 12.2, 1.0)):
+        cube(1, u, z, mu, lift0)
+    return np.array(out, BODY_DESC), info
+
+
+def reference_box_pool():
+    """scene_7 (src/scene_manager.hpp:744-823): floor, a panel above, four
+    static walls around an 8 x 8 pit and 432 unit cubes (mass 5, friction
+    0.6) raining in from y = 12..43 in 12 waves of 6 x 6, with the scene's
+    index-hashed jitter and 4-13 deg tilts.  Known answer (:739-743): every
+    cube ends asleep."""
+    f32 = np.float32
+    out = _ref_floor_and_panel(0.6, (0.0, 50.0, 14.0), (8.0, 0.2, 8.0))
+    for pos, half in (((4.25, 5.0, 14.0), (0.25, 5.0, 4.5)), ((-4.25, 5.0, 14.0), (0.25, 5.0, 4.5)),
+                      ((0.0, 5.0, 18.25), (4.5, 5.0, 0.25)), ((0.0, 5.0, 9.75), (4.5, 5.0, 0.25))):
+        out.append(_ref_body(pos, (0, 0, 0, 1), half, 0.0, 0.6, False))
+    for iy in range(12):
+        for iz in range(6):
+            for ix in range(6):
+                h1 = ix * 3 + iy * 5 + iz * 7
+                h2 = ix * 7 + iy * 3 + iz * 5
+                h3 = ix * 5 + iy * 7 + iz * 3
+                jx = (f32(h1 % 11) - f32(5.0)) * f32(0.05)
+                jz = (f32(h2 % 11) - f32(5.0)) * f32(0.05)
+                jy = (f32(h3 % 11) - f32(5.0)) * f32(0.20)
+                phi = f32(h3 % 16) * f32(0.3927)
+                ang = f32(0.07) + f32(h1 % 7) * f32(0.025)
+                sh = f32(np.sin(np.float64(f32(0.5) * ang)))
+                q = (f32(np.cos(np.float64(phi))) * sh, 0.0, f32(np.sin(np.float64(phi))) * sh,
+                     f32(np.cos(np.float64(f32(0.5) * ang))))
+                p = (f32(-3.0) + f32(1.2) * f32(ix) + jx, f32(12.0) + f32(2.8) * f32(iy) + jy,
+                     f32(11.0) + f32(1.2) * f32(iz) + jz)
+                out.append(_ref_body(p, q, (0.5, 0.5, 0.5), 5.0, 0.6, True))
+    return np.array(out, BODY_DESC)

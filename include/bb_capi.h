@@ -345,6 +345,20 @@ bb_status bb_world_energy(bb_world* w, double* out);
  * call returns when it is filled. */
 bb_status bb_world_instance_transforms(bb_world* w, float* dst, uint32_t n,
                                        int dst_on_device);
+/* RigidBodyManager::get_lbvh_node_buffer (src/rigid_body_manager.cpp:
+ * 1601-1604): the BVH the last step built, as the 2N-1 records of the
+ * reference's LBVHNode layout (bb_lbvh_node, 40 bytes: min, max, left and
+ * right child, primitive index = the leaf's row or 0xFFFFFFFF for an inner
+ * node, pad), node 0 the root.  The reference's renderer builds its BLAS from
+ * this buffer as procedural AABB geometry with a 40-byte stride
+ * (AccelerationStructureManager::update, src/acceleration_structure_manager.
+ * cpp:490-535).  *count receives 2N-1 (0 before the first step); with dst ==
+ * NULL nothing else happens.  cap = capacity of dst in nodes
+ * (BB_ERR_CAPACITY if short).  With dst_on_device != 0 the records are
+ * written by a kernel queued on the world's stream (bb_world_sync waits for
+ * it); otherwise dst is host memory, filled when the call returns. */
+bb_status bb_world_lbvh_nodes(bb_world* w, bb_lbvh_node* dst, uint32_t cap,
+                              uint32_t* count, int dst_on_device);
 /* Checkpoint of a world for a later bb_world_restore, in this process or
  * another.  The reference keeps a paused simulation's buffers only in memory
  * (StatusManager::switch_simulating, src/status_manager.hpp:133-152, with the
@@ -450,6 +464,9 @@ bb_status bb_batch_profile(bb_batch* b, uint32_t nsteps, float* step_ms, float* 
  * counterpart): sum over dynamic bodies of m|v|^2/2 + w.I.w/2 - m*g*y with
  * the physical inertia of the shape.  per_world[n], n >= world count. */
 bb_status bb_batch_energy(bb_batch* b, double* per_world, uint32_t n);
+/* bb_world_lbvh_nodes for one world of a batch. */
+bb_status bb_batch_lbvh_nodes(bb_batch* b, uint32_t world, bb_lbvh_node* dst,
+                              uint32_t cap, uint32_t* count, int dst_on_device);
 /* bb_world_instance_transforms for one world of a batch. */
 bb_status bb_batch_instance_transforms(bb_batch* b, uint32_t world, float* dst,
                                        uint32_t n, int dst_on_device);

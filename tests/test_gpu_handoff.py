@@ -85,3 +85,53 @@ def test_batch_instance_transforms_equal_alone():
         assert b.instance_transforms(k, len(bodies)).tobytes() == w.instance_transforms(len(bodies)).tobytes()
         w.destroy()
     b.destroy()
+
+
+def test_lbvh_export_matches_tap_and_oracle():
+    """The BVH hand-off (get_lbvh_node_buffer): the 2N-1 records in the
+    reference's LBVHNode layout equal the debug tap and the oracle's BVH bit
+    for bit, on the host path and, byte for byte, on the device path; a
+    world of a batch exports the same nodes as the world alone."""
+    from oracle_bind import OracleWorld
+
+    desc = beatbox.WorldDesc.make(**scenes.world_desc(iterations=8))
+    bodies = scenes.random_boxes(0xBB53, 200, extent=4.0, y0=0.55, y1=6.0)
+    w = beatbox.RigidBodyManager(desc)
+    w.add_bodies(bodies)
+    assert len(w.lbvh_nodes()) == 0  # nothing built before the first step
+    ora = OracleWorld(desc)
+    ora.add_bodies(bodies)
+    w.simulate(15)
+    ora.step(15)
+    nodes = w.lbvh_nodes()
+    n = len(bodies)
+    assert len(nodes) == 2 * n - 1
+    assert nodes.tobytes() == w.tap(beatbox.TAP_LBVH, beatbox.LBVH_NODE).tobytes()
+    assert nodes.tobytes() == ora.tap(beatbox.TAP_LBVH, beatbox.LBVH_NODE).tobytes()
+    leaves = nodes["left"] < 0
+    assert leaves.sum() == n and np.array_equal(np.sort(nodes["primitive_idx"][leaves]), np.arange(n))
+    assert np.all(nodes["primitive_idx"][~leaves] == 0xFFFFFFFF)
+    hip = hip_runtime()
+    dev = C.c_void_p()
+    assert hip.hipMalloc(C.byref(dev), len(nodes) * 40) == 0
+    try:
+        with pytest.raises(beatbox.BBError):
+            w.lbvh_nodes(device_ptr=dev.value, cap=len(nodes) - 1)
+        assert w.lbvh_nodes(device_ptr=dev.value, cap=len(nodes)) == len(nodes)
+        w.sync()
+        back = np.zeros(len(nodes), beatbox.LBVH_NODE)
+        assert hip.hipMemcpy(back.ctypes.data_as(C.c_void_p), dev, len(nodes) * 40, 2) == 0
+    finally:
+        hip.hipFree(dev)
+    assert back.tobytes() == nodes.tobytes()
+    b = beatbox.Batch()
+    b.add_world(desc, scenes.random_boxes(0xBB54, 30, extent=2.0, y0=0.55, y1=3.0))
+    b.add_world(desc, bodies)
+    b.step(15)
+    b.sync()
+    cnt = C.c_uint32(0)
+    got = np.zeros(len(nodes), beatbox.LBVH_NODE)
+    assert beatbox.lib().bb_batch_lbvh_nodes(b._h, 1, got.ctypes.data_as(C.c_void_p), len(got), C.byref(cnt), 0) == 0
+    assert cnt.value == len(nodes) and got.tobytes() == nodes.tobytes()
+    b.destroy()
+    w.destroy()
