@@ -245,7 +245,12 @@ EXPORTS = {
     "bb_server_destroy": (None, [C.c_void_p]),
     "bb_version": (C.c_char_p, []),
     "bb_copy_bandwidth": (C.c_int, [C.c_uint64, C.c_uint32, C.POINTER(C.c_float)]),
+    "bb_world_step_noalloc": (C.c_int, [C.POINTER(WorldDesc), C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p,
+                                        C.c_uint32, C.POINTER(StepStats), C.c_void_p]),
 }
+
+NOALLOC_MAX_BODIES = 2048
+NOALLOC_SCRATCH_BYTES = 64 << 20
 
 
 def lib() -> C.CDLL:
@@ -281,6 +286,21 @@ def _bodies_ptr(bodies: np.ndarray):
     from scenes import BODY_DESC  # local module next to this file
     arr = np.ascontiguousarray(bodies, dtype=BODY_DESC)
     return arr, arr.ctypes.data_as(C.c_void_p)
+
+
+def step_noalloc(desc: WorldDesc, bodies: np.ndarray, nsteps: int) -> tuple[int, np.ndarray, StepStats]:
+    """bb_world_step_noalloc: a fresh world of `bodies` stepped nsteps times
+    in the library's static scratch (no device allocation).  Returns the
+    status code (BB_OK, or BB_ERR_OVERFLOW with results), the bodies by id
+    and the last step's counters; raises on any other status."""
+    arr, ptr = _bodies_ptr(bodies)
+    out = np.zeros(len(arr), BODY_STATE)
+    stats = StepStats()
+    rc = lib().bb_world_step_noalloc(C.byref(desc), ptr, len(arr), nsteps, out.ctypes.data_as(C.c_void_p),
+                                     len(out), C.byref(stats), None)
+    if rc not in (0, 5):
+        _check(rc, "bb_world_step_noalloc")
+    return rc, out, stats
 
 
 # ------------------------------------------------------------ world object

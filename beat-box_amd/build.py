@@ -13,7 +13,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libbeatbox_amd.so")
-SOURCES = ["bb_step.hip", "bb_capi.cpp", "bb_server.cpp"]
+SOURCES = ["bb_step.hip", "bb_capi.cpp", "bb_server.cpp", "bb_noalloc.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [

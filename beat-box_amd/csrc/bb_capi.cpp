@@ -225,6 +225,8 @@ bb_status alloc_arrays(bb_batch* b, Buffers& A, uint32_t W, uint32_t NB, uint32_
   b->vox_dirty = true;
   if (A.count_only) return BB_OK;
   if (std::find(A.ptrs.begin(), A.ptrs.end(), nullptr) != A.ptrs.end()) return BB_ERR_CAPACITY;
+  // a carved layout is cleared by its owner, on its own stream
+  if (A.carve_base) return BB_OK;
   HIPCHK(hipMemset(d.teams, 0, sizeof(TeamTable)));
   HIPCHK(hipMemset(d.team_ctl, 0, sizeof(TeamCtl) * kMaxTeams));
   // tables that must start at zero: colour owner cells (zero between

@@ -56,11 +56,17 @@ struct HostWorld {
 
 // Device allocations of a batch.  With count_only set nothing is allocated
 // and `bytes` sums what would be (used to price a world for admission).
+// With carve_base set nothing is allocated either: the arrays are cut, 256-byte
+// aligned, from the caller's range [carve_base, carve_base + carve_cap), a
+// request past its end yields a null pointer, `bytes` is the end of the last
+// cut, and release() frees nothing.
 struct Buffers {
   std::vector<void*> ptrs;
   std::vector<size_t> sizes;  // bytes of each allocation, in ptrs order
   uint64_t bytes = 0;
   bool count_only = false;
+  char* carve_base = nullptr;
+  size_t carve_cap = 0;
   template <class T>
   T* alloc(size_t count) {
     void* p = nullptr;
@@ -69,15 +75,25 @@ struct Buffers {
       bytes += n;
       return nullptr;
     }
-    if (hipMalloc(&p, n) != hipSuccess) p = nullptr;
-    else bytes += n;
+    if (carve_base) {
+      const size_t off = (bytes + 255u) & ~(size_t)255u;
+      if (off <= carve_cap && n <= carve_cap - off) {
+        p = carve_base + off;
+        bytes = off + n;
+      }
+    } else if (hipMalloc(&p, n) != hipSuccess) {
+      p = nullptr;
+    } else {
+      bytes += n;
+    }
     ptrs.push_back(p);
     sizes.push_back(n);
     return (T*)p;
   }
   void release() {
-    for (void* p : ptrs)
-      if (p) (void)hipFree(p);
+    if (!carve_base)
+      for (void* p : ptrs)
+        if (p) (void)hipFree(p);
     ptrs.clear();
     sizes.clear();
     bytes = 0;
@@ -175,9 +191,11 @@ bb_status upload_voxels(bb_batch* b);
 // Parity mode keeps the reference cap of 1,024 bodies per world.
 bool over_reference_cap(const bb_world_desc& d, uint32_t n);
 WorldInfo info_of(const HostWorld& hw);
-// Allocates (or, with A.count_only, prices) every device array of a batch
-// with W world slots, NB body rows, NA active slots and NP pair slots, and
-// points b->base / rows / man at them.
+// Allocates (or, with A.count_only, prices, or, with A.carve_base, cuts from
+// the caller's range) every device array of a batch with W world slots, NB
+// body rows, NA active slots and NP pair slots, and points b->base / rows /
+// man at them.  Allocated arrays get their zero-start tables cleared; a carved
+// layout is left for its owner to clear.
 bb_status alloc_arrays(bb_batch* b, Buffers& A, uint32_t W, uint32_t NB, uint32_t NA, uint32_t NP);
 // Device bytes of a batch of these capacities.
 uint64_t device_bytes(uint32_t W, uint32_t NB, uint32_t NA, uint32_t NP);
@@ -220,6 +238,12 @@ void world_order(const std::vector<WorldInfo>& infos, uint32_t* out);
 Dev dev_for(const bb_batch* b, int cur);
 // 1 when a world of the batch runs the per-island solver path.
 uint32_t any_island(const bb_batch* b);
+// One step of every world of a built batch on b->stream (flips the parity).
+bb_status launch_one(bb_batch* b);
+// Bodies of world w (n >= its body count) in `order`, after b->stream drains.
+bb_status read_world(bb_batch* b, uint32_t w, bb_body_state* out, uint32_t n, bb_order order);
+// Counters of world w's last step.
+bb_status world_stats(bb_batch* b, uint32_t w, bb_step_stats* out);
 
 }  // namespace host
 }  // namespace bb

@@ -382,6 +382,32 @@ bb_status bb_world_restore(const void* src, size_t size, int warm_start,
 /* RigidBodyManager::destroy (src/rigid_body_manager.cpp:1550-1585). */
 void bb_world_destroy(bb_world* w);
 
+/* ------------------------------------------------ allocation-free stepping */
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code next to HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): what bb_world_create, bb_world_add_bodies (bodies[0..n)),
+ * bb_world_step(nsteps), bb_world_read_bodies(out, out_n, BB_ORDER_BY_ID) and
+ * bb_world_read_stats(stats, may be NULL) give for a fresh world, in one call
+ * that allocates no device memory.  The world's device arrays are cut from
+ * one static device array of BB_NOALLOC_SCRATCH_BYTES, which every GPU the
+ * library runs on holds until the process ends; its pair capacity is the
+ * largest of 16, 8, 4, 2 or 1 times bb_world_step's starting capacity that
+ * fits there.  Limits: n from 1 to BB_NOALLOC_MAX_BODIES (BB_ERR_CAPACITY
+ * beyond, with a message on stderr), boxes and spheres only (voxel bodies:
+ * BB_ERR_INVALID).  Where bb_world_step would grow the pair capacity and
+ * replay a step, this call keeps the pairs that fit and returns
+ * BB_ERR_OVERFLOW, with out and stats filled.  Work is queued on `stream`
+ * (a hipStream_t; NULL = the null stream) and the call returns when out is
+ * filled.  The array keeps the last call's contents and each call clears the
+ * part it uses.  All calls on one GPU share that GPU's array: calls that
+ * overlapped on one GPU would race, so the library runs them one at a time
+ * within a process (a lock per GPU). */
+#define BB_NOALLOC_MAX_BODIES 2048u
+#define BB_NOALLOC_SCRATCH_BYTES (64ull << 20)
+bb_status bb_world_step_noalloc(const bb_world_desc* desc, const bb_body_desc* bodies, uint32_t n,
+                                uint32_t nsteps, bb_body_state* out, uint32_t out_n, bb_step_stats* stats,
+                                void* stream);
+
 /* ------------------------------------------------------------- batch API */
 /* Many independent worlds of uneven size stepped together: segmented sorts,
  * BVHs and colour sweeps, one workgroup per world.  No reference counterpart

@@ -203,3 +203,36 @@ def test_restore_refuses_blobs_with_bad_indices(what):
         assert rc in (0, 4)  # BB_OK with a GPU, BB_ERR_NO_DEVICE without
     else:
         assert rc == 1, what
+
+
+def test_noalloc_limits_checked_before_the_device():
+    """bb_world_step_noalloc refuses what its static scratch cannot hold
+    (more than NOALLOC_MAX_BODIES bodies: BB_ERR_CAPACITY), voxel bodies and
+    empty or short outputs (BB_ERR_INVALID), before it touches a device; a
+    valid call gets as far as needing one (BB_ERR_NO_DEVICE here)."""
+    desc = beatbox.WorldDesc.make(**scenes.world_desc(morton_bits=63, quirks=0))
+
+    def call(bodies, out_n=None):
+        arr, ptr = beatbox._bodies_ptr(bodies)
+        out = np.zeros(max(len(arr), 1), beatbox.BODY_STATE)
+        n_out = len(arr) if out_n is None else out_n
+        return beatbox.lib().bb_world_step_noalloc(C.byref(desc), ptr, len(arr), 3,
+                                                   out.ctypes.data_as(C.c_void_p), n_out, None, None)
+
+    big = scenes.random_boxes(7, beatbox.NOALLOC_MAX_BODIES, extent=20.0, with_ground=False)
+    assert len(big) == beatbox.NOALLOC_MAX_BODIES
+    over = np.concatenate([big, scenes.random_boxes(8, 1, with_ground=False)])
+    assert call(over) == 2  # BB_ERR_CAPACITY
+    small = scenes.random_boxes(9, 10)
+    assert call(small, out_n=len(small) - 1) == 1
+    assert call(small[:0]) == 1
+    vox = small.copy()
+    vox["shape"][3] = scenes.voxel_shape_word(1)
+    assert call(vox) == 1
+    parity = beatbox.WorldDesc.make(**scenes.world_desc())  # the reference cap of 1,024 bodies
+    arr, ptr = beatbox._bodies_ptr(big)
+    out = np.zeros(len(arr), beatbox.BODY_STATE)
+    assert beatbox.lib().bb_world_step_noalloc(C.byref(parity), ptr, len(arr), 1,
+                                               out.ctypes.data_as(C.c_void_p), len(out), None, None) == 2
+    if not os.path.exists("/dev/kfd"):
+        assert call(big) == 4  # BB_ERR_NO_DEVICE: the maximum itself is accepted
