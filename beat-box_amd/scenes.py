@@ -42,6 +42,7 @@ BODY_DYNAMIC, BODY_GRAVITY = 2, 4
 SOLVER_PGS, SOLVER_PGS_SOFT, SOLVER_AVBD = 0, 1, 2
 SIM_DEFAULT_FLAGS = 1 | 2 | 4 | 8 | 16 | 32 | 128
 SIM_USE_GRAPH_COLORING = 32
+SIM_SLEEPING_ENABLED = 128
 QUIRKS_PARITY = 1 | 2 | 4
 TIME_STEP = np.float32(0.01667)
 

@@ -81,3 +81,30 @@ def test_snapshot_of_fresh_world_and_bad_blobs():
             beatbox.RigidBodyManager.restore(bad)
     a.destroy()
     b.destroy()
+
+
+@pytest.mark.parametrize("k", [0, 1, 2])
+def test_restored_world_equals_the_oracle(k):
+    """A world checkpointed at step 30, restored (warm) and stepped 20 more
+    equals the oracle's uninterrupted world at step 50, every tap included;
+    restored cold it equals the oracle with skip_warm_starting_once at 30."""
+    from oracle_bind import OracleWorld
+    from parity_util import compare_step
+    desc, bodies = _worlds()[k]
+    a = beatbox.RigidBodyManager(desc)
+    a.add_bodies(bodies)
+    a.simulate(30)
+    blob = a.snapshot()
+    a.destroy()
+    for warm in (True, False):
+        ora = OracleWorld(desc)
+        ora.add_bodies(bodies)
+        ora.step(30)
+        if not warm:
+            ora.skip_warm_start_once()
+        ora.step(20)
+        b = beatbox.RigidBodyManager.restore(blob, warm_start=warm)
+        b.simulate(20)
+        bad = compare_step(b, ora, desc.morton_bits, 50)
+        assert not bad, "\n".join(bad[:20])
+        b.destroy()
