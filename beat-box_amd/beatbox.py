@@ -83,7 +83,11 @@ class ServerDesc(C.Structure):
         ("max_resident_bodies", C.c_uint32),
         ("block", C.c_uint32),
         ("team_bodies", C.c_uint32),
+        ("split_bodies", C.c_uint32),
     ]
+
+
+SERVER_MAX_POOLS = 3
 
 
 class ServerReport(C.Structure):
@@ -92,10 +96,17 @@ class ServerReport(C.Structure):
         "bytes_in_use", "bytes_budget", "bytes_allocated", "server_steps", "world_steps",
         "body_steps")] + [(n, C.c_double) for n in (
         "device_ms", "step_ms_p50", "step_ms_p99", "queue_ms_p50", "queue_ms_p99")] + [
-        ("reruns", C.c_uint64)]
+        ("reruns", C.c_uint64), ("pools", C.c_uint32), ("pad", C.c_uint32),
+        ("pool_min_bodies", C.c_uint32 * SERVER_MAX_POOLS), ("pool_pad", C.c_uint32),
+        ("pool_world_steps", C.c_uint64 * SERVER_MAX_POOLS), ("pool_step_ms_p50", C.c_double * SERVER_MAX_POOLS),
+        ("pool_step_ms_p99", C.c_double * SERVER_MAX_POOLS)]
 
     def as_dict(self) -> dict:
-        return {n: getattr(self, n) for n, _ in self._fields_}
+        out = {n: getattr(self, n) for n, _ in self._fields_ if not n.startswith("pool") and "pad" not in n}
+        out["pools"] = [dict(min_bodies=self.pool_min_bodies[p], world_steps=self.pool_world_steps[p],
+                             step_ms_p50=self.pool_step_ms_p50[p], step_ms_p99=self.pool_step_ms_p99[p])
+                        for p in range(self.pools)]
+        return out
 
 
 REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_UNKNOWN, REQ_FAILED = 0, 1, 2, 3, 4
@@ -160,11 +171,18 @@ TAP_ISLANDS, TAP_EDGE_COLORS, TAP_BODY_COLORS, TAP_ROWS = 5, 6, 7, 8
 ORDER_BY_ID, ORDER_BY_ROW = 0, 1
 KERNEL_NAMES = ("k_build", "k_broad", "k_narrow", "k_post", "k_pgs", "k_avbd", "k_finish")
 KERNEL_COUNT = len(KERNEL_NAMES)
+BUILD_NAMES = ("keys", "sort", "hierarchy", "bounds", "reorder")
+BUILD_COUNT = len(BUILD_NAMES)
 COUNT_NAMES = ("bodies", "active", "pairs", "manifolds", "contacts", "color_work", "max_pen_mm", "sleeping")
 COUNT_FIELDS = len(COUNT_NAMES)
 PARAM_BLOCK, PARAM_PGS_LDS_ROWS, PARAM_TEAM_BODIES, PARAM_LANE_BODIES, PARAM_LANE_BLOCK, PARAM_LANE_LDS_ROWS = \
     0, 1, 2, 3, 4, 5
 PARAM_TEAM_SPIN = 6
+PARAM_PAIR_SCALE = 7
+PARAM_SORT_BLOCKS = 8
+PARAM_FUSED = 9
+PARAM_AVBD_LANES = 10
+PARAM_PGS_PREFETCH = 11
 
 STATUS = {0: "BB_OK", 1: "BB_ERR_INVALID", 2: "BB_ERR_CAPACITY", 3: "BB_ERR_DEVICE",
           4: "BB_ERR_NO_DEVICE", 5: "BB_ERR_OVERFLOW", 6: "BB_ERR_STATE", 7: "BB_ERR_BUDGET"}
@@ -224,6 +242,7 @@ EXPORTS = {
     "bb_batch_autotune": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_float)]),
     "bb_batch_profile": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]),
     "bb_batch_energy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32]),
+    "bb_batch_build_split": (C.c_int, [C.c_void_p, C.c_void_p]),
     "bb_world_energy": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
     "bb_world_instance_transforms": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_int]),
     "bb_world_lbvh_nodes": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32), C.c_int]),
@@ -509,6 +528,14 @@ class Batch:
                "bb_batch_profile")
         return (step_ms, kern_ms, cnt) if counts else (step_ms, kern_ms)
 
+    def build_split(self) -> np.ndarray:
+        """bb_batch_build_split: ms of the build stage's parts (BUILD_NAMES)
+        inside the slowest workgroup team, summed over the last profile()
+        call's steps (zeros without teams)."""
+        out = np.zeros(BUILD_COUNT, np.float32)
+        _check(lib().bb_batch_build_split(self._h, out.ctypes.data_as(C.c_void_p)), "bb_batch_build_split")
+        return out
+
     def energy(self) -> np.ndarray:
         """Mechanical energy of each world's current state (float64)."""
         out = np.zeros(max(self.world_count, 1), np.float64)
@@ -586,9 +613,10 @@ class Server:
     worlds admitted by HBM budget, retired when their steps are done."""
 
     def __init__(self, budget_bytes: int, max_worlds: int = 1024, max_bodies: int = 0,
-                 max_resident_bodies: int = 0, block: int = 0, team_bodies: int = 0):
+                 max_resident_bodies: int = 0, block: int = 0, team_bodies: int = 0, split_bodies: int = 0):
         self._h = C.c_void_p()
-        d = ServerDesc(budget_bytes, max_worlds, max_bodies, max_resident_bodies, block, team_bodies)
+        d = ServerDesc(budget_bytes, max_worlds, max_bodies, max_resident_bodies, block, team_bodies,
+                       split_bodies)
         _check(lib().bb_server_create(C.byref(d), C.byref(self._h)), "bb_server_create")
 
     def submit(self, desc: WorldDesc, bodies: np.ndarray, steps: int) -> int:

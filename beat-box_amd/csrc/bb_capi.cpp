@@ -211,6 +211,9 @@ bb_status alloc_arrays(bb_batch* b, Buffers& A, uint32_t W, uint32_t NB, uint32_
   d.stats = A.alloc<uint32_t>((size_t)W * ST_COUNT);
   d.ovf_any = A.alloc<uint32_t>(1);
   d.team_spin = b->team_spin;
+  d.sort_blocks = b->sort_blocks;
+  d.avbd_lanes = b->avbd_lanes;
+  d.pgs_prefetch = b->pgs_prefetch;
   d.teams = A.alloc<TeamTable>(1);
   d.team_ctl = A.alloc<TeamCtl>(kMaxTeams);
   d.team_hist = A.alloc<uint32_t>((size_t)kMaxTeamBlocks * 256);
@@ -473,7 +476,7 @@ bb_status build(bb_batch* b) {
     for (auto& bd : hw.bodies)
       if (BB_SHAPE_KIND(bd.shape) == BB_SHAPE_VOXEL) hw.has_voxel = true;
     // a pair with a voxel body takes six slots, one per contact-face code
-    hw.pair_cap = pair_capacity(hw.n) * (hw.has_voxel ? 6u : 1u) * hw.pair_scale;
+    hw.pair_cap = pair_capacity(hw.n) * (hw.has_voxel ? 6u : 1u) * hw.pair_scale * b->pair_scale;
     NB += hw.n;
     NA += hw.a;
     NP += hw.pair_cap;
@@ -620,11 +623,11 @@ int lanes_of(const bb_batch* b, const Dev& d, Lane* out) {
     uint32_t mx = 0;
     for (auto& hw : b->worlds) mx = std::max(mx, (uint32_t)hw.bodies.size());
     dw.pgs_lcap = std::min(mx, b->lane_lds);
-    out[nl++] = Lane{dw, b->lane_block, nw};
+    out[nl++] = Lane{dw, b->lane_block, nw, b->fused != 0};
   }
   Dev rest = d;
   rest.order = d.order + nw;
-  out[nl++] = Lane{rest, pick_block(b), d.W - nw};
+  out[nl++] = Lane{rest, pick_block(b), d.W - nw, b->fused != 0};
   return nl;
 }
 
@@ -782,6 +785,7 @@ bb_status profile_batch(bb_batch* b, uint32_t nsteps, float* step_ms, float* ker
   }
   if (kernel_ms)
     for (int k = 0; k < BB_KERNEL_COUNT; ++k) kernel_ms[k] = 0.0f;
+  for (int k = 0; k < BB_BUILD_COUNT; ++k) b->build_ms[k] = 0.0f;
   unsigned long long* dcounts = nullptr;
   if (counts && nsteps &&
       hipMalloc(&dcounts, (size_t)nsteps * BB_COUNT_FIELDS * sizeof(unsigned long long)) != hipSuccess)
@@ -845,8 +849,9 @@ bb_status profile_batch(bb_batch* b, uint32_t nsteps, float* step_ms, float* ker
       unsigned long long span = 0;
       for (uint32_t j = 0; j < nt; ++j) {
         const unsigned long long* m = c + (size_t)j * kTeamMarks;
-        if (m[kTeamMarks - 1] > m[0] && m[kTeamMarks - 1] - m[0] > span) {
-          span = m[kTeamMarks - 1] - m[0];
+        const unsigned long long last = m[kTeamStageMarks - 1];
+        if (last > m[0] && last - m[0] > span) {
+          span = last - m[0];
           slow = j;
         }
       }
@@ -854,7 +859,14 @@ bb_status profile_batch(bb_batch* b, uint32_t nsteps, float* step_ms, float* ker
       const unsigned long long* m = c + (size_t)slow * kTeamMarks;
       const float ms_per_tick = 1.0f / (float)khz;
       kernel_ms[0] = std::max(0.0f, kernel_ms[0] - (float)span * ms_per_tick);
-      for (uint32_t k = 0; k + 1 < kTeamMarks; ++k) kernel_ms[k] += (float)(m[k + 1] - m[k]) * ms_per_tick;
+      for (uint32_t k = 0; k + 1 < kTeamStageMarks; ++k) kernel_ms[k] += (float)(m[k + 1] - m[k]) * ms_per_tick;
+      // inside the build stage (a world without bodies leaves these unset)
+      const unsigned long long sub[BB_BUILD_COUNT + 1] = {m[0], m[kMarkKeys], m[kMarkSort], m[kMarkHierarchy],
+                                                          m[kMarkBounds], m[1]};
+      bool ordered = true;
+      for (int k = 0; k < BB_BUILD_COUNT; ++k) ordered = ordered && sub[k + 1] >= sub[k];
+      if (ordered)
+        for (int k = 0; k < BB_BUILD_COUNT; ++k) b->build_ms[k] += (float)(sub[k + 1] - sub[k]) * ms_per_tick;
     }
   }
   if (dcounts) {
@@ -1162,6 +1174,26 @@ bb_status set_param(bb_batch* b, bb_param p, uint32_t value) {
   } else if (p == BB_PARAM_TEAM_SPIN) {
     b->team_spin = value;
     b->base.team_spin = value;
+  } else if (p == BB_PARAM_AVBD_LANES) {
+    if (value != 0 && value != 1 && value != 2 && value != 4 && value != 8) return BB_ERR_INVALID;
+    b->avbd_lanes = b->base.avbd_lanes = value;
+  } else if (p == BB_PARAM_PGS_PREFETCH) {
+    if (value > 1) return BB_ERR_INVALID;
+    b->pgs_prefetch = b->base.pgs_prefetch = value;
+  } else if (p == BB_PARAM_FUSED) {
+    if (value > 1) return BB_ERR_INVALID;
+    b->fused = value;
+  } else if (p == BB_PARAM_SORT_BLOCKS) {
+    if (value > kMaxTeamBlocks) return BB_ERR_INVALID;
+    b->sort_blocks = b->base.sort_blocks = value;
+  } else if (p == BB_PARAM_PAIR_SCALE) {
+    if (value < 1 || value > 16) return BB_ERR_INVALID;
+    if (value != b->pair_scale) {
+      bb_status st = pull_state(b);  // a new layout, carrying the current state
+      if (st != BB_OK) return st;
+      b->pair_scale = value;
+      b->dirty = true;
+    }
   } else {
     return BB_ERR_INVALID;
   }
@@ -1207,8 +1239,29 @@ bb_status autotune_batch(bb_batch* b, uint32_t reps, float* best_ms) {
   hipEvent_t e0 = nullptr, e1 = nullptr;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
+  // one candidate: an untimed first launch, then `reps` timed replays of the
+  // same step from the same state
+  auto measure = [&](float& ms) -> bb_status {
+    b->latest = saved;
+    bb_status r = launch_one(b);
+    if (r != BB_OK) return r;
+    (void)hipEventRecord(e0, b->stream);
+    for (uint32_t k = 0; k < reps && r == BB_OK; ++k) {
+      b->latest = saved;
+      r = launch_one(b);
+    }
+    (void)hipEventRecord(e1, b->stream);
+    if (r != BB_OK) return r;
+    if (hipEventSynchronize(e1) != hipSuccess) return BB_ERR_DEVICE;
+    ms = 0.0f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    ms /= (float)reps;
+    return BB_OK;
+  };
   float best = 3.4e38f;
   uint32_t best_block = saved_block, best_lds = saved_lds;
+  const uint32_t saved_fused = b->fused, saved_sort = b->sort_blocks, saved_lanes = b->avbd_lanes,
+                 saved_prefetch = b->pgs_prefetch;
   bb_status rc = BB_OK;
   // The wide lane is not tuned here: one replayed step early in a run does
   // not show how much the large worlds come to dominate later (over the
@@ -1219,21 +1272,8 @@ bb_status autotune_batch(bb_batch* b, uint32_t reps, float* best_ms) {
       b->pgs_lds = lds;
       b->base.pgs_lcap = pgs_lds_rows(b);
       if ((rc = upload_teams(b)) != BB_OK) break;
-      b->latest = saved;
-      if ((rc = launch_one(b)) != BB_OK) break;  // untimed first launch
-      (void)hipEventRecord(e0, b->stream);
-      for (uint32_t r = 0; r < reps && rc == BB_OK; ++r) {
-        b->latest = saved;
-        rc = launch_one(b);
-      }
-      (void)hipEventRecord(e1, b->stream);
-      if (rc != BB_OK || hipEventSynchronize(e1) != hipSuccess) {
-        rc = rc != BB_OK ? rc : BB_ERR_DEVICE;
-        break;
-      }
       float ms = 0.0f;
-      (void)hipEventElapsedTime(&ms, e0, e1);
-      ms /= (float)reps;
+      if ((rc = measure(ms)) != BB_OK) break;
       if (ms < best) {
         best = ms;
         best_block = blk;
@@ -1241,6 +1281,56 @@ bb_status autotune_batch(bb_batch* b, uint32_t reps, float* best_ms) {
       }
     }
     if (rc != BB_OK) break;
+  }
+  // then, one at a time from the best so far: one launch per step or per
+  // stage; the solver tile of the per-world AVBD sweeps (lanes per body)
+  // when a world runs AVBD in them; the sort's sub-team when a world runs
+  // as a workgroup team
+  bool any_avbd = false;
+  for (const WorldInfo& wi : b->infos)
+    if (wi.solver == BB_SOLVER_AVBD && !(wi.mode & kModeTeam) && wi.n) any_avbd = true;
+  if (rc == BB_OK) {
+    b->block = best_block;
+    b->pgs_lds = best_lds;
+    b->base.pgs_lcap = pgs_lds_rows(b);
+    rc = upload_teams(b);
+  }
+  // `field` is the batch's setting, `mirror` the copy the kernels read
+  auto tune = [&](uint32_t& field, uint32_t* mirror, std::initializer_list<uint32_t> values) {
+    uint32_t keep = field;
+    for (uint32_t v : values) {
+      if (rc != BB_OK) return;
+      if (v == keep) continue;  // measured already, as the best so far
+      field = v;
+      if (mirror) *mirror = v;
+      float ms = 0.0f;
+      if ((rc = measure(ms)) != BB_OK) break;
+      if (ms < best) {
+        best = ms;
+        keep = v;
+      }
+    }
+    field = keep;
+    if (mirror) *mirror = keep;
+  };
+  tune(b->fused, nullptr, {0u, 1u});
+  bool any_pgs = false;
+  for (const WorldInfo& wi : b->infos)
+    if (wi.solver != BB_SOLVER_AVBD && wi.n) any_pgs = true;
+  if (any_pgs) tune(b->pgs_prefetch, &b->base.pgs_prefetch, {0u, 1u});
+  if (any_avbd) {
+    if (b->avbd_lanes == 0) b->avbd_lanes = b->base.avbd_lanes = 4;
+    tune(b->avbd_lanes, &b->base.avbd_lanes, {1u, 2u, 4u, 8u});
+  }
+  if (b->base.team_grid) {
+    if (b->sort_blocks == 0) b->sort_blocks = b->base.sort_blocks = kTeamSortBlocks;
+    tune(b->sort_blocks, &b->base.sort_blocks, {8u, 16u, 32u, 64u, 128u});
+  }
+  if (rc != BB_OK) {
+    b->fused = saved_fused;
+    b->sort_blocks = b->base.sort_blocks = saved_sort;
+    b->avbd_lanes = b->base.avbd_lanes = saved_lanes;
+    b->pgs_prefetch = b->base.pgs_prefetch = saved_prefetch;
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
@@ -1858,6 +1948,12 @@ bb_status bb_batch_profile(bb_batch* b, uint32_t nsteps, float* step_ms, float* 
   return b ? profile_batch(b, nsteps, step_ms, kernel_ms, counts) : BB_ERR_INVALID;
 }
 
+bb_status bb_batch_build_split(const bb_batch* b, float* build_ms) {
+  if (!b || !build_ms) return BB_ERR_INVALID;
+  for (int k = 0; k < BB_BUILD_COUNT; ++k) build_ms[k] = b->build_ms[k];
+  return BB_OK;
+}
+
 bb_status bb_batch_energy(bb_batch* b, double* per_world, uint32_t n) {
   return b ? energy_batch(b, per_world, n) : BB_ERR_INVALID;
 }
@@ -1895,6 +1991,11 @@ uint32_t bb_batch_get_param(const bb_batch* b, bb_param p) {
   if (p == BB_PARAM_LANE_BLOCK) return b->lane_block;
   if (p == BB_PARAM_LANE_LDS_ROWS) return b->lane_lds;
   if (p == BB_PARAM_TEAM_SPIN) return b->team_spin;
+  if (p == BB_PARAM_PAIR_SCALE) return b->pair_scale;
+  if (p == BB_PARAM_FUSED) return b->fused;
+  if (p == BB_PARAM_PGS_PREFETCH) return b->pgs_prefetch;
+  if (p == BB_PARAM_AVBD_LANES) return b->avbd_lanes ? b->avbd_lanes : 4u;
+  if (p == BB_PARAM_SORT_BLOCKS) return b->sort_blocks ? b->sort_blocks : kTeamSortBlocks;
   return 0;
 }
 

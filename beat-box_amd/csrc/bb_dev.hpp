@@ -65,21 +65,35 @@ struct VoxShape {  // 48 bytes
 static constexpr uint32_t kMaxTeamBlocks = 2048;
 // Threads per workgroup of k_team (independent of the per-world kernels').
 static constexpr uint32_t kTeamBlock = 512;
+// The radix sort runs on the first workgroups of the team only (a sub-team,
+// sort_blocks of them), with a barrier of its own (scount, sgen); `sorted`
+// hands the side the sorted keys ended on to the rest of the team.
 struct TeamCtl {
   uint32_t count;
   uint32_t gen;
   uint32_t error;
   uint32_t diff;
   uint32_t exited;
+  uint32_t scount;
+  uint32_t sgen;
+  uint32_t sorted;
   uint32_t part[kMaxTeamBlocks];
 };
+// Workgroups of a team that run its radix sort when the launch parameter is
+// 0: fewer than the team's own, so the sort's barriers stay cheap.
+static constexpr uint32_t kTeamSortBlocks = 32;
 // Team j steps world world[j] with workgroups [first[j], first[j+1]) of the
 // k_team launch; workgroups from first[count] on have nothing to do.
 static constexpr uint32_t kMaxTeams = 256, kTeamSharedBytes = 4096;
 // Stage clock marks of a team step (profiling): start, then the end of the
 // build, broad, narrow (with contacts), post, PGS (with levels), AVBD and
-// finish stages.
-static constexpr uint32_t kTeamMarks = 8;
+// finish stages (marks 0-7, kTeamStageMarks); then marks inside the build
+// stage: Morton keys done, sort done, LBVH hierarchy done, the teamwide part
+// of the bounds pass done (the reorder and the lead workgroup's climb of the
+// top nodes follow, up to mark 1).
+static constexpr uint32_t kTeamStageMarks = 8;
+static constexpr uint32_t kMarkKeys = 8, kMarkSort = 9, kMarkHierarchy = 10, kMarkBounds = 11;
+static constexpr uint32_t kTeamMarks = 12;
 struct TeamTable {
   uint32_t count;
   uint32_t first[kMaxTeams + 1];
@@ -206,6 +220,9 @@ struct Dev {
   char* team_shared;
   uint32_t team_grid;     // workgroups of the k_team launch
   uint32_t team_spin;     // spins before a team barrier wait gives up (0: default)
+  uint32_t sort_blocks;   // workgroups of a team that sort (0: kTeamSortBlocks)
+  uint32_t avbd_lanes;    // lanes per body in the per-world AVBD primal sweeps (1, 2, 4, 8; 0: 4)
+  uint32_t pgs_prefetch;  // 1: the colour sweeps fetch each thread's next-colour row early
   // set (bit 0) by any world whose step flags an overflow; the host reads
   // and clears it
   uint32_t* ovf_any;
@@ -222,6 +239,9 @@ struct Dev {
 struct Lane {
   Dev d;
   uint32_t block, grid;
+  // the whole step of each world in one launch (k_fused) instead of one
+  // launch per stage
+  bool fused = false;
 };
 
 }  // namespace bb

@@ -155,6 +155,11 @@ struct bb_batch {
   uint32_t lane_block = 512;
   uint32_t lane_lds = 1024;  // BB_PARAM_LANE_LDS_ROWS: LDS rows of k_pgs in that lane
   uint32_t team_spin = 0;    // BB_PARAM_TEAM_SPIN: spins of a team barrier wait (0: default)
+  uint32_t pair_scale = 1;   // BB_PARAM_PAIR_SCALE: pair slots per world, times the default
+  uint32_t fused = 0;        // BB_PARAM_FUSED: each lane's step as one k_fused launch
+  uint32_t sort_blocks = 0;  // BB_PARAM_SORT_BLOCKS (0: kTeamSortBlocks)
+  uint32_t avbd_lanes = 0;   // BB_PARAM_AVBD_LANES (0: 4)
+  uint32_t pgs_prefetch = 0; // BB_PARAM_PGS_PREFETCH
   uint32_t n_wide = 0;  // worlds in that lane: the first n_wide of the order table
   hipStream_t stream2 = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
@@ -163,6 +168,9 @@ struct bb_batch {
   std::vector<bb::WorldInfo> infos;
   hipGraphExec_t graph = nullptr;
   int graph_parity = -1;
+  // bb_batch_build_split: the build stage of the slowest team, split by the
+  // team's clock marks, summed over the last bb_batch_profile call's steps
+  float build_ms[BB_BUILD_COUNT] = {};
   // voxel shape pools shared by the worlds (host copies; the device arrays
   // are allocated at their maximum sizes and refreshed when vox_dirty)
   std::vector<bb::VoxShape> vox_shapes;

@@ -1,28 +1,39 @@
 // bb_server.cpp — serving loop over a capacity-allocated batch.
 //
-// The device arrays are allocated once for `max_worlds` slots and the body,
+// The device arrays are allocated once for the world slots and the body,
 // active and pair capacities that the HBM budget holds.  A world occupies one
 // slot and one contiguous range in each arena (first fit); the kernels find
 // it through its slot's WorldInfo, so admitting or retiring a world only
 // rewrites its table entry and its ranges.  Empty slots have n = 0 and their
-// workgroups return at once, so the step launch (one hipGraph per parity over
-// all slots) never changes shape.
+// workgroups return at once, so a step launch (one hipGraph per parity over a
+// slot range) never changes shape.
 //
-// Host/device pipelining: per server step the host stages new worlds' images
-// and table entries in a pinned chunk, enqueues their uploads, the step and
-// the read-back of retiring worlds on one stream, and records an event.  A
-// chunk is reused only after its step's event has completed, and at most
-// kAhead steps are in flight.  A retired world's ranges are handed out again
-// at once: the new owner's uploads are enqueued after the read-back, so
-// stream order protects the old contents.
+// Pools: the slots are split into up to three pools by world size (small
+// worlds, one workgroup each; mid-sized worlds, one larger workgroup each;
+// large worlds, stepped by workgroup teams).  Each pool has its own slot
+// range, queue, stream, graphs, parity and pipeline, and steps at its own
+// cadence: a small world's step latency is its own pool's kernels' time, not
+// that of the slowest resident world.  The arenas and the HBM budget are
+// shared, and every pool touches only its own worlds' rows, so pools on
+// different streams never write the same memory.
+//
+// Host/device pipelining, per pool: per pool step the host stages new
+// worlds' images and table entries in a pinned chunk, enqueues their uploads,
+// the step and the read-back of retiring worlds on the pool's stream, and
+// records an event.  A chunk is reused only after its step's event has
+// completed, and at most kAhead steps of a pool are in flight.  A retired
+// world's ranges are handed out again at once: an upload into them by another
+// pool waits for the read-back's event first (same pool: stream order).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cstring>
 #include <deque>
 #include <map>
 #include <new>
 #include <unordered_map>
+#include <thread>
 #include <vector>
 
 #include "bb_host.hpp"
@@ -156,6 +167,29 @@ struct Result {
   bool failed = false;
 };
 
+// A pool: the worlds whose body count lies in [lo, hi), in slots [w0, w1).
+struct Pool {
+  uint32_t lo = 0, hi = 0xFFFFFFFFu;
+  uint32_t w0 = 0, w1 = 0;
+  uint32_t block = 256;  // per-world workgroup size
+  uint32_t lcap = 512;   // PGS LDS rows of its per-world kernels
+  bool teams = false;    // the pool launches k_team
+  hipStream_t stream = nullptr;
+  std::deque<Request> queue;
+  std::vector<uint32_t> free_slots;
+  std::deque<StepRecord> inflight;
+  Chunk chunks[kChunks];
+  hipGraphExec_t graph[2] = {nullptr, nullptr};
+  int latest = 1;  // parity holding the newest rows of this pool's worlds
+  uint64_t step_counter = 0;
+  bool table_dirty = false;
+  hipEvent_t last_done = nullptr;
+  bool have_prev_done = false;
+  std::vector<std::pair<float, uint64_t>> step_samples;  // (ms, world-steps)
+  uint64_t world_steps = 0;
+  uint32_t running = 0;
+};
+
 }  // namespace
 
 struct bb_server {
@@ -165,25 +199,22 @@ struct bb_server {
   uint64_t cW = 0, cB = 0, cA = 0, cP = 0;  // device bytes per slot/body/active/pair slot
   RangeAlloc rb, ra, rp;
   std::vector<Slot> slots;
-  std::vector<uint32_t> free_slots;
-  std::deque<Request> queue;
+  std::vector<Pool> pools;
   std::unordered_map<uint64_t, Result> results;
   std::unordered_map<uint64_t, uint32_t> running;  // ticket -> slot
   // requests on the device or in a read-back, kept to run again on overflow
   std::unordered_map<uint64_t, Request> origin;
+  // ranges freed by a read-back still in flight on a pool's stream: another
+  // pool that uploads into them waits for that event (hipStreamWaitEvent)
+  std::vector<std::pair<uint32_t, hipEvent_t>> released;  // (pool, its newest step event)
   uint64_t reruns = 0;
-  std::deque<StepRecord> inflight;
   std::vector<hipEvent_t> spare_events;
-  Chunk chunks[kChunks];
-  hipGraphExec_t graph[2] = {nullptr, nullptr};
   uint64_t next_ticket = 1, step_counter = 0;
   uint64_t bytes_in_use = 0, resident_bodies = 0;
   uint64_t submitted = 0, admitted_n = 0, retired_n = 0;
   uint64_t world_steps = 0, body_steps = 0;
-  bool table_dirty = false;  // a retirement changed the world table
-  hipEvent_t first = nullptr, last_done = nullptr;
-  bool have_prev_done = false;
-  std::vector<std::pair<float, uint64_t>> step_samples;  // (ms, world-steps)
+  hipEvent_t first = nullptr;
+  bool started = false;
   std::vector<double> queue_samples;
   double device_ms = 0.0;
 };
@@ -205,6 +236,12 @@ uint64_t price(const bb_server* s, uint32_t n, uint32_t a, uint32_t pair_scale =
   return s->cW + s->cB * n + s->cA * a + s->cP * (uint64_t)pair_capacity(n) * pair_scale;
 }
 
+uint32_t pool_of(const bb_server* s, uint32_t n) {
+  for (uint32_t p = 0; p < (uint32_t)s->pools.size(); ++p)
+    if (n >= s->pools[p].lo && n < s->pools[p].hi) return p;
+  return (uint32_t)s->pools.size() - 1;
+}
+
 void to_states(const Readback& r, Result& res) {
   res.bodies.resize(r.n);
   for (uint32_t row = 0; row < r.n; ++row) {
@@ -224,30 +261,31 @@ void to_states(const Readback& r, Result& res) {
   stats_from_raw(r.stats, r.step_index, &res.stats);
 }
 
-// Completes the oldest in-flight step: latency samples, results.
-bb_status finish_oldest(bb_server* s) {
-  StepRecord& rec = s->inflight.front();
+// Completes the oldest in-flight step of pool p: latency samples, results.
+bb_status finish_oldest(bb_server* s, uint32_t p) {
+  Pool& P = s->pools[p];
+  StepRecord& rec = P.inflight.front();
   HIPCHK(hipEventSynchronize(rec.done));
   float ms = 0.0f;
-  if (s->have_prev_done) {
-    HIPCHK(hipEventElapsedTime(&ms, s->last_done, rec.done));
-    if (rec.carried) s->step_samples.emplace_back(ms, rec.carried);
+  if (P.have_prev_done) {
+    HIPCHK(hipEventElapsedTime(&ms, P.last_done, rec.done));
+    if (rec.carried) P.step_samples.emplace_back(ms, rec.carried);
   }
   if (rec.admitted) {
     float am = 0.0f;
     HIPCHK(hipEventElapsedTime(&am, rec.admit, rec.done));
-    s->step_samples.emplace_back(am, rec.admitted);
+    P.step_samples.emplace_back(am, rec.admitted);
   }
   float total = 0.0f;
   HIPCHK(hipEventElapsedTime(&total, s->first, rec.done));
-  s->device_ms = total;
+  s->device_ms = std::max(s->device_ms, (double)total);
   for (const Readback& r : rec.readbacks) {
     auto oit = s->origin.find(r.ticket);
     const uint32_t ovf = r.stats[ST_OVF_STICKY];
     if (ovf && oit != s->origin.end() && oit->second.reruns < kMaxReruns) {
       // a step of this world dropped pairs or was left unfinished by its
-      // team: run the request again from its first step, ahead of the queue,
-      // if it fits the budget and the pair arena that way
+      // team: run the request again from its first step, ahead of its pool's
+      // queue, if it fits the budget and the pair arena that way
       const Request& o = oit->second;
       const uint32_t scale = (ovf & 1u) ? o.pair_scale * 2u : o.pair_scale;
       const uint32_t nb = (uint32_t)o.bodies.size();
@@ -260,7 +298,7 @@ bb_status finish_oldest(bb_server* s) {
         ++q.reruns;
         ++s->reruns;
         q.bytes = bytes;
-        s->queue.push_front(std::move(q));
+        P.queue.push_front(std::move(q));
         continue;
       }
     }
@@ -269,44 +307,57 @@ bb_status finish_oldest(bb_server* s) {
     res.failed = ovf != 0;
     if (oit != s->origin.end()) s->origin.erase(oit);
   }
-  if (s->have_prev_done) s->spare_events.push_back(s->last_done);
-  s->last_done = rec.done;
-  s->have_prev_done = true;
+  // read-backs of this pool whose event is complete no longer hold anyone up
+  for (size_t i = 0; i < s->released.size();)
+    if (s->released[i].second == rec.done) s->released.erase(s->released.begin() + i);
+    else ++i;
+  if (P.have_prev_done) s->spare_events.push_back(P.last_done);
+  P.last_done = rec.done;
+  P.have_prev_done = true;
   s->spare_events.push_back(rec.admit);
-  s->chunks[rec.chunk].used = 0;
-  s->inflight.pop_front();
+  P.chunks[rec.chunk].used = 0;
+  P.inflight.pop_front();
   return BB_OK;
 }
 
-bb_status ensure_graphs(bb_server* s) {
-  for (int p = 0; p < 2; ++p) {
-    if (s->graph[p]) continue;
+// The step graphs of pool p (one per parity): k_team first when the pool has
+// teams, then the per-world stage kernels over its slots in its order.
+bb_status ensure_graphs(bb_server* s, uint32_t p) {
+  Pool& P = s->pools[p];
+  for (int par = 0; par < 2; ++par) {
+    if (P.graph[par]) continue;
     hipGraph_t g;
-    HIPCHK(hipStreamBeginCapture(s->b->stream, hipStreamCaptureModeThreadLocal));
-    const Dev d = dev_for(s->b, p);
-    launch_step(Lane{d, s->b->block, d.W}, s->b->stream, true);
-    HIPCHK(hipStreamEndCapture(s->b->stream, &g));
-    HIPCHK(hipGraphInstantiate(&s->graph[p], g, nullptr, nullptr, 0));
+    HIPCHK(hipStreamBeginCapture(P.stream, hipStreamCaptureModeThreadLocal));
+    Dev d = dev_for(s->b, par);
+    d.order = d.order + P.w0;
+    d.pgs_lcap = P.lcap;
+    if (!P.teams) d.team_grid = 0;
+    launch_step(Lane{d, P.block, P.w1 - P.w0}, P.stream, true);
+    HIPCHK(hipStreamEndCapture(P.stream, &g));
+    HIPCHK(hipGraphInstantiate(&P.graph[par], g, nullptr, nullptr, 0));
     HIPCHK(hipGraphDestroy(g));
   }
   return BB_OK;
 }
 
-bb_status server_step(bb_server* s) {
+// One step of pool p: admission from its queue, the step of its slots,
+// read-back and release of its worlds that finish with it.
+bb_status pool_step(bb_server* s, uint32_t p) {
   bb_batch* b = s->b;
-  while ((int)s->inflight.size() >= kAhead) {
-    bb_status st = finish_oldest(s);
+  Pool& P = s->pools[p];
+  while ((int)P.inflight.size() >= kAhead) {
+    bb_status st = finish_oldest(s, p);
     if (st != BB_OK) return st;
   }
-  const int chunk = (int)(s->step_counter % kChunks);
-  Chunk& ch = s->chunks[chunk];
+  const int chunk = (int)(P.step_counter % kChunks);
+  Chunk& ch = P.chunks[chunk];
   // ---- admission: first come first served while the budget, the ranges,
   // a slot and the resident-body cap allow
   std::vector<uint32_t> admitted;
   size_t stage_bytes = 0;
   const double t_now = now_ms();
-  while (!s->queue.empty() && !s->free_slots.empty()) {
-    Request& q = s->queue.front();
+  while (!P.queue.empty() && !P.free_slots.empty()) {
+    Request& q = P.queue.front();
     const uint32_t n = (uint32_t)q.bodies.size();
     if (s->bytes_in_use + q.bytes > s->desc.hbm_budget_bytes) break;
     if (s->desc.max_resident_bodies && s->resident_bodies > 0 &&
@@ -324,8 +375,8 @@ bb_status server_step(bb_server* s) {
       s->ra.release(ao, q.active);
       break;
     }
-    const uint32_t w = s->free_slots.back();
-    s->free_slots.pop_back();
+    const uint32_t w = P.free_slots.back();
+    P.free_slots.pop_back();
     Slot& sl = s->slots[w];
     sl = Slot{};
     sl.used = true;
@@ -354,6 +405,7 @@ bb_status server_step(bb_server* s) {
     s->bytes_in_use += q.bytes;
     s->resident_bodies += n;
     s->running[q.ticket] = w;
+    ++P.running;
     if (!q.reruns) {
       s->queue_samples.push_back(t_now - q.submit_ms);
       ++s->admitted_n;
@@ -362,23 +414,28 @@ bb_status server_step(bb_server* s) {
     stage_bytes += kImageBodyBytes * std::max(n, 1u) + kImageActiveBytes * std::max(q.active, 1u) + 64;
     admitted.push_back(w);
     s->origin[tk] = std::move(q);
-    s->queue.pop_front();
+    P.queue.pop_front();
   }
   // ---- read-back sizes of the worlds that finish with this step
   std::vector<uint32_t> retiring;
-  for (uint32_t w = 0; w < s->W; ++w) {
+  for (uint32_t w = P.w0; w < P.w1; ++w) {
     const Slot& sl = s->slots[w];
     if (sl.used && sl.done + 1 == sl.steps) {
       retiring.push_back(w);
       stage_bytes += (size_t)sl.n * (4 * 3 + 16 * 4) + ST_COUNT * 4 + 128;
     }
   }
-  stage_bytes += (sizeof(WorldInfo) + sizeof(uint32_t)) * (size_t)s->W + sizeof(TeamTable) + 96;
+  stage_bytes += (sizeof(WorldInfo) + sizeof(uint32_t)) * (size_t)(P.w1 - P.w0) + sizeof(TeamTable) + 96;
   bb_status st = ch.reserve(stage_bytes + 4096);
   if (st != BB_OK) return st;
   ch.used = 0;
+  hipStream_t stream = P.stream;
+  // ---- ranges another pool freed this recently may still be read back on
+  // that pool's stream: order the uploads after those read-backs
+  if (!admitted.empty())
+    for (const auto& r : s->released)
+      if (r.first != p) HIPCHK(hipStreamWaitEvent(stream, r.second, 0));
   // ---- uploads of the admitted worlds and their table entries
-  hipStream_t stream = b->stream;
   for (uint32_t w : admitted) {
     HostWorld& hw = b->worlds[w];
     const uint32_t n = hw.n, a = hw.a;
@@ -396,22 +453,27 @@ bb_status server_step(bb_server* s) {
     hw.bodies.shrink_to_fit();
   }
   // ---- world table: after an admission or a retirement, choose the team
-  // worlds again and upload the whole table (and the team table)
-  if (!admitted.empty() || s->table_dirty) {
-    if (b->base.team_grid) {
+  // worlds again and upload this pool's part of the table (and the team
+  // table, which only a pool with teams uses)
+  if (!admitted.empty() || P.table_dirty) {
+    if (P.teams && b->base.team_grid) {
       assign_teams(b, b->base.team_grid);
       TeamTable* tt = (TeamTable*)ch.take(sizeof(TeamTable));
       *tt = b->team_table;
       HIPCHK(hipMemcpyAsync(b->base.teams, tt, sizeof(TeamTable), hipMemcpyHostToDevice, stream));
       HIPCHK(hipMemsetAsync(b->base.team_ctl, 0, sizeof(TeamCtl) * kMaxTeams, stream));
     }
-    WorldInfo* tab = (WorldInfo*)ch.take(sizeof(WorldInfo) * s->W);
-    std::memcpy(tab, b->infos.data(), sizeof(WorldInfo) * s->W);
-    HIPCHK(hipMemcpyAsync(b->base.worlds, tab, sizeof(WorldInfo) * s->W, hipMemcpyHostToDevice, stream));
-    uint32_t* ord = (uint32_t*)ch.take(sizeof(uint32_t) * s->W);
-    world_order(b->infos, ord);
-    HIPCHK(hipMemcpyAsync((void*)b->base.order, ord, sizeof(uint32_t) * s->W, hipMemcpyHostToDevice, stream));
-    s->table_dirty = false;
+    const uint32_t nw = P.w1 - P.w0;
+    WorldInfo* tab = (WorldInfo*)ch.take(sizeof(WorldInfo) * nw);
+    std::memcpy(tab, b->infos.data() + P.w0, sizeof(WorldInfo) * nw);
+    HIPCHK(hipMemcpyAsync(b->base.worlds + P.w0, tab, sizeof(WorldInfo) * nw, hipMemcpyHostToDevice, stream));
+    uint32_t* ord = (uint32_t*)ch.take(sizeof(uint32_t) * nw);
+    const std::vector<WorldInfo> part(b->infos.begin() + P.w0, b->infos.begin() + P.w1);
+    world_order(part, ord);
+    for (uint32_t i = 0; i < nw; ++i) ord[i] += P.w0;
+    HIPCHK(hipMemcpyAsync((void*)(b->base.order + P.w0), ord, sizeof(uint32_t) * nw, hipMemcpyHostToDevice,
+                          stream));
+    P.table_dirty = false;
   }
   StepRecord rec;
   rec.chunk = chunk;
@@ -419,16 +481,18 @@ bb_status server_step(bb_server* s) {
   rec.done = take_event(s);
   if (!rec.admit || !rec.done) return BB_ERR_DEVICE;
   HIPCHK(hipEventRecord(rec.admit, stream));
-  if (s->step_counter == 0) HIPCHK(hipEventRecord(s->first, stream));
-  // ---- the step: every slot, one launch sequence
-  st = ensure_graphs(s);
+  if (!s->started) {
+    HIPCHK(hipEventRecord(s->first, stream));
+    s->started = true;
+  }
+  // ---- the step: every slot of the pool, one launch sequence
+  st = ensure_graphs(s, p);
   if (st != BB_OK) return st;
-  const int cur = b->latest ^ 1;
-  HIPCHK(hipGraphLaunch(s->graph[cur], stream));
-  b->latest = cur;
-  ++b->step_index;
+  const int cur = P.latest ^ 1;
+  HIPCHK(hipGraphLaunch(P.graph[cur], stream));
+  P.latest = cur;
   // ---- per-world bookkeeping; read-back and release of finished worlds
-  for (uint32_t w = 0; w < s->W; ++w) {
+  for (uint32_t w = P.w0; w < P.w1; ++w) {
     Slot& sl = s->slots[w];
     if (!sl.used) continue;
     if (sl.fresh) ++rec.admitted;
@@ -466,27 +530,86 @@ bb_status server_step(bb_server* s) {
     // the slot becomes empty for the next step; its ranges are free now
     // (later uploads into them are ordered after this read-back)
     b->infos[w] = WorldInfo{};
-    s->table_dirty = true;
+    P.table_dirty = true;
     s->rb.release(sl.body_off, sl.n);
     s->ra.release(sl.act_off, sl.a);
     s->rp.release(sl.pair_off, sl.pair_cap);
     s->bytes_in_use -= sl.bytes;
     s->resident_bodies -= sl.n;
     s->running.erase(sl.ticket);
+    --P.running;
     b->worlds[w] = HostWorld{};
     sl = Slot{};
-    s->free_slots.push_back(w);
+    P.free_slots.push_back(w);
     ++s->retired_n;
   }
   HIPCHK(hipEventRecord(rec.done, stream));
+  if (!retiring.empty() && s->pools.size() > 1) s->released.emplace_back(p, rec.done);
+  P.world_steps += rec.carried + rec.admitted;
   s->world_steps += rec.carried + rec.admitted;
   s->body_steps += rec.bodies;
-  s->inflight.push_back(std::move(rec));
+  P.inflight.push_back(std::move(rec));
+  ++P.step_counter;
   ++s->step_counter;
   return BB_OK;
 }
 
-bool idle(const bb_server* s) { return s->queue.empty() && s->running.empty(); }
+bool pool_busy(const Pool& P) { return !P.queue.empty() || P.running > 0; }
+
+bool idle(const bb_server* s) {
+  for (const Pool& P : s->pools)
+    if (pool_busy(P)) return false;
+  return true;
+}
+
+bool any_inflight(const bb_server* s) {
+  for (const Pool& P : s->pools)
+    if (!P.inflight.empty()) return true;
+  return false;
+}
+
+// Completes the in-flight steps that have finished, without blocking.
+bb_status collect(bb_server* s) {
+  for (uint32_t p = 0; p < (uint32_t)s->pools.size(); ++p) {
+    Pool& P = s->pools[p];
+    while (!P.inflight.empty() && hipEventQuery(P.inflight.front().done) == hipSuccess) {
+      bb_status st = finish_oldest(s, p);
+      if (st != BB_OK) return st;
+    }
+  }
+  return BB_OK;
+}
+
+// One server round: every pool with work and room in its pipeline steps
+// once; when no pool has room, waits until one has.
+bb_status server_round(bb_server* s) {
+  bb_status st = collect(s);
+  if (st != BB_OK) return st;
+  for (;;) {
+    bool issued = false;
+    for (uint32_t p = 0; p < (uint32_t)s->pools.size(); ++p) {
+      Pool& P = s->pools[p];
+      if (!pool_busy(P) || (int)P.inflight.size() >= kAhead) continue;
+      st = pool_step(s, p);
+      if (st != BB_OK) return st;
+      issued = true;
+    }
+    if (issued) return BB_OK;
+    if (s->pools.size() == 1) {
+      Pool& P = s->pools[0];
+      if (!pool_busy(P)) return BB_OK;
+      st = finish_oldest(s, 0);  // one pool: wait for it, as a lockstep loop does
+      if (st != BB_OK) return st;
+      continue;
+    }
+    bool any = false;
+    for (const Pool& P : s->pools) any = any || (pool_busy(P) && !P.inflight.empty());
+    if (!any) return BB_OK;
+    std::this_thread::yield();
+    st = collect(s);
+    if (st != BB_OK) return st;
+  }
+}
 
 double percentile(std::vector<std::pair<float, uint64_t>> v, double q) {
   if (v.empty()) return 0.0;
@@ -513,7 +636,23 @@ bb_status bb_server_create(const bb_server_desc* desc, bb_server** out) {
   bb_server* s = new (std::nothrow) bb_server();
   if (!s) return BB_ERR_CAPACITY;
   s->desc = *desc;
-  s->W = desc->max_worlds;
+  const uint32_t team_bodies = desc->team_bodies == 0 ? 8192u : desc->team_bodies;  // 0xFFFFFFFF: never
+  // pools by body count: [0, split), [split, team), [team, inf) -- the
+  // middle one only when split < team; one pool without a split
+  std::vector<std::pair<uint32_t, uint32_t>> bounds;
+  const uint32_t split = desc->split_bodies;
+  if (split == 0) {
+    bounds.emplace_back(0u, 0xFFFFFFFFu);
+  } else {
+    bounds.emplace_back(0u, split);
+    if (team_bodies != 0xFFFFFFFFu && split < team_bodies) {
+      bounds.emplace_back(split, team_bodies);
+      bounds.emplace_back(team_bodies, 0xFFFFFFFFu);
+    } else {
+      bounds.emplace_back(split, 0xFFFFFFFFu);
+    }
+  }
+  s->W = desc->max_worlds * (uint32_t)bounds.size();
   const uint64_t base = device_bytes(1, 1, 1, 1);
   s->cW = device_bytes(2, 1, 1, 1) - base;
   s->cB = device_bytes(1, 2, 1, 1) - base;
@@ -544,9 +683,25 @@ bb_status bb_server_create(const bb_server_desc* desc, bb_server** out) {
   }
   bb_batch* b = s->b;
   b->block = desc->block ? desc->block : 256;
-  if (desc->team_bodies) b->team_bodies = desc->team_bodies == 0xFFFFFFFFu ? 0u : desc->team_bodies;
+  b->team_bodies = desc->team_bodies == 0xFFFFFFFFu ? 0u : team_bodies;
+  s->pools.resize(bounds.size());
   bb_status st = BB_OK;
-  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) st = BB_ERR_DEVICE;
+  for (uint32_t p = 0; p < (uint32_t)bounds.size() && st == BB_OK; ++p) {
+    Pool& P = s->pools[p];
+    P.lo = bounds[p].first;
+    P.hi = bounds[p].second;
+    P.w0 = p * desc->max_worlds;
+    P.w1 = P.w0 + desc->max_worlds;
+    // one pool: the descriptor's workgroup size for every world; pools: the
+    // small worlds' pool keeps it, the larger ones run 512-thread
+    // workgroups that keep 1,024 rows of body state in LDS
+    P.block = (p == 0 || desc->block) ? b->block : 512u;
+    P.lcap = (p == 0 || desc->block) ? b->pgs_lds : 1024u;
+    P.teams = b->team_bodies != 0 && P.hi > b->team_bodies;
+    for (uint32_t w = P.w1; w > P.w0; --w) P.free_slots.push_back(w - 1);
+    if (hipStreamCreateWithFlags(&P.stream, hipStreamNonBlocking) != hipSuccess) st = BB_ERR_DEVICE;
+  }
+  b->stream = s->pools[0].stream;
   if (st == BB_OK) st = alloc_arrays(b, b->buf, s->W, s->NB, s->NA, s->NP);
   if (st == BB_OK) {
     b->NB = s->NB;
@@ -562,7 +717,7 @@ bb_status bb_server_create(const bb_server_desc* desc, bb_server** out) {
     // the team table in device memory says which workgroups step which world
     b->base.team_grid = b->team_bodies ? std::min(team_capacity() / 2, kMaxTeamBlocks) : 0;
     std::vector<uint32_t> order(s->W);
-    world_order(b->infos, order.data());
+    for (uint32_t w = 0; w < s->W; ++w) order[w] = w;  // empty slots, in slot order
     if (hipMemcpy(b->base.worlds, b->infos.data(), s->W * sizeof(WorldInfo), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy((void*)b->base.order, order.data(), s->W * sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess ||
         hipEventCreate(&s->first) != hipSuccess)
@@ -579,7 +734,6 @@ bb_status bb_server_create(const bb_server_desc* desc, bb_server** out) {
   s->ra.reset(s->NA);
   s->rp.reset(s->NP);
   s->slots.resize(s->W);
-  for (uint32_t w = s->W; w > 0; --w) s->free_slots.push_back(w - 1);
   *out = s;
   return BB_OK;
 }
@@ -602,16 +756,17 @@ bb_status bb_server_submit(bb_server* s, const bb_world_desc* desc, const bb_bod
   q.steps = steps;
   q.active = a;
   q.submit_ms = now_ms();
-  s->queue.push_back(std::move(q));
+  Pool& P = s->pools[pool_of(s, n)];
+  P.queue.push_back(std::move(q));
   ++s->submitted;
-  if (ticket) *ticket = s->queue.back().ticket;
+  if (ticket) *ticket = P.queue.back().ticket;
   return BB_OK;
 }
 
 bb_status bb_server_step(bb_server* s, uint32_t nsteps) {
   if (!s) return BB_ERR_INVALID;
   for (uint32_t k = 0; k < nsteps; ++k) {
-    bb_status st = server_step(s);
+    bb_status st = server_round(s);
     if (st != BB_OK) return st;
   }
   return BB_OK;
@@ -619,23 +774,25 @@ bb_status bb_server_step(bb_server* s, uint32_t nsteps) {
 
 bb_status bb_server_drain(bb_server* s) {
   if (!s) return BB_ERR_INVALID;
-  // a read-back that completes may put its request back in the queue (a
-  // rerun after an overflow), so drain until nothing is left anywhere
-  while (!idle(s) || !s->inflight.empty()) {
-    while (!idle(s)) {
-      if (s->running.empty() && !s->queue.empty() && s->free_slots.size() == s->W) {
+  // a read-back that completes may put its request back in a queue (a rerun
+  // after an overflow), so drain until nothing is left anywhere
+  while (!idle(s) || any_inflight(s)) {
+    for (const Pool& P : s->pools)
+      if (P.running == 0 && !P.queue.empty() && P.inflight.empty() && s->running.empty()) {
         // nothing resident yet the head of the queue does not fit: cannot
         // happen for a request accepted by submit, but never spin on it
-        const Request& q = s->queue.front();
-        if (q.bytes > s->desc.hbm_budget_bytes) return BB_ERR_BUDGET;
+        if (P.queue.front().bytes > s->desc.hbm_budget_bytes) return BB_ERR_BUDGET;
       }
-      bb_status st = server_step(s);
+    if (!idle(s)) {
+      bb_status st = server_round(s);
       if (st != BB_OK) return st;
+      continue;
     }
-    while (!s->inflight.empty()) {
-      bb_status st = finish_oldest(s);
-      if (st != BB_OK) return st;
-    }
+    for (uint32_t p = 0; p < (uint32_t)s->pools.size(); ++p)
+      while (!s->pools[p].inflight.empty()) {
+        bb_status st = finish_oldest(s, p);
+        if (st != BB_OK) return st;
+      }
   }
   return BB_OK;
 }
@@ -644,27 +801,27 @@ bb_status bb_server_poll(bb_server* s, uint64_t ticket, uint32_t* state, bb_body
                          bb_step_stats* stats) {
   if (!s || !state) return BB_ERR_INVALID;
   // results of completed steps become visible without blocking
-  while (!s->inflight.empty() && hipEventQuery(s->inflight.front().done) == hipSuccess) {
-    bb_status st = finish_oldest(s);
-    if (st != BB_OK) return st;
-  }
+  bb_status cst = collect(s);
+  if (cst != BB_OK) return cst;
   auto it = s->results.find(ticket);
   if (it == s->results.end()) {
     if (s->running.count(ticket)) {
       *state = BB_REQ_RUNNING;
       return BB_OK;
     }
-    for (const StepRecord& rec : s->inflight)
-      for (const Readback& r : rec.readbacks)
-        if (r.ticket == ticket) {
-          *state = BB_REQ_RUNNING;
+    for (const Pool& P : s->pools) {
+      for (const StepRecord& rec : P.inflight)
+        for (const Readback& r : rec.readbacks)
+          if (r.ticket == ticket) {
+            *state = BB_REQ_RUNNING;
+            return BB_OK;
+          }
+      for (const Request& q : P.queue)
+        if (q.ticket == ticket) {
+          *state = BB_REQ_QUEUED;
           return BB_OK;
         }
-    for (const Request& q : s->queue)
-      if (q.ticket == ticket) {
-        *state = BB_REQ_QUEUED;
-        return BB_OK;
-      }
+    }
     *state = BB_REQ_UNKNOWN;
     return BB_OK;
   }
@@ -682,7 +839,7 @@ bb_status bb_server_report_get(bb_server* s, bb_server_report* out) {
   out->submitted = s->submitted;
   out->admitted = s->admitted_n;
   out->retired = s->retired_n;
-  out->queued = s->queue.size();
+  for (const Pool& P : s->pools) out->queued += P.queue.size();
   out->resident_worlds = s->running.size();
   out->resident_bodies = s->resident_bodies;
   out->bytes_in_use = s->bytes_in_use;
@@ -692,36 +849,50 @@ bb_status bb_server_report_get(bb_server* s, bb_server_report* out) {
   out->world_steps = s->world_steps;
   out->body_steps = s->body_steps;
   out->device_ms = s->device_ms;
-  out->step_ms_p50 = percentile(s->step_samples, 0.50);
-  out->step_ms_p99 = percentile(s->step_samples, 0.99);
+  std::vector<std::pair<float, uint64_t>> all;
+  for (const Pool& P : s->pools) all.insert(all.end(), P.step_samples.begin(), P.step_samples.end());
+  out->step_ms_p50 = percentile(all, 0.50);
+  out->step_ms_p99 = percentile(all, 0.99);
   std::vector<std::pair<float, uint64_t>> qs;
   qs.reserve(s->queue_samples.size());
   for (double v : s->queue_samples) qs.emplace_back((float)v, 1);
   out->queue_ms_p50 = percentile(qs, 0.50);
   out->queue_ms_p99 = percentile(qs, 0.99);
   out->reruns = s->reruns;
+  out->pools = (uint32_t)std::min<size_t>(s->pools.size(), BB_SERVER_MAX_POOLS);
+  for (uint32_t p = 0; p < out->pools; ++p) {
+    const Pool& P = s->pools[p];
+    out->pool_min_bodies[p] = P.lo;
+    out->pool_world_steps[p] = P.world_steps;
+    out->pool_step_ms_p50[p] = percentile(P.step_samples, 0.50);
+    out->pool_step_ms_p99[p] = percentile(P.step_samples, 0.99);
+  }
   return BB_OK;
 }
 
 void bb_server_destroy(bb_server* s) {
   if (!s) return;
-  if (s->b) {
-    if (s->b->stream) (void)hipStreamSynchronize(s->b->stream);
-    for (auto& g : s->graph)
+  for (Pool& P : s->pools)
+    if (P.stream) (void)hipStreamSynchronize(P.stream);
+  for (Pool& P : s->pools) {
+    for (auto& g : P.graph)
       if (g) (void)hipGraphExecDestroy(g);
+    for (auto& rec : P.inflight) {
+      if (rec.done) (void)hipEventDestroy(rec.done);
+      if (rec.admit) (void)hipEventDestroy(rec.admit);
+    }
+    if (P.have_prev_done && P.last_done) (void)hipEventDestroy(P.last_done);
+    for (auto& c : P.chunks)
+      if (c.mem) (void)hipHostFree(c.mem);
+    if (P.stream) (void)hipStreamDestroy(P.stream);
+  }
+  if (s->b) {
     s->b->buf.release();
-    if (s->b->stream) (void)hipStreamDestroy(s->b->stream);
+    s->b->stream = nullptr;  // one of the pools' streams, destroyed above
     delete s->b;
   }
-  for (auto& rec : s->inflight) {
-    if (rec.done) (void)hipEventDestroy(rec.done);
-    if (rec.admit) (void)hipEventDestroy(rec.admit);
-  }
   for (hipEvent_t e : s->spare_events) (void)hipEventDestroy(e);
-  if (s->have_prev_done && s->last_done) (void)hipEventDestroy(s->last_done);
   if (s->first) (void)hipEventDestroy(s->first);
-  for (auto& c : s->chunks)
-    if (c.mem) (void)hipHostFree(c.mem);
   delete s;
 }
 

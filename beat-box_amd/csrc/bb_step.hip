@@ -255,6 +255,9 @@ struct BlockTeam {
   BB_DI bool radix_sort(uint64_t* ka, uint32_t* ia, uint64_t* kb, uint32_t* ib, uint32_t n, int passes) const {
     return block_radix_sort<BLOCK>(ka, ia, kb, ib, n, passes, *sort_lds);
   }
+  // clock marks inside a stage are taken by workgroup teams only
+  BB_DI void mark(uint32_t) const {}
+  BB_DI bool sorted_in_a(bool in_a) const { return in_a; }
 };
 
 // Spins before a team barrier wait gives up (about a second).  A wait that
@@ -276,6 +279,8 @@ struct GroupTeam {
   uint32_t* stats; // the world's counters (ST_COUNT words)
   uint32_t* ovf_any;
   uint32_t spin;   // spins before a wait gives up
+  unsigned long long* clk;  // this team's clock marks (profiling launches), or null
+  uint32_t sort_blocks;     // the first sort_blocks workgroups run the radix sort
   // AVBD primal: one lane per body, a barrier after every cascade bucket
   static constexpr uint32_t kAvGroup = 1;
   static constexpr bool kSkipEmpty = false;
@@ -283,6 +288,11 @@ struct GroupTeam {
   BB_DI uint32_t rank() const { return blk * BLOCK + threadIdx.x; }
   BB_DI uint32_t size() const { return nblocks * BLOCK; }
   BB_DI void fence() const { __threadfence(); }
+  // Clock mark m of this step, called right after a team barrier: every
+  // workgroup has reached that point when rank 0 stamps it.
+  BB_DI void mark(uint32_t m) const {
+    if (clk && rank() == 0) clk[m] = wall_clock64();
+  }
   // With leaf i handled by rank i mod size(), a tree node over leaves
   // [first, last] inside one BLOCK-aligned run is touched by one workgroup
   // alone: a bottom-up pass orders its steps there with workgroup scope
@@ -299,6 +309,7 @@ struct GroupTeam {
     __threadfence();
     if (atomicAdd(&ctl->exited, 1u) == nblocks - 1u) {
       __hip_atomic_store(&ctl->count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&ctl->scount, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       __hip_atomic_store(&ctl->error, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       __hip_atomic_store(&ctl->exited, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       __threadfence();
@@ -316,20 +327,22 @@ struct GroupTeam {
     }
     __builtin_amdgcn_endpgm();
   }
-  BB_DI void sync() const {
+  // Barrier of `members` workgroups on the arrival counter *cnt and the
+  // generation word *gen.
+  BB_DI void barrier(uint32_t* cnt, uint32_t* gen, uint32_t members) const {
     __syncthreads();
     if (threadIdx.x == 0) {
       uint32_t bad = err();
       if (bad == 0u) {
-        const uint32_t g = __hip_atomic_load(&ctl->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const uint32_t g = __hip_atomic_load(gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __threadfence();
-        if (atomicAdd(&ctl->count, 1u) == nblocks - 1u) {
-          __hip_atomic_store(&ctl->count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (atomicAdd(cnt, 1u) == members - 1u) {
+          __hip_atomic_store(cnt, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
           __threadfence();
-          atomicAdd(&ctl->gen, 1u);
+          atomicAdd(gen, 1u);
         } else {
           uint32_t spins = 0;
-          while (__hip_atomic_load(&ctl->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == g) {
+          while (__hip_atomic_load(gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == g) {
             if (err() != 0u) break;
             if (++spins > spin) {
               atomicOr(&ctl->error, 1u);
@@ -346,15 +359,25 @@ struct GroupTeam {
     __syncthreads();
     if (*dead) abandon();
   }
+  BB_DI void sync() const { barrier(&ctl->count, &ctl->gen, nblocks); }
+  // Barrier of the sort's sub-team, the first `members` workgroups.
+  BB_DI void sync_first(uint32_t members) const { barrier(&ctl->scount, &ctl->sgen, members); }
   // Exclusive scan over the whole team (rank order); two team barriers.
-  BB_DI uint32_t scan(uint32_t v, uint32_t& total) const {
+  BB_DI uint32_t scan(uint32_t v, uint32_t& total) const { return scan_first(v, total, nblocks); }
+  // The same over the first `members` workgroups (rank order), with their
+  // barrier (the whole team's when members = nblocks).
+  BB_DI uint32_t scan_first(uint32_t v, uint32_t& total, uint32_t members) const {
+    auto bar = [&]() {
+      if (members == nblocks) sync();
+      else sync_first(members);
+    };
     uint32_t btot;
     const uint32_t ex = block_scan<BLOCK>(v, *scan_lds, btot);
     if (threadIdx.x == 0) ctl->part[blk] = btot;
-    sync();
+    bar();
     if (threadIdx.x < WAVE) {
       uint32_t pre = 0, all = 0;
-      for (uint32_t b = threadIdx.x; b < nblocks; b += WAVE) {
+      for (uint32_t b = threadIdx.x; b < members; b += WAVE) {
         const uint32_t x = ctl->part[b];
         all += x;
         if (b < blk) pre += x;
@@ -371,20 +394,29 @@ struct GroupTeam {
     __syncthreads();
     const uint32_t pre = scan_lds->w[0];
     total = scan_lds->w[1];
-    sync();
+    bar();
     return pre + ex;
   }
-  // Stable LSD radix sort over the team: per pass each workgroup histograms
-  // its contiguous chunk, one team scan over the (digit, workgroup) counts in
-  // digit-major order gives every workgroup its output offset per digit, and
-  // each workgroup scatters its chunk with the ballot ranking of the
-  // one-workgroup sort.  Passes whose digit is constant are skipped.
+  // Stable LSD radix sort over the sort's sub-team (the first sort_blocks
+  // workgroups; the others go on to the barrier after the sort): per pass
+  // each member histograms its contiguous chunk, one scan over the
+  // (digit, workgroup) counts in digit-major order gives every member its
+  // output offset per digit, and each member scatters its chunk with the
+  // ballot ranking of the one-workgroup sort.  Passes whose digit is constant
+  // are skipped.  Returns, in the members, the side the keys ended on, which
+  // sorted_in_a hands to the whole team after the next team barrier.
   BB_DI bool radix_sort(uint64_t* ka, uint32_t* ia, uint64_t* kb, uint32_t* ib, uint32_t n, int passes) const {
+    const uint32_t S = sort_blocks < 1u ? 1u : (sort_blocks < nblocks ? sort_blocks : nblocks);
+    if (blk >= S) return true;  // not a member: the result comes through ctl->sorted
     SortLds<BLOCK>& L = *sort_lds;
     const uint32_t tid = threadIdx.x, lane = tid & (WAVE - 1), wave = tid / WAVE;
-    const uint32_t chunk = (n + nblocks - 1) / nblocks;
+    const uint32_t chunk = (n + S - 1) / S;
     const uint32_t lo = blk * chunk < n ? blk * chunk : n;
     const uint32_t hi = lo + chunk < n ? lo + chunk : n;
+    auto bar = [&]() {
+      if (S == nblocks) sync();
+      else sync_first(S);
+    };
     for (uint32_t i = tid; i < (BLOCK / WAVE) * 256; i += BLOCK) L.wcnt[i] = 0;
     bool in_a = true;
     for (int pass = 0; pass < passes; ++pass) {
@@ -395,7 +427,7 @@ struct GroupTeam {
       uint32_t* iout = in_a ? ib : ia;
       for (uint32_t i = tid; i < 256; i += BLOCK) L.base[i] = 0;
       if (blk == 0 && tid == 0) ctl->diff = 0;
-      sync();
+      bar();
       const uint32_t d0 = n ? (uint32_t)(kin[0] >> shift) & 255u : 0u;
       bool differs = false;
       for (uint32_t i = lo + tid; i < hi; i += BLOCK) {
@@ -405,21 +437,21 @@ struct GroupTeam {
       }
       if (differs) atomicOr(&ctl->diff, 1u);
       __syncthreads();
-      for (uint32_t i = tid; i < 256; i += BLOCK) hist[i * nblocks + blk] = L.base[i];
-      sync();
-      if (ctl->diff == 0u) continue;  // team-uniform: read after the barrier
+      for (uint32_t i = tid; i < 256; i += BLOCK) hist[i * S + blk] = L.base[i];
+      bar();
+      if (ctl->diff == 0u) continue;  // uniform over the members: read after their barrier
       // offsets: exclusive scan of hist in (digit, workgroup) order
       uint32_t carry = 0;
-      for (uint32_t e0 = 0; e0 < 256u * nblocks; e0 += size()) {
-        const uint32_t e = e0 + rank();
-        const uint32_t v = e < 256u * nblocks ? hist[e] : 0u;
+      for (uint32_t e0 = 0; e0 < 256u * S; e0 += S * BLOCK) {
+        const uint32_t e = e0 + blk * BLOCK + tid;
+        const uint32_t v = e < 256u * S ? hist[e] : 0u;
         uint32_t tot;
-        const uint32_t ex = scan(v, tot);
-        if (e < 256u * nblocks) hist[e] = carry + ex;
+        const uint32_t ex = scan_first(v, tot, S);
+        if (e < 256u * S) hist[e] = carry + ex;
         carry += tot;
       }
-      sync();
-      for (uint32_t i = tid; i < 256; i += BLOCK) L.base[i] = hist[i * nblocks + blk];
+      bar();
+      for (uint32_t i = tid; i < 256; i += BLOCK) L.base[i] = hist[i * S + blk];
       __syncthreads();
       for (uint32_t c0 = lo; c0 < hi; c0 += BLOCK) {
         const uint32_t i = c0 + tid;
@@ -453,10 +485,17 @@ struct GroupTeam {
         }
         __syncthreads();
       }
-      sync();
+      bar();
       in_a = !in_a;
     }
+    if (blk == 0 && tid == 0) __hip_atomic_store(&ctl->sorted, in_a ? 1u : 0u, __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_AGENT);
     return in_a;
+  }
+  // After the team barrier that follows radix_sort: the side the sorted keys
+  // are on, the same in every workgroup.
+  BB_DI bool sorted_in_a(bool) const {
+    return __hip_atomic_load(&ctl->sorted, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u;
   }
 };
 
@@ -580,12 +619,14 @@ BB_DI void s_build(const Dev& d, uint32_t w, const T& t, BuildShared& L) {
     }
   }
   t.sync();
+  t.mark(kMarkKeys);
   // tap copy of the unsorted keys lives in keys_b until the sort overwrites it;
   // the host reads taps from a dedicated copy made by k_finish instead.
   // ---- stable LSD radix sort (LBVH.slang:34-152)
   const int passes = W.morton_bits == 30 ? 4 : 8;
   bool in_a = t.radix_sort(ka, ia, kb, ib, N, passes);
   t.sync();
+  in_a = t.sorted_in_a(in_a);
   if (!in_a) {
     for (uint32_t i = tid; i < N; i += NT) {
       ka[i] = kb[i];
@@ -593,6 +634,7 @@ BB_DI void s_build(const Dev& d, uint32_t w, const T& t, BuildShared& L) {
     }
     t.sync();
   }
+  t.mark(kMarkSort);
   // ---- LBVH hierarchy (RB_sim.slang:206-278)
   const uint32_t nb = 2 * off;
   KeyView K{ka, (int)N, W.morton_bits == 30 ? 30 : 63};
@@ -621,6 +663,7 @@ BB_DI void s_build(const Dev& d, uint32_t w, const T& t, BuildShared& L) {
   }
   if (tid == 0) L.nroots = 0;
   t.sync();
+  t.mark(kMarkHierarchy);
   // ---- bounds, bottom-up: the second child to arrive merges (exact min/max
   // through the order-preserving map, same values as the reference's integer
   // atomics, RB_sim.slang:282-349)
@@ -656,6 +699,7 @@ BB_DI void s_build(const Dev& d, uint32_t w, const T& t, BuildShared& L) {
       }
     }
     t.sync();
+    t.mark(kMarkBounds);
     if (t.lead_group()) {
       const uint32_t nr = L.nroots < N ? L.nroots : N;
       for (uint32_t k = threadIdx.x; k < nr; k += blockDim.x) {
@@ -2772,32 +2816,54 @@ struct VelCache {
 
 // Contacts per row whose solver data pgs_row loads up front.
 static constexpr int kPgsPre = 4;
-// kind 0: warm-start impulses of pre_step; 1: solve with bias; 2: relax.
-// kPlain: PGS (the per-island path) instead of PGS_SOFT.
-template <bool kPlain>
-BB_DI void pgs_row(const Dev& d, const VelCache& vc, uint32_t off, uint32_t k, const Pgs& s, int kind) {
+// A row's solver data that pgs_row loads before it touches the body state:
+// the header, the normal and the first kPgsPre contacts.  None of it is
+// written by any other row, so it can be loaded early: the colour sweeps
+// fetch a thread's row of the next colour while it solves this colour's.
+struct PgsRow {
+  int4 h;
+  float4 n;
+  float4 r1[kPgsPre], r2[kPgsPre], l[kPgsPre];
+  float tm[kPgsPre];
+};
+BB_DI void pgs_fetch(const Dev& d, uint32_t k, int kind, PgsRow& R) {
   const uint32_t NP = d.np;
-  const int4 h = d.sv_hdr[k];
+  R.h = d.sv_hdr[k];
+  const int count = R.h.z & 255;
+#pragma unroll
+  for (int c = 0; c < kPgsPre; ++c) {
+    R.r1[c] = R.r2[c] = R.l[c] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    R.tm[c] = 0.0f;
+    if (c < count) {
+      R.r1[c] = d.sv_r1[c * NP + k];
+      R.r2[c] = d.sv_r2[c * NP + k];
+      R.l[c] = d.sv_l[c * NP + k];
+      if (kind != 0) R.tm[c] = d.sv_tm1[c * NP + k];
+    }
+  }
+  R.n = d.sv_n[k];
+}
+// kind 0: warm-start impulses of pre_step; 1: solve with bias; 2: relax.
+// kPlain: PGS (the per-island path) instead of PGS_SOFT.  R: the row's
+// fetched data (pgs_fetch).
+template <bool kPlain>
+BB_DI void pgs_solve(const Dev& d, const VelCache& vc, uint32_t off, uint32_t k, const Pgs& s, int kind,
+                     const PgsRow& R) {
+  const uint32_t NP = d.np;
+  const int4 h = R.h;
   const int count = h.z & 255;
   if (kind == 0 && !s.accum) return;
   const bool da = (h.z & 256) != 0, db = (h.z & 512) != 0;
-  // the first kPgsPre contacts' solver data are loaded together, before the
-  // body state: a row costs two dependent global round trips (header, then
-  // contacts) instead of one per contact
   float4 pr1[kPgsPre], pr2[kPgsPre], pl[kPgsPre];
   float ptm[kPgsPre];
 #pragma unroll
   for (int c = 0; c < kPgsPre; ++c) {
-    pr1[c] = pr2[c] = pl[c] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    ptm[c] = 0.0f;
-    if (c < count) {
-      pr1[c] = d.sv_r1[c * NP + k];
-      pr2[c] = d.sv_r2[c * NP + k];
-      pl[c] = d.sv_l[c * NP + k];
-      if (kind != 0) ptm[c] = d.sv_tm1[c * NP + k];
-    }
+    pr1[c] = R.r1[c];
+    pr2[c] = R.r2[c];
+    pl[c] = R.l[c];
+    ptm[c] = R.tm[c];
   }
-  const v3 n = ld3(d.sv_n[k]);
+  const v3 n = ld3(R.n);
   float4 ia0, ia1, ia2, ib0, ib1, ib2;
   vc.load_iw(d, (uint32_t)h.x, ia0, ia1, ia2);
   vc.load_iw(d, (uint32_t)h.y, ib0, ib1, ib2);
@@ -2910,6 +2976,13 @@ BB_DI void pgs_row(const Dev& d, const VelCache& vc, uint32_t off, uint32_t k, c
   if (da) vc.store(d, (uint32_t)h.x, va, wa);
   if (db) vc.store(d, (uint32_t)h.y, vb, wb);
 }
+// One row, fetched and solved in turn.
+template <bool kPlain>
+BB_DI void pgs_row(const Dev& d, const VelCache& vc, uint32_t off, uint32_t k, const Pgs& s, int kind) {
+  PgsRow R;
+  pgs_fetch(d, k, kind, R);
+  pgs_solve<kPlain>(d, vc, off, k, s, kind, R);
+}
 
 // RigidBody::apply_angular_velocity (src/shaders/extensions.slang:148-166)
 BB_DI qt apply_angular_velocity(qt q, v3 om, float dt) {
@@ -2974,10 +3047,32 @@ BB_DI void s_pgs(const Dev& d, uint32_t w, const T& t, float4* pgs_lds, uint32_t
       }
       return;
     }
-    for (uint32_t c = 0; c < used; ++c) {
-      if (bo[c] == bo[c + 1]) continue;
-      for (uint32_t k = bo[c] + tid; k < bo[c + 1]; k += NT) row(k, kind);
-      t.sync();
+    if (d.pgs_prefetch) {
+      // each thread fetches its first row of the next colour as soon as it
+      // is done with this colour, before the barrier, so the loads are in
+      // flight while the slower threads finish (rows beyond the first, when a
+      // colour has more rows than threads, are fetched where they are solved)
+      uint32_t c = 0;
+      while (c < used && bo[c] == bo[c + 1]) ++c;
+      PgsRow R;
+      bool have = c < used && bo[c] + tid < bo[c + 1];
+      if (have) pgs_fetch(d, base + bo[c] + tid, kind, R);
+      while (c < used) {
+        if (have) pgs_solve<kPlain>(d, vc, off, base + bo[c] + tid, s, kind, R);
+        for (uint32_t k = bo[c] + tid + NT; k < bo[c + 1]; k += NT) row(k, kind);
+        uint32_t cn = c + 1;
+        while (cn < used && bo[cn] == bo[cn + 1]) ++cn;
+        have = cn < used && bo[cn] + tid < bo[cn + 1];
+        if (have) pgs_fetch(d, base + bo[cn] + tid, kind, R);
+        t.sync();
+        c = cn;
+      }
+    } else {
+      for (uint32_t c = 0; c < used; ++c) {
+        if (bo[c] == bo[c + 1]) continue;
+        for (uint32_t k = bo[c] + tid; k < bo[c + 1]; k += NT) row(k, kind);
+        t.sync();
+      }
     }
     if (ovf_hi > ovf_lo) {
       if (tid == 0)
@@ -3199,7 +3294,8 @@ BB_DI float av_dir_C(v3 dir, v3 anc_a, v3 anc_b, const AvBody& b1, const AvBody&
 }
 
 // entry_avbd_primal (avbd.slang:475-658) for row i, by a group of G lanes
-// (lane = 0..G-1; G = kAvG in the per-world kernels, 1 in teams).  The
+// (lane = 0..G-1; G = 1, 2, 4 or 8 in the per-world kernels, a launch
+// parameter, and 1 in teams).  The
 // Hessian and gradient are the inertial terms plus one partial sum per
 // incident manifold, added in ascending manifold order (the adjacency
 // order); lane e % G assembles manifold e, so the dependent loads of G
@@ -3401,10 +3497,10 @@ struct AvbdShared {
   uint32_t boff[AV_NB + 1];
 };
 
-template <class T>
+template <class T, uint32_t kLanes = T::kAvGroup>
 BB_DI void s_avbd(const Dev& d, uint32_t w, const T& t, AvbdShared& L) {
   const uint32_t tid = t.rank(), NT = t.size();
-  constexpr uint32_t G = T::kAvGroup;  // lanes per body in the primal sweeps
+  constexpr uint32_t G = kLanes;  // lanes per body in the primal sweeps
   const WorldInfo W = d.worlds[w];
   if (W.solver != BB_SOLVER_AVBD || W.n == 0) return;
   const uint32_t N = W.n, off = W.body_off, P = d.npairs_cur[w];
@@ -3700,7 +3796,12 @@ __global__ __launch_bounds__(BLOCK) void k_avbd(Dev d) {
   const uint32_t w = d.order[blockIdx.x];
   if (d.worlds[w].mode & kModeTeam) return;
   const BlockTeam<BLOCK> t{&S, nullptr};
-  s_avbd(d, w, t, L);
+  switch (d.avbd_lanes) {  // the solver tile: lanes per body in the primal sweeps
+    case 1: s_avbd<BlockTeam<BLOCK>, 1>(d, w, t, L); break;
+    case 2: s_avbd<BlockTeam<BLOCK>, 2>(d, w, t, L); break;
+    case 8: s_avbd<BlockTeam<BLOCK>, 8>(d, w, t, L); break;
+    default: s_avbd<BlockTeam<BLOCK>, 4>(d, w, t, L); break;
+  }
 }
 
 // ============================================================== k_finish
@@ -3772,16 +3873,15 @@ __global__ __launch_bounds__(BLOCK) void k_team(Dev d) {
   while (j + 1 < nt && blockIdx.x >= T.first[j + 1]) ++j;
   const uint32_t b0 = T.first[j], nb = T.first[j + 1] - b0;
   const uint32_t w = T.world[j];
-  const GroupTeam<BLOCK> t{&S, &Q, d.team_ctl + j, d.team_hist + (size_t)b0 * 256, blockIdx.x - b0, nb,
-                           &dead, d.stats + (size_t)w * ST_COUNT, d.ovf_any,
-                           d.team_spin ? d.team_spin : kTeamSpinLimit};
-  TeamShared& G = *reinterpret_cast<TeamShared*>(d.team_shared + (size_t)j * kTeamSharedBytes);
   // stage boundaries: every workgroup has passed the barrier before rank 0
   // stamps the clock
-  unsigned long long* clk = d.team_clock ? d.team_clock + (size_t)j * kTeamMarks : nullptr;
-  auto mark = [&](uint32_t m) {
-    if (clk && t.rank() == 0) clk[m] = wall_clock64();
-  };
+  const GroupTeam<BLOCK> t{&S, &Q, d.team_ctl + j, d.team_hist + (size_t)b0 * 256, blockIdx.x - b0, nb,
+                           &dead, d.stats + (size_t)w * ST_COUNT, d.ovf_any,
+                           d.team_spin ? d.team_spin : kTeamSpinLimit,
+                           d.team_clock ? d.team_clock + (size_t)j * kTeamMarks : nullptr,
+                           d.sort_blocks ? d.sort_blocks : kTeamSortBlocks};
+  TeamShared& G = *reinterpret_cast<TeamShared*>(d.team_shared + (size_t)j * kTeamSharedBytes);
+  auto mark = [&](uint32_t m) { t.mark(m); };
   mark(0);
   s_build(d, w, t, G.build);
   t.sync();
@@ -3813,6 +3913,69 @@ __global__ __launch_bounds__(BLOCK) void k_team(Dev d) {
   // every barrier of the step was met (a workgroup that saw one fail has
   // left through GroupTeam::abandon)
   if (threadIdx.x == 0) t.leave();
+}
+
+// ============================================================== k_fused
+// The whole step of a world in one launch, one workgroup per world: the
+// stage functions in the order k_team runs them, a workgroup barrier between
+// them.  A world's workgroup goes on to its next stage as soon as its own
+// stage is done, where the stage kernels wait for the slowest world of the
+// lane at every stage boundary.  Stage scalars share one LDS union (one stage
+// runs at a time); the PGS body rows are the dynamic LDS, as in k_pgs.
+template <int BLOCK>
+struct FusedShared {
+  union {
+    struct {
+      BuildShared b;
+      SortLds<BLOCK> q;
+    } build;
+    BroadShared broad;
+    struct {
+      PostShared p;
+      LevelLds lv;
+    } post;
+    AvbdShared avbd;
+  };
+};
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK, 4) void k_fused(Dev d) {
+  HIP_DYNAMIC_SHARED(float4, pgs_lds)
+  __shared__ FusedShared<BLOCK> G;
+  __shared__ ScanLds<BLOCK> S;
+  const uint32_t w = d.order[blockIdx.x];
+  if (d.worlds[w].mode & kModeTeam) return;
+  const BlockTeam<BLOCK> t{&S, &G.build.q};
+  s_build(d, w, t, G.build.b);
+  __syncthreads();
+  s_broad(d, w, t, G.broad);
+  __syncthreads();
+  s_narrow<false>(d, w, t);
+  if (d.any_voxel) s_narrow<true>(d, w, t);
+  __syncthreads();
+  s_contacts(d, w, t);
+  __syncthreads();
+  s_post(d, w, t, G.post.p);
+  __syncthreads();
+  if (d.any_island) {
+    s_levels(d, w, t, G.post.p, &G.post.lv);
+    __syncthreads();
+  }
+  s_pgs<false, false>(d, w, t, pgs_lds, d.pgs_lcap);
+  if (d.any_island) {
+    __syncthreads();
+    if (d.worlds[w].solver == BB_SOLVER_PGS) s_pgs<true, true>(d, w, t, pgs_lds, d.pgs_lcap);
+    else s_pgs<true, false>(d, w, t, pgs_lds, d.pgs_lcap);
+  }
+  __syncthreads();
+  switch (d.avbd_lanes) {
+    case 1: s_avbd<BlockTeam<BLOCK>, 1>(d, w, t, G.avbd); break;
+    case 2: s_avbd<BlockTeam<BLOCK>, 2>(d, w, t, G.avbd); break;
+    case 8: s_avbd<BlockTeam<BLOCK>, 8>(d, w, t, G.avbd); break;
+    default: s_avbd<BlockTeam<BLOCK>, 4>(d, w, t, G.avbd); break;
+  }
+  __syncthreads();
+  s_finish(d, w, t);
 }
 
 // Render hand-off (entry_update_acceleration_structures, src/shaders/
@@ -3984,7 +4147,8 @@ template __global__ void k_team<kTeamBlock>(Dev);
   template __global__ void k_pgs_island<B>(Dev); \
   template __global__ void k_levels<B>(Dev);    \
   template __global__ void k_avbd<B>(Dev);     \
-  template __global__ void k_finish<B>(Dev);
+  template __global__ void k_finish<B>(Dev);   \
+  template __global__ void k_fused<B>(Dev);
 
 // Stages of the per-world path, in launch order (k_team, when there is one,
 // goes before the first).  Stage k of the timed launch is bracketed by events
@@ -4025,8 +4189,21 @@ static void launch_team(const Dev& d, hipStream_t st) {
   if (d.team_grid) hipLaunchKernelGGL(k_team<kTeamBlock>, dim3(d.team_grid), dim3(kTeamBlock), 0, st, d);
 }
 
+template <int B>
+static void launch_fused(const Lane& l, hipStream_t st) {
+  hipLaunchKernelGGL(k_fused<B>, dim3(l.grid), dim3(B), (size_t)l.d.pgs_lcap * 16 * kBodyLds, st, l.d);
+}
+
 void launch_step(const Lane& l, hipStream_t st, bool team) {
   if (team) launch_team(l.d, st);
+  if (l.fused && l.grid) {
+    switch (l.block) {
+      case 512: launch_fused<512>(l, st); break;
+      case 1024: launch_fused<1024>(l, st); break;
+      default: launch_fused<256>(l, st); break;
+    }
+    return;
+  }
   for (int k = 0; k < kStages; ++k) launch_stage(k, l, st);
 }
 

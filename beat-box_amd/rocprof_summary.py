@@ -9,6 +9,9 @@ grid size, so the 4,096-world batch (4,096 workgroups) and the plain world
 
 usage: python beat-box_amd/rocprof_summary.py --trace T.db --fetch F.db
        --write W.db --cal-fetch CF.db --cal-write CW.db --cal-mib 256 [--out X.md]
+       python beat-box_amd/rocprof_summary.py --trace T.db [--bench-log L] [--out X.md]
+(without the counter databases the HBM columns stay empty; --bench-log copies
+the "[bench]" lines of the traced run's stderr under the table)
 """
 from __future__ import annotations
 
@@ -51,20 +54,26 @@ def counter_rows(path: str, counter: str):
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--trace", required=True)
-    ap.add_argument("--fetch", required=True)
-    ap.add_argument("--write", required=True)
-    ap.add_argument("--cal-fetch", required=True)
-    ap.add_argument("--cal-write", required=True)
+    ap.add_argument("--fetch", default=None)
+    ap.add_argument("--write", default=None)
+    ap.add_argument("--cal-fetch", default=None)
+    ap.add_argument("--cal-write", default=None)
     ap.add_argument("--cal-mib", type=int, default=256)
     ap.add_argument("--out", default=None)
     ap.add_argument("--json-out", default=None, help="per-kernel figures as JSON (read by bench.py)")
     ap.add_argument("--command", default="", help="command lines that produced the databases (recorded)")
+    ap.add_argument("--bench-log", default=None,
+                    help="stderr of the traced bench.py run: its [bench] lines are copied under the table")
+    ap.add_argument("--bench-json", default=None,
+                    help="the JSON line bench.py printed in the counter runs: its step window and the "
+                         "algorithmic bytes per launch of each stage are recorded with the counters")
     a = ap.parse_args()
     dur, meta = trace_rows(a.trace)
-    fetch = counter_rows(a.fetch, "FETCH_SIZE")
-    write = counter_rows(a.write, "WRITE_SIZE")
-    cf = counter_rows(a.cal_fetch, "FETCH_SIZE")
-    cw = counter_rows(a.cal_write, "WRITE_SIZE")
+    counters = all((a.fetch, a.write, a.cal_fetch, a.cal_write))
+    fetch = counter_rows(a.fetch, "FETCH_SIZE") if counters else {}
+    write = counter_rows(a.write, "WRITE_SIZE") if counters else {}
+    cf = counter_rows(a.cal_fetch, "FETCH_SIZE") if counters else {}
+    cw = counter_rows(a.cal_write, "WRITE_SIZE") if counters else {}
     kib = a.cal_mib * 1024
     cal_f = [v for k, vs in cf.items() if k[0] == "bb::k_copy_calibrate" for v in vs]
     cal_w = [v for k, vs in cw.items() if k[0] == "bb::k_copy_calibrate" for v in vs]
@@ -77,9 +86,12 @@ def main() -> None:
         for c in a.command.split(";;"):
             lines.append(f"    {c.strip()}")
         lines.append("")
-    lines.append(f"Copy-kernel calibration ({a.cal_mib} MiB read + {a.cal_mib} MiB written per dispatch): "
-                 f"FETCH_SIZE reads {ff:.3f}x, WRITE_SIZE reads {fw:.3f}x the true bytes; the HBM "
-                 f"columns below are divided by these factors.")
+    if counters:
+        lines.append(f"Copy-kernel calibration ({a.cal_mib} MiB read + {a.cal_mib} MiB written per dispatch): "
+                     f"FETCH_SIZE reads {ff:.3f}x, WRITE_SIZE reads {fw:.3f}x the true bytes; the HBM "
+                     f"columns below are divided by these factors.")
+    else:
+        lines.append("Kernel trace only (no counter passes): the HBM columns are empty.")
     lines.append("")
     lines.append("| kernel | workgroups | calls | avg us | min us | max us | VGPR | AGPR | SGPR | LDS B | "
                  "scratch B | HBM read MB/dispatch | HBM write MB/dispatch | GB/s (read+write / avg) |")
@@ -99,11 +111,22 @@ def main() -> None:
                      f"{max(ds) / 1e3:.1f} | {vg} | {ag} | {sg} | {lds} | {scr} | "
                      f"{'' if fr is None else f'{fr:.2f}'} | {'' if wr is None else f'{wr:.2f}'} | "
                      f"{'' if gbs is None else f'{gbs:.0f}'} |")
+    if a.bench_log:
+        with open(a.bench_log) as f:
+            logged = [x.rstrip() for x in f if x.startswith("[bench]")]
+        lines += ["", "bench.py output of the traced run:", ""] + [f"    {x}" for x in logged]
     text = "\n".join(lines) + "\n"
     if a.json_out:
         import json
+        extra = {}
+        if a.bench_json:
+            with open(a.bench_json) as f:
+                line = [x for x in f.read().splitlines() if x.startswith("{")][-1]
+            bench = json.loads(line)
+            extra = dict(window=dict(steps=bench["steps"], warmup=bench["warmup"]),
+                         model_bytes_per_launch={k: v["model_bytes"] for k, v in bench["config"]["stages"].items()})
         with open(a.json_out, "w") as f:
-            json.dump(dict(fetch_factor=ff, write_factor=fw, kernels=table), f, indent=1)
+            json.dump(dict(fetch_factor=ff, write_factor=fw, **extra, kernels=table), f, indent=1)
     if a.out:
         with open(a.out, "w") as f:
             f.write(text)

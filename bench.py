@@ -7,9 +7,10 @@ box, PGS_SOFT with 8 iterations, FP32, 63-bit Morton keys, hipGraph replay,
 240 timed steps after 10 untimed ones.  Beside it: per-stage kernel durations
 over the same step window (hipEvents on the library stream, eager launches of
 a fresh batch that reaches the same states), the plain single world (1,000
-cubes + ground, PGS_SOFT I=8, parity mode, 200 steps) on the same GPU, the
-batch energy change and maximum penetration, and the CPU oracle on a bounded
-sample of the batch.
+cubes + ground, PGS_SOFT I=8, parity mode) over the same step window on the
+same GPU, the configuration's own 240-step window when another one is timed,
+the worlds that flagged an overflow, the batch energy change and maximum
+penetration, and the CPU oracle on a bounded sample of the batch.
 
 Prints one JSON line; value = batch body-steps/s with all state resident in
 HBM.
@@ -32,6 +33,12 @@ import beatbox  # noqa: E402
 import scenes  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+# pair slots of each batch world, times the default 8n + 64: a batch world
+# cannot grow within bb_batch_step, and a few stacked pyramids need more than
+# 8n + 64 pairs in the configuration's window
+PAIR_SCALE = 2
+# the batched-worlds configuration's own window: 240 timed steps after 10
+CONFIG_STEPS, CONFIG_WARMUP = 240, 10
 TRAFFIC_FILE = os.path.join(REPO, "profiles", "bench_batch4096_traffic.json")
 
 
@@ -61,6 +68,28 @@ def stage_bytes(counts: np.ndarray, iters: int, kw: int) -> dict:
     }
 
 
+def build_bytes(n: int, kw: int) -> dict:
+    """Algorithmic HBM bytes of the build stage's parts for one world of n
+    bodies (kw = key bytes): keys read the previous positions (twice for
+    63-bit keys, whose box is the bodies' own bounds) and write key + index;
+    each of the 8 (63-bit) or 4 (30-bit) LSD passes reads the keys for the
+    histogram and reads and writes key + index in the scatter; the hierarchy
+    reads each leaf's index, id, pose and shape bounds and writes its node,
+    and per internal node reads two keys and writes children, parents, the
+    visit flag and the leaf range; the bounds pass reads a node and its two
+    children and writes it; the reorder moves every row (id, flags, sleep
+    timer and five float4 rows) and writes the id -> row map and two
+    per-row lookups."""
+    passes = 8 if kw == 8 else 4
+    return {
+        "keys": n * ((32 if kw == 8 else 16) + kw + 4),
+        "sort": passes * n * (3 * kw + 8),
+        "hierarchy": n * 104 + (n - 1) * 68,
+        "bounds": (n - 1) * 136,
+        "reorder": n * 212,
+    }
+
+
 def profiled_traffic(kernel: str, worlds: int):
     """HBM bytes per step (read + write) of `kernel` over the whole batch,
     from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs of this same
@@ -84,11 +113,37 @@ def profiled_traffic(kernel: str, worlds: int):
     return sum(float(v["read_bytes"]) + float(v["write_bytes"] or 0.0) for v in lanes)
 
 
+def traffic_fields(kernel: str, args, model_per_launch: float) -> dict:
+    """roofline.traffic: HBM bytes per launch of `kernel` from the committed
+    rocprofv3 counter runs (FETCH_SIZE and WRITE_SIZE, copy-kernel
+    calibrated), reported only when they were taken over this run's step
+    window; otherwise null, with the profiled window's figures beside it,
+    each set against the algorithmic bytes of its own window."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            meta = json.load(f)
+    except (OSError, ValueError):
+        meta = {}
+    prof = profiled_traffic(kernel, args.worlds)
+    window = meta.get("window")  # {"steps": S, "warmup": W} of the counter runs
+    model = meta.get("model_bytes_per_launch", {}).get(kernel)
+    same = bool(window) and (window.get("steps"), window.get("warmup")) == (args.steps, args.warmup)
+    out = {"traffic": prof if same else None,
+           "traffic_over_model": (prof / model_per_launch) if (same and prof) else None,
+           "traffic_source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of bench.py (copy-kernel calibrated), "
+                             "profiles/bench_batch4096_traffic.json"}
+    if prof is not None and not same:
+        out["traffic_profiled"] = dict(window=window, bytes_per_launch=prof, model_bytes_per_launch=model,
+                                       over_model=(prof / model) if model else None)
+    return out
+
+
 def make_batch(worlds: int, seed: int, lo: int, hi: int):
     t0 = time.time()
     sizes = scenes.batch_world_sizes(seed, worlds, lo, hi)
     desc = beatbox.WorldDesc.make(**scenes.world_desc(iterations=8, morton_bits=63, quirks=0))
     b = beatbox.Batch()
+    b.set_param(beatbox.PARAM_PAIR_SCALE, PAIR_SCALE)
     total = 0
     for w, n in enumerate(sizes):
         bodies = scenes.batch_world(seed, w, int(n))
@@ -105,28 +160,32 @@ def time_batch(b, steps: int, warmup: int, tune: bool):
     the timed steps and the chosen launch parameters."""
     e0 = float(b.energy().sum())
     b.step(warmup, use_graph=True)
-    b.sync()
+    b.sync(allow_overflow=True)
     if tune:
         b.autotune(2)
     params = (b.get_param(beatbox.PARAM_BLOCK), b.get_param(beatbox.PARAM_PGS_LDS_ROWS),
               b.get_param(beatbox.PARAM_LANE_BODIES), b.get_param(beatbox.PARAM_LANE_BLOCK))
     t0 = time.perf_counter()
     b.step(steps, use_graph=True)
-    b.sync()
+    b.sync(allow_overflow=True)
     dt = time.perf_counter() - t0
-    return dt, e0, float(b.energy().sum()), params
+    overflowed = b.overflow()
+    return dt, e0, float(b.energy().sum()), params, overflowed
 
 
 def plain_world(steps: int, warmup: int):
+    """The plain single world (1,000 cubes + ground, PGS_SOFT I=8, parity
+    mode) over the same step window as the batch: hipGraph replay of
+    `steps` steps after `warmup` untimed ones."""
     desc = beatbox.WorldDesc.make(**scenes.world_desc(iterations=8))
     w = beatbox.Batch()
     bodies = scenes.cubes_grid()
     w.add_world(desc, bodies)
     w.step(warmup, use_graph=True)
-    w.sync()
+    w.sync(allow_overflow=True)
     t0 = time.perf_counter()
     w.step(steps, use_graph=True)
-    w.sync()
+    w.sync(allow_overflow=True)
     dt = time.perf_counter() - t0
     return len(bodies) * steps / dt, dt / steps * 1e3
 
@@ -140,22 +199,28 @@ def single_world(name: str, desc, bodies, steps: int, warmup: int, profile: int)
     b = beatbox.Batch()
     b.add_world(desc, bodies)
     b.step(warmup, use_graph=True)
-    b.sync()
+    b.sync(allow_overflow=True)
     build_s = time.time() - t0
     t0 = time.perf_counter()
     b.step(steps, use_graph=True)
-    b.sync()
+    b.sync(allow_overflow=True)
     dt = time.perf_counter() - t0
     st, km, cnt = b.profile(profile, counts=True)
+    split = b.build_split() / profile
     s = b.stats(0)
     team = len(bodies) >= b.get_param(beatbox.PARAM_TEAM_BODIES) > 0
+    kw = 8 if desc.morton_bits == 63 else 4
+    bb = build_bytes(len(bodies), kw)
+    build = {n: dict(ms=round(float(split[i]), 4), bytes=bb[n],
+                     gbs=round(bb[n] / (float(split[i]) * 1e-3) / 1e9, 1) if split[i] > 0 else None)
+             for i, n in enumerate(beatbox.BUILD_NAMES)} if team else None
     out = dict(bodies=len(bodies), steps=steps, warmup=warmup, ms_per_step=dt / steps * 1e3,
                body_steps_per_s=len(bodies) * steps / dt, team=team,
                profiled_step_ms_p50=float(np.percentile(st, 50)),
                stages_ms={n: round(float(km[i]) / profile, 3) for i, n in enumerate(beatbox.KERNEL_NAMES)},
                manifolds=s["collision_count"], contacts=s["contact_count"], pairs=s["broad_phase_collision_count"],
                max_penetration_mm=int(cnt[:, beatbox.COUNT_NAMES.index("max_pen_mm")].max()),
-               overflow=s["overflow"])
+               overflow=s["overflow"], build_parts=build)
     b.destroy()
     log(f"{name}: {out['ms_per_step']:.2f} ms/step, {out['body_steps_per_s'] / 1e6:.2f} M body-steps/s "
         f"(setup {build_s:.1f}s), {out}")
@@ -167,6 +232,7 @@ def batch_run(name: str, worlds, shapes=(), steps: int = 60, warmup: int = 10, p
     replay of `steps` steps after `warmup`, then `profile` eager steps timed
     per stage."""
     b = beatbox.Batch()
+    b.set_param(beatbox.PARAM_PAIR_SCALE, PAIR_SCALE)
     for cells in shapes:
         b.add_voxel_shape(cells, scenes.VOXEL_SIZE, scenes.VOXEL_DENSITY)
     total = 0
@@ -174,17 +240,18 @@ def batch_run(name: str, worlds, shapes=(), steps: int = 60, warmup: int = 10, p
         b.add_world(desc, bodies)
         total += len(bodies)
     b.step(warmup, use_graph=True)
-    b.sync()
+    b.sync(allow_overflow=True)
     t0 = time.perf_counter()
     b.step(steps, use_graph=True)
-    b.sync()
+    b.sync(allow_overflow=True)
     dt = time.perf_counter() - t0
     _, km, cnt = b.profile(profile, counts=True)
     manifolds = int(cnt[:, beatbox.COUNT_NAMES.index("manifolds")].mean())
-    overflow = sum(b.stats(w)["overflow"] for w in range(len(worlds)))
+    overflow = len(b.overflow())
     b.destroy()
     out = dict(worlds=len(worlds), bodies=total, steps=steps, warmup=warmup, ms_per_step=dt / steps * 1e3,
-               body_steps_per_s=total * steps / dt, manifolds=manifolds, overflow=int(overflow),
+               body_steps_per_s=total * steps / dt, manifolds=manifolds, overflow_worlds=overflow,
+               pair_scale=PAIR_SCALE,
                stages_ms={n: round(float(km[i]) / profile, 3) for i, n in enumerate(beatbox.KERNEL_NAMES)})
     log(f"{name}: {out['ms_per_step']:.2f} ms/step, {out['body_steps_per_s'] / 1e6:.2f} M body-steps/s, {out}")
     return out
@@ -237,18 +304,19 @@ def next_rows(seed: int) -> dict:
     return out
 
 
-def serving(duration: float, budget_gib: float, resident_cap: int, team_bodies: int,
+def serving(duration: float, budget_gib: float, resident_cap: int, team_bodies: int, split_bodies: int,
             seed: int = 0xBB05) -> dict:
     """Serving stream: Poisson arrivals at 500 worlds/s for `duration` seconds
     (random piles of 64-20,000 bodies, AVBD 10 iterations, horizons 60-600
-    steps), admitted by the HBM budget and a resident-body cap, stepped
-    back to back; reports the server's counters and latency percentiles at
-    the end of the arrival window."""
+    steps), admitted by the HBM budget (and a resident-body cap unless 0)
+    into pools by size that step at their own cadence (split_bodies 0: one
+    lockstep pool); reports the server's counters and latency percentiles,
+    overall and per pool, at the end of the arrival window."""
     desc = beatbox.WorldDesc.make(**scenes.world_desc(solver=scenes.SOLVER_AVBD, iterations=10,
                                                       morton_bits=63, quirks=0))
     arr, nb, hz, sd = scenes.serving_requests(seed, duration=duration)
     srv = beatbox.Server(budget_bytes=int(budget_gib * 2**30), max_worlds=4096,
-                         max_resident_bodies=resident_cap, team_bodies=team_bodies)
+                         max_resident_bodies=resident_cap, team_bodies=team_bodies, split_bodies=split_bodies)
     t0 = time.perf_counter()
     k = 0
     while True:
@@ -263,7 +331,7 @@ def serving(duration: float, budget_gib: float, resident_cap: int, team_bodies: 
     wall = time.perf_counter() - t0
     srv.destroy()
     rep.update(duration_s=duration, wall_s=wall, arrival_rate=500.0, resident_body_cap=resident_cap,
-               team_bodies=team_bodies,
+               team_bodies=team_bodies, split_bodies=split_bodies,
                body_steps_per_s=rep["body_steps"] / (rep["device_ms"] * 1e-3) if rep["device_ms"] else 0.0)
     log(f"serving: {rep}")
     return rep
@@ -325,13 +393,20 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="default launch parameters, no autotuner")
     ap.add_argument("--no-extra", action="store_true", help="skip the large-pile, sphere-gas and serving runs")
+    ap.add_argument("--extra", default="large_pile,sphere_gas,serving,next_rows",
+                    help="comma-separated extra runs (large_pile, sphere_gas, serving, next_rows)")
+    ap.add_argument("--no-config-window", action="store_true",
+                    help="do not time the configuration's own 240-step window beside another one "
+                         "(counter runs, whose per-kernel figures must come from the timed window alone)")
     ap.add_argument("--serving-seconds", type=float, default=10.0)
     ap.add_argument("--serving-budget-gib", type=float, default=200.0)
     ap.add_argument("--serving-cap", type=int, default=200_000, help="resident-body cap of the serving run")
-    ap.add_argument("--serving-team-bodies", type=int, default=1024,
+    ap.add_argument("--serving-team-bodies", type=int, default=4096,
                     help="worlds this large are stepped as workgroup teams in the serving run")
+    ap.add_argument("--serving-split-bodies", type=int, default=1024,
+                    help="serving pools: worlds below this size, up to the team size, and above step at "
+                         "their own cadences (0: one lockstep pool)")
     ap.add_argument("--profile-steps", type=int, default=0, help="0: the timed steps")
-    ap.add_argument("--plain-steps", type=int, default=200)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run the copy kernel on MIB MiB (counter calibration) and exit")
     args = ap.parse_args()
@@ -352,7 +427,9 @@ def main() -> int:
 
     # 1) throughput: hipGraph replay of the timed steps
     batch, total = make_batch(args.worlds, args.seed, 16, 2048)
-    t, e0, e1, params = time_batch(batch, args.steps, args.warmup, not args.no_tune)
+    t, e0, e1, params, overflowed = time_batch(batch, args.steps, args.warmup, not args.no_tune)
+    if overflowed:
+        log(f"batch: {len(overflowed)} worlds flagged an overflow: {overflowed[:16]}")
     log(f"launch parameters: block {params[0]}, PGS LDS rows {params[1]}")
     dev_gib = batch.device_bytes / 2**30
     batch.destroy()
@@ -369,7 +446,7 @@ def main() -> int:
     batch.set_param(beatbox.PARAM_BLOCK, params[0])
     batch.set_param(beatbox.PARAM_PGS_LDS_ROWS, params[1])
     batch.step(args.warmup, use_graph=True)
-    batch.sync()
+    batch.sync(allow_overflow=True)
     psteps = args.profile_steps or args.steps
     step_ms, kern_ms, counts = batch.profile(psteps, counts=True)
     batch.destroy()
@@ -379,7 +456,7 @@ def main() -> int:
     per_launch_ms = {n: float(kern_ms[i]) / psteps for i, n in enumerate(beatbox.KERNEL_NAMES)}
     dominant = max(per_launch_ms, key=per_launch_ms.get)
     achieved = sb[dominant] / (float(kern_ms[beatbox.KERNEL_NAMES.index(dominant)]) * 1e-3) / 1e9
-    stages = {n: dict(ms=round(per_launch_ms[n], 4),
+    stages = {n: dict(ms=round(per_launch_ms[n], 4), model_bytes=float(sb[n]) / psteps,
                       gbs=round(float(sb[n]) / (float(kern_ms[i]) * 1e-3) / 1e9, 1) if kern_ms[i] > 0 else None)
               for i, n in enumerate(beatbox.KERNEL_NAMES)}
     avg = {n: float(counts[:, i].mean()) for i, n in enumerate(beatbox.COUNT_NAMES)}
@@ -387,22 +464,45 @@ def main() -> int:
     log(f"stages (avg per launch over {psteps} eager steps): {stages}")
     log(f"eager step p50 {p50:.3f} ms, p99 {p99:.3f} ms; avg counts {avg}")
 
-    plain_bs, plain_ms = plain_world(args.plain_steps, args.warmup)
-    log(f"plain: {plain_ms:.3f} ms/step, {plain_bs / 1e6:.3f} M body-steps/s, ratio {value / plain_bs:.1f}x")
+    plain_bs, plain_ms = plain_world(args.steps, args.warmup)
+    log(f"plain (steps {args.warmup}..{args.warmup + args.steps}): {plain_ms:.3f} ms/step, "
+        f"{plain_bs / 1e6:.3f} M body-steps/s, ratio {value / plain_bs:.1f}x")
+
+    # the configuration's own window (steps 10..250) when the timed window is
+    # another one: batch and plain world again, same launch parameters
+    config_window = None
+    if (args.steps, args.warmup) != (CONFIG_STEPS, CONFIG_WARMUP) and not args.no_config_window:
+        cb, _ = make_batch(args.worlds, args.seed, 16, 2048)
+        cb.set_param(beatbox.PARAM_BLOCK, params[0])
+        cb.set_param(beatbox.PARAM_PGS_LDS_ROWS, params[1])
+        ct, _, _, _, c_ovf = time_batch(cb, CONFIG_STEPS, CONFIG_WARMUP, False)
+        cb.destroy()
+        c_plain_bs, c_plain_ms = plain_world(CONFIG_STEPS, CONFIG_WARMUP)
+        cv = total * CONFIG_STEPS / ct
+        config_window = dict(steps=CONFIG_STEPS, warmup=CONFIG_WARMUP, value=cv, ms_per_step=ct / CONFIG_STEPS * 1e3,
+                             plain_body_steps_per_s=c_plain_bs, plain_ms_per_step=c_plain_ms,
+                             batch_over_plain=cv / c_plain_bs, overflow_worlds=len(c_ovf))
+        log(f"config window: {config_window}")
 
     extra = {}
-    if not args.no_extra:
+    runs = set() if args.no_extra else {x.strip() for x in args.extra.split(",") if x.strip()}
+    if "large_pile" in runs:
         avbd = beatbox.WorldDesc.make(**scenes.world_desc(solver=scenes.SOLVER_AVBD, iterations=10,
                                                           morton_bits=63, quirks=0))
         extra["large_pile"] = dict(workload="100,000 bodies (50/50 boxes/spheres, 0.25-1 m) falling in a "
                                             "60x200x60 m column, AVBD 10 iterations, 63-bit keys",
                                    **single_world("large pile", avbd, scenes.large_pile(), 300, 1, 5))
+    if "sphere_gas" in runs:
         gas = beatbox.WorldDesc.make(**scenes.world_desc(iterations=8, morton_bits=63, quirks=0, gravity=0.0))
         extra["sphere_gas"] = dict(workload="1,000,000 spheres (r 0.1-0.5 m) in a 1000 m cube, v ~ N(0, 2 m/s), "
                                             "no gravity, PGS_SOFT 8 iterations, 63-bit keys",
                                    **single_world("sphere gas", gas, scenes.sphere_gas(), 100, 1, 5))
+    if "serving" in runs:
         extra["serving"] = serving(args.serving_seconds, args.serving_budget_gib, args.serving_cap,
-                                   args.serving_team_bodies)
+                                   args.serving_team_bodies, args.serving_split_bodies)
+        extra["serving_budget_only"] = serving(args.serving_seconds, args.serving_budget_gib, 0,
+                                               args.serving_team_bodies, args.serving_split_bodies)
+    if "next_rows" in runs:
         extra["next_rows"] = next_rows(args.seed)
 
     threads = args.cpu_threads or min(16, os.cpu_count() or 1)
@@ -435,7 +535,11 @@ def main() -> int:
             "device_gib": dev_gib,
             "plain_body_steps_per_s": plain_bs,
             "plain_ms_per_step": plain_ms,
+            "plain_window": f"steps {args.warmup}..{args.warmup + args.steps}, the batch's window",
             "batch_over_plain": value / plain_bs,
+            "overflow_worlds": len(overflowed),
+            "pair_scale": PAIR_SCALE,
+            "config_window": config_window,
             "eager_step_ms_p50": p50,
             "eager_step_ms_p99": p99,
             "copy_kernel_gbs": copy_gbs,
@@ -453,9 +557,7 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": profiled_traffic(dominant, args.worlds),
-            "traffic_source": "rocprofv3 FETCH_SIZE/WRITE_SIZE of this command, calibrated; "
-                              "profiles/bench_batch4096_kernels.md",
+            **traffic_fields(dominant, args, sb[dominant] / psteps),
         },
         "cpu_baseline": cpu,
     }

@@ -486,6 +486,25 @@ enum {
  * be NULL. */
 bb_status bb_batch_profile(bb_batch* b, uint32_t nsteps, float* step_ms, float* kernel_ms,
                            uint64_t* counts);
+/* Parts of BB_KERNEL_BUILD (bb_batch_build_split indices): the Morton keys
+ * (RB_sim.slang:75-111), the LSD radix sort (LBVH.slang:34-152), the Karras
+ * hierarchy (RB_sim.slang:206-278), the bounds pass up to its last team-wide
+ * barrier (RB_sim.slang:282-349), then the reorder (RB_sim.slang:351-386)
+ * with the lead workgroup's climb of the top nodes. */
+enum {
+  BB_BUILD_KEYS = 0,
+  BB_BUILD_SORT = 1,
+  BB_BUILD_HIERARCHY = 2,
+  BB_BUILD_BOUNDS = 3,
+  BB_BUILD_REORDER = 4,
+  BB_BUILD_COUNT = 5
+};
+/* Measurement helper (no reference counterpart): the build stage of the
+ * slowest workgroup team, split into the parts above by the team's own clock
+ * marks, summed over the steps of the last bb_batch_profile call
+ * (build_ms[BB_BUILD_COUNT]; zeros when no world of the batch runs as a
+ * team). */
+bb_status bb_batch_build_split(const bb_batch* b, float* build_ms);
 /* Mechanical energy of each world's current state, in double (no reference
  * counterpart): sum over dynamic bodies of m|v|^2/2 + w.I.w/2 - m*g*y with
  * the physical inertia of the shape.  per_world[n], n >= world count. */
@@ -521,37 +540,75 @@ typedef enum bb_param {
                                 or 1024.  Default 512                       */
   BB_PARAM_LANE_LDS_ROWS = 5, /* BB_PARAM_PGS_LDS_ROWS of that lane: 0..2048.
                                 Default 1,024                               */
-  BB_PARAM_TEAM_SPIN = 6     /* spins of a team barrier wait before the team
+  BB_PARAM_TEAM_SPIN = 6,    /* spins of a team barrier wait before the team
                                 gives the step up (overflow bit 1); 0 = the
                                 default, about a second.  Small values force
                                 that failure path (tests): bb_world_step then
                                 replays the step by one workgroup, a batch
                                 flags the world                             */
+  BB_PARAM_PAIR_SCALE = 7,   /* pair slots of every world, as a multiple
+                                (1..16, default 1) of the starting capacity
+                                8n + 64 (six times that with voxel bodies):
+                                room a batch world cannot grow into during
+                                bb_batch_step.  Set it before the first step:
+                                a later change lays the batch out anew, like
+                                adding a world, and the next step starts
+                                without warm start                          */
+  BB_PARAM_SORT_BLOCKS = 8,  /* workgroups of a workgroup team that run its
+                                radix sort (the rest wait at the barrier
+                                after it): 1..2048, 0 = the default 32.
+                                Fewer members mean cheaper sort barriers and
+                                longer chunks each; the sort is stable, so
+                                the order is the same for every value      */
+  BB_PARAM_FUSED = 9,        /* 1: the per-world stages of a step run as one
+                                launch per lane (k_fused: a world's
+                                workgroup goes from stage to stage without
+                                waiting for the lane's slowest world); 0
+                                (default): one launch per stage.  Profiling
+                                (bb_batch_profile) always times the stages  */
+  BB_PARAM_AVBD_LANES = 10,  /* the AVBD solver tile of the per-world
+                                kernels: lanes that assemble one body's 6x6
+                                system together in the primal sweeps (1, 2,
+                                4 or 8; 0 = the default 4).  The partial
+                                sums are added in manifold order whatever
+                                the value, so results do not change        */
+  BB_PARAM_PGS_PREFETCH = 11 /* the PGS colour-sweep tile: 1 = each thread
+                                fetches its row of the next colour while it
+                                solves this colour's row (the next colour's
+                                rows are written by nobody else meanwhile);
+                                0 (default) = fetch where solved            */
 } bb_param;
 bb_status bb_batch_set_param(bb_batch* b, bb_param p, uint32_t value);
 /* Effective value (BB_PARAM_BLOCK resolves 0 to the automatic choice). */
 uint32_t bb_batch_get_param(const bb_batch* b, bb_param p);
 /* bb_batch_set_param for a single world (no reference counterpart). */
 bb_status bb_world_set_param(bb_world* w, bb_param p, uint32_t value);
-/* Autotuner: times `reps` replays of the next step for every candidate
- * (block size x LDS rows), each replay restarting from the same state, and
- * keeps the fastest.  The simulation state afterwards is exactly the state
- * before the call.  *best_ms (may be NULL) receives the chosen step time. */
+/* Autotuner: times `reps` replays of the next step for every candidate,
+ * each replay restarting from the same state, and keeps the fastest: first
+ * the grid of workgroup size x PGS LDS rows, then, one at a time from the
+ * best so far, one launch per step or per stage (BB_PARAM_FUSED), the PGS
+ * sweep tile (BB_PARAM_PGS_PREFETCH, when a world runs PGS), the AVBD
+ * solver tile (BB_PARAM_AVBD_LANES, when a world runs AVBD in the per-world
+ * kernels) and the radix sort's sub-team (BB_PARAM_SORT_BLOCKS, when a world
+ * runs as a workgroup team).  No candidate changes any result.  The
+ * simulation state afterwards is exactly the state before the call.
+ * *best_ms (may be NULL) receives the chosen step time. */
 bb_status bb_batch_autotune(bb_batch* b, uint32_t reps, float* best_ms);
 void bb_batch_destroy(bb_batch* b);
 
 /* ----------------------------------------------------------- serving loop */
 /* Continuous batching of independent worlds (no reference counterpart; the
  * reference steps one scene).  Requests are queued on submit and admitted at
- * step boundaries, first come first served, while the resident worlds'
- * device state fits the HBM budget (and, optionally, a resident-body cap that
- * bounds the step latency).  Every resident world advances one step per
- * server step in one launch sequence; a world that has run its requested
- * number of steps is retired: its final state is copied back and its device
- * memory and slot are reused.  Each world's trajectory is bit-identical to
- * stepping it alone (bb_world_*), wherever it sits in the device arrays and
- * whatever else is resident. */
+ * step boundaries, first come first served within their pool, while the
+ * resident worlds' device state fits the HBM budget (and, optionally, a
+ * resident-body cap).  The worlds of a pool advance one step per pool step in
+ * one launch sequence; pools (split_bodies) step independently.  A world that
+ * has run its requested number of steps is retired: its final state is
+ * copied back and its device memory and slot are reused.  Each world's
+ * trajectory is bit-identical to stepping it alone (bb_world_*), wherever it
+ * sits in the device arrays and whatever else is resident. */
 typedef struct bb_server bb_server;
+#define BB_SERVER_MAX_POOLS 3
 
 typedef struct bb_server_desc {
   uint64_t hbm_budget_bytes;    /* device bytes for resident state (admission) */
@@ -562,6 +619,14 @@ typedef struct bb_server_desc {
   uint32_t block;               /* threads per world workgroup, 0 = 256       */
   uint32_t team_bodies;         /* worlds this large run as workgroup teams;
                                    0 = 8192, 0xFFFFFFFF = never              */
+  uint32_t split_bodies;        /* 0: one pool, every resident world steps in
+                                   lockstep.  Else pools by body count, each
+                                   stepping at its own cadence on its own
+                                   stream: [0, split_bodies) (workgroups of
+                                   `block` threads), [split_bodies,
+                                   team_bodies) (512-thread workgroups) and
+                                   [team_bodies, ...) (teams); max_worlds
+                                   slots each                                */
 } bb_server_desc;
 
 typedef enum bb_request_state {
@@ -587,6 +652,14 @@ typedef struct bb_server_report {
   double queue_ms_p50, queue_ms_p99; /* submit -> admission, host clock       */
   uint64_t reruns;                   /* requests run again from their first
                                         step after an overflow             */
+  /* per pool (pools = 1 without a split): the smallest body count it takes,
+   * its world-steps and their latency percentiles, as step_ms_p50/p99 */
+  uint32_t pools, pad;
+  uint32_t pool_min_bodies[BB_SERVER_MAX_POOLS];
+  uint32_t pool_pad;
+  uint64_t pool_world_steps[BB_SERVER_MAX_POOLS];
+  double pool_step_ms_p50[BB_SERVER_MAX_POOLS];
+  double pool_step_ms_p99[BB_SERVER_MAX_POOLS];
 } bb_server_report;
 
 bb_status bb_server_create(const bb_server_desc* desc, bb_server** out);
@@ -595,9 +668,13 @@ bb_status bb_server_create(const bb_server_desc* desc, bb_server** out);
  * budget or the capacities; *ticket identifies the request. */
 bb_status bb_server_submit(bb_server* s, const bb_world_desc* desc, const bb_body_desc* bodies, uint32_t n,
                            uint32_t steps, uint64_t* ticket);
-/* Runs nsteps server steps (admission, one step of every resident world,
- * retirement).  Launches are asynchronous; at most two steps run ahead of
- * the host. */
+/* Runs nsteps server rounds.  In a round every pool with queued or resident
+ * worlds and fewer than two steps in flight takes one step (admission, one
+ * step of each of its resident worlds, retirement); a pool whose two steps
+ * are still running is passed over, so each pool keeps its own cadence, and
+ * when no pool can step the call waits until one can.  With one pool a
+ * round is one lockstep step of every resident world.  Launches are
+ * asynchronous. */
 bb_status bb_server_step(bb_server* s, uint32_t nsteps);
 /* Steps until every submitted request is done. */
 bb_status bb_server_drain(bb_server* s);

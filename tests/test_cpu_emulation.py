@@ -52,10 +52,10 @@ def _scene(path, worlds, shapes=()):
 
 
 def _run(exe, scene, out, steps, team_bodies, block, graph=1, lane_bodies=0, single=0, team_spin=0,
-         max_blocks=0):
+         max_blocks=0, sort_blocks=0):
     env = dict(ENV, EMU_MAX_BLOCKS=str(max_blocks)) if max_blocks else ENV
     r = subprocess.run([exe, scene, str(steps), str(team_bodies), str(block), str(graph), out, str(lane_bodies),
-                        str(single), str(team_spin)],
+                        str(single), str(team_spin), str(sort_blocks)],
                        capture_output=True, text=True, env=env, timeout=600)
     assert r.returncode == 0, r.stderr[-3000:]
     assert "AddressSanitizer" not in r.stderr, r.stderr[-3000:]
@@ -81,6 +81,10 @@ def test_team_and_workgroup_paths_in_bounds_and_identical(emu_main, tmp_path):
     log_t, t = _run(emu_main, scene, str(tmp_path / "t.bin"), 6, 600, 0)
     assert log_w == log_t
     assert w == t
+    # the radix sort on a sub-team of one workgroup (the others wait at the
+    # barrier after it): the same order, so the same steps
+    log_s, x = _run(emu_main, scene, str(tmp_path / "s.bin"), 6, 600, 0, sort_blocks=1)
+    assert log_s == log_w and x == w
     # the step did work: pairs and contacts on the last step of the AVBD pile
     last = [l for l in log_w.splitlines() if l.startswith("step 5 world 0")][0].split()
     assert int(last[5]) > 0 and int(last[9]) > 0

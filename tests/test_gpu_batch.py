@@ -192,16 +192,26 @@ def test_launch_params_do_not_change_results():
         solver=scenes.SOLVER_AVBD if w == 1 else scenes.SOLVER_PGS_SOFT,
         iterations=8, morton_bits=63, quirks=0)) for w in range(len(sizes))]
     runs = []
-    for blk, lds in ((256, 0), (512, 256), (1024, 768), (256, 512), (512, 2048)):
+    # (block, LDS rows, one launch per step, AVBD lanes per body, PGS prefetch)
+    for blk, lds, fused, lanes, pre in ((256, 0, 0, 4, 0), (512, 256, 0, 4, 0), (1024, 768, 0, 4, 1),
+                                        (256, 512, 0, 4, 1), (512, 2048, 0, 4, 0), (256, 512, 1, 4, 1),
+                                        (512, 256, 1, 2, 0), (256, 0, 0, 1, 1), (512, 512, 0, 8, 1),
+                                        (1024, 256, 1, 8, 0)):
+        print(f"launch params: block {blk} lds {lds} fused {fused} lanes {lanes} prefetch {pre}", flush=True)
         b = beatbox.Batch()
         for d, bodies in zip(descs, worlds):
             b.add_world(d, bodies)
         b.set_param(beatbox.PARAM_BLOCK, blk)
         b.set_param(beatbox.PARAM_PGS_LDS_ROWS, lds)
+        b.set_param(beatbox.PARAM_FUSED, fused)
+        b.set_param(beatbox.PARAM_AVBD_LANES, lanes)
+        b.set_param(beatbox.PARAM_PGS_PREFETCH, pre)
         b.step(15, use_graph=True)
         b.sync()
         assert b.get_param(beatbox.PARAM_BLOCK) == blk
         assert b.get_param(beatbox.PARAM_PGS_LDS_ROWS) == min(lds, max(sizes) + 1)
+        assert (b.get_param(beatbox.PARAM_FUSED), b.get_param(beatbox.PARAM_AVBD_LANES),
+                b.get_param(beatbox.PARAM_PGS_PREFETCH)) == (fused, lanes, pre)
         runs.append(b)
     for r in runs[1:]:
         _bodies_equal(runs[0], r, len(sizes), [s + 1 for s in sizes])
@@ -211,13 +221,19 @@ def test_autotune_leaves_state_unchanged():
     """The autotuner replays the next step from the same state; afterwards
     the batch continues exactly as an untuned batch, and the counters of the
     last real step are restored."""
-    sizes = [400, 64, 900, 150]
+    sizes = [400, 64, 900, 150, 1200]
     worlds = _worlds(0xBB04, sizes)
     desc = beatbox.WorldDesc.make(**scenes.world_desc(iterations=8, morton_bits=63, quirks=0))
+    avbd = beatbox.WorldDesc.make(**scenes.world_desc(solver=scenes.SOLVER_AVBD, iterations=10,
+                                                      morton_bits=63, quirks=0))
     tuned, plain = beatbox.Batch(), beatbox.Batch()
-    for bodies in worlds:
-        tuned.add_world(desc, bodies)
-        plain.add_world(desc, bodies)
+    # world 1 runs AVBD (the lanes-per-body tile is tuned), world 4 runs as a
+    # workgroup team (the sort's sub-team is tuned)
+    for b in (tuned, plain):
+        b.set_param(beatbox.PARAM_TEAM_BODIES, 1000)
+    for w, bodies in enumerate(worlds):
+        tuned.add_world(avbd if w == 1 else desc, bodies)
+        plain.add_world(avbd if w == 1 else desc, bodies)
     tuned.step(6, use_graph=True)
     plain.step(6, use_graph=True)
     before = [tuned.stats(w) for w in range(len(sizes))]
@@ -250,12 +266,15 @@ def test_team_path_equals_workgroup_path():
               (avbd, scenes.serving_pile(22, 700))]
     runs = []
     # 0: every world by one workgroup (the two largest in the 512-thread lane);
-    # 600: four worlds as teams; 2000: two teams running beside both lanes
-    for team in (0, 600, 2000):
+    # 600: four worlds as teams, the radix sort on 1 or (default) 32 of their
+    # workgroups; 2000: two teams running beside both lanes
+    for team, sort_blocks in ((0, 0), (600, 1), (600, 0), (2000, 2)):
         b = beatbox.Batch()
         for d, bodies in worlds:
             b.add_world(d, bodies)
         b.set_param(beatbox.PARAM_TEAM_BODIES, team)
+        b.set_param(beatbox.PARAM_SORT_BLOCKS, sort_blocks)
+        b.set_param(beatbox.PARAM_PGS_PREFETCH, 1 if sort_blocks == 1 else 0)
         b.step(8, use_graph=True)
         b.sync()
         runs.append(b)

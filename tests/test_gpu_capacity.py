@@ -109,3 +109,32 @@ def test_grown_world_checkpoint_continues_bit_identically():
         assert x[f].tobytes() == y[f].tobytes(), f
     a.destroy()
     b.destroy()
+
+
+def test_batch_pair_scale_gives_room_and_equals_growth():
+    """BB_PARAM_PAIR_SCALE: the same dense world that overflows a batch at
+    the default capacity fits at four times it, reports no overflow, and
+    equals the single world that grows its capacity and replays."""
+    desc = beatbox.WorldDesc.make(**DESC)
+    bodies = scenes.dense_spheres(0xBB76)
+    b = beatbox.Batch()
+    b.set_param(beatbox.PARAM_PAIR_SCALE, 4)
+    assert b.get_param(beatbox.PARAM_PAIR_SCALE) == 4
+    b.add_world(desc, bodies)
+    b.add_world(desc, scenes.random_boxes(0xBB79, 40, extent=2.0, y0=0.55, y1=3.0))
+    b.step(1)
+    b.sync()  # no overflow mark although the first step has more pairs than 8n + 64
+    assert b.stats(0)["broad_phase_collision_count"] > 8 * len(bodies) + 64
+    b.step(3, use_graph=True)
+    b.sync()
+    assert b.overflow() == []
+    w = beatbox.RigidBodyManager(desc)
+    w.add_bodies(bodies)
+    w.simulate(4)
+    got, ref = b.read_world(0, len(bodies)), w.read_bodies()
+    for f in ("position", "rotation", "velocity", "omega", "flags"):
+        assert got[f].tobytes() == ref[f].tobytes(), f
+    with pytest.raises(beatbox.BBError):
+        b.set_param(beatbox.PARAM_PAIR_SCALE, 0)
+    b.destroy()
+    w.destroy()

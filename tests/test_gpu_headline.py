@@ -77,13 +77,19 @@ def test_batch_4096_worlds_sampled_equal_oracle_and_alone():
     b.destroy()
 
 
-def test_serving_stream_shape_equals_alone():
+@pytest.mark.parametrize("team_bodies,split_bodies", [(1024, 0), (1024, 1024), (8192, 1024)])
+def test_serving_stream_shape_equals_alone(team_bodies, split_bodies):
+    """One lockstep pool, two pools (one workgroup per world below 1,024
+    bodies, teams above) and three pools (one workgroup, a 512-thread
+    workgroup from 1,024 bodies, teams from 8,192), each stepping at its own
+    cadence: every world equals itself stepped alone."""
     desc = beatbox.WorldDesc.make(**scenes.world_desc(solver=scenes.SOLVER_AVBD, iterations=10,
                                                       morton_bits=63, quirks=0))
     sizes = [20000, 64, 3000, 150, 1500, 700, 9000, 90, 2500, 400, 1100, 5000]
     horizons = [7, 12, 9, 6, 11, 8, 6, 10, 7, 9, 8, 6]
     reqs = [(scenes.serving_pile(0xBB50 + k, n), h) for k, (n, h) in enumerate(zip(sizes, horizons))]
-    srv = beatbox.Server(budget_bytes=24 << 30, max_worlds=6, team_bodies=1024)
+    print(f"serving stream: team_bodies {team_bodies}, split_bodies {split_bodies}", flush=True)
+    srv = beatbox.Server(budget_bytes=24 << 30, max_worlds=6, team_bodies=team_bodies, split_bodies=split_bodies)
     tickets = []
     for k, (bodies, h) in enumerate(reqs):
         tickets.append(srv.submit(desc, bodies, h))
@@ -92,6 +98,13 @@ def test_serving_stream_shape_equals_alone():
     rep = srv.report()
     assert rep["retired"] == len(reqs) and rep["reruns"] == 0
     assert rep["step_ms_p99"] >= rep["step_ms_p50"] > 0
+    pools = rep["pools"]
+    assert len(pools) == (1 if not split_bodies else (2 if team_bodies <= split_bodies else 3))
+    for k, p in enumerate(pools):  # every world's steps counted once, in the pool of its size
+        hi = pools[k + 1]["min_bodies"] if k + 1 < len(pools) else 1 << 32
+        want = sum(h for n, h in zip(sizes, horizons) if p["min_bodies"] <= n < hi)
+        assert p["world_steps"] == want, (k, p)
+        assert p["step_ms_p99"] >= p["step_ms_p50"] > 0
     for t, (bodies, h) in zip(tickets, reqs):
         state, got, stats = srv.poll(t, len(bodies))
         assert state == beatbox.REQ_DONE
