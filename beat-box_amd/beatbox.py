@@ -183,6 +183,8 @@ PARAM_SORT_BLOCKS = 8
 PARAM_FUSED = 9
 PARAM_AVBD_LANES = 10
 PARAM_PGS_PREFETCH = 11
+PARAM_WALK_BUDGET = 12
+PARAM_TEAM_BLOCKS = 13
 
 STATUS = {0: "BB_OK", 1: "BB_ERR_INVALID", 2: "BB_ERR_CAPACITY", 3: "BB_ERR_DEVICE",
           4: "BB_ERR_NO_DEVICE", 5: "BB_ERR_OVERFLOW", 6: "BB_ERR_STATE", 7: "BB_ERR_BUDGET"}
@@ -261,6 +263,8 @@ EXPORTS = {
     "bb_server_poll": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(C.c_uint32), C.c_void_p, C.c_uint32,
                                  C.POINTER(StepStats)]),
     "bb_server_report_get": (C.c_int, [C.c_void_p, C.POINTER(ServerReport)]),
+    "bb_server_checkpoint": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.POINTER(C.c_size_t)]),
+    "bb_server_restore": (C.c_int, [C.POINTER(ServerDesc), C.c_void_p, C.c_size_t, C.POINTER(C.c_void_p)]),
     "bb_server_destroy": (None, [C.c_void_p]),
     "bb_version": (C.c_char_p, []),
     "bb_copy_bandwidth": (C.c_int, [C.c_uint64, C.c_uint32, C.POINTER(C.c_float)]),
@@ -631,6 +635,28 @@ class Server:
 
     def drain(self) -> None:
         _check(lib().bb_server_drain(self._h), "bb_server_drain")
+
+    def checkpoint(self) -> bytes:
+        """The serving loop's checkpoint (bb_server_checkpoint): resident
+        worlds with their state, queued requests, unpolled results."""
+        size = C.c_size_t(0)
+        _check(lib().bb_server_checkpoint(self._h, None, 0, C.byref(size)), "bb_server_checkpoint")
+        buf = (C.c_char * size.value)()
+        _check(lib().bb_server_checkpoint(self._h, buf, size.value, C.byref(size)), "bb_server_checkpoint")
+        return bytes(buf)
+
+    @classmethod
+    def restore(cls, blob: bytes, budget_bytes: int, max_worlds: int = 1024, max_bodies: int = 0,
+                max_resident_bodies: int = 0, block: int = 0, team_bodies: int = 0,
+                split_bodies: int = 0) -> "Server":
+        """A new server holding a checkpoint's worlds (bb_server_restore)."""
+        self = cls.__new__(cls)
+        self._h = C.c_void_p()
+        d = ServerDesc(budget_bytes, max_worlds, max_bodies, max_resident_bodies, block, team_bodies,
+                       split_bodies)
+        buf = (C.c_char * len(blob)).from_buffer_copy(blob)
+        _check(lib().bb_server_restore(C.byref(d), buf, len(blob), C.byref(self._h)), "bb_server_restore")
+        return self
 
     def poll(self, ticket: int, n: int = 0):
         """(state, bodies or None, stats or None); results are released once

@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <functional>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -214,6 +215,7 @@ bb_status alloc_arrays(bb_batch* b, Buffers& A, uint32_t W, uint32_t NB, uint32_
   d.sort_blocks = b->sort_blocks;
   d.avbd_lanes = b->avbd_lanes;
   d.pgs_prefetch = b->pgs_prefetch;
+  d.walk_budget = b->walk_budget;
   d.teams = A.alloc<TeamTable>(1);
   d.team_ctl = A.alloc<TeamCtl>(kMaxTeams);
   d.team_hist = A.alloc<uint32_t>((size_t)kMaxTeamBlocks * 256);
@@ -431,8 +433,17 @@ void world_order(const std::vector<WorldInfo>& infos, uint32_t* out) {
   std::stable_sort(out, out + W, [&](uint32_t x, uint32_t y) { return key(x) > key(y); });
 }
 
+// Workgroups a k_team launch may use: BB_PARAM_TEAM_BLOCKS, never more than
+// the device holds resident at once (every member of a team must run at the
+// same time); by default half of that, leaving room for the lanes' kernels.
+uint32_t team_grid_cap(const bb_batch* b) {
+  const uint32_t resident = team_capacity();
+  if (b->team_blocks) return std::min(b->team_blocks, resident);
+  return resident / 2;
+}
+
 bb_status upload_teams(bb_batch* b) {
-  const uint32_t cap = team_capacity() / 2;
+  const uint32_t cap = team_grid_cap(b);
   b->base.team_grid = assign_teams(b, cap);
   HIPCHK(hipMemcpy(b->base.teams, &b->team_table, sizeof(TeamTable), hipMemcpyHostToDevice));
   HIPCHK(hipMemset(b->base.team_ctl, 0, sizeof(TeamCtl) * kMaxTeams));
@@ -963,38 +974,39 @@ struct SnapShape {
 // the voxel shape section
 size_t snap_bytes(uint32_t n, uint32_t p, uint32_t nshapes = 0, uint32_t shape_words = 0) {
   return sizeof(SnapHeader) + (size_t)nshapes * sizeof(SnapShape) + (size_t)shape_words * 4 +
-         (size_t)n * sizeof(bb_body_desc) + (size_t)n * (6 * 4 + 5 * 16) + (size_t)p * (8 + 6 * 16) +
-         (size_t)p * 8 * (6 * 16 + 4) + ST_COUNT * 4;
+         (size_t)n * sizeof(bb_body_desc) + world_state_bytes(n, p);
 }
 
-// The blob's arrays in order, each with its device source (world slice of the
-// newest parity) and size; `fn(dev, bytes)` sees them in blob order.
-template <class F>
-void snap_arrays(bb_batch* b, uint32_t n, uint32_t p, F&& fn) {
-  const HostWorld& hw = b->worlds[0];
+size_t world_state_bytes(uint32_t n, uint32_t p) {
+  return (size_t)n * (6 * 4 + 5 * 16) + (size_t)p * (8 + 6 * 16) + (size_t)p * 8 * (6 * 16 + 4) + ST_COUNT * 4;
+}
+
+// The state arrays of world slot w in parity `parity`, in blob order, each
+// with its device address and size.
+void world_state_arrays(bb_batch* b, uint32_t w, int parity, uint32_t n, uint32_t p,
+                        const std::function<void(void*, size_t)>& fn) {
+  const HostWorld& hw = b->worlds[w];
   const uint32_t g0 = hw.body_off, s0 = hw.pair_off;
-  const int L = b->latest;
-  const Rows& R = b->rows[L];
-  const Manifolds& M = b->man[L];
+  const Rows& R = b->rows[parity];
+  const Manifolds& M = b->man[parity];
   fn((void*)(R.id + g0), (size_t)n * 4);
   fn((void*)(R.flags + g0), (size_t)n * 4);
   fn((void*)(R.sleep + g0), (size_t)n * 4);
-  fn((void*)(b->entries[L] + g0), (size_t)n * 4);
-  fn((void*)(b->pstart[L] + g0), (size_t)n * 4);
+  fn((void*)(b->entries[parity] + g0), (size_t)n * 4);
+  fn((void*)(b->pstart[parity] + g0), (size_t)n * 4);
   fn((void*)(b->base.islot + g0), (size_t)n * 4);
   fn((void*)(R.pos + g0), (size_t)n * 16);
   fn((void*)(R.rot + g0), (size_t)n * 16);
   fn((void*)(R.vel + g0), (size_t)n * 16);
   fn((void*)(R.omg + g0), (size_t)n * 16);
   fn((void*)(R.pvel + g0), (size_t)n * 16);
-  fn((void*)(b->pairs[L] + s0), (size_t)p * 8);
+  fn((void*)(b->pairs[parity] + s0), (size_t)p * 8);
   for (float4* a : {M.nrm, M.t0, M.t1, M.anca, M.ancb}) fn((void*)(a + s0), (size_t)p * 16);
   fn((void*)(M.hdr + s0), (size_t)p * 16);
   for (float4* a : {M.cpos, M.clam, M.cmas, M.cana, M.canb}) fn((void*)(a + (size_t)s0 * 8), (size_t)p * 8 * 16);
   fn((void*)(M.cfp + (size_t)s0 * 8), (size_t)p * 8 * 16);
   fn((void*)(M.cws + (size_t)s0 * 8), (size_t)p * 8 * 4);
-  fn((void*)(b->base.stats), (size_t)ST_COUNT * 4);
-  (void)hw;
+  fn((void*)(b->base.stats + (size_t)w * ST_COUNT), (size_t)ST_COUNT * 4);
 }
 
 bb_status snapshot_world(bb_batch* b, void* dst, size_t cap, size_t* size) {
@@ -1038,7 +1050,7 @@ bb_status snapshot_world(bb_batch* b, void* dst, size_t cap, size_t* size) {
   std::memcpy(o, hw.bodies.data(), (size_t)n * sizeof(bb_body_desc));
   o += (size_t)n * sizeof(bb_body_desc);
   hipError_t rc = hipSuccess;
-  snap_arrays(b, n, p, [&](void* src, size_t bytes) {
+  world_state_arrays(b, 0, b->latest, n, p, [&](void* src, size_t bytes) {
     if (rc == hipSuccess && bytes) rc = hipMemcpy(o, src, bytes, hipMemcpyDeviceToHost);
     o += bytes;
   });
@@ -1128,7 +1140,7 @@ bb_status restore_world(const void* src, size_t size, int warm_start, bb_world**
   if (h.npairs > b->worlds[0].pair_cap) return fail(BB_ERR_INVALID);
   HIPCHK(hipStreamSynchronize(b->stream));
   hipError_t rc = hipSuccess;
-  snap_arrays(b, h.n, h.npairs, [&](void* dst, size_t bytes) {
+  world_state_arrays(b, 0, b->latest, h.n, h.npairs, [&](void* dst, size_t bytes) {
     if (rc == hipSuccess && bytes) rc = hipMemcpy(dst, in, bytes, hipMemcpyHostToDevice);
     in += bytes;
   });
@@ -1186,6 +1198,11 @@ bb_status set_param(bb_batch* b, bb_param p, uint32_t value) {
   } else if (p == BB_PARAM_SORT_BLOCKS) {
     if (value > kMaxTeamBlocks) return BB_ERR_INVALID;
     b->sort_blocks = b->base.sort_blocks = value;
+  } else if (p == BB_PARAM_WALK_BUDGET) {
+    b->walk_budget = b->base.walk_budget = value;
+  } else if (p == BB_PARAM_TEAM_BLOCKS) {
+    if (value > kMaxTeamBlocks) return BB_ERR_INVALID;
+    b->team_blocks = value;
   } else if (p == BB_PARAM_PAIR_SCALE) {
     if (value < 1 || value > 16) return BB_ERR_INVALID;
     if (value != b->pair_scale) {
@@ -1202,7 +1219,8 @@ bb_status set_param(bb_batch* b, bb_param p, uint32_t value) {
     (void)hipGraphExecDestroy(b->graph);
     b->graph = nullptr;
   }
-  if (b->built && (p == BB_PARAM_BLOCK || p == BB_PARAM_TEAM_BODIES || p == BB_PARAM_LANE_BODIES))
+  if (b->built && (p == BB_PARAM_BLOCK || p == BB_PARAM_TEAM_BODIES || p == BB_PARAM_LANE_BODIES ||
+                   p == BB_PARAM_TEAM_BLOCKS))
     return upload_teams(b);
   return BB_OK;
 }
@@ -1261,7 +1279,8 @@ bb_status autotune_batch(bb_batch* b, uint32_t reps, float* best_ms) {
   float best = 3.4e38f;
   uint32_t best_block = saved_block, best_lds = saved_lds;
   const uint32_t saved_fused = b->fused, saved_sort = b->sort_blocks, saved_lanes = b->avbd_lanes,
-                 saved_prefetch = b->pgs_prefetch;
+                 saved_prefetch = b->pgs_prefetch, saved_walk = b->walk_budget,
+                 saved_team_blocks = b->team_blocks;
   bb_status rc = BB_OK;
   // The wide lane is not tuned here: one replayed step early in a run does
   // not show how much the large worlds come to dominate later (over the
@@ -1326,7 +1345,23 @@ bb_status autotune_batch(bb_batch* b, uint32_t reps, float* best_ms) {
     if (b->sort_blocks == 0) b->sort_blocks = b->base.sort_blocks = kTeamSortBlocks;
     tune(b->sort_blocks, &b->base.sort_blocks, {8u, 16u, 32u, 64u, 128u});
   }
+  if (b->walk_budget == 0) b->walk_budget = b->base.walk_budget = kWalkBudgetDefault;
+  tune(b->walk_budget, &b->base.walk_budget, {32u, 64u, 128u, 512u, kMAX});
+  // the team launch: half the resident workgroups (the default) or all of
+  // them (the teams are laid out anew for each)
+  if (b->base.team_grid && rc == BB_OK) {
+    const uint32_t keep = b->team_blocks, all = team_capacity();
+    b->team_blocks = all;
+    if ((rc = upload_teams(b)) == BB_OK) {
+      float ms = 0.0f;
+      if ((rc = measure(ms)) == BB_OK && ms < best) best = ms;
+      else b->team_blocks = keep;
+      if (rc == BB_OK) rc = upload_teams(b);
+    }
+  }
   if (rc != BB_OK) {
+    b->walk_budget = b->base.walk_budget = saved_walk;
+    b->team_blocks = saved_team_blocks;
     b->fused = saved_fused;
     b->sort_blocks = b->base.sort_blocks = saved_sort;
     b->avbd_lanes = b->base.avbd_lanes = saved_lanes;
@@ -1996,6 +2031,8 @@ uint32_t bb_batch_get_param(const bb_batch* b, bb_param p) {
   if (p == BB_PARAM_PGS_PREFETCH) return b->pgs_prefetch;
   if (p == BB_PARAM_AVBD_LANES) return b->avbd_lanes ? b->avbd_lanes : 4u;
   if (p == BB_PARAM_SORT_BLOCKS) return b->sort_blocks ? b->sort_blocks : kTeamSortBlocks;
+  if (p == BB_PARAM_WALK_BUDGET) return b->walk_budget ? b->walk_budget : kWalkBudgetDefault;
+  if (p == BB_PARAM_TEAM_BLOCKS) return has_device() ? team_grid_cap(b) : b->team_blocks;
   return 0;
 }
 

@@ -21,6 +21,9 @@ static constexpr uint32_t kBuckets = kColors + 1;  // + overflow bucket
 static constexpr uint32_t kAvbdColors = 32;
 static constexpr uint32_t kAvbdLayers = 12;
 static constexpr uint32_t kMAX = 0xFFFFFFFFu;
+// Broad phase: nodes one thread walks for its row before the row is handed to
+// a whole workgroup (BB_PARAM_WALK_BUDGET's default)
+static constexpr uint32_t kWalkBudgetDefault = 128;
 
 // per-world statistics slots (bb_step_stats order)
 enum StatSlot : uint32_t {
@@ -223,6 +226,8 @@ struct Dev {
   uint32_t sort_blocks;   // workgroups of a team that sort (0: kTeamSortBlocks)
   uint32_t avbd_lanes;    // lanes per body in the per-world AVBD primal sweeps (1, 2, 4, 8; 0: 4)
   uint32_t pgs_prefetch;  // 1: the colour sweeps fetch each thread's next-colour row early
+  uint32_t walk_budget;   // nodes a broad-phase thread walks before its workgroup takes the row
+                          // (0: kWalkBudgetDefault)
   // set (bit 0) by any world whose step flags an overflow; the host reads
   // and clears it
   uint32_t* ovf_any;

@@ -7,6 +7,7 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <functional>
 #include <vector>
 
 #include "../../include/bb_capi.h"
@@ -160,6 +161,8 @@ struct bb_batch {
   uint32_t sort_blocks = 0;  // BB_PARAM_SORT_BLOCKS (0: kTeamSortBlocks)
   uint32_t avbd_lanes = 0;   // BB_PARAM_AVBD_LANES (0: 4)
   uint32_t pgs_prefetch = 0; // BB_PARAM_PGS_PREFETCH
+  uint32_t walk_budget = 0;  // BB_PARAM_WALK_BUDGET (0: kWalkBudgetDefault)
+  uint32_t team_blocks = 0;  // BB_PARAM_TEAM_BLOCKS (0: half the resident workgroups)
   uint32_t n_wide = 0;  // worlds in that lane: the first n_wide of the order table
   hipStream_t stream2 = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
@@ -252,6 +255,19 @@ bb_status launch_one(bb_batch* b);
 bb_status read_world(bb_batch* b, uint32_t w, bb_body_state* out, uint32_t n, bb_order order);
 // Counters of world w's last step.
 bb_status world_stats(bb_batch* b, uint32_t w, bb_step_stats* out);
+// Checkpoint arrays of a world: what its next step reads from the device
+// (rows of one parity in row order, the id -> row map, pair-row offsets,
+// island slots, the pair list and the manifold slots that warm start it) and
+// its counters.  world_state_bytes is their size for n rows and p pairs;
+// world_state_arrays hands each array's device address (world slot w,
+// parity `parity`) and size to fn, in blob order.
+size_t world_state_bytes(uint32_t n, uint32_t p);
+void world_state_arrays(bb_batch* b, uint32_t w, int parity, uint32_t n, uint32_t p,
+                        const std::function<void(void*, size_t)>& fn);
+// Host check of the indices in a world's checkpoint arrays (ids a
+// permutation of [0, n), the id -> row map consistent with them, pair rows
+// and manifold bodies below n, at most 8 contacts per manifold).
+bool valid_blob_indices(const char* in, uint32_t n, uint32_t p);
 
 }  // namespace host
 }  // namespace bb

@@ -128,6 +128,11 @@ struct Request {
   uint32_t pair_scale = 1;
   bool no_team = false;
   uint32_t reruns = 0;
+  // a world carried over by a checkpoint: its state arrays (the order of
+  // world_state_arrays), its pair count and the steps it had done; admission
+  // puts them into the newest parity of its pool instead of a fresh start
+  std::vector<char> state;
+  uint32_t npairs = 0, done = 0;
 };
 
 // Pair-capacity doublings and team fallbacks a request may take before it
@@ -293,6 +298,9 @@ bb_status finish_oldest(bb_server* s, uint32_t p) {
       if (bytes <= s->desc.hbm_budget_bytes && (uint64_t)pair_capacity(nb) * scale <= s->NP) {
         Request q = std::move(oit->second);
         s->origin.erase(oit);
+        q.state.clear();  // a rerun starts from the first step
+        q.done = 0;
+        q.npairs = 0;
         q.pair_scale = scale;
         if (ovf & 2u) q.no_team = true;
         ++q.reruns;
@@ -383,6 +391,7 @@ bb_status pool_step(bb_server* s, uint32_t p) {
     sl.fresh = true;
     sl.ticket = q.ticket;
     sl.steps = q.steps;
+    sl.done = q.done;
     sl.bytes = q.bytes;
     sl.body_off = bo;
     sl.n = n;
@@ -406,12 +415,13 @@ bb_status pool_step(bb_server* s, uint32_t p) {
     s->resident_bodies += n;
     s->running[q.ticket] = w;
     ++P.running;
-    if (!q.reruns) {
+    if (!q.reruns && q.state.empty()) {
       s->queue_samples.push_back(t_now - q.submit_ms);
       ++s->admitted_n;
     }
     const uint64_t tk = q.ticket;
     stage_bytes += kImageBodyBytes * std::max(n, 1u) + kImageActiveBytes * std::max(q.active, 1u) + 64;
+    stage_bytes += q.state.size() + 64;
     admitted.push_back(w);
     s->origin[tk] = std::move(q);
     P.queue.pop_front();
@@ -446,6 +456,27 @@ bb_status pool_step(bb_server* s, uint32_t p) {
     if (st != BB_OK) return st;
     st = reset_world_state(b, w, stream);
     if (st != BB_OK) return st;
+    Request& org = s->origin[s->slots[w].ticket];
+    if (!org.state.empty()) {
+      // a checkpointed world: its rows, pairs, manifolds and counters go
+      // into the parity the next step reads (after the fresh image above,
+      // in stream order), with its pair count, so it warm starts as it
+      // would have in the server that wrote the checkpoint
+      char* src = (char*)ch.take(org.state.size() + 4);
+      std::memcpy(src, org.state.data(), org.state.size());
+      std::memcpy(src + org.state.size(), &org.npairs, 4);
+      hipError_t rc = hipSuccess;
+      const char* at = src;
+      world_state_arrays(b, w, P.latest, n, org.npairs, [&](void* dst, size_t bytes) {
+        if (rc == hipSuccess && bytes) rc = hipMemcpyAsync(dst, at, bytes, hipMemcpyHostToDevice, stream);
+        at += bytes;
+      });
+      if (rc == hipSuccess)
+        rc = hipMemcpyAsync(b->npairs[P.latest] + w, src + org.state.size(), 4, hipMemcpyHostToDevice, stream);
+      HIPCHK(rc);
+      org.state.clear();
+      org.state.shrink_to_fit();
+    }
     WorldInfo wi = info_of(hw);
     wi.mode = 0;  // the cleared pair counts already give an empty previous step
     b->infos[w] = wi;
@@ -830,6 +861,242 @@ bb_status bb_server_poll(bb_server* s, uint64_t ticket, uint32_t* state, bb_body
   if (stats) *stats = it->second.stats;
   *state = it->second.failed ? BB_REQ_FAILED : BB_REQ_DONE;
   s->results.erase(it);
+  return BB_OK;
+}
+
+// ---- checkpoint of the serving loop.  Blob: a header, then one record per
+// world (resident worlds first, in slot order, then the queues, pool by pool,
+// in queue order), then one per result not yet polled.  A resident world's
+// record carries its request (descriptor, body descriptors, steps, reruns)
+// and its state arrays; a queued world's carries its request only.
+struct CkptHeader {
+  char magic[8];
+  uint32_t version, worlds, results, pad;
+  uint64_t next_ticket;
+};
+struct CkptWorld {
+  uint64_t ticket;
+  bb_world_desc desc;
+  uint32_t n, active, steps, done;
+  uint32_t pair_scale, no_team, reruns, npairs;
+  uint32_t resident, pad;
+};
+struct CkptResult {
+  uint64_t ticket;
+  uint32_t n, failed;
+  bb_step_stats stats;
+};
+constexpr char kCkptMagic[8] = {'B', 'B', 'S', 'R', 'V', 'C', 'K', '1'};
+constexpr uint32_t kCkptVersion = 1;
+// pair-capacity doublings a checkpointed request may carry
+constexpr uint32_t kMaxPairScale = 1u << kMaxReruns;
+
+bb_status bb_server_checkpoint(bb_server* s, void* dst, size_t cap, size_t* size) {
+  if (!s || !size) return BB_ERR_INVALID;
+  // the steps in flight complete first: their read-backs become results, or
+  // go back to a queue to run again
+  for (uint32_t p = 0; p < (uint32_t)s->pools.size(); ++p)
+    while (!s->pools[p].inflight.empty()) {
+      bb_status st = finish_oldest(s, p);
+      if (st != BB_OK) return st;
+    }
+  for (const Pool& P : s->pools) HIPCHK(hipStreamSynchronize(P.stream));
+  bb_batch* b = s->b;
+  struct Res {
+    uint32_t w, npairs;
+    int parity;  // the newest parity of the world's pool
+    const Request* q;
+  };
+  std::vector<Res> res;
+  for (uint32_t p = 0; p < (uint32_t)s->pools.size(); ++p) {
+    const Pool& P = s->pools[p];
+    for (uint32_t w = P.w0; w < P.w1; ++w) {
+      const Slot& sl = s->slots[w];
+      if (!sl.used) continue;
+      auto it = s->origin.find(sl.ticket);
+      if (it == s->origin.end()) return BB_ERR_INVALID;
+      uint32_t np = 0;
+      HIPCHK(hipMemcpy(&np, b->npairs[P.latest] + w, 4, hipMemcpyDeviceToHost));
+      res.push_back(Res{w, std::min(np, sl.pair_cap), P.latest, &it->second});
+    }
+  }
+  size_t bytes = sizeof(CkptHeader);
+  uint32_t nworlds = (uint32_t)res.size();
+  for (const Res& r : res)
+    bytes += sizeof(CkptWorld) + r.q->bodies.size() * sizeof(bb_body_desc) +
+             world_state_bytes(s->slots[r.w].n, r.npairs);
+  for (const Pool& P : s->pools)
+    for (const Request& q : P.queue) {
+      bytes += sizeof(CkptWorld) + q.bodies.size() * sizeof(bb_body_desc) + q.state.size();
+      ++nworlds;
+    }
+  for (const auto& kv : s->results) bytes += sizeof(CkptResult) + kv.second.bodies.size() * sizeof(bb_body_state);
+  *size = bytes;
+  if (!dst) return BB_OK;
+  if (cap < bytes) return BB_ERR_CAPACITY;
+  char* o = (char*)dst;
+  auto put = [&](const void* src, size_t n) {
+    if (n) std::memcpy(o, src, n);
+    o += n;
+  };
+  CkptHeader h{};
+  std::memcpy(h.magic, kCkptMagic, 8);
+  h.version = kCkptVersion;
+  h.worlds = nworlds;
+  h.results = (uint32_t)s->results.size();
+  h.next_ticket = s->next_ticket;
+  put(&h, sizeof(h));
+  auto record = [&](const Request& q, uint32_t done, uint32_t npairs, bool resident) {
+    CkptWorld r{};
+    r.ticket = q.ticket;
+    r.desc = q.desc;
+    r.n = (uint32_t)q.bodies.size();
+    r.active = q.active;
+    r.steps = q.steps;
+    r.done = done;
+    r.pair_scale = q.pair_scale;
+    r.no_team = q.no_team ? 1u : 0u;
+    r.reruns = q.reruns;
+    r.npairs = npairs;
+    r.resident = resident ? 1u : 0u;
+    put(&r, sizeof(r));
+    put(q.bodies.data(), q.bodies.size() * sizeof(bb_body_desc));
+  };
+  for (const Res& r : res) {
+    const Slot& sl = s->slots[r.w];
+    record(*r.q, sl.done, r.npairs, true);
+    hipError_t rc = hipSuccess;
+    world_state_arrays(b, r.w, r.parity, sl.n, r.npairs, [&](void* src, size_t n) {
+      if (rc == hipSuccess && n) rc = hipMemcpy(o, src, n, hipMemcpyDeviceToHost);
+      o += n;
+    });
+    HIPCHK(rc);
+  }
+  for (const Pool& P : s->pools)
+    for (const Request& q : P.queue) {
+      // a request queued with a checkpointed state (restored, not yet
+      // admitted) keeps it
+      record(q, q.done, q.npairs, !q.state.empty());
+      put(q.state.data(), q.state.size());
+    }
+  for (const auto& kv : s->results) {
+    CkptResult r{};
+    r.ticket = kv.first;
+    r.n = (uint32_t)kv.second.bodies.size();
+    r.failed = kv.second.failed ? 1u : 0u;
+    r.stats = kv.second.stats;
+    put(&r, sizeof(r));
+    put(kv.second.bodies.data(), kv.second.bodies.size() * sizeof(bb_body_state));
+  }
+  return BB_OK;
+}
+
+bb_status bb_server_restore(const bb_server_desc* desc, const void* src, size_t size, bb_server** out) {
+  if (!desc || !src || !out || size < sizeof(CkptHeader)) return BB_ERR_INVALID;
+  *out = nullptr;
+  const char* in = (const char*)src;
+  const char* end = in + size;
+  CkptHeader h;
+  std::memcpy(&h, in, sizeof(h));
+  in += sizeof(h);
+  if (std::memcmp(h.magic, kCkptMagic, 8) != 0 || h.version != kCkptVersion) return BB_ERR_INVALID;
+  // the whole blob is checked on the host before a server is made
+  struct Rec {
+    CkptWorld r;
+    const char* bodies;
+    const char* state;
+  };
+  std::vector<Rec> recs;
+  uint64_t max_ticket = 0;
+  auto take = [&](size_t n) -> const char* {
+    if ((size_t)(end - in) < n) return nullptr;
+    const char* p = in;
+    in += n;
+    return p;
+  };
+  for (uint32_t k = 0; k < h.worlds; ++k) {
+    const char* p = take(sizeof(CkptWorld));
+    if (!p) return BB_ERR_INVALID;
+    Rec rc{};
+    std::memcpy(&rc.r, p, sizeof(CkptWorld));
+    const CkptWorld& r = rc.r;
+    if (!valid_desc(&r.desc) || r.n == 0 || r.steps == 0 || r.done >= r.steps || r.pair_scale == 0 ||
+        r.pair_scale > kMaxPairScale || r.reruns > kMaxReruns || r.ticket == 0 || r.resident > 1 ||
+        (!r.resident && (r.done || r.npairs)) || over_reference_cap(r.desc, r.n))
+      return BB_ERR_INVALID;
+    if ((uint64_t)r.npairs > (uint64_t)pair_capacity(r.n) * r.pair_scale) return BB_ERR_INVALID;
+    rc.bodies = take((size_t)r.n * sizeof(bb_body_desc));
+    if (!rc.bodies) return BB_ERR_INVALID;
+    uint32_t a = 0;
+    for (uint32_t i = 0; i < r.n; ++i) {
+      bb_body_desc bd;
+      std::memcpy(&bd, rc.bodies + (size_t)i * sizeof(bb_body_desc), sizeof(bd));
+      if (!valid_body(bd)) return BB_ERR_INVALID;
+      if (bd.flags & BB_BODY_DYNAMIC) ++a;
+    }
+    if (a != r.active) return BB_ERR_INVALID;
+    if (r.resident) {
+      rc.state = take(world_state_bytes(r.n, r.npairs));
+      if (!rc.state || !valid_blob_indices(rc.state, r.n, r.npairs)) return BB_ERR_INVALID;
+    }
+    max_ticket = std::max(max_ticket, r.ticket);
+    recs.push_back(rc);
+  }
+  struct ResRec {
+    CkptResult r;
+    const char* bodies;
+  };
+  std::vector<ResRec> results;
+  for (uint32_t k = 0; k < h.results; ++k) {
+    const char* p = take(sizeof(CkptResult));
+    if (!p) return BB_ERR_INVALID;
+    ResRec rr{};
+    std::memcpy(&rr.r, p, sizeof(CkptResult));
+    if (rr.r.n > (1u << 30) || rr.r.failed > 1) return BB_ERR_INVALID;
+    rr.bodies = take((size_t)rr.r.n * sizeof(bb_body_state));
+    if (!rr.bodies) return BB_ERR_INVALID;
+    max_ticket = std::max(max_ticket, rr.r.ticket);
+    results.push_back(rr);
+  }
+  if (in != end) return BB_ERR_INVALID;
+  bb_server* s = nullptr;
+  bb_status st = bb_server_create(desc, &s);
+  if (st != BB_OK) return st;
+  for (const Rec& rc : recs) {
+    const CkptWorld& r = rc.r;
+    Request q;
+    q.ticket = r.ticket;
+    q.desc = r.desc;
+    q.bodies.resize(r.n);
+    std::memcpy(q.bodies.data(), rc.bodies, (size_t)r.n * sizeof(bb_body_desc));
+    q.steps = r.steps;
+    q.active = r.active;
+    q.pair_scale = r.pair_scale;
+    q.no_team = r.no_team != 0;
+    q.reruns = r.reruns;
+    q.bytes = price(s, r.n, r.active, r.pair_scale);
+    q.submit_ms = now_ms();
+    if (q.bytes > s->desc.hbm_budget_bytes || r.n > s->NB || (uint64_t)pair_capacity(r.n) * r.pair_scale > s->NP) {
+      bb_server_destroy(s);
+      return BB_ERR_BUDGET;
+    }
+    if (r.resident) {
+      q.state.assign(rc.state, rc.state + world_state_bytes(r.n, r.npairs));
+      q.npairs = r.npairs;
+      q.done = r.done;
+    }
+    s->pools[pool_of(s, r.n)].queue.push_back(std::move(q));
+    ++s->submitted;
+  }
+  for (const ResRec& rr : results) {
+    Result& res = s->results[rr.r.ticket];
+    res.bodies.resize(rr.r.n);
+    std::memcpy(res.bodies.data(), rr.bodies, (size_t)rr.r.n * sizeof(bb_body_state));
+    res.stats = rr.r.stats;
+    res.failed = rr.r.failed != 0;
+  }
+  s->next_ticket = std::max<uint64_t>(h.next_ticket, max_ticket + 1);
+  *out = s;
   return BB_OK;
 }
 

@@ -244,8 +244,12 @@ struct BlockTeam {
   static constexpr uint32_t kAvGroup = kAvG;
   static constexpr bool kSkipEmpty = true;
   static constexpr bool kGlobalScalars = false;  // stage scalars in LDS
+  static constexpr int kBlock = BLOCK;
   BB_DI uint32_t rank() const { return threadIdx.x; }
   BB_DI uint32_t size() const { return BLOCK; }
+  // this workgroup's index in the team, and the team's workgroups
+  BB_DI uint32_t group() const { return 0; }
+  BB_DI uint32_t groups() const { return 1; }
   BB_DI void sync() const { __syncthreads(); }
   BB_DI void fence() const { __threadfence_block(); }
   // every leaf range is this workgroup's (see GroupTeam::leaves_local)
@@ -281,12 +285,17 @@ struct GroupTeam {
   uint32_t spin;   // spins before a wait gives up
   unsigned long long* clk;  // this team's clock marks (profiling launches), or null
   uint32_t sort_blocks;     // the first sort_blocks workgroups run the radix sort
-  // AVBD primal: one lane per body, a barrier after every cascade bucket
-  static constexpr uint32_t kAvGroup = 1;
-  static constexpr bool kSkipEmpty = false;
+  // AVBD primal: kAvG lanes per body; empty cascade buckets skip their
+  // barrier (the bucket offsets are team scalars every workgroup reads after
+  // the same barrier, so all of them skip the same buckets)
+  static constexpr uint32_t kAvGroup = kAvG;
+  static constexpr bool kSkipEmpty = true;
   static constexpr bool kGlobalScalars = true;  // stage scalars in a global slot
+  static constexpr int kBlock = BLOCK;
   BB_DI uint32_t rank() const { return blk * BLOCK + threadIdx.x; }
   BB_DI uint32_t size() const { return nblocks * BLOCK; }
+  BB_DI uint32_t group() const { return blk; }
+  BB_DI uint32_t groups() const { return nblocks; }
   BB_DI void fence() const { __threadfence(); }
   // Clock mark m of this step, called right after a team barrier: every
   // workgroup has reached that point when rank 0 stamps it.
@@ -752,27 +761,51 @@ BB_DI bool overlap(v3 amn, v3 amx, float4 bmn, float4 bmx) {  // Aabb::overlap, 
          amx.z >= bmn.z;
 }
 
-// LBVH::broad_phase (LBVH.slang:223-286): stack traversal from the root,
-// one pair (i, j) per overlapping leaf with row j > i.  Each node record
-// carries its children's indices (nmin.w, nmax.w), so descending needs one
-// dependent load per level and the stack holds child-index pairs.  emit(k, j)
-// receives the k-th partner j (traversal order); returns the partner count.
-template <class Emit>
-BB_DI uint32_t traverse(const Dev& d, uint32_t nb, uint32_t N, uint32_t i, v3 qmn, v3 qmx, Emit&& emit) {
-  if (N < 2) return 0;
+// LBVH::broad_phase (LBVH.slang:223-286): stack traversal, one pair (i, j)
+// per overlapping leaf with row j > i.  Each node record carries its
+// children's indices (nmin.w, nmax.w), so descending needs one dependent load
+// per level and the stack holds child-index pairs.  walk() starts below a
+// node whose children are (l, r); emit(k, j) receives the k-th partner j.
+// Partners come out in ascending row order: leaf k holds row k, and a Karras
+// node's left subtree covers the rows before its right subtree's.
+//
+// A left subtree whose rows all lie at or below i is skipped without loading
+// it: the left child of a node with split s covers rows [first, s], and s
+// follows from the right child's index (s + 1, or the leaf N - 1 + s + 1),
+// so the pairs are those of the full traversal.
+//
+// `budget` bounds the nodes a thread expands; when it runs out, over_budget()
+// decides: true hands the row over (walk returns kMAX), false walks on.
+template <class Emit, class Over>
+BB_DI uint32_t walk(const Dev& d, uint32_t nb, uint32_t N, uint32_t i, v3 qmn, v3 qmx, int l, int r,
+                    uint32_t budget, Over&& over_budget, Emit&& emit) {
   // depth of a Karras tree is at most key bits + index bits (63 + 32 < 128)
-  const int nn = (int)(2 * N - 1);
+  const int nn = (int)(2 * N - 1), leaf_off = (int)N - 1;
   int2 stack[128];
-  uint32_t sp = 0;
-  const float4 r0m = d.nmin[nb], r0x = d.nmax[nb];
-  int l = __float_as_int(r0m.w), r = __float_as_int(r0x.w);
+  uint32_t sp = 0, visits = 0;
   uint32_t cnt = 0;
   for (;;) {
     if (l < 0 || r < 0 || l >= nn || r >= nn) break;
-    const float4 lm = d.nmin[nb + l], lx = d.nmax[nb + l], rm = d.nmin[nb + r], rx = d.nmax[nb + r];
-    const bool ol = overlap(qmn, qmx, lm, lx), orr = overlap(qmn, qmx, rm, rx);
-    const bool ll = __float_as_int(lm.w) < 0, rl = __float_as_int(rm.w) < 0;
-    const uint32_t lp = (uint32_t)__float_as_int(lx.w), rp = (uint32_t)__float_as_int(rx.w);
+    if (++visits > budget) {
+      if (over_budget()) return kMAX;
+      budget = kMAX;
+    }
+    // rows of the left subtree end at its split
+    const bool lskip = (r >= leaf_off ? r - leaf_off : r) - 1 <= (int)i;
+    const float4 rm = d.nmin[nb + r], rx = d.nmax[nb + r];
+    bool ol = false, ll = false;
+    uint32_t lp = 0;
+    float4 lm = rm, lx = rx;
+    if (!lskip) {
+      lm = d.nmin[nb + l];
+      lx = d.nmax[nb + l];
+      ol = overlap(qmn, qmx, lm, lx);
+      ll = __float_as_int(lm.w) < 0;
+      lp = (uint32_t)__float_as_int(lx.w);
+    }
+    const bool orr = overlap(qmn, qmx, rm, rx);
+    const bool rl = __float_as_int(rm.w) < 0;
+    const uint32_t rp = (uint32_t)__float_as_int(rx.w);
     if (ol && ll && i < lp) emit(cnt++, lp);
     if (orr && rl && i < rp) emit(cnt++, rp);
     const bool tl = ol && !ll, tr = orr && !rl;
@@ -791,6 +824,121 @@ BB_DI uint32_t traverse(const Dev& d, uint32_t nb, uint32_t N, uint32_t i, v3 qm
     }
   }
   return cnt;
+}
+
+// The whole traversal of row i from the root.
+template <class Emit>
+BB_DI uint32_t traverse(const Dev& d, uint32_t nb, uint32_t N, uint32_t i, v3 qmn, v3 qmx, Emit&& emit) {
+  if (N < 2) return 0;
+  const float4 r0m = d.nmin[nb], r0x = d.nmax[nb];
+  return walk(d, nb, N, i, qmn, qmx, __float_as_int(r0m.w), __float_as_int(r0x.w), kMAX, [] { return false; },
+              emit);
+}
+
+// A row whose walk runs past its budget (the ground under a pile overlaps
+// thousands of bodies) is walked by a whole workgroup instead: the frontier
+// below the root is widened level by level, in leaf order, until it holds a
+// node per thread, then each thread walks below its frontier nodes.  The
+// frontier's nodes cover disjoint runs of rows in ascending order, so an
+// exclusive scan of the per-thread counts places every partner where the
+// one-thread walk would have put it.
+template <int BLOCK>
+struct WalkLds {
+  uint32_t fr[2][2 * BLOCK];  // frontier nodes (leaf index N - 1 + row for a partner)
+};
+static constexpr uint32_t kMaxDefer = 64;   // rows per world handed to a workgroup
+static constexpr uint32_t kDeferBit = 0x80000000u;  // pcount mark of such a row
+
+// Count (out == nullptr) or write the first `keep` partners of row i to
+// out[0..keep): called by every thread of the workgroup; returns the count.
+template <int BLOCK>
+BB_DI uint32_t coop_walk(const Dev& d, uint32_t nb, uint32_t N, uint32_t i, v3 qmn, v3 qmx, WalkLds<BLOCK>& F,
+                         ScanLds<BLOCK>& S, uint2* out, uint32_t keep) {
+  const uint32_t tid = threadIdx.x;
+  const int nn = (int)(2 * N - 1), leaf_off = (int)N - 1;
+  if (N < 2) return 0;
+  uint32_t cur = 0, nf = 1;
+  if (tid == 0) F.fr[0][0] = 0u;  // the root (tested no more than walk tests it)
+  __syncthreads();
+  for (uint32_t round = 0; round < 128; ++round) {
+    uint32_t n = 0, a = 0, b = 0, inner = 0;
+    if (tid < nf) {
+      const int x = (int)F.fr[cur][tid];
+      if (x >= leaf_off) {
+        a = (uint32_t)x;
+        n = 1;
+      } else {
+        const int l = __float_as_int(d.nmin[nb + x].w), r = __float_as_int(d.nmax[nb + x].w);
+        if (l >= 0 && r >= 0 && l < nn && r < nn) {
+          const bool lskip = (r >= leaf_off ? r - leaf_off : r) - 1 <= (int)i;
+          const float4 rm = d.nmin[nb + r], rx = d.nmax[nb + r];
+          const bool orr = overlap(qmn, qmx, rm, rx);
+          const bool rl = __float_as_int(rm.w) < 0;
+          bool ol = false, ll = false;
+          uint32_t lp = 0;
+          if (!lskip) {
+            const float4 lm = d.nmin[nb + l], lx = d.nmax[nb + l];
+            ol = overlap(qmn, qmx, lm, lx);
+            ll = __float_as_int(lm.w) < 0;
+            lp = (uint32_t)__float_as_int(lx.w);
+          }
+          const uint32_t rp = (uint32_t)__float_as_int(rx.w);
+          uint32_t o[2];
+          if (ol && (!ll || i < lp)) {
+            o[n++] = (uint32_t)l;
+            inner += ll ? 0u : 1u;
+          }
+          if (orr && (!rl || i < rp)) {
+            o[n++] = (uint32_t)r;
+            inner += rl ? 0u : 1u;
+          }
+          a = o[0];
+          b = o[1];
+        }
+      }
+    }
+    // one scan places the new frontier and counts its inner nodes (high
+    // half; the frontier stays below 2 * BLOCK <= 2048 entries)
+    uint32_t total;
+    const uint32_t ex = block_scan<BLOCK>(n | inner << 16, S, total) & 0xFFFFu;  // the old frontier is read
+    if (n > 0) F.fr[cur ^ 1u][ex] = a;
+    if (n > 1) F.fr[cur ^ 1u][ex + 1] = b;
+    __syncthreads();
+    cur ^= 1u;
+    nf = total & 0xFFFFu;
+    if ((total >> 16) == 0u || nf >= (uint32_t)BLOCK) break;
+  }
+  // each thread walks below its frontier nodes (two each past BLOCK nodes)
+  const uint32_t per = nf > (uint32_t)BLOCK ? 2u : 1u;
+  auto below = [&](uint32_t e, auto&& emit) -> uint32_t {
+    const int x = (int)F.fr[cur][e];
+    if (x >= leaf_off) {
+      emit(0u, (uint32_t)(x - leaf_off));
+      return 1u;
+    }
+    return walk(d, nb, N, i, qmn, qmx, __float_as_int(d.nmin[nb + x].w), __float_as_int(d.nmax[nb + x].w), kMAX,
+                [] { return false; }, emit);
+  };
+  uint32_t mine = 0;
+  for (uint32_t q = 0; q < per; ++q) {
+    const uint32_t e = tid * per + q;
+    if (e < nf) mine += below(e, [](uint32_t, uint32_t) {});
+  }
+  uint32_t total;
+  uint32_t at = block_scan<BLOCK>(mine, S, total);
+  if (out) {
+    for (uint32_t q = 0; q < per; ++q) {
+      const uint32_t e = tid * per + q;
+      if (e < nf) {
+        const uint32_t base = at;
+        at += below(e, [&](uint32_t k, uint32_t j) {
+          if (base + k < keep) out[base + k] = make_uint2(i, j);
+        });
+      }
+    }
+  }
+  __syncthreads();  // the frontier is free for the next row
+  return total;
 }
 
 // Rows keep up to kPairStage partners in a staging area (the colouring
@@ -815,11 +963,13 @@ BB_DI void sort8(uint32_t (&k)[8]) {
 
 struct BroadShared {
   uint32_t carry;
-  uint32_t pairs;  // voxel worlds: pairs, as opposed to pair slots
+  uint32_t pairs;   // voxel worlds: pairs, as opposed to pair slots
+  uint32_t ndefer;  // rows handed to a workgroup (coop_walk), listed in defer
+  uint32_t defer[kMaxDefer];
 };
 
 template <class T>
-BB_DI void s_broad(const Dev& d, uint32_t w, const T& t, BroadShared& L) {
+BB_DI void s_broad(const Dev& d, uint32_t w, const T& t, BroadShared& L, WalkLds<T::kBlock>& F) {
   const uint32_t tid = t.rank(), NT = t.size();
   const WorldInfo W = d.worlds[w];
   const uint32_t N = W.n, off = W.body_off, nb = 2 * off;
@@ -834,21 +984,50 @@ BB_DI void s_broad(const Dev& d, uint32_t w, const T& t, BroadShared& L) {
   if (tid == 0) {
     L.carry = 0;
     L.pairs = 0;
+    L.ndefer = 0;
   }
   t.sync();
+  // partner counts (slots of voxel worlds parked in poff); a row whose walk
+  // runs past the budget is listed for a workgroup and marked kDeferBit
+  const uint32_t budget = vox ? kMAX : (d.walk_budget ? d.walk_budget : kWalkBudgetDefault);
+  for (uint32_t i = tid; i < N; i += NT) {
+    const float4 mn = d.nmin[nb + N - 1 + i], mx = d.nmax[nb + N - 1 + i];
+    uint32_t* st = stage + i * kPairStage;
+    const bool vi = vox && is_vox(i);
+    uint32_t slots = 0, cnt = 0;
+    if (N >= 2) {
+      const float4 r0m = d.nmin[nb], r0x = d.nmax[nb];
+      cnt = walk(
+          d, nb, N, i, ld3(mn), ld3(mx), __float_as_int(r0m.w), __float_as_int(r0x.w), budget,
+          [&]() {
+            const uint32_t k = atomicAdd(&L.ndefer, 1u);
+            if (k >= kMaxDefer) return false;
+            L.defer[k] = i;
+            return true;
+          },
+          [&](uint32_t k, uint32_t j) {
+            if (k < kPairStage) st[k] = j;
+            if (vox) slots += (vi || is_vox(j)) ? 6u : 1u;
+          });
+    }
+    d.pcount[off + i] = cnt == kMAX ? kDeferBit : cnt;
+    if (vox) d.poff[off + i] = slots;
+  }
+  t.sync();
+  const uint32_t nd = L.ndefer < kMaxDefer ? L.ndefer : kMaxDefer;
+  for (uint32_t k = t.group(); k < nd; k += t.groups()) {
+    const uint32_t i = L.defer[k];
+    const float4 mn = d.nmin[nb + N - 1 + i], mx = d.nmax[nb + N - 1 + i];
+    const uint32_t cnt = coop_walk<T::kBlock>(d, nb, N, i, ld3(mn), ld3(mx), F, *t.scan_lds, nullptr, 0u);
+    if (threadIdx.x == 0) d.pcount[off + i] = kDeferBit | cnt;
+  }
+  if (nd) t.sync();
   for (uint32_t c0 = 0; c0 < N; c0 += NT) {
     uint32_t i = c0 + tid;
     uint32_t cnt = 0, slots = 0;
     if (i < N) {
-      float4 mn = d.nmin[nb + N - 1 + i], mx = d.nmax[nb + N - 1 + i];
-      uint32_t* st = stage + i * kPairStage;
-      const bool vi = vox && is_vox(i);
-      cnt = traverse(d, nb, N, i, ld3(mn), ld3(mx), [&](uint32_t k, uint32_t j) {
-        if (k < kPairStage) st[k] = j;
-        if (vox) slots += (vi || is_vox(j)) ? 6u : 1u;
-      });
-      if (!vox) slots = cnt;
-      d.pcount[off + i] = cnt;
+      cnt = d.pcount[off + i] & ~kDeferBit;
+      slots = vox ? d.poff[off + i] : cnt;
     }
     uint32_t total;
     uint32_t ex = t.scan(slots, total);
@@ -938,6 +1117,7 @@ BB_DI void s_broad(const Dev& d, uint32_t w, const T& t, BroadShared& L) {
     if (o >= cap) continue;
     const uint32_t room = cap - o;
     uint32_t cnt = d.pcount[off + i];
+    if (cnt & kDeferBit) continue;  // written by its workgroup below
     const uint32_t keep = cnt < room ? cnt : room;  // first `room` in traversal order
     if (cnt <= kPairStage) {
       // canonical order: ascending partner row
@@ -969,16 +1149,26 @@ BB_DI void s_broad(const Dev& d, uint32_t w, const T& t, BroadShared& L) {
       pairs[o + b2] = x;
     }
   }
+  // the rows handed to a workgroup: the first `room` partners, in order
+  for (uint32_t k = t.group(); k < nd; k += t.groups()) {
+    const uint32_t i = L.defer[k];
+    const uint32_t o = d.poff[off + i];
+    if (o >= cap) continue;  // workgroup-uniform
+    const uint32_t cnt = d.pcount[off + i] & ~kDeferBit, room = cap - o;
+    const float4 mn = d.nmin[nb + N - 1 + i], mx = d.nmax[nb + N - 1 + i];
+    (void)coop_walk<T::kBlock>(d, nb, N, i, ld3(mn), ld3(mx), F, *t.scan_lds, pairs + o, cnt < room ? cnt : room);
+  }
 }
 
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_broad(Dev d) {
   __shared__ BroadShared L;
   __shared__ ScanLds<BLOCK> S;
+  __shared__ WalkLds<BLOCK> F;
   const uint32_t w = d.order[blockIdx.x];
   if (d.worlds[w].mode & kModeTeam) return;
   const BlockTeam<BLOCK> t{&S, nullptr};
-  s_broad(d, w, t, L);
+  s_broad(d, w, t, L, F);
 }
 
 // ==================================================== narrow phase (SAT)
@@ -3861,10 +4051,18 @@ union TeamShared {
 };
 static_assert(sizeof(TeamShared) <= kTeamSharedBytes, "team scalars exceed their slot");
 
+// LDS of a team workgroup: the radix sort's in the build stage, the broad
+// phase's shared walks after it
+template <int BLOCK>
+union TeamLds {
+  SortLds<BLOCK> sort;
+  WalkLds<BLOCK> walk;
+};
+
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_team(Dev d) {
   __shared__ ScanLds<BLOCK> S;
-  __shared__ SortLds<BLOCK> Q;
+  __shared__ TeamLds<BLOCK> Q;
   __shared__ uint32_t dead;
   const TeamTable& T = *d.teams;
   const uint32_t nt = T.count < kMaxTeams ? T.count : kMaxTeams;
@@ -3875,7 +4073,7 @@ __global__ __launch_bounds__(BLOCK) void k_team(Dev d) {
   const uint32_t w = T.world[j];
   // stage boundaries: every workgroup has passed the barrier before rank 0
   // stamps the clock
-  const GroupTeam<BLOCK> t{&S, &Q, d.team_ctl + j, d.team_hist + (size_t)b0 * 256, blockIdx.x - b0, nb,
+  const GroupTeam<BLOCK> t{&S, &Q.sort, d.team_ctl + j, d.team_hist + (size_t)b0 * 256, blockIdx.x - b0, nb,
                            &dead, d.stats + (size_t)w * ST_COUNT, d.ovf_any,
                            d.team_spin ? d.team_spin : kTeamSpinLimit,
                            d.team_clock ? d.team_clock + (size_t)j * kTeamMarks : nullptr,
@@ -3886,7 +4084,7 @@ __global__ __launch_bounds__(BLOCK) void k_team(Dev d) {
   s_build(d, w, t, G.build);
   t.sync();
   mark(1);
-  s_broad(d, w, t, G.broad);
+  s_broad(d, w, t, G.broad, Q.walk);
   t.sync();
   mark(2);
   s_narrow<false>(d, w, t);  // voxel worlds never run as teams
@@ -3929,7 +4127,10 @@ struct FusedShared {
       BuildShared b;
       SortLds<BLOCK> q;
     } build;
-    BroadShared broad;
+    struct {
+      BroadShared b;
+      WalkLds<BLOCK> w;
+    } broad;
     struct {
       PostShared p;
       LevelLds lv;
@@ -3948,7 +4149,7 @@ __global__ __launch_bounds__(BLOCK, 4) void k_fused(Dev d) {
   const BlockTeam<BLOCK> t{&S, &G.build.q};
   s_build(d, w, t, G.build.b);
   __syncthreads();
-  s_broad(d, w, t, G.broad);
+  s_broad(d, w, t, G.broad.b, G.broad.w);
   __syncthreads();
   s_narrow<false>(d, w, t);
   if (d.any_voxel) s_narrow<true>(d, w, t);

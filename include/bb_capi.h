@@ -572,11 +572,22 @@ typedef enum bb_param {
                                 4 or 8; 0 = the default 4).  The partial
                                 sums are added in manifold order whatever
                                 the value, so results do not change        */
-  BB_PARAM_PGS_PREFETCH = 11 /* the PGS colour-sweep tile: 1 = each thread
+  BB_PARAM_PGS_PREFETCH = 11, /* the PGS colour-sweep tile: 1 = each thread
                                 fetches its row of the next colour while it
                                 solves this colour's row (the next colour's
                                 rows are written by nobody else meanwhile);
                                 0 (default) = fetch where solved            */
+  BB_PARAM_WALK_BUDGET = 12, /* broad phase: tree nodes one thread walks for
+                                its body before the body is handed to its
+                                whole workgroup (up to 64 bodies per world;
+                                a ground under a pile overlaps thousands);
+                                0 = the default 128, 0xFFFFFFFF = never.
+                                The pair list is the same for every value  */
+  BB_PARAM_TEAM_BLOCKS = 13  /* workgroups of the k_team launch at most
+                                (1..2048); never more than the device holds
+                                resident at once.  0 = half of that (the
+                                default).  More workgroups cost more per
+                                team barrier; results do not change        */
 } bb_param;
 bb_status bb_batch_set_param(bb_batch* b, bb_param p, uint32_t value);
 /* Effective value (BB_PARAM_BLOCK resolves 0 to the automatic choice). */
@@ -589,8 +600,10 @@ bb_status bb_world_set_param(bb_world* w, bb_param p, uint32_t value);
  * best so far, one launch per step or per stage (BB_PARAM_FUSED), the PGS
  * sweep tile (BB_PARAM_PGS_PREFETCH, when a world runs PGS), the AVBD
  * solver tile (BB_PARAM_AVBD_LANES, when a world runs AVBD in the per-world
- * kernels) and the radix sort's sub-team (BB_PARAM_SORT_BLOCKS, when a world
- * runs as a workgroup team).  No candidate changes any result.  The
+ * kernels), the radix sort's sub-team (BB_PARAM_SORT_BLOCKS, when a world
+ * runs as a workgroup team), the broad phase's walk budget
+ * (BB_PARAM_WALK_BUDGET) and the team launch's size (BB_PARAM_TEAM_BLOCKS:
+ * half or all of the resident workgroups).  No candidate changes any result.  The
  * simulation state afterwards is exactly the state before the call.
  * *best_ms (may be NULL) receives the chosen step time. */
 bb_status bb_batch_autotune(bb_batch* b, uint32_t reps, float* best_ms);
@@ -687,6 +700,23 @@ bb_status bb_server_drain(bb_server* s);
  * equals the world stepped alone by bb_world_step, which grows the same way. */
 bb_status bb_server_poll(bb_server* s, uint64_t ticket, uint32_t* state, bb_body_state* out, uint32_t n,
                          bb_step_stats* stats);
+/* Checkpoint of the serving loop (no reference counterpart): waits for
+ * the steps in flight, then writes every resident world (its request and
+ * the state its next step reads: rows, pairs, warm-start manifolds,
+ * counters, steps done), every queued request and every result not yet
+ * polled.  dst = NULL: *size receives the blob size only.  The server
+ * does not step between the two calls, so the size holds.  Returns
+ * BB_ERR_CAPACITY when cap < *size. */
+bb_status bb_server_checkpoint(bb_server* s, void* dst, size_t cap, size_t* size);
+/* A new server (from desc, which may differ from the one that wrote the
+ * blob) holding the blob's worlds: resident worlds are admitted first and
+ * continue from their state, bit for bit as in the server that wrote the
+ * checkpoint; queued requests follow in order; results can be polled; new
+ * tickets continue after the blob's.  The blob is checked on the host
+ * (sizes, descriptors, every index the step uses) before anything reaches
+ * the device: BB_ERR_INVALID if it fails, BB_ERR_BUDGET if a world does not
+ * fit desc's budget. */
+bb_status bb_server_restore(const bb_server_desc* desc, const void* src, size_t size, bb_server** out);
 bb_status bb_server_report_get(bb_server* s, bb_server_report* out);
 void bb_server_destroy(bb_server* s);
 

@@ -82,10 +82,12 @@ int main(int argc, char** argv) {
     if (argc > 7) bb_batch_set_param(b, BB_PARAM_LANE_BODIES, (uint32_t)std::strtoul(argv[7], nullptr, 0));
     if (spin) bb_batch_set_param(b, BB_PARAM_TEAM_SPIN, spin);
     if (argc > 10) bb_batch_set_param(b, BB_PARAM_SORT_BLOCKS, (uint32_t)std::strtoul(argv[10], nullptr, 0));
+    if (argc > 11) bb_batch_set_param(b, BB_PARAM_WALK_BUDGET, (uint32_t)std::strtoul(argv[11], nullptr, 0));
   } else {
     bb_world_set_param(sw, BB_PARAM_TEAM_BODIES, (uint32_t)std::strtoul(argv[3], nullptr, 0));
     if (spin) bb_world_set_param(sw, BB_PARAM_TEAM_SPIN, spin);
     if (argc > 10) bb_world_set_param(sw, BB_PARAM_SORT_BLOCKS, (uint32_t)std::strtoul(argv[10], nullptr, 0));
+    if (argc > 11) bb_world_set_param(sw, BB_PARAM_WALK_BUDGET, (uint32_t)std::strtoul(argv[11], nullptr, 0));
   }
   const int graph = std::atoi(argv[5]);
   for (int s = 0; s < steps; ++s) {

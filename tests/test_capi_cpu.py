@@ -236,3 +236,60 @@ def test_noalloc_limits_checked_before_the_device():
                                                out.ctypes.data_as(C.c_void_p), len(out), None, None) == 2
     if not os.path.exists("/dev/kfd"):
         assert call(big) == 4  # BB_ERR_NO_DEVICE: the maximum itself is accepted
+
+
+def _server_blob(mutate_state=None, **over):
+    """A serving-loop checkpoint in the layout of bb_server_checkpoint: the
+    header, one resident world (its record, body descriptors and the state
+    arrays of _blob), nothing queued, no results."""
+    import struct
+
+    class CkptWorld(C.Structure):
+        _fields_ = [("ticket", C.c_uint64), ("desc", beatbox.WorldDesc)] + [
+            (k, C.c_uint32) for k in ("n", "active", "steps", "done", "pair_scale", "no_team", "reruns",
+                                      "npairs", "resident", "pad")]
+
+    n, p = 3, 1
+    snap = _blob(n, p, mutate_state)
+    desc = beatbox.WorldDesc.make(**scenes.world_desc(iterations=8, morton_bits=63, quirks=0))
+    head_len = 8 + 16 + 8 + len(bytes(desc)) + 8
+    bodies = snap[head_len:head_len + n * scenes.BODY_DESC.itemsize]
+    state = snap[head_len + n * scenes.BODY_DESC.itemsize:]
+    arr = np.frombuffer(bodies, dtype=scenes.BODY_DESC)
+    rec = CkptWorld(ticket=7, desc=desc, n=n, active=int((arr["flags"] & scenes.BODY_DYNAMIC > 0).sum()), steps=10, done=4,
+                    pair_scale=1, no_team=0, reruns=0, npairs=p, resident=1, pad=0)
+    for k, v in over.items():
+        setattr(rec, k, v)
+    head = b"BBSRVCK1" + struct.pack("<4IQ", 1, 1, 0, 0, 8)
+    return head + bytes(rec) + bodies + state
+
+
+def _server_restore_status(blob: bytes) -> int:
+    out = C.c_void_p()
+    d = beatbox.ServerDesc(1 << 30, 4, 0, 0, 0, 0, 0)
+    buf = C.create_string_buffer(blob, len(blob))
+    rc = beatbox.lib().bb_server_restore(C.byref(d), buf, len(blob), C.byref(out))
+    if rc == 0:
+        beatbox.lib().bb_server_destroy(out)
+    return rc
+
+
+def test_server_restore_checks_the_blob_before_the_device():
+    """bb_server_restore reads the whole checkpoint on the host first: a
+    consistent blob gets as far as making the server (BB_ERR_NO_DEVICE
+    here); a wrong magic, a truncated or overlong blob, a world with its
+    steps all done, a wrong active count or a bad index in its state arrays
+    is BB_ERR_INVALID."""
+    good = _server_blob()
+    assert _server_restore_status(good) == 4  # BB_ERR_NO_DEVICE
+    assert _server_restore_status(b"XXSRVCK1" + good[8:]) == 1
+    assert _server_restore_status(good[:-4]) == 1
+    assert _server_restore_status(good + b"\0" * 4) == 1
+    assert _server_restore_status(_server_blob(done=10)) == 1
+    assert _server_restore_status(_server_blob(active=0)) == 1
+    assert _server_restore_status(_server_blob(npairs=100000)) == 1
+
+    def repeat(a):
+        a["id"][1] = a["id"][0]
+
+    assert _server_restore_status(_server_blob(repeat)) == 1

@@ -52,10 +52,10 @@ def _scene(path, worlds, shapes=()):
 
 
 def _run(exe, scene, out, steps, team_bodies, block, graph=1, lane_bodies=0, single=0, team_spin=0,
-         max_blocks=0, sort_blocks=0):
+         max_blocks=0, sort_blocks=0, walk_budget=0):
     env = dict(ENV, EMU_MAX_BLOCKS=str(max_blocks)) if max_blocks else ENV
     r = subprocess.run([exe, scene, str(steps), str(team_bodies), str(block), str(graph), out, str(lane_bodies),
-                        str(single), str(team_spin), str(sort_blocks)],
+                        str(single), str(team_spin), str(sort_blocks), str(walk_budget)],
                        capture_output=True, text=True, env=env, timeout=600)
     assert r.returncode == 0, r.stderr[-3000:]
     assert "AddressSanitizer" not in r.stderr, r.stderr[-3000:]
@@ -318,3 +318,24 @@ def test_single_world_team_failure_replays_by_one_workgroup(emu_main, tmp_path):
     got = np.frombuffer(raw, dtype=beatbox.BODY_STATE)
     for f in ("position", "rotation", "velocity", "omega", "flags", "id"):
         assert got[f].tobytes() == ref[f].tobytes(), f
+
+
+def test_broad_phase_rows_handed_to_the_workgroup_equal_one_thread_walks(emu_main, tmp_path):
+    """Walk budgets of 1 and 4 nodes hand most rows (64 per world at most)
+    to their workgroup's shared walk (frontier widened in leaf order, one
+    walk per frontier node, scanned into place); budget 0xFFFFFFFF hands
+    none over.  Per-world kernels and teams, 256- and 1,024-thread
+    workgroups: the same pair lists, so the same trajectories, and every
+    access in bounds."""
+    avbd = beatbox.WorldDesc.make(**scenes.world_desc(solver=scenes.SOLVER_AVBD, iterations=10,
+                                                      morton_bits=63, quirks=0))
+    pgs = beatbox.WorldDesc.make(**scenes.world_desc(iterations=8, morton_bits=63, quirks=0))
+    parity = beatbox.WorldDesc.make(**scenes.world_desc(iterations=8))
+    scene = str(tmp_path / "scene.bin")
+    _scene(scene, [(avbd, scenes.serving_pile(22, 700)), (pgs, scenes.serving_pile(26, 300)),
+                   (parity, scenes.random_boxes(0xBB14, 64, extent=3.0, y0=0.55, y1=5.0)),
+                   (pgs, scenes.dense_spheres(0xBB72, side=4))])
+    ref_log, ref = _run(emu_main, scene, str(tmp_path / "r.bin"), 4, 0, 256, walk_budget=0xFFFFFFFF)
+    for team, block, budget in ((0, 256, 1), (0, 1024, 4), (600, 256, 1), (600, 0, 4)):
+        log, x = _run(emu_main, scene, str(tmp_path / "x.bin"), 4, team, block, walk_budget=budget)
+        assert log == ref_log and x == ref, (team, block, budget)

@@ -265,15 +265,22 @@ def test_team_path_equals_workgroup_path():
               (pgs63, scenes.batch_world(0xBB04, 2, 64)),
               (avbd, scenes.serving_pile(22, 700))]
     runs = []
-    # 0: every world by one workgroup (the two largest in the 512-thread lane);
-    # 600: four worlds as teams, the radix sort on 1 or (default) 32 of their
-    # workgroups; 2000: two teams running beside both lanes
-    for team, sort_blocks in ((0, 0), (600, 1), (600, 0), (2000, 2)):
+    # 0: every world by one workgroup (the two largest in the 512-thread lane),
+    # no broad-phase row handed to a workgroup; 600: four worlds as teams, the
+    # radix sort on 1 or (default) 32 of their workgroups, walk budgets of 1
+    # node (64 rows per world walked by a workgroup) or the default, the team
+    # launch at half or all of the resident workgroups; 2000: two teams
+    # running beside both lanes
+    for team, sort_blocks, budget, team_blocks in ((0, 0, 0xFFFFFFFF, 0), (600, 1, 1, 0), (600, 0, 0, 2048),
+                                                   (2000, 2, 4, 0)):
+        print(f"team {team} sort {sort_blocks} walk budget {budget} team blocks {team_blocks}", flush=True)
         b = beatbox.Batch()
         for d, bodies in worlds:
             b.add_world(d, bodies)
         b.set_param(beatbox.PARAM_TEAM_BODIES, team)
         b.set_param(beatbox.PARAM_SORT_BLOCKS, sort_blocks)
+        b.set_param(beatbox.PARAM_WALK_BUDGET, budget)
+        b.set_param(beatbox.PARAM_TEAM_BLOCKS, team_blocks)
         b.set_param(beatbox.PARAM_PGS_PREFETCH, 1 if sort_blocks == 1 else 0)
         b.step(8, use_graph=True)
         b.sync()

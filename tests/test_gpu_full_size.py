@@ -85,10 +85,14 @@ def test_large_pile_full_size_team_equals_workgroup():
     desc = beatbox.WorldDesc.make(**scenes.world_desc(solver=scenes.SOLVER_AVBD, iterations=10,
                                                       morton_bits=63, quirks=0))
     runs = []
-    for team in (8192, 0):
+    # the team at all of the resident workgroups, rows handed to workgroups
+    # past the default walk budget; the one workgroup walking every row alone
+    for team, budget, team_blocks in ((8192, 0, 2048), (0, 0xFFFFFFFF, 0)):
         b = beatbox.Batch()
         b.add_world(desc, bodies)
         b.set_param(beatbox.PARAM_TEAM_BODIES, team)
+        b.set_param(beatbox.PARAM_WALK_BUDGET, budget)
+        b.set_param(beatbox.PARAM_TEAM_BLOCKS, team_blocks)
         b.step(2)
         b.sync()
         runs.append((b.read_world(0, len(bodies)), b.stats(0)))

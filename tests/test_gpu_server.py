@@ -130,3 +130,45 @@ def test_server_reruns_a_world_that_overflowed():
         assert stats["overflow"] == 0
         assert stats["broad_phase_collision_count"] == ref_stats["broad_phase_collision_count"]
     srv.destroy()
+
+
+def test_server_checkpoint_restores_bit_exact():
+    """A checkpoint taken mid-run (worlds resident in every pool, a team
+    world among them, requests still queued, results not yet polled) and
+    restored into a new server, with the same pools or with one pool: every
+    request's result equals the original server's and the world stepped
+    alone, bit for bit; new tickets continue after the checkpoint's."""
+    avbd = beatbox.WorldDesc.make(**scenes.world_desc(solver=scenes.SOLVER_AVBD, iterations=10,
+                                                      morton_bits=63, quirks=0))
+    reqs = _requests() + [(avbd, scenes.serving_pile(15, 1300), 9), (avbd, scenes.serving_pile(16, 600), 8)]
+    kw = dict(budget_bytes=1 << 31, max_worlds=3, team_bodies=1000, split_bodies=256)
+    a = beatbox.Server(**kw)
+    tickets = [a.submit(d, b, s) for d, b, s in reqs]
+    a.step(4)
+    blob = a.checkpoint()
+    rep = a.report()
+    print(f"checkpoint: {len(blob)} bytes, resident {rep['resident_worlds']}, queued {rep['queued']}, "
+          f"retired {rep['retired']}", flush=True)
+    assert rep["resident_worlds"] >= 3 and rep["queued"] >= 1 and rep["retired"] >= 1
+    restored = [beatbox.Server.restore(blob, **kw),
+                beatbox.Server.restore(blob, budget_bytes=1 << 31, max_worlds=16, team_bodies=1000)]
+    a.drain()
+    for srv in restored:
+        srv.drain()
+    for t, (d, bodies, steps) in zip(tickets, reqs):
+        state, want, want_stats = a.poll(t, len(bodies))
+        assert state == beatbox.REQ_DONE, t
+        ref, _ = _alone(d, bodies, steps)
+        for f in FIELDS:
+            assert want[f].tobytes() == ref[f].tobytes(), (t, f)
+        for k, srv in enumerate(restored):
+            state, got, stats = srv.poll(t, len(bodies))
+            assert state == beatbox.REQ_DONE, (k, t)
+            for f in FIELDS:
+                assert got[f].tobytes() == want[f].tobytes(), (k, t, f)
+            for key in ("collision_count", "contact_count", "broad_phase_collision_count", "overflow"):
+                assert stats[key] == want_stats[key], (k, t, key)
+    for srv in restored:
+        assert srv.submit(reqs[0][0], reqs[0][1], 1) > max(tickets)
+        srv.destroy()
+    a.destroy()
