@@ -185,6 +185,7 @@ PARAM_AVBD_LANES = 10
 PARAM_PGS_PREFETCH = 11
 PARAM_WALK_BUDGET = 12
 PARAM_TEAM_BLOCKS = 13
+PARAM_PGS_BLOCK = 14
 
 STATUS = {0: "BB_OK", 1: "BB_ERR_INVALID", 2: "BB_ERR_CAPACITY", 3: "BB_ERR_DEVICE",
           4: "BB_ERR_NO_DEVICE", 5: "BB_ERR_OVERFLOW", 6: "BB_ERR_STATE", 7: "BB_ERR_BUDGET"}

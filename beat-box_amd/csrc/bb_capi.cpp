@@ -638,7 +638,7 @@ int lanes_of(const bb_batch* b, const Dev& d, Lane* out) {
   }
   Dev rest = d;
   rest.order = d.order + nw;
-  out[nl++] = Lane{rest, pick_block(b), d.W - nw, b->fused != 0};
+  out[nl++] = Lane{rest, pick_block(b), d.W - nw, b->fused != 0, b->pgs_block};
   return nl;
 }
 
@@ -1203,6 +1203,10 @@ bb_status set_param(bb_batch* b, bb_param p, uint32_t value) {
   } else if (p == BB_PARAM_TEAM_BLOCKS) {
     if (value > kMaxTeamBlocks) return BB_ERR_INVALID;
     b->team_blocks = value;
+  } else if (p == BB_PARAM_PGS_BLOCK) {
+    if (value != 0 && value != 64 && value != 128 && value != 256 && value != 512 && value != 1024)
+      return BB_ERR_INVALID;
+    b->pgs_block = value;
   } else if (p == BB_PARAM_PAIR_SCALE) {
     if (value < 1 || value > 16) return BB_ERR_INVALID;
     if (value != b->pair_scale) {
@@ -1280,7 +1284,7 @@ bb_status autotune_batch(bb_batch* b, uint32_t reps, float* best_ms) {
   uint32_t best_block = saved_block, best_lds = saved_lds;
   const uint32_t saved_fused = b->fused, saved_sort = b->sort_blocks, saved_lanes = b->avbd_lanes,
                  saved_prefetch = b->pgs_prefetch, saved_walk = b->walk_budget,
-                 saved_team_blocks = b->team_blocks;
+                 saved_team_blocks = b->team_blocks, saved_pgs_block = b->pgs_block;
   bb_status rc = BB_OK;
   // The wide lane is not tuned here: one replayed step early in a run does
   // not show how much the large worlds come to dominate later (over the
@@ -1332,6 +1336,32 @@ bb_status autotune_batch(bb_batch* b, uint32_t reps, float* best_ms) {
     field = keep;
     if (mirror) *mirror = keep;
   };
+  // the colour sweeps of the small worlds in narrower workgroups, with their
+  // own LDS rows (a 64-thread workgroup keeping 512 rows would hold a CU's
+  // LDS for four waves)
+  if (rc == BB_OK) {
+    const uint32_t keep_pb = b->pgs_block, keep_lds = b->pgs_lds;
+    uint32_t best_pb = keep_pb, best_pl = keep_lds;
+    for (uint32_t pb : {64u, 128u}) {
+      for (uint32_t lds : {128u, 256u, 512u}) {
+        if (lds > kMaxPgsLdsRows) continue;
+        b->pgs_block = pb;
+        b->pgs_lds = lds;
+        b->base.pgs_lcap = pgs_lds_rows(b);
+        float ms = 0.0f;
+        if ((rc = measure(ms)) != BB_OK) break;
+        if (ms < best) {
+          best = ms;
+          best_pb = pb;
+          best_pl = lds;
+        }
+      }
+      if (rc != BB_OK) break;
+    }
+    b->pgs_block = best_pb;
+    b->pgs_lds = best_pl;
+    b->base.pgs_lcap = pgs_lds_rows(b);
+  }
   tune(b->fused, nullptr, {0u, 1u});
   bool any_pgs = false;
   for (const WorldInfo& wi : b->infos)
@@ -1360,6 +1390,7 @@ bb_status autotune_batch(bb_batch* b, uint32_t reps, float* best_ms) {
     }
   }
   if (rc != BB_OK) {
+    b->pgs_block = saved_pgs_block;
     b->walk_budget = b->base.walk_budget = saved_walk;
     b->team_blocks = saved_team_blocks;
     b->fused = saved_fused;
@@ -2033,6 +2064,7 @@ uint32_t bb_batch_get_param(const bb_batch* b, bb_param p) {
   if (p == BB_PARAM_SORT_BLOCKS) return b->sort_blocks ? b->sort_blocks : kTeamSortBlocks;
   if (p == BB_PARAM_WALK_BUDGET) return b->walk_budget ? b->walk_budget : kWalkBudgetDefault;
   if (p == BB_PARAM_TEAM_BLOCKS) return has_device() ? team_grid_cap(b) : b->team_blocks;
+  if (p == BB_PARAM_PGS_BLOCK) return b->pgs_block;
   return 0;
 }
 

@@ -247,6 +247,10 @@ struct Lane {
   // the whole step of each world in one launch (k_fused) instead of one
   // launch per stage
   bool fused = false;
+  // threads per workgroup of this lane's colour sweeps (k_pgs); 0: block.
+  // Small worlds have a few dozen rows per colour: one wave per world turns
+  // each colour's barrier into a wave's and lets more worlds share a CU.
+  uint32_t pgs_block = 0;
 };
 
 }  // namespace bb

@@ -163,6 +163,7 @@ struct bb_batch {
   uint32_t pgs_prefetch = 0; // BB_PARAM_PGS_PREFETCH
   uint32_t walk_budget = 0;  // BB_PARAM_WALK_BUDGET (0: kWalkBudgetDefault)
   uint32_t team_blocks = 0;  // BB_PARAM_TEAM_BLOCKS (0: half the resident workgroups)
+  uint32_t pgs_block = 0;    // BB_PARAM_PGS_BLOCK (0: the lane's workgroup size)
   uint32_t n_wide = 0;  // worlds in that lane: the first n_wide of the order table
   hipStream_t stream2 = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;

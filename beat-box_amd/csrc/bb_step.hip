@@ -4337,6 +4337,8 @@ __global__ __launch_bounds__(256) void k_copy_calibrate(const float4* __restrict
 // ============================================================ host launch
 namespace bb {
 template __global__ void k_team<kTeamBlock>(Dev);
+template __global__ void k_pgs<64>(Dev);
+template __global__ void k_pgs<128>(Dev);
 #define BB_INSTANTIATE(B)                      \
   template __global__ void k_build<B>(Dev);    \
   template __global__ void k_broad<B>(Dev);    \
@@ -4366,9 +4368,8 @@ static void launch_stage(int stage, const Dev& d, hipStream_t st, uint32_t grid)
       hipLaunchKernelGGL(k_contacts<B>, g, b, 0, st, d);
       break;
     case 3: hipLaunchKernelGGL(k_post<B>, g, b, 0, st, d); break;
-    case 4:
+    case 4:  // the colour sweeps (k_pgs) are launched by the lane
       if (d.any_island) hipLaunchKernelGGL(k_levels<B>, g, b, 0, st, d);
-      hipLaunchKernelGGL(k_pgs<B>, g, b, (size_t)d.pgs_lcap * 16 * kBodyLds, st, d);
       if (d.any_island) hipLaunchKernelGGL(k_pgs_island<B>, g, b, (size_t)d.pgs_lcap * 16 * kBodyLds, st, d);
       break;
     case 5: hipLaunchKernelGGL(k_avbd<B>, g, b, 0, st, d); break;
@@ -4377,12 +4378,26 @@ static void launch_stage(int stage, const Dev& d, hipStream_t st, uint32_t grid)
 }
 static constexpr int kStages = 7;
 
+template <int B>
+static void launch_pgs(const Dev& d, hipStream_t st, uint32_t grid) {
+  hipLaunchKernelGGL(k_pgs<B>, dim3(grid), dim3(B), (size_t)d.pgs_lcap * 16 * kBodyLds, st, d);
+}
+
 static void launch_stage(int stage, const Lane& l, hipStream_t st) {
   if (l.grid == 0) return;
   switch (l.block) {
     case 512: launch_stage<512>(stage, l.d, st, l.grid); break;
     case 1024: launch_stage<1024>(stage, l.d, st, l.grid); break;
     default: launch_stage<256>(stage, l.d, st, l.grid); break;
+  }
+  if (stage == 4) {
+    switch (l.pgs_block ? l.pgs_block : l.block) {
+      case 64: launch_pgs<64>(l.d, st, l.grid); break;
+      case 128: launch_pgs<128>(l.d, st, l.grid); break;
+      case 512: launch_pgs<512>(l.d, st, l.grid); break;
+      case 1024: launch_pgs<1024>(l.d, st, l.grid); break;
+      default: launch_pgs<256>(l.d, st, l.grid); break;
+    }
   }
 }
 

@@ -153,6 +153,17 @@ def make_batch(worlds: int, seed: int, lo: int, hi: int):
     return b, total
 
 
+# Launch parameters the autotuner chooses; re-made batches (per-stage profile,
+# configuration window) are given the same values.
+TUNED = ("BLOCK", "PGS_LDS_ROWS", "PGS_BLOCK", "FUSED", "PGS_PREFETCH", "WALK_BUDGET", "LANE_BODIES",
+         "LANE_BLOCK")
+
+
+def apply_params(b, params: dict) -> None:
+    for name in TUNED:
+        b.set_param(getattr(beatbox, "PARAM_" + name), params[name.lower()])
+
+
 def time_batch(b, steps: int, warmup: int, tune: bool):
     """Graph replay of `steps` steps after `warmup` untimed ones (and the
     autotuner, which replays the next step per candidate and leaves the state
@@ -163,8 +174,7 @@ def time_batch(b, steps: int, warmup: int, tune: bool):
     b.sync(allow_overflow=True)
     if tune:
         b.autotune(2)
-    params = (b.get_param(beatbox.PARAM_BLOCK), b.get_param(beatbox.PARAM_PGS_LDS_ROWS),
-              b.get_param(beatbox.PARAM_LANE_BODIES), b.get_param(beatbox.PARAM_LANE_BLOCK))
+    params = {name.lower(): b.get_param(getattr(beatbox, "PARAM_" + name)) for name in TUNED}
     t0 = time.perf_counter()
     b.step(steps, use_graph=True)
     b.sync(allow_overflow=True)
@@ -430,7 +440,7 @@ def main() -> int:
     t, e0, e1, params, overflowed = time_batch(batch, args.steps, args.warmup, not args.no_tune)
     if overflowed:
         log(f"batch: {len(overflowed)} worlds flagged an overflow: {overflowed[:16]}")
-    log(f"launch parameters: block {params[0]}, PGS LDS rows {params[1]}")
+    log(f"launch parameters: {params}")
     dev_gib = batch.device_bytes / 2**30
     batch.destroy()
     value = total * args.steps / t
@@ -443,8 +453,7 @@ def main() -> int:
     # same seed reaches the same states (the step is deterministic), then
     # eager steps are timed with hipEvents on the library stream
     batch, _ = make_batch(args.worlds, args.seed, 16, 2048)
-    batch.set_param(beatbox.PARAM_BLOCK, params[0])
-    batch.set_param(beatbox.PARAM_PGS_LDS_ROWS, params[1])
+    apply_params(batch, params)
     batch.step(args.warmup, use_graph=True)
     batch.sync(allow_overflow=True)
     psteps = args.profile_steps or args.steps
@@ -473,8 +482,7 @@ def main() -> int:
     config_window = None
     if (args.steps, args.warmup) != (CONFIG_STEPS, CONFIG_WARMUP) and not args.no_config_window:
         cb, _ = make_batch(args.worlds, args.seed, 16, 2048)
-        cb.set_param(beatbox.PARAM_BLOCK, params[0])
-        cb.set_param(beatbox.PARAM_PGS_LDS_ROWS, params[1])
+        apply_params(cb, params)
         ct, _, _, _, c_ovf = time_batch(cb, CONFIG_STEPS, CONFIG_WARMUP, False)
         cb.destroy()
         c_plain_bs, c_plain_ms = plain_world(CONFIG_STEPS, CONFIG_WARMUP)
@@ -543,10 +551,7 @@ def main() -> int:
             "eager_step_ms_p50": p50,
             "eager_step_ms_p99": p99,
             "copy_kernel_gbs": copy_gbs,
-            "block": params[0],
-            "pgs_lds_rows": params[1],
-            "lane_bodies": params[2],
-            "lane_block": params[3],
+            "launch_params": params,
             **extra,
             "stages": stages,
         },

@@ -583,11 +583,19 @@ typedef enum bb_param {
                                 a ground under a pile overlaps thousands);
                                 0 = the default 128, 0xFFFFFFFF = never.
                                 The pair list is the same for every value  */
-  BB_PARAM_TEAM_BLOCKS = 13  /* workgroups of the k_team launch at most
+  BB_PARAM_TEAM_BLOCKS = 13, /* workgroups of the k_team launch at most
                                 (1..2048); never more than the device holds
                                 resident at once.  0 = half of that (the
                                 default).  More workgroups cost more per
                                 team barrier; results do not change        */
+  BB_PARAM_PGS_BLOCK = 14    /* threads per workgroup of the colour sweeps
+                                (k_pgs) of the worlds below the second
+                                lane's threshold: 64, 128, 256, 512 or
+                                1024; 0 (default) = BB_PARAM_BLOCK.  A small
+                                world has a few dozen rows per colour: one
+                                wave per world makes each colour's barrier
+                                a wave's and lets more worlds share a CU.
+                                Its LDS rows are BB_PARAM_PGS_LDS_ROWS     */
 } bb_param;
 bb_status bb_batch_set_param(bb_batch* b, bb_param p, uint32_t value);
 /* Effective value (BB_PARAM_BLOCK resolves 0 to the automatic choice). */

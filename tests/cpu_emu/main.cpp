@@ -83,6 +83,8 @@ int main(int argc, char** argv) {
     if (spin) bb_batch_set_param(b, BB_PARAM_TEAM_SPIN, spin);
     if (argc > 10) bb_batch_set_param(b, BB_PARAM_SORT_BLOCKS, (uint32_t)std::strtoul(argv[10], nullptr, 0));
     if (argc > 11) bb_batch_set_param(b, BB_PARAM_WALK_BUDGET, (uint32_t)std::strtoul(argv[11], nullptr, 0));
+    if (argc > 12) bb_batch_set_param(b, BB_PARAM_PGS_BLOCK, (uint32_t)std::strtoul(argv[12], nullptr, 0));
+    if (argc > 13) bb_batch_set_param(b, BB_PARAM_PGS_LDS_ROWS, (uint32_t)std::strtoul(argv[13], nullptr, 0));
   } else {
     bb_world_set_param(sw, BB_PARAM_TEAM_BODIES, (uint32_t)std::strtoul(argv[3], nullptr, 0));
     if (spin) bb_world_set_param(sw, BB_PARAM_TEAM_SPIN, spin);

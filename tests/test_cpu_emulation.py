@@ -52,10 +52,11 @@ def _scene(path, worlds, shapes=()):
 
 
 def _run(exe, scene, out, steps, team_bodies, block, graph=1, lane_bodies=0, single=0, team_spin=0,
-         max_blocks=0, sort_blocks=0, walk_budget=0):
+         max_blocks=0, sort_blocks=0, walk_budget=0, pgs_block=0, pgs_lds=512):
     env = dict(ENV, EMU_MAX_BLOCKS=str(max_blocks)) if max_blocks else ENV
     r = subprocess.run([exe, scene, str(steps), str(team_bodies), str(block), str(graph), out, str(lane_bodies),
-                        str(single), str(team_spin), str(sort_blocks), str(walk_budget)],
+                        str(single), str(team_spin), str(sort_blocks), str(walk_budget), str(pgs_block),
+                        str(pgs_lds)],
                        capture_output=True, text=True, env=env, timeout=600)
     assert r.returncode == 0, r.stderr[-3000:]
     assert "AddressSanitizer" not in r.stderr, r.stderr[-3000:]
@@ -148,6 +149,12 @@ def test_two_lanes_equal_one_lane(emu_main, tmp_path):
         a, x = _run(emu_main, scene, str(tmp_path / "a.bin"), 5, 0, 0, graph=graph, lane_bodies=0)
         b, y = _run(emu_main, scene, str(tmp_path / "b.bin"), 5, 0, 0, graph=graph, lane_bodies=300)
         assert a == b and x == y, graph
+    # the first lane's colour sweeps in one-wave and two-wave workgroups,
+    # with fewer LDS rows than the worlds have bodies
+    for pgs_block, lds in ((64, 128), (128, 64)):
+        c, z = _run(emu_main, scene, str(tmp_path / "c.bin"), 5, 0, 0, lane_bodies=300, pgs_block=pgs_block,
+                    pgs_lds=lds)
+        assert c == a and z == x, pgs_block
 
 
 def test_emulated_island_path_equals_oracle(emu_main, tmp_path):

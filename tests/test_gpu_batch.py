@@ -192,12 +192,16 @@ def test_launch_params_do_not_change_results():
         solver=scenes.SOLVER_AVBD if w == 1 else scenes.SOLVER_PGS_SOFT,
         iterations=8, morton_bits=63, quirks=0)) for w in range(len(sizes))]
     runs = []
-    # (block, LDS rows, one launch per step, AVBD lanes per body, PGS prefetch)
-    for blk, lds, fused, lanes, pre in ((256, 0, 0, 4, 0), (512, 256, 0, 4, 0), (1024, 768, 0, 4, 1),
-                                        (256, 512, 0, 4, 1), (512, 2048, 0, 4, 0), (256, 512, 1, 4, 1),
-                                        (512, 256, 1, 2, 0), (256, 0, 0, 1, 1), (512, 512, 0, 8, 1),
-                                        (1024, 256, 1, 8, 0)):
-        print(f"launch params: block {blk} lds {lds} fused {fused} lanes {lanes} prefetch {pre}", flush=True)
+    # (block, LDS rows, one launch per step, AVBD lanes per body, PGS prefetch,
+    # colour-sweep workgroup of the first lane)
+    for blk, lds, fused, lanes, pre, pb in ((256, 0, 0, 4, 0, 0), (512, 256, 0, 4, 0, 0), (1024, 768, 0, 4, 1, 0),
+                                            (256, 512, 0, 4, 1, 0), (512, 2048, 0, 4, 0, 0),
+                                            (256, 512, 1, 4, 1, 0), (512, 256, 1, 2, 0, 0), (256, 0, 0, 1, 1, 0),
+                                            (512, 512, 0, 8, 1, 0), (1024, 256, 1, 8, 0, 0),
+                                            (256, 128, 0, 4, 0, 64), (256, 256, 0, 4, 1, 128),
+                                            (512, 64, 0, 4, 0, 64)):
+        print(f"launch params: block {blk} lds {lds} fused {fused} lanes {lanes} prefetch {pre} pgs block {pb}",
+              flush=True)
         b = beatbox.Batch()
         for d, bodies in zip(descs, worlds):
             b.add_world(d, bodies)
@@ -206,6 +210,7 @@ def test_launch_params_do_not_change_results():
         b.set_param(beatbox.PARAM_FUSED, fused)
         b.set_param(beatbox.PARAM_AVBD_LANES, lanes)
         b.set_param(beatbox.PARAM_PGS_PREFETCH, pre)
+        b.set_param(beatbox.PARAM_PGS_BLOCK, pb)
         b.step(15, use_graph=True)
         b.sync()
         assert b.get_param(beatbox.PARAM_BLOCK) == blk
