@@ -1167,7 +1167,7 @@ bb_status restore_world(const void* src, size_t size, int warm_start, bb_world**
 
 bb_status set_param(bb_batch* b, bb_param p, uint32_t value) {
   if (p == BB_PARAM_BLOCK) {
-    if (value != 0 && value != 256 && value != 512 && value != 1024) return BB_ERR_INVALID;
+    if (value != 0 && value != 128 && value != 256 && value != 512 && value != 1024) return BB_ERR_INVALID;
     b->block = value;
   } else if (p == BB_PARAM_PGS_LDS_ROWS) {
     if (value > kMaxPgsLdsRows) return BB_ERR_INVALID;
@@ -1289,7 +1289,7 @@ bb_status autotune_batch(bb_batch* b, uint32_t reps, float* best_ms) {
   // The wide lane is not tuned here: one replayed step early in a run does
   // not show how much the large worlds come to dominate later (over the
   // 240-step batch the lane gains 11% while the first steps favour one lane).
-  for (uint32_t blk : {256u, 512u, 1024u}) {
+  for (uint32_t blk : {128u, 256u, 512u, 1024u}) {
     for (uint32_t lds : lds_opts) {
       b->block = blk;
       b->pgs_lds = lds;
@@ -1375,8 +1375,11 @@ bb_status autotune_batch(bb_batch* b, uint32_t reps, float* best_ms) {
     if (b->sort_blocks == 0) b->sort_blocks = b->base.sort_blocks = kTeamSortBlocks;
     tune(b->sort_blocks, &b->base.sort_blocks, {8u, 16u, 32u, 64u, 128u});
   }
+  // never below the default: the tuner replays an early step, before the
+  // piles settle and a ground's walk grows long, and a low budget then hands
+  // many rows to slow shared walks
   if (b->walk_budget == 0) b->walk_budget = b->base.walk_budget = kWalkBudgetDefault;
-  tune(b->walk_budget, &b->base.walk_budget, {32u, 64u, 128u, 512u, kMAX});
+  tune(b->walk_budget, &b->base.walk_budget, {kWalkBudgetDefault, 512u, kMAX});
   // the team launch: half the resident workgroups (the default) or all of
   // them (the teams are laid out anew for each)
   if (b->base.team_grid && rc == BB_OK) {

@@ -662,7 +662,8 @@ extern "C" {
 
 bb_status bb_server_create(const bb_server_desc* desc, bb_server** out) {
   if (!desc || !out || desc->max_worlds == 0 || desc->hbm_budget_bytes == 0) return BB_ERR_INVALID;
-  if (desc->block != 0 && desc->block != 256 && desc->block != 512 && desc->block != 1024) return BB_ERR_INVALID;
+  if (desc->block != 0 && desc->block != 128 && desc->block != 256 && desc->block != 512 && desc->block != 1024)
+    return BB_ERR_INVALID;
   if (!has_device()) return BB_ERR_NO_DEVICE;
   bb_server* s = new (std::nothrow) bb_server();
   if (!s) return BB_ERR_CAPACITY;

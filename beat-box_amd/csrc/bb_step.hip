@@ -187,10 +187,23 @@ BB_DI bool block_radix_sort(uint64_t* ka, uint32_t* ia, uint64_t* kb, uint32_t* 
     __syncthreads();
     if (skip) continue;
     {
-      uint32_t total;
-      uint32_t v = tid < 256 ? L.base[tid] : 0u;
-      uint32_t ex = block_scan<BLOCK>(v, L.scan, total);
-      if (tid < 256) L.base[tid] = ex;
+      // exclusive scan of the 256 digit counts (256 / BLOCK consecutive
+      // digits per thread in a smaller block)
+      constexpr uint32_t per = BLOCK >= 256 ? 1u : 256u / BLOCK;
+      uint32_t v[per], sum = 0, total;
+#pragma unroll
+      for (uint32_t q = 0; q < per; ++q) {
+        const uint32_t dg = tid * per + q;
+        v[q] = dg < 256 ? L.base[dg] : 0u;
+        sum += v[q];
+      }
+      uint32_t ex = block_scan<BLOCK>(sum, L.scan, total);
+#pragma unroll
+      for (uint32_t q = 0; q < per; ++q) {
+        const uint32_t dg = tid * per + q;
+        if (dg < 256) L.base[dg] = ex;
+        ex += v[q];
+      }
       __syncthreads();
     }
     for (uint32_t c0 = 0; c0 < n; c0 += BLOCK) {
@@ -4179,6 +4192,7 @@ __global__ __launch_bounds__(BLOCK, 4) void k_fused(Dev d) {
   s_finish(d, w, t);
 }
 
+#ifndef BB_BLOCK_TU
 // Render hand-off (entry_update_acceleration_structures, src/shaders/
 // AS_mngr.slang:5-36, with RigidBody::get_instance_transform,
 // include/shared.inl:377-419): for every body of world w, at its id, the
@@ -4267,6 +4281,8 @@ __global__ __launch_bounds__(1024) void k_sum_counts(const uint32_t* __restrict_
 // physical inertia of the shape (box: m/12 diag(ey^2+ez^2, ...) over full
 // extents; sphere: 2/5 m r^2; voxel shape: the inverse of the stored inverse
 // inertia) and w_b the angular velocity in the body frame.
+#endif  // BB_BLOCK_TU
+
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_energy(Dev d, double* __restrict__ out) {
   __shared__ double red[BLOCK / 64];
@@ -4326,38 +4342,39 @@ __global__ __launch_bounds__(BLOCK) void k_energy(Dev d, double* __restrict__ ou
   }
 }
 
+#ifndef BB_BLOCK_TU
 // Grid-stride float4 copy of known size (byte-counter calibration).
 __global__ __launch_bounds__(256) void k_copy_calibrate(const float4* __restrict__ src, float4* __restrict__ dst,
                                                         size_t n) {
   for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) dst[i] = src[i];
 }
 
+#endif  // BB_BLOCK_TU
+
 }  // namespace bb
 
 // ============================================================ host launch
+// The per-world kernels of each workgroup size and their launchers are
+// compiled in a unit of their own (bb_step_block.hip includes this file with
+// BB_BLOCK_TU and BB_BLOCK set), so the sizes build in parallel; this unit
+// holds the team kernel, the launch sequence and the small utility kernels.
 namespace bb {
-template __global__ void k_team<kTeamBlock>(Dev);
-template __global__ void k_pgs<64>(Dev);
-template __global__ void k_pgs<128>(Dev);
-#define BB_INSTANTIATE(B)                      \
-  template __global__ void k_build<B>(Dev);    \
-  template __global__ void k_broad<B>(Dev);    \
-  template __global__ void k_narrow<B>(Dev);   \
-  template __global__ void k_narrow_vox<B>(Dev); \
-  template __global__ void k_contacts<B>(Dev); \
-  template __global__ void k_post<B>(Dev);     \
-  template __global__ void k_pgs<B>(Dev);      \
-  template __global__ void k_pgs_island<B>(Dev); \
-  template __global__ void k_levels<B>(Dev);    \
-  template __global__ void k_avbd<B>(Dev);     \
-  template __global__ void k_finish<B>(Dev);   \
-  template __global__ void k_fused<B>(Dev);
+static constexpr int kStages = 7;
+template <int B>
+void launch_stage_b(int stage, const Dev& d, hipStream_t st, uint32_t grid);
+template <int B>
+void launch_pgs_b(const Dev& d, hipStream_t st, uint32_t grid);
+template <int B>
+void launch_fused_b(const Lane& l, hipStream_t st);
 
+#ifdef BB_BLOCK_TU
 // Stages of the per-world path, in launch order (k_team, when there is one,
 // goes before the first).  Stage k of the timed launch is bracketed by events
-// ev[k] and ev[k + 1]; the narrow phase's two kernels share stage 2.
+// ev[k] and ev[k + 1]; the narrow phase's two kernels share stage 2; the
+// colour sweeps (k_pgs) of stage 4 are launched by the lane, with its own
+// workgroup size.
 template <int B>
-static void launch_stage(int stage, const Dev& d, hipStream_t st, uint32_t grid) {
+void launch_stage_b(int stage, const Dev& d, hipStream_t st, uint32_t grid) {
   const dim3 g(grid), b(B);
   switch (stage) {
     case 0: hipLaunchKernelGGL(k_build<B>, g, b, 0, st, d); break;
@@ -4368,7 +4385,7 @@ static void launch_stage(int stage, const Dev& d, hipStream_t st, uint32_t grid)
       hipLaunchKernelGGL(k_contacts<B>, g, b, 0, st, d);
       break;
     case 3: hipLaunchKernelGGL(k_post<B>, g, b, 0, st, d); break;
-    case 4:  // the colour sweeps (k_pgs) are launched by the lane
+    case 4:
       if (d.any_island) hipLaunchKernelGGL(k_levels<B>, g, b, 0, st, d);
       if (d.any_island) hipLaunchKernelGGL(k_pgs_island<B>, g, b, (size_t)d.pgs_lcap * 16 * kBodyLds, st, d);
       break;
@@ -4376,27 +4393,39 @@ static void launch_stage(int stage, const Dev& d, hipStream_t st, uint32_t grid)
     default: hipLaunchKernelGGL(k_finish<B>, g, b, 0, st, d); break;
   }
 }
-static constexpr int kStages = 7;
-
 template <int B>
-static void launch_pgs(const Dev& d, hipStream_t st, uint32_t grid) {
+void launch_pgs_b(const Dev& d, hipStream_t st, uint32_t grid) {
   hipLaunchKernelGGL(k_pgs<B>, dim3(grid), dim3(B), (size_t)d.pgs_lcap * 16 * kBodyLds, st, d);
 }
+template <int B>
+void launch_fused_b(const Lane& l, hipStream_t st) {
+  hipLaunchKernelGGL(k_fused<B>, dim3(l.grid), dim3(B), (size_t)l.d.pgs_lcap * 16 * kBodyLds, st, l.d);
+}
+template void launch_stage_b<BB_BLOCK>(int, const Dev&, hipStream_t, uint32_t);
+template void launch_pgs_b<BB_BLOCK>(const Dev&, hipStream_t, uint32_t);
+template void launch_fused_b<BB_BLOCK>(const Lane&, hipStream_t);
+#if BB_BLOCK == 128
+// one-wave colour sweeps ride along with the 128-thread unit
+template void launch_pgs_b<64>(const Dev&, hipStream_t, uint32_t);
+#endif
+#else
+template __global__ void k_team<kTeamBlock>(Dev);
 
 static void launch_stage(int stage, const Lane& l, hipStream_t st) {
   if (l.grid == 0) return;
   switch (l.block) {
-    case 512: launch_stage<512>(stage, l.d, st, l.grid); break;
-    case 1024: launch_stage<1024>(stage, l.d, st, l.grid); break;
-    default: launch_stage<256>(stage, l.d, st, l.grid); break;
+    case 128: launch_stage_b<128>(stage, l.d, st, l.grid); break;
+    case 512: launch_stage_b<512>(stage, l.d, st, l.grid); break;
+    case 1024: launch_stage_b<1024>(stage, l.d, st, l.grid); break;
+    default: launch_stage_b<256>(stage, l.d, st, l.grid); break;
   }
   if (stage == 4) {
     switch (l.pgs_block ? l.pgs_block : l.block) {
-      case 64: launch_pgs<64>(l.d, st, l.grid); break;
-      case 128: launch_pgs<128>(l.d, st, l.grid); break;
-      case 512: launch_pgs<512>(l.d, st, l.grid); break;
-      case 1024: launch_pgs<1024>(l.d, st, l.grid); break;
-      default: launch_pgs<256>(l.d, st, l.grid); break;
+      case 64: launch_pgs_b<64>(l.d, st, l.grid); break;
+      case 128: launch_pgs_b<128>(l.d, st, l.grid); break;
+      case 512: launch_pgs_b<512>(l.d, st, l.grid); break;
+      case 1024: launch_pgs_b<1024>(l.d, st, l.grid); break;
+      default: launch_pgs_b<256>(l.d, st, l.grid); break;
     }
   }
 }
@@ -4405,18 +4434,14 @@ static void launch_team(const Dev& d, hipStream_t st) {
   if (d.team_grid) hipLaunchKernelGGL(k_team<kTeamBlock>, dim3(d.team_grid), dim3(kTeamBlock), 0, st, d);
 }
 
-template <int B>
-static void launch_fused(const Lane& l, hipStream_t st) {
-  hipLaunchKernelGGL(k_fused<B>, dim3(l.grid), dim3(B), (size_t)l.d.pgs_lcap * 16 * kBodyLds, st, l.d);
-}
-
 void launch_step(const Lane& l, hipStream_t st, bool team) {
   if (team) launch_team(l.d, st);
   if (l.fused && l.grid) {
     switch (l.block) {
-      case 512: launch_fused<512>(l, st); break;
-      case 1024: launch_fused<1024>(l, st); break;
-      default: launch_fused<256>(l, st); break;
+      case 128: launch_fused_b<128>(l, st); break;
+      case 512: launch_fused_b<512>(l, st); break;
+      case 1024: launch_fused_b<1024>(l, st); break;
+      default: launch_fused_b<256>(l, st); break;
     }
     return;
   }
@@ -4464,4 +4489,5 @@ uint32_t team_capacity() {
   return (uint32_t)per * (uint32_t)cus;
 }
 
+#endif  // BB_BLOCK_TU
 }  // namespace bb

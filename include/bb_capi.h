@@ -525,7 +525,7 @@ bb_status bb_batch_set_block_size(bb_batch* b, uint32_t threads);
  * result: every world's trajectory is bit-identical for every setting. */
 typedef enum bb_param {
   BB_PARAM_BLOCK = 0,        /* threads per world workgroup: 0 = automatic,
-                                256, 512 or 1024                            */
+                                128, 256, 512 or 1024                       */
   BB_PARAM_PGS_LDS_ROWS = 1, /* rows per world whose velocities and world
                                 inverse inertia the PGS sweeps keep in LDS
                                 (80 B each): 0..2048                        */

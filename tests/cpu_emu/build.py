@@ -18,9 +18,14 @@ CSRC = os.path.join(REPO, "beat-box_amd", "csrc")
 OUT_DIR = os.path.join(HERE, "_build")
 FLAGS = ["-std=c++17", "-O1", "-g", "-fno-omit-frame-pointer", "-fsanitize=address",
          "-ffp-contract=off", "-w", "-I", HERE]
-UNITS = [(os.path.join(CSRC, "bb_step.hip"), True), (os.path.join(CSRC, "bb_capi.cpp"), False),
-         (os.path.join(CSRC, "bb_server.cpp"), False), (os.path.join(HERE, "emu.cpp"), False),
-         (os.path.join(HERE, "main.cpp"), False)]
+# (source, compile as C++, extra flags, object name): bb_step_block.hip once
+# per workgroup size, as the library's build does
+UNITS = [(os.path.join(CSRC, "bb_step.hip"), True, [], "bb_step.o")] + [
+    (os.path.join(CSRC, "bb_step_block.hip"), True, [f"-DBB_BLOCK={b}"], f"bb_step_block_{b}.o")
+    for b in (128, 256, 512, 1024)] + [
+    (os.path.join(CSRC, "bb_capi.cpp"), False, [], "bb_capi.o"),
+    (os.path.join(CSRC, "bb_server.cpp"), False, [], "bb_server.o"),
+    (os.path.join(HERE, "emu.cpp"), False, [], "emu.o"), (os.path.join(HERE, "main.cpp"), False, [], "main.o")]
 
 
 def build() -> str:
@@ -36,13 +41,13 @@ def _build() -> str:
     exe = os.path.join(OUT_DIR, "emu_main")
 
     def compile_one(unit):
-        src, as_cxx = unit
-        obj = os.path.join(OUT_DIR, os.path.basename(src) + ".o")
-        cmd = ["g++", *FLAGS, "-c"] + (["-x", "c++"] if as_cxx else []) + [src, "-o", obj]
+        src, as_cxx, extra, name = unit
+        obj = os.path.join(OUT_DIR, name)
+        cmd = ["g++", *FLAGS, *extra, "-c"] + (["-x", "c++"] if as_cxx else []) + [src, "-o", obj]
         subprocess.run(cmd, check=True)
         return obj
 
-    with ThreadPoolExecutor(max_workers=len(UNITS)) as ex:
+    with ThreadPoolExecutor(max_workers=min(len(UNITS), os.cpu_count() or 1)) as ex:
         objs = list(ex.map(compile_one, UNITS))
     subprocess.run(["g++", *FLAGS, *objs, "-o", exe, "-lpthread"], check=True)
     return exe

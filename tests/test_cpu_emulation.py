@@ -93,7 +93,7 @@ def test_team_and_workgroup_paths_in_bounds_and_identical(emu_main, tmp_path):
 
 def test_large_workgroups_in_bounds(emu_main, tmp_path):
     """1,024-thread per-world kernels (heavy register pressure on the GPU) on
-    an AVBD and a PGS world, without graphs."""
+    an AVBD and a PGS world, without graphs; 128-thread ones the same."""
     avbd = beatbox.WorldDesc.make(**scenes.world_desc(solver=scenes.SOLVER_AVBD, iterations=10,
                                                       morton_bits=63, quirks=0))
     pgs = beatbox.WorldDesc.make(**scenes.world_desc(iterations=8, morton_bits=63, quirks=0))
@@ -102,6 +102,9 @@ def test_large_workgroups_in_bounds(emu_main, tmp_path):
     a, x = _run(emu_main, scene, str(tmp_path / "a.bin"), 4, 0, 1024, graph=0)
     b, y = _run(emu_main, scene, str(tmp_path / "b.bin"), 4, 0, 256, graph=0)
     assert a == b and x == y
+    # 128-thread workgroups: the radix sort's digit scan takes two digits per thread
+    c, z = _run(emu_main, scene, str(tmp_path / "c.bin"), 4, 0, 128, graph=0)
+    assert c == a and z == x
 
 
 def test_emulated_step_equals_oracle(emu_main, tmp_path):

@@ -199,7 +199,8 @@ def test_launch_params_do_not_change_results():
                                             (256, 512, 1, 4, 1, 0), (512, 256, 1, 2, 0, 0), (256, 0, 0, 1, 1, 0),
                                             (512, 512, 0, 8, 1, 0), (1024, 256, 1, 8, 0, 0),
                                             (256, 128, 0, 4, 0, 64), (256, 256, 0, 4, 1, 128),
-                                            (512, 64, 0, 4, 0, 64)):
+                                            (512, 64, 0, 4, 0, 64), (128, 128, 0, 4, 0, 64),
+                                            (128, 256, 1, 2, 1, 0), (128, 0, 0, 8, 0, 128)):
         print(f"launch params: block {blk} lds {lds} fused {fused} lanes {lanes} prefetch {pre} pgs block {pb}",
               flush=True)
         b = beatbox.Batch()
