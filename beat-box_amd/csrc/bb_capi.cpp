@@ -1283,7 +1283,7 @@ bb_status autotune_batch(bb_batch* b, uint32_t reps, float* best_ms) {
   float best = 3.4e38f;
   uint32_t best_block = saved_block, best_lds = saved_lds;
   const uint32_t saved_fused = b->fused, saved_sort = b->sort_blocks, saved_lanes = b->avbd_lanes,
-                 saved_prefetch = b->pgs_prefetch, saved_walk = b->walk_budget,
+                 saved_prefetch = b->pgs_prefetch,
                  saved_team_blocks = b->team_blocks, saved_pgs_block = b->pgs_block;
   bb_status rc = BB_OK;
   // The wide lane is not tuned here: one replayed step early in a run does
@@ -1375,11 +1375,10 @@ bb_status autotune_batch(bb_batch* b, uint32_t reps, float* best_ms) {
     if (b->sort_blocks == 0) b->sort_blocks = b->base.sort_blocks = kTeamSortBlocks;
     tune(b->sort_blocks, &b->base.sort_blocks, {8u, 16u, 32u, 64u, 128u});
   }
-  // never below the default: the tuner replays an early step, before the
-  // piles settle and a ground's walk grows long, and a low budget then hands
-  // many rows to slow shared walks
-  if (b->walk_budget == 0) b->walk_budget = b->base.walk_budget = kWalkBudgetDefault;
-  tune(b->walk_budget, &b->base.walk_budget, {kWalkBudgetDefault, 512u, kMAX});
+  // The walk budget is not tuned: the tuner replays an early step, before
+  // the piles settle and a ground's walk grows long.  Chosen there, a budget
+  // of 64 made the 240-step window 11% slower (many rows in slow shared
+  // walks) and "never" 7% slower (the grounds' walks on one thread).
   // the team launch: half the resident workgroups (the default) or all of
   // them (the teams are laid out anew for each)
   if (b->base.team_grid && rc == BB_OK) {
@@ -1394,7 +1393,6 @@ bb_status autotune_batch(bb_batch* b, uint32_t reps, float* best_ms) {
   }
   if (rc != BB_OK) {
     b->pgs_block = saved_pgs_block;
-    b->walk_budget = b->base.walk_budget = saved_walk;
     b->team_blocks = saved_team_blocks;
     b->fused = saved_fused;
     b->sort_blocks = b->base.sort_blocks = saved_sort;

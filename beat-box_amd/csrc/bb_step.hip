@@ -2156,7 +2156,14 @@ BB_DI void s_contacts(const Dev& d, uint32_t w, const T& t) {
         uint32_t L0 = d.pstart_prev[off + lo];
         const uint32_t L1 = lo + 1 < W.prev_n ? d.pstart_prev[off + lo + 1] : Pprev;
         const uint32_t end = L1 < Pprev ? L1 : Pprev;
-        while (L0 < end && ppairs[L0].y < hi) ++L0;
+        // the first entry of row lo's list whose partner is not below hi: a
+        // binary search (a ground's list holds hundreds of partners, and a
+        // walk along it per pair was the stage's tail)
+        for (uint32_t R = end; L0 < R;) {
+          const uint32_t mid = L0 + (R - L0) / 2;
+          if (ppairs[mid].y < hi) L0 = mid + 1;
+          else R = mid;
+        }
         if (key >= 90 && key <= 95) {
           // voxel manifolds (keys 90..95) match only the same ordered pair
           // under the same key (find_old_collision's exact_only,

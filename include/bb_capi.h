@@ -609,9 +609,11 @@ bb_status bb_world_set_param(bb_world* w, bb_param p, uint32_t value);
  * sweep tile (BB_PARAM_PGS_PREFETCH, when a world runs PGS), the AVBD
  * solver tile (BB_PARAM_AVBD_LANES, when a world runs AVBD in the per-world
  * kernels), the radix sort's sub-team (BB_PARAM_SORT_BLOCKS, when a world
- * runs as a workgroup team), the broad phase's walk budget
- * (BB_PARAM_WALK_BUDGET) and the team launch's size (BB_PARAM_TEAM_BLOCKS:
- * half or all of the resident workgroups).  No candidate changes any result.  The
+ * runs as a workgroup team), the colour sweeps' workgroup and LDS rows
+ * (BB_PARAM_PGS_BLOCK) and the team launch's size (BB_PARAM_TEAM_BLOCKS:
+ * half or all of the resident workgroups).  The broad phase's walk budget
+ * keeps its value: an early step cannot show how long the walks grow once
+ * piles settle.  No candidate changes any result.  The
  * simulation state afterwards is exactly the state before the call.
  * *best_ms (may be NULL) receives the chosen step time. */
 bb_status bb_batch_autotune(bb_batch* b, uint32_t reps, float* best_ms);
