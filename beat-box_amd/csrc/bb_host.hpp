@@ -152,7 +152,7 @@ struct bb_batch {
   uint32_t team_bodies = 8192;  // BB_PARAM_TEAM_BODIES: worlds this large run in k_team
   // BB_PARAM_LANE_BODIES / _LANE_BLOCK: worlds this large (0: none) run the
   // per-world kernels with lane_block threads on a second stream
-  uint32_t lane_bodies = 512;
+  uint32_t lane_bodies = 384;
   uint32_t lane_block = 512;
   uint32_t lane_lds = 1024;  // BB_PARAM_LANE_LDS_ROWS: LDS rows of k_pgs in that lane
   uint32_t team_spin = 0;    // BB_PARAM_TEAM_SPIN: spins of a team barrier wait (0: default)

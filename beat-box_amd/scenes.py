@@ -313,6 +313,50 @@ def expanding_spheres(seed: int, side: int = 5, rate: float = 100.0):
     return bodies
 
 
+def crossing_cloud(seed: int, planks: int = 20, side: int = 5, speed: float = 20.0):
+    """Overflow in a few middle steps only: `planks` static thin boards (half
+    extents 10 x 0.05 x 0.05 m, turned 45 degrees about z, all on the
+    diagonal y = x through the origin, so each board's box is a 14 x 14 m
+    square 0.1 m thick in z) and a side x side layer of spheres (r 0.3 m, 2 m
+    apart, at least 1.5 m off the diagonal) 3 m below them in z, moving
+    through their plane at `speed` m/s along +z.  The boards' mutual pairs
+    (planks * (planks - 1) / 2) fit the 8n + 64 pair slots; while the layer
+    crosses the boards' plane every sphere's box overlaps every board's
+    (planks more pairs per sphere), which does not; no sphere touches a
+    board or another sphere, so the layer flies on unchanged and the pairs
+    are gone again.  Run with gravity 0."""
+    pl = np.zeros(planks, BODY_DESC)
+    pl["shape"] = SHAPE_BOX
+    pl["flags"] = 0
+    c = math.cos(math.pi / 8.0)
+    sn = math.sin(math.pi / 8.0)
+    pl["rotation"] = (0.0, 0.0, sn, c)
+    pl["minimum"] = (-10.0, -0.05, -0.05)
+    pl["maximum"] = (10.0, 0.05, 0.05)
+    pl["friction"] = 0.5
+    pts = []
+    k = np.arange(-(side - 1), side, 2, dtype=np.float64) * 1.0
+    for x in k:
+        for y in k:
+            if abs(y - x) >= 1.5:
+                pts.append((x, y))
+    cnt = np.arange(len(pts), dtype=np.uint64) * np.uint64(3)
+    jitter = uniform(seed, cnt, -0.05, 0.05)
+    sp = np.zeros(len(pts), BODY_DESC)
+    sp["shape"] = SHAPE_SPHERE
+    sp["flags"] = BODY_DYNAMIC
+    for i, (x, y) in enumerate(pts):
+        sp["position"][i] = (x + jitter[i], y - jitter[i], -3.0)
+    sp["rotation"] = (0.0, 0.0, 0.0, 1.0)
+    sp["radius"] = 0.3
+    sp["minimum"] = -0.3
+    sp["maximum"] = 0.3
+    sp["mass"] = (4.0 / 3.0 * math.pi * 0.3 ** 3)
+    sp["velocity"] = (0.0, 0.0, speed)
+    sp["friction"] = 0.5
+    return np.concatenate([pl, sp])
+
+
 def serving_pile(seed: int, n: int):
     """One serving request: a random pile of n bodies, 50/50 boxes and
     spheres with sizes (edge / diameter) U(0.2, 1.0) m, centres in a column

@@ -535,7 +535,7 @@ typedef enum bb_param {
   BB_PARAM_LANE_BODIES = 3,  /* worlds with at least this many bodies (and
                                 below the team threshold) run the per-world
                                 kernels in a second lane, concurrently with
-                                the rest; 0 = one lane.  Default 512        */
+                                the rest; 0 = one lane.  Default 384        */
   BB_PARAM_LANE_BLOCK = 4,   /* threads per workgroup in that lane: 256, 512
                                 or 1024.  Default 512                       */
   BB_PARAM_LANE_LDS_ROWS = 5, /* BB_PARAM_PGS_LDS_ROWS of that lane: 0..2048.

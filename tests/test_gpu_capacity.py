@@ -138,3 +138,47 @@ def test_batch_pair_scale_gives_room_and_equals_growth():
         b.set_param(beatbox.PARAM_PAIR_SCALE, 0)
     b.destroy()
     w.destroy()
+
+
+def test_overflow_in_a_middle_step_only():
+    """A layer of spheres flying through the plane of twenty static boards
+    (scenes.crossing_cloud): 190 board pairs against 384 pair slots, 590
+    pairs in steps 9 and 10 while every sphere's box overlaps every board's,
+    190 again after.  A batch world flags steps 9 and 10 only; its mark
+    stays until collected, through the later calm steps.  The single world
+    grows its capacity in step 9 and replays that step from the state it
+    read, and every step, taps included, equals the oracle bit for bit."""
+    desc = beatbox.WorldDesc.make(**DESC)
+    bodies = scenes.crossing_cloud(0xBB7D)
+    cap = 8 * len(bodies) + 64
+    b = beatbox.Batch()
+    b.add_world(desc, bodies)
+    flagged = []
+    for s in range(14):
+        b.step(1)
+        try:
+            b.sync()
+        except beatbox.BBError as e:
+            assert e.code == 5 and s in (9, 10), s  # BB_ERR_OVERFLOW until the mark is collected
+        st = b.stats(0)
+        if st["overflow"]:
+            flagged.append(s)
+        assert st["broad_phase_collision_count"] == (590 if s in (9, 10) else 190), s
+        if s == 10:
+            # the mark outlives the counters: collected only now
+            assert b.overflow() == [0]
+            assert b.overflow() == []
+    assert flagged == [9, 10]
+    b.destroy()
+    gpu = beatbox.RigidBodyManager(desc)
+    gpu.add_bodies(bodies)
+    ora = OracleWorld(desc)
+    ora.add_bodies(bodies)
+    for s in range(14):
+        gpu.simulate(1)
+        ora.step(1)
+        bad = compare_step(gpu, ora, desc.morton_bits, s)
+        assert not bad, (s, "\n".join(bad[:20]))
+        assert gpu.read_back_sim_config()["overflow"] == 0, s
+    assert 590 > cap
+    gpu.destroy()
