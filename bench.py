@@ -90,30 +90,42 @@ def build_bytes(n: int, kw: int) -> dict:
     }
 
 
-def profiled_traffic(kernel: str, worlds: int):
+def lane_launches(kernel: str, params: dict, n_wide: int, worlds: int) -> list:
+    """The dispatches of `kernel` in one step of the batch, as the
+    (workgroup size, workgroups) of each lane: the first lane steps the
+    worlds below the wide lane's threshold with BB_PARAM_BLOCK workgroups
+    (its colour sweeps with BB_PARAM_PGS_BLOCK, when set), the wide lane
+    the n_wide others with BB_PARAM_LANE_BLOCK ones."""
+    first = params["block"]
+    if kernel == "k_pgs" and params.get("pgs_block"):
+        first = params["pgs_block"]
+    out = [(first, worlds - n_wide)]
+    if n_wide:
+        out.append((params["lane_block"], n_wide))
+    return out
+
+
+def profiled_traffic(kernel: str, params: dict, n_wide: int, worlds: int):
     """HBM bytes per step (read + write) of `kernel` over the whole batch,
     from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs of this same
-    command (copy-kernel calibrated), or None when there is none.  The step
-    launches the kernel once per lane; the lanes' dispatches are the ones
-    that ran every step (the autotuner's candidates ran a few times each),
-    and their workgroups add up to the batch's worlds."""
+    command (copy-kernel calibrated): the sum over the step's dispatches of
+    the kernel (one per lane, named by workgroup size and count), or None
+    when the profile has no such dispatches (other launch parameters)."""
     try:
         with open(TRAFFIC_FILE) as f:
             k = json.load(f)["kernels"]
     except (OSError, ValueError, KeyError):
         return None
-    rows = [v for v in k.values() if v["kernel"].startswith(f"bb::{kernel}<") and v["read_bytes"] is not None
-            and v["workgroups"] > 1]
-    if not rows:
-        return None
-    most = max(v["calls"] for v in rows)
-    lanes = [v for v in rows if v["calls"] * 2 >= most]
-    if sum(v["workgroups"] for v in lanes) != worlds:
-        return None
-    return sum(float(v["read_bytes"]) + float(v["write_bytes"] or 0.0) for v in lanes)
+    total = 0.0
+    for blk, grid in lane_launches(kernel, params, n_wide, worlds):
+        v = k.get(f"bb::{kernel}<{blk}>@{grid}")
+        if v is None or v["read_bytes"] is None:
+            return None
+        total += float(v["read_bytes"]) + float(v["write_bytes"] or 0.0)
+    return total
 
 
-def traffic_fields(kernel: str, args, model_per_launch: float) -> dict:
+def traffic_fields(kernel: str, args, model_per_launch: float, params: dict, n_wide: int) -> dict:
     """roofline.traffic: HBM bytes per launch of `kernel` from the committed
     rocprofv3 counter runs (FETCH_SIZE and WRITE_SIZE, copy-kernel
     calibrated), reported only when they were taken over this run's step
@@ -124,7 +136,7 @@ def traffic_fields(kernel: str, args, model_per_launch: float) -> dict:
             meta = json.load(f)
     except (OSError, ValueError):
         meta = {}
-    prof = profiled_traffic(kernel, args.worlds)
+    prof = profiled_traffic(kernel, params, n_wide, args.worlds)
     window = meta.get("window")  # {"steps": S, "warmup": W} of the counter runs
     model = meta.get("model_bytes_per_launch", {}).get(kernel)
     same = bool(window) and (window.get("steps"), window.get("warmup")) == (args.steps, args.warmup)
@@ -145,10 +157,12 @@ def make_batch(worlds: int, seed: int, lo: int, hi: int):
     b = beatbox.Batch()
     b.set_param(beatbox.PARAM_PAIR_SCALE, PAIR_SCALE)
     total = 0
+    b.world_bodies = []
     for w, n in enumerate(sizes):
         bodies = scenes.batch_world(seed, w, int(n))
         b.add_world(desc, bodies)
         total += len(bodies)
+        b.world_bodies.append(len(bodies))
     log(f"batch: {worlds} worlds, {total} bodies, built in {time.time() - t0:.1f}s")
     return b, total
 
@@ -441,6 +455,8 @@ def main() -> int:
     if overflowed:
         log(f"batch: {len(overflowed)} worlds flagged an overflow: {overflowed[:16]}")
     log(f"launch parameters: {params}")
+    # worlds in the wide lane (no world of this batch reaches the team threshold)
+    n_wide = sum(1 for n in batch.world_bodies if params["lane_bodies"] and n >= params["lane_bodies"])
     dev_gib = batch.device_bytes / 2**30
     batch.destroy()
     value = total * args.steps / t
@@ -562,7 +578,7 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            **traffic_fields(dominant, args, sb[dominant] / psteps),
+            **traffic_fields(dominant, args, sb[dominant] / psteps, params, n_wide),
         },
         "cpu_baseline": cpu,
     }
