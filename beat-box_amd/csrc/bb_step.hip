@@ -3036,22 +3036,22 @@ struct PgsRow {
   float4 r1[kPgsPre], r2[kPgsPre], l[kPgsPre];
   float tm[kPgsPre];
 };
+// The first kPgsPre contacts' data are loaded whatever the row's contact
+// count: the contact arrays hold 8 entries per row slot, so the loads stay in
+// bounds, and issuing them with the header instead of after it (the count is
+// in the header) saves a dependent round trip per colour phase.  Entries past
+// the count are never used.
 BB_DI void pgs_fetch(const Dev& d, uint32_t k, int kind, PgsRow& R) {
   const uint32_t NP = d.np;
   R.h = d.sv_hdr[k];
-  const int count = R.h.z & 255;
+  R.n = d.sv_n[k];
 #pragma unroll
   for (int c = 0; c < kPgsPre; ++c) {
-    R.r1[c] = R.r2[c] = R.l[c] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    R.tm[c] = 0.0f;
-    if (c < count) {
-      R.r1[c] = d.sv_r1[c * NP + k];
-      R.r2[c] = d.sv_r2[c * NP + k];
-      R.l[c] = d.sv_l[c * NP + k];
-      if (kind != 0) R.tm[c] = d.sv_tm1[c * NP + k];
-    }
+    R.r1[c] = d.sv_r1[c * NP + k];
+    R.r2[c] = d.sv_r2[c * NP + k];
+    R.l[c] = d.sv_l[c * NP + k];
+    R.tm[c] = kind != 0 ? d.sv_tm1[c * NP + k] : 0.0f;
   }
-  R.n = d.sv_n[k];
 }
 // kind 0: warm-start impulses of pre_step; 1: solve with bias; 2: relax.
 // kPlain: PGS (the per-island path) instead of PGS_SOFT.  R: the row's
