@@ -3278,10 +3278,17 @@ BB_DI void s_pgs(const Dev& d, uint32_t w, const T& t, float4* pgs_lds, uint32_t
         c = cn;
       }
     } else {
+      // the next colour's end offset is loaded while this colour's rows are
+      // solved, so no phase starts with a dependent load of its range
+      uint32_t b0 = bo[0], b1 = used > 0 ? bo[1] : 0u;
       for (uint32_t c = 0; c < used; ++c) {
-        if (bo[c] == bo[c + 1]) continue;
-        for (uint32_t k = bo[c] + tid; k < bo[c + 1]; k += NT) row(k, kind);
-        t.sync();
+        const uint32_t b2 = c + 1 < used ? bo[c + 2] : 0u;
+        if (b0 != b1) {
+          for (uint32_t k = b0 + tid; k < b1; k += NT) row(k, kind);
+          t.sync();
+        }
+        b0 = b1;
+        b1 = b2;
       }
     }
     if (ovf_hi > ovf_lo) {
