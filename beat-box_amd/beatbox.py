@@ -87,7 +87,7 @@ class ServerDesc(C.Structure):
     ]
 
 
-SERVER_MAX_POOLS = 3
+SERVER_MAX_POOLS = 4
 
 
 class ServerReport(C.Structure):
@@ -97,7 +97,7 @@ class ServerReport(C.Structure):
         "body_steps")] + [(n, C.c_double) for n in (
         "device_ms", "step_ms_p50", "step_ms_p99", "queue_ms_p50", "queue_ms_p99")] + [
         ("reruns", C.c_uint64), ("pools", C.c_uint32), ("pad", C.c_uint32),
-        ("pool_min_bodies", C.c_uint32 * SERVER_MAX_POOLS), ("pool_pad", C.c_uint32),
+        ("pool_min_bodies", C.c_uint32 * SERVER_MAX_POOLS),
         ("pool_world_steps", C.c_uint64 * SERVER_MAX_POOLS), ("pool_step_ms_p50", C.c_double * SERVER_MAX_POOLS),
         ("pool_step_ms_p99", C.c_double * SERVER_MAX_POOLS)]
 

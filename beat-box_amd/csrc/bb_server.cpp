@@ -669,20 +669,26 @@ bb_status bb_server_create(const bb_server_desc* desc, bb_server** out) {
   if (!s) return BB_ERR_CAPACITY;
   s->desc = *desc;
   const uint32_t team_bodies = desc->team_bodies == 0 ? 8192u : desc->team_bodies;  // 0xFFFFFFFF: never
-  // pools by body count: [0, split), [split, team), [team, inf) -- the
-  // middle one only when split < team; one pool without a split
+  // pools by body count: [0, s), [s, 4s), [4s, team), [team, inf) with
+  // s = split (the middle ones only below team, at most BB_SERVER_MAX_POOLS
+  // pools); one pool without a split
   std::vector<std::pair<uint32_t, uint32_t>> bounds;
   const uint32_t split = desc->split_bodies;
   if (split == 0) {
     bounds.emplace_back(0u, 0xFFFFFFFFu);
   } else {
-    bounds.emplace_back(0u, split);
-    if (team_bodies != 0xFFFFFFFFu && split < team_bodies) {
-      bounds.emplace_back(split, team_bodies);
-      bounds.emplace_back(team_bodies, 0xFFFFFFFFu);
-    } else {
-      bounds.emplace_back(split, 0xFFFFFFFFu);
+    std::vector<uint32_t> cuts{split};
+    const bool teams = team_bodies != 0xFFFFFFFFu && split < team_bodies;
+    for (uint64_t next = 4ull * split; teams && next < team_bodies && cuts.size() + 2 < BB_SERVER_MAX_POOLS;
+         next *= 4)
+      cuts.push_back((uint32_t)next);
+    if (teams) cuts.push_back(team_bodies);
+    uint32_t lo = 0;
+    for (uint32_t c : cuts) {
+      bounds.emplace_back(lo, c);
+      lo = c;
     }
+    bounds.emplace_back(lo, 0xFFFFFFFFu);
   }
   s->W = desc->max_worlds * (uint32_t)bounds.size();
   const uint64_t base = device_bytes(1, 1, 1, 1);

@@ -427,9 +427,9 @@ def main() -> int:
     ap.add_argument("--serving-cap", type=int, default=200_000, help="resident-body cap of the serving run")
     ap.add_argument("--serving-team-bodies", type=int, default=4096,
                     help="worlds this large are stepped as workgroup teams in the serving run")
-    ap.add_argument("--serving-split-bodies", type=int, default=1024,
-                    help="serving pools: worlds below this size, up to the team size, and above step at "
-                         "their own cadences (0: one lockstep pool)")
+    ap.add_argument("--serving-split-bodies", type=int, default=256,
+                    help="serving pools: worlds below this size s, in [s, 4s), up to the team size, and "
+                         "above step at their own cadences (0: one lockstep pool)")
     ap.add_argument("--profile-steps", type=int, default=0, help="0: the timed steps")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run the copy kernel on MIB MiB (counter calibration) and exit")

@@ -631,7 +631,7 @@ void bb_batch_destroy(bb_batch* b);
  * trajectory is bit-identical to stepping it alone (bb_world_*), wherever it
  * sits in the device arrays and whatever else is resident. */
 typedef struct bb_server bb_server;
-#define BB_SERVER_MAX_POOLS 3
+#define BB_SERVER_MAX_POOLS 4
 
 typedef struct bb_server_desc {
   uint64_t hbm_budget_bytes;    /* device bytes for resident state (admission) */
@@ -645,10 +645,13 @@ typedef struct bb_server_desc {
   uint32_t split_bodies;        /* 0: one pool, every resident world steps in
                                    lockstep.  Else pools by body count, each
                                    stepping at its own cadence on its own
-                                   stream: [0, split_bodies) (workgroups of
-                                   `block` threads), [split_bodies,
-                                   team_bodies) (512-thread workgroups) and
-                                   [team_bodies, ...) (teams); max_worlds
+                                   stream: [0, s) with s = split_bodies
+                                   (workgroups of `block` threads), [s, 4s)
+                                   and [4s, team_bodies) when 4s is below
+                                   team_bodies (else [s, team_bodies)),
+                                   both with 512-thread workgroups, and
+                                   [team_bodies, ...) (teams); at most
+                                   BB_SERVER_MAX_POOLS pools, max_worlds
                                    slots each                                */
 } bb_server_desc;
 
@@ -679,7 +682,6 @@ typedef struct bb_server_report {
    * its world-steps and their latency percentiles, as step_ms_p50/p99 */
   uint32_t pools, pad;
   uint32_t pool_min_bodies[BB_SERVER_MAX_POOLS];
-  uint32_t pool_pad;
   uint64_t pool_world_steps[BB_SERVER_MAX_POOLS];
   double pool_step_ms_p50[BB_SERVER_MAX_POOLS];
   double pool_step_ms_p99[BB_SERVER_MAX_POOLS];

@@ -77,12 +77,14 @@ def test_batch_4096_worlds_sampled_equal_oracle_and_alone():
     b.destroy()
 
 
-@pytest.mark.parametrize("team_bodies,split_bodies", [(1024, 0), (1024, 1024), (8192, 1024)])
-def test_serving_stream_shape_equals_alone(team_bodies, split_bodies):
+@pytest.mark.parametrize("team_bodies,split_bodies,npools", [(1024, 0, 1), (1024, 1024, 2), (8192, 1024, 4),
+                                                              (4096, 256, 4)])
+def test_serving_stream_shape_equals_alone(team_bodies, split_bodies, npools):
     """One lockstep pool, two pools (one workgroup per world below 1,024
-    bodies, teams above) and three pools (one workgroup, a 512-thread
-    workgroup from 1,024 bodies, teams from 8,192), each stepping at its own
-    cadence: every world equals itself stepped alone."""
+    bodies, teams above) and four pools (one workgroup per world below the
+    split s, 512-thread workgroups over [s, 4s) and [4s, team), teams above),
+    each stepping at its own cadence: every world equals itself stepped
+    alone."""
     desc = beatbox.WorldDesc.make(**scenes.world_desc(solver=scenes.SOLVER_AVBD, iterations=10,
                                                       morton_bits=63, quirks=0))
     sizes = [20000, 64, 3000, 150, 1500, 700, 9000, 90, 2500, 400, 1100, 5000]
@@ -99,7 +101,7 @@ def test_serving_stream_shape_equals_alone(team_bodies, split_bodies):
     assert rep["retired"] == len(reqs) and rep["reruns"] == 0
     assert rep["step_ms_p99"] >= rep["step_ms_p50"] > 0
     pools = rep["pools"]
-    assert len(pools) == (1 if not split_bodies else (2 if team_bodies <= split_bodies else 3))
+    assert len(pools) == npools
     for k, p in enumerate(pools):  # every world's steps counted once, in the pool of its size
         hi = pools[k + 1]["min_bodies"] if k + 1 < len(pools) else 1 << 32
         want = sum(h for n, h in zip(sizes, horizons) if p["min_bodies"] <= n < hi)
