@@ -730,11 +730,11 @@ bb_status bb_server_create(const bb_server_desc* desc, bb_server** out) {
     P.hi = bounds[p].second;
     P.w0 = p * desc->max_worlds;
     P.w1 = P.w0 + desc->max_worlds;
-    // one pool: the descriptor's workgroup size for every world; pools: the
-    // small worlds' pool keeps it, the larger ones run 512-thread
-    // workgroups that keep 1,024 rows of body state in LDS
-    P.block = (p == 0 || desc->block) ? b->block : 512u;
-    P.lcap = (p == 0 || desc->block) ? b->pgs_lds : 1024u;
+    // the first pool (the only one without a split) runs the descriptor's
+    // workgroup size; the larger worlds' pools run 512-thread workgroups
+    // that keep 1,024 rows of body state in LDS
+    P.block = p == 0 ? b->block : 512u;
+    P.lcap = p == 0 ? b->pgs_lds : 1024u;
     P.teams = b->team_bodies != 0 && P.hi > b->team_bodies;
     for (uint32_t w = P.w1; w > P.w0; --w) P.free_slots.push_back(w - 1);
     if (hipStreamCreateWithFlags(&P.stream, hipStreamNonBlocking) != hipSuccess) st = BB_ERR_DEVICE;

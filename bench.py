@@ -328,6 +328,19 @@ def next_rows(seed: int) -> dict:
     return out
 
 
+def serving_stream(seed: int, duration: float):
+    """The stream's arrival times and request bodies (generated once per
+    seed and duration, and kept for the next run)."""
+    key = (seed, duration)
+    if getattr(serving_stream, "key", None) != key:
+        arr, nb, hz, sd = scenes.serving_requests(seed, duration=duration)
+        serving_stream.key = key
+        serving_stream.arrivals = arr
+        serving_stream.horizons = hz
+        serving_stream.bodies = [scenes.serving_pile(int(sd[k]) & 0xFFFFFFFF, int(nb[k])) for k in range(len(arr))]
+    return serving_stream.arrivals, serving_stream.bodies
+
+
 def serving(duration: float, budget_gib: float, resident_cap: int, team_bodies: int, split_bodies: int,
             seed: int = 0xBB05) -> dict:
     """Serving stream: Poisson arrivals at 500 worlds/s for `duration` seconds
@@ -335,10 +348,13 @@ def serving(duration: float, budget_gib: float, resident_cap: int, team_bodies: 
     steps), admitted by the HBM budget (and a resident-body cap unless 0)
     into pools by size that step at their own cadence (split_bodies 0: one
     lockstep pool); reports the server's counters and latency percentiles,
-    overall and per pool, at the end of the arrival window."""
+    overall and per pool, at the end of the arrival window.  The requests'
+    bodies are generated before the clock starts, as a server receives them
+    ready-made."""
     desc = beatbox.WorldDesc.make(**scenes.world_desc(solver=scenes.SOLVER_AVBD, iterations=10,
                                                       morton_bits=63, quirks=0))
-    arr, nb, hz, sd = scenes.serving_requests(seed, duration=duration)
+    arr, bodies = serving_stream(seed, duration)
+    hz = serving_stream.horizons
     srv = beatbox.Server(budget_bytes=int(budget_gib * 2**30), max_worlds=4096,
                          max_resident_bodies=resident_cap, team_bodies=team_bodies, split_bodies=split_bodies)
     t0 = time.perf_counter()
@@ -348,7 +364,7 @@ def serving(duration: float, budget_gib: float, resident_cap: int, team_bodies: 
         if now >= duration:
             break
         while k < len(arr) and arr[k] <= now:
-            srv.submit(desc, scenes.serving_pile(int(sd[k]) & 0xFFFFFFFF, int(nb[k])), int(hz[k]))
+            srv.submit(desc, bodies[k], int(hz[k]))
             k += 1
         srv.step(1)
     rep = srv.report()

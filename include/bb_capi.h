@@ -639,7 +639,10 @@ typedef struct bb_server_desc {
   uint32_t max_bodies;          /* body rows; 0 = as many as the budget holds */
   uint32_t max_resident_bodies; /* 0 = no cap; else admit while the resident
                                    bodies stay within it (latency bound)      */
-  uint32_t block;               /* threads per world workgroup, 0 = 256       */
+  uint32_t block;               /* threads per world workgroup of the first
+                                   pool (the only one without a split):
+                                   128, 256, 512 or 1024, 0 = 256; the other
+                                   pools run 512                             */
   uint32_t team_bodies;         /* worlds this large run as workgroup teams;
                                    0 = 8192, 0xFFFFFFFF = never              */
   uint32_t split_bodies;        /* 0: one pool, every resident world steps in
