@@ -1897,9 +1897,14 @@ BB_DI int vox_collide(const Dev& d, const BodyView& A, const BodyView& B, uint32
 // IslandBuilder::link_bodies (src/shaders/simulation/islands.slang:153-213):
 // lock-free union-find, links always point to the lower index, so every
 // component's root is its minimum active index whatever the timing.
+// A root as seen now: the links are read with device-scope atomic loads,
+// not read-modify-writes (an atomicAdd of 0 per hop serialises every thread
+// of a world on the popular root's address).  A stale root only makes the
+// CAS in uf_link fail and walk again; the final roots (the minimum index of
+// each set) do not depend on the interleaving.
 BB_DI uint32_t uf_root(uint32_t* link, uint32_t x) {
   for (;;) {
-    uint32_t y = atomicAdd(&link[x], 0u);
+    const uint32_t y = __hip_atomic_load(&link[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (y == x) return x;
     x = y;
   }
