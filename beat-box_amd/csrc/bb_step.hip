@@ -768,6 +768,43 @@ __global__ __launch_bounds__(BLOCK) void k_build(Dev d) {
   s_build(d, w, t, L);
 }
 
+// Exclusive scan of value(i) over i in [0, n), in index order, by the whole
+// team, with one team scan: each wave takes a contiguous run of 64-row
+// chunks and sums it, the team scan of the wave sums orders the waves, and
+// each wave walks its run again, handing sink(i, offset) every row (a scan
+// per NT-row chunk costs a team scan and two barriers per chunk, which adds
+// up over a million rows).  value must read what no sink call writes, except
+// at its own index; returns the total to every thread.  All threads call it.
+template <class T, class Val, class Sink>
+BB_DI uint32_t scan_range(const T& t, uint32_t n, Val&& value, Sink&& sink) {
+  const uint32_t lane = t.rank() & (WAVE - 1), wv = t.rank() / WAVE, nw = t.size() / WAVE;
+  const uint32_t chunks = (n + WAVE - 1) / WAVE, per = (chunks + nw - 1) / nw;
+  const uint32_t c0 = wv * per < chunks ? wv * per : chunks;
+  const uint32_t c1 = c0 + per < chunks ? c0 + per : chunks;
+  uint32_t sum = 0;
+  for (uint32_t c = c0; c < c1; ++c) {
+    const uint32_t i = c * WAVE + lane;
+    if (i < n) sum += value(i);
+  }
+  for (int o = 32; o > 0; o >>= 1) sum += __shfl_down(sum, o, WAVE);
+  uint32_t total;
+  const uint32_t ex = t.scan(lane == 0 ? sum : 0u, total);
+  uint32_t carry = __shfl(ex, 0, WAVE);  // the sums of the waves before this one
+  for (uint32_t c = c0; c < c1; ++c) {
+    const uint32_t i = c * WAVE + lane;
+    const uint32_t v = i < n ? value(i) : 0u;
+    uint32_t x = v;
+#pragma unroll
+    for (int o = 1; o < WAVE; o <<= 1) {
+      const uint32_t y = __shfl_up(x, o, WAVE);
+      if (lane >= (uint32_t)o) x += y;
+    }
+    if (i < n) sink(i, carry + x - v);
+    carry += __shfl(x, WAVE - 1, WAVE);
+  }
+  return total;
+}
+
 // ================================================================ k_broad
 BB_DI bool overlap(v3 amn, v3 amx, float4 bmn, float4 bmx) {  // Aabb::overlap, math.hpp:213-217
   return amn.x <= bmx.x && amx.x >= bmn.x && amn.y <= bmx.y && amx.y >= bmn.y && amn.z <= bmx.z &&
@@ -1035,33 +1072,22 @@ BB_DI void s_broad(const Dev& d, uint32_t w, const T& t, BroadShared& L, WalkLds
     if (threadIdx.x == 0) d.pcount[off + i] = kDeferBit | cnt;
   }
   if (nd) t.sync();
-  for (uint32_t c0 = 0; c0 < N; c0 += NT) {
-    uint32_t i = c0 + tid;
-    uint32_t cnt = 0, slots = 0;
-    if (i < N) {
-      cnt = d.pcount[off + i] & ~kDeferBit;
-      slots = vox ? d.poff[off + i] : cnt;
-    }
-    uint32_t total;
-    uint32_t ex = t.scan(slots, total);
-    if (i < N) {
-      d.poff[off + i] = L.carry + ex;
-      d.pstart_cur[off + i] = L.carry + ex;
-    }
-    if (vox) {
-      uint32_t tp;
-      (void)t.scan(cnt, tp);
-      if (tid == 0) L.pairs += tp;
-    }
-    t.sync();
-    if (tid == 0) L.carry += total;
-    t.sync();
-  }
-  const uint32_t P = L.carry;
+  // per-row offsets of the pair slots (voxel worlds: slots parked in poff,
+  // and the pairs counted apart)
+  const uint32_t vox_pairs =
+      vox ? scan_range(t, N, [&](uint32_t i) { return d.pcount[off + i] & ~kDeferBit; }, [](uint32_t, uint32_t) {})
+          : 0u;
+  const uint32_t P = scan_range(
+      t, N, [&](uint32_t i) { return vox ? d.poff[off + i] : (d.pcount[off + i] & ~kDeferBit); },
+      [&](uint32_t i, uint32_t ex) {
+        d.poff[off + i] = ex;
+        d.pstart_cur[off + i] = ex;
+      });
+  t.sync();
   const uint32_t cap = W.pair_cap;
   if (tid == 0) {
     d.npairs_cur[w] = P < cap ? P : cap;
-    d.stats[w * ST_COUNT + ST_P] = vox ? L.pairs : P;
+    d.stats[w * ST_COUNT + ST_P] = vox ? vox_pairs : P;
     if (P > cap) d.stats[w * ST_COUNT + ST_OVERFLOW] = 1;
   }
   uint2* pairs = d.pairs_cur + W.pair_off;
@@ -2277,21 +2303,13 @@ template <class T>
 BB_DI void build_adj(const Dev& d, const WorldInfo& W, uint32_t P, const T& t, uint32_t& carry) {
   const uint32_t tid = t.rank(), NT = t.size();
   const uint32_t N = W.n, off = W.body_off;
-  if (tid == 0) carry = 0;
+  (void)carry;
+  (void)scan_range(t, N, [&](uint32_t i) { return d.deg[off + i]; },
+                   [&](uint32_t i, uint32_t ex) {
+                     d.adj_off[off + i] = ex;
+                     d.pcount[off + i] = 0;  // fill cursor
+                   });
   t.sync();
-  for (uint32_t c0 = 0; c0 < N; c0 += NT) {
-    uint32_t i = c0 + tid;
-    uint32_t v = i < N ? d.deg[off + i] : 0u;
-    uint32_t tot;
-    uint32_t ex = t.scan(v, tot);
-    if (i < N) {
-      d.adj_off[off + i] = carry + ex;
-      d.pcount[off + i] = 0;  // fill cursor
-    }
-    t.sync();
-    if (tid == 0) carry += tot;
-    t.sync();
-  }
   uint32_t* adj = d.adj + 2 * W.pair_off;
   for (uint32_t p = tid; p < P; p += NT) {
     const uint32_t slot = W.pair_off + p;
@@ -2424,25 +2442,17 @@ BB_DI void s_post(const Dev& d, uint32_t w, const T& t, PostShared& L) {
   }
   // ---- islands (islands.slang:215-302): root = min active index; slots in
   // ascending root order
-  if (tid == 0) L.carry = 0;
-  t.sync();
-  for (uint32_t c0 = 0; c0 < A; c0 += NT) {
-    uint32_t k = c0 + tid;
-    uint32_t is_root = 0;
-    if (k < A) {
-      uint32_t x = k;
-      while (link[x] != x) x = link[x];
-      d.ascratch[W.act_off + k] = x;
-      is_root = x == k ? 1u : 0u;
-    }
-    uint32_t tot;
-    uint32_t ex = t.scan(is_root, tot);
-    if (k < A && is_root) d.poff[off + k] = L.carry + ex;  // slot of root k
-    t.sync();
-    if (tid == 0) L.carry += tot;
-    t.sync();
+  for (uint32_t k = tid; k < A; k += NT) {
+    uint32_t x = k;
+    while (link[x] != x) x = link[x];
+    d.ascratch[W.act_off + k] = x;
   }
-  const uint32_t islands = L.carry;
+  t.sync();
+  const uint32_t* root = d.ascratch + W.act_off;
+  const uint32_t islands = scan_range(t, A, [&](uint32_t k) { return root[k] == k ? 1u : 0u; },
+                                      [&](uint32_t k, uint32_t ex) {
+                                        if (root[k] == k) d.poff[off + k] = ex;  // slot of root k
+                                      });
   for (uint32_t i = tid; i < N; i += NT) {
     d.island[off + i] = kMAX;
     d.islot[off + i] = kMAX;
@@ -2556,32 +2566,23 @@ BB_DI void s_post(const Dev& d, uint32_t w, const T& t, PostShared& L) {
   uint32_t* pick = d.pick + W.pair_off;
   int4* rec = d.sv_hdr + W.pair_off;
   for (uint32_t i = tid; i < N; i += NT) d.mask[off + i] = 0;
-  if (tid == 0) L.carry = 0;
+  const uint32_t npend0 = scan_range(
+      t, P, [&](uint32_t p) { return d.man_cur.hdr[W.pair_off + p].w > 0 ? 1u : 0u; },
+      [&](uint32_t p, uint32_t ex) {
+        const int4 h = d.man_cur.hdr[W.pair_off + p];
+        const bool f = h.w > 0;
+        color[p] = f ? kMAX : kMAX - 1;
+        if (!f) return;
+        const uint32_t ga = off + h.x, gb = off + h.y;
+        const uint32_t sa = (d.cur.flags[ga] & BB_BODY_DYNAMIC) ? 0u : 0x80000000u;
+        const uint32_t sb = (d.cur.flags[gb] & BB_BODY_DYNAMIC) ? 0u : 0x80000000u;
+        const uint32_t ida = d.cur.id[ga], idb = d.cur.id[gb];
+        const uint32_t lo = ida < idb ? ida : idb, hi = ida < idb ? idb : ida;
+        const uint32_t pid = gc_hash(lo * 0x9e3779b9u + hi * 0x85ebca6bu + (uint32_t)h.z * 0xc2b2ae35u);
+        rec[ex] = make_int4((int)(ga | sa), (int)(gb | sb), (int)pid, (int)p);
+      });
+  if (tid == 0) L.carry = npend0;
   t.sync();
-  for (uint32_t c0 = 0; c0 < P; c0 += NT) {
-    const uint32_t p = c0 + tid;
-    int4 h = make_int4(0, 0, 0, 0);
-    uint32_t f = 0;
-    if (p < P) {
-      h = d.man_cur.hdr[W.pair_off + p];
-      f = h.w > 0 ? 1u : 0u;
-      color[p] = f ? kMAX : kMAX - 1;
-    }
-    uint32_t tot;
-    const uint32_t ex = t.scan(f, tot);
-    if (f) {
-      const uint32_t ga = off + h.x, gb = off + h.y;
-      const uint32_t sa = (d.cur.flags[ga] & BB_BODY_DYNAMIC) ? 0u : 0x80000000u;
-      const uint32_t sb = (d.cur.flags[gb] & BB_BODY_DYNAMIC) ? 0u : 0x80000000u;
-      const uint32_t ida = d.cur.id[ga], idb = d.cur.id[gb];
-      const uint32_t lo = ida < idb ? ida : idb, hi = ida < idb ? idb : ida;
-      const uint32_t pid = gc_hash(lo * 0x9e3779b9u + hi * 0x85ebca6bu + (uint32_t)h.z * 0xc2b2ae35u);
-      rec[L.carry + ex] = make_int4((int)(ga | sa), (int)(gb | sb), (int)pid, (int)p);
-    }
-    t.sync();
-    if (tid == 0) L.carry += tot;
-    t.sync();
-  }
   for (uint32_t round = 1; round <= kColors; ++round) {
     const uint32_t npend = L.carry;
     if (npend == 0) break;

@@ -214,7 +214,7 @@ def plain_world(steps: int, warmup: int):
     return len(bodies) * steps / dt, dt / steps * 1e3
 
 
-def single_world(name: str, desc, bodies, steps: int, warmup: int, profile: int) -> dict:
+def single_world(name: str, desc, bodies, steps: int, warmup: int, profile: int, tune: bool = True) -> dict:
     """One large world stepped alone (hipGraph replay of `steps` steps after
     `warmup`), then `profile` eager steps timed per stage; worlds at or above
     the team threshold run as workgroup teams (k_team, whose time the profile
@@ -225,6 +225,12 @@ def single_world(name: str, desc, bodies, steps: int, warmup: int, profile: int)
     b.step(warmup, use_graph=True)
     b.sync(allow_overflow=True)
     build_s = time.time() - t0
+    if tune:
+        # the tuner replays the next step per candidate (the sort's sub-team
+        # and the team launch's size for a team world) and leaves the state
+        b.autotune(2)
+    params = {n: b.get_param(getattr(beatbox, "PARAM_" + n.upper()))
+              for n in ("sort_blocks", "team_blocks", "block", "avbd_lanes")}
     t0 = time.perf_counter()
     b.step(steps, use_graph=True)
     b.sync(allow_overflow=True)
@@ -244,7 +250,7 @@ def single_world(name: str, desc, bodies, steps: int, warmup: int, profile: int)
                stages_ms={n: round(float(km[i]) / profile, 3) for i, n in enumerate(beatbox.KERNEL_NAMES)},
                manifolds=s["collision_count"], contacts=s["contact_count"], pairs=s["broad_phase_collision_count"],
                max_penetration_mm=int(cnt[:, beatbox.COUNT_NAMES.index("max_pen_mm")].max()),
-               overflow=s["overflow"], build_parts=build)
+               overflow=s["overflow"], build_parts=build, launch_params=params)
     b.destroy()
     log(f"{name}: {out['ms_per_step']:.2f} ms/step, {out['body_steps_per_s'] / 1e6:.2f} M body-steps/s "
         f"(setup {build_s:.1f}s), {out}")
@@ -531,12 +537,14 @@ def main() -> int:
                                                           morton_bits=63, quirks=0))
         extra["large_pile"] = dict(workload="100,000 bodies (50/50 boxes/spheres, 0.25-1 m) falling in a "
                                             "60x200x60 m column, AVBD 10 iterations, 63-bit keys",
-                                   **single_world("large pile", avbd, scenes.large_pile(), 300, 1, 5))
+                                   # not tuned: its first steps are free fall, unlike the rest
+                                   **single_world("large pile", avbd, scenes.large_pile(), 300, 1, 5, tune=False))
     if "sphere_gas" in runs:
         gas = beatbox.WorldDesc.make(**scenes.world_desc(iterations=8, morton_bits=63, quirks=0, gravity=0.0))
         extra["sphere_gas"] = dict(workload="1,000,000 spheres (r 0.1-0.5 m) in a 1000 m cube, v ~ N(0, 2 m/s), "
                                             "no gravity, PGS_SOFT 8 iterations, 63-bit keys",
-                                   **single_world("sphere gas", gas, scenes.sphere_gas(), 100, 1, 5))
+                                   **single_world("sphere gas", gas, scenes.sphere_gas(), 100, 1, 5,
+                                                  tune=not args.no_tune))
     if "serving" in runs:
         extra["serving"] = serving(args.serving_seconds, args.serving_budget_gib, args.serving_cap,
                                    args.serving_team_bodies, args.serving_split_bodies)
