@@ -171,7 +171,7 @@ TAP_ISLANDS, TAP_EDGE_COLORS, TAP_BODY_COLORS, TAP_ROWS = 5, 6, 7, 8
 ORDER_BY_ID, ORDER_BY_ROW = 0, 1
 KERNEL_NAMES = ("k_build", "k_broad", "k_narrow", "k_post", "k_pgs", "k_avbd", "k_finish")
 KERNEL_COUNT = len(KERNEL_NAMES)
-BUILD_NAMES = ("keys", "sort", "hierarchy", "bounds", "reorder")
+BUILD_NAMES = ("keys", "sort", "hierarchy", "bounds", "climb", "reorder")
 BUILD_COUNT = len(BUILD_NAMES)
 COUNT_NAMES = ("bodies", "active", "pairs", "manifolds", "contacts", "color_work", "max_pen_mm", "sleeping")
 COUNT_FIELDS = len(COUNT_NAMES)

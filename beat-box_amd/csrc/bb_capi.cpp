@@ -873,7 +873,7 @@ bb_status profile_batch(bb_batch* b, uint32_t nsteps, float* step_ms, float* ker
       for (uint32_t k = 0; k + 1 < kTeamStageMarks; ++k) kernel_ms[k] += (float)(m[k + 1] - m[k]) * ms_per_tick;
       // inside the build stage (a world without bodies leaves these unset)
       const unsigned long long sub[BB_BUILD_COUNT + 1] = {m[0], m[kMarkKeys], m[kMarkSort], m[kMarkHierarchy],
-                                                          m[kMarkBounds], m[1]};
+                                                          m[kMarkBounds], m[kMarkClimb], m[1]};
       bool ordered = true;
       for (int k = 0; k < BB_BUILD_COUNT; ++k) ordered = ordered && sub[k + 1] >= sub[k];
       if (ordered)

@@ -92,11 +92,12 @@ static constexpr uint32_t kMaxTeams = 256, kTeamSharedBytes = 4096;
 // build, broad, narrow (with contacts), post, PGS (with levels), AVBD and
 // finish stages (marks 0-7, kTeamStageMarks); then marks inside the build
 // stage: Morton keys done, sort done, LBVH hierarchy done, the teamwide part
-// of the bounds pass done (the reorder and the lead workgroup's climb of the
-// top nodes follow, up to mark 1).
+// of the bounds pass done, the lead workgroup's climb of the top nodes done
+// (stamped by rank 0, a member of the lead workgroup; the reorder follows,
+// up to mark 1).
 static constexpr uint32_t kTeamStageMarks = 8;
-static constexpr uint32_t kMarkKeys = 8, kMarkSort = 9, kMarkHierarchy = 10, kMarkBounds = 11;
-static constexpr uint32_t kTeamMarks = 12;
+static constexpr uint32_t kMarkKeys = 8, kMarkSort = 9, kMarkHierarchy = 10, kMarkBounds = 11, kMarkClimb = 12;
+static constexpr uint32_t kTeamMarks = 13;
 struct TeamTable {
   uint32_t count;
   uint32_t first[kMaxTeams + 1];

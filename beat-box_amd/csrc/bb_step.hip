@@ -274,6 +274,7 @@ struct BlockTeam {
   }
   // clock marks inside a stage are taken by workgroup teams only
   BB_DI void mark(uint32_t) const {}
+  BB_DI void mark_lead(uint32_t) const {}
   BB_DI bool sorted_in_a(bool in_a) const { return in_a; }
 };
 
@@ -314,6 +315,13 @@ struct GroupTeam {
   // workgroup has reached that point when rank 0 stamps it.
   BB_DI void mark(uint32_t m) const {
     if (clk && rank() == 0) clk[m] = wall_clock64();
+  }
+  // Clock mark m once the lead workgroup (rank 0's) is past a point of its
+  // own work (a workgroup barrier, not a team one; profiling launches only).
+  BB_DI void mark_lead(uint32_t m) const {
+    if (!clk) return;  // workgroup-uniform
+    __syncthreads();
+    if (rank() == 0) clk[m] = wall_clock64();
   }
   // With leaf i handled by rank i mod size(), a tree node over leaves
   // [first, last] inside one BLOCK-aligned run is touched by one workgroup
@@ -571,8 +579,13 @@ BB_DI int lb_split(const KeyView& K, int first, int last) {
 // ================================================================ k_build
 struct BuildShared {
   uint32_t lo[3], hi[3];
-  uint32_t nroots;  // bounds pass: nodes handed from the runs to the lead workgroup
+  uint32_t nroots;   // bounds pass: nodes handed up from the 512-leaf runs
+  uint32_t nroots2;  // and from the spans of kClimbRuns runs (teams)
 };
+// Leaf runs per span of the bounds pass's middle level (teams): the nodes
+// above the runs but inside one span are climbed by one workgroup each, so
+// the lead workgroup only climbs what lies above the spans.
+static constexpr uint32_t kClimbRuns = 16;
 
 template <class T>
 BB_DI void s_build(const Dev& d, uint32_t w, const T& t, BuildShared& L) {
@@ -683,7 +696,10 @@ BB_DI void s_build(const Dev& d, uint32_t w, const T& t, BuildShared& L) {
     d.visit[nb + i] = 0;
     kb[i] = (uint64_t)(uint32_t)first | ((uint64_t)(uint32_t)last << 32);  // leaf range (the sort is done with kb)
   }
-  if (tid == 0) L.nroots = 0;
+  if (tid == 0) {
+    L.nroots = 0;
+    L.nroots2 = 0;
+  }
   t.sync();
   t.mark(kMarkHierarchy);
   // ---- bounds, bottom-up: the second child to arrive merges (exact min/max
@@ -722,10 +738,42 @@ BB_DI void s_build(const Dev& d, uint32_t w, const T& t, BuildShared& L) {
     }
     t.sync();
     t.mark(kMarkBounds);
-    if (t.lead_group()) {
-      const uint32_t nr = L.nroots < N ? L.nroots : N;
-      for (uint32_t k = threadIdx.x; k < nr; k += blockDim.x) {
+    // the nodes handed up from the runs: a team climbs those inside a span of
+    // kClimbRuns runs in the span's own workgroup (span s: workgroup s mod
+    // team size; no other workgroup touches them in this phase, so
+    // workgroup-scope ordering suffices again) and hands the first node above
+    // each span on (pcount, free until the broad phase); after a team barrier
+    // the lead workgroup climbs the rest.  A one-workgroup world hands
+    // nothing up.
+    const uint32_t* top = ib;
+    uint32_t ntop = L.nroots < N ? L.nroots : N;
+    if constexpr (T::kGlobalScalars) {
+      const uint32_t span = kClimbRuns * (uint32_t)T::kBlock;
+      uint32_t* ib2 = d.pcount + off;
+      for (uint32_t k = threadIdx.x; k < ntop; k += blockDim.x) {
         uint32_t p = ib[k];
+        if (((uint32_t)kb[p] / span) % t.groups() != t.group()) continue;
+        while (p + 1 < N) {
+          const uint64_t rg = kb[p];
+          if ((uint32_t)rg / span != (uint32_t)(rg >> 32) / span) {
+            const uint32_t k2 = atomicAdd(&L.nroots2, 1u);
+            if (k2 < N) ib2[k2] = p;
+            break;
+          }
+          __threadfence_block();
+          if (atomicAdd(&d.visit[nb + p], 1u) == 0u) break;
+          __threadfence_block();
+          if (!merge(p) || p == 0) break;
+          p = d.parent[nb + p];
+        }
+      }
+      t.sync();
+      top = ib2;
+      ntop = L.nroots2 < N ? L.nroots2 : N;
+    }
+    if (t.lead_group()) {
+      for (uint32_t k = threadIdx.x; k < ntop; k += blockDim.x) {
+        uint32_t p = top[k];
         while (p + 1 < N) {
           __threadfence_block();
           if (atomicAdd(&d.visit[nb + p], 1u) == 0u) break;
@@ -734,6 +782,7 @@ BB_DI void s_build(const Dev& d, uint32_t w, const T& t, BuildShared& L) {
           p = d.parent[nb + p];
         }
       }
+      t.mark_lead(kMarkClimb);
     }
   }
   // ---- reorder: rows[i] = prev rows[sorted index] (RB_sim.slang:351-386)

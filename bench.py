@@ -77,15 +77,17 @@ def build_bytes(n: int, kw: int) -> dict:
     reads each leaf's index, id, pose and shape bounds and writes its node,
     and per internal node reads two keys and writes children, parents, the
     visit flag and the leaf range; the bounds pass reads a node and its two
-    children and writes it; the reorder moves every row (id, flags, sleep
-    timer and five float4 rows) and writes the id -> row map and two
-    per-row lookups."""
+    children and writes it (the lead workgroup's climb does the same for the
+    about 2n/512 nodes above the 512-leaf runs); the reorder moves every row
+    (id, flags, sleep timer and five float4 rows) and writes the id -> row
+    map and two per-row lookups."""
     passes = 8 if kw == 8 else 4
     return {
         "keys": n * ((32 if kw == 8 else 16) + kw + 4),
         "sort": passes * n * (3 * kw + 8),
         "hierarchy": n * 104 + (n - 1) * 68,
         "bounds": (n - 1) * 136,
+        "climb": (2 * n // 512) * 136,
         "reorder": n * 212,
     }
 

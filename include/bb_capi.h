@@ -489,15 +489,16 @@ bb_status bb_batch_profile(bb_batch* b, uint32_t nsteps, float* step_ms, float* 
 /* Parts of BB_KERNEL_BUILD (bb_batch_build_split indices): the Morton keys
  * (RB_sim.slang:75-111), the LSD radix sort (LBVH.slang:34-152), the Karras
  * hierarchy (RB_sim.slang:206-278), the bounds pass up to its last team-wide
- * barrier (RB_sim.slang:282-349), then the reorder (RB_sim.slang:351-386)
- * with the lead workgroup's climb of the top nodes. */
+ * barrier (RB_sim.slang:282-349), the lead workgroup's climb of the top
+ * nodes above it, then the reorder (RB_sim.slang:351-386). */
 enum {
   BB_BUILD_KEYS = 0,
   BB_BUILD_SORT = 1,
   BB_BUILD_HIERARCHY = 2,
   BB_BUILD_BOUNDS = 3,
-  BB_BUILD_REORDER = 4,
-  BB_BUILD_COUNT = 5
+  BB_BUILD_CLIMB = 4,   /* the lead workgroup's climb of the top nodes */
+  BB_BUILD_REORDER = 5,
+  BB_BUILD_COUNT = 6
 };
 /* Measurement helper (no reference counterpart): the build stage of the
  * slowest workgroup team, split into the parts above by the team's own clock

@@ -165,6 +165,12 @@ def test_profile_splits_team_time_by_stage():
             total = float(step_ms.sum())
             assert abs(float(kern_ms.sum()) - total) < 0.05 * total, (kern_ms, total)
             assert kern_ms[0] < 0.6 * total and kern_ms[5] > kern_ms[0], kern_ms
+            # the build stage's parts by the team's marks inside it: every part
+            # gets time (the lead workgroup's climb too) and they add up to
+            # the build stage
+            parts = b.build_split()
+            assert parts.shape == (beatbox.BUILD_COUNT,) and np.all(parts > 0), parts
+            assert abs(float(parts.sum()) - float(kern_ms[0])) < 0.05 * float(kern_ms[0]), (parts, kern_ms)
         else:
             b.step(4)
         b.sync()
