@@ -166,6 +166,16 @@ struct SortLds {
   uint32_t wcnt[(BLOCK / WAVE) * 256];
   ScanLds<BLOCK> scan;
 };
+// The team sort's scatter takes kSortTile keys per thread per tile (sub-tile
+// j holds keys [j*BLOCK, (j+1)*BLOCK) of the tile): per (sub-tile, wave,
+// digit) counts, zero between tiles, and the offsets made from them.
+static constexpr uint32_t kSortTile = 4;
+template <int BLOCK>
+struct TeamSortLds {
+  SortLds<BLOCK> s;
+  uint32_t cnt[kSortTile * (BLOCK / WAVE) * 256];
+  uint32_t off[kSortTile * (BLOCK / WAVE) * 256];
+};
 
 template <int BLOCK>
 BB_DI bool block_radix_sort(uint64_t* ka, uint32_t* ia, uint64_t* kb, uint32_t* ib, uint32_t n, int passes,
@@ -289,7 +299,7 @@ static constexpr uint32_t kTeamSpinLimit = 1u << 24;
 template <int BLOCK>
 struct GroupTeam {
   ScanLds<BLOCK>* scan_lds;
-  SortLds<BLOCK>* sort_lds;
+  TeamSortLds<BLOCK>* sort_lds;
   TeamCtl* ctl;
   uint32_t* hist;  // [nblocks * 256] this team's radix histograms / offsets
   uint32_t blk, nblocks;
@@ -438,7 +448,9 @@ struct GroupTeam {
   BB_DI bool radix_sort(uint64_t* ka, uint32_t* ia, uint64_t* kb, uint32_t* ib, uint32_t n, int passes) const {
     const uint32_t S = sort_blocks < 1u ? 1u : (sort_blocks < nblocks ? sort_blocks : nblocks);
     if (blk >= S) return true;  // not a member: the result comes through ctl->sorted
-    SortLds<BLOCK>& L = *sort_lds;
+    SortLds<BLOCK>& L = sort_lds->s;
+    TeamSortLds<BLOCK>& X = *sort_lds;
+    constexpr uint32_t NW = BLOCK / WAVE, K = kSortTile;
     const uint32_t tid = threadIdx.x, lane = tid & (WAVE - 1), wave = tid / WAVE;
     const uint32_t chunk = (n + S - 1) / S;
     const uint32_t lo = blk * chunk < n ? blk * chunk : n;
@@ -447,7 +459,7 @@ struct GroupTeam {
       if (S == nblocks) sync();
       else sync_first(S);
     };
-    for (uint32_t i = tid; i < (BLOCK / WAVE) * 256; i += BLOCK) L.wcnt[i] = 0;
+    for (uint32_t i = tid; i < K * NW * 256; i += BLOCK) X.cnt[i] = 0;
     bool in_a = true;
     for (int pass = 0; pass < passes; ++pass) {
       const uint32_t shift = 8u * pass;
@@ -460,10 +472,20 @@ struct GroupTeam {
       bar();
       const uint32_t d0 = n ? (uint32_t)(kin[0] >> shift) & 255u : 0u;
       bool differs = false;
-      for (uint32_t i = lo + tid; i < hi; i += BLOCK) {
-        const uint32_t dg = (uint32_t)(kin[i] >> shift) & 255u;
-        atomicAdd(&L.base[dg], 1u);
-        differs |= dg != d0;
+      // histogram, K keys per thread in flight
+      for (uint32_t c0 = lo; c0 < hi; c0 += K * BLOCK) {
+        uint32_t dgs[K];
+#pragma unroll
+        for (uint32_t j = 0; j < K; ++j) {
+          const uint32_t i = c0 + j * BLOCK + tid;
+          dgs[j] = i < hi ? (uint32_t)(kin[i] >> shift) & 255u : 256u;
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < K; ++j)
+          if (dgs[j] < 256u) {
+            atomicAdd(&L.base[dgs[j]], 1u);
+            differs |= dgs[j] != d0;
+          }
       }
       if (differs) atomicOr(&ctl->diff, 1u);
       __syncthreads();
@@ -483,37 +505,55 @@ struct GroupTeam {
       bar();
       for (uint32_t i = tid; i < 256; i += BLOCK) L.base[i] = hist[i * S + blk];
       __syncthreads();
-      for (uint32_t c0 = lo; c0 < hi; c0 += BLOCK) {
-        const uint32_t i = c0 + tid;
-        const bool valid = i < hi;
-        const uint64_t key = valid ? kin[i] : 0ull;
-        const uint32_t idx = valid ? iin[i] : 0u;
-        const uint32_t dg = (uint32_t)(key >> shift) & 255u;
-        uint64_t peers = __ballot(valid);
+      // scatter, a tile of K * BLOCK keys at a time: every key's rank among
+      // its wave's peers by ballots, the (sub-tile, wave, digit) counts, one
+      // walk per digit over them in key order for the offsets, then the
+      // writes (two workgroup barriers per tile)
+      for (uint32_t c0 = lo; c0 < hi; c0 += K * BLOCK) {
+        uint64_t key[K];
+        uint32_t idx[K], dg[K], rk[K];
+        bool valid[K];
 #pragma unroll
-        for (int b = 0; b < 8; ++b) {
-          const uint64_t m = __ballot((dg >> b) & 1u);
-          peers &= ((dg >> b) & 1u) ? m : ~m;
+        for (uint32_t j = 0; j < K; ++j) {
+          const uint32_t i = c0 + j * BLOCK + tid;
+          valid[j] = i < hi;
+          key[j] = valid[j] ? kin[i] : 0ull;
+          idx[j] = valid[j] ? iin[i] : 0u;
+          dg[j] = (uint32_t)(key[j] >> shift) & 255u;
         }
-        const uint32_t rk = __popcll(peers & ((1ull << lane) - 1ull));
-        const uint32_t cnt = __popcll(peers);
-        const bool leader = valid && lane == (uint32_t)(__ffsll((unsigned long long)peers) - 1);
-        if (leader) L.wcnt[wave * 256 + dg] = cnt;
-        __syncthreads();
-        if (valid) {
-          uint32_t o = L.base[dg] + rk;
-          for (uint32_t w2 = 0; w2 < wave; ++w2) o += L.wcnt[w2 * 256 + dg];
-          if (o < n) {  // always true unless a barrier wait ran out
-            kout[o] = key;
-            iout[o] = idx;
+#pragma unroll
+        for (uint32_t j = 0; j < K; ++j) {
+          uint64_t peers = __ballot(valid[j]);
+#pragma unroll
+          for (int b = 0; b < 8; ++b) {
+            const uint64_t m = __ballot((dg[j] >> b) & 1u);
+            peers &= ((dg[j] >> b) & 1u) ? m : ~m;
           }
+          rk[j] = __popcll(peers & ((1ull << lane) - 1ull));
+          if (valid[j] && lane == (uint32_t)(__ffsll((unsigned long long)peers) - 1))
+            X.cnt[(j * NW + wave) * 256 + dg[j]] = __popcll(peers);
         }
         __syncthreads();
-        if (leader) {
-          atomicAdd(&L.base[dg], cnt);
-          L.wcnt[wave * 256 + dg] = 0;
+        for (uint32_t dd = tid; dd < 256; dd += BLOCK) {
+          uint32_t run = L.base[dd];
+          for (uint32_t s2 = 0; s2 < K * NW; ++s2) {
+            const uint32_t c = X.cnt[s2 * 256 + dd];
+            X.off[s2 * 256 + dd] = run;
+            X.cnt[s2 * 256 + dd] = 0;
+            run += c;
+          }
+          L.base[dd] = run;
         }
         __syncthreads();
+#pragma unroll
+        for (uint32_t j = 0; j < K; ++j)
+          if (valid[j]) {
+            const uint32_t o = X.off[(j * NW + wave) * 256 + dg[j]] + rk[j];
+            if (o < n) {  // always true unless a barrier wait ran out
+              kout[o] = key[j];
+              iout[o] = idx[j];
+            }
+          }
       }
       bar();
       in_a = !in_a;
@@ -3951,7 +3991,7 @@ BB_DI void s_avbd(const Dev& d, uint32_t w, const T& t, AvbdShared& L) {
   // it does not change the sweeps.
   uint32_t* lc = nullptr;
   if constexpr (T::kGlobalScalars) {
-    lc = t.sort_lds->wcnt;
+    lc = t.sort_lds->s.wcnt;
     for (uint32_t k = threadIdx.x; k < 2 * AV_NB; k += blockDim.x) lc[k] = 0;
   }
   for (uint32_t k = tid; k < AV_NB; k += NT) L.bcount[k] = 0;
@@ -4137,7 +4177,7 @@ static_assert(sizeof(TeamShared) <= kTeamSharedBytes, "team scalars exceed their
 // phase's shared walks after it
 template <int BLOCK>
 union TeamLds {
-  SortLds<BLOCK> sort;
+  TeamSortLds<BLOCK> sort;
   WalkLds<BLOCK> walk;
 };
 
