@@ -2648,12 +2648,16 @@ BB_DI void s_post(const Dev& d, uint32_t w, const T& t, PostShared& L) {
   // ---- edge colouring (coloring.slang:44-292).  Phase 2 commits at the
   // colour picked in phase 1; the owner cells touched in a round are cleared
   // by a third phase so the table is zero between rounds.  Only uncoloured
-  // manifolds are visited: they sit in a pending list of records (the solver
-  // row headers are free until the rows are built), {a row | static << 31,
-  // b row | static << 31, pair hash, slot}, compacted after every round.
+  // manifolds are visited: they sit in a pending list of records, {a row |
+  // static << 31, b row | static << 31, pair hash, slot}, compacted after
+  // every round.
   uint32_t* color = d.color + W.pair_off;
   uint32_t* pick = d.pick + W.pair_off;
-  int4* rec = d.sv_hdr + W.pair_off;
+  // pending records in two buffers, taken in turn (the solver rows' headers
+  // and normals, both free until the rows are built)
+  int4* const rec_a = d.sv_hdr + W.pair_off;
+  int4* const rec_b = reinterpret_cast<int4*>(d.sv_n + W.pair_off);
+  int4* rec = rec_a;
   for (uint32_t i = tid; i < N; i += NT) d.mask[off + i] = 0;
   const uint32_t npend0 = scan_range(
       t, P, [&](uint32_t p) { return d.man_cur.hdr[W.pair_off + p].w > 0 ? 1u : 0u; },
@@ -2721,23 +2725,19 @@ BB_DI void s_post(const Dev& d, uint32_t w, const T& t, PostShared& L) {
     if (tid == 0) L.carry = 0;
     t.sync();
     if (!any) break;
-    // keep the still uncoloured records (in place: every record is read
-    // before the scan that orders the writes, and moves only to lower slots)
-    for (uint32_t c0 = 0; c0 < npend; c0 += NT) {
-      const uint32_t i = c0 + tid;
-      int4 r = make_int4(0, 0, 0, 0);
-      uint32_t f = 0;
-      if (i < npend) {
-        r = rec[i];
-        f = color[(uint32_t)r.w] == kMAX ? 1u : 0u;
-      }
-      uint32_t tot;
-      const uint32_t ex = t.scan(f, tot);
-      if (f) rec[L.carry + ex] = r;
-      t.sync();
-      if (tid == 0) L.carry += tot;
-      t.sync();
-    }
+    // keep the still uncoloured records, in order, in the other buffer (one
+    // scan over the range; a second walk over one buffer would overwrite
+    // records not yet read)
+    const int4* cur = rec;
+    int4* nxt = rec == rec_a ? rec_b : rec_a;
+    const uint32_t kept = scan_range(t, npend, [&](uint32_t i) { return color[(uint32_t)cur[i].w] == kMAX ? 1u : 0u; },
+                                     [&](uint32_t i, uint32_t ex) {
+                                       const int4 r = cur[i];
+                                       if (color[(uint32_t)r.w] == kMAX) nxt[ex] = r;
+                                     });
+    rec = nxt;
+    if (tid == 0) L.carry = kept;
+    t.sync();
   }
   // ---- validator (coloring.slang:214-292) and colour buckets
   if (tid < kBuckets) L.bcount[tid] = 0;
