@@ -173,6 +173,8 @@ KERNEL_NAMES = ("k_build", "k_broad", "k_narrow", "k_post", "k_pgs", "k_avbd", "
 KERNEL_COUNT = len(KERNEL_NAMES)
 BUILD_NAMES = ("keys", "sort", "hierarchy", "bounds", "climb", "reorder")
 BUILD_COUNT = len(BUILD_NAMES)
+AVBD_NAMES = ("colour", "group", "sweeps", "post")
+AVBD_COUNT = len(AVBD_NAMES)
 COUNT_NAMES = ("bodies", "active", "pairs", "manifolds", "contacts", "color_work", "max_pen_mm", "sleeping")
 COUNT_FIELDS = len(COUNT_NAMES)
 PARAM_BLOCK, PARAM_PGS_LDS_ROWS, PARAM_TEAM_BODIES, PARAM_LANE_BODIES, PARAM_LANE_BLOCK, PARAM_LANE_LDS_ROWS = \
@@ -246,6 +248,7 @@ EXPORTS = {
     "bb_batch_profile": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]),
     "bb_batch_energy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32]),
     "bb_batch_build_split": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "bb_batch_avbd_split": (C.c_int, [C.c_void_p, C.c_void_p]),
     "bb_world_energy": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
     "bb_world_instance_transforms": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_int]),
     "bb_world_lbvh_nodes": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32), C.c_int]),
@@ -539,6 +542,14 @@ class Batch:
         call's steps (zeros without teams)."""
         out = np.zeros(BUILD_COUNT, np.float32)
         _check(lib().bb_batch_build_split(self._h, out.ctypes.data_as(C.c_void_p)), "bb_batch_build_split")
+        return out
+
+    def avbd_split(self) -> np.ndarray:
+        """bb_batch_avbd_split: ms of the AVBD stage's parts (AVBD_NAMES)
+        inside the slowest workgroup team, summed over the last profile()
+        call's steps (zeros without an AVBD team)."""
+        out = np.zeros(AVBD_COUNT, np.float32)
+        _check(lib().bb_batch_avbd_split(self._h, out.ctypes.data_as(C.c_void_p)), "bb_batch_avbd_split")
         return out
 
     def energy(self) -> np.ndarray:

@@ -797,6 +797,7 @@ bb_status profile_batch(bb_batch* b, uint32_t nsteps, float* step_ms, float* ker
   if (kernel_ms)
     for (int k = 0; k < BB_KERNEL_COUNT; ++k) kernel_ms[k] = 0.0f;
   for (int k = 0; k < BB_BUILD_COUNT; ++k) b->build_ms[k] = 0.0f;
+  for (int k = 0; k < BB_AVBD_COUNT; ++k) b->avbd_ms[k] = 0.0f;
   unsigned long long* dcounts = nullptr;
   if (counts && nsteps &&
       hipMalloc(&dcounts, (size_t)nsteps * BB_COUNT_FIELDS * sizeof(unsigned long long)) != hipSuccess)
@@ -878,6 +879,13 @@ bb_status profile_batch(bb_batch* b, uint32_t nsteps, float* step_ms, float* ker
       for (int k = 0; k < BB_BUILD_COUNT; ++k) ordered = ordered && sub[k + 1] >= sub[k];
       if (ordered)
         for (int k = 0; k < BB_BUILD_COUNT; ++k) b->build_ms[k] += (float)(sub[k + 1] - sub[k]) * ms_per_tick;
+      // inside the AVBD stage (PGS worlds leave these unset)
+      const unsigned long long av[BB_AVBD_COUNT + 1] = {m[5], m[kMarkAvColour], m[kMarkAvGroup], m[kMarkAvSweeps],
+                                                         m[6]};
+      ordered = true;
+      for (int k = 0; k < BB_AVBD_COUNT; ++k) ordered = ordered && av[k + 1] >= av[k] && av[k] != 0;
+      if (ordered)
+        for (int k = 0; k < BB_AVBD_COUNT; ++k) b->avbd_ms[k] += (float)(av[k + 1] - av[k]) * ms_per_tick;
     }
   }
   if (dcounts) {
@@ -2018,6 +2026,12 @@ bb_status bb_batch_profile(bb_batch* b, uint32_t nsteps, float* step_ms, float* 
 bb_status bb_batch_build_split(const bb_batch* b, float* build_ms) {
   if (!b || !build_ms) return BB_ERR_INVALID;
   for (int k = 0; k < BB_BUILD_COUNT; ++k) build_ms[k] = b->build_ms[k];
+  return BB_OK;
+}
+
+bb_status bb_batch_avbd_split(const bb_batch* b, float* avbd_ms) {
+  if (!b || !avbd_ms) return BB_ERR_INVALID;
+  for (int k = 0; k < BB_AVBD_COUNT; ++k) avbd_ms[k] = b->avbd_ms[k];
   return BB_OK;
 }
 

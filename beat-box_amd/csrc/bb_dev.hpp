@@ -94,10 +94,14 @@ static constexpr uint32_t kMaxTeams = 256, kTeamSharedBytes = 4096;
 // stage: Morton keys done, sort done, LBVH hierarchy done, the teamwide part
 // of the bounds pass done, the lead workgroup's climb of the top nodes done
 // (stamped by rank 0, a member of the lead workgroup; the reorder follows,
-// up to mark 1).
+// up to mark 1); and inside the AVBD stage (from mark 5): adjacency,
+// colouring and prepare done, warm start, support depth and the (colour,
+// layer) grouping done, the main sweeps with their dual updates done (the
+// finalize and the post-stabilisation follow, up to mark 6).
 static constexpr uint32_t kTeamStageMarks = 8;
 static constexpr uint32_t kMarkKeys = 8, kMarkSort = 9, kMarkHierarchy = 10, kMarkBounds = 11, kMarkClimb = 12;
-static constexpr uint32_t kTeamMarks = 13;
+static constexpr uint32_t kMarkAvColour = 13, kMarkAvGroup = 14, kMarkAvSweeps = 15;
+static constexpr uint32_t kTeamMarks = 16;
 struct TeamTable {
   uint32_t count;
   uint32_t first[kMaxTeams + 1];

@@ -175,6 +175,7 @@ struct bb_batch {
   // bb_batch_build_split: the build stage of the slowest team, split by the
   // team's clock marks, summed over the last bb_batch_profile call's steps
   float build_ms[BB_BUILD_COUNT] = {};
+  float avbd_ms[BB_AVBD_COUNT] = {};
   // voxel shape pools shared by the worlds (host copies; the device arrays
   // are allocated at their maximum sizes and refreshed when vox_dirty)
   std::vector<bb::VoxShape> vox_shapes;

@@ -26,8 +26,9 @@
 namespace bb {
 
 #define WAVE 64
-// Lanes per body in the AVBD primal sweeps of the per-world kernels.
-static constexpr uint32_t kAvG = 4;
+// Lanes per body in the AVBD primal sweeps of the per-world kernels (the
+// default of BB_PARAM_AVBD_LANES) and of workgroup teams.
+static constexpr uint32_t kAvG = 4, kAvGTeam = 8;
 
 // ============================================================ block helpers
 template <int BLOCK>
@@ -309,10 +310,11 @@ struct GroupTeam {
   uint32_t spin;   // spins before a wait gives up
   unsigned long long* clk;  // this team's clock marks (profiling launches), or null
   uint32_t sort_blocks;     // the first sort_blocks workgroups run the radix sort
-  // AVBD primal: kAvG lanes per body; empty cascade buckets skip their
-  // barrier (the bucket offsets are team scalars every workgroup reads after
-  // the same barrier, so all of them skip the same buckets)
-  static constexpr uint32_t kAvGroup = kAvG;
+  // AVBD primal: kAvGTeam lanes per body (a pile's bodies have half a dozen
+  // manifolds, so eight lanes take about one each); empty cascade buckets
+  // skip their barrier (the bucket offsets are team scalars every workgroup
+  // reads after the same barrier, so all of them skip the same buckets)
+  static constexpr uint32_t kAvGroup = kAvGTeam;
   static constexpr bool kSkipEmpty = true;
   static constexpr bool kGlobalScalars = true;  // stage scalars in a global slot
   static constexpr int kBlock = BLOCK;
@@ -3917,6 +3919,7 @@ BB_DI void s_avbd(const Dev& d, uint32_t w, const T& t, AvbdShared& L) {
     d.cur.rot[g] = stq(qtl);
   }
   t.sync();
+  t.mark(kMarkAvColour);
   // ---- warm start (avbd.slang:389-473)
   {
     uint32_t sticks = 0;
@@ -4045,6 +4048,7 @@ BB_DI void s_avbd(const Dev& d, uint32_t w, const T& t, AvbdShared& L) {
     if (o < N) bb[o] = i;  // the buckets hold at most the world's N rows
   }
   t.sync();
+  t.mark(kMarkAvGroup);
   const uint32_t used = L.used;
   // ---- main sweeps: 10 x (colours, dual) (rigid_body_manager.cpp:1366-1373)
   for (uint32_t it = 0; it < AV_ITERS; ++it) {
@@ -4057,6 +4061,7 @@ BB_DI void s_avbd(const Dev& d, uint32_t w, const T& t, AvbdShared& L) {
       if (d.man_cur.hdr[W.pair_off + p].w > 0) av_dual(d, W, W.pair_off + p);
     t.sync();
   }
+  t.mark(kMarkAvSweeps);
   // ---- finalize (avbd.slang:130-202)
   {
     const float inv_dt = dt > 0.0f ? 1.0f / dt : 0.0f;

@@ -239,6 +239,7 @@ def single_world(name: str, desc, bodies, steps: int, warmup: int, profile: int,
     dt = time.perf_counter() - t0
     st, km, cnt = b.profile(profile, counts=True)
     split = b.build_split() / profile
+    av_split = b.avbd_split() / profile
     s = b.stats(0)
     team = len(bodies) >= b.get_param(beatbox.PARAM_TEAM_BODIES) > 0
     kw = 8 if desc.morton_bits == 63 else 4
@@ -252,7 +253,9 @@ def single_world(name: str, desc, bodies, steps: int, warmup: int, profile: int,
                stages_ms={n: round(float(km[i]) / profile, 3) for i, n in enumerate(beatbox.KERNEL_NAMES)},
                manifolds=s["collision_count"], contacts=s["contact_count"], pairs=s["broad_phase_collision_count"],
                max_penetration_mm=int(cnt[:, beatbox.COUNT_NAMES.index("max_pen_mm")].max()),
-               overflow=s["overflow"], build_parts=build, launch_params=params)
+               overflow=s["overflow"], build_parts=build, launch_params=params,
+               avbd_parts={n: round(float(av_split[i]), 3) for i, n in enumerate(beatbox.AVBD_NAMES)}
+               if team and av_split.sum() > 0 else None)
     b.destroy()
     log(f"{name}: {out['ms_per_step']:.2f} ms/step, {out['body_steps_per_s'] / 1e6:.2f} M body-steps/s "
         f"(setup {build_s:.1f}s), {out}")

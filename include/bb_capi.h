@@ -506,6 +506,14 @@ enum {
  * (build_ms[BB_BUILD_COUNT]; zeros when no world of the batch runs as a
  * team). */
 bb_status bb_batch_build_split(const bb_batch* b, float* build_ms);
+/* The AVBD stage of the slowest workgroup team, the same way (no reference
+ * counterpart): adjacency, colouring and prepare (avbd.slang:70-128,
+ * 812-981); warm start, support depth and the grouping by (colour, layer)
+ * (avbd.slang:389-473, 667-724); the main sweeps with their dual updates
+ * (avbd.slang:475-658, 726-808); the finalize and the post-stabilisation
+ * (avbd.slang:130-202).  avbd_ms[BB_AVBD_COUNT]. */
+enum { BB_AVBD_COLOUR = 0, BB_AVBD_GROUP = 1, BB_AVBD_SWEEPS = 2, BB_AVBD_POST = 3, BB_AVBD_COUNT = 4 };
+bb_status bb_batch_avbd_split(const bb_batch* b, float* avbd_ms);
 /* Mechanical energy of each world's current state, in double (no reference
  * counterpart): sum over dynamic bodies of m|v|^2/2 + w.I.w/2 - m*g*y with
  * the physical inertia of the shape.  per_world[n], n >= world count. */

@@ -171,6 +171,10 @@ def test_profile_splits_team_time_by_stage():
             parts = b.build_split()
             assert parts.shape == (beatbox.BUILD_COUNT,) and np.all(parts > 0), parts
             assert abs(float(parts.sum()) - float(kern_ms[0])) < 0.05 * float(kern_ms[0]), (parts, kern_ms)
+            # and the AVBD stage's parts
+            av = b.avbd_split()
+            assert av.shape == (beatbox.AVBD_COUNT,) and np.all(av > 0), av
+            assert abs(float(av.sum()) - float(kern_ms[5])) < 0.05 * float(kern_ms[5]), (av, kern_ms)
         else:
             b.step(4)
         b.sync()
