@@ -259,6 +259,8 @@ EXPORTS = {
     "bb_batch_instance_transforms": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_int]),
     "bb_batch_device_bytes": (C.c_uint64, [C.c_void_p]),
     "bb_batch_destroy": (None, [C.c_void_p]),
+    "bb_batch_snapshot": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.POINTER(C.c_size_t)]),
+    "bb_batch_restore": (C.c_int, [C.c_void_p, C.c_size_t, C.c_int, C.POINTER(C.c_void_p)]),
     "bb_server_create": (C.c_int, [C.POINTER(ServerDesc), C.POINTER(C.c_void_p)]),
     "bb_server_submit": (C.c_int, [C.c_void_p, C.POINTER(WorldDesc), C.c_void_p, C.c_uint32, C.c_uint32,
                                    C.POINTER(C.c_uint64)]),
@@ -611,6 +613,23 @@ class Batch:
     def tap(self, w: int, kind: int, dtype) -> np.ndarray:
         return _tap(lambda dst, cap, wr: lib().bb_batch_debug_tap(self._h, w, kind, dst, cap, wr),
                     dtype)
+
+    def snapshot(self) -> bytes:
+        """Checkpoint of every world of the batch (bb_batch_snapshot)."""
+        size = C.c_size_t(0)
+        _check(lib().bb_batch_snapshot(self._h, None, 0, C.byref(size)), "bb_batch_snapshot")
+        buf = C.create_string_buffer(size.value)
+        _check(lib().bb_batch_snapshot(self._h, buf, size.value, C.byref(size)), "bb_batch_snapshot")
+        return buf.raw[: size.value]
+
+    @classmethod
+    def restore(cls, blob: bytes, warm_start: bool = True) -> "Batch":
+        """A batch created from a checkpoint (bb_batch_restore)."""
+        b = cls.__new__(cls)
+        b._h = C.c_void_p()
+        _check(lib().bb_batch_restore(blob, len(blob), 1 if warm_start else 0, C.byref(b._h)),
+               "bb_batch_restore")
+        return b
 
     def destroy(self) -> None:
         if self._h:

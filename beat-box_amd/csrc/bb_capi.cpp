@@ -1017,6 +1017,48 @@ void world_state_arrays(bb_batch* b, uint32_t w, int parity, uint32_t n, uint32_
   fn((void*)(b->base.stats + (size_t)w * ST_COUNT), (size_t)ST_COUNT * 4);
 }
 
+// The voxel shapes of a batch, rebuilt on restore by the same
+// build_voxel_shape: per shape its SnapShape record, then its occupancy words.
+char* put_shapes(const bb_batch* b, char* o) {
+  for (size_t k = 0; k < b->vox_shapes.size(); ++k) {
+    const VoxShape& vs = b->vox_shapes[k];
+    const uint32_t cells = vs.dims[0] * vs.dims[1] * vs.dims[2];
+    SnapShape r{{vs.dims[0], vs.dims[1], vs.dims[2]}, vs.vs, b->vox_density[k], (cells + 31u) / 32u};
+    std::memcpy(o, &r, sizeof(r));
+    o += sizeof(r);
+    std::memcpy(o, b->vox_occ.data() + vs.occ_off, (size_t)r.words * 4);
+    o += (size_t)r.words * 4;
+  }
+  return o;
+}
+
+// Adds a blob's shape section (nshapes records, shape_words occupancy words
+// in all, within [in, end)) to b as shapes 1, 2, ... in blob order.  Returns
+// the position after the section, or null for a section that does not add up
+// or a shape that is refused.
+const char* take_shapes(bb_batch* b, const char* in, const char* end, uint32_t nshapes, uint32_t shape_words) {
+  if (nshapes > BB_MAX_VOXEL_SHAPES || shape_words > BB_MAX_VOXEL_OCC_WORDS) return nullptr;
+  uint32_t words_seen = 0;
+  for (uint32_t k = 0; k < nshapes; ++k) {
+    SnapShape r;
+    if ((size_t)(end - in) < sizeof(r)) return nullptr;
+    std::memcpy(&r, in, sizeof(r));
+    in += sizeof(r);
+    const uint64_t cells = (uint64_t)r.dims[0] * r.dims[1] * r.dims[2];
+    if (r.words > shape_words - words_seen || r.words != (cells + 31u) / 32u ||
+        (size_t)(end - in) < (size_t)r.words * 4)
+      return nullptr;
+    words_seen += r.words;
+    std::vector<uint32_t> occ(r.words);
+    std::memcpy(occ.data(), in, (size_t)r.words * 4);
+    in += (size_t)r.words * 4;
+    bb_voxel_shape_desc vd{{r.dims[0], r.dims[1], r.dims[2]}, r.voxel_size, r.density, occ.data()};
+    bb_voxel_shape_info info{};
+    if (add_voxel_shape(b, &vd, &info) != BB_OK || info.shape_id != k + 1) return nullptr;
+  }
+  return words_seen == shape_words ? in : nullptr;
+}
+
 bb_status snapshot_world(bb_batch* b, void* dst, size_t cap, size_t* size) {
   if (!size || b->worlds.size() != 1) return BB_ERR_INVALID;
   bb_status st = build(b);
@@ -1044,17 +1086,7 @@ bb_status snapshot_world(bb_batch* b, void* dst, size_t cap, size_t* size) {
   h.shape_words = shape_words;
   char* o = (char*)dst;
   std::memcpy(o, &h, sizeof(h));
-  o += sizeof(h);
-  // the voxel shapes, rebuilt on restore by the same build_voxel_shape
-  for (uint32_t k = 0; k < nshapes; ++k) {
-    const VoxShape& vs = b->vox_shapes[k];
-    const uint32_t cells = vs.dims[0] * vs.dims[1] * vs.dims[2];
-    SnapShape r{{vs.dims[0], vs.dims[1], vs.dims[2]}, vs.vs, b->vox_density[k], (cells + 31u) / 32u};
-    std::memcpy(o, &r, sizeof(r));
-    o += sizeof(r);
-    std::memcpy(o, b->vox_occ.data() + vs.occ_off, (size_t)r.words * 4);
-    o += (size_t)r.words * 4;
-  }
+  o = put_shapes(b, o + sizeof(h));
   std::memcpy(o, hw.bodies.data(), (size_t)n * sizeof(bb_body_desc));
   o += (size_t)n * sizeof(bb_body_desc);
   hipError_t rc = hipSuccess;
@@ -1117,23 +1149,8 @@ bb_status restore_world(const void* src, size_t size, int warm_start, bb_world**
     bb_world_destroy(w);
     return s;
   };
-  uint32_t words_seen = 0;
-  for (uint32_t k = 0; k < h.nshapes; ++k) {
-    SnapShape r;
-    std::memcpy(&r, in, sizeof(r));
-    in += sizeof(r);
-    words_seen += r.words;
-    const uint64_t cells = (uint64_t)r.dims[0] * r.dims[1] * r.dims[2];
-    if (words_seen > h.shape_words || r.words != (cells + 31u) / 32u) return fail(BB_ERR_INVALID);
-    std::vector<uint32_t> occ(r.words);
-    std::memcpy(occ.data(), in, (size_t)r.words * 4);
-    in += (size_t)r.words * 4;
-    bb_voxel_shape_desc vd{{r.dims[0], r.dims[1], r.dims[2]}, r.voxel_size, r.density, occ.data()};
-    bb_voxel_shape_info info{};
-    st = bb_world_add_voxel_shape(w, &vd, &info);
-    if (st != BB_OK || info.shape_id != k + 1) return fail(st != BB_OK ? st : BB_ERR_INVALID);
-  }
-  if (words_seen != h.shape_words) return fail(BB_ERR_INVALID);
+  in = take_shapes(w->b, in, (const char*)src + size, h.nshapes, h.shape_words);
+  if (!in) return fail(BB_ERR_INVALID);
   if (h.n) {
     st = bb_world_add_bodies(w, (const bb_body_desc*)in, h.n, nullptr);
     if (st != BB_OK) return fail(st);
@@ -1170,6 +1187,166 @@ bb_status restore_world(const void* src, size_t size, int warm_start, bb_world**
     hw.skip_ws_once = true;
   }
   *out = w;
+  return BB_OK;
+}
+
+// ---- batch checkpoints.  A batch's blob holds its voxel shapes and pair
+// scale, then one record per world in world order: the world's counts, warm
+// flags and pair scale, its world and body descriptors and its state arrays
+// (world_state_arrays, as in a world's blob).
+struct BatchSnapHeader {
+  char magic[8];
+  uint32_t version, nworlds, nshapes, shape_words, pair_scale, reserved;
+  uint64_t step_index;
+};
+struct BatchSnapWorld {
+  uint32_t n, npairs;
+  uint32_t warm;  // bit 0: the state warm starts the next step; bit 1: skip_ws_once
+  uint32_t pair_scale, no_team, reserved;
+  bb_world_desc desc;
+};
+constexpr char kBatchSnapMagic[8] = {'B', 'B', 'B', 'A', 'T', 'C', 'H', '1'};
+constexpr uint32_t kBatchSnapVersion = 1;
+constexpr uint32_t kMaxWorldPairScale = 1u << 20;
+
+bb_status snapshot_batch(bb_batch* b, void* dst, size_t cap, size_t* size) {
+  if (!size) return BB_ERR_INVALID;
+  bb_status st = build(b);
+  if (st != BB_OK) return st;
+  HIPCHK(hipStreamSynchronize(b->stream));
+  const uint32_t nw = (uint32_t)b->worlds.size();
+  std::vector<uint32_t> np(nw, 0);
+  if (b->stepped && nw) HIPCHK(hipMemcpy(np.data(), b->npairs[b->latest], (size_t)nw * 4, hipMemcpyDeviceToHost));
+  size_t total = sizeof(BatchSnapHeader) + b->vox_shapes.size() * sizeof(SnapShape) + b->vox_occ.size() * 4;
+  for (uint32_t w = 0; w < nw; ++w) {
+    const HostWorld& hw = b->worlds[w];
+    // warm-start state exists once a step has run and no one-shot drop is pending
+    if (!(b->stepped && !hw.drop_prev && !(b->infos[w].mode & kModeDropPrev))) np[w] = 0;
+    total += sizeof(BatchSnapWorld) + (size_t)hw.n * sizeof(bb_body_desc) + world_state_bytes(hw.n, np[w]);
+  }
+  *size = total;
+  if (!dst) return BB_OK;
+  if (cap < total) return BB_ERR_CAPACITY;
+  BatchSnapHeader h{};
+  std::memcpy(h.magic, kBatchSnapMagic, 8);
+  h.version = kBatchSnapVersion;
+  h.nworlds = nw;
+  h.nshapes = (uint32_t)b->vox_shapes.size();
+  h.shape_words = (uint32_t)b->vox_occ.size();
+  h.pair_scale = b->pair_scale;
+  h.step_index = b->step_index;
+  char* o = (char*)dst;
+  std::memcpy(o, &h, sizeof(h));
+  o = put_shapes(b, o + sizeof(h));
+  hipError_t rc = hipSuccess;
+  for (uint32_t w = 0; w < nw && rc == hipSuccess; ++w) {
+    const HostWorld& hw = b->worlds[w];
+    const bool warm = b->stepped && !hw.drop_prev && !(b->infos[w].mode & kModeDropPrev);
+    BatchSnapWorld r{};
+    r.n = hw.n;
+    r.npairs = np[w];
+    r.warm = (warm ? 1u : 0u) | (hw.skip_ws_once ? 2u : 0u);
+    r.pair_scale = hw.pair_scale;
+    r.no_team = hw.no_team ? 1u : 0u;
+    r.desc = hw.desc;
+    std::memcpy(o, &r, sizeof(r));
+    o += sizeof(r);
+    std::memcpy(o, hw.bodies.data(), (size_t)hw.n * sizeof(bb_body_desc));
+    o += (size_t)hw.n * sizeof(bb_body_desc);
+    world_state_arrays(b, w, b->latest, hw.n, np[w], [&](void* src, size_t bytes) {
+      if (rc == hipSuccess && bytes) rc = hipMemcpy(o, src, bytes, hipMemcpyDeviceToHost);
+      o += bytes;
+    });
+  }
+  return rc == hipSuccess ? BB_OK : BB_ERR_DEVICE;
+}
+
+// Every record of the blob is read and checked on the host (descriptors by
+// bb_batch_add_world, indices by valid_blob_indices, sizes against the blob's
+// length) before anything reaches the device.
+bb_status restore_batch(const void* src, size_t size, int warm_start, bb_batch** out) {
+  if (!src || !out || size < sizeof(BatchSnapHeader)) return BB_ERR_INVALID;
+  BatchSnapHeader h;
+  std::memcpy(&h, src, sizeof(h));
+  if (std::memcmp(h.magic, kBatchSnapMagic, 8) != 0 || h.version != kBatchSnapVersion) return BB_ERR_INVALID;
+  const char* in = (const char*)src + sizeof(h);
+  const char* end = (const char*)src + size;
+  // every world record takes at least its fixed part
+  if (h.nworlds > (size_t)(end - in) / sizeof(BatchSnapWorld)) return BB_ERR_INVALID;
+  bb_batch* b = nullptr;
+  bb_status st = bb_batch_create(h.nworlds, 0, &b);
+  if (st != BB_OK) return st;
+  auto fail = [&](bb_status s) {
+    bb_batch_destroy(b);
+    return s;
+  };
+  st = bb_batch_set_param(b, BB_PARAM_PAIR_SCALE, h.pair_scale);
+  if (st != BB_OK) return fail(BB_ERR_INVALID);
+  in = take_shapes(b, in, end, h.nshapes, h.shape_words);
+  if (!in) return fail(BB_ERR_INVALID);
+  struct Rec {
+    BatchSnapWorld r;
+    const char* state;
+  };
+  std::vector<Rec> recs(h.nworlds);
+  for (uint32_t w = 0; w < h.nworlds; ++w) {
+    BatchSnapWorld& r = recs[w].r;
+    if ((size_t)(end - in) < sizeof(r)) return fail(BB_ERR_INVALID);
+    std::memcpy(&r, in, sizeof(r));
+    in += sizeof(r);
+    // at most six slots per body pair (voxel pairs)
+    if (r.pair_scale == 0 || r.pair_scale > kMaxWorldPairScale || (r.pair_scale & (r.pair_scale - 1)) ||
+        r.no_team > 1 || r.warm > 3 || (uint64_t)r.npairs > 3ull * r.n * (r.n ? r.n - 1ull : 0ull) ||
+        (size_t)(end - in) / sizeof(bb_body_desc) < r.n)
+      return fail(BB_ERR_INVALID);
+    const char* bodies = in;
+    in += (size_t)r.n * sizeof(bb_body_desc);
+    if ((size_t)(end - in) < world_state_bytes(r.n, r.npairs) || !valid_blob_indices(in, r.n, r.npairs))
+      return fail(BB_ERR_INVALID);
+    recs[w].state = in;
+    in += world_state_bytes(r.n, r.npairs);
+    std::vector<bb_body_desc> bd(r.n);
+    if (r.n) std::memcpy(bd.data(), bodies, (size_t)r.n * sizeof(bb_body_desc));
+    st = bb_batch_add_world(b, &r.desc, bd.data(), r.n, nullptr);
+    if (st != BB_OK) return fail(st);
+    b->worlds[w].pair_scale = r.pair_scale;
+    b->worlds[w].no_team = r.no_team != 0;
+  }
+  if (in != end) return fail(BB_ERR_INVALID);
+  st = build(b);
+  if (st != BB_OK) return fail(st);
+  for (uint32_t w = 0; w < h.nworlds; ++w)
+    if (recs[w].r.npairs > b->worlds[w].pair_cap) return fail(BB_ERR_INVALID);
+  HIPCHK(hipStreamSynchronize(b->stream));
+  hipError_t rc = hipSuccess;
+  std::vector<uint32_t> np(h.nworlds, 0);
+  for (uint32_t w = 0; w < h.nworlds && rc == hipSuccess; ++w) {
+    const Rec& R = recs[w];
+    const char* at = R.state;
+    world_state_arrays(b, w, b->latest, R.r.n, R.r.npairs, [&](void* dst, size_t bytes) {
+      if (rc == hipSuccess && bytes) rc = hipMemcpy(dst, at, bytes, hipMemcpyHostToDevice);
+      at += bytes;
+    });
+    HostWorld& hw = b->worlds[w];
+    if (warm_start && (R.r.warm & 1u)) {
+      // the restored pairs and manifolds warm start the next step
+      np[w] = R.r.npairs;
+      b->infos[w].mode &= ~kModeDropPrev;
+      hw.skip_ws_once = (R.r.warm & 2u) != 0;
+    } else {
+      // like a resume after a pause: the next step starts without warm start
+      hw.skip_ws_once = true;
+    }
+  }
+  if (rc == hipSuccess && h.nworlds) {
+    rc = hipMemcpy(b->npairs[b->latest], np.data(), (size_t)h.nworlds * 4, hipMemcpyHostToDevice);
+    if (rc == hipSuccess)
+      rc = hipMemcpy(b->base.worlds, b->infos.data(), sizeof(WorldInfo) * h.nworlds, hipMemcpyHostToDevice);
+  }
+  if (rc != hipSuccess) return fail(BB_ERR_DEVICE);
+  b->step_index = h.step_index;
+  b->stepped = true;
+  *out = b;
   return BB_OK;
 }
 
@@ -2276,6 +2453,14 @@ bb_status bb_world_snapshot(bb_world* w, void* dst, size_t cap, size_t* size) {
 
 bb_status bb_world_restore(const void* src, size_t size, int warm_start, bb_world** out) {
   return restore_world(src, size, warm_start, out);
+}
+
+bb_status bb_batch_snapshot(bb_batch* b, void* dst, size_t cap, size_t* size) {
+  return b ? snapshot_batch(b, dst, cap, size) : BB_ERR_INVALID;
+}
+
+bb_status bb_batch_restore(const void* src, size_t size, int warm_start, bb_batch** out) {
+  return restore_batch(src, size, warm_start, out);
 }
 
 bb_status bb_world_debug_tap(bb_world* w, bb_tap kind, void* dst, size_t cap, size_t* written) {

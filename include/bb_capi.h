@@ -626,6 +626,23 @@ bb_status bb_world_set_param(bb_world* w, bb_param p, uint32_t value);
  * simulation state afterwards is exactly the state before the call.
  * *best_ms (may be NULL) receives the chosen step time. */
 bb_status bb_batch_autotune(bb_batch* b, uint32_t reps, float* best_ms);
+/* Checkpoint of a whole batch for a later bb_batch_restore: its voxel shapes
+ * and pair scale, then per world what bb_world_snapshot keeps (descriptors,
+ * newest rows, pairs, warm-start manifolds, counters) and its pair scale.
+ * The reference pauses one world in memory (src/status_manager.hpp:133-152,
+ * src/acceleration_structure_manager.cpp:813-948); no reference counterpart
+ * for a batch.  Launch parameters are not part of the blob.  With
+ * dst == NULL, *size receives the bytes needed; otherwise cap must be at least
+ * that (BB_ERR_CAPACITY).  Overflow marks not yet collected are not kept. */
+bb_status bb_batch_snapshot(bb_batch* b, void* dst, size_t cap, size_t* size);
+/* Creates a batch from a bb_batch_snapshot blob, its worlds in the same order.
+ * warm_start != 0: every world continues bit-identically to the batch that
+ * wrote the blob; warm_start == 0: each world's next step starts without warm
+ * start (src/status_manager.hpp:140-146).  Every record is checked on the
+ * host first; BB_ERR_INVALID for a blob that is not a batch checkpoint of
+ * this version. */
+bb_status bb_batch_restore(const void* src, size_t size, int warm_start,
+                           bb_batch** out);
 void bb_batch_destroy(bb_batch* b);
 
 /* ----------------------------------------------------------- serving loop */

@@ -140,11 +140,10 @@ def test_server_without_gpu_fails_loudly():
     assert e.value.code == 1  # BB_ERR_INVALID
 
 
-def _blob(n=3, p=1, mutate=None):
-    """A checkpoint blob of n boxes and p manifold slots in the layout of
-    bb_world_snapshot (header, body descriptors, then the arrays), with
+def _world_parts(n=3, p=1, mutate=None):
+    """The world descriptor, body descriptors and state arrays of a
+    checkpoint of n boxes and p manifold slots, in blob order, with
     consistent indices unless `mutate` changes them."""
-    import struct
     desc = beatbox.WorldDesc.make(**scenes.world_desc(iterations=8, morton_bits=63, quirks=0))
     bodies = scenes.random_boxes(0xBB90, n - 1, extent=2.0, y0=0.55, y1=3.0)
     assert len(bodies) == n
@@ -158,11 +157,30 @@ def _blob(n=3, p=1, mutate=None):
                   contacts=np.zeros(p * 8 * (6 * 16 + 4), np.uint8), stats=np.zeros(22, np.uint32))
     if mutate:
         mutate(arrays)
-    head = b"BBSNAP02" + struct.pack("<4IQ", 2, n, p, 1, 0) + bytes(desc) + struct.pack("<2I", 0, 0)
     body = np.ascontiguousarray(bodies, dtype=scenes.BODY_DESC).tobytes()
     order = ("id", "flags", "sleep", "entries", "pstart", "islot", "rows", "pairs", "geo", "hdr", "contacts",
              "stats")
-    return head + body + b"".join(arrays[k].tobytes() for k in order)
+    return bytes(desc), body, b"".join(arrays[k].tobytes() for k in order)
+
+
+def _blob(n=3, p=1, mutate=None):
+    """A checkpoint blob in the layout of bb_world_snapshot (header, body
+    descriptors, then the arrays)."""
+    import struct
+    desc, body, state = _world_parts(n, p, mutate)
+    return b"BBSNAP02" + struct.pack("<4IQ", 2, n, p, 1, 0) + desc + struct.pack("<2I", 0, 0) + body + state
+
+
+def _batch_blob(worlds, pair_scale=1, world_pair_scale=1):
+    """A blob in the layout of bb_batch_snapshot: header, no voxel shapes,
+    then per world its record (counts, warm flags, pair scale, descriptor),
+    body descriptors and arrays; worlds = [(n, p, mutate), ...]."""
+    import struct
+    out = b"BBBATCH1" + struct.pack("<6IQ", 1, len(worlds), 0, 0, pair_scale, 0, 7)
+    for n, p, mutate in worlds:
+        desc, body, state = _world_parts(n, p, mutate)
+        out += struct.pack("<6I", n, p, 1, world_pair_scale, 0, 0) + desc + body + state
+    return out
 
 
 def _restore_status(blob: bytes) -> int:
@@ -199,6 +217,44 @@ def test_restore_refuses_blobs_with_bad_indices(what):
     layout_size = 72 + 3 * 100 + 3 * 104 + 1 * 88 + 16 + 8 * (6 * 16 + 4) + 22 * 4
     assert len(blob) == layout_size
     rc = _restore_status(blob)
+    if what == "ok":
+        assert rc in (0, 4)  # BB_OK with a GPU, BB_ERR_NO_DEVICE without
+    else:
+        assert rc == 1, what
+
+
+@pytest.mark.parametrize("what", ["ok", "id_repeat", "hdr_row", "short", "long", "magic", "pair_scale",
+                                  "world_pair_scale", "count"])
+def test_batch_restore_refuses_bad_blobs_before_the_device(what):
+    """bb_batch_restore reads and checks every world record of a batch
+    checkpoint on the host: a consistent two-world blob gets as far as needing
+    the device (BB_ERR_NO_DEVICE here, no GPU); a bad index in the second
+    world, a blob one byte short or long, a wrong magic, a pair scale out of
+    range or a world count the blob cannot hold is refused with
+    BB_ERR_INVALID."""
+    import struct
+
+    def bad_ids(a):
+        a["id"][1] = a["id"][0]
+
+    def bad_hdr(a):
+        a["hdr"][0, 0] = 9
+    second = {"id_repeat": bad_ids, "hdr_row": bad_hdr}.get(what)
+    blob = _batch_blob([(3, 1, None), (4, 1, second)], pair_scale=17 if what == "pair_scale" else 1,
+                       world_pair_scale=3 if what == "world_pair_scale" else 1)
+    if what == "short":
+        blob = blob[:-1]
+    elif what == "long":
+        blob = blob + b"\0"
+    elif what == "magic":
+        blob = b"BBBATCH0" + blob[8:]
+    elif what == "count":
+        blob = blob[:12] + struct.pack("<I", 1 << 30) + blob[16:]
+    out = C.c_void_p()
+    buf = C.create_string_buffer(blob, len(blob))
+    rc = beatbox.lib().bb_batch_restore(buf, len(blob), 1, C.byref(out))
+    if rc == 0:
+        beatbox.lib().bb_batch_destroy(out)
     if what == "ok":
         assert rc in (0, 4)  # BB_OK with a GPU, BB_ERR_NO_DEVICE without
     else:

@@ -108,3 +108,74 @@ def test_restored_world_equals_the_oracle(k):
         bad = compare_step(b, ora, desc.morton_bits, 50)
         assert not bad, "\n".join(bad[:20])
         b.destroy()
+
+
+def _batch(team_bodies):
+    """The three worlds above and the reference's voxel scene under AVBD in
+    one batch; worlds of team_bodies bodies or more run as workgroup teams."""
+    shapes, vox = scenes.voxel_pile()
+    vdesc = beatbox.WorldDesc.make(**scenes.world_desc(solver=scenes.SOLVER_AVBD, iterations=10,
+                                                       morton_bits=63, quirks=0))
+    worlds = _worlds() + [(vdesc, vox)]
+    b = beatbox.Batch()
+    b.set_param(beatbox.PARAM_TEAM_BODIES, team_bodies)
+    for _, cells in shapes:
+        b.add_voxel_shape(cells, scenes.VOXEL_SIZE, scenes.VOXEL_DENSITY)
+    for desc, bodies in worlds:
+        b.add_world(desc, bodies)
+    return b, worlds, shapes
+
+
+def _batch_state(b, worlds):
+    b.sync()
+    return [(b.read_world(w, len(bodies)), b.stats(w)) for w, (_, bodies) in enumerate(worlds)]
+
+
+def _batch_same(x, y):
+    for w, ((xa, sa), (xb, sb)) in enumerate(zip(x, y)):
+        for f in FIELDS:
+            assert xa[f].tobytes() == xb[f].tobytes(), (w, f)
+        assert sa == sb, w
+
+
+def test_batch_restore_continues_bit_identically():
+    """A batch checkpointed at step 30 (bb_batch_snapshot) and restored
+    (bb_batch_restore) reads back as it was and continues, graph replay
+    included, bit for bit like the batch that wrote it: per world rows,
+    counters and step index, for PGS_SOFT and AVBD, a team world and voxel
+    shapes.  Restored cold, each world equals the world stepped alone that
+    skipped warm starting once at step 30; bad blobs are refused."""
+    a, worlds, shapes = _batch(team_bodies=900)
+    a.step(30, use_graph=True)
+    at_snap = _batch_state(a, worlds)
+    blob = a.snapshot()
+    a.step(20, use_graph=True)
+    after = _batch_state(a, worlds)
+    b = beatbox.Batch.restore(blob)
+    b.set_param(beatbox.PARAM_TEAM_BODIES, 900)
+    _batch_same(_batch_state(b, worlds), at_snap)
+    b.step(20, use_graph=True)
+    _batch_same(_batch_state(b, worlds), after)
+    assert all(s["contact_count"] > 0 for _, s in after)
+    assert at_snap[2][1]["step_index"] == 29  # the index of the last step taken
+    c = beatbox.Batch.restore(blob, warm_start=False)
+    c.step(20)
+    cold = _batch_state(c, worlds)
+    for w, (desc, bodies) in enumerate(worlds):
+        r = beatbox.RigidBodyManager(desc)
+        if w == 3:
+            for _, cells in shapes:
+                r.add_voxel_shape(cells, scenes.VOXEL_SIZE, scenes.VOXEL_DENSITY)
+        r.add_bodies(bodies)
+        r.simulate(30)
+        r.skip_warm_starting_once()
+        r.simulate(20)
+        x = r.read_bodies()
+        for f in ("position", "rotation", "velocity", "omega", "flags"):
+            assert x[f].tobytes() == cold[w][0][f].tobytes(), (w, f)
+        r.destroy()
+    for bad in (blob[:-1], blob + b"\0", b"x" * len(blob), blob[:16]):
+        with pytest.raises(beatbox.BBError):
+            beatbox.Batch.restore(bad)
+    for x in (a, b, c):
+        x.destroy()
