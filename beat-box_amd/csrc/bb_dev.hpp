@@ -72,13 +72,11 @@ static constexpr uint32_t kTeamBlock = 512;
 // sort_blocks of them), with a barrier of its own (scount, sgen); `sorted`
 // hands the side the sorted keys ended on to the rest of the team.
 struct TeamCtl {
-  uint32_t count;
-  uint32_t gen;
+  uint32_t count;   // team barrier arrivals in this launch
   uint32_t error;
   uint32_t diff;
   uint32_t exited;
-  uint32_t scount;
-  uint32_t sgen;
+  uint32_t scount;  // the sort sub-team's barrier arrivals in this launch
   uint32_t sorted;
   uint32_t part[kMaxTeamBlocks];
 };

@@ -304,7 +304,8 @@ struct GroupTeam {
   TeamCtl* ctl;
   uint32_t* hist;  // [nblocks * 256] this team's radix histograms / offsets
   uint32_t blk, nblocks;
-  uint32_t* dead;  // LDS word: thread 0's verdict of the last barrier
+  uint32_t* dead;  // LDS words: thread 0's verdict of the last barrier, then its
+                   // barrier counts on the team's and the sub-team's counter
   uint32_t* stats; // the world's counters (ST_COUNT words)
   uint32_t* ovf_any;
   uint32_t spin;   // spins before a wait gives up
@@ -369,41 +370,40 @@ struct GroupTeam {
     }
     __builtin_amdgcn_endpgm();
   }
-  // Barrier of `members` workgroups on the arrival counter *cnt and the
-  // generation word *gen.
-  BB_DI void barrier(uint32_t* cnt, uint32_t* gen, uint32_t members) const {
+  // Barrier of `members` workgroups on the arrival counter *cnt, which only
+  // grows during a launch (the last workgroup to leave resets it): this
+  // workgroup's e-th barrier on it is met once it reads e * members.  *epoch
+  // (LDS, thread 0's) counts this workgroup's barriers on the counter.  A
+  // member that finds the team's error word set while it waits, or that waits
+  // past the spin limit (and sets the word), leaves the step; one that met the
+  // barrier sees a failure at its next one.
+  BB_DI void barrier(uint32_t* cnt, uint32_t* epoch, uint32_t members) const {
     __syncthreads();
     if (threadIdx.x == 0) {
-      uint32_t bad = err();
-      if (bad == 0u) {
-        const uint32_t g = __hip_atomic_load(gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __threadfence();
-        if (atomicAdd(cnt, 1u) == members - 1u) {
-          __hip_atomic_store(cnt, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          __threadfence();
-          atomicAdd(gen, 1u);
-        } else {
-          uint32_t spins = 0;
-          while (__hip_atomic_load(gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == g) {
-            if (err() != 0u) break;
-            if (++spins > spin) {
-              atomicOr(&ctl->error, 1u);
-              break;
-            }
-            __builtin_amdgcn_s_sleep(1);
+      const uint32_t target = (*epoch += 1u) * members;
+      uint32_t bad = 0u;
+      __threadfence();
+      if (atomicAdd(cnt, 1u) + 1u != target) {
+        uint32_t spins = 0;
+        while ((int32_t)(__hip_atomic_load(cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) - target) < 0) {
+          if ((bad = err()) != 0u) break;
+          if (++spins > spin) {
+            atomicOr(&ctl->error, 1u);
+            bad = 1u;
+            break;
           }
+          __builtin_amdgcn_s_sleep(1);
         }
-        __threadfence();
-        bad = err();
       }
-      *dead = bad;
+      __threadfence();
+      dead[0] = bad;
     }
     __syncthreads();
-    if (*dead) abandon();
+    if (dead[0]) abandon();
   }
-  BB_DI void sync() const { barrier(&ctl->count, &ctl->gen, nblocks); }
+  BB_DI void sync() const { barrier(&ctl->count, &dead[1], nblocks); }
   // Barrier of the sort's sub-team, the first `members` workgroups.
-  BB_DI void sync_first(uint32_t members) const { barrier(&ctl->scount, &ctl->sgen, members); }
+  BB_DI void sync_first(uint32_t members) const { barrier(&ctl->scount, &dead[2], members); }
   // Exclusive scan over the whole team (rank order); two team barriers.
   BB_DI uint32_t scan(uint32_t v, uint32_t& total) const { return scan_first(v, total, nblocks); }
   // The same over the first `members` workgroups (rank order), with their
@@ -4190,7 +4190,7 @@ template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_team(Dev d) {
   __shared__ ScanLds<BLOCK> S;
   __shared__ TeamLds<BLOCK> Q;
-  __shared__ uint32_t dead;
+  __shared__ uint32_t dead[3];
   const TeamTable& T = *d.teams;
   const uint32_t nt = T.count < kMaxTeams ? T.count : kMaxTeams;
   if (nt == 0 || blockIdx.x >= T.first[nt]) return;  // workgroup-uniform
@@ -4201,11 +4201,12 @@ __global__ __launch_bounds__(BLOCK) void k_team(Dev d) {
   // stage boundaries: every workgroup has passed the barrier before rank 0
   // stamps the clock
   const GroupTeam<BLOCK> t{&S, &Q.sort, d.team_ctl + j, d.team_hist + (size_t)b0 * 256, blockIdx.x - b0, nb,
-                           &dead, d.stats + (size_t)w * ST_COUNT, d.ovf_any,
+                           dead, d.stats + (size_t)w * ST_COUNT, d.ovf_any,
                            d.team_spin ? d.team_spin : kTeamSpinLimit,
                            d.team_clock ? d.team_clock + (size_t)j * kTeamMarks : nullptr,
                            d.sort_blocks ? d.sort_blocks : kTeamSortBlocks};
   TeamShared& G = *reinterpret_cast<TeamShared*>(d.team_shared + (size_t)j * kTeamSharedBytes);
+  if (threadIdx.x == 0) dead[1] = dead[2] = 0u;  // thread 0's own words: no barrier needed
   auto mark = [&](uint32_t m) { t.mark(m); };
   mark(0);
   s_build(d, w, t, G.build);
