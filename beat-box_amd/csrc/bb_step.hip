@@ -382,7 +382,8 @@ struct GroupTeam {
     if (threadIdx.x == 0) {
       const uint32_t target = (*epoch += 1u) * members;
       uint32_t bad = 0u;
-      __threadfence();
+      // release: the workgroup's writes before the arrival (L2 write-back)
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
       if (atomicAdd(cnt, 1u) + 1u != target) {
         uint32_t spins = 0;
         while ((int32_t)(__hip_atomic_load(cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) - target) < 0) {
@@ -395,7 +396,8 @@ struct GroupTeam {
           __builtin_amdgcn_s_sleep(1);
         }
       }
-      __threadfence();
+      // acquire: the other members' writes before their arrivals (L2 invalidate)
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
       dead[0] = bad;
     }
     __syncthreads();

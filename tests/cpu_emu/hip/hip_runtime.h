@@ -73,6 +73,7 @@ static inline unsigned long long wall_clock64() {
 static inline void __threadfence() { __atomic_thread_fence(__ATOMIC_SEQ_CST); }
 static inline void __threadfence_block() { __atomic_thread_fence(__ATOMIC_SEQ_CST); }
 #define __builtin_amdgcn_s_sleep(x) emu::yield()
+#define __builtin_amdgcn_fence(order, scope) __atomic_thread_fence(order)
 // s_endpgm: the work-item's fiber ends here
 #define __builtin_amdgcn_endpgm() emu::end_program()
 

@@ -167,10 +167,11 @@ def test_profile_splits_team_time_by_stage():
             assert kern_ms[0] < 0.6 * total and kern_ms[5] > kern_ms[0], kern_ms
             # the build stage's parts by the team's marks inside it: every part
             # gets time (the lead workgroup's climb too) and they add up to
-            # the build stage
+            # the build stage, which also holds the team kernel's launch and
+            # exit (outside the marks; tens of microseconds)
             parts = b.build_split()
             assert parts.shape == (beatbox.BUILD_COUNT,) and np.all(parts > 0), parts
-            assert abs(float(parts.sum()) - float(kern_ms[0])) < 0.05 * float(kern_ms[0]), (parts, kern_ms)
+            assert abs(float(parts.sum()) - float(kern_ms[0])) < 0.05 * float(kern_ms[0]) + 0.05, (parts, kern_ms)
             # and the AVBD stage's parts
             av = b.avbd_split()
             assert av.shape == (beatbox.AVBD_COUNT,) and np.all(av > 0), av
