@@ -81,3 +81,27 @@ def slope_speed(world, body: int, ramp: int) -> float:
     th = RAMP_THETA[ramp]
     downhill = np.array([-np.cos(th), -np.sin(th), 0.0]) if ramp == 0 else np.array([np.cos(th), -np.sin(th), 0.0])
     return float(v @ downhill)
+
+
+def spinning_stack(w_bottom: float, w_top: float):
+    """Two frictionless unit cubes stacked on a frictionless floor under AVBD
+    (which neither damps nor assists a body with fewer than three contacts),
+    each spinning about the vertical axis at the given rate.  Frictionless
+    contacts apply no torque about the vertical, so each cube keeps its rate;
+    one above sqrt(0.0225) = 0.15 rad/s keeps that cube's sleep timer from
+    counting (RB_sim.slang:556-737).  Returns (desc, bodies): rows 1 and 2 are
+    the bottom and top cubes."""
+    desc = beatbox.WorldDesc.make(**scenes.world_desc(solver=scenes.SOLVER_AVBD, iterations=8))
+    b = np.zeros(3, scenes.BODY_DESC)
+    scenes._ground(b[0], friction=0.0)
+    for k, (y, w) in ((1, (0.5, w_bottom)), (2, (1.5, w_top))):
+        b[k]["shape"] = scenes.SHAPE_BOX
+        b[k]["flags"] = scenes.BODY_DYNAMIC | scenes.BODY_GRAVITY
+        b[k]["position"] = (0.0, y, 0.0)
+        b[k]["rotation"] = (0.0, 0.0, 0.0, 1.0)
+        b[k]["minimum"] = (-0.5, -0.5, -0.5)
+        b[k]["maximum"] = (0.5, 0.5, 0.5)
+        b[k]["mass"] = 1.0
+        b[k]["friction"] = 0.0
+        b[k]["omega"] = (0.0, w, 0.0)
+    return desc, b

@@ -16,7 +16,7 @@ import pytest
 import beatbox
 import scenes
 from known_answers import (CUBE_MASS, G, RAMP_IDS, RAMP_THETA, TOWER_HEIGHTS, contact_forces, ramp_cases,
-                           reference_desc)
+                           reference_desc, spinning_stack)
 from oracle_bind import OracleWorld
 from parity_util import compare_step
 
@@ -132,3 +132,17 @@ def test_sleep_and_warm_start_rules_on_gpu():
         print(f"rule case: solver {kw['solver']} v {bodies['velocity'][-1][0]}", flush=True)
         gpu, ora = _pair(beatbox.WorldDesc.make(**kw), bodies)
         _advance(gpu, ora, steps, check_every=1)
+
+
+def test_sleep_veto_by_a_loud_neighbour_on_gpu():
+    """The oracle's neighbourhood-veto rule case on the GPU, equal to the
+    oracle every 10 steps: the resting stack sleeps at step 30, and a cube
+    spinning at 0.2 rad/s (above or below) keeps the other awake."""
+    for w_bottom, w_top in ((0.0, 0.0), (0.0, 0.2), (0.2, 0.0)):
+        print(f"veto case: spin {w_bottom} below, {w_top} above", flush=True)
+        gpu, ora = _pair(*spinning_stack(w_bottom, w_top))
+        _advance(gpu, ora, 29, check_every=29)
+        assert not np.any(gpu.read_bodies()["flags"][1:] & 8)
+        _advance(gpu, ora, 91, check_every=10)
+        asleep = gpu.read_bodies()["flags"][1:] & 8
+        assert np.all(asleep) if w_bottom == w_top == 0.0 else not np.any(asleep), (w_bottom, w_top)

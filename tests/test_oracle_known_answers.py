@@ -33,6 +33,7 @@ import numpy as np
 import beatbox
 import scenes
 from known_answers import (CUBE_MASS, G, RAMP_IDS, RAMP_THETA, TOWER_HEIGHTS, contact_forces, ramp_cases,
+                           spinning_stack,
                            reference_desc, slope_speed)
 from oracle_bind import OracleWorld
 from beatbox import MANIFOLD_TAP, TAP_MANIFOLDS
@@ -205,6 +206,37 @@ def test_sleep_timer_rules_on_a_lone_body():
             got = _asleep_steps(_lone_body(solver, w=(0.0, w, 0.0)), 90)
             want = [s for s in range(c + 29, 91, 11)] if c is not None else []
             assert got == want, (solver, w, got, want)
+
+
+def test_sleep_veto_by_a_loud_neighbour():
+    """The neighbourhood veto (RB_sim.slang:556-737): a dynamic body in
+    contact with a dynamic neighbour whose sleep timer has not reached 30 may
+    not sleep, whatever its own timer; a static neighbour never vetoes.  Two
+    cubes at rest on the floor are calm from step 1, so both sleep at step 30
+    (= 1 + 29).  With either cube spinning at 0.2 rad/s (never calm, never
+    asleep), the other stays calm but never sleeps: the veto works in both
+    directions of a contact."""
+    for w_bottom, w_top in ((0.0, 0.0), (0.0, 0.2), (0.2, 0.0)):
+        desc, bodies = spinning_stack(w_bottom, w_top)
+        o = _world(desc, bodies)
+        asleep = {1: [], 2: []}
+        for s in range(1, 121):
+            o.step(1)
+            r = o.read_bodies()
+            for k in (1, 2):
+                if r["flags"][k] & 8:
+                    asleep[k].append(s)
+                spin = (w_bottom, w_top)[k - 1]
+                v2 = float(np.dot(r["velocity"][k], r["velocity"][k]))
+                w2 = float(np.dot(r["omega"][k], r["omega"][k]))
+                if spin == 0.0:
+                    assert v2 < 0.0144 and w2 < 0.0225, (w_bottom, w_top, s, k)  # calm every step
+                else:
+                    assert abs(np.sqrt(w2) - spin) < 1e-3, (w_bottom, w_top, s, k)  # keeps its rate
+        if w_bottom == w_top == 0.0:
+            assert asleep == {1: list(range(30, 121)), 2: list(range(30, 121))}, asleep
+        else:
+            assert asleep == {1: [], 2: []}, (w_bottom, w_top, asleep)
 
 
 def _slide(speed: float, steps: int):
