@@ -124,6 +124,7 @@ def main() -> None:
                 line = [x for x in f.read().splitlines() if x.startswith("{")][-1]
             bench = json.loads(line)
             extra = dict(window=dict(steps=bench["steps"], warmup=bench["warmup"]),
+                         launch_params=bench["config"].get("launch_params"),
                          model_bytes_per_launch={k: v["model_bytes"] for k, v in bench["config"]["stages"].items()})
         with open(a.json_out, "w") as f:
             json.dump(dict(fetch_factor=ff, write_factor=fw, **extra, kernels=table), f, indent=1)

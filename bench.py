@@ -141,13 +141,16 @@ def traffic_fields(kernel: str, args, model_per_launch: float, params: dict, n_w
     prof = profiled_traffic(kernel, params, n_wide, args.worlds)
     window = meta.get("window")  # {"steps": S, "warmup": W} of the counter runs
     model = meta.get("model_bytes_per_launch", {}).get(kernel)
-    same = bool(window) and (window.get("steps"), window.get("warmup")) == (args.steps, args.warmup)
+    # the same step window and, where the profile records them, the same launch parameters
+    same = (bool(window) and (window.get("steps"), window.get("warmup")) == (args.steps, args.warmup)
+            and meta.get("launch_params") in (None, params))
     out = {"traffic": prof if same else None,
            "traffic_over_model": (prof / model_per_launch) if (same and prof) else None,
            "traffic_source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of bench.py (copy-kernel calibrated), "
                              "profiles/bench_batch4096_traffic.json"}
     if prof is not None and not same:
-        out["traffic_profiled"] = dict(window=window, bytes_per_launch=prof, model_bytes_per_launch=model,
+        out["traffic_profiled"] = dict(window=window, launch_params=meta.get("launch_params"),
+                                       bytes_per_launch=prof, model_bytes_per_launch=model,
                                        over_model=(prof / model) if model else None)
     return out
 
@@ -180,15 +183,18 @@ def apply_params(b, params: dict) -> None:
         b.set_param(getattr(beatbox, "PARAM_" + name), params[name.lower()])
 
 
-def time_batch(b, steps: int, warmup: int, tune: bool):
+def time_batch(b, steps: int, warmup: int, tune: bool, fixed: dict | None = None):
     """Graph replay of `steps` steps after `warmup` untimed ones (and the
     autotuner, which replays the next step per candidate and leaves the state
-    unchanged); returns the wall time, the batch energy at step 0 and after
-    the timed steps and the chosen launch parameters."""
+    unchanged, or the `fixed` launch parameters); returns the wall time, the
+    batch energy at step 0 and after the timed steps and the launch
+    parameters used."""
     e0 = float(b.energy().sum())
     b.step(warmup, use_graph=True)
     b.sync(allow_overflow=True)
-    if tune:
+    if fixed:
+        apply_params(b, fixed)
+    elif tune:
         b.autotune(2)
     params = {name.lower(): b.get_param(getattr(beatbox, "PARAM_" + name)) for name in TUNED}
     t0 = time.perf_counter()
@@ -443,6 +449,9 @@ def main() -> int:
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: min(16, host cores)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="default launch parameters, no autotuner")
+    ap.add_argument("--params", default="", metavar="JSON",
+                    help="launch parameters of the batch (the names of `launch_params`), instead of the "
+                         "autotuner: pins one configuration across separate profiler runs")
     ap.add_argument("--no-extra", action="store_true", help="skip the large-pile, sphere-gas and serving runs")
     ap.add_argument("--extra", default="large_pile,sphere_gas,serving,next_rows",
                     help="comma-separated extra runs (large_pile, sphere_gas, serving, next_rows)")
@@ -478,7 +487,8 @@ def main() -> int:
 
     # 1) throughput: hipGraph replay of the timed steps
     batch, total = make_batch(args.worlds, args.seed, 16, 2048)
-    t, e0, e1, params, overflowed = time_batch(batch, args.steps, args.warmup, not args.no_tune)
+    fixed = json.loads(args.params) if args.params else None
+    t, e0, e1, params, overflowed = time_batch(batch, args.steps, args.warmup, not args.no_tune, fixed)
     if overflowed:
         log(f"batch: {len(overflowed)} worlds flagged an overflow: {overflowed[:16]}")
     log(f"launch parameters: {params}")
