@@ -221,6 +221,7 @@ bb_status alloc_arrays(bb_batch* b, Buffers& A, uint32_t W, uint32_t NB, uint32_
   d.team_hist = A.alloc<uint32_t>((size_t)kMaxTeamBlocks * 256);
   d.team_shared = A.alloc<char>((size_t)kMaxTeams * kTeamSharedBytes);
   d.team_grid = 0;
+  d.team_vox = 0;
   b->d_vshape = A.alloc<VoxShape>(BB_MAX_VOXEL_SHAPES);
   b->d_vocc = A.alloc<uint32_t>(BB_MAX_VOXEL_OCC_WORDS);
   b->d_vsurf = A.alloc<uint32_t>(BB_MAX_VOXEL_SURFACE);
@@ -391,14 +392,13 @@ bb_status reset_world_state(bb_batch* b, uint32_t w, hipStream_t st) {
 uint32_t assign_teams(bb_batch* b, uint32_t grid_cap) {
   TeamTable& T = b->team_table;
   T = TeamTable{};
+  b->base.team_vox = 0;
   for (auto& wi : b->infos) wi.mode &= ~kModeTeam;
   if (b->team_bodies == 0) return 0;
   grid_cap = std::min(grid_cap, kMaxTeamBlocks);
   std::vector<uint32_t> cand;
   for (uint32_t w = 0; w < (uint32_t)b->worlds.size(); ++w)
-    if (b->infos[w].n >= b->team_bodies && b->infos[w].n > 0 && !(b->infos[w].mode & kModeVoxel) &&
-        !b->worlds[w].no_team)
-      cand.push_back(w);  // voxel worlds run one workgroup per world
+    if (b->infos[w].n >= b->team_bodies && b->infos[w].n > 0 && !b->worlds[w].no_team) cand.push_back(w);
   // largest worlds first; each team gets at least two workgroups
   std::sort(cand.begin(), cand.end(), [&](uint32_t x, uint32_t y) {
     return b->infos[x].n != b->infos[y].n ? b->infos[x].n > b->infos[y].n : x < y;
@@ -421,6 +421,7 @@ uint32_t assign_teams(bb_batch* b, uint32_t grid_cap) {
     T.world[k] = cand[k];
     first += nb;
     b->infos[cand[k]].mode |= kModeTeam;
+    if (b->infos[cand[k]].mode & kModeVoxel) b->base.team_vox = 1;
   }
   T.first[cand.size()] = first;
   return first;
@@ -582,6 +583,7 @@ bb_status grow_pairs(bb_batch* b) {
   b->base.any_voxel = base0.any_voxel;
   b->base.any_island = base0.any_island;
   b->base.team_grid = base0.team_grid;
+  b->base.team_vox = base0.team_vox;
   b->vox_dirty = false;  // the pools came over with the rest
   b->NP = cap;
   hw.pair_cap = cap;

@@ -225,6 +225,7 @@ struct Dev {
   uint32_t* team_hist;
   char* team_shared;
   uint32_t team_grid;     // workgroups of the k_team launch
+  uint32_t team_vox;      // a team world holds voxel bodies (the k_team instance with their narrow phase)
   uint32_t team_spin;     // spins before a team barrier wait gives up (0: default)
   uint32_t sort_blocks;   // workgroups of a team that sort (0: kTeamSortBlocks)
   uint32_t avbd_lanes;    // lanes per body in the per-world AVBD primal sweeps (1, 2, 4, 8; 0: 4)

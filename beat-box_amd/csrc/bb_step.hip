@@ -4188,7 +4188,10 @@ union TeamLds {
   WalkLds<BLOCK> walk;
 };
 
-template <int BLOCK>
+// kVox: the instance that also runs the voxel narrow phase, launched when a
+// team world holds voxel bodies (its code would raise every team's register
+// count otherwise).
+template <int BLOCK, bool kVox>
 __global__ __launch_bounds__(BLOCK) void k_team(Dev d) {
   __shared__ ScanLds<BLOCK> S;
   __shared__ TeamLds<BLOCK> Q;
@@ -4217,7 +4220,8 @@ __global__ __launch_bounds__(BLOCK) void k_team(Dev d) {
   s_broad(d, w, t, G.broad, Q.walk);
   t.sync();
   mark(2);
-  s_narrow<false>(d, w, t);  // voxel worlds never run as teams
+  s_narrow<false>(d, w, t);
+  if constexpr (kVox) s_narrow<true>(d, w, t);  // each instance returns at once on the other kind
   t.sync();
   s_contacts(d, w, t);
   t.sync();
@@ -4526,7 +4530,8 @@ template void launch_fused_b<BB_BLOCK>(const Lane&, hipStream_t);
 template void launch_pgs_b<64>(const Dev&, hipStream_t, uint32_t);
 #endif
 #else
-template __global__ void k_team<kTeamBlock>(Dev);
+template __global__ void k_team<kTeamBlock, false>(Dev);
+template __global__ void k_team<kTeamBlock, true>(Dev);
 
 static void launch_stage(int stage, const Lane& l, hipStream_t st) {
   if (l.grid == 0) return;
@@ -4548,7 +4553,9 @@ static void launch_stage(int stage, const Lane& l, hipStream_t st) {
 }
 
 static void launch_team(const Dev& d, hipStream_t st) {
-  if (d.team_grid) hipLaunchKernelGGL(k_team<kTeamBlock>, dim3(d.team_grid), dim3(kTeamBlock), 0, st, d);
+  if (!d.team_grid) return;
+  if (d.team_vox) hipLaunchKernelGGL((k_team<kTeamBlock, true>), dim3(d.team_grid), dim3(kTeamBlock), 0, st, d);
+  else hipLaunchKernelGGL((k_team<kTeamBlock, false>), dim3(d.team_grid), dim3(kTeamBlock), 0, st, d);
 }
 
 void launch_step(const Lane& l, hipStream_t st, bool team) {
@@ -4598,12 +4605,15 @@ void launch_energy(const Dev& d, double* out, hipStream_t st) {
 
 // Workgroups of k_team that can be resident at once on the device.
 uint32_t team_capacity() {
-  int dev = 0, cus = 0, per = 0;
+  int dev = 0, cus = 0, per = 0, per_vox = 0;
   if (hipGetDevice(&dev) != hipSuccess) return 0;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
-  const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_team<kTeamBlock>, kTeamBlock, 0);
-  if (e != hipSuccess || per <= 0 || cus <= 0) return 0;
-  return (uint32_t)per * (uint32_t)cus;
+  // the smaller of the two instances' residency: either may be launched
+  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_team<kTeamBlock, false>, kTeamBlock, 0);
+  if (e == hipSuccess)
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_vox, k_team<kTeamBlock, true>, kTeamBlock, 0);
+  if (e != hipSuccess || per <= 0 || per_vox <= 0 || cus <= 0) return 0;
+  return (uint32_t)std::min(per, per_vox) * (uint32_t)cus;
 }
 
 #endif  // BB_BLOCK_TU

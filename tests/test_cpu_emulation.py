@@ -197,7 +197,7 @@ def test_emulated_voxel_worlds_equal_oracle(emu_main, tmp_path):
     """Voxel bodies (concave L, cross and frame shapes, six pair slots per
     voxel pair, key-exact warm start) with boxes and spheres, PGS_SOFT and
     AVBD, next to a plain world: the emulated kernels reproduce the oracle
-    bit for bit."""
+    bit for bit, one workgroup per world and as workgroup teams."""
     from oracle_bind import OracleWorld
 
     shapes, ref = scenes.voxel_pile()
@@ -227,6 +227,15 @@ def test_emulated_voxel_worlds_equal_oracle(emu_main, tmp_path):
     # the voxel worlds made contact
     last = [l for l in log.splitlines() if l.startswith(f"step {steps - 1} world 1")][0].split()
     assert int(last[7]) > 0
+    # every world as a workgroup team (the k_team instance with the voxel
+    # narrow phase): in bounds and the same steps
+    log_t, raw_t = _run(emu_main, scene, str(tmp_path / "t.bin"), steps, 12, 0)
+    assert _world_lines(log_t) == _world_lines(log) and raw_t == raw
+    # they did run as teams: with one workgroup resident at a time no team
+    # barrier is met, and each world's step is flagged (overflow bit 1)
+    log_x, _ = _run(emu_main, scene, str(tmp_path / "x.bin"), 1, 12, 0, graph=0, team_spin=200, max_blocks=1)
+    for w in range(len(worlds)):
+        assert f"step 0 overflow world {w}" in log_x, log_x
 
 
 def test_emulated_pair_overflow_grows_and_replays_exactly(emu_main, tmp_path):

@@ -146,3 +146,36 @@ def test_large_voxel_pile_batch_equals_alone():
     assert st["overflow"] == 0 and st["collision_count"] > 100
     w.destroy()
     b.destroy()
+
+
+def test_voxel_worlds_as_teams_equal_one_workgroup():
+    """The 600-body voxel pile (PGS_SOFT) and the reference's voxel scene
+    (AVBD) stepped by workgroup teams (the k_team instance with the voxel
+    narrow phase) give bit-identical trajectories and counters to one
+    workgroup per world, graph replay included."""
+    shapes, vox = scenes.voxel_mixed_pile(0xBB64, 600)
+    ref_shapes, ref = scenes.voxel_pile()
+    # the reference scene's shape ids name the same shapes in the mixed pile's list
+    assert all(np.array_equal(a, c) for (_, a), (_, c) in zip(ref_shapes, shapes))
+    pgs = beatbox.WorldDesc.make(**scenes.world_desc(iterations=8, morton_bits=63, quirks=0))
+    avbd = beatbox.WorldDesc.make(**scenes.world_desc(solver=scenes.SOLVER_AVBD, iterations=10,
+                                                      morton_bits=63, quirks=0))
+    worlds = [(pgs, vox), (avbd, ref)]
+    runs = []
+    for team in (len(ref), 0):
+        print(f"voxel teams: team threshold {team}", flush=True)
+        b = beatbox.Batch()
+        b.set_param(beatbox.PARAM_TEAM_BODIES, team)
+        for _, cells in shapes:
+            b.add_voxel_shape(cells, scenes.VOXEL_SIZE, scenes.VOXEL_DENSITY)
+        for d, bodies in worlds:
+            b.add_world(d, bodies)
+        b.step(40, use_graph=True)
+        b.sync()
+        runs.append([(b.read_world(w, len(bodies)), b.stats(w)) for w, (_, bodies) in enumerate(worlds)])
+        b.destroy()
+    for w, ((x, sx), (y, sy)) in enumerate(zip(*runs)):
+        for f in ("position", "rotation", "velocity", "omega", "flags"):
+            assert x[f].tobytes() == y[f].tobytes(), (w, f)
+        assert sx == sy and sx["overflow"] == 0, w
+    assert runs[0][0][1]["collision_count"] > 100 and runs[0][1][1]["collision_count"] > 0
