@@ -271,6 +271,7 @@ EXPORTS = {
     "bb_server_report_get": (C.c_int, [C.c_void_p, C.POINTER(ServerReport)]),
     "bb_server_checkpoint": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.POINTER(C.c_size_t)]),
     "bb_server_restore": (C.c_int, [C.POINTER(ServerDesc), C.c_void_p, C.c_size_t, C.POINTER(C.c_void_p)]),
+    "bb_server_add_voxel_shape": (C.c_int, [C.c_void_p, C.POINTER(VoxelShapeDesc), C.POINTER(VoxelShapeInfo)]),
     "bb_server_destroy": (None, [C.c_void_p]),
     "bb_version": (C.c_char_p, []),
     "bb_copy_bandwidth": (C.c_int, [C.c_uint64, C.c_uint32, C.POINTER(C.c_float)]),
@@ -653,6 +654,14 @@ class Server:
         d = ServerDesc(budget_bytes, max_worlds, max_bodies, max_resident_bodies, block, team_bodies,
                        split_bodies)
         _check(lib().bb_server_create(C.byref(d), C.byref(self._h)), "bb_server_create")
+
+    def add_voxel_shape(self, solid: np.ndarray, voxel_size: float, density: float) -> dict:
+        """A voxel shape for the requests submitted afterwards (bb_server_add_voxel_shape)."""
+        d, words = voxel_desc(solid, voxel_size, density)
+        info = VoxelShapeInfo()
+        _check(lib().bb_server_add_voxel_shape(self._h, C.byref(d), C.byref(info)), "bb_server_add_voxel_shape")
+        del words
+        return info.as_dict()
 
     def submit(self, desc: WorldDesc, bodies: np.ndarray, steps: int) -> int:
         arr, ptr = _bodies_ptr(bodies)

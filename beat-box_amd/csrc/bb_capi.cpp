@@ -1061,6 +1061,26 @@ const char* take_shapes(bb_batch* b, const char* in, const char* end, uint32_t n
   return words_seen == shape_words ? in : nullptr;
 }
 
+size_t shapes_bytes(const bb_batch* b) { return b->vox_shapes.size() * sizeof(SnapShape) + b->vox_occ.size() * 4; }
+
+size_t shapes_section_bytes(const char* in, const char* end, uint32_t nshapes, uint32_t shape_words) {
+  if (nshapes > BB_MAX_VOXEL_SHAPES || shape_words > BB_MAX_VOXEL_OCC_WORDS) return 0;
+  const char* at = in;
+  uint32_t words_seen = 0;
+  for (uint32_t k = 0; k < nshapes; ++k) {
+    SnapShape r;
+    if ((size_t)(end - at) < sizeof(r)) return 0;
+    std::memcpy(&r, at, sizeof(r));
+    at += sizeof(r);
+    const uint64_t cells = (uint64_t)r.dims[0] * r.dims[1] * r.dims[2];
+    if (r.words > shape_words - words_seen || r.words != (cells + 31u) / 32u || (size_t)(end - at) < (size_t)r.words * 4)
+      return 0;
+    words_seen += r.words;
+    at += (size_t)r.words * 4;
+  }
+  return words_seen == shape_words ? (size_t)(at - in) : 0;
+}
+
 bb_status snapshot_world(bb_batch* b, void* dst, size_t cap, size_t* size) {
   if (!size || b->worlds.size() != 1) return BB_ERR_INVALID;
   bb_status st = build(b);

@@ -270,6 +270,15 @@ void world_state_arrays(bb_batch* b, uint32_t w, int parity, uint32_t n, uint32_
 // permutation of [0, n), the id -> row map consistent with them, pair rows
 // and manifold bodies below n, at most 8 contacts per manifold).
 bool valid_blob_indices(const char* in, uint32_t n, uint32_t p);
+// The voxel-shape section of a checkpoint: per shape its record (dims, cell
+// size, density, word count) and occupancy words.  put_shapes writes b's
+// (shapes_bytes of them); take_shapes adds a section's shapes to b as ids
+// 1, 2, ... (null: refused); shapes_section_bytes checks one on the host
+// (0: malformed; a section without shapes is also 0 bytes).
+size_t shapes_bytes(const bb_batch* b);
+char* put_shapes(const bb_batch* b, char* o);
+const char* take_shapes(bb_batch* b, const char* in, const char* end, uint32_t nshapes, uint32_t shape_words);
+size_t shapes_section_bytes(const char* in, const char* end, uint32_t nshapes, uint32_t shape_words);
 
 }  // namespace host
 }  // namespace bb

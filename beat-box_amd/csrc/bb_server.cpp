@@ -237,6 +237,15 @@ hipEvent_t take_event(bb_server* s) {
   return e;
 }
 
+// Pair slots of a request per unit of its pair scale and per default pair
+// capacity: six when a body is a voxel shape (a voxel pair takes up to six
+// slots, one per contact-face code), else one.
+uint32_t pair_factor(const std::vector<bb_body_desc>& bodies) {
+  for (const bb_body_desc& bd : bodies)
+    if (BB_SHAPE_KIND(bd.shape) == BB_SHAPE_VOXEL) return 6u;
+  return 1u;
+}
+
 uint64_t price(const bb_server* s, uint32_t n, uint32_t a, uint32_t pair_scale = 1) {
   return s->cW + s->cB * n + s->cA * a + s->cP * (uint64_t)pair_capacity(n) * pair_scale;
 }
@@ -293,9 +302,9 @@ bb_status finish_oldest(bb_server* s, uint32_t p) {
       // queue, if it fits the budget and the pair arena that way
       const Request& o = oit->second;
       const uint32_t scale = (ovf & 1u) ? o.pair_scale * 2u : o.pair_scale;
-      const uint32_t nb = (uint32_t)o.bodies.size();
-      const uint64_t bytes = price(s, nb, o.active, scale);
-      if (bytes <= s->desc.hbm_budget_bytes && (uint64_t)pair_capacity(nb) * scale <= s->NP) {
+      const uint32_t nb = (uint32_t)o.bodies.size(), pf = pair_factor(o.bodies);
+      const uint64_t bytes = price(s, nb, o.active, scale * pf);
+      if (bytes <= s->desc.hbm_budget_bytes && (uint64_t)pair_capacity(nb) * scale * pf <= s->NP) {
         Request q = std::move(oit->second);
         s->origin.erase(oit);
         q.state.clear();  // a rerun starts from the first step
@@ -340,6 +349,8 @@ bb_status ensure_graphs(bb_server* s, uint32_t p) {
     d.order = d.order + P.w0;
     d.pgs_lcap = P.lcap;
     if (!P.teams) d.team_grid = 0;
+    // with voxel shapes registered any team may hold voxel bodies
+    d.team_vox = s->b->vox_shapes.empty() ? 0u : 1u;
     launch_step(Lane{d, P.block, P.w1 - P.w0}, P.stream, true);
     HIPCHK(hipStreamEndCapture(P.stream, &g));
     HIPCHK(hipGraphInstantiate(&P.graph[par], g, nullptr, nullptr, 0));
@@ -377,7 +388,8 @@ bb_status pool_step(bb_server* s, uint32_t p) {
       s->rb.release(bo, n);
       break;
     }
-    const uint32_t pcap = pair_capacity(n) * q.pair_scale;
+    const uint32_t pf = pair_factor(q.bodies);
+    const uint32_t pcap = pair_capacity(n) * q.pair_scale * pf;
     if ((uint64_t)pcap > s->NP || !s->rp.alloc(pcap, &po)) {
       s->rb.release(bo, n);
       s->ra.release(ao, q.active);
@@ -404,6 +416,7 @@ bb_status pool_step(bb_server* s, uint32_t p) {
     hw.desc = q.desc;
     hw.bodies = q.bodies;
     hw.pair_scale = q.pair_scale;
+    hw.has_voxel = pf > 1u;
     hw.no_team = q.no_team;
     hw.n = n;
     hw.a = q.active;
@@ -451,7 +464,7 @@ bb_status pool_step(bb_server* s, uint32_t p) {
     const uint32_t n = hw.n, a = hw.a;
     void* mem = ch.take(kImageBodyBytes * std::max(n, 1u) + kImageActiveBytes * std::max(a, 1u));
     const Images img = carve_images(mem, std::max(n, 1u), std::max(a, 1u));
-    fill_images(hw, 0, 0, img);
+    fill_images(hw, 0, 0, img, &b->vox_info);
     st = upload_images(b, img, hw.body_off, n, hw.act_off, a, stream);
     if (st != BB_OK) return st;
     st = reset_world_state(b, w, stream);
@@ -478,7 +491,7 @@ bb_status pool_step(bb_server* s, uint32_t p) {
       org.state.shrink_to_fit();
     }
     WorldInfo wi = info_of(hw);
-    wi.mode = 0;  // the cleared pair counts already give an empty previous step
+    wi.mode &= kModeVoxel;  // the cleared pair counts already give an empty previous step
     b->infos[w] = wi;
     hw.bodies.clear();
     hw.bodies.shrink_to_fit();
@@ -776,21 +789,61 @@ bb_status bb_server_create(const bb_server_desc* desc, bb_server** out) {
   return BB_OK;
 }
 
+// Voxel shapes for the server's requests: every pool's steps complete first,
+// the shapes go to the device pools the batch allocated at creation, and the
+// pools' step graphs are made again (now with the voxel narrow phase, and the
+// k_team instance that has it).  add: one shape from a descriptor; the
+// checkpoint path adds a blob's section.
+static bb_status shapes_changed(bb_server* s) {
+  for (uint32_t p = 0; p < (uint32_t)s->pools.size(); ++p)
+    while (!s->pools[p].inflight.empty()) {
+      bb_status st = finish_oldest(s, p);
+      if (st != BB_OK) return st;
+    }
+  for (Pool& P : s->pools) {
+    HIPCHK(hipStreamSynchronize(P.stream));
+    for (hipGraphExec_t& g : P.graph)
+      if (g) {
+        HIPCHK(hipGraphExecDestroy(g));
+        g = nullptr;
+      }
+  }
+  bb_batch* b = s->b;
+  bb_status st = upload_voxels(b);
+  if (st != BB_OK) return st;
+  b->base.any_voxel = 1;
+  return BB_OK;
+}
+
+static bb_status adopt_shapes(bb_server* s, const char* in, const char* end, uint32_t nshapes, uint32_t shape_words) {
+  if (!s->b->vox_shapes.empty() || !take_shapes(s->b, in, end, nshapes, shape_words)) return BB_ERR_INVALID;
+  return shapes_changed(s);
+}
+
+bb_status bb_server_add_voxel_shape(bb_server* s, const bb_voxel_shape_desc* desc, bb_voxel_shape_info* out) {
+  if (!s) return BB_ERR_INVALID;
+  bb_status st = add_voxel_shape(s->b, desc, out);
+  if (st != BB_OK) return st;
+  return shapes_changed(s);
+}
+
 bb_status bb_server_submit(bb_server* s, const bb_world_desc* desc, const bb_body_desc* bodies, uint32_t n,
                            uint32_t steps, uint64_t* ticket) {
   if (!s || !valid_desc(desc) || n == 0 || !bodies || steps == 0) return BB_ERR_INVALID;
   for (uint32_t i = 0; i < n; ++i)
-    if (!valid_body(bodies[i])) return BB_ERR_INVALID;
+    if (!valid_body_with_voxels(bodies[i], s->b->vox_shapes.size())) return BB_ERR_INVALID;
   if (over_reference_cap(*desc, n)) return BB_ERR_CAPACITY;
   uint32_t a = 0;
   for (uint32_t i = 0; i < n; ++i)
     if (bodies[i].flags & BB_BODY_DYNAMIC) ++a;
   Request q;
-  q.bytes = price(s, n, a);
-  if (q.bytes > s->desc.hbm_budget_bytes || n > s->NB || (uint64_t)pair_capacity(n) > s->NP) return BB_ERR_BUDGET;
+  q.bodies.assign(bodies, bodies + n);
+  const uint32_t pf = pair_factor(q.bodies);
+  q.bytes = price(s, n, a, pf);
+  if (q.bytes > s->desc.hbm_budget_bytes || n > s->NB || (uint64_t)pair_capacity(n) * pf > s->NP)
+    return BB_ERR_BUDGET;
   q.ticket = s->next_ticket++;
   q.desc = *desc;
-  q.bodies.assign(bodies, bodies + n);
   q.steps = steps;
   q.active = a;
   q.submit_ms = now_ms();
@@ -878,8 +931,9 @@ bb_status bb_server_poll(bb_server* s, uint64_t ticket, uint32_t* state, bb_body
 // and its state arrays; a queued world's carries its request only.
 struct CkptHeader {
   char magic[8];
-  uint32_t version, worlds, results, pad;
+  uint32_t version, worlds, results, nshapes;
   uint64_t next_ticket;
+  uint32_t shape_words, pad;  // the voxel shapes' section follows the header
 };
 struct CkptWorld {
   uint64_t ticket;
@@ -893,8 +947,8 @@ struct CkptResult {
   uint32_t n, failed;
   bb_step_stats stats;
 };
-constexpr char kCkptMagic[8] = {'B', 'B', 'S', 'R', 'V', 'C', 'K', '1'};
-constexpr uint32_t kCkptVersion = 1;
+constexpr char kCkptMagic[8] = {'B', 'B', 'S', 'R', 'V', 'C', 'K', '2'};
+constexpr uint32_t kCkptVersion = 2;
 // pair-capacity doublings a checkpointed request may carry
 constexpr uint32_t kMaxPairScale = 1u << kMaxReruns;
 
@@ -927,7 +981,7 @@ bb_status bb_server_checkpoint(bb_server* s, void* dst, size_t cap, size_t* size
       res.push_back(Res{w, std::min(np, sl.pair_cap), P.latest, &it->second});
     }
   }
-  size_t bytes = sizeof(CkptHeader);
+  size_t bytes = sizeof(CkptHeader) + shapes_bytes(b);
   uint32_t nworlds = (uint32_t)res.size();
   for (const Res& r : res)
     bytes += sizeof(CkptWorld) + r.q->bodies.size() * sizeof(bb_body_desc) +
@@ -952,7 +1006,10 @@ bb_status bb_server_checkpoint(bb_server* s, void* dst, size_t cap, size_t* size
   h.worlds = nworlds;
   h.results = (uint32_t)s->results.size();
   h.next_ticket = s->next_ticket;
+  h.nshapes = (uint32_t)b->vox_shapes.size();
+  h.shape_words = (uint32_t)b->vox_occ.size();
   put(&h, sizeof(h));
+  o = put_shapes(b, o);
   auto record = [&](const Request& q, uint32_t done, uint32_t npairs, bool resident) {
     CkptWorld r{};
     r.ticket = q.ticket;
@@ -1008,6 +1065,10 @@ bb_status bb_server_restore(const bb_server_desc* desc, const void* src, size_t 
   in += sizeof(h);
   if (std::memcmp(h.magic, kCkptMagic, 8) != 0 || h.version != kCkptVersion) return BB_ERR_INVALID;
   // the whole blob is checked on the host before a server is made
+  const char* shapes = in;
+  const size_t shapes_len = shapes_section_bytes(in, end, h.nshapes, h.shape_words);
+  if (h.nshapes && !shapes_len) return BB_ERR_INVALID;
+  in += shapes_len;
   struct Rec {
     CkptWorld r;
     const char* bodies;
@@ -1031,17 +1092,18 @@ bb_status bb_server_restore(const bb_server_desc* desc, const void* src, size_t 
         r.pair_scale > kMaxPairScale || r.reruns > kMaxReruns || r.ticket == 0 || r.resident > 1 ||
         (!r.resident && (r.done || r.npairs)) || over_reference_cap(r.desc, r.n))
       return BB_ERR_INVALID;
-    if ((uint64_t)r.npairs > (uint64_t)pair_capacity(r.n) * r.pair_scale) return BB_ERR_INVALID;
     rc.bodies = take((size_t)r.n * sizeof(bb_body_desc));
     if (!rc.bodies) return BB_ERR_INVALID;
-    uint32_t a = 0;
+    uint32_t a = 0, pf = 1;
     for (uint32_t i = 0; i < r.n; ++i) {
       bb_body_desc bd;
       std::memcpy(&bd, rc.bodies + (size_t)i * sizeof(bb_body_desc), sizeof(bd));
-      if (!valid_body(bd)) return BB_ERR_INVALID;
+      if (!valid_body_with_voxels(bd, h.nshapes)) return BB_ERR_INVALID;
       if (bd.flags & BB_BODY_DYNAMIC) ++a;
+      if (BB_SHAPE_KIND(bd.shape) == BB_SHAPE_VOXEL) pf = 6;
     }
     if (a != r.active) return BB_ERR_INVALID;
+    if ((uint64_t)r.npairs > (uint64_t)pair_capacity(r.n) * r.pair_scale * pf) return BB_ERR_INVALID;
     if (r.resident) {
       rc.state = take(world_state_bytes(r.n, r.npairs));
       if (!rc.state || !valid_blob_indices(rc.state, r.n, r.npairs)) return BB_ERR_INVALID;
@@ -1069,6 +1131,13 @@ bb_status bb_server_restore(const bb_server_desc* desc, const void* src, size_t 
   bb_server* s = nullptr;
   bb_status st = bb_server_create(desc, &s);
   if (st != BB_OK) return st;
+  if (h.nshapes) {
+    st = adopt_shapes(s, shapes, shapes + shapes_len, h.nshapes, h.shape_words);
+    if (st != BB_OK) {
+      bb_server_destroy(s);
+      return st;
+    }
+  }
   for (const Rec& rc : recs) {
     const CkptWorld& r = rc.r;
     Request q;
@@ -1081,9 +1150,11 @@ bb_status bb_server_restore(const bb_server_desc* desc, const void* src, size_t 
     q.pair_scale = r.pair_scale;
     q.no_team = r.no_team != 0;
     q.reruns = r.reruns;
-    q.bytes = price(s, r.n, r.active, r.pair_scale);
+    const uint32_t pf = pair_factor(q.bodies);
+    q.bytes = price(s, r.n, r.active, r.pair_scale * pf);
     q.submit_ms = now_ms();
-    if (q.bytes > s->desc.hbm_budget_bytes || r.n > s->NB || (uint64_t)pair_capacity(r.n) * r.pair_scale > s->NP) {
+    if (q.bytes > s->desc.hbm_budget_bytes || r.n > s->NB ||
+        (uint64_t)pair_capacity(r.n) * r.pair_scale * pf > s->NP) {
       bb_server_destroy(s);
       return BB_ERR_BUDGET;
     }

@@ -720,6 +720,12 @@ bb_status bb_server_create(const bb_server_desc* desc, bb_server** out);
 /* Queues a world (bodies follow the load_scene rules, as bb_world_add_bodies)
  * that runs `steps` steps.  BB_ERR_BUDGET when the world alone exceeds the
  * budget or the capacities; *ticket identifies the request. */
+/* bb_world_add_voxel_shape for the server: requests submitted afterwards may
+ * hold voxel bodies of this shape (BB_SHAPE_VOXEL_OF(out->shape_id)); such a
+ * world takes six pair slots per default pair slot.  The server's steps in
+ * flight complete first.  No reference counterpart (the reference builds its
+ * shapes per scene, src/scene_manager.hpp:478-572). */
+bb_status bb_server_add_voxel_shape(bb_server* s, const bb_voxel_shape_desc* desc, bb_voxel_shape_info* out);
 bb_status bb_server_submit(bb_server* s, const bb_world_desc* desc, const bb_body_desc* bodies, uint32_t n,
                            uint32_t steps, uint64_t* ticket);
 /* Runs nsteps server rounds.  In a round every pool with queued or resident

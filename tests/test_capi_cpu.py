@@ -316,7 +316,7 @@ def _server_blob(mutate_state=None, **over):
                     pair_scale=1, no_team=0, reruns=0, npairs=p, resident=1, pad=0)
     for k, v in over.items():
         setattr(rec, k, v)
-    head = b"BBSRVCK1" + struct.pack("<4IQ", 1, 1, 0, 0, 8)
+    head = b"BBSRVCK2" + struct.pack("<4IQ2I", 2, 1, 0, 0, 8, 0, 0)
     return head + bytes(rec) + bodies + state
 
 

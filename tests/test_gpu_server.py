@@ -172,3 +172,49 @@ def test_server_checkpoint_restores_bit_exact():
         assert srv.submit(reqs[0][0], reqs[0][1], 1) > max(tickets)
         srv.destroy()
     a.destroy()
+
+
+def test_server_voxel_requests_equal_alone_and_checkpoint():
+    """Voxel shapes registered with the server: requests with concave voxel
+    bodies (PGS_SOFT and AVBD, six pair slots per default one) are served
+    next to a box pile, each equal to the world stepped alone; a checkpoint
+    taken with them resident carries the shapes, and the restored server
+    finishes them the same way."""
+    shapes, mixed = scenes.voxel_mixed_pile(0xBB65, 150)
+    _, ref = scenes.voxel_pile()
+    pgs = beatbox.WorldDesc.make(**scenes.world_desc(iterations=8, morton_bits=63, quirks=0))
+    avbd = beatbox.WorldDesc.make(**scenes.world_desc(solver=scenes.SOLVER_AVBD, iterations=10,
+                                                      morton_bits=63, quirks=0))
+    reqs = [(pgs, mixed, 14), (avbd, ref, 12), (pgs, scenes.serving_pile(16, 120), 10)]
+
+    def alone(d, bodies, steps):
+        w = beatbox.RigidBodyManager(d)
+        for _, cells in shapes:
+            w.add_voxel_shape(cells, scenes.VOXEL_SIZE, scenes.VOXEL_DENSITY)
+        w.add_bodies(bodies)
+        w.simulate(steps)
+        out = w.read_bodies()
+        w.destroy()
+        return out
+
+    want = [alone(*r) for r in reqs]
+    srv = beatbox.Server(budget_bytes=1 << 30, max_worlds=4)
+    with pytest.raises(beatbox.BBError):  # no shapes registered yet
+        srv.submit(pgs, mixed, 1)
+    for _, cells in shapes:
+        srv.add_voxel_shape(cells, scenes.VOXEL_SIZE, scenes.VOXEL_DENSITY)
+    tickets = [srv.submit(*r) for r in reqs]
+    srv.step(5)
+    blob = srv.checkpoint()
+    srv.drain()
+    restored = beatbox.Server.restore(blob, budget_bytes=1 << 30, max_worlds=4)
+    restored.drain()
+    for server in (srv, restored):
+        for t, (d, bodies, steps), w in zip(tickets, reqs, want):
+            state, got, stats = server.poll(t, len(bodies))
+            assert state == beatbox.REQ_DONE, t
+            for f in FIELDS:
+                assert got[f].tobytes() == w[f].tobytes(), (t, f)
+        assert server.report()["reruns"] == 0
+    srv.destroy()
+    restored.destroy()
